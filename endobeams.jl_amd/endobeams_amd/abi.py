@@ -1,0 +1,115 @@
+"""ctypes mirror of include/endobeams_hip.h (the C ABI of libendobeams_hip.so).
+
+Only declarations live here: structs, constants and the prototype table shared by every loader
+of a library that exports this ABI.
+"""
+import ctypes as C
+
+EB_ABI_VERSION = 1
+EB_NPROPS = 14
+
+EB_OK, EB_ERR_ARG, EB_ERR_DEVICE, EB_ERR_SOLVE, EB_ERR_STATE = 0, -1, -2, -3, -4
+
+EB_SURFACE_NONE, EB_SURFACE_SPHERE, EB_SURFACE_SDF = 0, 1, 2
+EB_PRECISION_FP64, EB_PRECISION_FP32 = 0, 1
+
+FIELDS = [
+    "U", "UDOT", "UDDOT", "W", "WDOT", "WDDOT", "R", "DR",
+    "U_N", "UDOT_N", "UDDOT_N", "W_N", "WDOT_N", "WDDOT_N", "R_N", "DR_N",
+    "FEXT", "TINT", "TK", "TC", "FEXT_N", "TINT_N", "TK_N", "TC_N",
+    "D", "DDOT", "DDDOT", "RES", "DELTA_D",
+    "K", "C", "M", "K_N", "C_N", "M_N", "KTAN",
+    "RES_FREE", "KTAN_FREE", "DELTA_D_FREE", "IMPOSED", "ENERGY",
+]
+F = {name: i for i, name in enumerate(FIELDS)}
+
+NODE_VEC_FIELDS = {"U", "UDOT", "UDDOT", "W", "WDOT", "WDDOT", "U_N", "UDOT_N", "UDDOT_N", "W_N", "WDOT_N", "WDDOT_N"}
+NODE_MAT_FIELDS = {"R", "DR", "R_N", "DR_N"}
+DOF_FIELDS = {"FEXT", "TINT", "TK", "TC", "FEXT_N", "TINT_N", "TK_N", "TC_N", "D", "DDOT", "DDDOT", "RES", "DELTA_D"}
+NNZ_FIELDS = {"K", "C", "M", "K_N", "C_N", "M_N", "KTAN"}
+
+_dp = C.POINTER(C.c_double)
+_lp = C.POINTER(C.c_int64)
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int32), ("precision", C.c_int32),
+        ("n_nodes", C.c_int64), ("n_beams", C.c_int64),
+        ("X0", _dp), ("conn", _lp), ("l0", _dp), ("Re0", _dp), ("props", _dp),
+        ("n_fixed", C.c_int64), ("fixed_dofs", _lp),
+        ("n_gauss", C.c_int32), ("pad0", C.c_int32),
+        ("gauss_w", _dp), ("gauss_z", _dp),
+        ("surface", C.c_int32), ("pad1", C.c_int32),
+        ("kn", C.c_double), ("mu", C.c_double), ("eps_t", C.c_double), ("eta_n", C.c_double),
+        ("n_contact_beams", C.c_int64), ("contact_beams", _lp),
+        ("sphere_center", C.c_double * 3), ("sphere_radius", C.c_double),
+        ("sdf_n", C.c_int64 * 3), ("sdf_d", C.c_double * 3), ("sdf_x0", C.c_double * 3),
+        ("sdf_samples", _dp),
+    ]
+
+
+class NewtonParams(C.Structure):
+    _fields_ = [
+        ("alpha", C.c_double), ("beta", C.c_double), ("gamma", C.c_double),
+        ("tol_res", C.c_double), ("tol_disp", C.c_double),
+        ("max_iterations", C.c_int32), ("krylov_maxit", C.c_int32),
+        ("krylov_tol", C.c_double),
+    ]
+
+
+class Sizes(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("n_nodes", "n_beams", "ndofs", "nnz", "n_free", "n_fixed", "nnz_free")]
+
+
+_vp = C.c_void_p
+_i = C.c_int
+_d = C.c_double
+_NP = C.POINTER(NewtonParams)
+_IP = C.POINTER(C.c_int)
+
+# name -> (restype, argtypes); the context argument is a void* here
+PROTOTYPES = {
+    "create": (_i, [C.POINTER(ModelDesc), C.POINTER(_vp)]),
+    "destroy": (_i, [_vp]),
+    "last_error": (C.c_char_p, [_vp]),
+    "get_sizes": (_i, [_vp, C.POINTER(Sizes)]),
+    "get_pattern": (_i, [_vp, _lp, _lp, _lp, _lp]),
+    "get_field": (_i, [_vp, _i, _dp, C.c_int64]),
+    "set_field": (_i, [_vp, _i, _dp, C.c_int64]),
+    "assemble": (_i, [_vp]),
+    "assemble_contact": (_i, [_vp, _d]),
+    "tangent_residuals_corrector": (_i, [_vp, _d, _d, _d, _d]),
+    "tangent_residuals_predictor": (_i, [_vp, _d, _d, _d, _d]),
+    "apply_boundary_conditions": (_i, [_vp]),
+    "extract_free_dofs": (_i, [_vp]),
+    "solve_free_dofs": (_i, [_vp, _NP, _IP]),
+    "initialize_global_corrector": (_i, [_vp]),
+    "initialize_global_predictor": (_i, [_vp]),
+    "compute_global_corrector": (_i, [_vp, _d, _d, _d]),
+    "compute_global_predictor": (_i, [_vp, _d, _d, _d]),
+    "compute_local_corrector": (_i, [_vp, _d, _d, _d]),
+    "compute_local_predictor": (_i, [_vp, _d, _d, _d]),
+    "compute_norms_corrector": (_i, [_vp, _dp, _dp]),
+    "update_loads": (_i, [_vp, C.c_int64, _lp, _dp]),
+    "update_boundary_conditions": (_i, [_vp, _dp, C.c_int64]),
+    "initialize_state_simulation": (_i, [_vp]),
+    "predictor": (_i, [_vp, _NP, _d]),
+    "corrector": (_i, [_vp, _NP, _d, _IP]),
+    "solve_step_dynamics": (_i, [_vp, _NP, _d, _IP]),
+    "update_converged": (_i, [_vp]),
+    "update_not_converged": (_i, [_vp]),
+    "get_counters": (_i, [_vp, _lp]),
+}
+
+
+def bind(lib, prefix):
+    """Attach restype/argtypes for every ABI function found in `lib` under `prefix`; returns
+    {short_name: function}. Raises if any declared function is missing."""
+    out = {}
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, prefix + name)  # AttributeError if the symbol is absent
+        fn.restype = res
+        fn.argtypes = args
+        out[name] = fn
+    return out

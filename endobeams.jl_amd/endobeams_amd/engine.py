@@ -1,0 +1,223 @@
+"""Python handle on a library that exports the EndoBeams C ABI (include/endobeams_hip.h).
+
+`Engine` wraps one eb_ctx: it packs the model, uploads the initial node state and Forces(conf)
+(src/def_structures.jl:10-27), and exposes one method per ABI entry point with the reference's
+function names. `HipEngine` is the product: it loads libendobeams_hip.so and refuses to run
+without it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+from .model import BeamsConfiguration, ModelArrays, RigidInteraction, SimulationParams, initial_fields
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "lib", "libendobeams_hip.so")
+
+
+class EBError(RuntimeError):
+    pass
+
+
+class Engine:
+    prefix = "eb_"
+
+    def __init__(self, lib: C.CDLL, conf: BeamsConfiguration, params: SimulationParams,
+                 inter: Optional[RigidInteraction] = None, precision: int = abi.EB_PRECISION_FP64):
+        self.lib = lib
+        self.fn = abi.bind(lib, self.prefix)
+        self.conf, self.params, self.inter = conf, params, inter
+        self.arrays = ModelArrays(conf, params, inter, precision)
+        h = C.c_void_p()
+        st = self.fn["create"](C.byref(self.arrays.desc), C.byref(h))
+        if st != abi.EB_OK:
+            raise EBError(f"{self.prefix}create failed with status {st}")
+        self.h = h
+        s = abi.Sizes()
+        self._check(self.fn["get_sizes"](self.h, C.byref(s)), "get_sizes")
+        self.sizes = s
+        for name, arr in initial_fields(conf).items():
+            self.set(name, arr)
+        # Forces(conf): fᵉˣᵗ[i] = f(0, i) in both forcesⁿ and forcesⁿ⁺¹
+        ld = conf.loads
+        if ld is not None and ld.concentrated_force is not None:
+            f0 = np.zeros(conf.ndofs)
+            dofs = np.asarray(ld.concentrated_force.loaded_dofs, dtype=np.int64)
+            f0[dofs - 1] = ld.concentrated_force.values(0.0)
+            self.set("FEXT", f0)
+            self.set("FEXT_N", f0)
+        self._np = params.newton()
+
+    # -- plumbing -------------------------------------------------------------------------------
+    def _check(self, st, what):
+        if st != abi.EB_OK:
+            msg = self.fn["last_error"](self.h)
+            raise EBError(f"{self.prefix}{what} failed ({st}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.fn["destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def field_len(self, name: str) -> int:
+        s = self.sizes
+        if name in abi.NODE_VEC_FIELDS:
+            return 3 * s.n_nodes
+        if name in abi.NODE_MAT_FIELDS:
+            return 9 * s.n_nodes
+        if name in abi.DOF_FIELDS:
+            return s.ndofs
+        if name in abi.NNZ_FIELDS:
+            return s.nnz
+        return {"RES_FREE": s.n_free, "DELTA_D_FREE": s.n_free, "KTAN_FREE": s.nnz_free,
+                "IMPOSED": s.n_fixed, "ENERGY": 3}[name]
+
+    def get(self, name: str) -> np.ndarray:
+        n = self.field_len(name)
+        out = np.empty(n, dtype=np.float64)
+        self._check(self.fn["get_field"](self.h, abi.F[name], out.ctypes.data_as(C.POINTER(C.c_double)), n), f"get_field({name})")
+        return out
+
+    def set(self, name: str, arr) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        self._check(self.fn["set_field"](self.h, abi.F[name], a.ctypes.data_as(C.POINTER(C.c_double)), a.size), f"set_field({name})")
+
+    def pattern(self):
+        s = self.sizes
+        colptr = np.empty(s.ndofs + 1, np.int64)
+        rowval = np.empty(s.nnz, np.int64)
+        smap = np.empty(144 * s.n_beams, np.int64)
+        sfree = np.empty(max(s.nnz_free, 1), np.int64)
+        lp = lambda a: a.ctypes.data_as(C.POINTER(C.c_int64))
+        self._check(self.fn["get_pattern"](self.h, lp(colptr), lp(rowval), lp(smap), lp(sfree)), "get_pattern")
+        return colptr, rowval, smap.reshape(s.n_beams, 144), sfree[: s.nnz_free]
+
+    def counters(self):
+        out = np.zeros(4, np.int64)
+        self._check(self.fn["get_counters"](self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "get_counters")
+        return out
+
+    # -- reference functions ----------------------------------------------------------------------
+    def assemble(self):
+        self._check(self.fn["assemble"](self.h), "assemble")
+
+    def assemble_contact(self, alpha):
+        self._check(self.fn["assemble_contact"](self.h, alpha), "assemble_contact")
+
+    def compute_tangent_and_residuals_corrector(self, dt, a, b, g):
+        self._check(self.fn["tangent_residuals_corrector"](self.h, dt, a, b, g), "tangent_residuals_corrector")
+
+    def compute_tangent_and_residuals_predictor(self, dt, a, b, g):
+        self._check(self.fn["tangent_residuals_predictor"](self.h, dt, a, b, g), "tangent_residuals_predictor")
+
+    def apply_boundary_conditions(self):
+        self._check(self.fn["apply_boundary_conditions"](self.h), "apply_boundary_conditions")
+
+    def extract_free_dofs(self):
+        self._check(self.fn["extract_free_dofs"](self.h), "extract_free_dofs")
+
+    def solve_free_dofs(self):
+        its = C.c_int(0)
+        self._check(self.fn["solve_free_dofs"](self.h, C.byref(self._np), C.byref(its)), "solve_free_dofs")
+        return its.value
+
+    def initialize_global_corrector(self):
+        self._check(self.fn["initialize_global_corrector"](self.h), "initialize_global_corrector")
+
+    def initialize_global_predictor(self):
+        self._check(self.fn["initialize_global_predictor"](self.h), "initialize_global_predictor")
+
+    def compute_global_corrector(self, dt, b, g):
+        self._check(self.fn["compute_global_corrector"](self.h, dt, b, g), "compute_global_corrector")
+
+    def compute_global_predictor(self, dt, b, g):
+        self._check(self.fn["compute_global_predictor"](self.h, dt, b, g), "compute_global_predictor")
+
+    def compute_local_corrector(self, dt, b, g):
+        self._check(self.fn["compute_local_corrector"](self.h, dt, b, g), "compute_local_corrector")
+
+    def compute_local_predictor(self, dt, b, g):
+        self._check(self.fn["compute_local_predictor"](self.h, dt, b, g), "compute_local_predictor")
+
+    def compute_norms_corrector(self):
+        r, d = C.c_double(0), C.c_double(0)
+        self._check(self.fn["compute_norms_corrector"](self.h, C.byref(r), C.byref(d)), "compute_norms_corrector")
+        return r.value, d.value
+
+    def update_loads(self, t):
+        ld = self.conf.loads
+        if ld is None or ld.concentrated_force is None:
+            return
+        cf = ld.concentrated_force
+        dofs = np.ascontiguousarray(cf.loaded_dofs, dtype=np.int64)
+        vals = cf.values(t)
+        self._check(self.fn["update_loads"](self.h, dofs.size, dofs.ctypes.data_as(C.POINTER(C.c_int64)),
+                                            vals.ctypes.data_as(C.POINTER(C.c_double))), "update_loads")
+
+    def update_boundary_conditions(self, t):
+        bcs = self.conf.bcs
+        if bcs.imposed_displacements_function is None or len(bcs.displaced_indices) == 0:
+            return
+        for i in bcs.displaced_indices:
+            bcs.imposed_displacements[i] = bcs.imposed_displacements_function(t)
+        v = np.ascontiguousarray(bcs.imposed_displacements, dtype=np.float64)
+        self._check(self.fn["update_boundary_conditions"](self.h, v.ctypes.data_as(C.POINTER(C.c_double)), v.size),
+                    "update_boundary_conditions")
+
+    def initialize_state_simulation(self):
+        self._check(self.fn["initialize_state_simulation"](self.h), "initialize_state_simulation")
+
+    def predictor(self, dt):
+        self._check(self.fn["predictor"](self.h, C.byref(self._np), dt), "predictor")
+
+    def corrector(self, dt):
+        k = C.c_int(0)
+        self._check(self.fn["corrector"](self.h, C.byref(self._np), dt, C.byref(k)), "corrector")
+        return k.value
+
+    def solve_step_dynamics(self, t, dt):
+        """solve_step_dynamics! (src/solve_step_dynamics.jl:1-14): loads and imposed displacements
+        at tⁿ⁺¹, then predictor and corrector; returns the corrector's k."""
+        self.update_loads(t)
+        self.update_boundary_conditions(t)
+        k = C.c_int(0)
+        self._check(self.fn["solve_step_dynamics"](self.h, C.byref(self._np), dt, C.byref(k)), "solve_step_dynamics")
+        return k.value
+
+    def update_converged(self):
+        self._check(self.fn["update_converged"](self.h), "update_converged")
+
+    def update_not_converged(self):
+        self._check(self.fn["update_not_converged"](self.h), "update_not_converged")
+
+
+_HIP_LIB = None
+
+
+def load_hip_library(path: str = LIB_PATH) -> C.CDLL:
+    """Loads libendobeams_hip.so; raises if it has not been built (there is no CPU fallback)."""
+    global _HIP_LIB
+    if _HIP_LIB is None:
+        if not os.path.exists(path):
+            raise EBError(f"{path} is missing: build it with __graft_entry__.build() (no CPU fallback exists)")
+        _HIP_LIB = C.CDLL(path, mode=C.RTLD_GLOBAL)
+    return _HIP_LIB
+
+
+class HipEngine(Engine):
+    """The MI355X path: every entry point runs HIP kernels on the current device."""
+    prefix = "eb_"
+
+    def __init__(self, conf, params, inter=None, precision=abi.EB_PRECISION_FP64, lib_path: str = LIB_PATH):
+        super().__init__(load_hip_library(lib_path), conf, params, inter, precision)

@@ -1,0 +1,221 @@
+/*
+ * endobeams_hip.h — C ABI of libendobeams_hip.so, the MI355X (gfx950) implementation of the
+ * per-Newton-iteration hot path of EndoBeams.jl (assembly + beam-to-rigid contact + HHT-alpha
+ * residual/tangent + linear solve + Newmark/rotation update).
+ *
+ * Conventions (all entry points):
+ *   - plain C types only; every function returns int status (0 = OK, < 0 = error code) and never
+ *     throws; the message of the last error is returned by eb_last_error(ctx).
+ *   - the caller owns every host buffer; the library owns every device buffer.
+ *   - calls are synchronous with respect to the host buffers they read or write.
+ *   - a context is not thread safe: use one per host task.
+ *   - index arrays are 1-based Int64, exactly as Julia stores them (node indices, dofs, beams).
+ *   - vector fields use Julia's in-memory layouts: Vector{SVector{3,Float64}} is 3*N doubles
+ *     node-major (x,y,z); Vector{SMatrix{3,3,Float64}} is 9*N doubles, column-major per node;
+ *     SparseMatrixCSC nzval arrays are in CSC order of the pattern built by sparsity_beams.
+ *
+ * Each function names the reference function it replaces (EndoBeams.jl, path:line).
+ */
+#ifndef ENDOBEAMS_HIP_H
+#define ENDOBEAMS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EB_ABI_VERSION 1
+
+/* status codes */
+#define EB_OK 0
+#define EB_ERR_ARG (-1)      /* invalid argument / size mismatch */
+#define EB_ERR_DEVICE (-2)   /* HIP runtime error */
+#define EB_ERR_SOLVE (-3)    /* linear solve failed (breakdown / singular) */
+#define EB_ERR_STATE (-4)    /* call not valid in the current state */
+
+/* per-beam property record, 14 doubles per beam, in the order of BeamMaterialProperties
+ * (src/beams/def_beam.jl:19-26, :74-85, :135-149):
+ * radius, E, K̄ū, K̄Θ̅[1..3], K̄Θ̅Θ̅[1..3], Jᵨ[1..3], Aᵨ, damping */
+#define EB_NPROPS 14
+
+/* rigid master surfaces (src/interactions/def_interactions.jl:24-63) */
+#define EB_SURFACE_NONE 0
+#define EB_SURFACE_SPHERE 1   /* SphereSurface(center, radius) */
+#define EB_SURFACE_SDF 2      /* DiscreteSignedDistanceField, quadratic B-spline Reflect(OnCell()) */
+
+/* arithmetic of the element and contact kernels */
+#define EB_PRECISION_FP64 0   /* everything FP64 (parity form) */
+#define EB_PRECISION_FP32 1   /* FP32 element/contact/SDF kernels and FP32 K, C, M storage;
+                                 rotations, residuals, norms, Krylov and Newmark stay FP64 */
+
+typedef struct eb_model_desc {
+    int32_t abi_version;          /* must be EB_ABI_VERSION */
+    int32_t precision;            /* EB_PRECISION_* */
+    int64_t n_nodes;
+    int64_t n_beams;
+    const double* X0;             /* 3*n_nodes  nodes.X₀ */
+    const int64_t* conn;          /* 2*n_beams  (node1, node2) per beam, 1-based */
+    const double* l0;             /* n_beams    beams.l₀ */
+    const double* Re0;            /* 9*n_beams  beams.Rₑ⁰ (column-major) */
+    const double* props;          /* EB_NPROPS*n_beams, beam-major */
+    int64_t n_fixed;              /* bcs.fixed_dofs (1-based, strictly increasing) */
+    const int64_t* fixed_dofs;
+    int32_t n_gauss;              /* params.nᴳ_beams (<= 8) */
+    int32_t pad0;
+    const double* gauss_w;        /* params.ωᴳ_beams */
+    const double* gauss_z;        /* params.zᴳ_beams */
+    /* contact (RigidInteraction); surface = EB_SURFACE_NONE disables it */
+    int32_t surface;
+    int32_t pad1;
+    double kn, mu, eps_t, eta_n;  /* InteractionProperties kₙ, μ, εᵗ, ηₙ (kₜ, ηₜ, u̇ₛ are unused by the reference) */
+    int64_t n_contact_beams;      /* BeamElementSurface.contact_beams (1-based) */
+    const int64_t* contact_beams;
+    double sphere_center[3];
+    double sphere_radius;
+    /* SDF: samples x-fastest (as read_vtk_sdf returns them), already sign-adjusted for `inside` */
+    int64_t sdf_n[3];
+    double sdf_d[3];              /* dx, dy, dz */
+    double sdf_x0[3];             /* grid origin */
+    const double* sdf_samples;    /* n[0]*n[1]*n[2] values; prefiltered inside the library */
+} eb_model_desc;
+
+/* Newton / time-integration parameters (SimulationParams, src/def_params.jl:1-35) plus the
+ * device Krylov controls, which have no counterpart in the reference (it calls Pardiso). */
+typedef struct eb_newton_params {
+    double alpha, beta, gamma;    /* HHT α, β, γ */
+    double tol_res, tol_disp;     /* tolerance_residual, tolerance_displacement */
+    int32_t max_iterations;
+    int32_t krylov_maxit;         /* BiCGStab iteration cap per linear solve (<= 0: default 200) */
+    double krylov_tol;            /* BiCGStab relative residual target (<= 0: default 1e-13) */
+} eb_newton_params;
+
+/* state fields addressable by eb_get_field / eb_set_field */
+enum eb_field {
+    /* node state, current configuration (NodeBeam, src/beams/def_node_beam.jl:20-27) */
+    EB_F_U = 0, EB_F_UDOT, EB_F_UDDOT, EB_F_W, EB_F_WDOT, EB_F_WDDOT, /* 3N each */
+    EB_F_R, EB_F_DR,                                                  /* 9N each */
+    /* node state at step n (def_node_beam.jl:30-37) */
+    EB_F_U_N, EB_F_UDOT_N, EB_F_UDDOT_N, EB_F_W_N, EB_F_WDOT_N, EB_F_WDDOT_N,
+    EB_F_R_N, EB_F_DR_N,
+    /* Forces at n+1 and n (src/def_structures.jl:2-7), ndofs each */
+    EB_F_FEXT, EB_F_TINT, EB_F_TK, EB_F_TC,
+    EB_F_FEXT_N, EB_F_TINT_N, EB_F_TK_N, EB_F_TC_N,
+    /* Solution (def_structures.jl:33-45), ndofs each */
+    EB_F_D, EB_F_DDOT, EB_F_DDDOT, EB_F_RES, EB_F_DELTA_D,
+    /* Matrices at n+1 and n, nzval in CSC order (nnz each) */
+    EB_F_K, EB_F_C, EB_F_M, EB_F_K_N, EB_F_C_N, EB_F_M_N,
+    EB_F_KTAN,                     /* Solution.Ktan nzval (nnz) */
+    EB_F_RES_FREE,                 /* Solution.r_free (n_free) */
+    EB_F_KTAN_FREE,                /* Solution.Ktan_free nzval (nnz_free) */
+    EB_F_DELTA_D_FREE,             /* Solution.ΔD_free (n_free) */
+    EB_F_IMPOSED,                  /* BoundaryConditions.imposed_displacements (n_fixed) */
+    EB_F_ENERGY,                   /* Energy: strain, kinetic, contact (3) */
+    EB_F_COUNT
+};
+
+typedef struct eb_ctx eb_ctx;
+
+/* sizes of the assembled problem */
+typedef struct eb_sizes {
+    int64_t n_nodes, n_beams, ndofs, nnz, n_free, n_fixed, nnz_free;
+} eb_sizes;
+
+/* --- lifetime ----------------------------------------------------------------------------- */
+
+/* Uploads the model, builds the CSC pattern, the element→nzval map and the free-dof gather
+ * exactly as sparse_matrices_beams!/sparsity_beams (src/sparse_matrices.jl:6-104), prefilters
+ * the SDF (src/interactions/def_interactions.jl:67-82) and zero-initialises every state field
+ * except R, Rⁿ, ΔR, ΔRⁿ (identity), matching setup_state_simulation (src/inizialization.jl:17-32). */
+int eb_create(const eb_model_desc* desc, eb_ctx** out);
+int eb_destroy(eb_ctx* ctx);
+const char* eb_last_error(const eb_ctx* ctx);
+int eb_get_sizes(const eb_ctx* ctx, eb_sizes* out);
+
+/* CSC pattern (colptr ndofs+1, rowval nnz, both 1-based), beams' global_sparsity_map
+ * (144*n_beams, 1-based, column-major local index) and sparsity_free (nnz_free, 1-based);
+ * any pointer may be NULL. Replaces sparsity_beams (src/sparse_matrices.jl:6-77). */
+int eb_get_pattern(const eb_ctx* ctx, int64_t* colptr, int64_t* rowval, int64_t* sparsity_map,
+                   int64_t* sparsity_free);
+
+/* host <-> device copies of one state field; `count` must equal the field's length */
+int eb_get_field(eb_ctx* ctx, int field, double* out, int64_t count);
+int eb_set_field(eb_ctx* ctx, int field, const double* in, int64_t count);
+
+/* --- oracle-comparable kernels (one reference function each) ------------------------------ */
+
+/* assemble! (src/beams/assemble_beams.jl:1-70): zero K, C, M, Tⁱⁿᵗ, Tᵏ, Tᶜ and add every beam's
+ * compute_beams (src/beams/compute_beams.jl:1-538) contribution; energies accumulate. */
+int eb_assemble(eb_ctx* ctx);
+
+/* assemble_contact! (src/interactions/compute&assemble_interactions_beams.jl:447-500):
+ * Tᶜ += Tᶜₑ, K -= Kᶜₑ, C -= (1+α)Cᶜₑ over contact_beams. */
+int eb_assemble_contact(eb_ctx* ctx, double alpha);
+
+/* compute_tangent_and_residuals_corrector! / _predictor! (src/utils_solver.jl:80-89, :54-74) */
+int eb_tangent_residuals_corrector(eb_ctx* ctx, double dt, double alpha, double beta, double gamma);
+int eb_tangent_residuals_predictor(eb_ctx* ctx, double dt, double alpha, double beta, double gamma);
+
+/* apply_boundary_conditions! (src/utils_bcs_and_loads.jl:5-29) */
+int eb_apply_boundary_conditions(eb_ctx* ctx);
+
+/* extract_free_dofs! (src/utils_solver.jl:94-102) */
+int eb_extract_free_dofs(eb_ctx* ctx);
+
+/* solve_free_dofs! (src/utils_solver.jl:107-120; Pardiso protocol src/linearsolvers.jl:2-28):
+ * ΔD[free] = Ktan_free \ r_free. The device path is a chain-block-tridiagonal preconditioned
+ * BiCGStab with FP64 dot products; `iterations` (may be NULL) receives its iteration count. */
+int eb_solve_free_dofs(eb_ctx* ctx, const eb_newton_params* p, int* iterations);
+
+/* initialize_global_corrector! / initialize_global_predictor! (src/utils_solver.jl:6-48) */
+int eb_initialize_global_corrector(eb_ctx* ctx);
+int eb_initialize_global_predictor(eb_ctx* ctx);
+
+/* compute_global_corrector! / compute_global_predictor! (src/utils_solver.jl:178-190, :157-172) */
+int eb_compute_global_corrector(eb_ctx* ctx, double dt, double beta, double gamma);
+int eb_compute_global_predictor(eb_ctx* ctx, double dt, double beta, double gamma);
+
+/* compute_local_corrector! / compute_local_predictor! (src/utils_solver.jl:224-245, :197-217) */
+int eb_compute_local_corrector(eb_ctx* ctx, double dt, double beta, double gamma);
+int eb_compute_local_predictor(eb_ctx* ctx, double dt, double beta, double gamma);
+
+/* compute_norms_corrector (src/utils_solver.jl:126-151) */
+int eb_compute_norms_corrector(eb_ctx* ctx, double* res_norm, double* dD_norm);
+
+/* --- step-level entry points ---------------------------------------------------------------- */
+
+/* update_loads! (src/utils_bcs_and_loads.jl:56-64): fᵉˣᵗ[dofs[i]] = values[i] (1-based dofs) */
+int eb_update_loads(eb_ctx* ctx, int64_t n, const int64_t* dofs, const double* values);
+
+/* update_boundary_conditions! (src/utils_bcs_and_loads.jl:42-50): imposed_displacements[:] = values */
+int eb_update_boundary_conditions(eb_ctx* ctx, const double* values, int64_t n_fixed);
+
+/* initialize_state_simulation! (src/inizialization.jl:37-45): assemble!, then matricesⁿ = matricesⁿ⁺¹ */
+int eb_initialize_state_simulation(eb_ctx* ctx);
+
+/* predictor! (src/predictor.jl:1-27) */
+int eb_predictor(eb_ctx* ctx, const eb_newton_params* p, double dt);
+
+/* corrector! (src/corrector.jl:1-59); *k receives the reference's return value (iterations + 1).
+ * The whole Newton loop stays on the device; the host reads two norms per iteration. */
+int eb_corrector(eb_ctx* ctx, const eb_newton_params* p, double dt, int* k);
+
+/* solve_step_dynamics! (src/solve_step_dynamics.jl:1-14) without the load/BC update, which the
+ * caller does through eb_update_loads / eb_update_boundary_conditions (they hold host closures) */
+int eb_solve_step_dynamics(eb_ctx* ctx, const eb_newton_params* p, double dt, int* k);
+
+/* update_converged! / update_not_converged! (src/utils_solver.jl:252-304) */
+int eb_update_converged(eb_ctx* ctx);
+int eb_update_not_converged(eb_ctx* ctx);
+
+/* --- instrumentation (no counterpart in the reference) ------------------------------------ */
+
+/* cumulative counters since eb_create: [0] Newton iterations, [1] Krylov iterations,
+ * [2] linear solves, [3] element-kernel launches */
+int eb_get_counters(const eb_ctx* ctx, int64_t* out4);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ENDOBEAMS_HIP_H */
