@@ -1,0 +1,875 @@
+// eb_api.hip — libendobeams_hip.so: device state, kernel launches and the C ABI declared in
+// include/endobeams_hip.h. Every entry point runs on the context's own HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/endobeams_hip.h"
+#include "eb_beam_kernel.h"
+#include "eb_contact_kernel.h"
+#include "eb_linalg.h"
+#include "eb_setup.h"
+
+using namespace ebd;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct eb_ctx_impl;
+
+}  // namespace
+
+struct eb_ctx {
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int precision = EB_PRECISION_FP64;
+    ebh::Pattern pat;
+    ebh::Chains chains;
+    int64_t N = 0, E = 0, ndofs = 0, nnz = 0, n_free = 0, n_fixed = 0, nnz_free = 0, n_contact = 0;
+    GaussPts gauss{};
+    int surface = EB_SURFACE_NONE;
+    double kn = 0, mu = 0, eps_t = 0, eta_n = 0;
+    SphereDev sphere{};
+    int32_t sdf_n[3] = {0, 0, 0};
+    double sdf_d[3] = {0, 0, 0}, sdf_x0[3] = {0, 0, 0}, sdf_dom[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<void*> allocs;
+    // node state
+    double* node[16] = {};
+    double* X0 = nullptr;
+    int32_t *n1 = nullptr, *n2 = nullptr;
+    double *l0 = nullptr, *Re0c = nullptr, *propsc = nullptr;
+    // staging (precision dependent element type)
+    void *sK = nullptr, *sC = nullptr, *sM = nullptr, *sTint = nullptr, *sTk = nullptr, *sTc = nullptr;
+    double* senergy = nullptr;
+    // pattern
+    int32_t *bptr = nullptr, *bcol = nullptr, *blk_ptr = nullptr, *inc_ptr = nullptr;
+    uint32_t *blk_src = nullptr, *inc_src = nullptr;
+    int32_t *csc2bsr = nullptr, *sfree_bsr = nullptr, *freed = nullptr, *fixidx = nullptr;
+    uint8_t* is_fixed = nullptr;
+    // matrices (block-row order) and vectors
+    double *K = nullptr, *C = nullptr, *M = nullptr, *Kn = nullptr, *Cn = nullptr, *Mn = nullptr, *Ktan = nullptr;
+    bool np1_alias = false;  // K,C,M at n+1 currently equal the step-n matrices (after a swap)
+    double *fext = nullptr, *Tint = nullptr, *Tk = nullptr, *Tc = nullptr;
+    double *fext_n = nullptr, *Tint_n = nullptr, *Tk_n = nullptr, *Tc_n = nullptr;
+    double *D = nullptr, *Dd = nullptr, *Ddd = nullptr, *res = nullptr, *dD = nullptr;
+    double* imposed = nullptr;
+    double* energy = nullptr;  // running totals (3)
+    // Krylov workspace
+    double *kx = nullptr, *kr = nullptr, *krhat = nullptr, *kp = nullptr, *kv = nullptr, *ks = nullptr, *kt = nullptr,
+           *ky = nullptr, *kz = nullptr;
+    double* partials = nullptr;
+    KrylovCtl* kctl = nullptr;
+    int* kstatus = nullptr;
+    double* norms = nullptr;  // device (2)
+    // host-pinned mirrors for per-iteration reads
+    KrylovCtl* h_kctl = nullptr;
+    double* h_norms = nullptr;
+    int* h_status = nullptr;
+    // contact
+    int32_t* cbeams = nullptr;
+    void* sdf_coef = nullptr;
+    // chains
+    int32_t *ch_ptr = nullptr, *ch_node = nullptr, *ch_diag = nullptr, *ch_lo = nullptr, *ch_up = nullptr;
+    double *ch_Sinv = nullptr, *ch_G = nullptr, *ch_Lm = nullptr;
+    int precond_mode = 0;
+    double* tmp = nullptr;  // nnz scratch for layout conversion
+    int64_t counters[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+#define EB_HIP(call)                                                                               \
+    do {                                                                                           \
+        hipError_t _e = (call);                                                                    \
+        if (_e != hipSuccess) {                                                                    \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(_e);                          \
+            return EB_ERR_DEVICE;                                                                  \
+        }                                                                                          \
+    } while (0)
+
+#define EB_TRY(expr)                \
+    do {                            \
+        int _s = (expr);            \
+        if (_s != EB_OK) return _s; \
+    } while (0)
+
+inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+inline int red_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(RED_MAX_GRID, (n + RED_BLOCK - 1) / RED_BLOCK)); }
+
+template <typename T> int dalloc(eb_ctx* ctx, T*& out, int64_t count) {
+    void* p = nullptr;
+    size_t bytes = (size_t)std::max<int64_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) { ctx->err = std::string("hipMalloc: ") + hipGetErrorString(e); return EB_ERR_DEVICE; }
+    e = hipMemsetAsync(p, 0, bytes, ctx->stream);
+    if (e != hipSuccess) { ctx->err = std::string("hipMemsetAsync: ") + hipGetErrorString(e); return EB_ERR_DEVICE; }
+    ctx->allocs.push_back(p);
+    out = (T*)p;
+    return EB_OK;
+}
+template <typename T> int upload(eb_ctx* ctx, T*& out, const T* src, int64_t count) {
+    EB_TRY(dalloc(ctx, out, count));
+    if (count > 0) EB_HIP(hipMemcpyAsync(out, src, sizeof(T) * count, hipMemcpyHostToDevice, ctx->stream));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+template <typename T> int upload_vec(eb_ctx* ctx, T*& out, const std::vector<T>& v) {
+    return upload(ctx, out, v.data(), (int64_t)v.size());
+}
+template <typename T, typename S> std::vector<T> narrow(const std::vector<S>& v) {
+    return std::vector<T>(v.begin(), v.end());
+}
+
+NodeState node_state(eb_ctx* c) {
+    return NodeState{c->X0, c->node[EB_F_U], c->node[EB_F_UDOT], c->node[EB_F_UDDOT], c->node[EB_F_WDOT],
+                     c->node[EB_F_WDDOT], c->node[EB_F_R], c->node[EB_F_DR]};
+}
+NodeFields node_fields(eb_ctx* c) {
+    NodeFields f;
+    double** p = &f.u;
+    for (int k = 0; k < 16; ++k) p[k] = c->node[k];
+    return f;
+}
+BeamConsts beam_consts(eb_ctx* c) { return BeamConsts{c->n1, c->n2, c->l0, c->Re0c, c->propsc, c->E}; }
+template <typename P> BeamStage<P> beam_stage(eb_ctx* c) {
+    return BeamStage<P>{(P*)c->sK, (P*)c->sC, (P*)c->sM, (P*)c->sTint, (P*)c->sTk, c->senergy};
+}
+ChainDev chain_dev(eb_ctx* c) {
+    return ChainDev{c->ch_ptr, c->ch_node, c->ch_diag, c->ch_lo, c->ch_up, c->ch_Sinv, c->ch_G, c->ch_Lm,
+                    (int64_t)c->chains.ptr.size() - 1};
+}
+HHTc hht_consts(double dt, double b, double g) {
+    HHTc h;
+    h.c_w = g / (b * dt);
+    h.c_wd = (b - g) / b;
+    h.c_wdd = (dt * (b - g / 2)) / b;
+    h.d_w = 1 / (b * (dt * dt));
+    h.d_wd = 1 / (b * dt);
+    h.d_wdd = 1 / (2 * b) - 1;
+    return h;
+}
+
+int check_launch(eb_ctx* ctx, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { ctx->err = std::string(what) + ": " + hipGetErrorString(e); return EB_ERR_DEVICE; }
+    return EB_OK;
+}
+
+// materialise K,C,M at n+1 if they are currently aliased to step n
+int unalias(eb_ctx* ctx) {
+    if (!ctx->np1_alias) return EB_OK;
+    size_t b = sizeof(double) * ctx->nnz;
+    EB_HIP(hipMemcpyAsync(ctx->K, ctx->Kn, b, hipMemcpyDeviceToDevice, ctx->stream));
+    EB_HIP(hipMemcpyAsync(ctx->C, ctx->Cn, b, hipMemcpyDeviceToDevice, ctx->stream));
+    EB_HIP(hipMemcpyAsync(ctx->M, ctx->Mn, b, hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->np1_alias = false;
+    return EB_OK;
+}
+
+// ---- assembly -------------------------------------------------------------------------------
+
+template <typename P> int launch_beams(eb_ctx* ctx) {
+    hipLaunchKernelGGL(beam_kernel<P>, dim3(nblk(ctx->E, 64)), dim3(64), 0, ctx->stream, node_state(ctx),
+                       beam_consts(ctx), beam_stage<P>(ctx), ctx->gauss);
+    ctx->counters[3]++;
+    return check_launch(ctx, "beam_kernel");
+}
+
+template <typename P> int launch_contact(eb_ctx* ctx, double alpha) {
+    ContactArgs<P> ca;
+    ca.surface = ctx->surface;
+    ca.sphere = ctx->sphere;
+    ca.sdf.coef = (const P*)ctx->sdf_coef;
+    for (int a = 0; a < 3; ++a) { ca.sdf.n[a] = ctx->sdf_n[a]; ca.sdf.d[a] = ctx->sdf_d[a]; ca.sdf.x0[a] = ctx->sdf_x0[a]; }
+    for (int a = 0; a < 6; ++a) ca.sdf.dom[a] = ctx->sdf_dom[a];
+    ca.kn = ctx->kn; ca.mu = ctx->mu; ca.eps_t = ctx->eps_t; ca.eta_n = ctx->eta_n; ca.alpha = alpha;
+    ca.beams = ctx->cbeams;
+    ca.n = ctx->n_contact;
+    ca.Tc = (P*)ctx->sTc;
+    ca.energy = ctx->senergy;
+    hipLaunchKernelGGL(contact_kernel<P>, dim3(nblk(ctx->n_contact, 64)), dim3(64), 0, ctx->stream, node_state(ctx),
+                       beam_consts(ctx), beam_stage<P>(ctx), ctx->gauss, ca);
+    return check_launch(ctx, "contact_kernel");
+}
+
+template <typename P> int launch_gathers(eb_ctx* ctx, bool with_contact, bool tangent, double dt, double a, double b,
+                                         double g, bool residual, int energy_mask) {
+    double c1 = 1 + a, c2 = tangent ? g / (b * dt) : 0, c3 = tangent ? 1 / (b * (dt * dt)) : 0;
+    hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz, ctx->blk_ptr,
+                       ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM, ctx->K, ctx->C, ctx->M,
+                       ctx->Ktan, c1, c2, c3, tangent ? 1 : 0);
+    EB_TRY(check_launch(ctx, "gather_matrices"));
+    hipLaunchKernelGGL(gather_vectors<P>, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs,
+                       ctx->inc_ptr, ctx->inc_src, (const P*)ctx->sTint, (const P*)ctx->sTk, (const P*)ctx->sTc,
+                       with_contact ? 1 : 0, ctx->Tint, ctx->Tk, ctx->Tc, residual ? 1 : 0, a, ctx->fext, ctx->fext_n,
+                       ctx->Tc_n, ctx->Tint_n, ctx->res);
+    EB_TRY(check_launch(ctx, "gather_vectors"));
+    int grid = red_grid(ctx->E);
+    hipLaunchKernelGGL(energy_partials, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, ctx->E, ctx->senergy,
+                       ctx->partials, energy_mask);
+    hipLaunchKernelGGL(energy_finalize, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->energy);
+    ctx->np1_alias = false;
+    return check_launch(ctx, "energy");
+}
+
+// assemble! [+ assemble_contact!] [+ Ktan and the corrector residual]
+int assemble_all(eb_ctx* ctx, bool contact, double alpha, bool fused, double dt, double b, double g) {
+    bool wc = contact && ctx->surface != EB_SURFACE_NONE;
+    if (ctx->precision == EB_PRECISION_FP32) {
+        EB_TRY(launch_beams<float>(ctx));
+        if (wc) EB_TRY(launch_contact<float>(ctx, alpha));
+        return launch_gathers<float>(ctx, wc, fused, dt, alpha, b, g, fused, wc ? 3 : 1);
+    }
+    EB_TRY(launch_beams<double>(ctx));
+    if (wc) EB_TRY(launch_contact<double>(ctx, alpha));
+    return launch_gathers<double>(ctx, wc, fused, dt, alpha, b, g, fused, wc ? 3 : 1);
+}
+
+int bc_lift_launch(eb_ctx* ctx) {
+    if (ctx->n_fixed == 0) return EB_OK;
+    hipLaunchKernelGGL(bc_lift, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs, ctx->bptr, ctx->bcol,
+                       ctx->fixidx, ctx->imposed, ctx->D, ctx->Ktan, ctx->dD, ctx->res);
+    return check_launch(ctx, "bc_lift");
+}
+
+// ---- linear solve ------------------------------------------------------------------------
+
+int precond_factor(eb_ctx* ctx) {
+    int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
+    hipLaunchKernelGGL(chain_factor, dim3((unsigned)nch), dim3(64), 0, ctx->stream, chain_dev(ctx), ctx->Ktan,
+                       ctx->is_fixed, ctx->kstatus);
+    return check_launch(ctx, "chain_factor");
+}
+
+int precond_apply(eb_ctx* ctx, const double* b, double* x) {
+    int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
+    hipLaunchKernelGGL(chain_solve, dim3((unsigned)nch), dim3(64), 0, ctx->stream, chain_dev(ctx), b, x,
+                       &ctx->kctl->done);
+    return check_launch(ctx, "chain_solve");
+}
+
+int krylov_iteration(eb_ctx* ctx) {
+    const int64_t n = ctx->ndofs;
+    const int grid = red_grid(n);
+    int* done = &ctx->kctl->done;
+    hipLaunchKernelGGL(krylov_p, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, ctx->kctl, ctx->kr, ctx->kp, ctx->kv);
+    EB_TRY(precond_apply(ctx, ctx->kp, ctx->ky));
+    hipLaunchKernelGGL(spmv_masked<1>, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, n, ctx->bptr, ctx->bcol,
+                       ctx->is_fixed, ctx->Ktan, ctx->ky, ctx->kv, ctx->krhat, ctx->partials, done);
+    hipLaunchKernelGGL(krylov_alpha_fin, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->kctl);
+    hipLaunchKernelGGL(krylov_s, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, n, ctx->kctl, ctx->kr, ctx->kv, ctx->ky,
+                       ctx->ks, ctx->kx, ctx->partials);
+    hipLaunchKernelGGL(krylov_s_fin, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->kctl);
+    EB_TRY(precond_apply(ctx, ctx->ks, ctx->kz));
+    hipLaunchKernelGGL(spmv_masked<2>, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, n, ctx->bptr, ctx->bcol,
+                       ctx->is_fixed, ctx->Ktan, ctx->kz, ctx->kt, ctx->ks, ctx->partials, done);
+    hipLaunchKernelGGL(krylov_omega_fin, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->kctl);
+    hipLaunchKernelGGL(krylov_x, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, n, ctx->kctl, ctx->kz, ctx->ks, ctx->kt,
+                       ctx->krhat, ctx->kx, ctx->kr, ctx->partials);
+    hipLaunchKernelGGL(krylov_rho_fin, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->kctl);
+    return check_launch(ctx, "krylov_iteration");
+}
+
+// Ktan (block-row order) and res must be current; solves the masked free-dof system, writes ΔD[free]
+int solve_free(eb_ctx* ctx, const eb_newton_params* p, int* its_out) {
+    const int64_t n = ctx->ndofs;
+    const int grid = red_grid(n);
+    const int maxit = (p && p->krylov_maxit > 0) ? p->krylov_maxit : 200;
+    const double tol = (p && p->krylov_tol > 0) ? p->krylov_tol : 1e-13;
+    EB_HIP(hipMemsetAsync(ctx->kstatus, 0, sizeof(int), ctx->stream));
+    EB_TRY(precond_factor(ctx));
+    hipLaunchKernelGGL(krylov_init, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, n, ctx->is_fixed, ctx->res, ctx->kx,
+                       ctx->kr, ctx->krhat, ctx->kp, ctx->kv, ctx->partials);
+    hipLaunchKernelGGL(krylov_init_fin, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->kctl, tol);
+    EB_TRY(check_launch(ctx, "krylov_init"));
+    int launched = 0, batch = 1;
+    while (launched < maxit) {
+        int m = std::min(batch, maxit - launched);
+        for (int k = 0; k < m; ++k) EB_TRY(krylov_iteration(ctx));
+        launched += m;
+        EB_HIP(hipMemcpyAsync(ctx->h_kctl, ctx->kctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, ctx->stream));
+        EB_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->h_kctl->done) break;
+        batch = std::min(batch * 2, 16);
+    }
+    EB_HIP(hipMemcpyAsync(ctx->h_status, ctx->kstatus, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->counters[1] += ctx->h_kctl->iters;
+    ctx->counters[2] += 1;
+    if (its_out) *its_out = ctx->h_kctl->iters;
+    if (*ctx->h_status) { ctx->err = "singular 6x6 pivot block in the chain preconditioner"; return EB_ERR_SOLVE; }
+    const KrylovCtl& k = *ctx->h_kctl;
+    bool ok = k.done && (k.bnorm2 == 0.0 || k.rr2 <= k.tol2 * k.bnorm2 * 1.0000001 || k.ss2 <= k.tol2 * k.bnorm2);
+    if (!ok) {
+        // accept a solve that stalled at rounding level; fail otherwise
+        double rel = k.bnorm2 > 0 ? std::sqrt(std::min(k.rr2, k.ss2 > 0 ? k.ss2 : k.rr2) / k.bnorm2) : 0.0;
+        if (!(rel < 1e-6) || !std::isfinite(rel)) {
+            char buf[160];
+            snprintf(buf, sizeof buf, "BiCGStab did not converge: %d iterations, relative residual %.3e%s", k.iters, rel,
+                     k.breakdown ? " (breakdown)" : "");
+            ctx->err = buf;
+            return EB_ERR_SOLVE;
+        }
+    }
+    hipLaunchKernelGGL(scatter_solution, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, ctx->is_fixed, ctx->kx, ctx->dD);
+    return check_launch(ctx, "scatter_solution");
+}
+
+int norms(eb_ctx* ctx, double* rn, double* dn) {
+    const int grid = red_grid(ctx->ndofs);
+    hipLaunchKernelGGL(norms_partials, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, ctx->ndofs, ctx->is_fixed, ctx->dD,
+                       ctx->res, ctx->fext, ctx->Tc, ctx->Tk, ctx->partials);
+    hipLaunchKernelGGL(norms_finalize, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->norms);
+    EB_TRY(check_launch(ctx, "norms"));
+    EB_HIP(hipMemcpyAsync(ctx->h_norms, ctx->norms, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    *rn = ctx->h_norms[0];
+    *dn = ctx->h_norms[1];
+    return EB_OK;
+}
+
+int field_info(eb_ctx* ctx, int f, double*& ptr, int64_t& len, int& kind) {
+    // kind 0 plain, 1 matrix (block-row storage, CSC at the ABI), 2 gather by free index,
+    // 3 gather Ktan by sparsity_free, 4 energy
+    kind = 0;
+    if (f >= EB_F_U && f <= EB_F_DR_N) {
+        bool mat = (f == EB_F_R || f == EB_F_DR || f == EB_F_R_N || f == EB_F_DR_N);
+        ptr = ctx->node[f];
+        len = (mat ? 9 : 3) * ctx->N;
+        return EB_OK;
+    }
+    len = ctx->ndofs;
+    switch (f) {
+        case EB_F_FEXT: ptr = ctx->fext; return EB_OK;
+        case EB_F_TINT: ptr = ctx->Tint; return EB_OK;
+        case EB_F_TK: ptr = ctx->Tk; return EB_OK;
+        case EB_F_TC: ptr = ctx->Tc; return EB_OK;
+        case EB_F_FEXT_N: ptr = ctx->fext_n; return EB_OK;
+        case EB_F_TINT_N: ptr = ctx->Tint_n; return EB_OK;
+        case EB_F_TK_N: ptr = ctx->Tk_n; return EB_OK;
+        case EB_F_TC_N: ptr = ctx->Tc_n; return EB_OK;
+        case EB_F_D: ptr = ctx->D; return EB_OK;
+        case EB_F_DDOT: ptr = ctx->Dd; return EB_OK;
+        case EB_F_DDDOT: ptr = ctx->Ddd; return EB_OK;
+        case EB_F_RES: ptr = ctx->res; return EB_OK;
+        case EB_F_DELTA_D: ptr = ctx->dD; return EB_OK;
+        default: break;
+    }
+    kind = 1;
+    len = ctx->nnz;
+    switch (f) {
+        case EB_F_K: ptr = ctx->np1_alias ? ctx->Kn : ctx->K; return EB_OK;
+        case EB_F_C: ptr = ctx->np1_alias ? ctx->Cn : ctx->C; return EB_OK;
+        case EB_F_M: ptr = ctx->np1_alias ? ctx->Mn : ctx->M; return EB_OK;
+        case EB_F_K_N: ptr = ctx->Kn; return EB_OK;
+        case EB_F_C_N: ptr = ctx->Cn; return EB_OK;
+        case EB_F_M_N: ptr = ctx->Mn; return EB_OK;
+        case EB_F_KTAN: ptr = ctx->Ktan; return EB_OK;
+        default: break;
+    }
+    switch (f) {
+        case EB_F_RES_FREE: kind = 2; ptr = ctx->res; len = ctx->n_free; return EB_OK;
+        case EB_F_DELTA_D_FREE: kind = 2; ptr = ctx->dD; len = ctx->n_free; return EB_OK;
+        case EB_F_KTAN_FREE: kind = 3; ptr = ctx->Ktan; len = ctx->nnz_free; return EB_OK;
+        case EB_F_IMPOSED: kind = 0; ptr = ctx->imposed; len = ctx->n_fixed; return EB_OK;
+        case EB_F_ENERGY: kind = 4; ptr = ctx->energy; len = 3; return EB_OK;
+        default: break;
+    }
+    ctx->err = "unknown field " + std::to_string(f);
+    return EB_ERR_ARG;
+}
+
+}  // namespace
+
+// =============================================================================================
+extern "C" {
+
+int eb_create(const eb_model_desc* d, eb_ctx** out) {
+    if (!d || !out) return EB_ERR_ARG;
+    *out = nullptr;
+    if (d->abi_version != EB_ABI_VERSION) return EB_ERR_ARG;
+    if (d->n_nodes <= 0 || d->n_beams <= 0 || !d->X0 || !d->conn || !d->l0 || !d->Re0 || !d->props) return EB_ERR_ARG;
+    if (d->n_gauss <= 0 || d->n_gauss > 8 || !d->gauss_w || !d->gauss_z) return EB_ERR_ARG;
+    if (d->precision != EB_PRECISION_FP64 && d->precision != EB_PRECISION_FP32) return EB_ERR_ARG;
+    eb_ctx* ctx = new (std::nothrow) eb_ctx();
+    if (!ctx) return EB_ERR_ARG;
+    auto bail = [&](int code) { eb_destroy(ctx); return code; };
+    if (hipGetDevice(&ctx->device) != hipSuccess) return bail(EB_ERR_DEVICE);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return bail(EB_ERR_DEVICE);
+    ctx->precision = d->precision;
+    std::string err;
+    if (!ebh::build_pattern(ctx->pat, d->n_nodes, d->n_beams, d->conn, d->n_fixed, d->fixed_dofs, err)) {
+        fprintf(stderr, "eb_create: %s\n", err.c_str());
+        return bail(EB_ERR_ARG);
+    }
+    const ebh::Pattern& P = ctx->pat;
+    ctx->N = P.N; ctx->E = P.E; ctx->ndofs = P.ndofs; ctx->nnz = P.nnz; ctx->n_free = P.n_free;
+    ctx->n_fixed = d->n_fixed; ctx->nnz_free = P.nnz_free;
+    if (ctx->nnz >= (int64_t(1) << 31)) { fprintf(stderr, "eb_create: matrix too large\n"); return bail(EB_ERR_ARG); }
+    ctx->gauss.n = d->n_gauss;
+    for (int k = 0; k < d->n_gauss; ++k) { ctx->gauss.w[k] = d->gauss_w[k]; ctx->gauss.z[k] = d->gauss_z[k]; }
+    int st = EB_OK;
+#define CK(x) do { st = (x); if (st != EB_OK) { fprintf(stderr, "eb_create: %s\n", ctx->err.c_str()); return bail(st); } } while (0)
+    const int64_t N = ctx->N, E = ctx->E;
+    // node state
+    for (int f = EB_F_U; f <= EB_F_DR_N; ++f) {
+        bool mat = (f == EB_F_R || f == EB_F_DR || f == EB_F_R_N || f == EB_F_DR_N);
+        if (mat) {
+            std::vector<double> I((size_t)(9 * N), 0.0);
+            for (int64_t n = 0; n < N; ++n) { I[9 * n] = 1; I[9 * n + 4] = 1; I[9 * n + 8] = 1; }
+            CK(upload_vec(ctx, ctx->node[f], I));
+        } else {
+            CK(dalloc(ctx, ctx->node[f], 3 * N));
+        }
+    }
+    CK(upload(ctx, ctx->X0, d->X0, 3 * N));
+    CK(upload_vec(ctx, ctx->n1, P.n1));
+    CK(upload_vec(ctx, ctx->n2, P.n2));
+    CK(upload(ctx, ctx->l0, d->l0, E));
+    {
+        std::vector<double> re((size_t)(9 * E)), pr((size_t)(EB_NPROPS * E));
+        for (int64_t e = 0; e < E; ++e) {
+            for (int k = 0; k < 9; ++k) re[k * E + e] = d->Re0[9 * e + k];
+            for (int k = 0; k < EB_NPROPS; ++k) pr[k * E + e] = d->props[EB_NPROPS * e + k];
+        }
+        CK(upload_vec(ctx, ctx->Re0c, re));
+        CK(upload_vec(ctx, ctx->propsc, pr));
+    }
+    const size_t esz = ctx->precision == EB_PRECISION_FP32 ? sizeof(float) : sizeof(double);
+    {
+        double* tmp;
+        CK(dalloc(ctx, tmp, (int64_t)((144 * E * esz + 7) / 8))); ctx->sK = tmp;
+        CK(dalloc(ctx, tmp, (int64_t)((144 * E * esz + 7) / 8))); ctx->sC = tmp;
+        CK(dalloc(ctx, tmp, (int64_t)((144 * E * esz + 7) / 8))); ctx->sM = tmp;
+        CK(dalloc(ctx, tmp, (int64_t)((12 * E * esz + 7) / 8))); ctx->sTint = tmp;
+        CK(dalloc(ctx, tmp, (int64_t)((12 * E * esz + 7) / 8))); ctx->sTk = tmp;
+        CK(dalloc(ctx, tmp, (int64_t)((12 * E * esz + 7) / 8))); ctx->sTc = tmp;
+    }
+    CK(dalloc(ctx, ctx->senergy, 3 * E));
+    CK(upload_vec(ctx, ctx->bptr, narrow<int32_t>(P.bptr)));
+    CK(upload_vec(ctx, ctx->bcol, P.bcol));
+    CK(upload_vec(ctx, ctx->blk_ptr, narrow<int32_t>(P.blk_ptr)));
+    CK(upload_vec(ctx, ctx->blk_src, P.blk_src));
+    CK(upload_vec(ctx, ctx->inc_ptr, narrow<int32_t>(P.inc_ptr)));
+    CK(upload_vec(ctx, ctx->inc_src, P.inc_src));
+    CK(upload_vec(ctx, ctx->csc2bsr, narrow<int32_t>(P.csc2bsr)));
+    CK(upload_vec(ctx, ctx->sfree_bsr, P.sfree_bsr));
+    CK(upload_vec(ctx, ctx->freed, P.freed));
+    CK(upload_vec(ctx, ctx->fixidx, P.fixidx));
+    CK(upload_vec(ctx, ctx->is_fixed, P.is_fixed));
+    for (double** m : {&ctx->K, &ctx->C, &ctx->M, &ctx->Kn, &ctx->Cn, &ctx->Mn, &ctx->Ktan, &ctx->tmp}) CK(dalloc(ctx, *m, ctx->nnz));
+    for (double** v : {&ctx->fext, &ctx->Tint, &ctx->Tk, &ctx->Tc, &ctx->fext_n, &ctx->Tint_n, &ctx->Tk_n, &ctx->Tc_n,
+                       &ctx->D, &ctx->Dd, &ctx->Ddd, &ctx->res, &ctx->dD, &ctx->kx, &ctx->kr, &ctx->krhat, &ctx->kp,
+                       &ctx->kv, &ctx->ks, &ctx->kt, &ctx->ky, &ctx->kz})
+        CK(dalloc(ctx, *v, ctx->ndofs));
+    CK(dalloc(ctx, ctx->imposed, std::max<int64_t>(ctx->n_fixed, 1)));
+    CK(dalloc(ctx, ctx->energy, 3));
+    CK(dalloc(ctx, ctx->partials, 8 * RED_MAX_GRID));
+    CK(dalloc(ctx, ctx->kctl, 1));
+    CK(dalloc(ctx, ctx->kstatus, 1));
+    CK(dalloc(ctx, ctx->norms, 2));
+    if (hipHostMalloc((void**)&ctx->h_kctl, sizeof(KrylovCtl), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_norms, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_status, sizeof(int), hipHostMallocDefault) != hipSuccess)
+        return bail(EB_ERR_DEVICE);
+    // contact
+    ctx->surface = d->surface;
+    if (d->surface != EB_SURFACE_NONE) {
+        if (d->surface != EB_SURFACE_SPHERE && d->surface != EB_SURFACE_SDF) return bail(EB_ERR_ARG);
+        ctx->kn = d->kn; ctx->mu = d->mu; ctx->eps_t = d->eps_t; ctx->eta_n = d->eta_n;
+        std::vector<int32_t> cb((size_t)d->n_contact_beams);
+        std::vector<uint8_t> seen((size_t)E, 0);
+        for (int64_t k = 0; k < d->n_contact_beams; ++k) {
+            int64_t e = d->contact_beams[k] - 1;
+            if (e < 0 || e >= E || seen[e]) { fprintf(stderr, "eb_create: invalid or repeated contact beam\n"); return bail(EB_ERR_ARG); }
+            seen[e] = 1;
+            cb[k] = (int32_t)e;
+        }
+        ctx->n_contact = d->n_contact_beams;
+        CK(upload_vec(ctx, ctx->cbeams, cb));
+        if (d->surface == EB_SURFACE_SPHERE) {
+            for (int a = 0; a < 3; ++a) ctx->sphere.c[a] = d->sphere_center[a];
+            ctx->sphere.R = d->sphere_radius;
+        } else {
+            if (!d->sdf_samples || d->sdf_n[0] < 1 || d->sdf_n[1] < 1 || d->sdf_n[2] < 1) return bail(EB_ERR_ARG);
+            for (int a = 0; a < 3; ++a) {
+                ctx->sdf_n[a] = (int32_t)d->sdf_n[a]; ctx->sdf_d[a] = d->sdf_d[a]; ctx->sdf_x0[a] = d->sdf_x0[a];
+                ctx->sdf_dom[2 * a] = d->sdf_x0[a];
+                ctx->sdf_dom[2 * a + 1] = d->sdf_x0[a] + (d->sdf_n[a] - 1) * d->sdf_d[a];
+            }
+            std::vector<double> coef;
+            ebh::prefilter_sdf(d->sdf_n, d->sdf_samples, coef);
+            if (ctx->precision == EB_PRECISION_FP32) {
+                std::vector<float> cf(coef.begin(), coef.end());
+                float* p;
+                CK(upload_vec(ctx, p, cf));
+                ctx->sdf_coef = p;
+            } else {
+                double* p;
+                CK(upload_vec(ctx, p, coef));
+                ctx->sdf_coef = p;
+            }
+        }
+    }
+    // chains for the preconditioner
+    ebh::build_chains(ctx->chains, P, d->X0, 0);
+    {
+        const ebh::Chains& c = ctx->chains;
+        CK(upload_vec(ctx, ctx->ch_ptr, narrow<int32_t>(c.ptr)));
+        CK(upload_vec(ctx, ctx->ch_node, c.node));
+        CK(upload_vec(ctx, ctx->ch_diag, c.diag_blk));
+        CK(upload_vec(ctx, ctx->ch_lo, c.lo_blk));
+        CK(upload_vec(ctx, ctx->ch_up, c.up_blk));
+        CK(dalloc(ctx, ctx->ch_Sinv, 36 * N));
+        CK(dalloc(ctx, ctx->ch_G, 36 * N));
+        CK(dalloc(ctx, ctx->ch_Lm, 36 * N));
+    }
+#undef CK
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return bail(EB_ERR_DEVICE);
+    *out = ctx;
+    return EB_OK;
+}
+
+int eb_destroy(eb_ctx* ctx) {
+    if (!ctx) return EB_OK;
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (void* p : ctx->allocs) hipFree(p);
+    if (ctx->h_kctl) hipHostFree(ctx->h_kctl);
+    if (ctx->h_norms) hipHostFree(ctx->h_norms);
+    if (ctx->h_status) hipHostFree(ctx->h_status);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return EB_OK;
+}
+
+const char* eb_last_error(const eb_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int eb_get_sizes(const eb_ctx* ctx, eb_sizes* s) {
+    if (!ctx || !s) return EB_ERR_ARG;
+    s->n_nodes = ctx->N; s->n_beams = ctx->E; s->ndofs = ctx->ndofs; s->nnz = ctx->nnz;
+    s->n_free = ctx->n_free; s->n_fixed = ctx->n_fixed; s->nnz_free = ctx->nnz_free;
+    return EB_OK;
+}
+
+int eb_get_pattern(const eb_ctx* ctx, int64_t* colptr, int64_t* rowval, int64_t* smap, int64_t* sfree) {
+    if (!ctx) return EB_ERR_ARG;
+    const ebh::Pattern& P = ctx->pat;
+    if (colptr) for (int64_t j = 0; j <= P.ndofs; ++j) colptr[j] = P.colptr[j] + 1;
+    if (rowval) for (int64_t k = 0; k < P.nnz; ++k) rowval[k] = P.rowval[k] + 1;
+    if (smap) for (int64_t k = 0; k < 144 * P.E; ++k) smap[k] = P.smap_csc[k] + 1;
+    if (sfree) for (int64_t k = 0; k < P.nnz_free; ++k) sfree[k] = P.sfree_csc[k] + 1;
+    return EB_OK;
+}
+
+int eb_get_field(eb_ctx* ctx, int f, double* out, int64_t count) {
+    if (!ctx || (!out && count)) return EB_ERR_ARG;
+    double* ptr; int64_t len; int kind;
+    EB_TRY(field_info(ctx, f, ptr, len, kind));
+    if (count != len) { ctx->err = "field size mismatch"; return EB_ERR_ARG; }
+    if (len == 0) return EB_OK;
+    const double* src = ptr;
+    if (kind == 1) {
+        hipLaunchKernelGGL(gather_idx32, dim3(nblk(len, 256)), dim3(256), 0, ctx->stream, len, ctx->csc2bsr, ptr, ctx->tmp);
+        src = ctx->tmp;
+    } else if (kind == 2) {
+        hipLaunchKernelGGL(gather_idx32, dim3(nblk(len, 256)), dim3(256), 0, ctx->stream, len, ctx->freed, ptr, ctx->tmp);
+        src = ctx->tmp;
+    } else if (kind == 3) {
+        hipLaunchKernelGGL(gather_idx32, dim3(nblk(len, 256)), dim3(256), 0, ctx->stream, len, ctx->sfree_bsr, ptr, ctx->tmp);
+        src = ctx->tmp;
+    }
+    EB_TRY(check_launch(ctx, "get_field"));
+    EB_HIP(hipMemcpyAsync(out, src, sizeof(double) * len, hipMemcpyDeviceToHost, ctx->stream));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_set_field(eb_ctx* ctx, int f, const double* in, int64_t count) {
+    if (!ctx || (!in && count)) return EB_ERR_ARG;
+    if (f == EB_F_K || f == EB_F_C || f == EB_F_M) EB_TRY(unalias(ctx));
+    double* ptr; int64_t len; int kind;
+    EB_TRY(field_info(ctx, f, ptr, len, kind));
+    if (count != len) { ctx->err = "field size mismatch"; return EB_ERR_ARG; }
+    if (len == 0) return EB_OK;
+    if (kind == 0 || kind == 4) {
+        EB_HIP(hipMemcpyAsync(ptr, in, sizeof(double) * len, hipMemcpyHostToDevice, ctx->stream));
+    } else {
+        EB_HIP(hipMemcpyAsync(ctx->tmp, in, sizeof(double) * len, hipMemcpyHostToDevice, ctx->stream));
+        const int32_t* idx = kind == 1 ? ctx->csc2bsr : (kind == 2 ? ctx->freed : ctx->sfree_bsr);
+        hipLaunchKernelGGL(scatter_idx32, dim3(nblk(len, 256)), dim3(256), 0, ctx->stream, len, idx, ctx->tmp, ptr);
+        EB_TRY(check_launch(ctx, "set_field"));
+    }
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_assemble(eb_ctx* ctx) {
+    if (!ctx) return EB_ERR_ARG;
+    EB_TRY(assemble_all(ctx, false, 0.0, false, 1.0, 1.0, 1.0));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_assemble_contact(eb_ctx* ctx, double alpha) {
+    if (!ctx) return EB_ERR_ARG;
+    if (ctx->surface == EB_SURFACE_NONE) return EB_OK;
+    // the contact kernel folds into the element staging written by the last eb_assemble
+    if (ctx->precision == EB_PRECISION_FP32) {
+        EB_TRY(launch_contact<float>(ctx, alpha));
+        EB_TRY(launch_gathers<float>(ctx, true, false, 1.0, alpha, 1.0, 1.0, false, 2));
+    } else {
+        EB_TRY(launch_contact<double>(ctx, alpha));
+        EB_TRY(launch_gathers<double>(ctx, true, false, 1.0, alpha, 1.0, 1.0, false, 2));
+    }
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_tangent_residuals_corrector(eb_ctx* ctx, double dt, double a, double b, double g) {
+    if (!ctx) return EB_ERR_ARG;
+    EB_TRY(unalias(ctx));
+    hipLaunchKernelGGL(combine_tangent, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz, ctx->K, ctx->C,
+                       ctx->M, ctx->Ktan, 1 + a, g / (b * dt), 1 / (b * (dt * dt)));
+    hipLaunchKernelGGL(residual_corrector, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs, a,
+                       ctx->fext, ctx->Tc, ctx->Tint, ctx->fext_n, ctx->Tc_n, ctx->Tint_n, ctx->Tk, ctx->res);
+    EB_TRY(check_launch(ctx, "tangent_residuals_corrector"));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+static int tangent_residuals_predictor_async(eb_ctx* ctx, double dt, double a, double b, double g) {
+    double c3 = 1 / (b * (dt * dt));
+    hipLaunchKernelGGL(combine_tangent, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz, ctx->Kn, ctx->Cn,
+                       ctx->Mn, ctx->Ktan, 1 + a, g / (b * dt), c3);
+    hipLaunchKernelGGL(residual_predictor, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs, ctx->bptr,
+                       ctx->bcol, a, g / b, ((dt / 2) * (2 * b - g)) / b, dt, c3, ctx->fext, ctx->Tint_n, ctx->Tk_n,
+                       ctx->fext_n, ctx->Dd, ctx->Ddd, ctx->Cn, ctx->Mn, ctx->res);
+    return check_launch(ctx, "tangent_residuals_predictor");
+}
+
+int eb_tangent_residuals_predictor(eb_ctx* ctx, double dt, double a, double b, double g) {
+    if (!ctx) return EB_ERR_ARG;
+    EB_TRY(tangent_residuals_predictor_async(ctx, dt, a, b, g));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_apply_boundary_conditions(eb_ctx* ctx) {
+    if (!ctx) return EB_ERR_ARG;
+    EB_TRY(bc_lift_launch(ctx));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+// r_free and Ktan_free are views of r and Ktan through the free-dof gathers (see eb_get_field)
+int eb_extract_free_dofs(eb_ctx* ctx) { return ctx ? EB_OK : EB_ERR_ARG; }
+
+int eb_solve_free_dofs(eb_ctx* ctx, const eb_newton_params* p, int* its) {
+    if (!ctx) return EB_ERR_ARG;
+    EB_TRY(solve_free(ctx, p, its));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_initialize_global_corrector(eb_ctx* ctx) {
+    if (!ctx) return EB_ERR_ARG;
+    hipLaunchKernelGGL(init_global, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx), ctx->D,
+                       ctx->Dd, ctx->Ddd, ctx->dD, 0);
+    EB_TRY(check_launch(ctx, "init_global"));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_initialize_global_predictor(eb_ctx* ctx) {
+    if (!ctx) return EB_ERR_ARG;
+    hipLaunchKernelGGL(init_global, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx), ctx->D,
+                       ctx->Dd, ctx->Ddd, ctx->dD, 1);
+    EB_TRY(check_launch(ctx, "init_global"));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+static int global_corrector_async(eb_ctx* ctx, double dt, double b, double g) {
+    hipLaunchKernelGGL(global_corrector, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs,
+                       g / (b * dt), 1 / (b * (dt * dt)), ctx->D, ctx->Dd, ctx->Ddd, ctx->dD);
+    return check_launch(ctx, "global_corrector");
+}
+static int local_corrector_async(eb_ctx* ctx, double dt, double b, double g) {
+    hipLaunchKernelGGL(local_corrector, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx),
+                       ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, hht_consts(dt, b, g));
+    return check_launch(ctx, "local_corrector");
+}
+static int global_predictor_async(eb_ctx* ctx, double dt, double b, double g) {
+    hipLaunchKernelGGL(global_predictor, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs, dt, g / b,
+                       g / (b * dt), 1 / (b * (dt * dt)), dt * (2 * b - g) / (2 * b), ctx->D, ctx->Dd, ctx->Ddd, ctx->dD);
+    return check_launch(ctx, "global_predictor");
+}
+static int local_predictor_async(eb_ctx* ctx, double dt, double b, double g) {
+    hipLaunchKernelGGL(local_predictor, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx),
+                       ctx->D, ctx->Dd, ctx->Ddd, hht_consts(dt, b, g));
+    return check_launch(ctx, "local_predictor");
+}
+
+int eb_compute_global_corrector(eb_ctx* ctx, double dt, double b, double g) {
+    if (!ctx) return EB_ERR_ARG;
+    EB_TRY(global_corrector_async(ctx, dt, b, g));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+int eb_compute_global_predictor(eb_ctx* ctx, double dt, double b, double g) {
+    if (!ctx) return EB_ERR_ARG;
+    EB_TRY(global_predictor_async(ctx, dt, b, g));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+int eb_compute_local_corrector(eb_ctx* ctx, double dt, double b, double g) {
+    if (!ctx) return EB_ERR_ARG;
+    EB_TRY(local_corrector_async(ctx, dt, b, g));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+int eb_compute_local_predictor(eb_ctx* ctx, double dt, double b, double g) {
+    if (!ctx) return EB_ERR_ARG;
+    EB_TRY(local_predictor_async(ctx, dt, b, g));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_compute_norms_corrector(eb_ctx* ctx, double* rn, double* dn) {
+    if (!ctx || !rn || !dn) return EB_ERR_ARG;
+    return norms(ctx, rn, dn);
+}
+
+int eb_update_loads(eb_ctx* ctx, int64_t n, const int64_t* dofs, const double* values) {
+    if (!ctx) return EB_ERR_ARG;
+    if (n == 0) return EB_OK;
+    // loads are few or dense: upload the touched dofs through the scratch buffers
+    std::vector<int32_t> idx((size_t)n);
+    for (int64_t k = 0; k < n; ++k) {
+        if (dofs[k] < 1 || dofs[k] > ctx->ndofs) { ctx->err = "load dof out of range"; return EB_ERR_ARG; }
+        idx[k] = (int32_t)(dofs[k] - 1);
+    }
+    double* vals = ctx->tmp;
+    int32_t* didx = (int32_t*)(ctx->kt);  // Krylov scratch is idle between solves
+    if (n > ctx->ndofs) { ctx->err = "too many loads"; return EB_ERR_ARG; }
+    EB_HIP(hipMemcpyAsync(vals, values, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    EB_HIP(hipMemcpyAsync(didx, idx.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(scatter_idx32, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, didx, vals, ctx->fext);
+    EB_TRY(check_launch(ctx, "update_loads"));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_update_boundary_conditions(eb_ctx* ctx, const double* values, int64_t n_fixed) {
+    if (!ctx) return EB_ERR_ARG;
+    if (n_fixed != ctx->n_fixed) { ctx->err = "imposed displacement count mismatch"; return EB_ERR_ARG; }
+    if (n_fixed == 0) return EB_OK;
+    EB_HIP(hipMemcpyAsync(ctx->imposed, values, sizeof(double) * n_fixed, hipMemcpyHostToDevice, ctx->stream));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_initialize_state_simulation(eb_ctx* ctx) {
+    if (!ctx) return EB_ERR_ARG;
+    EB_TRY(assemble_all(ctx, false, 0.0, false, 1.0, 1.0, 1.0));
+    size_t b = sizeof(double) * ctx->nnz;
+    EB_HIP(hipMemcpyAsync(ctx->Kn, ctx->K, b, hipMemcpyDeviceToDevice, ctx->stream));
+    EB_HIP(hipMemcpyAsync(ctx->Cn, ctx->C, b, hipMemcpyDeviceToDevice, ctx->stream));
+    EB_HIP(hipMemcpyAsync(ctx->Mn, ctx->M, b, hipMemcpyDeviceToDevice, ctx->stream));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_predictor(eb_ctx* ctx, const eb_newton_params* p, double dt) {
+    if (!ctx || !p) return EB_ERR_ARG;
+    hipLaunchKernelGGL(init_global, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx), ctx->D,
+                       ctx->Dd, ctx->Ddd, ctx->dD, 1);
+    EB_TRY(tangent_residuals_predictor_async(ctx, dt, p->alpha, p->beta, p->gamma));
+    EB_TRY(bc_lift_launch(ctx));
+    EB_TRY(solve_free(ctx, p, nullptr));
+    EB_TRY(global_predictor_async(ctx, dt, p->beta, p->gamma));
+    EB_TRY(local_predictor_async(ctx, dt, p->beta, p->gamma));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_corrector(eb_ctx* ctx, const eb_newton_params* p, double dt, int* kout) {
+    if (!ctx || !p || !kout) return EB_ERR_ARG;
+    hipLaunchKernelGGL(init_global, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx), ctx->D,
+                       ctx->Dd, ctx->Ddd, ctx->dD, 0);
+    EB_TRY(check_launch(ctx, "init_global"));
+    int k = 1;
+    double rn = INFINITY, dn = INFINITY;
+    while ((rn > p->tol_res || dn > p->tol_disp) && k <= p->max_iterations) {
+        EB_TRY(assemble_all(ctx, true, p->alpha, true, dt, p->beta, p->gamma));
+        EB_TRY(bc_lift_launch(ctx));
+        EB_TRY(solve_free(ctx, p, nullptr));
+        EB_TRY(global_corrector_async(ctx, dt, p->beta, p->gamma));
+        EB_TRY(local_corrector_async(ctx, dt, p->beta, p->gamma));
+        EB_TRY(norms(ctx, &rn, &dn));
+        ctx->counters[0]++;
+        k += 1;
+    }
+    *kout = k;
+    return EB_OK;
+}
+
+int eb_solve_step_dynamics(eb_ctx* ctx, const eb_newton_params* p, double dt, int* k) {
+    EB_TRY(eb_predictor(ctx, p, dt));
+    return eb_corrector(ctx, p, dt, k);
+}
+
+int eb_update_converged(eb_ctx* ctx) {
+    if (!ctx) return EB_ERR_ARG;
+    for (int f = 0; f < 8; ++f)
+        EB_HIP(hipMemcpyAsync(ctx->node[EB_F_U_N + f], ctx->node[EB_F_U + f], sizeof(double) * (f >= 6 ? 9 : 3) * ctx->N,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+    size_t vb = sizeof(double) * ctx->ndofs;
+    EB_HIP(hipMemcpyAsync(ctx->Tint_n, ctx->Tint, vb, hipMemcpyDeviceToDevice, ctx->stream));
+    EB_HIP(hipMemcpyAsync(ctx->Tk_n, ctx->Tk, vb, hipMemcpyDeviceToDevice, ctx->stream));
+    EB_HIP(hipMemcpyAsync(ctx->Tc_n, ctx->Tc, vb, hipMemcpyDeviceToDevice, ctx->stream));
+    EB_HIP(hipMemcpyAsync(ctx->fext_n, ctx->fext, vb, hipMemcpyDeviceToDevice, ctx->stream));
+    if (!ctx->np1_alias) {
+        // matricesⁿ = matricesⁿ⁺¹ by exchanging buffers; n+1 now aliases n until the next assembly
+        std::swap(ctx->K, ctx->Kn);
+        std::swap(ctx->C, ctx->Cn);
+        std::swap(ctx->M, ctx->Mn);
+        ctx->np1_alias = true;
+    }
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_update_not_converged(eb_ctx* ctx) {
+    if (!ctx) return EB_ERR_ARG;
+    for (int f = 0; f < 8; ++f)
+        EB_HIP(hipMemcpyAsync(ctx->node[EB_F_U + f], ctx->node[EB_F_U_N + f], sizeof(double) * (f >= 6 ? 9 : 3) * ctx->N,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+    size_t vb = sizeof(double) * ctx->ndofs;
+    EB_HIP(hipMemcpyAsync(ctx->Tint, ctx->Tint_n, vb, hipMemcpyDeviceToDevice, ctx->stream));
+    EB_HIP(hipMemcpyAsync(ctx->Tk, ctx->Tk_n, vb, hipMemcpyDeviceToDevice, ctx->stream));
+    EB_HIP(hipMemcpyAsync(ctx->Tc, ctx->Tc_n, vb, hipMemcpyDeviceToDevice, ctx->stream));
+    EB_HIP(hipMemcpyAsync(ctx->fext, ctx->fext_n, vb, hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->np1_alias = true;
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_get_counters(const eb_ctx* ctx, int64_t* out4) {
+    if (!ctx || !out4) return EB_ERR_ARG;
+    for (int k = 0; k < 4; ++k) out4[k] = ctx->counters[k];
+    return EB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,377 @@
+// eb_beam_kernel.h — corotational beam element (compute_beams, src/beams/compute_beams.jl:1-538)
+// as a gfx950 kernel: one lane per element. Element constants are component-major ([k][E]) so a
+// wavefront's loads are contiguous rows; outputs are element-major so that the assembly gather
+// reads each 6x6 sub-block from one 576-byte span.
+//
+// Precision: the local frame, R̅ = Rₑᵀ R Rₑ⁰, the log map Θ̅, Tₛ⁻¹(Θ̅), η/μ and K̄ₕ are always FP64;
+// everything downstream of them runs in the physics type P (double for the parity form, float
+// for the precision form) and is stored as P.
+#pragma once
+#include "eb_dmath.h"
+
+namespace ebd {
+
+struct NodeState {  // device pointers, Julia layouts (3N node-major vectors, 9N column-major matrices)
+    const double *X0, *u, *ud, *udd, *wd, *wdd, *R, *dR;
+};
+
+struct BeamConsts {  // component-major element constants
+    const int32_t *n1, *n2;  // 0-based
+    const double* l0;        // [E]
+    const double* Re0;       // [9][E]
+    const double* props;     // [14][E]
+    int64_t E;
+};
+
+template <typename P> struct BeamStage {  // element-major outputs: beam e owns a contiguous record
+    P *K, *C, *M;        // [E][144], 12x12 column-major (local dof order u₁ θ₁ u₂ θ₂)
+    P *Tint, *Tk;        // [E][12]
+    double* energy;      // [E][3]: strain, kinetic, contact
+};
+
+struct GaussPts { int n; double w[8], z[8]; };
+
+EBD V3<double> ldv(const double* a, int64_t i) { return {a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
+EBD M3<double> ldm(const double* a, int64_t i) { M3<double> m; for (int k = 0; k < 9; ++k) m.a[k] = a[9 * i + k]; return m; }
+
+template <typename P> EBD void st_block(P* base, int64_t E, int64_t e, int bi, int bj, const M3<P>& B) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) base[e * 144 + (3 * bj + j) * 12 + 3 * bi + i] = B.a[i + 3 * j];
+}
+template <typename P> EBD void st_vec(P* base, int64_t E, int64_t e, int b, V3<P> v) {
+    base[e * 12 + 3 * b] = v.x; base[e * 12 + 3 * b + 1] = v.y; base[e * 12 + 3 * b + 2] = v.z;
+}
+
+struct ShapeF { double N1, N2, N3, N4, N5, N6, N7, N8; };
+EBD ShapeF shape_fn(double l0, double z) {
+    ShapeF s;
+    double xi = l0 * (z + 1) / 2;
+    s.N1 = 1 - xi / l0;
+    s.N2 = 1 - s.N1;
+    double a = 1 - xi / l0;
+    s.N3 = xi * (a * a);
+    s.N4 = -(1 - xi / l0) * ((xi * xi) / l0);
+    s.N5 = (1 - 3 * xi / l0) * (1 - xi / l0);
+    s.N6 = (3 * xi / l0 - 2) * (xi / l0);
+    s.N7 = s.N3 + s.N4;
+    s.N8 = s.N5 + s.N6 - 1;
+    return s;
+}
+
+// Static (internal force / stiffness) part, compute_beams.jl:13-159. Writes Tⁱⁿᵗ and Kⁱⁿᵗ.
+// Returns the quantities the dynamic part needs.
+struct StaticOut {
+    M3<double> Re;
+    V3<double> Tb1, Tb2, v1;
+    double eta, eta11, eta12, eta21, eta22, ln;
+};
+
+template <typename P>
+EBD StaticOut beam_static(int64_t e, const NodeState& ns, const BeamConsts& bc, const BeamStage<P>& st) {
+    const int64_t E = bc.E;
+    const int32_t a = bc.n1[e], b = bc.n2[e];
+    V3<double> x1 = ldv(ns.X0, a) + ldv(ns.u, a), x2 = ldv(ns.X0, b) + ldv(ns.u, b);
+    double ln = norm(x2 - x1);
+    double l0 = bc.l0[e];
+    M3<double> Re0;
+    for (int k = 0; k < 9; ++k) Re0.a[k] = bc.Re0[k * E + e];
+    M3<double> R1 = ldm(ns.R, a), R2 = ldm(ns.R, b);
+    LocalFrame f = local_frame(x1, x2, R1, R2, col(Re0, 1), ln);
+    M3<double> ReT = tr(f.Re);
+    M3<double> Rb1 = (ReT * R1) * Re0, Rb2 = (ReT * R2) * Re0;
+    V3<double> Tb1 = toangle(Rb1), Tb2 = toangle(Rb2);
+    M3<double> Ti1 = Ts_inv(Tb1), Ti2 = Ts_inv(Tb2);
+    double pr[14];
+    for (int k = 0; k < 14; ++k) pr[k] = bc.props[k * E + e];
+    const double Ku = pr[2];
+    const double KT0 = pr[3], KT1 = pr[4], KT2 = pr[5], KTT0 = pr[6], KTT1 = pr[7], KTT2 = pr[8];
+    double ub = ln - l0;
+    double Tu = Ku * ub;
+    V3<double> TT1 = {KT0 * Tb1.x + KTT0 * Tb2.x, KT1 * Tb1.y + KTT1 * Tb2.y, KT2 * Tb1.z + KTT2 * Tb2.z};
+    V3<double> TT2 = {KTT0 * Tb1.x + KT0 * Tb2.x, KTT1 * Tb1.y + KT1 * Tb2.y, KTT2 * Tb1.z + KT2 * Tb2.z};
+    st.energy[3 * e] = (ub * Tu + dot(Tb1, TT1) + dot(Tb2, TT2)) / 2;
+    double eta1, mu1, eta2, mu2;
+    eta_mu(Tb1, eta1, mu1);
+    eta_mu(Tb2, eta2, mu2);
+    M3<double> Kh1 = Kh_bar(Tb1, TT1, Ti1, eta1, mu1);
+    M3<double> Kh2 = Kh_bar(Tb2, TT2, Ti2, eta2, mu2);
+
+    // Gᵀ matrices (local_Rₑ_and_aux): G1 = Gᵀu₁, G2 = GᵀΘ₁, G3 = -G1, G4 = GᵀΘ₂
+    const M3<double> I = eye<double>();
+    M3<double> G1 = rows(0.0, 0.0, f.eta / ln, 0.0, 0.0, 1 / ln, 0.0, -1 / ln, 0.0);
+    M3<double> G2 = rows(f.eta12 / 2, -f.eta11 / 2, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
+    M3<double> G4 = rows(f.eta22 / 2, -f.eta21 / 2, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
+    M3<double> P11 = -G1, P12 = I - G2, P22 = -G2, P14 = -G4, P24 = I - G4;
+    M3<double> Bp11 = P11 * ReT, Bp12 = P12 * ReT, Bp22 = P22 * ReT, Bp14 = P14 * ReT, Bp24 = P24 * ReT;
+    const M3<double>& Bp21 = Bp11;
+    V3<double> B1 = -f.v1;
+    M3<double> B11 = Ti1 * Bp11, B12 = Ti1 * Bp12, B14 = Ti1 * Bp14;
+    M3<double> B21 = Ti2 * Bp21, B22 = Ti2 * Bp22, B24 = Ti2 * Bp24;
+
+    // Tⁱⁿᵗ = Bᵀ T̄ⁱⁿᵗ
+    V3<double> T1 = (B1 * Tu + tmul(B11, TT1)) + tmul(B21, TT2);
+    V3<double> T2 = tmul(B12, TT1) + tmul(B22, TT2);
+    V3<double> T4 = tmul(B14, TT1) + tmul(B24, TT2);
+    st_vec<P>(st.Tint, E, e, 0, cvt<P>(T1));
+    st_vec<P>(st.Tint, E, e, 1, cvt<P>(T2));
+    st_vec<P>(st.Tint, E, e, 2, cvt<P>(-T1));
+    st_vec<P>(st.Tint, E, e, 3, cvt<P>(T4));
+
+    // Kₘ, K̃, BᵀKB
+    V3<double> Mp1 = tmul(Ti1, TT1), Mp2 = tmul(Ti2, TT2);
+    M3<double> Q1 = skew(tmul(P11, Mp1) + tmul(P11, Mp2));
+    M3<double> Q2 = skew(tmul(P12, Mp1) + tmul(P22, Mp2));
+    M3<double> Q4 = skew(tmul(P14, Mp1) + tmul(P24, Mp2));
+    V3<double> av = {0.0, f.eta * (Mp1.x + Mp2.x) / ln + (Mp1.y + Mp2.y) / ln, (Mp1.z + Mp2.z) / ln};
+    M3<double> D3 = (I - outer(f.v1, f.v1)) / ln;
+    V3<double> EGa1 = f.Re * tmul(G1, av);
+    M3<double> Km11 = (D3 * Tu - (f.Re * (Q1 * G1)) * ReT) + outer(EGa1, B1);
+    M3<double> Km12 = -((f.Re * (Q1 * G2)) * ReT);
+    M3<double> Km14 = -((f.Re * (Q1 * G4)) * ReT);
+    M3<double> Km22 = -((f.Re * (Q2 * G2)) * ReT);
+    M3<double> Km24 = -((f.Re * (Q2 * G4)) * ReT);
+    M3<double> Km44 = -((f.Re * (Q4 * G4)) * ReT);
+    M3<double> tB11 = tr(Bp11), tB12 = tr(Bp12), tB14 = tr(Bp14), tB22 = tr(Bp22), tB24 = tr(Bp24);
+    M3<double> Kt11 = ((tB11 * Kh1) * Bp11 + (tB11 * Kh2) * Bp21) + Km11;
+    M3<double> Kt12 = ((tB11 * Kh1) * Bp12 + (tB11 * Kh2) * Bp22) + Km12;
+    M3<double> Kt14 = ((tB11 * Kh1) * Bp14 + (tB11 * Kh2) * Bp24) + Km14;
+    M3<double> Kt22 = ((tB12 * Kh1) * Bp12 + (tB22 * Kh2) * Bp22) + Km22;
+    M3<double> Kt24 = ((tB12 * Kh1) * Bp14 + (tB22 * Kh2) * Bp24) + Km24;
+    M3<double> Kt44 = ((tB14 * Kh1) * Bp14 + (tB24 * Kh2) * Bp24) + Km44;
+    M3<double> KT = diag(KT0, KT1, KT2), KTT = diag(KTT0, KTT1, KTT2);
+    M3<double> XB12 = KT * B12 + KTT * B22, YB12 = KTT * B12 + KT * B22;
+    M3<double> XB14 = KT * B14 + KTT * B24, YB14 = KTT * B14 + KT * B24;
+    M3<double> tb11 = tr(B11), tb21 = tr(B21), tb12 = tr(B12), tb22 = tr(B22), tb14 = tr(B14), tb24 = tr(B24);
+    M3<double> K11 = ((outer(B1 * Ku, B1) + tb11 * (KT * B11 + KTT * B21)) + tb21 * (KTT * B11 + KT * B21)) + Kt11;
+    M3<double> K12 = (tb11 * XB12 + tb21 * YB12) + Kt12;
+    M3<double> K14 = (tb11 * XB14 + tb21 * YB14) + Kt14;
+    M3<double> K22 = (tb12 * XB12 + tb22 * YB12) + Kt22;
+    M3<double> K24 = (tb12 * XB14 + tb22 * YB14) + Kt24;
+    M3<double> K44 = (tb14 * XB14 + tb24 * YB14) + Kt44;
+    // block layout of Kⁱⁿᵗ (compute_beams.jl:148-159)
+    M3<double> tK12 = tr(K12), tK14 = tr(K14);
+    st_block<P>(st.K, E, e, 0, 0, cvt<P>(K11));
+    st_block<P>(st.K, E, e, 1, 0, cvt<P>(tK12));
+    st_block<P>(st.K, E, e, 2, 0, cvt<P>(-tr(K11)));      // K¹³ᵀ with K¹³ = -K¹¹
+    st_block<P>(st.K, E, e, 3, 0, cvt<P>(tK14));
+    st_block<P>(st.K, E, e, 0, 1, cvt<P>(K12));
+    st_block<P>(st.K, E, e, 1, 1, cvt<P>(K22));
+    st_block<P>(st.K, E, e, 2, 1, cvt<P>(-K12));          // K²³ᵀ = (-K¹²ᵀ)ᵀ
+    st_block<P>(st.K, E, e, 3, 1, cvt<P>(tr(K24)));
+    st_block<P>(st.K, E, e, 0, 2, cvt<P>(-K11));
+    st_block<P>(st.K, E, e, 1, 2, cvt<P>(-tK12));
+    st_block<P>(st.K, E, e, 2, 2, cvt<P>(K11));
+    st_block<P>(st.K, E, e, 3, 2, cvt<P>(-tK14));         // K³⁴ᵀ = (-K¹⁴)ᵀ
+    st_block<P>(st.K, E, e, 0, 3, cvt<P>(K14));
+    st_block<P>(st.K, E, e, 1, 3, cvt<P>(K24));
+    st_block<P>(st.K, E, e, 2, 3, cvt<P>(-K14));
+    st_block<P>(st.K, E, e, 3, 3, cvt<P>(K44));
+
+    StaticOut so;
+    so.Re = f.Re; so.Tb1 = Tb1; so.Tb2 = Tb2; so.v1 = f.v1;
+    so.eta = f.eta; so.eta11 = f.eta11; so.eta12 = f.eta12; so.eta21 = f.eta21; so.eta22 = f.eta22; so.ln = ln;
+    return so;
+}
+
+}  // namespace ebd
+
+namespace ebd {
+
+// Dynamic part (inertia, gyroscopic and damping forces, M and Cᵏ), compute_beams.jl:161-517.
+template <typename P>
+EBD void beam_dynamic(int64_t e, const NodeState& ns, const BeamConsts& bc, const BeamStage<P>& st,
+                      const GaussPts& gp, const StaticOut& so) {
+    const int64_t E = bc.E;
+    const int32_t na = bc.n1[e], nb = bc.n2[e];
+    const M3<P> I = eye<P>();
+    const M3<P> Re = cvt<P>(so.Re);
+    const M3<P> ReT = tr(Re);
+    const P ln = P(so.ln), l0 = P(bc.l0[e]);
+    const P Ar = P(bc.props[12 * E + e]), damping = P(bc.props[13 * E + e]);
+    const M3<P> Jr = diag(P(bc.props[9 * E + e]), P(bc.props[10 * E + e]), P(bc.props[11 * E + e]));
+    const V3<P> Tb1 = cvt<P>(so.Tb1), Tb2 = cvt<P>(so.Tb2);
+    const M3<P> G1 = rows(P(0), P(0), P(so.eta / so.ln), P(0), P(0), P(1 / so.ln), P(0), P(-1 / so.ln), P(0));
+    const M3<P> G2 = rows(P(so.eta12 / 2), P(-so.eta11 / 2), P(0), P(0), P(0), P(0), P(0), P(0), P(0));
+    const M3<P> G3 = -G1;
+    const M3<P> G4 = rows(P(so.eta22 / 2), P(-so.eta21 / 2), P(0), P(0), P(0), P(0), P(0), P(0), P(0));
+
+    const V3<P> ud1 = cvt<P>(ldv(ns.ud, na)), ud2 = cvt<P>(ldv(ns.ud, nb));
+    const V3<P> Ud1 = tmul(Re, ud1), Ud2 = tmul(Re, ud2);
+    const V3<P> Wd1 = tmul(Re, cvt<P>(ldv(ns.wd, na))), Wd2 = tmul(Re, cvt<P>(ldv(ns.wd, nb)));
+    const V3<P> Udd1 = tmul(Re, cvt<P>(ldv(ns.udd, na))), Udd2 = tmul(Re, cvt<P>(ldv(ns.udd, nb)));
+    const V3<P> Wdd1 = tmul(Re, cvt<P>(ldv(ns.wdd, na))), Wdd2 = tmul(Re, cvt<P>(ldv(ns.wdd, nb)));
+    const M3<P> SUd1 = skew(Ud1), SUd2 = skew(Ud2), SWd1 = skew(Wd1), SWd2 = skew(Wd2);
+    const V3<P> We = ((G1 * Ud1 + G2 * Wd1) + G3 * Ud2) + G4 * Wd2;
+    const M3<P> SWe = skew(We);
+    const V3<P> v1 = cvt<P>(so.v1);
+    const P rd = dot(-v1, ud1) + dot(v1, ud2);
+
+    P kinetic = P(0);
+    V3<P> Tk1 = zv<P>(), Tk2 = zv<P>(), Tk3 = zv<P>(), Tk4 = zv<P>();
+    M3<P> M11 = zm<P>(), M12 = zm<P>(), M13 = zm<P>(), M14 = zm<P>(), M22 = zm<P>(), M23 = zm<P>(), M24 = zm<P>(),
+          M33 = zm<P>(), M34 = zm<P>(), M44 = zm<P>();
+    M3<P> C11 = zm<P>(), C12 = zm<P>(), C14 = zm<P>(), C21 = zm<P>(), C22 = zm<P>(), C24 = zm<P>(), C31 = zm<P>(),
+          C32 = zm<P>(), C34 = zm<P>(), C41 = zm<P>(), C42 = zm<P>(), C44 = zm<P>();
+
+    for (int ig = 0; ig < gp.n; ++ig) {
+        const P w = P(gp.w[ig]);
+        ShapeF sd = shape_fn(bc.l0[e], gp.z[ig]);
+        const P N1 = P(sd.N1), N2 = P(sd.N2), N3 = P(sd.N3), N4 = P(sd.N4), N5 = P(sd.N5), N6 = P(sd.N6), N7 = P(sd.N7),
+                N8 = P(sd.N8);
+        V3<P> ut = {P(0), N3 * Tb1.z + N4 * Tb2.z, (-N3) * Tb1.y + (-N4) * Tb2.y};
+        V3<P> Thb = {N1 * Tb1.x + N2 * Tb2.x, N5 * Tb1.y + N6 * Tb2.y, N5 * Tb1.z + N6 * Tb2.z};
+        M3<P> Sut = skew(ut);
+        M3<P> Rbar = I + skew(Thb);
+        M3<P> Ir = (Rbar * Jr) * tr(Rbar);
+        P N7l = N7 / ln, N7l2 = N7l / ln, N8l = N8 / ln, N8l2 = N8l / ln;
+        M3<P> PP1 = rows(P(0), P(0), P(0), P(0), N7l, P(0), P(0), P(0), N7l);
+        M3<P> PP2 = rows(P(0), P(0), P(0), P(0), P(0), N3, P(0), -N3, P(0));
+        M3<P> PP3 = -PP1;
+        M3<P> PP4 = rows(P(0), P(0), P(0), P(0), P(0), N4, P(0), -N4, P(0));
+        M3<P> H11 = (N1 * I + PP1) - Sut * G1;
+        M3<P> H12 = PP2 - Sut * G2;
+        M3<P> H13 = (N2 * I + PP3) - Sut * G3;
+        M3<P> H14 = PP4 - Sut * G4;
+        M3<P> H21 = rows(P(0), P(0), P(0), P(0), P(0), -N8l, P(0), N8l, P(0));
+        M3<P> H22 = diag(N1, N5, N5);
+        M3<P> H23 = -H21;
+        M3<P> H24 = diag(N2, N6, N6);
+        V3<P> udt = ((PP1 * Ud1 + PP2 * Wd1) + PP3 * Ud2) + PP4 * Wd2;
+        M3<P> Sudt = skew(udt);
+        P N7rd = N7l2 * rd, N8rd = N8l2 * rd;
+        M3<P> Hd11 = diag(P(0), -N7rd, -N7rd) - Sudt * G1;
+        M3<P> Hd12 = -(Sudt * G2);
+        M3<P> Hd14 = -(Sudt * G4);
+        M3<P> Hd21 = rows(P(0), P(0), P(0), P(0), P(0), N8rd, P(0), -N8rd, P(0));
+        V3<P> h1 = ((H11 * Ud1 + H12 * Wd1) + H13 * Ud2) + H14 * Wd2;
+        V3<P> h2 = ((H21 * Ud1 + H22 * Wd1) + H23 * Ud2) + H24 * Wd2;
+        M3<P> Sh1 = skew(h1), Sh2 = skew(h2);
+        M3<P> C1_1 = (SWe * H11 + Hd11) - H11 * SWe;
+        M3<P> C1_2 = (SWe * H12 + Hd12) - H12 * SWe;
+        M3<P> C1_4 = (SWe * H14 + Hd14) - H14 * SWe;
+        M3<P> C2_1 = (SWe * H21 + Hd21) - H21 * SWe;
+        M3<P> C2_2 = SWe * H22 - H22 * SWe;
+        M3<P> C2_4 = SWe * H24 - H24 * SWe;
+        M3<P> H1F1 = ((H11 * SUd1 + H12 * SWd1) + H13 * SUd2) + H14 * SWd2;
+        M3<P> H2F1 = ((H21 * SUd1 + H22 * SWd1) + H23 * SUd2) + H24 * SWd2;
+        M3<P> A1 = rows(P(0), P(0), P(0), Ud1.y - Ud2.y, P(0), P(0), Ud1.z - Ud2.z, P(0), P(0));
+        M3<P> C3_1 = (((-Sh1) * G1 + N7l2 * A1) + SWe * PP1) + H1F1 * G1;
+        M3<P> C3_2 = ((-Sh1) * G2 + SWe * PP2) + H1F1 * G2;
+        M3<P> C3_4 = ((-Sh1) * G4 + SWe * PP4) + H1F1 * G4;
+        M3<P> A2 = rows(P(0), P(0), P(0), -Ud1.z + Ud2.z, P(0), P(0), Ud1.y - Ud2.y, P(0), P(0));
+        M3<P> C4_1 = ((-Sh2) * G1 + N8l2 * A2) + H2F1 * G1;
+        M3<P> C4_2 = (-Sh2) * G2 + H2F1 * G2;
+        M3<P> C4_4 = (-Sh2) * G4 + H2F1 * G4;
+        V3<P> ud0 = Re * h1;
+        V3<P> ReTudd0 = (((H11 * Udd1 + H12 * Wdd1) + H13 * Udd2) + H14 * Wdd2) +
+                        (((C1_1 * Ud1 + C1_2 * Wd1) + (-C1_1) * Ud2) + C1_4 * Wd2);
+        V3<P> W0 = h2;
+        V3<P> wd0 = Re * W0;
+        V3<P> ReTwdd0 = (((H21 * Udd1 + H22 * Wdd1) + H23 * Udd2) + H24 * Wdd2) +
+                        (((C2_1 * Ud1 + C2_2 * Wd1) + (-C2_1) * Ud2) + C2_4 * Wd2);
+        M3<P> SW0Ir = skew(W0) * Ir;
+        V3<P> X = Ir * ReTwdd0 + SW0Ir * W0;
+        M3<P> AH11 = Ar * tr(H11), AH12 = Ar * tr(H12), AH14 = Ar * tr(H14);
+        V3<P> Tk1G = w * (AH11 * ReTudd0 + tmul(H21, X));
+        Tk1 += Tk1G;
+        Tk2 += w * (AH12 * ReTudd0 + tmul(H22, X));
+        Tk3 += (-Tk1G) + (w * Ar) * ReTudd0;
+        Tk4 += w * (AH14 * ReTudd0 + tmul(H24, X));
+        if (damping > P(0)) {
+            V3<P> Irh2 = Ir * h2;
+            V3<P> Td1G = (w * damping) * (AH11 * h1 + tmul(H21, Irh2));
+            Tk1 += Td1G;
+            Tk2 += (w * damping) * (AH12 * h1 + tmul(H22, Irh2));
+            Tk3 += (-Td1G) + ((w * Ar) * damping) * h1;
+            Tk4 += (w * damping) * (AH14 * h1 + tmul(H24, Irh2));
+        }
+        M3<P> tH21Ir = tr(H21) * Ir, tH22Ir = tr(H22) * Ir, tH24Ir = tr(H24) * Ir;
+        M3<P> M11G = w * (AH11 * H11 + tH21Ir * H21);
+        M3<P> M12G = w * (AH11 * H12 + tH21Ir * H22);
+        M3<P> M14G = w * (AH11 * H14 + tH21Ir * H24);
+        M11 += M11G;
+        M12 += M12G;
+        M13 += (-M11G) + w * AH11;
+        M14 += M14G;
+        M22 += w * (AH12 * H12 + tH22Ir * H22);
+        M23 += (-tr(M12G)) + w * AH12;
+        M24 += w * (AH12 * H14 + tH22Ir * H24);
+        M33 += M11G + (w * Ar) * ((I - H11) - tr(H11));
+        M34 += (-M14G) + (w * Ar) * H14;
+        M44 += w * (AH14 * H14 + tH24Ir * H24);
+        M3<P> SW0IrmS = SW0Ir - skew(Ir * W0);
+        M3<P> AC1 = Ar * (C1_1 + C3_1), AC2 = Ar * (C1_2 + C3_2), AC4 = Ar * (C1_4 + C3_4);
+        M3<P> Y1 = Ir * (C2_1 + C4_1) + SW0IrmS * H21;
+        M3<P> Y2 = Ir * (C2_2 + C4_2) + SW0IrmS * H22;
+        M3<P> Y4 = Ir * (C2_4 + C4_4) + SW0IrmS * H24;
+        M3<P> tH11 = tr(H11), tH12 = tr(H12), tH14 = tr(H14), tH21 = tr(H21), tH22 = tr(H22), tH24 = tr(H24);
+        M3<P> C11G = w * (tH11 * AC1 + tH21 * Y1);
+        M3<P> C12G = w * (tH11 * AC2 + tH21 * Y2);
+        M3<P> C14G = w * (tH11 * AC4 + tH21 * Y4);
+        C11 += C11G; C12 += C12G; C14 += C14G;
+        C21 += w * (tH12 * AC1 + tH22 * Y1);
+        C22 += w * (tH12 * AC2 + tH22 * Y2);
+        C24 += w * (tH12 * AC4 + tH22 * Y4);
+        C31 += (-C11G) + w * AC1;
+        C32 += (-C12G) + w * AC2;
+        C34 += (-C14G) + w * AC4;
+        C41 += w * (tH14 * AC1 + tH24 * Y1);
+        C42 += w * (tH14 * AC2 + tH24 * Y2);
+        C44 += w * (tH14 * AC4 + tH24 * Y4);
+        M3<P> Irg = (Re * Ir) * ReT;
+        kinetic += (w / P(2)) * (Ar * dot(ud0, ud0) + dot(wd0, Irg * wd0));
+    }
+    const P l02 = l0 / P(2);
+    const M3<P> l02Re = l02 * Re;
+    st.energy[3 * e + 1] = double(l02 * kinetic);
+    st_vec<P>(st.Tk, E, e, 0, l02Re * Tk1);
+    st_vec<P>(st.Tk, E, e, 1, l02Re * Tk2);
+    st_vec<P>(st.Tk, E, e, 2, l02Re * Tk3);
+    st_vec<P>(st.Tk, E, e, 3, l02Re * Tk4);
+    M11 = (l02Re * M11) * ReT; M12 = (l02Re * M12) * ReT; M13 = (l02Re * M13) * ReT; M14 = (l02Re * M14) * ReT;
+    M22 = (l02Re * M22) * ReT; M23 = (l02Re * M23) * ReT; M24 = (l02Re * M24) * ReT;
+    M33 = (l02Re * M33) * ReT; M34 = (l02Re * M34) * ReT; M44 = (l02Re * M44) * ReT;
+    C11 = (l02Re * C11) * ReT; C12 = (l02Re * C12) * ReT; C14 = (l02Re * C14) * ReT;
+    C21 = (l02Re * C21) * ReT; C22 = (l02Re * C22) * ReT; C24 = (l02Re * C24) * ReT;
+    C31 = (l02Re * C31) * ReT; C32 = (l02Re * C32) * ReT; C34 = (l02Re * C34) * ReT;
+    C41 = (l02Re * C41) * ReT; C42 = (l02Re * C42) * ReT; C44 = (l02Re * C44) * ReT;
+    M3<P> C13 = -C11, C23 = -C21, C33 = -C31, C43 = -C41;
+    if (damping > P(0)) {
+        C11 += damping * M11; C12 += damping * M12; C13 += damping * M13; C14 += damping * M14;
+        C21 += damping * tr(M12); C22 += damping * M22; C23 += damping * M23; C24 += damping * M24;
+        C31 += damping * tr(M13); C32 += damping * tr(M23); C33 += damping * M33; C34 += damping * M34;
+        C41 += damping * tr(M14); C42 += damping * tr(M24); C43 += damping * tr(M34); C44 += damping * M44;
+    }
+    // exact Tₛ⁻¹(log ΔR)ᵀ post-multiplication of block rows 2 and 4, in place (compute_beams.jl:480-506)
+    const M3<P> TT1t = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, na)))));
+    const M3<P> TT2t = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, nb)))));
+    st_block<P>(st.M, E, e, 0, 0, M11); st_block<P>(st.M, E, e, 0, 1, M12);
+    st_block<P>(st.M, E, e, 0, 2, M13); st_block<P>(st.M, E, e, 0, 3, M14);
+    st_block<P>(st.M, E, e, 1, 0, tr(M12) * TT1t);
+    M3<P> M23t = M23 * TT1t, M24t = M24 * TT1t;
+    st_block<P>(st.M, E, e, 1, 1, M22 * TT1t); st_block<P>(st.M, E, e, 1, 2, M23t); st_block<P>(st.M, E, e, 1, 3, M24t);
+    st_block<P>(st.M, E, e, 2, 0, tr(M13)); st_block<P>(st.M, E, e, 2, 1, tr(M23t));
+    st_block<P>(st.M, E, e, 2, 2, M33); st_block<P>(st.M, E, e, 2, 3, M34);
+    st_block<P>(st.M, E, e, 3, 0, tr(M14) * TT2t); st_block<P>(st.M, E, e, 3, 1, tr(M24t) * TT2t);
+    st_block<P>(st.M, E, e, 3, 2, tr(M34) * TT2t); st_block<P>(st.M, E, e, 3, 3, M44 * TT2t);
+    st_block<P>(st.C, E, e, 0, 0, C11); st_block<P>(st.C, E, e, 0, 1, C12);
+    st_block<P>(st.C, E, e, 0, 2, C13); st_block<P>(st.C, E, e, 0, 3, C14);
+    st_block<P>(st.C, E, e, 1, 0, C21 * TT1t); st_block<P>(st.C, E, e, 1, 1, C22 * TT1t);
+    st_block<P>(st.C, E, e, 1, 2, C23 * TT1t); st_block<P>(st.C, E, e, 1, 3, C24 * TT1t);
+    st_block<P>(st.C, E, e, 2, 0, C31); st_block<P>(st.C, E, e, 2, 1, C32);
+    st_block<P>(st.C, E, e, 2, 2, C33); st_block<P>(st.C, E, e, 2, 3, C34);
+    st_block<P>(st.C, E, e, 3, 0, C41 * TT2t); st_block<P>(st.C, E, e, 3, 1, C42 * TT2t);
+    st_block<P>(st.C, E, e, 3, 2, C43 * TT2t); st_block<P>(st.C, E, e, 3, 3, C44 * TT2t);
+}
+
+template <typename P>
+__global__ void __launch_bounds__(64) beam_kernel(NodeState ns, BeamConsts bc, BeamStage<P> st, GaussPts gp) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= bc.E) return;
+    StaticOut so = beam_static<P>(e, ns, bc, st);
+    beam_dynamic<P>(e, ns, bc, st, gp, so);
+}
+
+}  // namespace ebd

@@ -1,0 +1,336 @@
+// eb_contact_kernel.h — beam-to-rigid penalty/friction contact at Gauss points
+// (compute_contact_beams, src/interactions/compute&assemble_interactions_beams.jl:1-445) with the
+// gap functions of src/interactions/utils_compute_interactions_beams.jl:1-75 and the
+// regularisation of src/interactions/interactions_beams_regularization.jl:1-45.
+//
+// One lane per contact beam. The lane first places its Gauss points (xᴳ) and runs the contact
+// test; only lanes with a Gauss point in contact build the consistent tangent. Contributions are
+// folded into the element staging in place: K -= Kᶜ, C -= (1+α)Cᶜ (assemble_contact!, :493-497),
+// so one gather assembles beams and contact together.
+#pragma once
+#include "eb_beam_kernel.h"
+
+namespace ebd {
+
+// Quadratic B-spline SDF with Reflect(OnCell()) prefiltered coefficients, padded by one cell
+// (src/interactions/def_interactions.jl:67-82).
+template <typename P> struct SDFDev {
+    const P* coef;       // (n0+2)(n1+2)(n2+2), x fastest
+    int32_t n[3];
+    double d[3], x0[3];
+    double dom[6];
+};
+
+struct SphereDev { double c[3], R; };
+
+template <typename P> struct ContactArgs {
+    int surface;  // EB_SURFACE_*
+    SphereDev sphere;
+    SDFDev<P> sdf;
+    double kn, mu, eps_t, eta_n, alpha;
+    const int32_t* beams;  // 0-based contact beam list
+    int64_t n;
+    P* Tc;                 // [E][12] staging (written for every listed beam)
+    double* energy;        // [E][3] staging, contact energy in slot 2
+};
+
+EBD bool isinside_dom(V3<double> p, const double* dom) {
+    double lx = p.x - dom[0], ly = p.y - dom[2], lz = p.z - dom[4];
+    return lx >= 0 && lx <= (dom[1] - dom[0]) && ly >= 0 && ly <= (dom[3] - dom[2]) && lz >= 0 && lz <= (dom[5] - dom[4]);
+}
+
+// 27-coefficient stencil: value (and optionally gradient, Hessian) of the interpolant.
+// t = (x - x₀)/dx + 1, knot = round-half-up(t) clamped to [1, n], δ = t - knot.
+template <typename P>
+EBD void sdf_eval(const SDFDev<P>& s, V3<double> p, P& val, V3<P>* grad, M3<P>* hess) {
+    int ix[3];
+    P w[3][3], wd[3][3], wdd[3][3];
+    double pc[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double t = (pc[a] - s.x0[a]) / s.d[a] + 1.0;
+        double r = floor(t + 0.5);
+        r = r < 1.0 ? 1.0 : r;
+        r = r > double(s.n[a]) ? double(s.n[a]) : r;
+        ix[a] = int(r);
+        P dd = P(t - r);
+        P h = P(0.5);
+        w[a][0] = (dd - h) * (dd - h) / P(2); w[a][1] = P(0.75) - dd * dd; w[a][2] = (dd + h) * (dd + h) / P(2);
+        P id = P(1.0 / s.d[a]);
+        wd[a][0] = (dd - h) * id; wd[a][1] = (P(-2) * dd) * id; wd[a][2] = (dd + h) * id;
+        P ih2 = P(1.0 / (s.d[a] * s.d[a]));
+        wdd[a][0] = ih2; wdd[a][1] = P(-2) * ih2; wdd[a][2] = ih2;
+    }
+    const int64_t P0 = s.n[0] + 2, P1 = s.n[1] + 2;
+    P v = 0, gx = 0, gy = 0, gz = 0, hxx = 0, hyy = 0, hzz = 0, hxy = 0, hxz = 0, hyz = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const P* row = s.coef + ((int64_t)(ix[2] - 1 + c) * P1 + (ix[1] - 1 + b)) * P0 + (ix[0] - 1);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                P q = row[a];
+                v += w[0][a] * w[1][b] * w[2][c] * q;
+                if (grad) {
+                    gx += wd[0][a] * w[1][b] * w[2][c] * q;
+                    gy += w[0][a] * wd[1][b] * w[2][c] * q;
+                    gz += w[0][a] * w[1][b] * wd[2][c] * q;
+                    hxx += wdd[0][a] * w[1][b] * w[2][c] * q;
+                    hyy += w[0][a] * wdd[1][b] * w[2][c] * q;
+                    hzz += w[0][a] * w[1][b] * wdd[2][c] * q;
+                    hxy += wd[0][a] * wd[1][b] * w[2][c] * q;
+                    hxz += wd[0][a] * w[1][b] * wd[2][c] * q;
+                    hyz += w[0][a] * wd[1][b] * wd[2][c] * q;
+                }
+            }
+        }
+    val = v;
+    if (grad) {
+        *grad = V3<P>{gx, gy, gz};
+        *hess = rows(hxx, hxy, hxz, hxy, hyy, hyz, hxz, hyz, hzz);
+    }
+}
+
+// incontact_beams (utils_compute_interactions_beams.jl:23-36, :67-75)
+template <typename P> EBD bool in_contact(const ContactArgs<P>& ca, V3<double> p, double rb) {
+    double gbar = rb / 4;
+    if (ca.surface == 1) {
+        V3<double> aux = {p.x - ca.sphere.c[0], p.y - ca.sphere.c[1], p.z - ca.sphere.c[2]};
+        return (norm(aux) - ca.sphere.R) - rb <= gbar;
+    }
+    if (!isinside_dom(p, ca.sdf.dom)) return false;
+    P v;
+    sdf_eval<P>(ca.sdf, p, v, nullptr, nullptr);
+    return double(v) - rb <= gbar;
+}
+
+// contact_gap_beams (utils_compute_interactions_beams.jl:2-20, :50-64)
+template <typename P> EBD void contact_gap(const ContactArgs<P>& ca, V3<double> p, double rb, P& g, V3<P>& n, M3<P>& H) {
+    if (ca.surface == 1) {
+        V3<P> aux = {P(p.x - ca.sphere.c[0]), P(p.y - ca.sphere.c[1]), P(p.z - ca.sphere.c[2])};
+        P na = norm(aux);
+        g = (na - P(ca.sphere.R)) - P(rb);
+        P inv = P(1) / na;
+        n = inv * aux;
+        H = inv * eye<P>() - ((inv * inv) * inv) * outer(aux, aux);
+        return;
+    }
+    P v;
+    V3<P> gr;
+    M3<P> h;
+    sdf_eval<P>(ca.sdf, p, v, &gr, &h);
+    P nn = dot(gr, gr);
+    P nmaginv = P(1) / sqrt(nn);
+    n = gr * nmaginv;
+    H = (h * nmaginv) * (eye<P>() - outer(n, n) / nn);
+    g = v - P(rb);
+}
+
+template <typename P>
+__global__ void __launch_bounds__(64) contact_kernel(NodeState ns, BeamConsts bc, BeamStage<P> st, GaussPts gp,
+                                                     ContactArgs<P> ca) {
+    int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= ca.n) return;
+    const int64_t E = bc.E;
+    const int64_t e = ca.beams[li];
+    const int32_t na = bc.n1[e], nb = bc.n2[e];
+    const double rb = bc.props[0 * E + e];
+    V3<double> x1 = ldv(ns.X0, na) + ldv(ns.u, na), x2 = ldv(ns.X0, nb) + ldv(ns.u, nb);
+    double lnd = norm(x2 - x1);
+    M3<double> Re0;
+    for (int k = 0; k < 9; ++k) Re0.a[k] = bc.Re0[k * E + e];
+    M3<double> R1 = ldm(ns.R, na), R2 = ldm(ns.R, nb);
+    LocalFrame f = local_frame(x1, x2, R1, R2, col(Re0, 1), lnd);
+    M3<double> ReTd = tr(f.Re);
+    V3<double> Tb1 = toangle((ReTd * R1) * Re0), Tb2 = toangle((ReTd * R2) * Re0);
+    const double l0 = bc.l0[e];
+
+    // Gauss point placement and contact test (FP64 positions)
+    unsigned mask = 0;
+    for (int ig = 0; ig < gp.n; ++ig) {
+        ShapeF s = shape_fn(l0, gp.z[ig]);
+        V3<double> ut = {0.0, s.N3 * Tb1.z + s.N4 * Tb2.z, (-s.N3) * Tb1.y + (-s.N4) * Tb2.y};
+        V3<double> xG = (s.N1 * x1 + s.N2 * x2) + f.Re * ut;
+        if (in_contact<P>(ca, xG, rb)) mask |= 1u << ig;
+    }
+    if (!mask) {
+        for (int k = 0; k < 12; ++k) ca.Tc[e * 12 + k] = P(0);
+        ca.energy[3 * e + 2] = 0.0;
+        return;
+    }
+
+    const M3<P> I = eye<P>();
+    const M3<P> Re = cvt<P>(f.Re), ReT = tr(Re);
+    const P ln = P(lnd);
+    const M3<P> G1 = rows(P(0), P(0), P(f.eta / lnd), P(0), P(0), P(1 / lnd), P(0), P(-1 / lnd), P(0));
+    const M3<P> G2 = rows(P(f.eta12 / 2), P(-f.eta11 / 2), P(0), P(0), P(0), P(0), P(0), P(0), P(0));
+    const M3<P> G3 = -G1;
+    const M3<P> G4 = rows(P(f.eta22 / 2), P(-f.eta21 / 2), P(0), P(0), P(0), P(0), P(0), P(0), P(0));
+    const V3<P> Ud1 = tmul(Re, cvt<P>(ldv(ns.ud, na))), Ud2 = tmul(Re, cvt<P>(ldv(ns.ud, nb)));
+    const V3<P> Wd1 = tmul(Re, cvt<P>(ldv(ns.wd, na))), Wd2 = tmul(Re, cvt<P>(ldv(ns.wd, nb)));
+    const M3<P> SUd1 = skew(Ud1), SUd2 = skew(Ud2), SWd1 = skew(Wd1), SWd2 = skew(Wd2);
+    const M3<P> G1R = G1 * ReT, G2R = G2 * ReT, G4R = G4 * ReT;
+    const M3<P> RG1 = Re * tr(G1), RG2 = Re * tr(G2), RG4 = Re * tr(G4);
+    const V3<P> v1 = cvt<P>(f.v1);
+    const P kn = P(ca.kn), mu = P(ca.mu), eps_t = P(ca.eps_t);
+    const V3<P> tb1 = cvt<P>(Tb1), tb2 = cvt<P>(Tb2);
+    P eta_n = P(ca.eta_n);  // overwritten by the smoothstep at each contacting point (:100)
+    double energy = 0;
+    V3<P> Tc[4];
+    M3<P> Kc[4][4], Cc[4][4];
+    for (int a = 0; a < 4; ++a) {
+        Tc[a] = zv<P>();
+        for (int b = 0; b < 4; ++b) { Kc[a][b] = zm<P>(); Cc[a][b] = zm<P>(); }
+    }
+    for (int ig = 0; ig < gp.n; ++ig) {
+        if (!(mask & (1u << ig))) continue;
+        const P w = P(gp.w[ig]);
+        ShapeF sd = shape_fn(l0, gp.z[ig]);
+        V3<double> utd = {0.0, sd.N3 * Tb1.z + sd.N4 * Tb2.z, (-sd.N3) * Tb1.y + (-sd.N4) * Tb2.y};
+        V3<double> xG = (sd.N1 * x1 + sd.N2 * x2) + f.Re * utd;
+        const P N1 = P(sd.N1), N2 = P(sd.N2), N3 = P(sd.N3), N4 = P(sd.N4), N7 = P(sd.N7);
+        V3<P> ut = {P(0), N3 * tb1.z + N4 * tb2.z, (-N3) * tb1.y + (-N4) * tb2.y};
+        M3<P> Sut = skew(ut);
+        P N7l = N7 / ln, N7l2 = N7l / ln;
+        M3<P> PP1 = rows(P(0), P(0), P(0), P(0), N7l, P(0), P(0), P(0), N7l);
+        M3<P> PP2 = rows(P(0), P(0), P(0), P(0), P(0), N3, P(0), -N3, P(0));
+        M3<P> PP3 = -PP1;
+        M3<P> PP4 = rows(P(0), P(0), P(0), P(0), P(0), N4, P(0), -N4, P(0));
+        M3<P> H1[4];
+        H1[0] = (N1 * I + PP1) - Sut * G1;
+        H1[1] = PP2 - Sut * G2;
+        H1[2] = (N2 * I + PP3) - Sut * G3;
+        H1[3] = PP4 - Sut * G4;
+        V3<P> h1 = ((H1[0] * Ud1 + H1[1] * Wd1) + H1[2] * Ud2) + H1[3] * Wd2;
+        M3<P> Sh1 = skew(h1);
+        M3<P> H1F1 = ((H1[0] * SUd1 + H1[1] * SWd1) + H1[2] * SUd2) + H1[3] * SWd2;
+        V3<P> ud0 = Re * h1;
+
+        P gn;
+        V3<P> n;
+        M3<P> Hg;
+        contact_gap<P>(ca, xG, rb, gn, n, Hg);
+        // regularize_gap_penalty_beams
+        P gb = P(rb) / P(4), hh = gb / P(2), pn, dpn, Pi;
+        if (gn <= P(0)) {
+            pn = hh - gn; dpn = P(-1);
+            Pi = ((gn * gn) / P(2) - hh * gn) + (gb * gb) / P(6);
+        } else {
+            P aux = (gb - hh) / (gb * gb);
+            pn = (aux * (gn * gn) - gn) + hh;
+            dpn = (P(2) * aux) * gn - P(1);
+            Pi = (((gb - hh) / (P(3) * (gb * gb))) * ((gn * gn) * gn) - (gn * gn) / P(2) + hh * gn) - (gb * gb) / P(6);
+        }
+        // smoothstep_transition_beams
+        P deta;
+        if (gn <= P(0)) {
+            deta = P(0);
+        } else {
+            P ss = (gn - gb) / gb;
+            deta = (((P(6) / gb) * ss) * (P(1) + ss)) * eta_n;
+            eta_n = ((ss * ss) * (P(3) + P(2) * ss)) * eta_n;
+        }
+        P udn_mag = dot(ud0, n);
+        V3<P> udn = udn_mag * n;
+        V3<P> udt = ud0 - udn;
+        P udt2 = dot(udt, udt);
+        energy += double((w * kn) * Pi);
+        V3<P> fn = (kn * pn) * n - eta_n * udn;
+        P mureg = mu / sqrt(udt2 + eps_t);
+        V3<P> ft = (((-kn) * pn) * mureg) * udt;
+        V3<P> fc = fn + ft;
+        V3<P> Fc = tmul(Re, fc);
+        M3<P> RHR[4];
+        for (int a = 0; a < 4; ++a) RHR[a] = (Re * H1[a]) * ReT;
+        for (int a = 0; a < 4; ++a) Tc[a] += w * tmul(RHR[a], fc);
+        M3<P> A1F = rows(P(0), P(0), P(0), Fc.y * v1.x, Fc.y * v1.y, Fc.y * v1.z, Fc.z * v1.x, Fc.z * v1.y, Fc.z * v1.z);
+        M3<P> SF = skew(Fc);
+        M3<P> SFP1 = (SF * PP1) * ReT, SFP2 = (SF * PP2) * ReT, SFP4 = (SF * PP4) * ReT;
+        M3<P> aux = dot(n, ud0) * Hg + outer(n, Hg * ud0);
+        M3<P> RSh1 = Re * Sh1;
+        M3<P> A1Dr = rows(P(0), P(0), P(0), v1.x * (Ud1.y - Ud2.y), v1.y * (Ud1.y - Ud2.y), v1.z * (Ud1.y - Ud2.y),
+                          v1.x * (Ud1.z - Ud2.z), v1.y * (Ud1.z - Ud2.z), v1.z * (Ud1.z - Ud2.z));
+        M3<P> RSGD = Re * skew(((G1 * Ud1 + G2 * Wd1) + G3 * Ud2) + G4 * Wd2);
+        M3<P> RH1SD = Re * H1F1;
+        M3<P> du0dd[4];
+        du0dd[0] = ((-(RSh1 * G1R)) + ((N7l2 * Re) * A1Dr + (RSGD * PP1) * ReT)) + RH1SD * G1R;
+        du0dd[1] = ((-(RSh1 * G2R)) + (RSGD * PP2) * ReT) + RH1SD * G2R;
+        du0dd[2] = -du0dd[0];
+        du0dd[3] = ((-(RSh1 * G4R)) + (RSGD * PP4) * ReT) + RH1SD * G4R;
+        M3<P> nn = outer(n, n);
+        M3<P> auxn = ((kn * dpn) * nn + (kn * pn) * Hg) - outer(deta * udn, n);
+        M3<P> aux1 = outer((((-mureg) * kn) * dpn) * udt, n);
+        M3<P> aux2 = (((-mureg) * kn) * pn) * (I - outer((P(1) / (udt2 + eps_t)) * udt, udt));
+        M3<P> Kfc[4], Cfc[4];
+        for (int b = 0; b < 4; ++b) {
+            M3<P> dundd = aux * RHR[b] + nn * du0dd[b];
+            M3<P> dutdd = du0dd[b] - dundd;
+            M3<P> dutdv = RHR[b] - nn * RHR[b];
+            M3<P> Kfn = auxn * RHR[b] - eta_n * dundd;
+            M3<P> Cfn = ((-eta_n) * nn) * RHR[b];
+            M3<P> Kft = aux1 * RHR[b] + aux2 * dutdd;
+            M3<P> Cft = aux2 * dutdv;
+            Kfc[b] = Kfn + Kft;
+            Cfc[b] = Cfn + Cft;
+        }
+        for (int a = 0; a < 4; ++a) {
+            M3<P> RS = Re * skew(tmul(H1[a], Fc));
+            M3<P> RHS = (Re * tr(H1[a])) * SF;
+            M3<P> tRHR = tr(RHR[a]);
+            M3<P> RGa = a == 0 ? RG1 : (a == 1 ? RG2 : (a == 2 ? -RG1 : RG4));
+            for (int b = 0; b < 4; ++b) {
+                M3<P> t1, t2, t3;
+                // t₁ᵃᵇ = -Rₑ S(H₁ᵃᵀ𝓕ᶜ) Gᵇ Rₑᵀ with G³ = -G¹
+                M3<P> GbR = b == 0 ? G1R : (b == 1 ? G2R : (b == 2 ? G1R : G4R));
+                t1 = b == 2 ? RS * GbR : -(RS * GbR);
+                t3 = b == 2 ? -(RHS * GbR) : RHS * GbR;
+                // t₂ (:166-184): rows 1 and 3 carry the A₁ᵀ𝓕ᶜr term; row 3 mirrors row 1
+                M3<P> SFPb = b == 0 ? SFP1 : (b == 1 ? SFP2 : (b == 2 ? SFP1 : SFP4));
+                if (a == 0 || a == 2) {
+                    M3<P> base;
+                    if (b == 0 || b == 2) base = (N7l2 * Re) * A1F - RG1 * SFP1;
+                    else base = -(RG1 * SFPb);
+                    // t₂¹¹ = base₁, t₂¹³ = -base₁, t₂³¹ = t₂¹³, t₂³³ = t₂¹¹; t₂¹² = base₂, t₂³² = -t₂¹²; likewise col 4
+                    bool neg = (b == 2) != (a == 2);
+                    t2 = neg ? -base : base;
+                } else {
+                    M3<P> base = -(RGa * SFP1);
+                    if (b == 0) t2 = base;
+                    else if (b == 2) t2 = -base;
+                    else t2 = -(RGa * SFPb);
+                }
+                M3<P> t4 = tRHR * Kfc[b];
+                Kc[a][b] += w * (((t1 + t2) + t3) + t4);
+                Cc[a][b] += (w * tRHR) * Cfc[b];
+            }
+        }
+    }
+    const P l02 = P(l0 / 2);
+    ca.energy[3 * e + 2] = double(l02) * energy;
+    const M3<P> TT1t = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, na)))));
+    const M3<P> TT2t = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, nb)))));
+    const P c1a = P(1 + ca.alpha);
+    for (int a = 0; a < 4; ++a) {
+        V3<P> t = l02 * Tc[a];
+        ca.Tc[e * 12 + 3 * a] = t.x;
+        ca.Tc[e * 12 + 3 * a + 1] = t.y;
+        ca.Tc[e * 12 + 3 * a + 2] = t.z;
+        for (int b = 0; b < 4; ++b) {
+            M3<P> K = l02 * Kc[a][b];
+            M3<P> Cm = l02 * Cc[a][b];
+            if (a == 1) Cm = Cm * TT1t;
+            if (a == 3) Cm = Cm * TT2t;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    int64_t idx = e * 144 + (3 * b + j) * 12 + 3 * a + i;
+                    st.K[idx] += -K.a[i + 3 * j];
+                    st.C[idx] += -(c1a * Cm.a[i + 3 * j]);
+                }
+        }
+    }
+}
+
+}  // namespace ebd

@@ -1,0 +1,624 @@
+// eb_linalg.h — assembly gathers, HHT-α combine, boundary-condition lift, masked block-row SpMV,
+// chain block-tridiagonal preconditioner, BiCGStab kernels, Newmark/rotation updates and FP64
+// reductions for the device Newton iteration.
+#pragma once
+#include "eb_dmath.h"
+
+namespace ebd {
+
+constexpr int RED_BLOCK = 256;
+constexpr int RED_MAX_GRID = 1024;
+
+// ---- deterministic reductions ---------------------------------------------------------------
+
+template <int NQ> EBD void block_sum_store(double (&v)[NQ], double* partials, int grid) {
+    __shared__ double sh[NQ][RED_BLOCK / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double x = v[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) sh[q][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double s = 0;
+            for (int w = 0; w < RED_BLOCK / 64; ++w) s += sh[q][w];
+            partials[q * RED_MAX_GRID + blockIdx.x] = s;
+        }
+    }
+}
+
+// sums partials[q][0..grid) in fixed order inside one 256-thread block; result to out[q]
+template <int NQ> EBD void finalize_sum(const double* partials, int grid, double (&out)[NQ]) {
+    __shared__ double sh[NQ][RED_BLOCK / 64];
+    __shared__ double res[NQ];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double x = 0;
+        for (int i = threadIdx.x; i < grid; i += RED_BLOCK) x += partials[q * RED_MAX_GRID + i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) sh[q][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double s = 0;
+            for (int w = 0; w < RED_BLOCK / 64; ++w) s += sh[q][w];
+            res[q] = s;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) out[q] = res[q];
+}
+
+// ---- assembly gathers (atomic-free, beam order = the reference's serial scatter order) ------
+
+// K, C, M (and optionally Ktan = (1+α)K + γ/(βΔt) C + M/(βΔt²)) for every block-row entry
+template <typename P>
+__global__ void __launch_bounds__(256) gather_matrices(int64_t nnz, const int32_t* __restrict__ blk_ptr,
+                                                       const uint32_t* __restrict__ blk_src, const P* __restrict__ sK,
+                                                       const P* __restrict__ sC, const P* __restrict__ sM, double* K,
+                                                       double* C, double* M, double* Ktan, double c1, double c2,
+                                                       double c3, int with_tangent) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nnz) return;
+    const int64_t b = q / 36;
+    const int r = (int)(q % 36);
+    const int a = r / 6, c = r % 6;
+    double k = 0, cc = 0, m = 0;
+    for (int s = blk_ptr[b]; s < blk_ptr[b + 1]; ++s) {
+        uint32_t src = blk_src[s];
+        int64_t e = src >> 2;
+        int sr = (src >> 1) & 1, sc = src & 1;
+        int64_t idx = e * 144 + (6 * sc + c) * 12 + 6 * sr + a;
+        k += double(sK[idx]);
+        cc += double(sC[idx]);
+        m += double(sM[idx]);
+    }
+    K[q] = k;
+    C[q] = cc;
+    M[q] = m;
+    if (with_tangent) Ktan[q] = (c1 * k + c2 * cc) + c3 * m;
+}
+
+// Tⁱⁿᵗ, Tᵏ, Tᶜ per dof, optionally followed by the corrector residual (utils_solver.jl:87)
+template <typename P>
+__global__ void __launch_bounds__(256) gather_vectors(int64_t ndofs, const int32_t* __restrict__ inc_ptr,
+                                                      const uint32_t* __restrict__ inc_src, const P* __restrict__ sTint,
+                                                      const P* __restrict__ sTk, const P* __restrict__ sTc, int with_contact,
+                                                      double* Tint, double* Tk, double* Tc, int with_residual, double alpha,
+                                                      const double* fext, const double* fext_n, const double* Tc_n,
+                                                      const double* Tint_n, double* res) {
+    int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= ndofs) return;
+    const int64_t n = d / 6;
+    const int c = (int)(d % 6);
+    double ti = 0, tk = 0, tc = 0;
+    for (int s = inc_ptr[n]; s < inc_ptr[n + 1]; ++s) {
+        uint32_t src = inc_src[s];
+        int64_t e = src >> 1;
+        int slot = src & 1;
+        int64_t idx = e * 12 + 6 * slot + c;
+        tk += double(sTk[idx]);
+        ti += double(sTint[idx]);
+        if (with_contact) tc += double(sTc[idx]);
+    }
+    Tint[d] = ti;
+    Tk[d] = tk;
+    Tc[d] = tc;
+    if (with_residual)
+        res[d] = ((1 + alpha) * ((fext[d] + tc) - ti) - alpha * ((fext_n[d] + Tc_n[d]) - Tint_n[d])) - tk;
+}
+
+// energies: Σ over beams of [E][3] staging, added to the running totals (never reset, as in
+// the reference's Energy accumulation, assemble_beams.jl:56-57)
+__global__ void __launch_bounds__(RED_BLOCK) energy_partials(int64_t E, const double* st, double* partials, int mask) {
+    double v[3] = {0, 0, 0};
+    for (int64_t e = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; e < E; e += (int64_t)gridDim.x * RED_BLOCK) {
+        if (mask & 1) { v[0] += st[3 * e]; v[1] += st[3 * e + 1]; }
+        if (mask & 2) v[2] += st[3 * e + 2];
+    }
+    block_sum_store<3>(v, partials, gridDim.x);
+}
+__global__ void __launch_bounds__(RED_BLOCK) energy_finalize(const double* partials, int grid, double* energy) {
+    double out[3];
+    finalize_sum<3>(partials, grid, out);
+    if (threadIdx.x == 0) { energy[0] += out[0]; energy[1] += out[1]; energy[2] += out[2]; }
+}
+
+// ---- HHT combine and residuals (src/utils_solver.jl:54-89) ----------------------------------
+
+__global__ void __launch_bounds__(256) combine_tangent(int64_t nnz, const double* K, const double* C, const double* M,
+                                                       double* Ktan, double c1, double c2, double c3) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nnz) Ktan[q] = (c1 * K[q] + c2 * C[q]) + c3 * M[q];
+}
+
+__global__ void __launch_bounds__(256) residual_corrector(int64_t ndofs, double alpha, const double* fext,
+                                                          const double* Tc, const double* Tint, const double* fext_n,
+                                                          const double* Tc_n, const double* Tint_n, const double* Tk,
+                                                          double* res) {
+    int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < ndofs)
+        res[d] = ((1 + alpha) * ((fext[d] + Tc[d]) - Tint[d]) - alpha * ((fext_n[d] + Tc_n[d]) - Tint_n[d])) - Tk[d];
+}
+
+// predictor residual: r = (1+α)f - Tⁱⁿᵗⁿ - Tᵏⁿ - αfⁿ + Cⁿ·t₁ + (1/(βΔt²))·Mⁿ·t₂, the two products
+// accumulated row-wise in increasing column order (what the CSC column sweep of mul! yields per row)
+__global__ void __launch_bounds__(256) residual_predictor(int64_t ndofs, const int32_t* __restrict__ bptr,
+                                                          const int32_t* __restrict__ bcol, double alpha, double t1c,
+                                                          double t2c, double dt, double c3, const double* fext,
+                                                          const double* Tint_n, const double* Tk_n,
+                                                          const double* fext_n, const double* Dd, const double* Ddd,
+                                                          const double* Cn, const double* Mn, double* res) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndofs) return;
+    const int64_t n = i / 6;
+    const int a = (int)(i % 6);
+    double r = (((1 + alpha) * fext[i] - Tint_n[i]) - Tk_n[i]) - alpha * fext_n[i];
+    for (int b = bptr[n]; b < bptr[n + 1]; ++b) {
+        const int64_t m = bcol[b];
+        const double* row = Cn + (int64_t)36 * b + 6 * a;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            int64_t j = 6 * m + c;
+            r += row[c] * (t1c * Dd[j] - t2c * Ddd[j]);
+        }
+    }
+    for (int b = bptr[n]; b < bptr[n + 1]; ++b) {
+        const int64_t m = bcol[b];
+        const double* row = Mn + (int64_t)36 * b + 6 * a;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            int64_t j = 6 * m + c;
+            r += row[c] * (c3 * (dt * Dd[j] + ((dt * dt) / 2) * Ddd[j]));
+        }
+    }
+    res[i] = r;
+}
+
+// apply_boundary_conditions! (utils_bcs_and_loads.jl:5-29): ΔD[d] = imposed - D[d] on fixed dofs
+// and r -= ΔD[d]·Ktan[:, d], subtracted per row in increasing fixed-dof order
+__global__ void __launch_bounds__(256) bc_lift(int64_t ndofs, const int32_t* __restrict__ bptr,
+                                               const int32_t* __restrict__ bcol, const int32_t* __restrict__ fixidx,
+                                               const double* imposed, const double* D, const double* Ktan, double* dD,
+                                               double* res) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndofs) return;
+    if (fixidx[i] >= 0) dD[i] = imposed[fixidx[i]] - D[i];
+    const int64_t n = i / 6;
+    const int a = (int)(i % 6);
+    double r = res[i];
+    bool any = false;
+    for (int b = bptr[n]; b < bptr[n + 1]; ++b) {
+        const int64_t m = bcol[b];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            int64_t j = 6 * m + c;
+            int fj = fixidx[j];
+            if (fj >= 0) {
+                r -= (imposed[fj] - D[j]) * Ktan[(int64_t)36 * b + 6 * a + c];
+                any = true;
+            }
+        }
+    }
+    if (any) res[i] = r;
+}
+
+// ---- masked block-row SpMV: y = A_ff x on free dofs, y = x on fixed dofs ---------------------
+// (x is zero on fixed dofs throughout the Krylov iteration). Optional dot products with y.
+template <int NQ>
+__global__ void __launch_bounds__(RED_BLOCK) spmv_masked(int64_t ndofs, const int32_t* __restrict__ bptr,
+                                                         const int32_t* __restrict__ bcol, const uint8_t* __restrict__ fixed,
+                                                         const double* __restrict__ A, const double* __restrict__ x,
+                                                         double* __restrict__ y, const double* __restrict__ dotw,
+                                                         double* partials, const int* done) {
+    if (*done) return;
+    double v[NQ > 0 ? NQ : 1] = {0};
+    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < ndofs; i += (int64_t)gridDim.x * RED_BLOCK) {
+        double acc;
+        if (fixed[i]) {
+            acc = x[i];
+        } else {
+            const int64_t n = i / 6;
+            const int a = (int)(i % 6);
+            acc = 0;
+            for (int b = bptr[n]; b < bptr[n + 1]; ++b) {
+                const double* row = A + (int64_t)36 * b + 6 * a;
+                const double* xs = x + 6 * (int64_t)bcol[b];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) acc += row[c] * xs[c];
+            }
+        }
+        y[i] = acc;
+        if (NQ >= 1) v[0] += dotw[i] * acc;
+        if (NQ >= 2) v[NQ - 1] += acc * acc;
+    }
+    if (NQ > 0) block_sum_store<(NQ > 0 ? NQ : 1)>(v, partials, gridDim.x);
+}
+
+// ---- chain block-tridiagonal preconditioner ------------------------------------------------
+// The node graph is covered by chains (paths); along a chain the masked tangent restricted to
+// the chain is block-tridiagonal with 6x6 blocks and is factored exactly (block LU without
+// inter-block pivoting, Gauss-Jordan with partial pivoting inside each 6x6 Schur complement).
+// Couplings between chains are left to the Krylov iteration. One 64-lane workgroup per chain.
+
+struct ChainDev {
+    const int32_t* ptr;   // n_chains+1
+    const int32_t* node;  // chain-ordered nodes
+    const int32_t *diag_blk, *lo_blk, *up_blk;
+    double *Sinv, *G, *Lm;  // 36 doubles per chain position
+    int64_t n_chains;
+};
+
+EBD double masked_entry(const double* A, int64_t blk, int32_t nr, int32_t nc, int a, int c, const uint8_t* fixed) {
+    bool fx = fixed[6 * (int64_t)nr + a] || fixed[6 * (int64_t)nc + c];
+    if (fx) return (nr == nc && a == c) ? 1.0 : 0.0;
+    return A[36 * blk + 6 * a + c];
+}
+
+__global__ void __launch_bounds__(64) chain_factor(ChainDev cd, const double* __restrict__ A,
+                                                   const uint8_t* __restrict__ fixed, int* status) {
+    const int64_t ch = blockIdx.x;
+    const int l = threadIdx.x;
+    const int r = l / 6, c = l % 6;
+    const bool act = l < 36;
+    __shared__ double W[6][13];
+    __shared__ double Gs[36], Ls[36];
+    __shared__ int piv;
+    const int32_t s0 = cd.ptr[ch], s1 = cd.ptr[ch + 1];
+    for (int32_t i = s0; i < s1; ++i) {
+        const int32_t n = cd.node[i];
+        double d = 0;
+        if (act) d = masked_entry(A, cd.diag_blk[i], n, n, r, c, fixed);
+        if (i > s0) {
+            if (act) {
+                double lv = masked_entry(A, cd.lo_blk[i], n, cd.node[i - 1], r, c, fixed);
+                Ls[l] = lv;
+                cd.Lm[(int64_t)36 * i + l] = lv;
+            }
+            __syncthreads();
+            if (act) {
+                double acc = 0;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) acc += Ls[6 * r + k] * Gs[6 * k + c];
+                d -= acc;
+            }
+        }
+        if (act) { W[r][c] = d; W[r][6 + c] = (r == c) ? 1.0 : 0.0; }
+        __syncthreads();
+        // Gauss-Jordan with partial pivoting on [S | I]
+        for (int p = 0; p < 6; ++p) {
+            if (l == 0) {
+                int best = p;
+                double bv = fabs(W[p][p]);
+                for (int q = p + 1; q < 6; ++q)
+                    if (fabs(W[q][p]) > bv) { bv = fabs(W[q][p]); best = q; }
+                piv = best;
+                if (bv == 0.0) *status = 1;
+            }
+            __syncthreads();
+            const int pv = piv;
+            if (pv != p && l < 12) { double t = W[p][l]; W[p][l] = W[pv][l]; W[pv][l] = t; }
+            __syncthreads();
+            double f = 0, pr_c = 0, pr_c6 = 0;
+            const double pivot = W[p][p];
+            if (act) { f = W[r][p]; pr_c = W[p][c]; pr_c6 = W[p][6 + c]; }
+            __syncthreads();
+            if (act) {
+                if (r == p) {
+                    W[r][c] = pr_c / pivot;
+                    W[r][6 + c] = pr_c6 / pivot;
+                } else {
+                    double fp = f / pivot;
+                    W[r][c] -= fp * pr_c;
+                    W[r][6 + c] -= fp * pr_c6;
+                }
+            }
+            __syncthreads();
+        }
+        double sinv = act ? W[r][6 + c] : 0.0;
+        if (act) cd.Sinv[(int64_t)36 * i + l] = sinv;
+        if (i + 1 < s1) {
+            // G_{i+1} = S_i⁻¹ · A(n_i, n_{i+1})
+            if (act) Ls[l] = masked_entry(A, cd.up_blk[i], n, cd.node[i + 1], r, c, fixed);
+            __syncthreads();
+            if (act) {
+                double acc = 0;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) acc += W[r][6 + k] * Ls[6 * k + c];
+                Gs[l] = acc;
+                cd.G[(int64_t)36 * (i + 1) + l] = acc;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// z = M⁻¹ b along each chain: forward y_i = b_i - L_i z_{i-1}, z_i = S_i⁻¹ y_i;
+// backward x_i = z_i - G_{i+1} x_{i+1}. x and b may not alias.
+__global__ void __launch_bounds__(64) chain_solve(ChainDev cd, const double* __restrict__ b, double* __restrict__ x,
+                                                  const int* done) {
+    if (*done) return;
+    const int64_t ch = blockIdx.x;
+    const int l = threadIdx.x;
+    __shared__ double zv[6], yv[6];
+    const int32_t s0 = cd.ptr[ch], s1 = cd.ptr[ch + 1];
+    for (int32_t i = s0; i < s1; ++i) {
+        const int64_t base = 6 * (int64_t)cd.node[i];
+        if (l < 6) {
+            double y = b[base + l];
+            if (i > s0) {
+                const double* L = cd.Lm + (int64_t)36 * i + 6 * l;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) y -= L[k] * zv[k];
+            }
+            yv[l] = y;
+        }
+        __syncthreads();
+        if (l < 6) {
+            const double* S = cd.Sinv + (int64_t)36 * i + 6 * l;
+            double z = 0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) z += S[k] * yv[k];
+            x[base + l] = z;
+            zv[l] = z;
+        }
+        __syncthreads();
+    }
+    // backward: zv holds x_{i+1}
+    for (int32_t i = s1 - 2; i >= s0; --i) {
+        const int64_t base = 6 * (int64_t)cd.node[i];
+        double xi = 0;
+        if (l < 6) {
+            const double* Gm = cd.G + (int64_t)36 * (i + 1) + 6 * l;
+            xi = x[base + l];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) xi -= Gm[k] * zv[k];
+        }
+        __syncthreads();
+        if (l < 6) { x[base + l] = xi; zv[l] = xi; }
+        __syncthreads();
+    }
+}
+
+// ---- BiCGStab (van der Vorst), right-preconditioned, FP64 scalars on the device -------------
+
+struct KrylovCtl {
+    double rho, rho_old, alpha, omega, bnorm2, rr2, ss2, tol2;
+    int done, iters, breakdown, pad;
+};
+
+// x = 0, r = r̂ = b (b = residual with fixed dofs zeroed), ‖b‖²
+__global__ void __launch_bounds__(RED_BLOCK) krylov_init(int64_t ndofs, const uint8_t* fixed, const double* res, double* x,
+                                                         double* r, double* rhat, double* p, double* v, double* partials) {
+    double s[1] = {0};
+    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < ndofs; i += (int64_t)gridDim.x * RED_BLOCK) {
+        double bi = fixed[i] ? 0.0 : res[i];
+        x[i] = 0; r[i] = bi; rhat[i] = bi; p[i] = 0; v[i] = 0;
+        s[0] += bi * bi;
+    }
+    block_sum_store<1>(s, partials, gridDim.x);
+}
+__global__ void __launch_bounds__(RED_BLOCK) krylov_init_fin(const double* partials, int grid, KrylovCtl* k, double tol) {
+    double o[1];
+    finalize_sum<1>(partials, grid, o);
+    if (threadIdx.x == 0) {
+        k->bnorm2 = o[0]; k->rho = o[0]; k->rho_old = 1; k->alpha = 1; k->omega = 1;
+        k->rr2 = o[0]; k->ss2 = 0; k->tol2 = tol * tol; k->iters = 0; k->breakdown = 0;
+        k->done = (o[0] == 0.0) ? 1 : 0;
+    }
+}
+// p = r + β(p - ωv), β = (ρ/ρ_old)(α/ω); first iteration p = r
+__global__ void __launch_bounds__(256) krylov_p(int64_t ndofs, const KrylovCtl* k, const double* r, double* p, const double* v) {
+    if (k->done) return;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndofs) return;
+    if (k->iters == 0) { p[i] = r[i]; return; }
+    double beta = (k->rho / k->rho_old) * (k->alpha / k->omega);
+    p[i] = r[i] + beta * (p[i] - k->omega * v[i]);
+}
+__global__ void __launch_bounds__(RED_BLOCK) krylov_alpha_fin(const double* partials, int grid, KrylovCtl* k) {
+    if (k->done) return;
+    double o[1];
+    finalize_sum<1>(partials, grid, o);
+    if (threadIdx.x == 0) {
+        if (o[0] == 0.0 || !isfinite(o[0])) { k->breakdown = 1; k->done = 1; return; }
+        k->alpha = k->rho / o[0];
+    }
+}
+// s = r - αv, x += αy, ‖s‖²
+__global__ void __launch_bounds__(RED_BLOCK) krylov_s(int64_t ndofs, const KrylovCtl* k, const double* r, const double* v,
+                                                      const double* y, double* s, double* x, double* partials) {
+    if (k->done) return;
+    const double al = k->alpha;
+    double q[1] = {0};
+    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < ndofs; i += (int64_t)gridDim.x * RED_BLOCK) {
+        double si = r[i] - al * v[i];
+        s[i] = si;
+        x[i] += al * y[i];
+        q[0] += si * si;
+    }
+    block_sum_store<1>(q, partials, gridDim.x);
+}
+__global__ void __launch_bounds__(RED_BLOCK) krylov_s_fin(const double* partials, int grid, KrylovCtl* k) {
+    if (k->done) return;
+    double o[1];
+    finalize_sum<1>(partials, grid, o);
+    if (threadIdx.x == 0) {
+        k->ss2 = o[0];
+        if (o[0] <= k->tol2 * k->bnorm2) { k->rr2 = o[0]; k->iters += 1; k->done = 1; }
+    }
+}
+__global__ void __launch_bounds__(RED_BLOCK) krylov_omega_fin(const double* partials, int grid, KrylovCtl* k) {
+    if (k->done) return;
+    double o[2];
+    finalize_sum<2>(partials, grid, o);
+    if (threadIdx.x == 0) {
+        if (o[1] == 0.0 || !isfinite(o[1])) { k->breakdown = 1; k->done = 1; return; }
+        k->omega = o[0] / o[1];
+    }
+}
+// x += ωz, r = s - ωt, (r̂·r), ‖r‖²
+__global__ void __launch_bounds__(RED_BLOCK) krylov_x(int64_t ndofs, const KrylovCtl* k, const double* z, const double* s,
+                                                      const double* t, const double* rhat, double* x, double* r,
+                                                      double* partials) {
+    if (k->done) return;
+    const double om = k->omega;
+    double q[2] = {0, 0};
+    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < ndofs; i += (int64_t)gridDim.x * RED_BLOCK) {
+        x[i] += om * z[i];
+        double ri = s[i] - om * t[i];
+        r[i] = ri;
+        q[0] += rhat[i] * ri;
+        q[1] += ri * ri;
+    }
+    block_sum_store<2>(q, partials, gridDim.x);
+}
+__global__ void __launch_bounds__(RED_BLOCK) krylov_rho_fin(const double* partials, int grid, KrylovCtl* k) {
+    if (k->done) return;
+    double o[2];
+    finalize_sum<2>(partials, grid, o);
+    if (threadIdx.x == 0) {
+        k->rho_old = k->rho;
+        k->rho = o[0];
+        k->rr2 = o[1];
+        k->iters += 1;
+        if (o[1] <= k->tol2 * k->bnorm2) k->done = 1;
+        else if (o[0] == 0.0 || !isfinite(o[0])) { k->breakdown = 1; k->done = 1; }
+    }
+}
+// ΔD[free] = x[free]
+__global__ void __launch_bounds__(256) scatter_solution(int64_t ndofs, const uint8_t* fixed, const double* x, double* dD) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ndofs && !fixed[i]) dD[i] = x[i];
+}
+
+// ---- Newmark / rotation updates (src/utils_solver.jl:6-48, :157-245) -----------------------
+
+struct NodeFields { double *u, *ud, *udd, *w, *wd, *wdd, *R, *dR, *u_n, *ud_n, *udd_n, *w_n, *wd_n, *wdd_n, *R_n, *dR_n; };
+
+EBD void stv(double* a, int64_t n, V3<double> v) { a[3 * n] = v.x; a[3 * n + 1] = v.y; a[3 * n + 2] = v.z; }
+EBD V3<double> ldv3(const double* a, int64_t n) { return {a[3 * n], a[3 * n + 1], a[3 * n + 2]}; }
+EBD M3<double> ldm3(const double* a, int64_t n) { M3<double> m; for (int k = 0; k < 9; ++k) m.a[k] = a[9 * n + k]; return m; }
+EBD void stm(double* a, int64_t n, const M3<double>& m) { for (int k = 0; k < 9; ++k) a[9 * n + k] = m.a[k]; }
+
+// initialize_global_corrector! (predictor=0) / initialize_global_predictor! (predictor=1)
+__global__ void __launch_bounds__(256) init_global(int64_t N, NodeFields nf, double* D, double* Dd, double* Ddd, double* dD,
+                                                   int predictor) {
+    int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    for (int k = 0; k < 3; ++k) {
+        dD[6 * n + k] = 0; D[6 * n + k] = nf.u[3 * n + k]; Dd[6 * n + k] = nf.ud[3 * n + k]; Ddd[6 * n + k] = nf.udd[3 * n + k];
+        dD[6 * n + 3 + k] = 0; D[6 * n + 3 + k] = predictor ? 0.0 : nf.w[3 * n + k];
+        Dd[6 * n + 3 + k] = nf.wd[3 * n + k]; Ddd[6 * n + 3 + k] = nf.wdd[3 * n + k];
+    }
+}
+
+// compute_global_corrector! per dof (all dofs, fixed included)
+__global__ void __launch_bounds__(256) global_corrector(int64_t ndofs, double gb, double inv, double* D, double* Dd,
+                                                        double* Ddd, const double* dD) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndofs) return;
+    D[i] += dD[i]; Dd[i] += gb * dD[i]; Ddd[i] += inv * dD[i];
+}
+// compute_global_predictor!
+__global__ void __launch_bounds__(256) global_predictor(int64_t ndofs, double dt, double gob, double gbdt, double inv,
+                                                        double dt2b, double* D, double* Dd, double* Ddd, const double* dD) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ndofs) return;
+    double Ddn = Dd[i];
+    D[i] += dD[i];
+    Dd[i] += (gbdt * dD[i] - gob * Ddn) + dt2b * Ddd[i];
+    Ddd[i] += inv * ((dD[i] - dt * Ddn) - ((dt * dt) / 2) * Ddd[i]);
+}
+
+struct HHTc { double c_w, c_wd, c_wdd, d_w, d_wd, d_wdd; };  // ẇ = ΔR(c_w w + c_wd ẇⁿ + c_wdd ẅⁿ), ẅ likewise
+
+// compute_local_corrector!: u, u̇, ü from D; ΔR ← exp(ΔD_θ)ΔR; w = log ΔR; ẇ, ẅ; R = ΔR Rⁿ
+__global__ void __launch_bounds__(256) local_corrector(int64_t N, NodeFields nf, const double* D, const double* Dd,
+                                                       const double* Ddd, const double* dD, HHTc h) {
+    int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    stv(nf.u, n, {D[6 * n], D[6 * n + 1], D[6 * n + 2]});
+    stv(nf.ud, n, {Dd[6 * n], Dd[6 * n + 1], Dd[6 * n + 2]});
+    stv(nf.udd, n, {Ddd[6 * n], Ddd[6 * n + 1], Ddd[6 * n + 2]});
+    M3<double> dR = rotation_matrix({dD[6 * n + 3], dD[6 * n + 4], dD[6 * n + 5]}) * ldm3(nf.dR, n);
+    stm(nf.dR, n, dR);
+    V3<double> wdn = ldv3(nf.wd_n, n), wddn = ldv3(nf.wdd_n, n);
+    V3<double> w = toangle(dR);
+    V3<double> a = ((h.c_w * w + h.c_wd * wdn) + h.c_wdd * wddn);
+    V3<double> c = ((h.d_w * w - h.d_wd * wdn) - h.d_wdd * wddn);
+    stv(nf.wd, n, dR * a);
+    stv(nf.wdd, n, dR * c);
+    stm(nf.R, n, dR * ldm3(nf.R_n, n));
+}
+
+// compute_local_predictor!: w = θ̃ = D_θ, ΔR = exp(θ̃), R = ΔR Rⁿ, ẇ, ẅ from θ̃
+__global__ void __launch_bounds__(256) local_predictor(int64_t N, NodeFields nf, const double* D, const double* Dd,
+                                                       const double* Ddd, HHTc h) {
+    int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    stv(nf.u, n, {D[6 * n], D[6 * n + 1], D[6 * n + 2]});
+    stv(nf.ud, n, {Dd[6 * n], Dd[6 * n + 1], Dd[6 * n + 2]});
+    stv(nf.udd, n, {Ddd[6 * n], Ddd[6 * n + 1], Ddd[6 * n + 2]});
+    V3<double> th = {D[6 * n + 3], D[6 * n + 4], D[6 * n + 5]};
+    stv(nf.w, n, th);
+    M3<double> dR = rotation_matrix(th);
+    stm(nf.dR, n, dR);
+    stm(nf.R, n, dR * ldm3(nf.R_n, n));
+    V3<double> wdn = ldv3(nf.wd_n, n), wddn = ldv3(nf.wdd_n, n);
+    V3<double> a = ((h.c_w * th + h.c_wd * wdn) + h.c_wdd * wddn);
+    V3<double> c = ((h.d_w * th - h.d_wd * wdn) - h.d_wdd * wddn);
+    stv(nf.wd, n, dR * a);
+    stv(nf.wdd, n, dR * c);
+}
+
+// compute_norms_corrector (utils_solver.jl:126-151): Σ over free dofs of ΔD², r², (f+Tᶜ+Tᵏ)² and
+// Σ over fixed dofs of (f+Tᶜ+Tᵏ)²
+__global__ void __launch_bounds__(RED_BLOCK) norms_partials(int64_t ndofs, const uint8_t* fixed, const double* dD,
+                                                            const double* res, const double* fext, const double* Tc,
+                                                            const double* Tk, double* partials) {
+    double q[4] = {0, 0, 0, 0};
+    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < ndofs; i += (int64_t)gridDim.x * RED_BLOCK) {
+        double f = (fext[i] + Tc[i]) + Tk[i];
+        if (fixed[i]) {
+            q[3] += f * f;
+        } else {
+            q[0] += dD[i] * dD[i];
+            q[1] += res[i] * res[i];
+            q[2] += f * f;
+        }
+    }
+    block_sum_store<4>(q, partials, gridDim.x);
+}
+__global__ void __launch_bounds__(RED_BLOCK) norms_finalize(const double* partials, int grid, double* out2) {
+    double o[4];
+    finalize_sum<4>(partials, grid, o);
+    if (threadIdx.x == 0) {
+        double dn = sqrt(o[0]), rn = sqrt(o[1]), fn = sqrt(o[2]), en = sqrt(o[3]);
+        if (fn + en > 1e-12) rn = rn / (fn + en);
+        out2[0] = rn;
+        out2[1] = dn;
+    }
+}
+
+// ---- layout conversions at the ABI ----------------------------------------------------------
+__global__ void __launch_bounds__(256) gather_idx32(int64_t n, const int32_t* idx, const double* src, double* dst) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+__global__ void __launch_bounds__(256) scatter_idx32(int64_t n, const int32_t* idx, const double* src, double* dst) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[idx[i]] = src[i];
+}
+
+}  // namespace ebd
