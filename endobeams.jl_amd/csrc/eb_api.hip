@@ -85,6 +85,13 @@ struct eb_ctx {
     int precond_mode = 0;
     double* tmp = nullptr;  // nnz scratch for layout conversion
     int64_t counters[4] = {0, 0, 0, 0};
+    // launch timing
+    uint32_t prof_mask = 0;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    double prof_ms[EB_KERNEL_COUNT] = {};
+    int64_t prof_n[EB_KERNEL_COUNT] = {};
+    hipEvent_t ev_open[EB_KERNEL_COUNT] = {};
 };
 
 namespace {
@@ -103,6 +110,41 @@ namespace {
         int _s = (expr);            \
         if (_s != EB_OK) return _s; \
     } while (0)
+
+hipEvent_t pool_event(eb_ctx* ctx) {
+    if (!ctx->ev_pool.empty()) {
+        hipEvent_t e = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+inline void prof_begin(eb_ctx* ctx, int id) {
+    if (!(ctx->prof_mask & (1u << id))) return;
+    hipEvent_t e = pool_event(ctx);
+    if (e && hipEventRecord(e, ctx->stream) == hipSuccess) ctx->ev_open[id] = e;
+}
+inline void prof_end(eb_ctx* ctx, int id) {
+    if (!(ctx->prof_mask & (1u << id)) || !ctx->ev_open[id]) return;
+    hipEvent_t e = pool_event(ctx);
+    if (e && hipEventRecord(e, ctx->stream) == hipSuccess) ctx->ev_pending.push_back({id, {ctx->ev_open[id], e}});
+    ctx->ev_open[id] = nullptr;
+}
+// folds completed event pairs into the totals (the stream has been synchronised by the caller)
+void prof_harvest(eb_ctx* ctx) {
+    for (auto& p : ctx->ev_pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+            ctx->prof_ms[p.first] += ms;
+            ctx->prof_n[p.first] += 1;
+        }
+        ctx->ev_pool.push_back(p.second.first);
+        ctx->ev_pool.push_back(p.second.second);
+    }
+    ctx->ev_pending.clear();
+}
 
 inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 inline int red_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(RED_MAX_GRID, (n + RED_BLOCK - 1) / RED_BLOCK)); }
@@ -180,8 +222,10 @@ int unalias(eb_ctx* ctx) {
 // ---- assembly -------------------------------------------------------------------------------
 
 template <typename P> int launch_beams(eb_ctx* ctx) {
+    prof_begin(ctx, EB_KERNEL_BEAM);
     hipLaunchKernelGGL(beam_kernel<P>, dim3(nblk(ctx->E, 64)), dim3(64), 0, ctx->stream, node_state(ctx),
                        beam_consts(ctx), beam_stage<P>(ctx), ctx->gauss);
+    prof_end(ctx, EB_KERNEL_BEAM);
     ctx->counters[3]++;
     return check_launch(ctx, "beam_kernel");
 }
@@ -198,14 +242,17 @@ template <typename P> int launch_contact(eb_ctx* ctx, double alpha) {
     ca.n = ctx->n_contact;
     ca.Tc = (P*)ctx->sTc;
     ca.energy = ctx->senergy;
+    prof_begin(ctx, EB_KERNEL_CONTACT);
     hipLaunchKernelGGL(contact_kernel<P>, dim3(nblk(ctx->n_contact, 64)), dim3(64), 0, ctx->stream, node_state(ctx),
                        beam_consts(ctx), beam_stage<P>(ctx), ctx->gauss, ca);
+    prof_end(ctx, EB_KERNEL_CONTACT);
     return check_launch(ctx, "contact_kernel");
 }
 
 template <typename P> int launch_gathers(eb_ctx* ctx, bool with_contact, bool tangent, double dt, double a, double b,
                                          double g, bool residual, int energy_mask) {
     double c1 = 1 + a, c2 = tangent ? g / (b * dt) : 0, c3 = tangent ? 1 / (b * (dt * dt)) : 0;
+    prof_begin(ctx, EB_KERNEL_GATHER);
     hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz, ctx->blk_ptr,
                        ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM, ctx->K, ctx->C, ctx->M,
                        ctx->Ktan, c1, c2, c3, tangent ? 1 : 0);
@@ -219,6 +266,7 @@ template <typename P> int launch_gathers(eb_ctx* ctx, bool with_contact, bool ta
     hipLaunchKernelGGL(energy_partials, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, ctx->E, ctx->senergy,
                        ctx->partials, energy_mask);
     hipLaunchKernelGGL(energy_finalize, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->energy);
+    prof_end(ctx, EB_KERNEL_GATHER);
     ctx->np1_alias = false;
     return check_launch(ctx, "energy");
 }
@@ -247,8 +295,10 @@ int bc_lift_launch(eb_ctx* ctx) {
 
 int precond_factor(eb_ctx* ctx) {
     int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
+    prof_begin(ctx, EB_KERNEL_PRECOND);
     hipLaunchKernelGGL(chain_factor, dim3((unsigned)nch), dim3(64), 0, ctx->stream, chain_dev(ctx), ctx->Ktan,
                        ctx->is_fixed, ctx->kstatus);
+    prof_end(ctx, EB_KERNEL_PRECOND);
     return check_launch(ctx, "chain_factor");
 }
 
@@ -263,6 +313,7 @@ int krylov_iteration(eb_ctx* ctx) {
     const int64_t n = ctx->ndofs;
     const int grid = red_grid(n);
     int* done = &ctx->kctl->done;
+    prof_begin(ctx, EB_KERNEL_KRYLOV);
     hipLaunchKernelGGL(krylov_p, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, ctx->kctl, ctx->kr, ctx->kp, ctx->kv);
     EB_TRY(precond_apply(ctx, ctx->kp, ctx->ky));
     hipLaunchKernelGGL(spmv_masked<1>, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, n, ctx->bptr, ctx->bcol,
@@ -278,6 +329,7 @@ int krylov_iteration(eb_ctx* ctx) {
     hipLaunchKernelGGL(krylov_x, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, n, ctx->kctl, ctx->kz, ctx->ks, ctx->kt,
                        ctx->krhat, ctx->kx, ctx->kr, ctx->partials);
     hipLaunchKernelGGL(krylov_rho_fin, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->kctl);
+    prof_end(ctx, EB_KERNEL_KRYLOV);
     return check_launch(ctx, "krylov_iteration");
 }
 
@@ -328,12 +380,15 @@ int solve_free(eb_ctx* ctx, const eb_newton_params* p, int* its_out) {
 
 int norms(eb_ctx* ctx, double* rn, double* dn) {
     const int grid = red_grid(ctx->ndofs);
+    prof_begin(ctx, EB_KERNEL_UPDATE);
     hipLaunchKernelGGL(norms_partials, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, ctx->ndofs, ctx->is_fixed, ctx->dD,
                        ctx->res, ctx->fext, ctx->Tc, ctx->Tk, ctx->partials);
     hipLaunchKernelGGL(norms_finalize, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->norms);
+    prof_end(ctx, EB_KERNEL_UPDATE);
     EB_TRY(check_launch(ctx, "norms"));
     EB_HIP(hipMemcpyAsync(ctx->h_norms, ctx->norms, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     EB_HIP(hipStreamSynchronize(ctx->stream));
+    if (!ctx->ev_pending.empty()) prof_harvest(ctx);
     *rn = ctx->h_norms[0];
     *dn = ctx->h_norms[1];
     return EB_OK;
@@ -544,6 +599,8 @@ int eb_create(const eb_model_desc* d, eb_ctx** out) {
 int eb_destroy(eb_ctx* ctx) {
     if (!ctx) return EB_OK;
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (auto& p : ctx->ev_pending) { ctx->ev_pool.push_back(p.second.first); ctx->ev_pool.push_back(p.second.second); }
+    for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
     for (void* p : ctx->allocs) hipFree(p);
     if (ctx->h_kctl) hipHostFree(ctx->h_kctl);
     if (ctx->h_norms) hipHostFree(ctx->h_norms);
@@ -869,6 +926,24 @@ int eb_update_not_converged(eb_ctx* ctx) {
 int eb_get_counters(const eb_ctx* ctx, int64_t* out4) {
     if (!ctx || !out4) return EB_ERR_ARG;
     for (int k = 0; k < 4; ++k) out4[k] = ctx->counters[k];
+    return EB_OK;
+}
+
+int eb_profile_enable(eb_ctx* ctx, uint32_t mask) {
+    if (!ctx) return EB_ERR_ARG;
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    prof_harvest(ctx);
+    for (int k = 0; k < EB_KERNEL_COUNT; ++k) { ctx->prof_ms[k] = 0; ctx->prof_n[k] = 0; ctx->ev_open[k] = nullptr; }
+    ctx->prof_mask = mask;
+    return EB_OK;
+}
+
+int eb_profile_read(eb_ctx* ctx, int id, double* total_ms, int64_t* launches) {
+    if (!ctx || id < 0 || id >= EB_KERNEL_COUNT || !total_ms || !launches) return EB_ERR_ARG;
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    prof_harvest(ctx);
+    *total_ms = ctx->prof_ms[id];
+    *launches = ctx->prof_n[id];
     return EB_OK;
 }
 

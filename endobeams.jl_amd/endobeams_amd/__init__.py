@@ -8,7 +8,7 @@ from .model import (Beams, BeamElementSurface, BeamsConfiguration, BoundaryCondi
                     DiscreteSignedDistanceField, Encastre, ImposedDisplacement, InteractionProperties, Loads,
                     NodesBeams, RigidInteraction, SimulationParams, SphereSurface, read_vtk_sdf)
 from .engine import EBError, Engine, HipEngine, LIB_PATH, load_hip_library
-from .simulation import RunStats, SimulationAborted, run_simulation
+from .simulation import RunStats, SimulationAborted, Stepper, run_simulation
 from . import configs
 
 __all__ = [
@@ -16,5 +16,5 @@ __all__ = [
     "DiscreteSignedDistanceField", "Encastre", "ImposedDisplacement", "InteractionProperties", "Loads",
     "NodesBeams", "RigidInteraction", "SimulationParams", "SphereSurface", "read_vtk_sdf",
     "EBError", "Engine", "HipEngine", "LIB_PATH", "load_hip_library",
-    "RunStats", "SimulationAborted", "run_simulation", "configs",
+    "RunStats", "SimulationAborted", "Stepper", "run_simulation", "configs",
 ]

@@ -100,7 +100,11 @@ PROTOTYPES = {
     "update_converged": (_i, [_vp]),
     "update_not_converged": (_i, [_vp]),
     "get_counters": (_i, [_vp, _lp]),
+    "profile_enable": (_i, [_vp, C.c_uint32]),
+    "profile_read": (_i, [_vp, _i, _dp, _lp]),
 }
+
+KERNELS = {"beam": 0, "contact": 1, "gather": 2, "precond": 3, "krylov": 4, "update": 5}
 
 
 def bind(lib, prefix):

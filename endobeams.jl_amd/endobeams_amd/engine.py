@@ -108,6 +108,17 @@ class Engine:
         self._check(self.fn["get_counters"](self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "get_counters")
         return out
 
+    def profile_enable(self, kernels=()):
+        mask = 0
+        for k in kernels:
+            mask |= 1 << abi.KERNELS[k]
+        self._check(self.fn["profile_enable"](self.h, mask), "profile_enable")
+
+    def profile_read(self, kernel):
+        ms, n = C.c_double(0), C.c_int64(0)
+        self._check(self.fn["profile_read"](self.h, abi.KERNELS[kernel], C.byref(ms), C.byref(n)), "profile_read")
+        return ms.value, n.value
+
     # -- reference functions ----------------------------------------------------------------------
     def assemble(self):
         self._check(self.fn["assemble"](self.h), "assemble")

@@ -2,14 +2,13 @@
 
 The loop (adaptive Δt, rollback on Newton failure, abort below min_timestep, wall-clock stop) is
 the reference's caller of the hot path; it stays on the host and calls the engine once per step.
+`Stepper` exposes the same loop one converged step at a time (used by the benchmark).
 """
 from __future__ import annotations
 
 import time
 from dataclasses import dataclass, field
 from typing import Callable, List, Optional
-
-import numpy as np
 
 from .engine import Engine, HipEngine
 from .model import BeamsConfiguration, RigidInteraction, SimulationParams
@@ -33,6 +32,64 @@ class RunStats:
         return self.newton_iterations / max(self.steps, 1)
 
 
+class Stepper:
+    """State of run_simulation!'s time loop (run_simulation.jl:21-112) between steps."""
+
+    def __init__(self, engine: Engine, params: SimulationParams):
+        self.eng = engine
+        self.p = params
+        self.tn = 0.0
+        self.dt = params.initial_timestep
+        self.successive = 0
+        self.step_index = 1
+        self.stats = RunStats()
+        self.start = time.perf_counter()
+        self.stop = False
+
+    def initialize(self):
+        self.eng.initialize_state_simulation()
+
+    @property
+    def finished(self) -> bool:
+        return not (self.p.simulation_end_time - self.tn > 1.0e-10)
+
+    def advance(self, on_converged: Optional[Callable] = None) -> bool:
+        """Attempts one step; returns True when it converged (and was committed)."""
+        p = self.p
+        tnp1 = self.tn + self.dt
+        tcomp = time.perf_counter() - self.start
+        k = self.eng.solve_step_dynamics(tnp1, self.dt)
+        if k > p.max_iterations:
+            self.dt = self.dt / 2
+            self.stats.failed_steps += 1
+            if self.dt < p.min_timestep:
+                raise SimulationAborted(f"Δt < {p.min_timestep:.2e}: aborting simulation")
+            self.eng.update_not_converged()
+            self.successive = 0
+            return False
+        self.successive += 1
+        self.stats.steps += 1
+        self.stats.newton_iterations += k - 1
+        self.stats.iterations_per_step.append(k - 1)
+        if self.successive >= p.accelerate_after_success_it and k <= 8:
+            self.dt = min(self.dt * 1.5, p.max_timestep)
+            self.successive = 0
+        if on_converged is not None:
+            on_converged(self.step_index, tnp1, self.eng)
+        self.stop = tcomp > p.tcompt_max and p.stop_long_simulation
+        if self.stop:
+            return True
+        self.eng.update_converged()
+        self.tn = tnp1
+        self.step_index += 1
+        if p.stop_on_energy_threshold:
+            # the reference reads an undefined `energyⁿ⁺¹` here (run_simulation.jl:107); the
+            # intended quantity, state.energyⁿ⁺¹.kinetic_energy, is used
+            if self.eng.get("ENERGY")[1] < p.energy_threshold:
+                self.stop = True
+        return True
+
+
 def run_simulation(conf: BeamsConfiguration, params: SimulationParams, inter: Optional[RigidInteraction] = None,
                    engine: Optional[Engine] = None, on_converged: Optional[Callable] = None,
                    max_steps: Optional[int] = None) -> RunStats:
@@ -42,46 +99,12 @@ def run_simulation(conf: BeamsConfiguration, params: SimulationParams, inter: Op
     converged step, before update_converged! (where the reference writes VTK output).
     `max_steps` bounds the number of converged steps (None: run to simulation_end_time)."""
     eng = engine if engine is not None else HipEngine(conf, params, inter)
-    stats = RunStats()
-    tn = 0.0
-    dt = params.initial_timestep
-    successive = 0
-    step = 1
-    start = time.perf_counter()
-    eng.initialize_state_simulation()
-    while params.simulation_end_time - tn > 1.0e-10:
-        if max_steps is not None and stats.steps >= max_steps:
+    st = Stepper(eng, params)
+    st.initialize()
+    while not st.finished and not st.stop:
+        if max_steps is not None and st.stats.steps >= max_steps:
             break
-        tnp1 = tn + dt
-        tcomp = time.perf_counter() - start
-        k = eng.solve_step_dynamics(tnp1, dt)
-        if k > params.max_iterations:
-            dt = dt / 2
-            stats.failed_steps += 1
-            if dt < params.min_timestep:
-                raise SimulationAborted(f"Δt < {params.min_timestep:.2e}: aborting simulation")
-            eng.update_not_converged()
-            successive = 0
-            continue
-        successive += 1
-        stats.steps += 1
-        stats.newton_iterations += k - 1
-        stats.iterations_per_step.append(k - 1)
-        if successive >= params.accelerate_after_success_it and k <= 8:
-            dt = min(dt * 1.5, params.max_timestep)
-            successive = 0
-        if on_converged is not None:
-            on_converged(step, tnp1, eng)
-        if tcomp > params.tcompt_max and params.stop_long_simulation:
-            break
-        eng.update_converged()
-        tn = tnp1
-        step += 1
-        if params.stop_on_energy_threshold:
-            # the reference reads an undefined `energyⁿ⁺¹` here (run_simulation.jl:107); the
-            # intended quantity, state.energyⁿ⁺¹.kinetic_energy, is used
-            if eng.get("ENERGY")[1] < params.energy_threshold:
-                break
-    stats.t_end = tn
-    stats.wall_s = time.perf_counter() - start
-    return stats
+        st.advance(on_converged)
+    st.stats.t_end = st.tn
+    st.stats.wall_s = time.perf_counter() - st.start
+    return st.stats

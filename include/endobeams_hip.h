@@ -214,6 +214,19 @@ int eb_update_not_converged(eb_ctx* ctx);
  * [2] linear solves, [3] element-kernel launches */
 int eb_get_counters(const eb_ctx* ctx, int64_t* out4);
 
+/* per-kernel launch timing with HIP events recorded on the context's stream around every launch
+ * of the kernels selected in `mask` (bit i = kernel id i); enabling resets the totals */
+#define EB_KERNEL_BEAM 0          /* corotational element kernel */
+#define EB_KERNEL_CONTACT 1       /* contact + SDF kernel */
+#define EB_KERNEL_GATHER 2        /* assembly gathers (matrices, vectors, energies) */
+#define EB_KERNEL_PRECOND 3       /* chain factorisation */
+#define EB_KERNEL_KRYLOV 4        /* one BiCGStab iteration (all its kernels) */
+#define EB_KERNEL_UPDATE 5        /* Newmark/rotation update and norms */
+#define EB_KERNEL_COUNT 6
+int eb_profile_enable(eb_ctx* ctx, uint32_t mask);
+/* total milliseconds and number of timed launches of kernel `id` since enabling */
+int eb_profile_read(eb_ctx* ctx, int id, double* total_ms, int64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -687,6 +687,10 @@ int ebo_get_counters(void* h, int64_t* out4) {
     return EB_OK;
 }
 
+// kernel timing has no CPU counterpart: the oracle reports zero launches
+int ebo_profile_enable(void*, uint32_t) { return EB_OK; }
+int ebo_profile_read(void*, int, double* ms, int64_t* n) { *ms = 0; *n = 0; return EB_OK; }
+
 // ---- single-element probes for the oracle's own invariant tests ---------------------------
 
 // in: X1,X2,u1,u2,ud1,ud2,udd1,udd2,wd1,wd2,wdd1,wdd2 (12×3), R1,R2,dR1,dR2 (4×9), l0, Re0 (9)

@@ -1,0 +1,48 @@
+"""The C-ABI library loads (no GPU needed to load it) and exports every symbol the header
+declares; the oracle exports the same set under its own prefix."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import REPO
+
+HEADER = os.path.join(REPO, "include", "endobeams_hip.h")
+LIB = os.path.join(REPO, "endobeams.jl_amd", "lib", "libendobeams_hip.so")
+ORACLE = os.path.join(REPO, "oracle", "liboracle.so")
+
+
+def declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(eb_\w+)\(", text, flags=re.M)))
+
+
+def test_header_declares_the_entry_points():
+    names = declared()
+    for n in ("eb_create", "eb_assemble", "eb_assemble_contact", "eb_corrector", "eb_predictor",
+              "eb_solve_free_dofs", "eb_update_converged"):
+        assert n in names
+    assert len(names) >= 30
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_oracle_exports_the_same_abi():
+    import oracle as O
+    lib = O.load()
+    missing = [n for n in declared() if not hasattr(lib, "ebo_" + n[3:])]
+    assert not missing, missing
+
+
+def test_python_bindings_cover_the_abi():
+    import sys
+    sys.path.insert(0, os.path.join(REPO, "endobeams.jl_amd"))
+    from endobeams_amd import abi
+    bound = {"eb_" + k for k in abi.PROTOTYPES}
+    assert bound == set(declared())
