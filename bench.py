@@ -24,7 +24,7 @@ METRIC = "element-steps/s & Newton iters/step, learned+FP32 vs plain FP64 GPU, s
 # FP64 operations (+, -, ×, ÷; transcendentals and sqrt excluded) of one compute_beams call with
 # 3 Gauss points and no damping, counted by the op-counting build of the CPU restatement
 # (tests/test_flop_count.py keeps this figure in sync).
-FLOPS_PER_BEAM = 26469
+FLOPS_PER_BEAM = 26547
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 

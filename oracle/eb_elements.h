@@ -12,11 +12,11 @@
 namespace ebo {
 
 struct BeamProps {  // BeamMaterialProperties (src/beams/def_beam.jl:19-26)
-    double radius, E, Ku;
+    real radius, E, Ku;
     V3 KT, KTT, Jrho;
-    double Arho, damping;
+    real Arho, damping;
 };
-inline BeamProps props_from(const double* p) {
+inline BeamProps props_from(const real* p) {
     BeamProps b;
     b.radius = p[0]; b.E = p[1]; b.Ku = p[2];
     b.KT = v3(p[3], p[4], p[5]); b.KTT = v3(p[6], p[7], p[8]); b.Jrho = v3(p[9], p[10], p[11]);
@@ -24,18 +24,18 @@ inline BeamProps props_from(const double* p) {
     return b;
 }
 
-struct Gauss { int n; double w[8], z[8]; };
+struct Gauss { int n; real w[8], z[8]; };
 
 struct ElemIn {
     V3 X1, X2, u1, u2, ud1, ud2, udd1, udd2, wd1, wd2, wdd1, wdd2;
     M3 R1, R2, dR1, dR2;
-    double l0;
+    real l0;
     M3 Re0;
 };
 
 // 12x12 column-major, local dof order [u₁ θ₁ u₂ θ₂]
 struct Mat12 {
-    double a[144];
+    real a[144];
     void set_block(int bi, int bj, const M3& B) {
         for (int j = 0; j < 3; ++j)
             for (int i = 0; i < 3; ++i) a[(3 * bj + j) * 12 + 3 * bi + i] = B(i, j);
@@ -43,21 +43,21 @@ struct Mat12 {
 };
 
 struct BeamOut {
-    double strain, kinetic;
-    double Tint[12], Tk[12];
+    real strain, kinetic;
+    real Tint[12], Tk[12];
     Mat12 K, M, C;
 };
 
-inline void set_vec(double* t, int b, const V3& v) { t[3 * b] = v[0]; t[3 * b + 1] = v[1]; t[3 * b + 2] = v[2]; }
+inline void set_vec(real* t, int b, const V3& v) { t[3 * b] = v[0]; t[3 * b + 1] = v[1]; t[3 * b + 2] = v[2]; }
 
 // shape functions of compute_beams.jl:238-248 (identical in the contact kernel :58-68)
-struct Shape { double N1, N2, N3, N4, N5, N6, N7, N8; };
-inline Shape shape(double l0, double z) {
+struct Shape { real N1, N2, N3, N4, N5, N6, N7, N8; };
+inline Shape shape(real l0, real z) {
     Shape s;
-    double xi = l0 * (z + 1) / 2;
+    real xi = l0 * (z + 1) / 2;
     s.N1 = 1 - xi / l0;
     s.N2 = 1 - s.N1;
-    double a = 1 - xi / l0;
+    real a = 1 - xi / l0;
     s.N3 = xi * (a * a);
     s.N4 = -(1 - xi / l0) * ((xi * xi) / l0);
     s.N5 = (1 - 3 * xi / l0) * (1 - xi / l0);
@@ -71,8 +71,8 @@ inline Shape shape(double l0, double z) {
 inline void compute_beams(const ElemIn& in, const BeamProps& bp, const Gauss& g, BeamOut& out) {
     const M3 I = eye();
     V3 x1 = in.X1 + in.u1, x2 = in.X2 + in.u2;
-    double ln = norm(x2 - x1);
-    double ub = ln - in.l0;
+    real ln = norm(x2 - x1);
+    real ub = ln - in.l0;
     LocalFrame f = local_Re_and_aux(x1, x2, in.R1, in.R2, col(in.Re0, 1), ln);
     const M3& Re = f.Re;
     M3 ReT = tr(Re);
@@ -89,9 +89,9 @@ inline void compute_beams(const ElemIn& in, const BeamProps& bp, const Gauss& g,
     M3 B11 = Ti1 * Bp11, B12 = Ti1 * Bp12, B14 = Ti1 * Bp14;
     M3 B21 = Ti2 * Bp21, B22 = Ti2 * Bp22, B24 = Ti2 * Bp24;
 
-    double Ku = bp.Ku;
+    real Ku = bp.Ku;
     M3 KT = diag(bp.KT[0], bp.KT[1], bp.KT[2]), KTT = diag(bp.KTT[0], bp.KTT[1], bp.KTT[2]);
-    double Tu = Ku * ub;
+    real Tu = Ku * ub;
     V3 TT1 = KT * Tb1 + KTT * Tb2;
     V3 TT2 = KTT * Tb1 + KT * Tb2;
     out.strain = (ub * Tu + dot(Tb1, TT1) + dot(Tb2, TT2)) / 2;
@@ -102,7 +102,7 @@ inline void compute_beams(const ElemIn& in, const BeamProps& bp, const Gauss& g,
     V3 T4 = tr(B14) * TT1 + tr(B24) * TT2;
     set_vec(out.Tint, 0, T1); set_vec(out.Tint, 1, T2); set_vec(out.Tint, 2, T3); set_vec(out.Tint, 3, T4);
 
-    double Nb = Tu;
+    real Nb = Tu;
     V3 Mp1 = tr(Ti1) * TT1, Mp2 = tr(Ti2) * TT2;
     V3 Qs1 = tr(P11) * Mp1 + tr(P21) * Mp2;
     V3 Qs2 = tr(P12) * Mp1 + tr(P22) * Mp2;
@@ -120,7 +120,7 @@ inline void compute_beams(const ElemIn& in, const BeamProps& bp, const Gauss& g,
     M3 Km24 = -((Re * QG24) * ReT);
     M3 Km44 = -((Re * QG44) * ReT);
 
-    double eta1, mu1, eta2, mu2;
+    real eta1, mu1, eta2, mu2;
     eta_mu(Tb1, eta1, mu1);
     eta_mu(Tb2, eta2, mu2);
     M3 Kh1 = Kh_bar(Tb1, TT1, Ti1, eta1, mu1);
@@ -149,12 +149,12 @@ inline void compute_beams(const ElemIn& in, const BeamProps& bp, const Gauss& g,
     out.K.set_block(0, 3, K14); out.K.set_block(1, 3, K24); out.K.set_block(2, 3, K34); out.K.set_block(3, 3, K44);
 
     // ---- dynamic part, compute_beams.jl:161-517
-    double kinetic = 0.0;
+    real kinetic = 0.0;
     V3 Tk1 = zv(), Tk2 = zv(), Tk3 = zv(), Tk4 = zv();
     M3 M11 = zm(), M12 = zm(), M13 = zm(), M14 = zm(), M22 = zm(), M23 = zm(), M24 = zm(), M33 = zm(), M34 = zm(), M44 = zm();
     M3 C11 = zm(), C12 = zm(), C14 = zm(), C21 = zm(), C22 = zm(), C24 = zm(), C31 = zm(), C32 = zm(), C34 = zm(),
        C41 = zm(), C42 = zm(), C44 = zm();
-    const double Ar = bp.Arho, damping = bp.damping;
+    const real Ar = bp.Arho, damping = bp.damping;
     M3 Jr = diag(bp.Jrho[0], bp.Jrho[1], bp.Jrho[2]);
 
     V3 Ud1 = ReT * in.ud1, Ud2 = ReT * in.ud2, Wd1 = ReT * in.wd1, Wd2 = ReT * in.wd2;
@@ -162,17 +162,17 @@ inline void compute_beams(const ElemIn& in, const BeamProps& bp, const Gauss& g,
     M3 SUd1 = skew(Ud1), SUd2 = skew(Ud2), SWd1 = skew(Wd1), SWd2 = skew(Wd2);
     V3 We = ((G1 * Ud1 + G2 * Wd1) + G3 * Ud2) + G4 * Wd2;
     M3 SWe = skew(We);
-    double rd = dot(f.r1, in.ud1) + dot(f.r3, in.ud2);
+    real rd = dot(f.r1, in.ud1) + dot(f.r3, in.ud2);
 
     for (int ig = 0; ig < g.n; ++ig) {
-        double z = g.z[ig], w = g.w[ig];
+        real z = g.z[ig], w = g.w[ig];
         Shape s = shape(in.l0, z);
         V3 ut = v3(0, s.N3 * Tb1[2] + s.N4 * Tb2[2], (-s.N3) * Tb1[1] + (-s.N4) * Tb2[1]);
         V3 Thb = v3(s.N1 * Tb1[0] + s.N2 * Tb2[0], s.N5 * Tb1[1] + s.N6 * Tb2[1], s.N5 * Tb1[2] + s.N6 * Tb2[2]);
         M3 Sut = skew(ut), STh = skew(Thb);
         M3 Rbar = I + STh;
         M3 Ir = (Rbar * Jr) * tr(Rbar);
-        double N7l = s.N7 / ln, N7l2 = N7l / ln, N8l = s.N8 / ln, N8l2 = N8l / ln;
+        real N7l = s.N7 / ln, N7l2 = N7l / ln, N8l = s.N8 / ln, N8l2 = N8l / ln;
         M3 PP1 = rows(0, 0, 0, 0, N7l, 0, 0, 0, N7l);
         M3 PP2 = rows(0, 0, 0, 0, 0, s.N3, 0, -s.N3, 0);
         M3 PP3 = -PP1;
@@ -187,11 +187,11 @@ inline void compute_beams(const ElemIn& in, const BeamProps& bp, const Gauss& g,
         M3 H24 = diag(s.N2, s.N6, s.N6);
         V3 udt = ((PP1 * Ud1 + PP2 * Wd1) + PP3 * Ud2) + PP4 * Wd2;
         M3 Sudt = skew(udt);
-        double N7rd = N7l2 * rd;
+        real N7rd = N7l2 * rd;
         M3 Hd11 = diag(0, -N7rd, -N7rd) - Sudt * G1;
         M3 Hd12 = -(Sudt * G2);
         M3 Hd14 = -(Sudt * G4);
-        double N8rd = N8l2 * rd;
+        real N8rd = N8l2 * rd;
         M3 Hd21 = rows(0, 0, 0, 0, 0, N8rd, 0, -N8rd, 0);
         V3 h1 = ((H11 * Ud1 + H12 * Wd1) + H13 * Ud2) + H14 * Wd2;
         V3 h2 = ((H21 * Ud1 + H22 * Wd1) + H23 * Ud2) + H24 * Wd2;
@@ -281,7 +281,7 @@ inline void compute_beams(const ElemIn& in, const BeamProps& bp, const Gauss& g,
         M3 Irg = (Re * Ir) * ReT;
         kinetic += (w / 2) * (Ar * dot(ud0, ud0) + dot(wd0, Irg * wd0));
     }
-    double l02 = in.l0 / 2;
+    real l02 = in.l0 / 2;
     M3 l02Re = l02 * Re;
     Tk1 = l02Re * Tk1; Tk2 = l02Re * Tk2; Tk3 = l02Re * Tk3; Tk4 = l02Re * Tk4;
     M11 = (l02Re * M11) * ReT; M12 = (l02Re * M12) * ReT; M13 = (l02Re * M13) * ReT; M14 = (l02Re * M14) * ReT;
@@ -327,31 +327,31 @@ inline void compute_beams(const ElemIn& in, const BeamProps& bp, const Gauss& g,
 
 // ---- contact -------------------------------------------------------------------------------
 
-struct ContactProps { double kn, mu, eps_t, eta_n; };
+struct ContactProps { real kn, mu, eps_t, eta_n; };
 
 // quadratic B-spline SDF, Interpolations.jl BSpline(Quadratic(Reflect(OnCell()))) + scale
 // (src/interactions/def_interactions.jl:67-82). Coefficients padded by one cell on each side.
 struct SDF {
     int64_t n[3];
-    double d[3], x0[3];
-    std::vector<double> coef;  // (n0+2)*(n1+2)*(n2+2), x fastest
-    double dom[6];
-    double c(int64_t i, int64_t j, int64_t k) const {
+    real d[3], x0[3];
+    std::vector<real> coef;  // (n0+2)*(n1+2)*(n2+2), x fastest
+    real dom[6];
+    real c(int64_t i, int64_t j, int64_t k) const {
         return coef[(size_t)((k * (n[1] + 2) + j) * (n[0] + 2) + i)];
     }
 };
 
 // in-place solve of the (n+2)-point system  c₀ = c₁,  c_{i-1}/8 + 3c_i/4 + c_{i+1}/8 = y_i,
 // c_{n+1} = c_n  along one line (Thomas algorithm on the reduced n×n system)
-inline void prefilter_line(const double* y, double* c, int64_t n, std::vector<double>& cp, std::vector<double>& dp) {
+inline void prefilter_line(const real* y, real* c, int64_t n, std::vector<real>& cp, std::vector<real>& dp) {
     if (n == 1) { c[1] = y[0]; c[0] = c[1]; c[2] = c[1]; return; }
     cp.resize(n); dp.resize(n);
     // row 0: (3/4 + 1/8) c1 + 1/8 c2 = y0 ; row n-1: 1/8 c_{n-1} + (3/4+1/8) c_n = y_{n-1}
     for (int64_t i = 0; i < n; ++i) {
-        double a = (i == 0) ? 0.0 : 0.125;
-        double b = (i == 0 || i == n - 1) ? 0.875 : 0.75;
-        double cc = (i == n - 1) ? 0.0 : 0.125;
-        double den = b - (i > 0 ? a * cp[i - 1] : 0.0);
+        real a = (i == 0) ? 0.0 : 0.125;
+        real b = (i == 0 || i == n - 1) ? 0.875 : 0.75;
+        real cc = (i == n - 1) ? 0.0 : 0.125;
+        real den = b - (i > 0 ? a * cp[i - 1] : 0.0);
         cp[i] = cc / den;
         dp[i] = (y[i] - (i > 0 ? a * dp[i - 1] : 0.0)) / den;
     }
@@ -361,15 +361,15 @@ inline void prefilter_line(const double* y, double* c, int64_t n, std::vector<do
     c[n + 1] = c[n];
 }
 
-inline void sdf_build(SDF& s, const int64_t n[3], const double d[3], const double x0[3], const double* samples) {
+inline void sdf_build(SDF& s, const int64_t n[3], const real d[3], const real x0[3], const real* samples) {
     for (int a = 0; a < 3; ++a) { s.n[a] = n[a]; s.d[a] = d[a]; s.x0[a] = x0[a]; }
     s.dom[0] = x0[0]; s.dom[1] = x0[0] + (n[0] - 1) * d[0];
     s.dom[2] = x0[1]; s.dom[3] = x0[1] + (n[1] - 1) * d[1];
     s.dom[4] = x0[2]; s.dom[5] = x0[2] + (n[2] - 1) * d[2];
     int64_t P0 = n[0] + 2, P1 = n[1] + 2, P2 = n[2] + 2;
     s.coef.assign((size_t)(P0 * P1 * P2), 0.0);
-    auto at = [&](int64_t i, int64_t j, int64_t k) -> double& { return s.coef[(size_t)((k * P1 + j) * P0 + i)]; };
-    std::vector<double> y, c, cp, dp;
+    auto at = [&](int64_t i, int64_t j, int64_t k) -> real& { return s.coef[(size_t)((k * P1 + j) * P0 + i)]; };
+    std::vector<real> y, c, cp, dp;
     // x lines (interior j,k)
     y.resize(n[0]); c.resize(n[0] + 2);
     for (int64_t k = 0; k < n[2]; ++k)
@@ -398,33 +398,33 @@ inline void sdf_build(SDF& s, const int64_t n[3], const double d[3], const doubl
 
 // nearest knot and offset along one axis: t = (x - x0)/dx + 1 (1-based data index),
 // ix = round-half-up(t) limited to [1, n], δ = t - ix
-inline void sdf_axis(const SDF& s, int a, double x, int64_t& ix, double& dl) {
-    double t = (x - s.x0[a]) / s.d[a] + 1.0;
-    double r = std::floor(t + 0.5);
+inline void sdf_axis(const SDF& s, int a, real x, int64_t& ix, real& dl) {
+    real t = (x - s.x0[a]) / s.d[a] + 1.0;
+    real r = floor(t + 0.5);
     if (r < 1) r = 1;
-    if (r > (double)s.n[a]) r = (double)s.n[a];
+    if (r > (real)s.n[a]) r = (real)s.n[a];
     ix = (int64_t)r;
     dl = t - r;
 }
 
 // value, gradient, Hessian of the interpolant (Interpolations.jl value/gradient/hessian weights)
-inline void sdf_eval(const SDF& s, const V3& p, double& val, V3& grad, M3& hess, bool need_derivs) {
-    int64_t ix[3]; double dl[3];
-    double w[3][3], wd[3][3], wdd[3][3];
+inline void sdf_eval(const SDF& s, const V3& p, real& val, V3& grad, M3& hess, bool need_derivs) {
+    int64_t ix[3]; real dl[3];
+    real w[3][3], wd[3][3], wdd[3][3];
     for (int a = 0; a < 3; ++a) {
         sdf_axis(s, a, p[a], ix[a], dl[a]);
-        double dd = dl[a];
+        real dd = dl[a];
         w[a][0] = (dd - 0.5) * (dd - 0.5) / 2; w[a][1] = 0.75 - dd * dd; w[a][2] = (dd + 0.5) * (dd + 0.5) / 2;
         wd[a][0] = (dd - 0.5) / s.d[a]; wd[a][1] = (-2 * dd) / s.d[a]; wd[a][2] = (dd + 0.5) / s.d[a];
-        double h2 = s.d[a] * s.d[a];
+        real h2 = s.d[a] * s.d[a];
         wdd[a][0] = 1.0 / h2; wdd[a][1] = -2.0 / h2; wdd[a][2] = 1.0 / h2;
     }
     // padded coefficient index of knot ix (1-based data index) is ix; neighbours ix-1, ix+1
-    double v = 0, gx = 0, gy = 0, gz = 0, hxx = 0, hyy = 0, hzz = 0, hxy = 0, hxz = 0, hyz = 0;
+    real v = 0, gx = 0, gy = 0, gz = 0, hxx = 0, hyy = 0, hzz = 0, hxy = 0, hxz = 0, hyz = 0;
     for (int c = 0; c < 3; ++c)
         for (int b = 0; b < 3; ++b)
             for (int a = 0; a < 3; ++a) {
-                double q = s.c(ix[0] - 1 + a, ix[1] - 1 + b, ix[2] - 1 + c);
+                real q = s.c(ix[0] - 1 + a, ix[1] - 1 + b, ix[2] - 1 + c);
                 v += w[0][a] * w[1][b] * w[2][c] * q;
                 if (need_derivs) {
                     gx += wd[0][a] * w[1][b] * w[2][c] * q;
@@ -447,60 +447,60 @@ inline void sdf_eval(const SDF& s, const V3& p, double& val, V3& grad, M3& hess,
 
 struct Surface {
     int kind = 0;  // EB_SURFACE_*
-    V3 center; double R = 0;
+    V3 center; real R = 0;
     SDF sdf;
 };
 
 // isinside, utils_compute_interactions_beams.jl:39-47
-inline bool isinside(const V3& p, const double* dom) {
-    double lx = p[0] - dom[0]; bool fx = lx >= 0 && lx <= (dom[1] - dom[0]);
-    double ly = p[1] - dom[2]; bool fy = ly >= 0 && ly <= (dom[3] - dom[2]);
-    double lz = p[2] - dom[4]; bool fz = lz >= 0 && lz <= (dom[5] - dom[4]);
+inline bool isinside(const V3& p, const real* dom) {
+    real lx = p[0] - dom[0]; bool fx = lx >= 0 && lx <= (dom[1] - dom[0]);
+    real ly = p[1] - dom[2]; bool fy = ly >= 0 && ly <= (dom[3] - dom[2]);
+    real lz = p[2] - dom[4]; bool fz = lz >= 0 && lz <= (dom[5] - dom[4]);
     return fx && fy && fz;
 }
 
 // incontact_beams, utils_compute_interactions_beams.jl:23-36 (sphere) and :67-75 (SDF)
-inline bool incontact(const V3& p, const Surface& s, double rb) {
-    double gbar = rb / 4;
+inline bool incontact(const V3& p, const Surface& s, real rb) {
+    real gbar = rb / 4;
     if (s.kind == 1) {
         V3 aux = p - s.center;
-        double g = (norm(aux) - s.R) - rb;
+        real g = (norm(aux) - s.R) - rb;
         return g <= gbar;
     }
     if (!isinside(p, s.sdf.dom)) return false;
-    double v; V3 gr; M3 h;
+    real v; V3 gr; M3 h;
     sdf_eval(s.sdf, p, v, gr, h, false);
     return v - rb <= gbar;
 }
 
 // contact_gap_beams, utils_compute_interactions_beams.jl:2-20 (sphere) and :50-64 (SDF)
-inline void contact_gap(const V3& p, const Surface& s, double rb, double& g, V3& n, M3& H) {
+inline void contact_gap(const V3& p, const Surface& s, real rb, real& g, V3& n, M3& H) {
     if (s.kind == 1) {
         V3 aux = p - s.center;
-        double na = norm(aux);
+        real na = norm(aux);
         g = (na - s.R) - rb;
-        double inv = 1 / na;
+        real inv = 1 / na;
         n = inv * aux;
         H = inv * eye() - ((inv * inv) * inv) * outer(aux, aux);
         return;
     }
-    double v; V3 gr; M3 h;
+    real v; V3 gr; M3 h;
     sdf_eval(s.sdf, p, v, gr, h, true);
-    double nn = dot(gr, gr);
-    double nmaginv = 1 / std::sqrt(nn);
+    real nn = dot(gr, gr);
+    real nmaginv = 1 / sqrt(nn);
     n = gr * nmaginv;
     H = (h * nmaginv) * (eye() - outer(n, n) / nn);
     g = v - rb;
 }
 
 // regularize_gap_penalty_beams, interactions_beams_regularization.jl:2-26
-inline void regularize_gap_penalty(double g, double rb, double& p, double& dp, double& Pi) {
-    double gb = rb / 4, h = gb / 2;
+inline void regularize_gap_penalty(real g, real rb, real& p, real& dp, real& Pi) {
+    real gb = rb / 4, h = gb / 2;
     if (g <= 0) {
         p = h - g; dp = -1.0;
         Pi = ((g * g) / 2 - h * g) + (gb * gb) / 6;
     } else {
-        double aux = (gb - h) / (gb * gb);
+        real aux = (gb - h) / (gb * gb);
         p = (aux * (g * g) - g) + h;
         dp = (2 * aux) * g - 1;
         Pi = (((gb - h) / (3 * (gb * gb))) * ((g * g) * g) - (g * g) / 2 + h * g) - (gb * gb) / 6;
@@ -508,27 +508,27 @@ inline void regularize_gap_penalty(double g, double rb, double& p, double& dp, d
 }
 
 // smoothstep_transition_beams, interactions_beams_regularization.jl:29-45
-inline void smoothstep(double tv, double pos, double rb, double& v, double& dv) {
-    double ul = rb / 4;
+inline void smoothstep(real tv, real pos, real rb, real& v, real& dv) {
+    real ul = rb / 4;
     if (pos <= 0) { v = tv; dv = 0.0; return; }
-    double s = (pos - ul) / ul;
+    real s = (pos - ul) / ul;
     v = ((s * s) * (3 + 2 * s)) * tv;
     dv = (((6 / ul) * s) * (1 + s)) * tv;
 }
 
 struct ContactOut {
-    double energy;
-    double Tc[12];
+    real energy;
+    real Tc[12];
     Mat12 K, C;
     int active;  // number of Gauss points in contact
 };
 
 // compute_contact_beams, src/interactions/compute&assemble_interactions_beams.jl:1-445
-inline void compute_contact_beams(const ElemIn& in, double rb, const ContactProps& cp, const Surface& surf,
+inline void compute_contact_beams(const ElemIn& in, real rb, const ContactProps& cp, const Surface& surf,
                                   const Gauss& g, ContactOut& out) {
     const M3 I = eye();
     V3 x1 = in.X1 + in.u1, x2 = in.X2 + in.u2;
-    double ln = norm(x2 - x1);
+    real ln = norm(x2 - x1);
     LocalFrame f = local_Re_and_aux(x1, x2, in.R1, in.R2, col(in.Re0, 1), ln);
     const M3& Re = f.Re;
     M3 ReT = tr(Re);
@@ -540,19 +540,19 @@ inline void compute_contact_beams(const ElemIn& in, double rb, const ContactProp
     M3 SUd1 = skew(Ud1), SUd2 = skew(Ud2), SWd1 = skew(Wd1), SWd2 = skew(Wd2);
     M3 G1R = G1 * ReT, G2R = G2 * ReT, G4R = G4 * ReT;
     M3 RG1 = Re * tr(G1), RG2 = Re * tr(G2), RG4 = Re * tr(G4);
-    double energy = 0;
+    real energy = 0;
     M3 Kc[4][4], Cc[4][4];
     V3 Tc[4];
     for (int a = 0; a < 4; ++a) { Tc[a] = zv(); for (int b = 0; b < 4; ++b) { Kc[a][b] = zm(); Cc[a][b] = zm(); } }
-    double eta_n = cp.eta_n;  // overwritten by the smoothstep at each contacting Gauss point (:100)
+    real eta_n = cp.eta_n;  // overwritten by the smoothstep at each contacting Gauss point (:100)
     int active = 0;
     for (int ig = 0; ig < g.n; ++ig) {
-        double z = g.z[ig], w = g.w[ig];
+        real z = g.z[ig], w = g.w[ig];
         Shape s = shape(in.l0, z);
         V3 ut = v3(0, s.N3 * Tb1[2] + s.N4 * Tb2[2], (-s.N3) * Tb1[1] + (-s.N4) * Tb2[1]);
         M3 Sut = skew(ut);
         V3 xG = (s.N1 * x1 + s.N2 * x2) + Re * ut;
-        double N7l = s.N7 / ln, N7l2 = N7l / ln;
+        real N7l = s.N7 / ln, N7l2 = N7l / ln;
         M3 PP1 = rows(0, 0, 0, 0, N7l, 0, 0, 0, N7l);
         M3 PP2 = rows(0, 0, 0, 0, 0, s.N3, 0, -s.N3, 0);
         M3 PP3 = -PP1;
@@ -568,19 +568,19 @@ inline void compute_contact_beams(const ElemIn& in, double rb, const ContactProp
         V3 ud0 = Re * h1;
         if (!incontact(xG, surf, rb)) continue;
         ++active;
-        double gn; V3 n; M3 Hg;
+        real gn; V3 n; M3 Hg;
         contact_gap(xG, surf, rb, gn, n, Hg);
-        double pn, dpn, Pi;
+        real pn, dpn, Pi;
         regularize_gap_penalty(gn, rb, pn, dpn, Pi);
-        double deta;
+        real deta;
         smoothstep(eta_n, gn, rb, eta_n, deta);
-        double udn_mag = dot(ud0, n);
+        real udn_mag = dot(ud0, n);
         V3 udn = udn_mag * n;
         V3 udt = ud0 - udn;
-        double udt2 = dot(udt, udt);
+        real udt2 = dot(udt, udt);
         energy += (w * cp.kn) * Pi;
         V3 fn = (cp.kn * pn) * n - eta_n * udn;
-        double mureg = cp.mu / std::sqrt(udt2 + cp.eps_t);
+        real mureg = cp.mu / sqrt(udt2 + cp.eps_t);
         V3 ft = (((-cp.kn) * pn) * mureg) * udt;
         V3 fc = fn + ft;
         V3 Fc = ReT * fc;
@@ -673,7 +673,7 @@ inline void compute_contact_beams(const ElemIn& in, double rb, const ContactProp
                 Cc[a][b] += (w * tr(RHR[a])) * Cfc[b];
             }
     }
-    double l02 = in.l0 / 2;
+    real l02 = in.l0 / 2;
     for (int a = 0; a < 4; ++a) {
         Tc[a] = l02 * Tc[a];
         for (int b = 0; b < 4; ++b) { Cc[a][b] = l02 * Cc[a][b]; Kc[a][b] = l02 * Kc[a][b]; }
