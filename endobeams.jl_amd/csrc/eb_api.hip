@@ -564,7 +564,11 @@ int eb_create(const eb_model_desc* d, eb_ctx** out) {
                 ctx->sdf_dom[2 * a + 1] = d->sdf_x0[a] + (d->sdf_n[a] - 1) * d->sdf_d[a];
             }
             std::vector<double> coef;
-            ebh::prefilter_sdf(d->sdf_n, d->sdf_samples, coef);
+            if (d->sdf_prefiltered) {
+                coef.assign(d->sdf_samples, d->sdf_samples + (d->sdf_n[0] + 2) * (d->sdf_n[1] + 2) * (d->sdf_n[2] + 2));
+            } else {
+                ebh::prefilter_sdf(d->sdf_n, d->sdf_samples, coef);
+            }
             if (ctx->precision == EB_PRECISION_FP32) {
                 std::vector<float> cf(coef.begin(), coef.end());
                 float* p;

@@ -40,7 +40,7 @@ class ModelDesc(C.Structure):
         ("n_fixed", C.c_int64), ("fixed_dofs", _lp),
         ("n_gauss", C.c_int32), ("pad0", C.c_int32),
         ("gauss_w", _dp), ("gauss_z", _dp),
-        ("surface", C.c_int32), ("pad1", C.c_int32),
+        ("surface", C.c_int32), ("sdf_prefiltered", C.c_int32),
         ("kn", C.c_double), ("mu", C.c_double), ("eps_t", C.c_double), ("eta_n", C.c_double),
         ("n_contact_beams", C.c_int64), ("contact_beams", _lp),
         ("sphere_center", C.c_double * 3), ("sphere_radius", C.c_double),

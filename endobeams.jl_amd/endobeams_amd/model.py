@@ -252,6 +252,7 @@ class DiscreteSignedDistanceField:
         if s.size != n[0] * n[1] * n[2]:
             raise ValueError("sample count does not match grid dimensions")
         self.samples = s if inside else -s
+        self.prefiltered = False  # True: `samples` holds the padded B-spline coefficients
         self.n = n
         self.spacing = tuple(float(v) for v in spacing)
         self.origin = tuple(float(v) for v in origin)
@@ -353,6 +354,7 @@ class ModelArrays:
                 d.sphere_radius = float(m.radius)
             elif isinstance(m, DiscreteSignedDistanceField):
                 d.surface = abi.EB_SURFACE_SDF
+                d.sdf_prefiltered = 1 if getattr(m, "prefiltered", False) else 0
                 self.sdf = np.ascontiguousarray(m.samples, dtype=np.float64)
                 for k in range(3):
                     d.sdf_n[k] = m.n[k]

@@ -1,0 +1,204 @@
+# EndoBeamsAMD.jl — Julia binding of libendobeams_hip.so (include/endobeams_hip.h).
+#
+# Drop-in for the per-Newton-iteration path of EndoBeams.jl: `run_simulation_gpu!` has the
+# signature and control flow of run_simulation! (src/run_simulation.jl:1-119) but hands
+# solve_step_dynamics!, update_converged! and update_not_converged! to the device. The node and
+# beam structures are read from the caller's BeamsConfiguration exactly as EndoBeams stores them
+# (StructArray fields, SVector/SMatrix layouts, 1-based indices), so no reshaping is needed.
+module EndoBeamsAMD
+
+using LinearAlgebra
+
+const LIB = joinpath(@__DIR__, "..", "lib", "libendobeams_hip.so")
+
+const EB_ABI_VERSION = Int32(1)
+const EB_SURFACE_NONE, EB_SURFACE_SPHERE, EB_SURFACE_SDF = Int32(0), Int32(1), Int32(2)
+
+# field ids (enum eb_field)
+const EB_F_U, EB_F_UDOT, EB_F_UDDOT, EB_F_W, EB_F_WDOT, EB_F_WDDOT, EB_F_R, EB_F_DR = 0:7
+const EB_F_U_N, EB_F_UDOT_N, EB_F_UDDOT_N, EB_F_W_N, EB_F_WDOT_N, EB_F_WDDOT_N, EB_F_R_N, EB_F_DR_N = 8:15
+const EB_F_FEXT, EB_F_FEXT_N = 16, 20
+
+# mirrors eb_model_desc field by field (isbits, C layout)
+struct ModelDesc
+    abi_version::Int32
+    precision::Int32
+    n_nodes::Int64
+    n_beams::Int64
+    X0::Ptr{Float64}
+    conn::Ptr{Int64}
+    l0::Ptr{Float64}
+    Re0::Ptr{Float64}
+    props::Ptr{Float64}
+    n_fixed::Int64
+    fixed_dofs::Ptr{Int64}
+    n_gauss::Int32
+    pad0::Int32
+    gauss_w::Ptr{Float64}
+    gauss_z::Ptr{Float64}
+    surface::Int32
+    sdf_prefiltered::Int32
+    kn::Float64
+    mu::Float64
+    eps_t::Float64
+    eta_n::Float64
+    n_contact_beams::Int64
+    contact_beams::Ptr{Int64}
+    sphere_center::NTuple{3,Float64}
+    sphere_radius::Float64
+    sdf_n::NTuple{3,Int64}
+    sdf_d::NTuple{3,Float64}
+    sdf_x0::NTuple{3,Float64}
+    sdf_samples::Ptr{Float64}
+end
+
+struct NewtonParams
+    alpha::Float64
+    beta::Float64
+    gamma::Float64
+    tol_res::Float64
+    tol_disp::Float64
+    max_iterations::Int32
+    krylov_maxit::Int32
+    krylov_tol::Float64
+end
+
+mutable struct Device
+    ctx::Ptr{Cvoid}
+    keep::Vector{Any}      # arrays the descriptor points at, kept alive for eb_create
+end
+
+check(st, ctx, what) = st == 0 || error("$what failed ($st): " * unsafe_string(ccall((:eb_last_error, LIB), Cstring, (Ptr{Cvoid},), ctx)))
+
+flatten(v::AbstractVector) = collect(reinterpret(Float64, collect(v)))   # SVector/SMatrix → Float64
+
+"""props record of the ABI (EB_NPROPS = 14) from a BeamMaterialProperties."""
+function beam_record(p)
+    K̄ū, K̄Θ̅, K̄Θ̅Θ̅ = p.K̄ⁱⁿᵗ
+    J = diag(p.Jᵨ)
+    return [p.radius, p.E, K̄ū, diag(K̄Θ̅)..., diag(K̄Θ̅Θ̅)..., J..., p.Aᵨ, p.damping]
+end
+
+"""Creates the device context from an EndoBeams BeamsConfiguration (+ optional RigidInteraction)."""
+function Device(conf, params, inter=nothing; precision::Integer=0)
+    nodes, beams = conf.nodes, conf.beams
+    X0 = flatten(nodes.X₀)
+    conn = Int64[]; for b in beams; push!(conn, b.node1, b.node2); end
+    l0 = collect(Float64, beams.l₀)
+    Re0 = flatten(beams.Rₑ⁰)
+    props = reduce(vcat, [beam_record(b.properties) for b in beams])
+    fixed = collect(Int64, conf.bcs.fixed_dofs)
+    gw, gz = collect(params.ωᴳ_beams), collect(params.zᴳ_beams)
+    keep = Any[X0, conn, l0, Re0, props, fixed, gw, gz]
+    surface, kn, mu, epst, etan = EB_SURFACE_NONE, 0.0, 0.0, 0.0, 0.0
+    cb = Int64[]; c = (0.0, 0.0, 0.0); R = 0.0
+    sdfn, sdfd, sdfx0, samples = (0, 0, 0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), Float64[]
+    if inter !== nothing
+        pr = inter.properties
+        kn, mu, epst, etan = pr.kₙ, pr.μ, pr.εᵗ, pr.ηₙ
+        cb = collect(Int64, inter.slave.contact_beams)
+        m = inter.master
+        if hasfield(typeof(m), :center)
+            surface, c, R = EB_SURFACE_SPHERE, Tuple(m.center), m.radius
+        else
+            # DiscreteSignedDistanceField: hand over the interpolant's prefiltered, padded
+            # coefficients (def_interactions.jl:77) with the grid of read_vtk_sdf
+            surface = EB_SURFACE_SDF
+            samples = collect(Float64, vec(parent(m.sitp.itp.coefs)))
+            sdfn = size(m.sitp.itp) .|> Int64
+            sdfd = (m.dx, m.dy, m.dz)
+            sdfx0 = (m.dom[1], m.dom[3], m.dom[5])
+        end
+    end
+    push!(keep, cb, samples)
+    d = ModelDesc(EB_ABI_VERSION, Int32(precision), length(nodes), length(beams), pointer(X0), pointer(conn),
+                  pointer(l0), pointer(Re0), pointer(props), length(fixed), pointer(fixed), Int32(length(gw)),
+                  Int32(0), pointer(gw), pointer(gz), surface, Int32(surface == EB_SURFACE_SDF), kn, mu, epst, etan, length(cb),
+                  pointer(cb), c, R, sdfn, sdfd, sdfx0, pointer(samples))
+    ref = Ref{Ptr{Cvoid}}(C_NULL)
+    st = GC.@preserve keep ccall((:eb_create, LIB), Cint, (Ref{ModelDesc}, Ref{Ptr{Cvoid}}), d, ref)
+    st == 0 || error("eb_create failed ($st)")
+    dev = Device(ref[], keep)
+    finalizer(x -> ccall((:eb_destroy, LIB), Cint, (Ptr{Cvoid},), x.ctx), dev)
+    # NodesBeams state: current and step-n copies, ΔR = I
+    for (f, v) in ((EB_F_U, nodes.u), (EB_F_UDOT, nodes.u̇), (EB_F_UDDOT, nodes.ü), (EB_F_W, nodes.w),
+                   (EB_F_WDOT, nodes.ẇ), (EB_F_WDDOT, nodes.ẅ), (EB_F_R, nodes.R), (EB_F_DR, nodes.ΔR))
+        set_field!(dev, f, flatten(v)); set_field!(dev, f + 8, flatten(v))
+    end
+    return dev
+end
+
+function set_field!(dev::Device, f::Integer, v::Vector{Float64})
+    st = GC.@preserve v ccall((:eb_set_field, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Int64), dev.ctx, f, v, length(v))
+    check(st, dev.ctx, "eb_set_field")
+end
+
+function get_field(dev::Device, f::Integer, n::Integer)
+    v = Vector{Float64}(undef, n)
+    st = GC.@preserve v ccall((:eb_get_field, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}, Int64), dev.ctx, f, v, n)
+    check(st, dev.ctx, "eb_get_field")
+    return v
+end
+
+newton(p) = NewtonParams(p.α, p.β, p.γ, p.tolerance_residual, p.tolerance_displacement, Int32(p.max_iterations), Int32(200), 1e-13)
+
+"""solve_step_dynamics!(conf, state, params, inter, tⁿ⁺¹, Δt, solver) (src/solve_step_dynamics.jl:1-14)."""
+function solve_step_dynamics!(dev::Device, conf, params, tⁿ⁺¹, Δt)
+    if conf.loads !== nothing && conf.loads.concentrated_force !== nothing
+        dofs = collect(Int64, conf.loads.concentrated_force.loaded_dofs)
+        vals = [conf.loads.concentrated_force.force_function(tⁿ⁺¹, i) for i in dofs]
+        st = GC.@preserve dofs vals ccall((:eb_update_loads, LIB), Cint, (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Float64}),
+                                          dev.ctx, length(dofs), dofs, vals)
+        check(st, dev.ctx, "eb_update_loads")
+    end
+    bcs = conf.bcs
+    if !isempty(bcs.displaced_indices)
+        for i in bcs.displaced_indices
+            bcs.imposed_displacements[i] = bcs.imposed_displacements_function(tⁿ⁺¹)
+        end
+        v = bcs.imposed_displacements
+        st = GC.@preserve v ccall((:eb_update_boundary_conditions, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64),
+                                  dev.ctx, v, length(v))
+        check(st, dev.ctx, "eb_update_boundary_conditions")
+    end
+    k = Ref{Cint}(0)
+    st = ccall((:eb_solve_step_dynamics, LIB), Cint, (Ptr{Cvoid}, Ref{NewtonParams}, Float64, Ref{Cint}),
+               dev.ctx, newton(params), Δt, k)
+    check(st, dev.ctx, "eb_solve_step_dynamics")
+    return Int(k[])
+end
+
+update_converged!(dev::Device) = check(ccall((:eb_update_converged, LIB), Cint, (Ptr{Cvoid},), dev.ctx), dev.ctx, "eb_update_converged")
+update_not_converged!(dev::Device) = check(ccall((:eb_update_not_converged, LIB), Cint, (Ptr{Cvoid},), dev.ctx), dev.ctx, "eb_update_not_converged")
+
+"""run_simulation!(conf, params, inter) with the hot path on the GPU (src/run_simulation.jl:1-119)."""
+function run_simulation_gpu!(conf, params, inter=nothing)
+    dev = Device(conf, params, inter)
+    check(ccall((:eb_initialize_state_simulation, LIB), Cint, (Ptr{Cvoid},), dev.ctx), dev.ctx, "initialize")
+    tⁿ, Δt, successive = 0.0, params.initial_timestep, 0
+    while params.simulation_end_time - tⁿ > 1.0e-10
+        tⁿ⁺¹ = tⁿ + Δt
+        n_it = solve_step_dynamics!(dev, conf, params, tⁿ⁺¹, Δt)
+        if n_it > params.max_iterations
+            Δt /= 2
+            Δt < params.min_timestep && error("Simulation aborted.")
+            update_not_converged!(dev)
+            successive = 0
+            continue
+        end
+        successive += 1
+        if successive ≥ params.accelerate_after_success_it && n_it <= 8
+            Δt = min(Δt * 1.5, params.max_timestep)
+            successive = 0
+        end
+        update_converged!(dev)
+        tⁿ = tⁿ⁺¹
+    end
+    # copy the converged node state back into the caller's StructArray
+    N = length(conf.nodes)
+    u = reinterpret(eltype(conf.nodes.uⁿ), get_field(dev, EB_F_U_N, 3N))
+    conf.nodes.uⁿ .= u
+    return dev
+end
+
+end # module

@@ -67,7 +67,9 @@ typedef struct eb_model_desc {
     const double* gauss_z;        /* params.zᴳ_beams */
     /* contact (RigidInteraction); surface = EB_SURFACE_NONE disables it */
     int32_t surface;
-    int32_t pad1;
+    int32_t sdf_prefiltered;      /* 1: sdf_samples already holds the padded quadratic B-spline
+                                     coefficients, (n[0]+2)(n[1]+2)(n[2]+2) values x-fastest, as
+                                     Interpolations.jl keeps them in parent(sitp.itp.coefs) */
     double kn, mu, eps_t, eta_n;  /* InteractionProperties kₙ, μ, εᵗ, ηₙ (kₜ, ηₜ, u̇ₛ are unused by the reference) */
     int64_t n_contact_beams;      /* BeamElementSurface.contact_beams (1-based) */
     const int64_t* contact_beams;

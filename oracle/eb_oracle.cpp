@@ -312,6 +312,8 @@ int ebo_create(const eb_model_desc* d, void** out) {
         } else if (d->surface == EB_SURFACE_SDF) {
             if (!d->sdf_samples || d->sdf_n[0] < 1 || d->sdf_n[1] < 1 || d->sdf_n[2] < 1) { delete o; return EB_ERR_ARG; }
             sdf_build(o->surf.sdf, d->sdf_n, d->sdf_d, d->sdf_x0, d->sdf_samples);
+            if (d->sdf_prefiltered)
+                o->surf.sdf.coef.assign(d->sdf_samples, d->sdf_samples + o->surf.sdf.coef.size());
         } else { delete o; return EB_ERR_ARG; }
     }
     if (build_pattern(o) != EB_OK) { delete o; return EB_ERR_ARG; }
