@@ -81,7 +81,9 @@ struct eb_ctx {
     void* sdf_coef = nullptr;
     // chains
     int32_t *ch_ptr = nullptr, *ch_node = nullptr, *ch_diag = nullptr, *ch_lo = nullptr, *ch_up = nullptr;
-    double *ch_Sinv = nullptr, *ch_G = nullptr, *ch_Lm = nullptr;
+    uint64_t *ch_mD = nullptr, *ch_mL = nullptr, *ch_mU = nullptr;
+    int32_t *ch_hbase = nullptr, *ch_rej = nullptr;
+    double *ch_D = nullptr, *ch_L = nullptr, *ch_U = nullptr, *ch_hist = nullptr, *ch_bw = nullptr, *ch_yw = nullptr;
     int precond_mode = 0;
     double* tmp = nullptr;  // nnz scratch for layout conversion
     int64_t counters[4] = {0, 0, 0, 0};
@@ -188,7 +190,8 @@ template <typename P> BeamStage<P> beam_stage(eb_ctx* c) {
     return BeamStage<P>{(P*)c->sK, (P*)c->sC, (P*)c->sM, (P*)c->sTint, (P*)c->sTk, c->senergy};
 }
 ChainDev chain_dev(eb_ctx* c) {
-    return ChainDev{c->ch_ptr, c->ch_node, c->ch_diag, c->ch_lo, c->ch_up, c->ch_Sinv, c->ch_G, c->ch_Lm,
+    return ChainDev{c->ch_ptr, c->ch_node, c->ch_diag, c->ch_lo, c->ch_up, c->ch_mD, c->ch_mL, c->ch_mU,
+                    c->ch_hbase, c->ch_D, c->ch_L, c->ch_U, c->ch_hist, c->ch_bw, c->ch_yw, c->ch_rej,
                     (int64_t)c->chains.ptr.size() - 1};
 }
 HHTc hht_consts(double dt, double b, double g) {
@@ -296,15 +299,15 @@ int bc_lift_launch(eb_ctx* ctx) {
 int precond_factor(eb_ctx* ctx) {
     int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
     prof_begin(ctx, EB_KERNEL_PRECOND);
-    hipLaunchKernelGGL(chain_factor, dim3((unsigned)nch), dim3(64), 0, ctx->stream, chain_dev(ctx), ctx->Ktan,
-                       ctx->is_fixed, ctx->kstatus);
+    hipLaunchKernelGGL(chain_factor, dim3((unsigned)nch), dim3(CHAIN_THREADS), 0, ctx->stream, chain_dev(ctx), ctx->Ktan,
+                       ctx->kstatus);
     prof_end(ctx, EB_KERNEL_PRECOND);
     return check_launch(ctx, "chain_factor");
 }
 
 int precond_apply(eb_ctx* ctx, const double* b, double* x) {
     int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
-    hipLaunchKernelGGL(chain_solve, dim3((unsigned)nch), dim3(64), 0, ctx->stream, chain_dev(ctx), b, x,
+    hipLaunchKernelGGL(chain_solve, dim3((unsigned)nch), dim3(CHAIN_THREADS), 0, ctx->stream, chain_dev(ctx), b, x,
                        &ctx->kctl->done);
     return check_launch(ctx, "chain_solve");
 }
@@ -590,9 +593,18 @@ int eb_create(const eb_model_desc* d, eb_ctx** out) {
         CK(upload_vec(ctx, ctx->ch_diag, c.diag_blk));
         CK(upload_vec(ctx, ctx->ch_lo, c.lo_blk));
         CK(upload_vec(ctx, ctx->ch_up, c.up_blk));
-        CK(dalloc(ctx, ctx->ch_Sinv, 36 * N));
-        CK(dalloc(ctx, ctx->ch_G, 36 * N));
-        CK(dalloc(ctx, ctx->ch_Lm, 36 * N));
+        CK(upload_vec(ctx, ctx->ch_mD, c.mD));
+        CK(upload_vec(ctx, ctx->ch_mL, c.mL));
+        CK(upload_vec(ctx, ctx->ch_mU, c.mU));
+        CK(upload_vec(ctx, ctx->ch_hbase, c.hbase));
+        CK(dalloc(ctx, ctx->ch_D, 36 * N));
+        CK(dalloc(ctx, ctx->ch_L, 36 * N));
+        CK(dalloc(ctx, ctx->ch_U, 36 * N));
+        CK(dalloc(ctx, ctx->ch_hist, 72 * c.n_hist));
+        CK(dalloc(ctx, ctx->ch_bw, 6 * N));
+        CK(dalloc(ctx, ctx->ch_yw, 6 * N));
+        CK(dalloc(ctx, ctx->ch_rej, N));
+        if (hipMemsetAsync(ctx->ch_rej, 0, sizeof(int32_t) * N, ctx->stream) != hipSuccess) return bail(EB_ERR_DEVICE);
     }
 #undef CK
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return bail(EB_ERR_DEVICE);

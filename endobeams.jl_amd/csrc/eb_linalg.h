@@ -247,146 +247,360 @@ __global__ void __launch_bounds__(RED_BLOCK) spmv_masked(int64_t ndofs, const in
 
 // ---- chain block-tridiagonal preconditioner ------------------------------------------------
 // The node graph is covered by chains (paths); along a chain the masked tangent restricted to
-// the chain is block-tridiagonal with 6x6 blocks and is factored exactly (block LU without
-// inter-block pivoting, Gauss-Jordan with partial pivoting inside each 6x6 Schur complement).
-// Couplings between chains are left to the Krylov iteration. One 64-lane workgroup per chain.
+// the chain is block-tridiagonal with 6x6 blocks (row i: L_i x_{i-1} + D_i x_i + U_i x_{i+1}) and
+// is factored exactly by block cyclic reduction. Couplings between chains are left to the Krylov
+// iteration.
+//
+// Cyclic reduction, one workgroup per chain. Level l (h = 2^l) eliminates the positions
+// j = h, 3h, 5h, ... from the positions still active (the multiples of h):
+//   phase A (eliminated j):  D_j <- D_j^-1,  GL_j = D_j^-1 L_j,  GU_j = D_j^-1 U_j
+//   phase B (kept i = 0, 2h, 4h, ...; j1 = i-h, j2 = i+h):
+//     D_i <- D_i - L_i GU_j1 - U_i GL_j2,   L_i <- -L_i GL_j1,   U_i <- -U_i GU_j2
+// until position 0 is left alone and inverted. The L_i, U_i a kept position holds at each level
+// are saved (hist) for the right-hand-side reduction of the solve. Depth: ceil(log2 n) levels
+// instead of n sequential 6x6 steps, and every level is parallel over the chain.
+
+constexpr int CHAIN_THREADS = 256;
 
 struct ChainDev {
     const int32_t* ptr;   // n_chains+1
     const int32_t* node;  // chain-ordered nodes
-    const int32_t *diag_blk, *lo_blk, *up_blk;
-    double *Sinv, *G, *Lm;  // 36 doubles per chain position
+    const int32_t *diag_blk, *lo_blk, *up_blk;  // per chain position; -1 when absent
+    const uint64_t *mD, *mL, *mU;  // bit k: entry k (row-major) touches a fixed dof row or column
+    const int32_t* hbase;          // per chain position: first history slot
+    double *Dm, *Lm, *Um;          // 36 per position: working blocks, then D^-1, GL, GU
+    double* hist;                  // 72 per slot: L then U of a kept position at one level
+    double *bw, *yw;               // 6 per position: solve work vectors
+    int32_t* rej;                  // per position: unpivoted inverse rejected (zero between calls)
     int64_t n_chains;
 };
 
-EBD double masked_entry(const double* A, int64_t blk, int32_t nr, int32_t nc, int a, int c, const uint8_t* fixed) {
-    bool fx = fixed[6 * (int64_t)nr + a] || fixed[6 * (int64_t)nc + c];
-    if (fx) return (nr == nc && a == c) ? 1.0 : 0.0;
-    return A[36 * blk + 6 * a + c];
+EBD int chain_levels(int32_t n) {
+    int nl = 0;
+    while ((int64_t(1) << nl) < n) ++nl;
+    return nl;
 }
 
-__global__ void __launch_bounds__(64) chain_factor(ChainDev cd, const double* __restrict__ A,
-                                                   const uint8_t* __restrict__ fixed, int* status) {
-    const int64_t ch = blockIdx.x;
-    const int l = threadIdx.x;
-    const int r = l / 6, c = l % 6;
-    const bool act = l < 36;
-    __shared__ double W[6][13];
-    __shared__ double Gs[36], Ls[36];
-    __shared__ int piv;
-    const int32_t s0 = cd.ptr[ch], s1 = cd.ptr[ch + 1];
-    for (int32_t i = s0; i < s1; ++i) {
-        const int32_t n = cd.node[i];
-        double d = 0;
-        if (act) d = masked_entry(A, cd.diag_blk[i], n, n, r, c, fixed);
-        if (i > s0) {
-            if (act) {
-                double lv = masked_entry(A, cd.lo_blk[i], n, cd.node[i - 1], r, c, fixed);
-                Ls[l] = lv;
-                cd.Lm[(int64_t)36 * i + l] = lv;
-            }
-            __syncthreads();
-            if (act) {
-                double acc = 0;
+// In-place inverse of a 6x6 row-major block: Gauss-Jordan on [S | I] with partial pivoting.
+// Rows are exchanged by conditional swaps so that every index is static and the 6x12 working
+// matrix stays in registers of one lane.
+EBD bool inv6(double* S) {
+    double W[6][12];
 #pragma unroll
-                for (int k = 0; k < 6; ++k) acc += Ls[6 * r + k] * Gs[6 * k + c];
-                d -= acc;
+    for (int r = 0; r < 6; ++r) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            W[r][c] = S[6 * r + c];
+            W[r][6 + c] = (r == c) ? 1.0 : 0.0;
+        }
+    }
+    bool ok = true;
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+#pragma unroll
+        for (int q = p + 1; q < 6; ++q) {
+            const bool sw = fabs(W[q][p]) > fabs(W[p][p]);
+#pragma unroll
+            for (int c = p; c < 12; ++c) {
+                const double a = W[p][c], b = W[q][c];
+                W[p][c] = sw ? b : a;
+                W[q][c] = sw ? a : b;
             }
         }
-        if (act) { W[r][c] = d; W[r][6 + c] = (r == c) ? 1.0 : 0.0; }
-        __syncthreads();
-        // Gauss-Jordan with partial pivoting on [S | I]
-        for (int p = 0; p < 6; ++p) {
-            if (l == 0) {
-                int best = p;
-                double bv = fabs(W[p][p]);
-                for (int q = p + 1; q < 6; ++q)
-                    if (fabs(W[q][p]) > bv) { bv = fabs(W[q][p]); best = q; }
-                piv = best;
-                if (bv == 0.0) *status = 1;
+        const double piv = W[p][p];
+        ok = ok && fabs(piv) > 0.0 && fabs(piv) < INFINITY;
+        const double ip = 1.0 / piv;
+#pragma unroll
+        for (int c = p + 1; c < 12; ++c) W[p][c] *= ip;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            if (r == p) continue;
+            const double f = W[r][p];
+#pragma unroll
+            for (int c = p + 1; c < 12; ++c) W[r][c] = fma(-f, W[p][c], W[r][c]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) S[6 * r + c] = W[r][6 + c];
+    }
+    return ok;
+}
+
+// In-place inverse of a 6x6 row-major block by Gauss-Jordan without row exchanges. Returns false
+// when a pivot has cancelled to below 1e-12 of the diagonal entry it started from (or is not
+// finite); the caller then falls back to inv6. The test is invariant under row and column
+// scaling, which matters here: translational and rotational entries differ by many decades.
+EBD bool inv6_nopiv(double* a) {
+    double d0[6];
+#pragma unroll
+    for (int p = 0; p < 6; ++p) d0[p] = fabs(a[7 * p]);
+    bool ok = true;
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+        const double piv = a[7 * p];
+        ok = ok && fabs(piv) > 1e-12 * d0[p] && fabs(piv) > 0.0 && fabs(piv) < INFINITY;
+        const double ip = 1.0 / piv;
+        a[7 * p] = 1.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a[6 * p + c] *= ip;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            if (r == p) continue;
+            const double f = a[6 * r + p];
+            a[6 * r + p] = 0.0;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) a[6 * r + c] = fma(-f, a[6 * p + c], a[6 * r + c]);
+        }
+    }
+    return ok;
+}
+
+// B <- A B in place, column by column (6x6 row-major, both in registers)
+EBD void mm6_inplace(const double* A, double* B) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        double t[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            double acc = 0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc = fma(A[6 * r + k], B[6 * k + c], acc);
+            t[r] = acc;
+        }
+#pragma unroll
+        for (int r = 0; r < 6; ++r) B[6 * r + c] = t[r];
+    }
+}
+
+// phase A for one eliminated position g: D <- D^-1, L <- D^-1 L, U <- D^-1 U. Without PIVOT,
+// returns false when the unpivoted inverse was rejected (nothing is written then).
+template <bool PIVOT> EBD bool chain_eliminate(const ChainDev& cd, int64_t g, bool& ok) {
+    double D[36];
+#pragma unroll
+    for (int k = 0; k < 36; ++k) D[k] = cd.Dm[36 * g + k];
+    if (PIVOT) {
+        ok = inv6(D) && ok;
+    } else if (!inv6_nopiv(D)) {
+        return false;
+    }
+    double B[36];
+#pragma unroll
+    for (int k = 0; k < 36; ++k) B[k] = cd.Lm[36 * g + k];
+    mm6_inplace(D, B);
+#pragma unroll
+    for (int k = 0; k < 36; ++k) cd.Lm[36 * g + k] = B[k];
+#pragma unroll
+    for (int k = 0; k < 36; ++k) B[k] = cd.Um[36 * g + k];
+    mm6_inplace(D, B);
+#pragma unroll
+    for (int k = 0; k < 36; ++k) { cd.Um[36 * g + k] = B[k]; cd.Dm[36 * g + k] = D[k]; }
+    return true;
+}
+
+__global__ void __launch_bounds__(CHAIN_THREADS) chain_factor(ChainDev cd, const double* __restrict__ A, int* status) {
+    const int64_t s0 = cd.ptr[blockIdx.x];
+    const int32_t n = cd.ptr[blockIdx.x + 1] - (int32_t)s0;
+    const int tid = threadIdx.x;
+    __shared__ double P2[CHAIN_THREADS][6];
+    __shared__ int redo;
+    if (tid == 0) redo = 0;
+    // masked blocks of the chain: a fixed row or column becomes a row/column of the identity
+    for (int64_t t = tid; t < 36 * (int64_t)n; t += CHAIN_THREADS) {
+        const int64_t g = s0 + t / 36;
+        const int k = (int)(t % 36);
+        const uint64_t bit = uint64_t(1) << k;
+        const double d = (cd.mD[g] & bit) ? ((k / 6 == k % 6) ? 1.0 : 0.0) : A[36 * (int64_t)cd.diag_blk[g] + k];
+        const int32_t lb = cd.lo_blk[g], ub = cd.up_blk[g];
+        const double lo = (lb >= 0 && !(cd.mL[g] & bit)) ? A[36 * (int64_t)lb + k] : 0.0;
+        const double up = (ub >= 0 && !(cd.mU[g] & bit)) ? A[36 * (int64_t)ub + k] : 0.0;
+        cd.Dm[36 * g + k] = d;
+        cd.Lm[36 * g + k] = lo;
+        cd.Um[36 * g + k] = up;
+    }
+    __syncthreads();
+    bool ok = true;
+    const int nl = chain_levels(n);
+    for (int l = 0; l < nl; ++l) {
+        const int32_t h = 1 << l;
+        // phase A: one lane per eliminated position; the rare rejected unpivoted inverses are
+        // redone with partial pivoting in a second pass
+        const int32_t ne = (n - h + 2 * h - 1) / (2 * h);
+        for (int32_t m = tid; m < ne; m += CHAIN_THREADS) {
+            const int64_t g = s0 + h + 2 * (int64_t)h * m;
+            if (!chain_eliminate<false>(cd, g, ok)) {
+                cd.rej[g] = 1;
+                redo = 1;
             }
-            __syncthreads();
-            const int pv = piv;
-            if (pv != p && l < 12) { double t = W[p][l]; W[p][l] = W[pv][l]; W[pv][l] = t; }
-            __syncthreads();
-            double f = 0, pr_c = 0, pr_c6 = 0;
-            const double pivot = W[p][p];
-            if (act) { f = W[r][p]; pr_c = W[p][c]; pr_c6 = W[p][6 + c]; }
-            __syncthreads();
-            if (act) {
-                if (r == p) {
-                    W[r][c] = pr_c / pivot;
-                    W[r][6 + c] = pr_c6 / pivot;
-                } else {
-                    double fp = f / pivot;
-                    W[r][c] -= fp * pr_c;
-                    W[r][6 + c] -= fp * pr_c6;
+        }
+        __syncthreads();
+        if (redo) {
+            for (int32_t m = tid; m < ne; m += CHAIN_THREADS) {
+                const int64_t g = s0 + h + 2 * (int64_t)h * m;
+                if (cd.rej[g]) {
+                    cd.rej[g] = 0;
+                    chain_eliminate<true>(cd, g, ok);
                 }
             }
             __syncthreads();
-        }
-        double sinv = act ? W[r][6 + c] : 0.0;
-        if (act) cd.Sinv[(int64_t)36 * i + l] = sinv;
-        if (i + 1 < s1) {
-            // G_{i+1} = S_i⁻¹ · A(n_i, n_{i+1})
-            if (act) Ls[l] = masked_entry(A, cd.up_blk[i], n, cd.node[i + 1], r, c, fixed);
+            if (tid == 0) redo = 0;
             __syncthreads();
-            if (act) {
-                double acc = 0;
+        }
+        // phase B: twelve lanes per kept position i: lane (side s, row r) forms row r of
+        //   s = 0: L_i GU_j1 (into D) and -L_i GL_j1 (new L)
+        //   s = 1: U_i GL_j2 (into D) and -U_i GU_j2 (new U)
+        // Each lane reads and writes only its own row of L_i, U_i or D_i; the side-1 part of D
+        // reaches the side-0 lane through LDS.
+        const int32_t nk = (n + 2 * h - 1) / (2 * h);
+        constexpr int PER = (CHAIN_THREADS / 12) * 12;
+        for (int32_t base = 0; base < 12 * nk; base += PER) {
+            const int32_t t = base + tid;
+            const int32_t m = t / 12, side = (t % 12) / 6, r = t % 6;
+            const bool act = tid < PER && m < nk;
+            const int32_t i = 2 * h * m;
+            const int64_t gi = s0 + i;
+            const bool has = act && (side == 0 ? i > 0 : i + h < n);
+            double row[6] = {}, G1[36] = {}, G2[36] = {}, Dr[6] = {};
+            if (has) {
+                const int64_t gj = side == 0 ? gi - h : gi + h;
+                const double* X = (side == 0 ? cd.Lm : cd.Um) + 36 * gi + 6 * r;
+                const double* Gd = (side == 0 ? cd.Um : cd.Lm) + 36 * gj;   // into D
+                const double* Gn = (side == 0 ? cd.Lm : cd.Um) + 36 * gj;   // new L or U
 #pragma unroll
-                for (int k = 0; k < 6; ++k) acc += W[r][6 + k] * Ls[6 * k + c];
-                Gs[l] = acc;
-                cd.G[(int64_t)36 * (i + 1) + l] = acc;
+                for (int k = 0; k < 6; ++k) row[k] = X[k];
+#pragma unroll
+                for (int k = 0; k < 36; ++k) { G1[k] = Gd[k]; G2[k] = Gn[k]; }
+            }
+            if (act && side == 0) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) Dr[k] = cd.Dm[36 * gi + 6 * r + k];
+            }
+            double pd[6], pn[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                double a1 = 0, a2 = 0;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) { a1 = fma(row[k], G1[6 * k + c], a1); a2 = fma(row[k], G2[6 * k + c], a2); }
+                pd[c] = has ? a1 : 0.0;
+                pn[c] = -a2;
+            }
+            if (act && side == 1) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) P2[tid][c] = pd[c];
+            }
+            __syncthreads();
+            if (has) {
+                double* hs = cd.hist + 72 * ((int64_t)cd.hbase[gi] + l) + 36 * side + 6 * r;
+                double* dst = (side == 0 ? cd.Lm : cd.Um) + 36 * gi + 6 * r;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) { hs[k] = row[k]; dst[k] = pn[k]; }
+            }
+            if (act && side == 0) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) cd.Dm[36 * gi + 6 * r + c] = Dr[c] - pd[c] - P2[tid + 6][c];
             }
             __syncthreads();
         }
     }
+    if (tid == 0) {
+        double D[36];
+#pragma unroll
+        for (int k = 0; k < 36; ++k) D[k] = cd.Dm[36 * s0 + k];
+        double C[36];
+#pragma unroll
+        for (int k = 0; k < 36; ++k) C[k] = D[k];
+        if (inv6_nopiv(C)) {
+#pragma unroll
+            for (int k = 0; k < 36; ++k) D[k] = C[k];
+        } else {
+            ok = inv6(D) && ok;
+        }
+#pragma unroll
+        for (int k = 0; k < 36; ++k) cd.Dm[36 * s0 + k] = D[k];
+    }
+    if (!ok) *status = 1;
 }
 
-// z = M⁻¹ b along each chain: forward y_i = b_i - L_i z_{i-1}, z_i = S_i⁻¹ y_i;
-// backward x_i = z_i - G_{i+1} x_{i+1}. x and b may not alias.
-__global__ void __launch_bounds__(64) chain_solve(ChainDev cd, const double* __restrict__ b, double* __restrict__ x,
-                                                  const int* done) {
+// x = M^-1 b along each chain, with the factors of chain_factor: the right-hand side is reduced
+// level by level (y_j = D_j^-1 b_j for eliminated j; b_i -= L_i y_j1 + U_i y_j2 for kept i with the
+// saved blocks), then solved back from position 0 (x_j = y_j - GL_j x_{j-h} - GU_j x_{j+h}).
+// Six lanes per position, one per row.
+__global__ void __launch_bounds__(CHAIN_THREADS) chain_solve(ChainDev cd, const double* __restrict__ b, double* __restrict__ x,
+                                                             const int* done) {
     if (*done) return;
-    const int64_t ch = blockIdx.x;
-    const int l = threadIdx.x;
-    __shared__ double zv[6], yv[6];
-    const int32_t s0 = cd.ptr[ch], s1 = cd.ptr[ch + 1];
-    for (int32_t i = s0; i < s1; ++i) {
-        const int64_t base = 6 * (int64_t)cd.node[i];
-        if (l < 6) {
-            double y = b[base + l];
-            if (i > s0) {
-                const double* L = cd.Lm + (int64_t)36 * i + 6 * l;
+    const int64_t s0 = cd.ptr[blockIdx.x];
+    const int32_t n = cd.ptr[blockIdx.x + 1] - (int32_t)s0;
+    const int tid = threadIdx.x;
+    for (int64_t t = tid; t < 6 * (int64_t)n; t += CHAIN_THREADS) {
+        const int64_t g = s0 + t / 6;
+        cd.bw[6 * g + t % 6] = b[6 * (int64_t)cd.node[g] + t % 6];
+    }
+    __syncthreads();
+    const int nl = chain_levels(n);
+    for (int l = 0; l < nl; ++l) {
+        const int32_t h = 1 << l;
+        const int32_t ne = (n - h + 2 * h - 1) / (2 * h);
+        for (int32_t t = tid; t < 6 * ne; t += CHAIN_THREADS) {
+            const int64_t g = s0 + h + 2 * (int64_t)h * (t / 6);
+            const int r = t % 6;
+            const double* Dr = cd.Dm + 36 * g + 6 * r;
+            double acc = 0;
 #pragma unroll
-                for (int k = 0; k < 6; ++k) y -= L[k] * zv[k];
-            }
-            yv[l] = y;
+            for (int c = 0; c < 6; ++c) acc = fma(Dr[c], cd.bw[6 * g + c], acc);
+            cd.yw[6 * g + r] = acc;
         }
         __syncthreads();
-        if (l < 6) {
-            const double* S = cd.Sinv + (int64_t)36 * i + 6 * l;
-            double z = 0;
+        const int32_t nk = (n + 2 * h - 1) / (2 * h);
+        for (int32_t t = tid; t < 6 * nk; t += CHAIN_THREADS) {
+            const int32_t i = 2 * h * (t / 6);
+            const int r = t % 6;
+            const int64_t gi = s0 + i;
+            const double* hs = cd.hist + 72 * ((int64_t)cd.hbase[gi] + l) + 6 * r;
+            double v = cd.bw[6 * gi + r];
+            if (i > 0) {
 #pragma unroll
-            for (int k = 0; k < 6; ++k) z += S[k] * yv[k];
-            x[base + l] = z;
-            zv[l] = z;
+                for (int c = 0; c < 6; ++c) v = fma(-hs[c], cd.yw[6 * (gi - h) + c], v);
+            }
+            if (i + h < n) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) v = fma(-hs[36 + c], cd.yw[6 * (gi + h) + c], v);
+            }
+            cd.bw[6 * gi + r] = v;
         }
         __syncthreads();
     }
-    // backward: zv holds x_{i+1}
-    for (int32_t i = s1 - 2; i >= s0; --i) {
-        const int64_t base = 6 * (int64_t)cd.node[i];
-        double xi = 0;
-        if (l < 6) {
-            const double* Gm = cd.G + (int64_t)36 * (i + 1) + 6 * l;
-            xi = x[base + l];
+    if (tid < 6) {
+        const double* Dr = cd.Dm + 36 * s0 + 6 * tid;
+        double acc = 0;
 #pragma unroll
-            for (int k = 0; k < 6; ++k) xi -= Gm[k] * zv[k];
+        for (int c = 0; c < 6; ++c) acc = fma(Dr[c], cd.bw[6 * s0 + c], acc);
+        cd.yw[6 * s0 + tid] = acc;
+    }
+    __syncthreads();
+    for (int l = nl - 1; l >= 0; --l) {
+        const int32_t h = 1 << l;
+        const int32_t ne = (n - h + 2 * h - 1) / (2 * h);
+        for (int32_t t = tid; t < 6 * ne; t += CHAIN_THREADS) {
+            const int32_t j = h + 2 * h * (t / 6);
+            const int r = t % 6;
+            const int64_t g = s0 + j;
+            const double* GL = cd.Lm + 36 * g + 6 * r;
+            const double* GU = cd.Um + 36 * g + 6 * r;
+            double v = cd.yw[6 * g + r];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) v = fma(-GL[c], cd.yw[6 * (g - h) + c], v);
+            if (j + h < n) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) v = fma(-GU[c], cd.yw[6 * (g + h) + c], v);
+            }
+            cd.yw[6 * g + r] = v;
         }
         __syncthreads();
-        if (l < 6) { x[base + l] = xi; zv[l] = xi; }
-        __syncthreads();
+    }
+    for (int64_t t = tid; t < 6 * (int64_t)n; t += CHAIN_THREADS) {
+        const int64_t g = s0 + t / 6;
+        x[6 * (int64_t)cd.node[g] + t % 6] = cd.yw[6 * g + t % 6];
     }
 }
 

@@ -176,6 +176,41 @@ void build_chains(Chains& c, const Pattern& p, const double* X0, int mode) {
             if (i + 1 < e) c.up_blk[i] = blk(c.node[i], c.node[i + 1]);
         }
     }
+    // entry k = 6a+b of block (n, m) is replaced when dof a of n or dof b of m is fixed
+    auto mask = [&](int32_t n, int32_t m) -> uint64_t {
+        uint64_t bits = 0;
+        for (int a = 0; a < 6; ++a)
+            for (int b = 0; b < 6; ++b)
+                if (p.is_fixed[6 * (int64_t)n + a] || p.is_fixed[6 * (int64_t)m + b]) bits |= uint64_t(1) << (6 * a + b);
+        return bits;
+    };
+    c.mD.assign(L, 0); c.mL.assign(L, 0); c.mU.assign(L, 0);
+    for (size_t ch = 0; ch + 1 < c.ptr.size(); ++ch) {
+        int64_t s = c.ptr[ch], e = c.ptr[ch + 1];
+        for (int64_t i = s; i < e; ++i) {
+            c.mD[i] = mask(c.node[i], c.node[i]);
+            if (i > s) c.mL[i] = mask(c.node[i], c.node[i - 1]);
+            if (i + 1 < e) c.mU[i] = mask(c.node[i], c.node[i + 1]);
+        }
+    }
+    // cyclic reduction keeps position p (chain-relative) through tz(p) levels (p > 0) and position 0
+    // through all ceil(log2 n) levels; one history slot per level kept
+    c.hbase.assign(L, 0);
+    int64_t slot = 0;
+    for (size_t ch = 0; ch + 1 < c.ptr.size(); ++ch) {
+        int64_t s = c.ptr[ch], e = c.ptr[ch + 1], n = e - s;
+        int nl = 0;
+        while ((int64_t(1) << nl) < n) ++nl;
+        for (int64_t i = s; i < e; ++i) {
+            int64_t pos = i - s;
+            int kept = 0;
+            if (pos == 0) kept = nl;
+            else while (((pos >> kept) & 1) == 0) ++kept;
+            c.hbase[i] = (int32_t)slot;
+            slot += kept;
+        }
+    }
+    c.n_hist = slot;
 }
 
 // Separable solve of  c₀ = c₁, c_{i-1}/8 + 3c_i/4 + c_{i+1}/8 = y_i, c_{n+1} = c_n  along each axis

@@ -40,6 +40,9 @@ struct Chains {
     std::vector<int64_t> ptr;                    // n_chains+1
     std::vector<int32_t> node;                   // chain-ordered nodes
     std::vector<int32_t> diag_blk, lo_blk, up_blk;  // per chain position; -1 when absent
+    std::vector<uint64_t> mD, mL, mU;           // per chain position: entries touching a fixed dof
+    std::vector<int32_t> hbase;                  // per chain position: first cyclic-reduction history slot
+    int64_t n_hist = 0;
     int64_t max_len = 0;
 };
 
