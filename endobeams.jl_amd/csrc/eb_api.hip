@@ -226,7 +226,7 @@ int unalias(eb_ctx* ctx) {
 
 template <typename P> int launch_beams(eb_ctx* ctx) {
     prof_begin(ctx, EB_KERNEL_BEAM);
-    hipLaunchKernelGGL(beam_kernel<P>, dim3(nblk(ctx->E, 64)), dim3(64), 0, ctx->stream, node_state(ctx),
+    hipLaunchKernelGGL(beam_kernel<P>, dim3(nblk(ctx->E, BEAM_TILE)), dim3(4 * BEAM_TILE), 0, ctx->stream, node_state(ctx),
                        beam_consts(ctx), beam_stage<P>(ctx), ctx->gauss);
     prof_end(ctx, EB_KERNEL_BEAM);
     ctx->counters[3]++;

@@ -175,14 +175,46 @@ EBD StaticOut beam_static(int64_t e, const NodeState& ns, const BeamConsts& bc, 
     return so;
 }
 
+// The element's corotational frame alone (the first lines of beam_static), for the parts of the
+// element computed in other wavefronts.
+EBD StaticOut beam_frame(int64_t e, const NodeState& ns, const BeamConsts& bc) {
+    const int64_t E = bc.E;
+    const int32_t a = bc.n1[e], b = bc.n2[e];
+    V3<double> x1 = ldv(ns.X0, a) + ldv(ns.u, a), x2 = ldv(ns.X0, b) + ldv(ns.u, b);
+    double ln = norm(x2 - x1);
+    M3<double> Re0;
+    for (int k = 0; k < 9; ++k) Re0.a[k] = bc.Re0[k * E + e];
+    M3<double> R1 = ldm(ns.R, a), R2 = ldm(ns.R, b);
+    LocalFrame f = local_frame(x1, x2, R1, R2, col(Re0, 1), ln);
+    M3<double> ReT = tr(f.Re);
+    StaticOut so;
+    so.Re = f.Re;
+    so.Tb1 = toangle((ReT * R1) * Re0);
+    so.Tb2 = toangle((ReT * R2) * Re0);
+    so.v1 = f.v1;
+    so.eta = f.eta; so.eta11 = f.eta11; so.eta12 = f.eta12; so.eta21 = f.eta21; so.eta22 = f.eta22; so.ln = ln;
+    return so;
+}
+
 }  // namespace ebd
 
 namespace ebd {
 
+// Parts of the dynamic computation, each run by its own wavefront of a workgroup
+// (PART_MASS: M, Tᵏ, kinetic energy; PART_C13: Cᵏ block columns 1 and 3; PART_C24: columns 2, 4).
+// Every part evaluates the shared Gauss-point kinematics; the compiler keeps only what the part's
+// outputs need, so each wavefront carries a fraction of the accumulators of the whole element.
+enum { PART_STATIC = 0, PART_MASS = 1, PART_C13 = 2, PART_C24 = 3 };
+
+// M blocks after l₀/2 Rₑ(·)Rₑᵀ and before the Tₛ⁻¹ post-multiplication, kept by the mass part
+// for the damping term.
+template <typename P> struct MassBlocks { M3<P> M11, M12, M13, M14, M22, M23, M24, M33, M34, M44; };
+
 // Dynamic part (inertia, gyroscopic and damping forces, M and Cᵏ), compute_beams.jl:161-517.
-template <typename P>
+template <typename P, int PART>
 EBD void beam_dynamic(int64_t e, const NodeState& ns, const BeamConsts& bc, const BeamStage<P>& st,
-                      const GaussPts& gp, const StaticOut& so) {
+                      const GaussPts& gp, const StaticOut& so, MassBlocks<P>& mb) {
+    constexpr bool MASS = PART == PART_MASS, CA = PART == PART_C13, CB = PART == PART_C24;
     const int64_t E = bc.E;
     const int32_t na = bc.n1[e], nb = bc.n2[e];
     const M3<P> I = eye<P>();
@@ -215,6 +247,7 @@ EBD void beam_dynamic(int64_t e, const NodeState& ns, const BeamConsts& bc, cons
     M3<P> C11 = zm<P>(), C12 = zm<P>(), C14 = zm<P>(), C21 = zm<P>(), C22 = zm<P>(), C24 = zm<P>(), C31 = zm<P>(),
           C32 = zm<P>(), C34 = zm<P>(), C41 = zm<P>(), C42 = zm<P>(), C44 = zm<P>();
 
+#pragma unroll 1
     for (int ig = 0; ig < gp.n; ++ig) {
         const P w = P(gp.w[ig]);
         ShapeF sd = shape_fn(bc.l0[e], gp.z[ig]);
@@ -274,6 +307,7 @@ EBD void beam_dynamic(int64_t e, const NodeState& ns, const BeamConsts& bc, cons
         M3<P> SW0Ir = skew(W0) * Ir;
         V3<P> X = Ir * ReTwdd0 + SW0Ir * W0;
         M3<P> AH11 = Ar * tr(H11), AH12 = Ar * tr(H12), AH14 = Ar * tr(H14);
+        if constexpr (MASS) {
         V3<P> Tk1G = w * (AH11 * ReTudd0 + tmul(H21, X));
         Tk1 += Tk1G;
         Tk2 += w * (AH12 * ReTudd0 + tmul(H22, X));
@@ -301,77 +335,126 @@ EBD void beam_dynamic(int64_t e, const NodeState& ns, const BeamConsts& bc, cons
         M33 += M11G + (w * Ar) * ((I - H11) - tr(H11));
         M34 += (-M14G) + (w * Ar) * H14;
         M44 += w * (AH14 * H14 + tH24Ir * H24);
+        M3<P> Irg = (Re * Ir) * ReT;
+        kinetic += (w / P(2)) * (Ar * dot(ud0, ud0) + dot(wd0, Irg * wd0));
+        }
         M3<P> SW0IrmS = SW0Ir - skew(Ir * W0);
         M3<P> AC1 = Ar * (C1_1 + C3_1), AC2 = Ar * (C1_2 + C3_2), AC4 = Ar * (C1_4 + C3_4);
         M3<P> Y1 = Ir * (C2_1 + C4_1) + SW0IrmS * H21;
         M3<P> Y2 = Ir * (C2_2 + C4_2) + SW0IrmS * H22;
         M3<P> Y4 = Ir * (C2_4 + C4_4) + SW0IrmS * H24;
         M3<P> tH11 = tr(H11), tH12 = tr(H12), tH14 = tr(H14), tH21 = tr(H21), tH22 = tr(H22), tH24 = tr(H24);
-        M3<P> C11G = w * (tH11 * AC1 + tH21 * Y1);
-        M3<P> C12G = w * (tH11 * AC2 + tH21 * Y2);
-        M3<P> C14G = w * (tH11 * AC4 + tH21 * Y4);
-        C11 += C11G; C12 += C12G; C14 += C14G;
-        C21 += w * (tH12 * AC1 + tH22 * Y1);
-        C22 += w * (tH12 * AC2 + tH22 * Y2);
-        C24 += w * (tH12 * AC4 + tH22 * Y4);
-        C31 += (-C11G) + w * AC1;
-        C32 += (-C12G) + w * AC2;
-        C34 += (-C14G) + w * AC4;
-        C41 += w * (tH14 * AC1 + tH24 * Y1);
-        C42 += w * (tH14 * AC2 + tH24 * Y2);
-        C44 += w * (tH14 * AC4 + tH24 * Y4);
-        M3<P> Irg = (Re * Ir) * ReT;
-        kinetic += (w / P(2)) * (Ar * dot(ud0, ud0) + dot(wd0, Irg * wd0));
+        if constexpr (CA) {
+            M3<P> C11G = w * (tH11 * AC1 + tH21 * Y1);
+            C11 += C11G;
+            C21 += w * (tH12 * AC1 + tH22 * Y1);
+            C31 += (-C11G) + w * AC1;
+            C41 += w * (tH14 * AC1 + tH24 * Y1);
+        }
+        if constexpr (CB) {
+            M3<P> C12G = w * (tH11 * AC2 + tH21 * Y2);
+            M3<P> C14G = w * (tH11 * AC4 + tH21 * Y4);
+            C12 += C12G; C14 += C14G;
+            C22 += w * (tH12 * AC2 + tH22 * Y2);
+            C24 += w * (tH12 * AC4 + tH22 * Y4);
+            C32 += (-C12G) + w * AC2;
+            C34 += (-C14G) + w * AC4;
+            C42 += w * (tH14 * AC2 + tH24 * Y2);
+            C44 += w * (tH14 * AC4 + tH24 * Y4);
+        }
     }
     const P l02 = l0 / P(2);
     const M3<P> l02Re = l02 * Re;
-    st.energy[3 * e + 1] = double(l02 * kinetic);
-    st_vec<P>(st.Tk, E, e, 0, l02Re * Tk1);
-    st_vec<P>(st.Tk, E, e, 1, l02Re * Tk2);
-    st_vec<P>(st.Tk, E, e, 2, l02Re * Tk3);
-    st_vec<P>(st.Tk, E, e, 3, l02Re * Tk4);
-    M11 = (l02Re * M11) * ReT; M12 = (l02Re * M12) * ReT; M13 = (l02Re * M13) * ReT; M14 = (l02Re * M14) * ReT;
-    M22 = (l02Re * M22) * ReT; M23 = (l02Re * M23) * ReT; M24 = (l02Re * M24) * ReT;
-    M33 = (l02Re * M33) * ReT; M34 = (l02Re * M34) * ReT; M44 = (l02Re * M44) * ReT;
-    C11 = (l02Re * C11) * ReT; C12 = (l02Re * C12) * ReT; C14 = (l02Re * C14) * ReT;
-    C21 = (l02Re * C21) * ReT; C22 = (l02Re * C22) * ReT; C24 = (l02Re * C24) * ReT;
-    C31 = (l02Re * C31) * ReT; C32 = (l02Re * C32) * ReT; C34 = (l02Re * C34) * ReT;
-    C41 = (l02Re * C41) * ReT; C42 = (l02Re * C42) * ReT; C44 = (l02Re * C44) * ReT;
-    M3<P> C13 = -C11, C23 = -C21, C33 = -C31, C43 = -C41;
-    if (damping > P(0)) {
-        C11 += damping * M11; C12 += damping * M12; C13 += damping * M13; C14 += damping * M14;
-        C21 += damping * tr(M12); C22 += damping * M22; C23 += damping * M23; C24 += damping * M24;
-        C31 += damping * tr(M13); C32 += damping * tr(M23); C33 += damping * M33; C34 += damping * M34;
-        C41 += damping * tr(M14); C42 += damping * tr(M24); C43 += damping * tr(M34); C44 += damping * M44;
-    }
-    // exact Tₛ⁻¹(log ΔR)ᵀ post-multiplication of block rows 2 and 4, in place (compute_beams.jl:480-506)
+    // exact Tₛ⁻¹(log ΔR)ᵀ post-multiplication of block rows 2 and 4 (compute_beams.jl:480-506)
     const M3<P> TT1t = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, na)))));
     const M3<P> TT2t = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, nb)))));
-    st_block<P>(st.M, E, e, 0, 0, M11); st_block<P>(st.M, E, e, 0, 1, M12);
-    st_block<P>(st.M, E, e, 0, 2, M13); st_block<P>(st.M, E, e, 0, 3, M14);
-    st_block<P>(st.M, E, e, 1, 0, tr(M12) * TT1t);
-    M3<P> M23t = M23 * TT1t, M24t = M24 * TT1t;
-    st_block<P>(st.M, E, e, 1, 1, M22 * TT1t); st_block<P>(st.M, E, e, 1, 2, M23t); st_block<P>(st.M, E, e, 1, 3, M24t);
-    st_block<P>(st.M, E, e, 2, 0, tr(M13)); st_block<P>(st.M, E, e, 2, 1, tr(M23t));
-    st_block<P>(st.M, E, e, 2, 2, M33); st_block<P>(st.M, E, e, 2, 3, M34);
-    st_block<P>(st.M, E, e, 3, 0, tr(M14) * TT2t); st_block<P>(st.M, E, e, 3, 1, tr(M24t) * TT2t);
-    st_block<P>(st.M, E, e, 3, 2, tr(M34) * TT2t); st_block<P>(st.M, E, e, 3, 3, M44 * TT2t);
-    st_block<P>(st.C, E, e, 0, 0, C11); st_block<P>(st.C, E, e, 0, 1, C12);
-    st_block<P>(st.C, E, e, 0, 2, C13); st_block<P>(st.C, E, e, 0, 3, C14);
-    st_block<P>(st.C, E, e, 1, 0, C21 * TT1t); st_block<P>(st.C, E, e, 1, 1, C22 * TT1t);
-    st_block<P>(st.C, E, e, 1, 2, C23 * TT1t); st_block<P>(st.C, E, e, 1, 3, C24 * TT1t);
-    st_block<P>(st.C, E, e, 2, 0, C31); st_block<P>(st.C, E, e, 2, 1, C32);
-    st_block<P>(st.C, E, e, 2, 2, C33); st_block<P>(st.C, E, e, 2, 3, C34);
-    st_block<P>(st.C, E, e, 3, 0, C41 * TT2t); st_block<P>(st.C, E, e, 3, 1, C42 * TT2t);
-    st_block<P>(st.C, E, e, 3, 2, C43 * TT2t); st_block<P>(st.C, E, e, 3, 3, C44 * TT2t);
+    if constexpr (MASS) {
+        st.energy[3 * e + 1] = double(l02 * kinetic);
+        st_vec<P>(st.Tk, E, e, 0, l02Re * Tk1);
+        st_vec<P>(st.Tk, E, e, 1, l02Re * Tk2);
+        st_vec<P>(st.Tk, E, e, 2, l02Re * Tk3);
+        st_vec<P>(st.Tk, E, e, 3, l02Re * Tk4);
+        M11 = (l02Re * M11) * ReT; M12 = (l02Re * M12) * ReT; M13 = (l02Re * M13) * ReT; M14 = (l02Re * M14) * ReT;
+        M22 = (l02Re * M22) * ReT; M23 = (l02Re * M23) * ReT; M24 = (l02Re * M24) * ReT;
+        M33 = (l02Re * M33) * ReT; M34 = (l02Re * M34) * ReT; M44 = (l02Re * M44) * ReT;
+        mb = MassBlocks<P>{M11, M12, M13, M14, M22, M23, M24, M33, M34, M44};
+        st_block<P>(st.M, E, e, 0, 0, M11); st_block<P>(st.M, E, e, 0, 1, M12);
+        st_block<P>(st.M, E, e, 0, 2, M13); st_block<P>(st.M, E, e, 0, 3, M14);
+        st_block<P>(st.M, E, e, 1, 0, tr(M12) * TT1t);
+        M3<P> M23t = M23 * TT1t, M24t = M24 * TT1t;
+        st_block<P>(st.M, E, e, 1, 1, M22 * TT1t); st_block<P>(st.M, E, e, 1, 2, M23t); st_block<P>(st.M, E, e, 1, 3, M24t);
+        st_block<P>(st.M, E, e, 2, 0, tr(M13)); st_block<P>(st.M, E, e, 2, 1, tr(M23t));
+        st_block<P>(st.M, E, e, 2, 2, M33); st_block<P>(st.M, E, e, 2, 3, M34);
+        st_block<P>(st.M, E, e, 3, 0, tr(M14) * TT2t); st_block<P>(st.M, E, e, 3, 1, tr(M24t) * TT2t);
+        st_block<P>(st.M, E, e, 3, 2, tr(M34) * TT2t); st_block<P>(st.M, E, e, 3, 3, M44 * TT2t);
+    }
+    if constexpr (CA) {
+        C11 = (l02Re * C11) * ReT; C21 = (l02Re * C21) * ReT; C31 = (l02Re * C31) * ReT; C41 = (l02Re * C41) * ReT;
+        st_block<P>(st.C, E, e, 0, 0, C11); st_block<P>(st.C, E, e, 0, 2, -C11);
+        st_block<P>(st.C, E, e, 1, 0, C21 * TT1t); st_block<P>(st.C, E, e, 1, 2, (-C21) * TT1t);
+        st_block<P>(st.C, E, e, 2, 0, C31); st_block<P>(st.C, E, e, 2, 2, -C31);
+        st_block<P>(st.C, E, e, 3, 0, C41 * TT2t); st_block<P>(st.C, E, e, 3, 2, (-C41) * TT2t);
+    }
+    if constexpr (CB) {
+        C12 = (l02Re * C12) * ReT; C14 = (l02Re * C14) * ReT; C22 = (l02Re * C22) * ReT; C24 = (l02Re * C24) * ReT;
+        C32 = (l02Re * C32) * ReT; C34 = (l02Re * C34) * ReT; C42 = (l02Re * C42) * ReT; C44 = (l02Re * C44) * ReT;
+        st_block<P>(st.C, E, e, 0, 1, C12); st_block<P>(st.C, E, e, 0, 3, C14);
+        st_block<P>(st.C, E, e, 1, 1, C22 * TT1t); st_block<P>(st.C, E, e, 1, 3, C24 * TT1t);
+        st_block<P>(st.C, E, e, 2, 1, C32); st_block<P>(st.C, E, e, 2, 3, C34);
+        st_block<P>(st.C, E, e, 3, 1, C42 * TT2t); st_block<P>(st.C, E, e, 3, 3, C44 * TT2t);
+    }
 }
 
+// Rayleigh-type damping term of Cᵏ (compute_beams.jl:462-479): Cᵏ += damping·M with the M blocks
+// taken before the Tₛ⁻¹ post-multiplication, which then applies to the sum on block rows 2 and 4.
+// Added by the mass part to the C blocks the other parts stored.
 template <typename P>
-__global__ void __launch_bounds__(64) beam_kernel(NodeState ns, BeamConsts bc, BeamStage<P> st, GaussPts gp) {
-    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= bc.E) return;
-    StaticOut so = beam_static<P>(e, ns, bc, st);
-    beam_dynamic<P>(e, ns, bc, st, gp, so);
+EBD void beam_damping(int64_t e, const NodeState& ns, const BeamConsts& bc, const BeamStage<P>& st,
+                      const MassBlocks<P>& m) {
+    const int64_t E = bc.E;
+    const P d = P(bc.props[13 * E + e]);
+    const M3<P> TT1t = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, bc.n1[e])))));
+    const M3<P> TT2t = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, bc.n2[e])))));
+    auto add = [&](int bi, int bj, const M3<P>& B) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) st.C[e * 144 + (3 * bj + j) * 12 + 3 * bi + i] += B.a[i + 3 * j];
+    };
+    add(0, 0, d * m.M11); add(0, 1, d * m.M12); add(0, 2, d * m.M13); add(0, 3, d * m.M14);
+    add(1, 0, (d * tr(m.M12)) * TT1t); add(1, 1, (d * m.M22) * TT1t);
+    add(1, 2, (d * m.M23) * TT1t); add(1, 3, (d * m.M24) * TT1t);
+    add(2, 0, d * tr(m.M13)); add(2, 1, d * tr(m.M23)); add(2, 2, d * m.M33); add(2, 3, d * m.M34);
+    add(3, 0, (d * tr(m.M14)) * TT2t); add(3, 1, (d * tr(m.M24)) * TT2t);
+    add(3, 2, (d * tr(m.M34)) * TT2t); add(3, 3, (d * m.M44) * TT2t);
+}
+
+// One workgroup per 64 elements, four wavefronts: the static part and the three dynamic parts of
+// the same 64 elements. Lane l of every wavefront works on element 64·blockIdx.x + l.
+constexpr int BEAM_TILE = 64;
+
+template <typename P>
+__global__ void __launch_bounds__(4 * BEAM_TILE) beam_kernel(NodeState ns, BeamConsts bc, BeamStage<P> st, GaussPts gp) {
+    const int part = __builtin_amdgcn_readfirstlane(threadIdx.x / BEAM_TILE);
+    const int64_t e = (int64_t)blockIdx.x * BEAM_TILE + threadIdx.x % BEAM_TILE;
+    const bool valid = e < bc.E;
+    MassBlocks<P> mb;
+    if (valid) {
+        if (part == PART_STATIC) {
+            beam_static<P>(e, ns, bc, st);
+        } else {
+            const StaticOut so = beam_frame(e, ns, bc);
+            if (part == PART_MASS) beam_dynamic<P, PART_MASS>(e, ns, bc, st, gp, so, mb);
+            else if (part == PART_C13) beam_dynamic<P, PART_C13>(e, ns, bc, st, gp, so, mb);
+            else beam_dynamic<P, PART_C24>(e, ns, bc, st, gp, so, mb);
+        }
+    }
+    // every wavefront sees the same 64 elements, so this test is uniform over the workgroup
+    const bool damped = __ballot(valid && bc.props[13 * bc.E + e] > 0.0) != 0;
+    if (damped) {
+        __syncthreads();
+        if (part == PART_MASS && valid && bc.props[13 * bc.E + e] > 0.0) beam_damping<P>(e, ns, bc, st, mb);
+    }
 }
 
 }  // namespace ebd
