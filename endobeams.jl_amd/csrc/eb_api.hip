@@ -78,6 +78,8 @@ struct eb_ctx {
     int* h_status = nullptr;
     // contact
     int32_t* cbeams = nullptr;
+    uint64_t* clist = nullptr;     // beams found in contact, (beam << 8 | Gauss point mask)
+    int32_t* clist_n = nullptr;
     void* sdf_coef = nullptr;
     // chains
     int32_t *ch_ptr = nullptr, *ch_node = nullptr, *ch_diag = nullptr, *ch_lo = nullptr, *ch_up = nullptr;
@@ -245,9 +247,13 @@ template <typename P> int launch_contact(eb_ctx* ctx, double alpha) {
     ca.n = ctx->n_contact;
     ca.Tc = (P*)ctx->sTc;
     ca.energy = ctx->senergy;
+    hipLaunchKernelGGL(contact_reset, dim3(1), dim3(1), 0, ctx->stream, ctx->clist_n);
+    hipLaunchKernelGGL(contact_detect<P>, dim3(nblk(ctx->n_contact, 256)), dim3(256), 0, ctx->stream, node_state(ctx),
+                       beam_consts(ctx), ctx->gauss, ca, ctx->clist, ctx->clist_n);
+    EB_TRY(check_launch(ctx, "contact_detect"));
     prof_begin(ctx, EB_KERNEL_CONTACT);
-    hipLaunchKernelGGL(contact_kernel<P>, dim3(nblk(ctx->n_contact, 64)), dim3(64), 0, ctx->stream, node_state(ctx),
-                       beam_consts(ctx), beam_stage<P>(ctx), ctx->gauss, ca);
+    hipLaunchKernelGGL(contact_kernel<P>, dim3(nblk(ctx->n_contact, CONTACT_TILE)), dim3(4 * CONTACT_TILE), 0, ctx->stream,
+                       node_state(ctx), beam_consts(ctx), beam_stage<P>(ctx), ctx->gauss, ca, ctx->clist, ctx->clist_n);
     prof_end(ctx, EB_KERNEL_CONTACT);
     return check_launch(ctx, "contact_kernel");
 }
@@ -556,6 +562,8 @@ int eb_create(const eb_model_desc* d, eb_ctx** out) {
         }
         ctx->n_contact = d->n_contact_beams;
         CK(upload_vec(ctx, ctx->cbeams, cb));
+        CK(dalloc(ctx, ctx->clist, ctx->n_contact));
+        CK(dalloc(ctx, ctx->clist_n, 1));
         if (d->surface == EB_SURFACE_SPHERE) {
             for (int a = 0; a < 3; ++a) ctx->sphere.c[a] = d->sphere_center[a];
             ctx->sphere.R = d->sphere_radius;

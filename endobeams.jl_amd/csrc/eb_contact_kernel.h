@@ -3,8 +3,9 @@
 // gap functions of src/interactions/utils_compute_interactions_beams.jl:1-75 and the
 // regularisation of src/interactions/interactions_beams_regularization.jl:1-45.
 //
-// One lane per contact beam. The lane first places its Gauss points (xᴳ) and runs the contact
-// test; only lanes with a Gauss point in contact build the consistent tangent. Contributions are
+// Two passes: a detection pass (one lane per contact beam) places the Gauss points (xᴳ), runs the
+// contact test and lists the beams in contact; the tangent pass spreads each listed beam over four
+// wavefronts, one per block column of Kᶜ/Cᶜ. Contributions are
 // folded into the element staging in place: K -= Kᶜ, C -= (1+α)Cᶜ (assemble_contact!, :493-497),
 // so one gather assembles beams and contact together.
 #pragma once
@@ -127,13 +128,12 @@ template <typename P> EBD void contact_gap(const ContactArgs<P>& ca, V3<double> 
     g = v - P(rb);
 }
 
-template <typename P>
-__global__ void __launch_bounds__(64) contact_kernel(NodeState ns, BeamConsts bc, BeamStage<P> st, GaussPts gp,
-                                                     ContactArgs<P> ca) {
-    int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (li >= ca.n) return;
+// Consistent contact tangent of one beam for block column B of Kᶜ and Cᶜ (rows a = 0..3), and
+// Tᶜ of node slot B; the beam has a Gauss point in contact (mask: the points in contact).
+template <typename P, int B>
+EBD void contact_column(int64_t e, unsigned mask, const NodeState& ns, const BeamConsts& bc, const BeamStage<P>& st,
+                        const GaussPts& gp, const ContactArgs<P>& ca) {
     const int64_t E = bc.E;
-    const int64_t e = ca.beams[li];
     const int32_t na = bc.n1[e], nb = bc.n2[e];
     const double rb = bc.props[0 * E + e];
     V3<double> x1 = ldv(ns.X0, na) + ldv(ns.u, na), x2 = ldv(ns.X0, nb) + ldv(ns.u, nb);
@@ -145,20 +145,6 @@ __global__ void __launch_bounds__(64) contact_kernel(NodeState ns, BeamConsts bc
     M3<double> ReTd = tr(f.Re);
     V3<double> Tb1 = toangle((ReTd * R1) * Re0), Tb2 = toangle((ReTd * R2) * Re0);
     const double l0 = bc.l0[e];
-
-    // Gauss point placement and contact test (FP64 positions)
-    unsigned mask = 0;
-    for (int ig = 0; ig < gp.n; ++ig) {
-        ShapeF s = shape_fn(l0, gp.z[ig]);
-        V3<double> ut = {0.0, s.N3 * Tb1.z + s.N4 * Tb2.z, (-s.N3) * Tb1.y + (-s.N4) * Tb2.y};
-        V3<double> xG = (s.N1 * x1 + s.N2 * x2) + f.Re * ut;
-        if (in_contact<P>(ca, xG, rb)) mask |= 1u << ig;
-    }
-    if (!mask) {
-        for (int k = 0; k < 12; ++k) ca.Tc[e * 12 + k] = P(0);
-        ca.energy[3 * e + 2] = 0.0;
-        return;
-    }
 
     const M3<P> I = eye<P>();
     const M3<P> Re = cvt<P>(f.Re), ReT = tr(Re);
@@ -177,12 +163,10 @@ __global__ void __launch_bounds__(64) contact_kernel(NodeState ns, BeamConsts bc
     const V3<P> tb1 = cvt<P>(Tb1), tb2 = cvt<P>(Tb2);
     P eta_n = P(ca.eta_n);  // overwritten by the smoothstep at each contacting point (:100)
     double energy = 0;
-    V3<P> Tc[4];
-    M3<P> Kc[4][4], Cc[4][4];
-    for (int a = 0; a < 4; ++a) {
-        Tc[a] = zv<P>();
-        for (int b = 0; b < 4; ++b) { Kc[a][b] = zm<P>(); Cc[a][b] = zm<P>(); }
-    }
+    V3<P> Tcb = zv<P>();
+    M3<P> Kc[4], Cc[4];
+    for (int a = 0; a < 4; ++a) { Kc[a] = zm<P>(); Cc[a] = zm<P>(); }
+#pragma unroll 1
     for (int ig = 0; ig < gp.n; ++ig) {
         if (!(mask & (1u << ig))) continue;
         const P w = P(gp.w[ig]);
@@ -243,7 +227,7 @@ __global__ void __launch_bounds__(64) contact_kernel(NodeState ns, BeamConsts bc
         V3<P> Fc = tmul(Re, fc);
         M3<P> RHR[4];
         for (int a = 0; a < 4; ++a) RHR[a] = (Re * H1[a]) * ReT;
-        for (int a = 0; a < 4; ++a) Tc[a] += w * tmul(RHR[a], fc);
+        Tcb += w * tmul(RHR[B], fc);
         M3<P> A1F = rows(P(0), P(0), P(0), Fc.y * v1.x, Fc.y * v1.y, Fc.y * v1.z, Fc.z * v1.x, Fc.z * v1.y, Fc.z * v1.z);
         M3<P> SF = skew(Fc);
         M3<P> SFP1 = (SF * PP1) * ReT, SFP2 = (SF * PP2) * ReT, SFP4 = (SF * PP4) * ReT;
@@ -262,75 +246,134 @@ __global__ void __launch_bounds__(64) contact_kernel(NodeState ns, BeamConsts bc
         M3<P> auxn = ((kn * dpn) * nn + (kn * pn) * Hg) - outer(deta * udn, n);
         M3<P> aux1 = outer((((-mureg) * kn) * dpn) * udt, n);
         M3<P> aux2 = (((-mureg) * kn) * pn) * (I - outer((P(1) / (udt2 + eps_t)) * udt, udt));
-        M3<P> Kfc[4], Cfc[4];
-        for (int b = 0; b < 4; ++b) {
-            M3<P> dundd = aux * RHR[b] + nn * du0dd[b];
-            M3<P> dutdd = du0dd[b] - dundd;
-            M3<P> dutdv = RHR[b] - nn * RHR[b];
-            M3<P> Kfn = auxn * RHR[b] - eta_n * dundd;
-            M3<P> Cfn = ((-eta_n) * nn) * RHR[b];
-            M3<P> Kft = aux1 * RHR[b] + aux2 * dutdd;
-            M3<P> Cft = aux2 * dutdv;
-            Kfc[b] = Kfn + Kft;
-            Cfc[b] = Cfn + Cft;
-        }
+        constexpr int b = B;
+        M3<P> dundd = aux * RHR[b] + nn * du0dd[b];
+        M3<P> dutdd = du0dd[b] - dundd;
+        M3<P> dutdv = RHR[b] - nn * RHR[b];
+        M3<P> Kfn = auxn * RHR[b] - eta_n * dundd;
+        M3<P> Cfn = ((-eta_n) * nn) * RHR[b];
+        M3<P> Kft = aux1 * RHR[b] + aux2 * dutdd;
+        M3<P> Cft = aux2 * dutdv;
+        const M3<P> Kfc = Kfn + Kft;
+        const M3<P> Cfc = Cfn + Cft;
+#pragma unroll
         for (int a = 0; a < 4; ++a) {
             M3<P> RS = Re * skew(tmul(H1[a], Fc));
             M3<P> RHS = (Re * tr(H1[a])) * SF;
             M3<P> tRHR = tr(RHR[a]);
             M3<P> RGa = a == 0 ? RG1 : (a == 1 ? RG2 : (a == 2 ? -RG1 : RG4));
-            for (int b = 0; b < 4; ++b) {
-                M3<P> t1, t2, t3;
-                // t₁ᵃᵇ = -Rₑ S(H₁ᵃᵀ𝓕ᶜ) Gᵇ Rₑᵀ with G³ = -G¹
-                M3<P> GbR = b == 0 ? G1R : (b == 1 ? G2R : (b == 2 ? G1R : G4R));
-                t1 = b == 2 ? RS * GbR : -(RS * GbR);
-                t3 = b == 2 ? -(RHS * GbR) : RHS * GbR;
-                // t₂ (:166-184): rows 1 and 3 carry the A₁ᵀ𝓕ᶜr term; row 3 mirrors row 1
-                M3<P> SFPb = b == 0 ? SFP1 : (b == 1 ? SFP2 : (b == 2 ? SFP1 : SFP4));
-                if (a == 0 || a == 2) {
-                    M3<P> base;
-                    if (b == 0 || b == 2) base = (N7l2 * Re) * A1F - RG1 * SFP1;
-                    else base = -(RG1 * SFPb);
-                    // t₂¹¹ = base₁, t₂¹³ = -base₁, t₂³¹ = t₂¹³, t₂³³ = t₂¹¹; t₂¹² = base₂, t₂³² = -t₂¹²; likewise col 4
-                    bool neg = (b == 2) != (a == 2);
-                    t2 = neg ? -base : base;
-                } else {
-                    M3<P> base = -(RGa * SFP1);
-                    if (b == 0) t2 = base;
-                    else if (b == 2) t2 = -base;
-                    else t2 = -(RGa * SFPb);
-                }
-                M3<P> t4 = tRHR * Kfc[b];
-                Kc[a][b] += w * (((t1 + t2) + t3) + t4);
-                Cc[a][b] += (w * tRHR) * Cfc[b];
+            M3<P> t1, t2, t3;
+            // t₁ᵃᵇ = -Rₑ S(H₁ᵃᵀ𝓕ᶜ) Gᵇ Rₑᵀ with G³ = -G¹
+            M3<P> GbR = b == 0 ? G1R : (b == 1 ? G2R : (b == 2 ? G1R : G4R));
+            t1 = b == 2 ? RS * GbR : -(RS * GbR);
+            t3 = b == 2 ? -(RHS * GbR) : RHS * GbR;
+            // t₂ (:166-184): rows 1 and 3 carry the A₁ᵀ𝓕ᶜr term; row 3 mirrors row 1
+            M3<P> SFPb = b == 0 ? SFP1 : (b == 1 ? SFP2 : (b == 2 ? SFP1 : SFP4));
+            if (a == 0 || a == 2) {
+                M3<P> base;
+                if (b == 0 || b == 2) base = (N7l2 * Re) * A1F - RG1 * SFP1;
+                else base = -(RG1 * SFPb);
+                // t₂¹¹ = base₁, t₂¹³ = -base₁, t₂³¹ = t₂¹³, t₂³³ = t₂¹¹; t₂¹² = base₂, t₂³² = -t₂¹²; likewise col 4
+                bool neg = (b == 2) != (a == 2);
+                t2 = neg ? -base : base;
+            } else {
+                M3<P> base = -(RGa * SFP1);
+                if (b == 0) t2 = base;
+                else if (b == 2) t2 = -base;
+                else t2 = -(RGa * SFPb);
             }
+            M3<P> t4 = tRHR * Kfc;
+            Kc[a] += w * (((t1 + t2) + t3) + t4);
+            Cc[a] += (w * tRHR) * Cfc;
         }
     }
     const P l02 = P(l0 / 2);
-    ca.energy[3 * e + 2] = double(l02) * energy;
+    if (B == 0) ca.energy[3 * e + 2] = double(l02) * energy;
     const M3<P> TT1t = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, na)))));
     const M3<P> TT2t = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, nb)))));
     const P c1a = P(1 + ca.alpha);
-    for (int a = 0; a < 4; ++a) {
-        V3<P> t = l02 * Tc[a];
-        ca.Tc[e * 12 + 3 * a] = t.x;
-        ca.Tc[e * 12 + 3 * a + 1] = t.y;
-        ca.Tc[e * 12 + 3 * a + 2] = t.z;
-        for (int b = 0; b < 4; ++b) {
-            M3<P> K = l02 * Kc[a][b];
-            M3<P> Cm = l02 * Cc[a][b];
-            if (a == 1) Cm = Cm * TT1t;
-            if (a == 3) Cm = Cm * TT2t;
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    int64_t idx = e * 144 + (3 * b + j) * 12 + 3 * a + i;
-                    st.K[idx] += -K.a[i + 3 * j];
-                    st.C[idx] += -(c1a * Cm.a[i + 3 * j]);
-                }
-        }
+    {
+        V3<P> t = l02 * Tcb;
+        ca.Tc[e * 12 + 3 * B] = t.x;
+        ca.Tc[e * 12 + 3 * B + 1] = t.y;
+        ca.Tc[e * 12 + 3 * B + 2] = t.z;
     }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        M3<P> K = l02 * Kc[a];
+        M3<P> Cm = l02 * Cc[a];
+        if (a == 1) Cm = Cm * TT1t;
+        if (a == 3) Cm = Cm * TT2t;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                int64_t idx = e * 144 + (3 * B + j) * 12 + 3 * a + i;
+                st.K[idx] += -K.a[i + 3 * j];
+                st.C[idx] += -(c1a * Cm.a[i + 3 * j]);
+            }
+    }
+}
+
+// Pass 1, one lane per listed contact beam: Gauss point placement (FP64 positions) and the
+// contact test. Beams with no point in contact get zero Tᶜ and contact energy; the others are
+// appended to the work list as (beam << 8 | point mask). List order does not matter: every beam
+// writes only its own staging record.
+template <typename P>
+__global__ void __launch_bounds__(256) contact_detect(NodeState ns, BeamConsts bc, GaussPts gp, ContactArgs<P> ca,
+                                                      uint64_t* list, int32_t* count) {
+    const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= ca.n) return;
+    const int64_t E = bc.E;
+    const int64_t e = ca.beams[li];
+    const int32_t na = bc.n1[e], nb = bc.n2[e];
+    const double rb = bc.props[0 * E + e];
+    V3<double> x1 = ldv(ns.X0, na) + ldv(ns.u, na), x2 = ldv(ns.X0, nb) + ldv(ns.u, nb);
+    double lnd = norm(x2 - x1);
+    M3<double> Re0;
+    for (int k = 0; k < 9; ++k) Re0.a[k] = bc.Re0[k * E + e];
+    M3<double> R1 = ldm(ns.R, na), R2 = ldm(ns.R, nb);
+    LocalFrame f = local_frame(x1, x2, R1, R2, col(Re0, 1), lnd);
+    M3<double> ReTd = tr(f.Re);
+    V3<double> Tb1 = toangle((ReTd * R1) * Re0), Tb2 = toangle((ReTd * R2) * Re0);
+    const double l0 = bc.l0[e];
+    unsigned mask = 0;
+    for (int ig = 0; ig < gp.n; ++ig) {
+        ShapeF s = shape_fn(l0, gp.z[ig]);
+        V3<double> ut = {0.0, s.N3 * Tb1.z + s.N4 * Tb2.z, (-s.N3) * Tb1.y + (-s.N4) * Tb2.y};
+        V3<double> xG = (s.N1 * x1 + s.N2 * x2) + f.Re * ut;
+        if (in_contact<P>(ca, xG, rb)) mask |= 1u << ig;
+    }
+    if (!mask) {
+        for (int k = 0; k < 12; ++k) ca.Tc[e * 12 + k] = P(0);
+        ca.energy[3 * e + 2] = 0.0;
+        return;
+    }
+    const int32_t slot = atomicAdd(count, 1);
+    list[slot] = (uint64_t(e) << 8) | mask;
+}
+
+__global__ void contact_reset(int32_t* count) { *count = 0; }
+
+// Pass 2: four wavefronts per 64 listed beams, wavefront b computing block column b. Launched for
+// the whole contact list; workgroups past the number of beams found in contact exit at once.
+constexpr int CONTACT_TILE = 64;
+
+template <typename P>
+__global__ void __launch_bounds__(4 * CONTACT_TILE) contact_kernel(NodeState ns, BeamConsts bc, BeamStage<P> st,
+                                                                   GaussPts gp, ContactArgs<P> ca,
+                                                                   const uint64_t* __restrict__ list,
+                                                                   const int32_t* __restrict__ count) {
+    const int part = __builtin_amdgcn_readfirstlane(threadIdx.x / CONTACT_TILE);
+    const int64_t li = (int64_t)blockIdx.x * CONTACT_TILE + threadIdx.x % CONTACT_TILE;
+    if (li >= *count) return;
+    const uint64_t ent = list[li];
+    const int64_t e = (int64_t)(ent >> 8);
+    const unsigned mask = (unsigned)(ent & 0xffu);
+    if (part == 0) contact_column<P, 0>(e, mask, ns, bc, st, gp, ca);
+    else if (part == 1) contact_column<P, 1>(e, mask, ns, bc, st, gp, ca);
+    else if (part == 2) contact_column<P, 2>(e, mask, ns, bc, st, gp, ca);
+    else contact_column<P, 3>(e, mask, ns, bc, st, gp, ca);
 }
 
 }  // namespace ebd
