@@ -374,29 +374,153 @@ EBD void mm6_inplace(const double* A, double* B) {
     }
 }
 
+// Masked block of position g (kind 0: D, 1: L, 2: U) read from the tangent: an entry in a fixed
+// row or column becomes the identity's; an absent block is zero.
+EBD double masked_entry(const ChainDev& cd, const double* __restrict__ A, int64_t g, int kind, int k) {
+    const uint64_t m = kind == 0 ? cd.mD[g] : (kind == 1 ? cd.mL[g] : cd.mU[g]);
+    const int32_t blk = kind == 0 ? cd.diag_blk[g] : (kind == 1 ? cd.lo_blk[g] : cd.up_blk[g]);
+    if ((m >> k) & 1) return (kind == 0 && k / 6 == k % 6) ? 1.0 : 0.0;
+    return blk >= 0 ? A[36 * (int64_t)blk + k] : 0.0;
+}
+
+// Block of position g: from the tangent (first level) or from the working arrays
+template <bool FROM_A> EBD void chain_block(const ChainDev& cd, const double* __restrict__ A, int64_t g, int kind, double* out) {
+    if (FROM_A) {
+        const uint64_t m = kind == 0 ? cd.mD[g] : (kind == 1 ? cd.mL[g] : cd.mU[g]);
+        const int32_t blk = kind == 0 ? cd.diag_blk[g] : (kind == 1 ? cd.lo_blk[g] : cd.up_blk[g]);
+        const double* src = A + 36 * (int64_t)(blk >= 0 ? blk : 0);
+#pragma unroll
+        for (int k = 0; k < 36; ++k) {
+            const double v = src[k];
+            out[k] = ((m >> k) & 1) ? ((kind == 0 && k / 6 == k % 6) ? 1.0 : 0.0) : (blk >= 0 ? v : 0.0);
+        }
+    } else {
+        const double* src = (kind == 0 ? cd.Dm : (kind == 1 ? cd.Lm : cd.Um)) + 36 * g;
+#pragma unroll
+        for (int k = 0; k < 36; ++k) out[k] = src[k];
+    }
+}
+
 // phase A for one eliminated position g: D <- D^-1, L <- D^-1 L, U <- D^-1 U. Without PIVOT,
 // returns false when the unpivoted inverse was rejected (nothing is written then).
-template <bool PIVOT> EBD bool chain_eliminate(const ChainDev& cd, int64_t g, bool& ok) {
-    double D[36];
-#pragma unroll
-    for (int k = 0; k < 36; ++k) D[k] = cd.Dm[36 * g + k];
-    if (PIVOT) {
+template <bool PIVOT, bool FROM_A>
+EBD bool chain_eliminate(const ChainDev& cd, const double* __restrict__ A, int64_t g, bool& ok) {
+    double D[36], Lb[36], Ub[36];
+    chain_block<FROM_A>(cd, A, g, 0, D);
+    if (!PIVOT) {  // all three blocks in flight together
+        chain_block<FROM_A>(cd, A, g, 1, Lb);
+        chain_block<FROM_A>(cd, A, g, 2, Ub);
+        if (!inv6_nopiv(D)) return false;
+    } else {
         ok = inv6(D) && ok;
-    } else if (!inv6_nopiv(D)) {
-        return false;
+        chain_block<FROM_A>(cd, A, g, 1, Lb);
+        chain_block<FROM_A>(cd, A, g, 2, Ub);
     }
-    double B[36];
+    mm6_inplace(D, Lb);
+    mm6_inplace(D, Ub);
 #pragma unroll
-    for (int k = 0; k < 36; ++k) B[k] = cd.Lm[36 * g + k];
-    mm6_inplace(D, B);
-#pragma unroll
-    for (int k = 0; k < 36; ++k) cd.Lm[36 * g + k] = B[k];
-#pragma unroll
-    for (int k = 0; k < 36; ++k) B[k] = cd.Um[36 * g + k];
-    mm6_inplace(D, B);
-#pragma unroll
-    for (int k = 0; k < 36; ++k) { cd.Um[36 * g + k] = B[k]; cd.Dm[36 * g + k] = D[k]; }
+    for (int k = 0; k < 36; ++k) { cd.Dm[36 * g + k] = D[k]; cd.Lm[36 * g + k] = Lb[k]; cd.Um[36 * g + k] = Ub[k]; }
     return true;
+}
+
+template <bool FROM_A>
+EBD void chain_level(const ChainDev& cd, const double* __restrict__ A, int64_t s0, int32_t n, int l, bool& ok,
+                     int& redo, double (*P2)[6]) {
+    const int tid = threadIdx.x;
+    const int32_t h = 1 << l;
+    // phase A: one lane per eliminated position; the rare rejected unpivoted inverses are
+    // redone with partial pivoting in a second pass
+    const int32_t ne = (n - h + 2 * h - 1) / (2 * h);
+    for (int32_t m = tid; m < ne; m += CHAIN_THREADS) {
+        const int64_t g = s0 + h + 2 * (int64_t)h * m;
+        if (!chain_eliminate<false, FROM_A>(cd, A, g, ok)) {
+            cd.rej[g] = 1;
+            redo = 1;
+        }
+    }
+    __syncthreads();
+    if (redo) {
+        for (int32_t m = tid; m < ne; m += CHAIN_THREADS) {
+            const int64_t g = s0 + h + 2 * (int64_t)h * m;
+            if (cd.rej[g]) {
+                cd.rej[g] = 0;
+                chain_eliminate<true, FROM_A>(cd, A, g, ok);
+            }
+        }
+        __syncthreads();
+        if (tid == 0) redo = 0;
+        __syncthreads();
+    }
+    // phase B: twelve lanes per kept position i: lane (side s, row r) forms row r of
+    //   s = 0: L_i GU_j1 (into D) and -L_i GL_j1 (new L)
+    //   s = 1: U_i GL_j2 (into D) and -U_i GU_j2 (new U)
+    // Each lane reads and writes only its own row of L_i, U_i or D_i; the side-1 part of D
+    // reaches the side-0 lane through LDS.
+    const int32_t nk = (n + 2 * h - 1) / (2 * h);
+    constexpr int PER = (CHAIN_THREADS / 12) * 12;
+    for (int32_t base = 0; base < 12 * nk; base += PER) {
+        const int32_t t = base + tid;
+        const int32_t m = t / 12, side = (t % 12) / 6, r = t % 6;
+        const bool act = tid < PER && m < nk;
+        const int32_t i = 2 * h * m;
+        const int64_t gi = s0 + i;
+        const bool has = act && (side == 0 ? i > 0 : i + h < n);
+        double row[6] = {}, G1[36] = {}, G2[36] = {}, Dr[6] = {};
+        if (has) {
+            const int64_t gj = side == 0 ? gi - h : gi + h;
+            const double* Gd = (side == 0 ? cd.Um : cd.Lm) + 36 * gj;   // into D
+            const double* Gn = (side == 0 ? cd.Lm : cd.Um) + 36 * gj;   // new L or U
+            if (FROM_A) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) row[k] = masked_entry(cd, A, gi, side == 0 ? 1 : 2, 6 * r + k);
+            } else {
+                const double* X = (side == 0 ? cd.Lm : cd.Um) + 36 * gi + 6 * r;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) row[k] = X[k];
+            }
+#pragma unroll
+            for (int k = 0; k < 36; ++k) { G1[k] = Gd[k]; G2[k] = Gn[k]; }
+        }
+        if (act && side == 0) {
+            if (FROM_A) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) Dr[k] = masked_entry(cd, A, gi, 0, 6 * r + k);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) Dr[k] = cd.Dm[36 * gi + 6 * r + k];
+            }
+        }
+        double pd[6], pn[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            double a1 = 0, a2 = 0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) { a1 = fma(row[k], G1[6 * k + c], a1); a2 = fma(row[k], G2[6 * k + c], a2); }
+            pd[c] = has ? a1 : 0.0;
+            pn[c] = -a2;
+        }
+        if (act && side == 1) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) P2[tid][c] = pd[c];
+        }
+        __syncthreads();
+        if (has) {
+            double* hs = cd.hist + 72 * ((int64_t)cd.hbase[gi] + l) + 36 * side + 6 * r;
+            double* dst = (side == 0 ? cd.Lm : cd.Um) + 36 * gi + 6 * r;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) { hs[k] = row[k]; dst[k] = pn[k]; }
+        } else if (FROM_A && act) {
+            // first level, no neighbour on this side: the coupling block is zero
+            double* dst = (side == 0 ? cd.Lm : cd.Um) + 36 * gi + 6 * r;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) dst[k] = 0.0;
+        }
+        if (act && side == 0) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) cd.Dm[36 * gi + 6 * r + c] = Dr[c] - pd[c] - P2[tid + 6][c];
+        }
+        __syncthreads();
+    }
 }
 
 __global__ void __launch_bounds__(CHAIN_THREADS) chain_factor(ChainDev cd, const double* __restrict__ A, int* status) {
@@ -406,108 +530,16 @@ __global__ void __launch_bounds__(CHAIN_THREADS) chain_factor(ChainDev cd, const
     __shared__ double P2[CHAIN_THREADS][6];
     __shared__ int redo;
     if (tid == 0) redo = 0;
-    // masked blocks of the chain: a fixed row or column becomes a row/column of the identity
-    for (int64_t t = tid; t < 36 * (int64_t)n; t += CHAIN_THREADS) {
-        const int64_t g = s0 + t / 36;
-        const int k = (int)(t % 36);
-        const uint64_t bit = uint64_t(1) << k;
-        const double d = (cd.mD[g] & bit) ? ((k / 6 == k % 6) ? 1.0 : 0.0) : A[36 * (int64_t)cd.diag_blk[g] + k];
-        const int32_t lb = cd.lo_blk[g], ub = cd.up_blk[g];
-        const double lo = (lb >= 0 && !(cd.mL[g] & bit)) ? A[36 * (int64_t)lb + k] : 0.0;
-        const double up = (ub >= 0 && !(cd.mU[g] & bit)) ? A[36 * (int64_t)ub + k] : 0.0;
-        cd.Dm[36 * g + k] = d;
-        cd.Lm[36 * g + k] = lo;
-        cd.Um[36 * g + k] = up;
-    }
     __syncthreads();
     bool ok = true;
     const int nl = chain_levels(n);
-    for (int l = 0; l < nl; ++l) {
-        const int32_t h = 1 << l;
-        // phase A: one lane per eliminated position; the rare rejected unpivoted inverses are
-        // redone with partial pivoting in a second pass
-        const int32_t ne = (n - h + 2 * h - 1) / (2 * h);
-        for (int32_t m = tid; m < ne; m += CHAIN_THREADS) {
-            const int64_t g = s0 + h + 2 * (int64_t)h * m;
-            if (!chain_eliminate<false>(cd, g, ok)) {
-                cd.rej[g] = 1;
-                redo = 1;
-            }
-        }
-        __syncthreads();
-        if (redo) {
-            for (int32_t m = tid; m < ne; m += CHAIN_THREADS) {
-                const int64_t g = s0 + h + 2 * (int64_t)h * m;
-                if (cd.rej[g]) {
-                    cd.rej[g] = 0;
-                    chain_eliminate<true>(cd, g, ok);
-                }
-            }
-            __syncthreads();
-            if (tid == 0) redo = 0;
-            __syncthreads();
-        }
-        // phase B: twelve lanes per kept position i: lane (side s, row r) forms row r of
-        //   s = 0: L_i GU_j1 (into D) and -L_i GL_j1 (new L)
-        //   s = 1: U_i GL_j2 (into D) and -U_i GU_j2 (new U)
-        // Each lane reads and writes only its own row of L_i, U_i or D_i; the side-1 part of D
-        // reaches the side-0 lane through LDS.
-        const int32_t nk = (n + 2 * h - 1) / (2 * h);
-        constexpr int PER = (CHAIN_THREADS / 12) * 12;
-        for (int32_t base = 0; base < 12 * nk; base += PER) {
-            const int32_t t = base + tid;
-            const int32_t m = t / 12, side = (t % 12) / 6, r = t % 6;
-            const bool act = tid < PER && m < nk;
-            const int32_t i = 2 * h * m;
-            const int64_t gi = s0 + i;
-            const bool has = act && (side == 0 ? i > 0 : i + h < n);
-            double row[6] = {}, G1[36] = {}, G2[36] = {}, Dr[6] = {};
-            if (has) {
-                const int64_t gj = side == 0 ? gi - h : gi + h;
-                const double* X = (side == 0 ? cd.Lm : cd.Um) + 36 * gi + 6 * r;
-                const double* Gd = (side == 0 ? cd.Um : cd.Lm) + 36 * gj;   // into D
-                const double* Gn = (side == 0 ? cd.Lm : cd.Um) + 36 * gj;   // new L or U
-#pragma unroll
-                for (int k = 0; k < 6; ++k) row[k] = X[k];
-#pragma unroll
-                for (int k = 0; k < 36; ++k) { G1[k] = Gd[k]; G2[k] = Gn[k]; }
-            }
-            if (act && side == 0) {
-#pragma unroll
-                for (int k = 0; k < 6; ++k) Dr[k] = cd.Dm[36 * gi + 6 * r + k];
-            }
-            double pd[6], pn[6];
-#pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                double a1 = 0, a2 = 0;
-#pragma unroll
-                for (int k = 0; k < 6; ++k) { a1 = fma(row[k], G1[6 * k + c], a1); a2 = fma(row[k], G2[6 * k + c], a2); }
-                pd[c] = has ? a1 : 0.0;
-                pn[c] = -a2;
-            }
-            if (act && side == 1) {
-#pragma unroll
-                for (int c = 0; c < 6; ++c) P2[tid][c] = pd[c];
-            }
-            __syncthreads();
-            if (has) {
-                double* hs = cd.hist + 72 * ((int64_t)cd.hbase[gi] + l) + 36 * side + 6 * r;
-                double* dst = (side == 0 ? cd.Lm : cd.Um) + 36 * gi + 6 * r;
-#pragma unroll
-                for (int k = 0; k < 6; ++k) { hs[k] = row[k]; dst[k] = pn[k]; }
-            }
-            if (act && side == 0) {
-#pragma unroll
-                for (int c = 0; c < 6; ++c) cd.Dm[36 * gi + 6 * r + c] = Dr[c] - pd[c] - P2[tid + 6][c];
-            }
-            __syncthreads();
-        }
-    }
+    // the first level reads the masked tangent blocks directly
+    if (nl > 0) chain_level<true>(cd, A, s0, n, 0, ok, redo, P2);
+    for (int l = 1; l < nl; ++l) chain_level<false>(cd, A, s0, n, l, ok, redo, P2);
     if (tid == 0) {
-        double D[36];
-#pragma unroll
-        for (int k = 0; k < 36; ++k) D[k] = cd.Dm[36 * s0 + k];
-        double C[36];
+        double D[36], C[36];
+        if (nl > 0) chain_block<false>(cd, A, s0, 0, D);
+        else chain_block<true>(cd, A, s0, 0, D);
 #pragma unroll
         for (int k = 0; k < 36; ++k) C[k] = D[k];
         if (inv6_nopiv(C)) {
