@@ -1,20 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark: element-steps/s and Newton iterations/step of the device Newton iteration.
 
-One "step" is one HHT-α time step of the workload (predictor + Newton corrector to tolerance
-1e-8, with the reference's adaptive Δt), inputs resident on the GPU. The default workload is the
-braided stent: 24 helical wires × 200 beams (4800 beams, 4824 nodes, 28 944 dofs) crimped to
-1.5 mm and released in a rigid vessel given as a 256³ FP32 signed distance field.
+One "step" is one HHT-α time step (predictor + Newton corrector to residual and displacement
+tolerance 1e-8, the reference's adaptive Δt), inputs resident on the GPU. Workload: the braided
+stent, 24 helical wires × 200 beams (4800 beams, 4824 nodes, 28 944 dofs), crimped to 1.5 mm and
+released in a rigid vessel given as a 256³ FP32 signed distance field.
 
-Prints one JSON line with the metric, the roofline of the element kernel (FP64 flops from the
-op-counting restatement ÷ average launch time measured with HIP events in this run) and the CPU
-restatement's throughput on a bounded sample of the same workload (host cores, 1 thread).
+Two forms are timed in the same run:
+  plain   — FP64 element/contact kernels, one stent per context;
+  fast    — the precision form (FP32 element matrices and contact tangent; forces, residuals,
+            rotations, Krylov and updates FP64) on a batch of seeded stent variants advanced in
+            one context (each instance its own vessel SDF).
+`value` is the fast form's whole-job throughput; the plain form is reported beside it with the
+ratio. Also reported: the roofline of the fast form's element kernel (algorithmic flops per
+launch ÷ average launch time from HIP events in this run), the CPU restatement's throughput on a
+bounded sample of the plain workload (1 thread) and the fast form's trajectory error against
+that FP64 CPU run at the same step.
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "endobeams.jl_amd"))
@@ -26,27 +35,13 @@ METRIC = "element-steps/s & Newton iters/step, learned+FP32 vs plain FP64 GPU, s
 # (tests/test_flop_count.py keeps this figure in sync).
 FLOPS_PER_BEAM = 26547
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector
 PEAK_HBM_GBS = 8000.0
 
 
-def workload(name, sdf_n):
+def stent(seed=None, sdf_n=256):
     from endobeams_amd import configs
-
-    if name == "braided_stent":
-        conf, params, inter = configs.braided_stent(sdf_n=sdf_n)
-        desc = f"braided stent: 24 helical wires x 200 beams (4800 beams, 4824 nodes), {sdf_n}^3 FP32 vessel SDF"
-    elif name == "net_on_sphere":
-        conf, params, inter = configs.net_on_sphere()
-        desc = "net of 40x40 wire crossings (3120 beams, 1600 nodes) on a 128^3 FP32 sphere SDF"
-    elif name == "ring_drop":
-        conf, params, inter = configs.ring_drop()
-        desc = "ring of 60 beams dropped on a 64x64x32 FP32 plane SDF"
-    elif name == "l_cantilever":
-        conf, params, inter = configs.l_cantilever()
-        desc = "L-shaped cantilever, 20 beams, tip load ramp"
-    else:
-        raise SystemExit(f"unknown workload {name}")
-    return conf, params, inter, desc
+    return configs.braided_stent(sdf_n=sdf_n, seed=seed)
 
 
 def run_steps(stepper, n):
@@ -59,95 +54,141 @@ def run_steps(stepper, n):
     return done, attempts
 
 
-def cpu_baseline(args):
+def timed(eng, params, E, warmup, steps, kernel="beam"):
+    from endobeams_amd import Stepper
+    st = Stepper(eng, params)
+    st.initialize()
+    run_steps(st, warmup)
+    eng.profile_enable([kernel])
+    n0, it0, t0sim, f0 = st.stats.steps, st.stats.newton_iterations, st.tn, st.stats.failed_steps
+    t0 = time.perf_counter()
+    done, attempts = run_steps(st, steps)
+    wall = time.perf_counter() - t0
+    ms, launches = eng.profile_read(kernel)
+    return {"steps": done, "attempts": attempts, "wall": wall, "E": E,
+            "value": E * done / wall, "ms_per_step": 1e3 * wall / max(done, 1),
+            "newton_iters_per_step": (st.stats.newton_iterations - it0) / max(done, 1),
+            "failed_steps": st.stats.failed_steps - f0,
+            "wall_s_per_simulated_s": wall / (st.tn - t0sim) if st.tn > t0sim else None,
+            "kernel_ms": ms, "kernel_launches": launches, "stepper": st}
+
+
+def cpu_baseline(seconds):
+    """The FP64 CPU restatement on the plain workload: its first step, then steps for `seconds`."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
     from endobeams_amd import Stepper
-
-    conf, params, inter, _ = workload(args.workload, args.sdf_n)
+    conf, params, inter = stent()
     eng = O.OracleEngine(conf, params, inter)
     st = Stepper(eng, params)
     st.initialize()
-    run_steps(st, 1)  # same first step as the GPU warm-up
+    run_steps(st, 1)
     t0 = time.perf_counter()
     n0, it0 = st.stats.steps, st.stats.newton_iterations
-    while time.perf_counter() - t0 < args.cpu_seconds and not st.finished:
+    while time.perf_counter() - t0 < seconds and not st.finished:
         run_steps(st, 1)
     wall = time.perf_counter() - t0
     steps = st.stats.steps - n0
     E = len(conf.beams)
-    return {"value": E * steps / wall, "unit": "element-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} converged steps of the same workload after its first step, "
-                      f"{st.stats.newton_iterations - it0} Newton iterations, {wall:.1f} s, "
-                      f"C++ FP64 restatement with banded-LU direct solve, 1 thread"}
+    out = {"value": E * steps / wall, "unit": "element-steps/s", "cores": 1, "kind": "port",
+           "sample": f"braided stent (4800 beams), {steps} converged steps after its first step, "
+                     f"{st.stats.newton_iterations - it0} Newton iterations, {wall:.1f} s, C++ FP64 "
+                     f"restatement with banded-LU direct solve, 1 thread"}
+    return out, eng, st.stats.steps
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="braided_stent")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=16, help="stents advanced together by the fast form")
     ap.add_argument("--sdf-n", type=int, default=256)
-    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["plain", "fast"], default=None, help="time one form only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(independent structures would shard across GPUs without communication)", file=sys.stderr)
         sys.exit(2)
 
-    from endobeams_amd import HipEngine, Stepper, abi
+    from endobeams_amd import HipBatchEngine, HipEngine, abi
 
-    conf, params, inter, desc = workload(args.workload, args.sdf_n)
-    prec = abi.EB_PRECISION_FP32 if args.precision == "fp32" else abi.EB_PRECISION_FP64
-    eng = HipEngine(conf, params, inter, precision=prec)
-    st = Stepper(eng, params)
-    st.initialize()
-    run_steps(st, args.warmup)
-    print(f"# warm-up done: {st.stats.steps} steps", file=sys.stderr, flush=True)
+    plain = fast = None
+    if args.only in (None, "plain"):
+        conf, params, inter = stent(sdf_n=args.sdf_n)
+        eng = HipEngine(conf, params, inter)
+        plain = timed(eng, params, len(conf.beams), args.warmup, args.steps)
+        print(f"# plain FP64: {plain['steps']} steps, {plain['ms_per_step']:.2f} ms/step, "
+              f"{plain['value']:.0f} element-steps/s", file=sys.stderr, flush=True)
+        eng.close()
+    if args.only in (None, "fast"):
+        t = time.perf_counter()
+        inst = [stent(seed=None if s == 0 else s, sdf_n=args.sdf_n) for s in range(args.batch)]
+        eng = HipBatchEngine(inst, precision=abi.EB_PRECISION_FP32)
+        E = int(eng.beam_off[-1])
+        print(f"# fast form: {args.batch} stents, {E} beams, set up in {time.perf_counter() - t:.1f} s",
+              file=sys.stderr, flush=True)
+        fast = timed(eng, inst[0][1], E, args.warmup, args.steps)
+        print(f"# fast: {fast['steps']} steps, {fast['ms_per_step']:.2f} ms/step, {fast['value']:.0f} element-steps/s",
+              file=sys.stderr, flush=True)
+        eng.close()
 
-    eng.profile_enable(["beam"])
-    n0, it0, t_sim0, fail0 = st.stats.steps, st.stats.newton_iterations, st.tn, st.stats.failed_steps
-    t0 = time.perf_counter()
-    steps, attempts = run_steps(st, args.steps)
-    wall = time.perf_counter() - t0
-    newton = st.stats.newton_iterations - it0
-    sim_t = st.tn - t_sim0
-    beam_ms, beam_n = eng.profile_read("beam")
-    print(f"# timed: {steps} steps, {newton} Newton iterations, {wall:.3f} s", file=sys.stderr, flush=True)
-
-    E = len(conf.beams)
-    value = E * steps / wall
-    avg_s = beam_ms * 1e-3 / max(beam_n, 1)
-    achieved = FLOPS_PER_BEAM * E / avg_s / 1e12
+    main_form = fast if fast is not None else plain
+    avg_s = main_form["kernel_ms"] * 1e-3 / max(main_form["kernel_launches"], 1)
+    flops = FLOPS_PER_BEAM * main_form["E"]
+    achieved = flops / avg_s / 1e12
+    peak = PEAK_FP32_TFLOPS if main_form is fast else PEAK_FP64_TFLOPS
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": main_form["value"],
         "unit": "element-steps/s",
         "n_gpus": 1,
-        "steps": steps,
+        "steps": main_form["steps"],
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / max(steps, 1),
+        "ms_per_step": main_form["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == abi.EB_PRECISION_FP64 else "f32+f64",
+        "dtype": "f32+f64" if main_form is fast else "f64",
         "data": "synthetic",
-        "config": {"workload": desc, "form": "plain FP64" if prec == abi.EB_PRECISION_FP64 else "FP32 element/contact",
-                   "dt": params.initial_timestep, "tolerance": params.tolerance_residual,
-                   "beams": E, "dofs": 6 * len(conf.nodes)},
-        "newton_iters_per_step": newton / max(steps, 1),
-        "failed_steps": st.stats.failed_steps - fail0,
-        "wall_s_per_simulated_s": wall / sim_t if sim_t > 0 else None,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "beam_kernel<double>", "avg_launch_us": avg_s * 1e6, "launches": beam_n,
-                     "flops_per_launch": FLOPS_PER_BEAM * E},
+        "config": {"workload": "braided stent: 24 helical wires x 200 beams (4800 beams, 4824 nodes), "
+                               f"{args.sdf_n}^3 FP32 vessel SDF, Newton tolerance 1e-8",
+                   "form": (f"FP32 element matrices + FP64 forces/solver, batch of {args.batch} seeded stents"
+                            if main_form is fast else "plain FP64, one stent"),
+                   "dt": 1e-3, "tolerance": 1e-8, "beams_per_step": main_form["E"]},
+        "newton_iters_per_step": main_form["newton_iters_per_step"],
+        "failed_steps": main_form["failed_steps"],
+        "wall_s_per_simulated_s": main_form["wall_s_per_simulated_s"],
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "beam_kernel<float>" if main_form is fast else "beam_kernel<double>",
+                     "avg_launch_us": avg_s * 1e6, "launches": main_form["kernel_launches"],
+                     "flops_per_launch": flops},
     }
+    if plain is not None:
+        out["plain_fp64"] = {"value": plain["value"], "ms_per_step": plain["ms_per_step"],
+                             "newton_iters_per_step": plain["newton_iters_per_step"],
+                             "wall_s_per_simulated_s": plain["wall_s_per_simulated_s"],
+                             "beam_kernel_us": 1e3 * plain["kernel_ms"] / max(plain["kernel_launches"], 1)}
+        if fast is not None:
+            out["speedup_vs_plain_fp64"] = fast["value"] / plain["value"]
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args)
+        cb, oracle_eng, n_cpu = cpu_baseline(args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        # precision form against the FP64 CPU restatement after the same number of steps
+        conf, params, inter = stent(sdf_n=args.sdf_n)
+        g = HipEngine(conf, params, inter, precision=abi.EB_PRECISION_FP32)
+        from endobeams_amd import Stepper
+        st = Stepper(g, params)
+        st.initialize()
+        run_steps(st, n_cpu)
+        err = {}
+        for f in ("U_N", "R_N"):
+            a, b = g.get(f), oracle_eng.get(f)
+            err[f] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+        out["fp32_error_vs_fp64_oracle"] = {"steps": n_cpu, **err}
     print(json.dumps(out))
 
 

@@ -40,10 +40,23 @@ struct eb_ctx {
     GaussPts gauss{};
     int surface = EB_SURFACE_NONE;
     double kn = 0, mu = 0, eps_t = 0, eta_n = 0;
-    SphereDev sphere{};
-    int32_t sdf_n[3] = {0, 0, 0};
-    double sdf_d[3] = {0, 0, 0}, sdf_x0[3] = {0, 0, 0}, sdf_dom[6] = {0, 0, 0, 0, 0, 0};
     std::vector<void*> allocs;
+    // instances (independent structures stored one after the other)
+    int64_t S = 1;
+    std::vector<int64_t> node_off, beam_off;          // S+1 each
+    int32_t *node_inst = nullptr, *beam_inst = nullptr, *blk_inst = nullptr, *chain_inst = nullptr;
+    uint8_t* active = nullptr;                        // [S] Newton activity
+    int32_t* kiter = nullptr;                         // [S] k at the end of the corrector loop
+    NewtonAll* nall = nullptr;
+    KrylovAll* kall = nullptr;
+    int64_t *seg_lo = nullptr, *seg_hi = nullptr;
+    int32_t *seg_inst = nullptr, *seg_cptr = nullptr;
+    int32_t nchunks = 0;
+    NewtonAll* h_nall = nullptr;
+    KrylovAll* h_kall = nullptr;
+    int32_t* h_kiter = nullptr;
+    void* d_sdf = nullptr;          // [S] SDFDev<P>
+    SphereDev* d_sphere = nullptr;  // [S]
     // node state
     double* node[16] = {};
     double* X0 = nullptr;
@@ -80,7 +93,6 @@ struct eb_ctx {
     int32_t* cbeams = nullptr;
     uint64_t* clist = nullptr;     // beams found in contact, (beam << 8 | Gauss point mask)
     int32_t* clist_n = nullptr;
-    void* sdf_coef = nullptr;
     // chains
     int32_t *ch_ptr = nullptr, *ch_node = nullptr, *ch_diag = nullptr, *ch_lo = nullptr, *ch_up = nullptr;
     uint64_t *ch_mD = nullptr, *ch_mL = nullptr, *ch_mU = nullptr;
@@ -189,13 +201,15 @@ NodeFields node_fields(eb_ctx* c) {
 }
 BeamConsts beam_consts(eb_ctx* c) { return BeamConsts{c->n1, c->n2, c->l0, c->Re0c, c->propsc, c->E}; }
 template <typename P> BeamStage<P> beam_stage(eb_ctx* c) {
-    return BeamStage<P>{(P*)c->sK, (P*)c->sC, (P*)c->sM, (P*)c->sTint, (P*)c->sTk, c->senergy};
+    return BeamStage<P>{(P*)c->sK, (P*)c->sC, (P*)c->sM, (double*)c->sTint, (double*)c->sTk, c->senergy};
 }
 ChainDev chain_dev(eb_ctx* c) {
     return ChainDev{c->ch_ptr, c->ch_node, c->ch_diag, c->ch_lo, c->ch_up, c->ch_mD, c->ch_mL, c->ch_mU,
-                    c->ch_hbase, c->ch_D, c->ch_L, c->ch_U, c->ch_hist, c->ch_bw, c->ch_yw, c->ch_rej,
+                    c->ch_hbase, c->ch_D, c->ch_L, c->ch_U, c->ch_hist, c->ch_bw, c->ch_yw, c->ch_rej, c->chain_inst,
                     (int64_t)c->chains.ptr.size() - 1};
 }
+Active active_dev(eb_ctx* c) { return Active{c->active, c->node_inst, c->beam_inst, c->blk_inst}; }
+Segs segs(eb_ctx* c) { return Segs{c->seg_lo, c->seg_hi, c->seg_inst, c->seg_cptr, c->nchunks, (int32_t)c->S}; }
 HHTc hht_consts(double dt, double b, double g) {
     HHTc h;
     h.c_w = g / (b * dt);
@@ -229,7 +243,7 @@ int unalias(eb_ctx* ctx) {
 template <typename P> int launch_beams(eb_ctx* ctx) {
     prof_begin(ctx, EB_KERNEL_BEAM);
     hipLaunchKernelGGL(beam_kernel<P>, dim3(nblk(ctx->E, BEAM_TILE)), dim3(4 * BEAM_TILE), 0, ctx->stream, node_state(ctx),
-                       beam_consts(ctx), beam_stage<P>(ctx), ctx->gauss);
+                       beam_consts(ctx), beam_stage<P>(ctx), ctx->gauss, ctx->active, ctx->beam_inst);
     prof_end(ctx, EB_KERNEL_BEAM);
     ctx->counters[3]++;
     return check_launch(ctx, "beam_kernel");
@@ -238,14 +252,14 @@ template <typename P> int launch_beams(eb_ctx* ctx) {
 template <typename P> int launch_contact(eb_ctx* ctx, double alpha) {
     ContactArgs<P> ca;
     ca.surface = ctx->surface;
-    ca.sphere = ctx->sphere;
-    ca.sdf.coef = (const P*)ctx->sdf_coef;
-    for (int a = 0; a < 3; ++a) { ca.sdf.n[a] = ctx->sdf_n[a]; ca.sdf.d[a] = ctx->sdf_d[a]; ca.sdf.x0[a] = ctx->sdf_x0[a]; }
-    for (int a = 0; a < 6; ++a) ca.sdf.dom[a] = ctx->sdf_dom[a];
+    ca.sphere = ctx->d_sphere;
+    ca.sdf = (const SDFDev<P>*)ctx->d_sdf;
+    ca.beam_inst = ctx->beam_inst;
+    ca.active = ctx->active;
     ca.kn = ctx->kn; ca.mu = ctx->mu; ca.eps_t = ctx->eps_t; ca.eta_n = ctx->eta_n; ca.alpha = alpha;
     ca.beams = ctx->cbeams;
     ca.n = ctx->n_contact;
-    ca.Tc = (P*)ctx->sTc;
+    ca.Tc = (double*)ctx->sTc;
     ca.energy = ctx->senergy;
     hipLaunchKernelGGL(contact_reset, dim3(1), dim3(1), 0, ctx->stream, ctx->clist_n);
     hipLaunchKernelGGL(contact_detect<P>, dim3(nblk(ctx->n_contact, 256)), dim3(256), 0, ctx->stream, node_state(ctx),
@@ -264,16 +278,16 @@ template <typename P> int launch_gathers(eb_ctx* ctx, bool with_contact, bool ta
     prof_begin(ctx, EB_KERNEL_GATHER);
     hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz, ctx->blk_ptr,
                        ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM, ctx->K, ctx->C, ctx->M,
-                       ctx->Ktan, c1, c2, c3, tangent ? 1 : 0);
+                       ctx->Ktan, c1, c2, c3, tangent ? 1 : 0, active_dev(ctx));
     EB_TRY(check_launch(ctx, "gather_matrices"));
     hipLaunchKernelGGL(gather_vectors<P>, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs,
-                       ctx->inc_ptr, ctx->inc_src, (const P*)ctx->sTint, (const P*)ctx->sTk, (const P*)ctx->sTc,
+                       ctx->inc_ptr, ctx->inc_src, (const double*)ctx->sTint, (const double*)ctx->sTk, (const double*)ctx->sTc,
                        with_contact ? 1 : 0, ctx->Tint, ctx->Tk, ctx->Tc, residual ? 1 : 0, a, ctx->fext, ctx->fext_n,
-                       ctx->Tc_n, ctx->Tint_n, ctx->res);
+                       ctx->Tc_n, ctx->Tint_n, ctx->res, active_dev(ctx));
     EB_TRY(check_launch(ctx, "gather_vectors"));
     int grid = red_grid(ctx->E);
     hipLaunchKernelGGL(energy_partials, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, ctx->E, ctx->senergy,
-                       ctx->partials, energy_mask);
+                       ctx->partials, energy_mask, active_dev(ctx));
     hipLaunchKernelGGL(energy_finalize, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->energy);
     prof_end(ctx, EB_KERNEL_GATHER);
     ctx->np1_alias = false;
@@ -296,7 +310,7 @@ int assemble_all(eb_ctx* ctx, bool contact, double alpha, bool fused, double dt,
 int bc_lift_launch(eb_ctx* ctx) {
     if (ctx->n_fixed == 0) return EB_OK;
     hipLaunchKernelGGL(bc_lift, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs, ctx->bptr, ctx->bcol,
-                       ctx->fixidx, ctx->imposed, ctx->D, ctx->Ktan, ctx->dD, ctx->res);
+                       ctx->fixidx, ctx->imposed, ctx->D, ctx->Ktan, ctx->dD, ctx->res, active_dev(ctx));
     return check_launch(ctx, "bc_lift");
 }
 
@@ -314,93 +328,107 @@ int precond_factor(eb_ctx* ctx) {
 int precond_apply(eb_ctx* ctx, const double* b, double* x) {
     int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
     hipLaunchKernelGGL(chain_solve, dim3((unsigned)nch), dim3(CHAIN_THREADS), 0, ctx->stream, chain_dev(ctx), b, x,
-                       &ctx->kctl->done);
+                       ctx->kctl);
     return check_launch(ctx, "chain_solve");
 }
 
 int krylov_iteration(eb_ctx* ctx) {
-    const int64_t n = ctx->ndofs;
-    const int grid = red_grid(n);
-    int* done = &ctx->kctl->done;
+    const Segs sg = segs(ctx);
+    const unsigned nc = (unsigned)ctx->nchunks;
     prof_begin(ctx, EB_KERNEL_KRYLOV);
-    hipLaunchKernelGGL(krylov_p, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, ctx->kctl, ctx->kr, ctx->kp, ctx->kv);
+    hipLaunchKernelGGL(krylov_p, dim3(nc), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->kctl, ctx->kr, ctx->kp, ctx->kv);
     EB_TRY(precond_apply(ctx, ctx->kp, ctx->ky));
-    hipLaunchKernelGGL(spmv_masked<1>, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, n, ctx->bptr, ctx->bcol,
-                       ctx->is_fixed, ctx->Ktan, ctx->ky, ctx->kv, ctx->krhat, ctx->partials, done);
-    hipLaunchKernelGGL(krylov_alpha_fin, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->kctl);
-    hipLaunchKernelGGL(krylov_s, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, n, ctx->kctl, ctx->kr, ctx->kv, ctx->ky,
+    hipLaunchKernelGGL(spmv_masked<1>, dim3(nc), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->bptr, ctx->bcol,
+                       ctx->is_fixed, ctx->Ktan, ctx->ky, ctx->kv, ctx->krhat, ctx->partials, ctx->kctl);
+    hipLaunchKernelGGL(krylov_alpha_fin, dim3(1), dim3(FIN_BLOCK), 0, ctx->stream, ctx->partials, sg, ctx->kctl);
+    hipLaunchKernelGGL(krylov_s, dim3(nc), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->kctl, ctx->kr, ctx->kv, ctx->ky,
                        ctx->ks, ctx->kx, ctx->partials);
-    hipLaunchKernelGGL(krylov_s_fin, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->kctl);
+    hipLaunchKernelGGL(krylov_s_fin, dim3(1), dim3(FIN_BLOCK), 0, ctx->stream, ctx->partials, sg, ctx->kctl);
     EB_TRY(precond_apply(ctx, ctx->ks, ctx->kz));
-    hipLaunchKernelGGL(spmv_masked<2>, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, n, ctx->bptr, ctx->bcol,
-                       ctx->is_fixed, ctx->Ktan, ctx->kz, ctx->kt, ctx->ks, ctx->partials, done);
-    hipLaunchKernelGGL(krylov_omega_fin, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->kctl);
-    hipLaunchKernelGGL(krylov_x, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, n, ctx->kctl, ctx->kz, ctx->ks, ctx->kt,
+    hipLaunchKernelGGL(spmv_masked<2>, dim3(nc), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->bptr, ctx->bcol,
+                       ctx->is_fixed, ctx->Ktan, ctx->kz, ctx->kt, ctx->ks, ctx->partials, ctx->kctl);
+    hipLaunchKernelGGL(krylov_omega_fin, dim3(1), dim3(FIN_BLOCK), 0, ctx->stream, ctx->partials, sg, ctx->kctl);
+    hipLaunchKernelGGL(krylov_x, dim3(nc), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->kctl, ctx->kz, ctx->ks, ctx->kt,
                        ctx->krhat, ctx->kx, ctx->kr, ctx->partials);
-    hipLaunchKernelGGL(krylov_rho_fin, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->kctl);
+    hipLaunchKernelGGL(krylov_rho_fin, dim3(1), dim3(FIN_BLOCK), 0, ctx->stream, ctx->partials, sg, ctx->kctl, ctx->kall);
     prof_end(ctx, EB_KERNEL_KRYLOV);
     return check_launch(ctx, "krylov_iteration");
 }
 
-// Ktan (block-row order) and res must be current; solves the masked free-dof system, writes ΔD[free]
+// Ktan (block-row order) and res must be current; solves the masked free-dof system of every
+// instance active in the Newton iteration, writes their ΔD[free]
 int solve_free(eb_ctx* ctx, const eb_newton_params* p, int* its_out) {
-    const int64_t n = ctx->ndofs;
-    const int grid = red_grid(n);
+    const Segs sg = segs(ctx);
     const int maxit = (p && p->krylov_maxit > 0) ? p->krylov_maxit : 200;
     const double tol = (p && p->krylov_tol > 0) ? p->krylov_tol : 1e-13;
     EB_HIP(hipMemsetAsync(ctx->kstatus, 0, sizeof(int), ctx->stream));
     EB_TRY(precond_factor(ctx));
-    hipLaunchKernelGGL(krylov_init, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, n, ctx->is_fixed, ctx->res, ctx->kx,
-                       ctx->kr, ctx->krhat, ctx->kp, ctx->kv, ctx->partials);
-    hipLaunchKernelGGL(krylov_init_fin, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->kctl, tol);
+    hipLaunchKernelGGL(krylov_init, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->is_fixed,
+                       ctx->res, ctx->kx, ctx->kr, ctx->krhat, ctx->kp, ctx->kv, ctx->partials);
+    hipLaunchKernelGGL(krylov_init_fin, dim3(1), dim3(FIN_BLOCK), 0, ctx->stream, ctx->partials, sg, ctx->kctl, tol,
+                       ctx->active, ctx->kall);
     EB_TRY(check_launch(ctx, "krylov_init"));
     int launched = 0, batch = 1;
     while (launched < maxit) {
         int m = std::min(batch, maxit - launched);
         for (int k = 0; k < m; ++k) EB_TRY(krylov_iteration(ctx));
         launched += m;
-        EB_HIP(hipMemcpyAsync(ctx->h_kctl, ctx->kctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, ctx->stream));
+        EB_HIP(hipMemcpyAsync(ctx->h_kall, ctx->kall, sizeof(KrylovAll), hipMemcpyDeviceToHost, ctx->stream));
         EB_HIP(hipStreamSynchronize(ctx->stream));
-        if (ctx->h_kctl->done) break;
+        if (ctx->h_kall->all_done) break;
         batch = std::min(batch * 2, 16);
     }
+    EB_HIP(hipMemcpyAsync(ctx->h_kctl, ctx->kctl, sizeof(KrylovCtl) * ctx->S, hipMemcpyDeviceToHost, ctx->stream));
     EB_HIP(hipMemcpyAsync(ctx->h_status, ctx->kstatus, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     EB_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->counters[1] += ctx->h_kctl->iters;
+    int its = 0;
+    for (int64_t s = 0; s < ctx->S; ++s) its = std::max(its, ctx->h_kctl[s].iters);
+    ctx->counters[1] += its;
     ctx->counters[2] += 1;
-    if (its_out) *its_out = ctx->h_kctl->iters;
+    if (its_out) *its_out = its;
     if (*ctx->h_status) { ctx->err = "singular 6x6 pivot block in the chain preconditioner"; return EB_ERR_SOLVE; }
-    const KrylovCtl& k = *ctx->h_kctl;
-    bool ok = k.done && (k.bnorm2 == 0.0 || k.rr2 <= k.tol2 * k.bnorm2 * 1.0000001 || k.ss2 <= k.tol2 * k.bnorm2);
-    if (!ok) {
+    for (int64_t s = 0; s < ctx->S; ++s) {
+        const KrylovCtl& k = ctx->h_kctl[s];
+        bool ok = k.done && (k.bnorm2 == 0.0 || k.rr2 <= k.tol2 * k.bnorm2 * 1.0000001 || k.ss2 <= k.tol2 * k.bnorm2);
+        if (ok || k.iters == 0) continue;   // iters == 0 and done: zero right-hand side or frozen instance
         // accept a solve that stalled at rounding level; fail otherwise
         double rel = k.bnorm2 > 0 ? std::sqrt(std::min(k.rr2, k.ss2 > 0 ? k.ss2 : k.rr2) / k.bnorm2) : 0.0;
         if (!(rel < 1e-6) || !std::isfinite(rel)) {
-            char buf[160];
-            snprintf(buf, sizeof buf, "BiCGStab did not converge: %d iterations, relative residual %.3e%s", k.iters, rel,
-                     k.breakdown ? " (breakdown)" : "");
+            char buf[200];
+            snprintf(buf, sizeof buf, "BiCGStab did not converge (instance %lld): %d iterations, relative residual %.3e%s",
+                     (long long)s, k.iters, rel, k.breakdown ? " (breakdown)" : "");
             ctx->err = buf;
             return EB_ERR_SOLVE;
         }
     }
-    hipLaunchKernelGGL(scatter_solution, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, ctx->is_fixed, ctx->kx, ctx->dD);
+    hipLaunchKernelGGL(scatter_solution, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs,
+                       ctx->is_fixed, ctx->kx, ctx->dD, active_dev(ctx));
     return check_launch(ctx, "scatter_solution");
 }
 
-int norms(eb_ctx* ctx, double* rn, double* dn) {
-    const int grid = red_grid(ctx->ndofs);
+// per-instance norms; in a Newton iteration (nc.newton) also the convergence bookkeeping. Returns
+// the number of instances still iterating (Newton) through *n_active.
+int norms_launch(eb_ctx* ctx, const NewtonCtl& nc, int* n_active) {
+    const Segs sg = segs(ctx);
     prof_begin(ctx, EB_KERNEL_UPDATE);
-    hipLaunchKernelGGL(norms_partials, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, ctx->ndofs, ctx->is_fixed, ctx->dD,
-                       ctx->res, ctx->fext, ctx->Tc, ctx->Tk, ctx->partials);
-    hipLaunchKernelGGL(norms_finalize, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->norms);
+    hipLaunchKernelGGL(norms_partials, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->is_fixed,
+                       ctx->dD, ctx->res, ctx->fext, ctx->Tc, ctx->Tk, ctx->partials);
+    hipLaunchKernelGGL(norms_finalize, dim3(1), dim3(FIN_BLOCK), 0, ctx->stream, ctx->partials, sg, ctx->norms, nc,
+                       ctx->active, ctx->kiter, ctx->nall);
     prof_end(ctx, EB_KERNEL_UPDATE);
     EB_TRY(check_launch(ctx, "norms"));
-    EB_HIP(hipMemcpyAsync(ctx->h_norms, ctx->norms, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    EB_HIP(hipMemcpyAsync(ctx->h_nall, ctx->nall, sizeof(NewtonAll), hipMemcpyDeviceToHost, ctx->stream));
+    EB_HIP(hipMemcpyAsync(ctx->h_norms, ctx->norms, 2 * sizeof(double) * ctx->S, hipMemcpyDeviceToHost, ctx->stream));
     EB_HIP(hipStreamSynchronize(ctx->stream));
     if (!ctx->ev_pending.empty()) prof_harvest(ctx);
-    *rn = ctx->h_norms[0];
-    *dn = ctx->h_norms[1];
+    if (n_active) *n_active = ctx->h_nall->n_active;
     return EB_OK;
+}
+
+int newton_reset_launch(eb_ctx* ctx) {
+    hipLaunchKernelGGL(newton_reset, dim3(nblk(ctx->S, 256)), dim3(256), 0, ctx->stream, (int)ctx->S, ctx->active,
+                       ctx->kiter);
+    return check_launch(ctx, "newton_reset");
 }
 
 int field_info(eb_ctx* ctx, int f, double*& ptr, int64_t& len, int& kind) {
@@ -457,17 +485,115 @@ int field_info(eb_ctx* ctx, int f, double*& ptr, int64_t& len, int& kind) {
 }  // namespace
 
 // =============================================================================================
-extern "C" {
+namespace {
 
-int eb_create(const eb_model_desc* d, eb_ctx** out) {
-    if (!d || !out) return EB_ERR_ARG;
+bool desc_ok(const eb_model_desc* d) {
+    if (d->abi_version != EB_ABI_VERSION) return false;
+    if (d->n_nodes <= 0 || d->n_beams <= 0 || !d->X0 || !d->conn || !d->l0 || !d->Re0 || !d->props) return false;
+    if (d->n_gauss <= 0 || d->n_gauss > 8 || !d->gauss_w || !d->gauss_z) return false;
+    if (d->precision != EB_PRECISION_FP64 && d->precision != EB_PRECISION_FP32) return false;
+    if (d->n_fixed < 0 || (d->n_fixed > 0 && !d->fixed_dofs)) return false;
+    return true;
+}
+
+// settings every instance of a batch shares
+bool same_settings(const eb_model_desc* a, const eb_model_desc* b) {
+    if (a->precision != b->precision || a->n_gauss != b->n_gauss || a->surface != b->surface) return false;
+    for (int k = 0; k < a->n_gauss; ++k)
+        if (a->gauss_w[k] != b->gauss_w[k] || a->gauss_z[k] != b->gauss_z[k]) return false;
+    if (a->surface != EB_SURFACE_NONE && (a->kn != b->kn || a->mu != b->mu || a->eps_t != b->eps_t || a->eta_n != b->eta_n))
+        return false;
+    return true;
+}
+
+// one model made of the instances one after the other (node, beam and dof numbers shifted)
+struct Concat {
+    std::vector<double> X0, l0, Re0, props;
+    std::vector<int64_t> conn, fixed, cbeams;
+    eb_model_desc d{};
+};
+
+void concat_descs(const eb_model_desc* ds, int64_t S, Concat& c, std::vector<int64_t>& noff, std::vector<int64_t>& boff) {
+    noff.assign(1, 0);
+    boff.assign(1, 0);
+    for (int64_t s = 0; s < S; ++s) {
+        const eb_model_desc& d = ds[s];
+        const int64_t n0 = noff.back(), b0 = boff.back();
+        c.X0.insert(c.X0.end(), d.X0, d.X0 + 3 * d.n_nodes);
+        for (int64_t k = 0; k < 2 * d.n_beams; ++k) c.conn.push_back(d.conn[k] + n0);
+        c.l0.insert(c.l0.end(), d.l0, d.l0 + d.n_beams);
+        c.Re0.insert(c.Re0.end(), d.Re0, d.Re0 + 9 * d.n_beams);
+        c.props.insert(c.props.end(), d.props, d.props + EB_NPROPS * d.n_beams);
+        for (int64_t k = 0; k < d.n_fixed; ++k) c.fixed.push_back(d.fixed_dofs[k] + 6 * n0);
+        if (d.surface != EB_SURFACE_NONE)
+            for (int64_t k = 0; k < d.n_contact_beams; ++k) c.cbeams.push_back(d.contact_beams[k] + b0);
+        noff.push_back(n0 + d.n_nodes);
+        boff.push_back(b0 + d.n_beams);
+    }
+    c.d = ds[0];
+    c.d.n_nodes = noff.back();
+    c.d.n_beams = boff.back();
+    c.d.X0 = c.X0.data();
+    c.d.conn = c.conn.data();
+    c.d.l0 = c.l0.data();
+    c.d.Re0 = c.Re0.data();
+    c.d.props = c.props.data();
+    c.d.n_fixed = (int64_t)c.fixed.size();
+    c.d.fixed_dofs = c.fixed.data();
+    c.d.n_contact_beams = (int64_t)c.cbeams.size();
+    c.d.contact_beams = c.cbeams.data();
+}
+
+template <typename P> int upload_sdfs(eb_ctx* ctx, const eb_model_desc* ds, int64_t S) {
+    std::vector<SDFDev<P>> h((size_t)S);
+    for (int64_t s = 0; s < S; ++s) {
+        const eb_model_desc* d = ds + s;
+        SDFDev<P>& f = h[s];
+        if (!d->sdf_samples || d->sdf_n[0] < 1 || d->sdf_n[1] < 1 || d->sdf_n[2] < 1) {
+            ctx->err = "instance " + std::to_string(s) + ": signed distance field missing";
+            return EB_ERR_ARG;
+        }
+        for (int a = 0; a < 3; ++a) {
+            f.n[a] = (int32_t)d->sdf_n[a]; f.d[a] = d->sdf_d[a]; f.x0[a] = d->sdf_x0[a];
+            f.dom[2 * a] = d->sdf_x0[a];
+            f.dom[2 * a + 1] = d->sdf_x0[a] + (d->sdf_n[a] - 1) * d->sdf_d[a];
+        }
+        std::vector<double> coef;
+        if (d->sdf_prefiltered) {
+            coef.assign(d->sdf_samples, d->sdf_samples + (d->sdf_n[0] + 2) * (d->sdf_n[1] + 2) * (d->sdf_n[2] + 2));
+        } else {
+            ebh::prefilter_sdf(d->sdf_n, d->sdf_samples, coef);
+        }
+        std::vector<P> cp(coef.begin(), coef.end());
+        P* dp;
+        EB_TRY(upload_vec(ctx, dp, cp));
+        f.coef = dp;
+    }
+    SDFDev<P>* dptr;
+    EB_TRY(upload_vec(ctx, dptr, h));
+    ctx->d_sdf = dptr;
+    return EB_OK;
+}
+
+int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
+    if (!ds || !out || S < 1) return EB_ERR_ARG;
     *out = nullptr;
-    if (d->abi_version != EB_ABI_VERSION) return EB_ERR_ARG;
-    if (d->n_nodes <= 0 || d->n_beams <= 0 || !d->X0 || !d->conn || !d->l0 || !d->Re0 || !d->props) return EB_ERR_ARG;
-    if (d->n_gauss <= 0 || d->n_gauss > 8 || !d->gauss_w || !d->gauss_z) return EB_ERR_ARG;
-    if (d->precision != EB_PRECISION_FP64 && d->precision != EB_PRECISION_FP32) return EB_ERR_ARG;
+    for (int64_t s = 0; s < S; ++s) {
+        if (!desc_ok(ds + s) || !same_settings(ds, ds + s)) {
+            fprintf(stderr, "eb_create: invalid descriptor or settings that differ from instance 0 (instance %lld)\n",
+                    (long long)s);
+            return EB_ERR_ARG;
+        }
+    }
+    Concat cc;
+    std::vector<int64_t> noff, boff;
+    concat_descs(ds, S, cc, noff, boff);
+    const eb_model_desc* d = &cc.d;
     eb_ctx* ctx = new (std::nothrow) eb_ctx();
     if (!ctx) return EB_ERR_ARG;
+    ctx->S = S;
+    ctx->node_off = noff;
+    ctx->beam_off = boff;
     auto bail = [&](int code) { eb_destroy(ctx); return code; };
     if (hipGetDevice(&ctx->device) != hipSuccess) return bail(EB_ERR_DEVICE);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return bail(EB_ERR_DEVICE);
@@ -516,9 +642,9 @@ int eb_create(const eb_model_desc* d, eb_ctx** out) {
         CK(dalloc(ctx, tmp, (int64_t)((144 * E * esz + 7) / 8))); ctx->sK = tmp;
         CK(dalloc(ctx, tmp, (int64_t)((144 * E * esz + 7) / 8))); ctx->sC = tmp;
         CK(dalloc(ctx, tmp, (int64_t)((144 * E * esz + 7) / 8))); ctx->sM = tmp;
-        CK(dalloc(ctx, tmp, (int64_t)((12 * E * esz + 7) / 8))); ctx->sTint = tmp;
-        CK(dalloc(ctx, tmp, (int64_t)((12 * E * esz + 7) / 8))); ctx->sTk = tmp;
-        CK(dalloc(ctx, tmp, (int64_t)((12 * E * esz + 7) / 8))); ctx->sTc = tmp;
+        CK(dalloc(ctx, tmp, 12 * E)); ctx->sTint = tmp;   // forces are FP64 in both forms
+        CK(dalloc(ctx, tmp, 12 * E)); ctx->sTk = tmp;
+        CK(dalloc(ctx, tmp, 12 * E)); ctx->sTc = tmp;
     }
     CK(dalloc(ctx, ctx->senergy, 3 * E));
     CK(upload_vec(ctx, ctx->bptr, narrow<int32_t>(P.bptr)));
@@ -539,13 +665,51 @@ int eb_create(const eb_model_desc* d, eb_ctx** out) {
         CK(dalloc(ctx, *v, ctx->ndofs));
     CK(dalloc(ctx, ctx->imposed, std::max<int64_t>(ctx->n_fixed, 1)));
     CK(dalloc(ctx, ctx->energy, 3));
-    CK(dalloc(ctx, ctx->partials, 8 * RED_MAX_GRID));
-    CK(dalloc(ctx, ctx->kctl, 1));
+    // instance tables and per-instance reduction chunks
+    {
+        std::vector<int32_t> ni((size_t)N), bi((size_t)E), ki;
+        for (int64_t s = 0; s < S; ++s) {
+            for (int64_t n = noff[s]; n < noff[s + 1]; ++n) ni[n] = (int32_t)s;
+            for (int64_t e = boff[s]; e < boff[s + 1]; ++e) bi[e] = (int32_t)s;
+        }
+        ki.resize((size_t)(ctx->nnz / 36));
+        for (int64_t n = 0; n < N; ++n)
+            for (int64_t b = P.bptr[n]; b < P.bptr[n + 1]; ++b) ki[b] = ni[n];
+        std::vector<int64_t> lo, hi;
+        std::vector<int32_t> ci, cp;
+        for (int64_t s = 0; s < S; ++s) {
+            cp.push_back((int32_t)lo.size());
+            for (int64_t c0 = 6 * noff[s]; c0 < 6 * noff[s + 1]; c0 += SEG_CHUNK) {
+                lo.push_back(c0);
+                hi.push_back(std::min<int64_t>(c0 + SEG_CHUNK, 6 * noff[s + 1]));
+                ci.push_back((int32_t)s);
+            }
+        }
+        cp.push_back((int32_t)lo.size());
+        ctx->nchunks = (int32_t)lo.size();
+        CK(upload_vec(ctx, ctx->node_inst, ni));
+        CK(upload_vec(ctx, ctx->beam_inst, bi));
+        CK(upload_vec(ctx, ctx->blk_inst, ki));
+        CK(upload_vec(ctx, ctx->seg_lo, lo));
+        CK(upload_vec(ctx, ctx->seg_hi, hi));
+        CK(upload_vec(ctx, ctx->seg_inst, ci));
+        CK(upload_vec(ctx, ctx->seg_cptr, cp));
+        std::vector<uint8_t> on((size_t)S, 1);
+        CK(upload_vec(ctx, ctx->active, on));
+        CK(dalloc(ctx, ctx->kiter, S));
+        CK(dalloc(ctx, ctx->nall, 1));
+        CK(dalloc(ctx, ctx->kall, 1));
+    }
+    CK(dalloc(ctx, ctx->partials, std::max<int64_t>(8 * (int64_t)ctx->nchunks, 8 * RED_MAX_GRID)));
+    CK(dalloc(ctx, ctx->kctl, S));
     CK(dalloc(ctx, ctx->kstatus, 1));
-    CK(dalloc(ctx, ctx->norms, 2));
-    if (hipHostMalloc((void**)&ctx->h_kctl, sizeof(KrylovCtl), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&ctx->h_norms, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&ctx->h_status, sizeof(int), hipHostMallocDefault) != hipSuccess)
+    CK(dalloc(ctx, ctx->norms, 2 * S));
+    if (hipHostMalloc((void**)&ctx->h_kctl, sizeof(KrylovCtl) * S, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_norms, 2 * sizeof(double) * S, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_status, sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_kall, sizeof(KrylovAll), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_nall, sizeof(NewtonAll), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_kiter, sizeof(int32_t) * S, hipHostMallocDefault) != hipSuccess)
         return bail(EB_ERR_DEVICE);
     // contact
     ctx->surface = d->surface;
@@ -565,37 +729,28 @@ int eb_create(const eb_model_desc* d, eb_ctx** out) {
         CK(dalloc(ctx, ctx->clist, ctx->n_contact));
         CK(dalloc(ctx, ctx->clist_n, 1));
         if (d->surface == EB_SURFACE_SPHERE) {
-            for (int a = 0; a < 3; ++a) ctx->sphere.c[a] = d->sphere_center[a];
-            ctx->sphere.R = d->sphere_radius;
+            std::vector<SphereDev> sp((size_t)S);
+            for (int64_t s = 0; s < S; ++s) {
+                for (int a = 0; a < 3; ++a) sp[s].c[a] = ds[s].sphere_center[a];
+                sp[s].R = ds[s].sphere_radius;
+            }
+            CK(upload_vec(ctx, ctx->d_sphere, sp));
+        } else if (ctx->precision == EB_PRECISION_FP32) {
+            CK(upload_sdfs<float>(ctx, ds, S));
         } else {
-            if (!d->sdf_samples || d->sdf_n[0] < 1 || d->sdf_n[1] < 1 || d->sdf_n[2] < 1) return bail(EB_ERR_ARG);
-            for (int a = 0; a < 3; ++a) {
-                ctx->sdf_n[a] = (int32_t)d->sdf_n[a]; ctx->sdf_d[a] = d->sdf_d[a]; ctx->sdf_x0[a] = d->sdf_x0[a];
-                ctx->sdf_dom[2 * a] = d->sdf_x0[a];
-                ctx->sdf_dom[2 * a + 1] = d->sdf_x0[a] + (d->sdf_n[a] - 1) * d->sdf_d[a];
-            }
-            std::vector<double> coef;
-            if (d->sdf_prefiltered) {
-                coef.assign(d->sdf_samples, d->sdf_samples + (d->sdf_n[0] + 2) * (d->sdf_n[1] + 2) * (d->sdf_n[2] + 2));
-            } else {
-                ebh::prefilter_sdf(d->sdf_n, d->sdf_samples, coef);
-            }
-            if (ctx->precision == EB_PRECISION_FP32) {
-                std::vector<float> cf(coef.begin(), coef.end());
-                float* p;
-                CK(upload_vec(ctx, p, cf));
-                ctx->sdf_coef = p;
-            } else {
-                double* p;
-                CK(upload_vec(ctx, p, coef));
-                ctx->sdf_coef = p;
-            }
+            CK(upload_sdfs<double>(ctx, ds, S));
         }
     }
-    // chains for the preconditioner
+    // chains for the preconditioner (a chain never leaves its instance: instances share no node)
     ebh::build_chains(ctx->chains, P, d->X0, 0);
     {
         const ebh::Chains& c = ctx->chains;
+        std::vector<int32_t> chi(c.ptr.size() - 1);
+        for (size_t k = 0; k + 1 < c.ptr.size(); ++k) {
+            const int64_t n0 = c.node[c.ptr[k]];
+            chi[k] = (int32_t)(std::upper_bound(noff.begin(), noff.end(), n0) - noff.begin() - 1);
+        }
+        CK(upload_vec(ctx, ctx->chain_inst, chi));
         CK(upload_vec(ctx, ctx->ch_ptr, narrow<int32_t>(c.ptr)));
         CK(upload_vec(ctx, ctx->ch_node, c.node));
         CK(upload_vec(ctx, ctx->ch_diag, c.diag_blk));
@@ -620,6 +775,17 @@ int eb_create(const eb_model_desc* d, eb_ctx** out) {
     return EB_OK;
 }
 
+
+}  // namespace
+
+extern "C" {
+
+int eb_create(const eb_model_desc* d, eb_ctx** out) { return create_impl(d, 1, out); }
+
+int eb_create_batch(const eb_model_desc* descs, int64_t n_instances, eb_ctx** out) {
+    return create_impl(descs, n_instances, out);
+}
+
 int eb_destroy(eb_ctx* ctx) {
     if (!ctx) return EB_OK;
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
@@ -629,6 +795,9 @@ int eb_destroy(eb_ctx* ctx) {
     if (ctx->h_kctl) hipHostFree(ctx->h_kctl);
     if (ctx->h_norms) hipHostFree(ctx->h_norms);
     if (ctx->h_status) hipHostFree(ctx->h_status);
+    if (ctx->h_kall) hipHostFree(ctx->h_kall);
+    if (ctx->h_nall) hipHostFree(ctx->h_nall);
+    if (ctx->h_kiter) hipHostFree(ctx->h_kiter);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return EB_OK;
@@ -640,6 +809,26 @@ int eb_get_sizes(const eb_ctx* ctx, eb_sizes* s) {
     if (!ctx || !s) return EB_ERR_ARG;
     s->n_nodes = ctx->N; s->n_beams = ctx->E; s->ndofs = ctx->ndofs; s->nnz = ctx->nnz;
     s->n_free = ctx->n_free; s->n_fixed = ctx->n_fixed; s->nnz_free = ctx->nnz_free;
+    return EB_OK;
+}
+
+int eb_get_instances(const eb_ctx* ctx, int64_t* n_instances, int64_t* node_offsets, int64_t* beam_offsets) {
+    if (!ctx) return EB_ERR_ARG;
+    if (n_instances) *n_instances = ctx->S;
+    if (node_offsets) for (int64_t s = 0; s <= ctx->S; ++s) node_offsets[s] = ctx->node_off[s];
+    if (beam_offsets) for (int64_t s = 0; s <= ctx->S; ++s) beam_offsets[s] = ctx->beam_off[s];
+    return EB_OK;
+}
+
+int eb_get_instance_iterations(const eb_ctx* ctx, int32_t* k_out) {
+    if (!ctx || !k_out) return EB_ERR_ARG;
+    for (int64_t s = 0; s < ctx->S; ++s) k_out[s] = ctx->h_kiter[s];
+    return EB_OK;
+}
+
+int eb_get_instance_norms(const eb_ctx* ctx, double* rn_dn) {
+    if (!ctx || !rn_dn) return EB_ERR_ARG;
+    for (int64_t s = 0; s < 2 * ctx->S; ++s) rn_dn[s] = ctx->h_norms[s];
     return EB_OK;
 }
 
@@ -783,12 +972,12 @@ int eb_initialize_global_predictor(eb_ctx* ctx) {
 
 static int global_corrector_async(eb_ctx* ctx, double dt, double b, double g) {
     hipLaunchKernelGGL(global_corrector, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs,
-                       g / (b * dt), 1 / (b * (dt * dt)), ctx->D, ctx->Dd, ctx->Ddd, ctx->dD);
+                       g / (b * dt), 1 / (b * (dt * dt)), ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, active_dev(ctx));
     return check_launch(ctx, "global_corrector");
 }
 static int local_corrector_async(eb_ctx* ctx, double dt, double b, double g) {
     hipLaunchKernelGGL(local_corrector, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx),
-                       ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, hht_consts(dt, b, g));
+                       ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, hht_consts(dt, b, g), active_dev(ctx));
     return check_launch(ctx, "local_corrector");
 }
 static int global_predictor_async(eb_ctx* ctx, double dt, double b, double g) {
@@ -829,7 +1018,17 @@ int eb_compute_local_predictor(eb_ctx* ctx, double dt, double b, double g) {
 
 int eb_compute_norms_corrector(eb_ctx* ctx, double* rn, double* dn) {
     if (!ctx || !rn || !dn) return EB_ERR_ARG;
-    return norms(ctx, rn, dn);
+    NewtonCtl nc{0, 0, 0, 0, 0};
+    EB_TRY(norms_launch(ctx, nc, nullptr));
+    // the largest over the instances (the one instance's norms for a single structure)
+    *rn = 0; *dn = 0;
+    for (int64_t s = 0; s < ctx->S; ++s) {
+        *rn = std::max(*rn, ctx->h_norms[2 * s]);
+        *dn = std::max(*dn, ctx->h_norms[2 * s + 1]);
+        if (std::isnan(ctx->h_norms[2 * s])) *rn = ctx->h_norms[2 * s];
+        if (std::isnan(ctx->h_norms[2 * s + 1])) *dn = ctx->h_norms[2 * s + 1];
+    }
+    return EB_OK;
 }
 
 int eb_update_loads(eb_ctx* ctx, int64_t n, const int64_t* dofs, const double* values) {
@@ -890,19 +1089,34 @@ int eb_corrector(eb_ctx* ctx, const eb_newton_params* p, double dt, int* kout) {
     hipLaunchKernelGGL(init_global, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx), ctx->D,
                        ctx->Dd, ctx->Ddd, ctx->dD, 0);
     EB_TRY(check_launch(ctx, "init_global"));
-    int k = 1;
-    double rn = INFINITY, dn = INFINITY;
-    while ((rn > p->tol_res || dn > p->tol_disp) && k <= p->max_iterations) {
-        EB_TRY(assemble_all(ctx, true, p->alpha, true, dt, p->beta, p->gamma));
-        EB_TRY(bc_lift_launch(ctx));
-        EB_TRY(solve_free(ctx, p, nullptr));
-        EB_TRY(global_corrector_async(ctx, dt, p->beta, p->gamma));
-        EB_TRY(local_corrector_async(ctx, dt, p->beta, p->gamma));
-        EB_TRY(norms(ctx, &rn, &dn));
+    EB_TRY(newton_reset_launch(ctx));
+    // corrector.jl:17-36 per instance: an instance leaves the loop when its norms pass or its
+    // iterations are used up; the others keep iterating with it frozen
+    int k = 1, n_active = (int)ctx->S;
+    int st = EB_OK;
+    while (n_active > 0 && k <= p->max_iterations) {
+        st = assemble_all(ctx, true, p->alpha, true, dt, p->beta, p->gamma);
+        if (st == EB_OK) st = bc_lift_launch(ctx);
+        if (st == EB_OK) st = solve_free(ctx, p, nullptr);
+        if (st == EB_OK) st = global_corrector_async(ctx, dt, p->beta, p->gamma);
+        if (st == EB_OK) st = local_corrector_async(ctx, dt, p->beta, p->gamma);
+        NewtonCtl nc{p->tol_res, p->tol_disp, k, p->max_iterations, 1};
+        if (st == EB_OK) st = norms_launch(ctx, nc, &n_active);
+        if (st != EB_OK) break;
         ctx->counters[0]++;
         k += 1;
     }
-    *kout = k;
+    if (st == EB_OK) {
+        EB_HIP(hipMemcpyAsync(ctx->h_kiter, ctx->kiter, sizeof(int32_t) * ctx->S, hipMemcpyDeviceToHost, ctx->stream));
+        EB_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    // every instance takes part in whatever runs outside the corrector loop
+    int st2 = newton_reset_launch(ctx);
+    if (st != EB_OK) return st;
+    EB_TRY(st2);
+    int kmax = 0;
+    for (int64_t s = 0; s < ctx->S; ++s) kmax = std::max(kmax, ctx->h_kiter[s] > 0 ? ctx->h_kiter[s] : k);
+    *kout = kmax;
     return EB_OK;
 }
 

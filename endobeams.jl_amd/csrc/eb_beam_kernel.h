@@ -23,9 +23,9 @@ struct BeamConsts {  // component-major element constants
     int64_t E;
 };
 
-template <typename P> struct BeamStage {  // element-major outputs: beam e owns a contiguous record
-    P *K, *C, *M;        // [E][144], 12x12 column-major (local dof order u₁ θ₁ u₂ θ₂)
-    P *Tint, *Tk;        // [E][12]
+template <typename S> struct BeamStage {  // element-major outputs: beam e owns a contiguous record
+    S *K, *C, *M;        // [E][144], 12x12 column-major (local dof order u₁ θ₁ u₂ θ₂), storage type S
+    double *Tint, *Tk;   // [E][12]: forces are always FP64 (they make the residual)
     double* energy;      // [E][3]: strain, kinetic, contact
 };
 
@@ -34,14 +34,14 @@ struct GaussPts { int n; double w[8], z[8]; };
 EBD V3<double> ldv(const double* a, int64_t i) { return {a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
 EBD M3<double> ldm(const double* a, int64_t i) { M3<double> m; for (int k = 0; k < 9; ++k) m.a[k] = a[9 * i + k]; return m; }
 
-template <typename P> EBD void st_block(P* base, int64_t E, int64_t e, int bi, int bj, const M3<P>& B) {
+template <typename S, typename P> EBD void st_block(S* base, int64_t E, int64_t e, int bi, int bj, const M3<P>& B) {
 #pragma unroll
     for (int j = 0; j < 3; ++j)
 #pragma unroll
-        for (int i = 0; i < 3; ++i) base[e * 144 + (3 * bj + j) * 12 + 3 * bi + i] = B.a[i + 3 * j];
+        for (int i = 0; i < 3; ++i) base[e * 144 + (3 * bj + j) * 12 + 3 * bi + i] = S(B.a[i + 3 * j]);
 }
-template <typename P> EBD void st_vec(P* base, int64_t E, int64_t e, int b, V3<P> v) {
-    base[e * 12 + 3 * b] = v.x; base[e * 12 + 3 * b + 1] = v.y; base[e * 12 + 3 * b + 2] = v.z;
+template <typename P> EBD void st_vec(double* base, int64_t E, int64_t e, int b, V3<P> v) {
+    base[e * 12 + 3 * b] = double(v.x); base[e * 12 + 3 * b + 1] = double(v.y); base[e * 12 + 3 * b + 2] = double(v.z);
 }
 
 struct ShapeF { double N1, N2, N3, N4, N5, N6, N7, N8; };
@@ -114,10 +114,10 @@ EBD StaticOut beam_static(int64_t e, const NodeState& ns, const BeamConsts& bc, 
     V3<double> T1 = (B1 * Tu + tmul(B11, TT1)) + tmul(B21, TT2);
     V3<double> T2 = tmul(B12, TT1) + tmul(B22, TT2);
     V3<double> T4 = tmul(B14, TT1) + tmul(B24, TT2);
-    st_vec<P>(st.Tint, E, e, 0, cvt<P>(T1));
-    st_vec<P>(st.Tint, E, e, 1, cvt<P>(T2));
-    st_vec<P>(st.Tint, E, e, 2, cvt<P>(-T1));
-    st_vec<P>(st.Tint, E, e, 3, cvt<P>(T4));
+    st_vec(st.Tint, E, e, 0, T1);
+    st_vec(st.Tint, E, e, 1, T2);
+    st_vec(st.Tint, E, e, 2, -T1);
+    st_vec(st.Tint, E, e, 3, T4);
 
     // Kₘ, K̃, BᵀKB
     V3<double> Mp1 = tmul(Ti1, TT1), Mp2 = tmul(Ti2, TT2);
@@ -211,8 +211,8 @@ enum { PART_STATIC = 0, PART_MASS = 1, PART_C13 = 2, PART_C24 = 3 };
 template <typename P> struct MassBlocks { M3<P> M11, M12, M13, M14, M22, M23, M24, M33, M34, M44; };
 
 // Dynamic part (inertia, gyroscopic and damping forces, M and Cᵏ), compute_beams.jl:161-517.
-template <typename P, int PART>
-EBD void beam_dynamic(int64_t e, const NodeState& ns, const BeamConsts& bc, const BeamStage<P>& st,
+template <typename P, typename S, int PART>
+EBD void beam_dynamic(int64_t e, const NodeState& ns, const BeamConsts& bc, const BeamStage<S>& st,
                       const GaussPts& gp, const StaticOut& so, MassBlocks<P>& mb) {
     constexpr bool MASS = PART == PART_MASS, CA = PART == PART_C13, CB = PART == PART_C24;
     const int64_t E = bc.E;
@@ -370,46 +370,46 @@ EBD void beam_dynamic(int64_t e, const NodeState& ns, const BeamConsts& bc, cons
     const M3<P> TT2t = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, nb)))));
     if constexpr (MASS) {
         st.energy[3 * e + 1] = double(l02 * kinetic);
-        st_vec<P>(st.Tk, E, e, 0, l02Re * Tk1);
-        st_vec<P>(st.Tk, E, e, 1, l02Re * Tk2);
-        st_vec<P>(st.Tk, E, e, 2, l02Re * Tk3);
-        st_vec<P>(st.Tk, E, e, 3, l02Re * Tk4);
+        st_vec(st.Tk, E, e, 0, l02Re * Tk1);
+        st_vec(st.Tk, E, e, 1, l02Re * Tk2);
+        st_vec(st.Tk, E, e, 2, l02Re * Tk3);
+        st_vec(st.Tk, E, e, 3, l02Re * Tk4);
         M11 = (l02Re * M11) * ReT; M12 = (l02Re * M12) * ReT; M13 = (l02Re * M13) * ReT; M14 = (l02Re * M14) * ReT;
         M22 = (l02Re * M22) * ReT; M23 = (l02Re * M23) * ReT; M24 = (l02Re * M24) * ReT;
         M33 = (l02Re * M33) * ReT; M34 = (l02Re * M34) * ReT; M44 = (l02Re * M44) * ReT;
         mb = MassBlocks<P>{M11, M12, M13, M14, M22, M23, M24, M33, M34, M44};
-        st_block<P>(st.M, E, e, 0, 0, M11); st_block<P>(st.M, E, e, 0, 1, M12);
-        st_block<P>(st.M, E, e, 0, 2, M13); st_block<P>(st.M, E, e, 0, 3, M14);
-        st_block<P>(st.M, E, e, 1, 0, tr(M12) * TT1t);
+        st_block(st.M, E, e, 0, 0, M11); st_block(st.M, E, e, 0, 1, M12);
+        st_block(st.M, E, e, 0, 2, M13); st_block(st.M, E, e, 0, 3, M14);
+        st_block(st.M, E, e, 1, 0, tr(M12) * TT1t);
         M3<P> M23t = M23 * TT1t, M24t = M24 * TT1t;
-        st_block<P>(st.M, E, e, 1, 1, M22 * TT1t); st_block<P>(st.M, E, e, 1, 2, M23t); st_block<P>(st.M, E, e, 1, 3, M24t);
-        st_block<P>(st.M, E, e, 2, 0, tr(M13)); st_block<P>(st.M, E, e, 2, 1, tr(M23t));
-        st_block<P>(st.M, E, e, 2, 2, M33); st_block<P>(st.M, E, e, 2, 3, M34);
-        st_block<P>(st.M, E, e, 3, 0, tr(M14) * TT2t); st_block<P>(st.M, E, e, 3, 1, tr(M24t) * TT2t);
-        st_block<P>(st.M, E, e, 3, 2, tr(M34) * TT2t); st_block<P>(st.M, E, e, 3, 3, M44 * TT2t);
+        st_block(st.M, E, e, 1, 1, M22 * TT1t); st_block(st.M, E, e, 1, 2, M23t); st_block(st.M, E, e, 1, 3, M24t);
+        st_block(st.M, E, e, 2, 0, tr(M13)); st_block(st.M, E, e, 2, 1, tr(M23t));
+        st_block(st.M, E, e, 2, 2, M33); st_block(st.M, E, e, 2, 3, M34);
+        st_block(st.M, E, e, 3, 0, tr(M14) * TT2t); st_block(st.M, E, e, 3, 1, tr(M24t) * TT2t);
+        st_block(st.M, E, e, 3, 2, tr(M34) * TT2t); st_block(st.M, E, e, 3, 3, M44 * TT2t);
     }
     if constexpr (CA) {
         C11 = (l02Re * C11) * ReT; C21 = (l02Re * C21) * ReT; C31 = (l02Re * C31) * ReT; C41 = (l02Re * C41) * ReT;
-        st_block<P>(st.C, E, e, 0, 0, C11); st_block<P>(st.C, E, e, 0, 2, -C11);
-        st_block<P>(st.C, E, e, 1, 0, C21 * TT1t); st_block<P>(st.C, E, e, 1, 2, (-C21) * TT1t);
-        st_block<P>(st.C, E, e, 2, 0, C31); st_block<P>(st.C, E, e, 2, 2, -C31);
-        st_block<P>(st.C, E, e, 3, 0, C41 * TT2t); st_block<P>(st.C, E, e, 3, 2, (-C41) * TT2t);
+        st_block(st.C, E, e, 0, 0, C11); st_block(st.C, E, e, 0, 2, -C11);
+        st_block(st.C, E, e, 1, 0, C21 * TT1t); st_block(st.C, E, e, 1, 2, (-C21) * TT1t);
+        st_block(st.C, E, e, 2, 0, C31); st_block(st.C, E, e, 2, 2, -C31);
+        st_block(st.C, E, e, 3, 0, C41 * TT2t); st_block(st.C, E, e, 3, 2, (-C41) * TT2t);
     }
     if constexpr (CB) {
         C12 = (l02Re * C12) * ReT; C14 = (l02Re * C14) * ReT; C22 = (l02Re * C22) * ReT; C24 = (l02Re * C24) * ReT;
         C32 = (l02Re * C32) * ReT; C34 = (l02Re * C34) * ReT; C42 = (l02Re * C42) * ReT; C44 = (l02Re * C44) * ReT;
-        st_block<P>(st.C, E, e, 0, 1, C12); st_block<P>(st.C, E, e, 0, 3, C14);
-        st_block<P>(st.C, E, e, 1, 1, C22 * TT1t); st_block<P>(st.C, E, e, 1, 3, C24 * TT1t);
-        st_block<P>(st.C, E, e, 2, 1, C32); st_block<P>(st.C, E, e, 2, 3, C34);
-        st_block<P>(st.C, E, e, 3, 1, C42 * TT2t); st_block<P>(st.C, E, e, 3, 3, C44 * TT2t);
+        st_block(st.C, E, e, 0, 1, C12); st_block(st.C, E, e, 0, 3, C14);
+        st_block(st.C, E, e, 1, 1, C22 * TT1t); st_block(st.C, E, e, 1, 3, C24 * TT1t);
+        st_block(st.C, E, e, 2, 1, C32); st_block(st.C, E, e, 2, 3, C34);
+        st_block(st.C, E, e, 3, 1, C42 * TT2t); st_block(st.C, E, e, 3, 3, C44 * TT2t);
     }
 }
 
 // Rayleigh-type damping term of Cᵏ (compute_beams.jl:462-479): Cᵏ += damping·M with the M blocks
 // taken before the Tₛ⁻¹ post-multiplication, which then applies to the sum on block rows 2 and 4.
 // Added by the mass part to the C blocks the other parts stored.
-template <typename P>
-EBD void beam_damping(int64_t e, const NodeState& ns, const BeamConsts& bc, const BeamStage<P>& st,
+template <typename P, typename S>
+EBD void beam_damping(int64_t e, const NodeState& ns, const BeamConsts& bc, const BeamStage<S>& st,
                       const MassBlocks<P>& m) {
     const int64_t E = bc.E;
     const P d = P(bc.props[13 * E + e]);
@@ -419,7 +419,7 @@ EBD void beam_damping(int64_t e, const NodeState& ns, const BeamConsts& bc, cons
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
-            for (int i = 0; i < 3; ++i) st.C[e * 144 + (3 * bj + j) * 12 + 3 * bi + i] += B.a[i + 3 * j];
+            for (int i = 0; i < 3; ++i) st.C[e * 144 + (3 * bj + j) * 12 + 3 * bi + i] += S(B.a[i + 3 * j]);
     };
     add(0, 0, d * m.M11); add(0, 1, d * m.M12); add(0, 2, d * m.M13); add(0, 3, d * m.M14);
     add(1, 0, (d * tr(m.M12)) * TT1t); add(1, 1, (d * m.M22) * TT1t);
@@ -433,27 +433,34 @@ EBD void beam_damping(int64_t e, const NodeState& ns, const BeamConsts& bc, cons
 // the same 64 elements. Lane l of every wavefront works on element 64·blockIdx.x + l.
 constexpr int BEAM_TILE = 64;
 
-template <typename P>
-__global__ void __launch_bounds__(4 * BEAM_TILE) beam_kernel(NodeState ns, BeamConsts bc, BeamStage<P> st, GaussPts gp) {
+// S: storage type of the element matrices (double: the FP64 form; float: the precision form, whose
+// Coriolis parts also compute in FP32). The static part always computes in FP64 and the mass part
+// too, because it yields the inertia forces Tᵏ, which enter the FP64 residual.
+template <typename S>
+__global__ void __launch_bounds__(4 * BEAM_TILE) beam_kernel(NodeState ns, BeamConsts bc, BeamStage<S> st, GaussPts gp,
+                                                              const uint8_t* __restrict__ active,
+                                                              const int32_t* __restrict__ beam_inst) {
     const int part = __builtin_amdgcn_readfirstlane(threadIdx.x / BEAM_TILE);
     const int64_t e = (int64_t)blockIdx.x * BEAM_TILE + threadIdx.x % BEAM_TILE;
-    const bool valid = e < bc.E;
-    MassBlocks<P> mb;
+    // elements of instances frozen in the Newton iteration are skipped
+    const bool valid = e < bc.E && active[beam_inst[e]];
+    MassBlocks<double> mb;
+    MassBlocks<S> unused;
     if (valid) {
         if (part == PART_STATIC) {
-            beam_static<P>(e, ns, bc, st);
+            beam_static<S>(e, ns, bc, st);
         } else {
             const StaticOut so = beam_frame(e, ns, bc);
-            if (part == PART_MASS) beam_dynamic<P, PART_MASS>(e, ns, bc, st, gp, so, mb);
-            else if (part == PART_C13) beam_dynamic<P, PART_C13>(e, ns, bc, st, gp, so, mb);
-            else beam_dynamic<P, PART_C24>(e, ns, bc, st, gp, so, mb);
+            if (part == PART_MASS) beam_dynamic<double, S, PART_MASS>(e, ns, bc, st, gp, so, mb);
+            else if (part == PART_C13) beam_dynamic<S, S, PART_C13>(e, ns, bc, st, gp, so, unused);
+            else beam_dynamic<S, S, PART_C24>(e, ns, bc, st, gp, so, unused);
         }
     }
     // every wavefront sees the same 64 elements, so this test is uniform over the workgroup
     const bool damped = __ballot(valid && bc.props[13 * bc.E + e] > 0.0) != 0;
     if (damped) {
         __syncthreads();
-        if (part == PART_MASS && valid && bc.props[13 * bc.E + e] > 0.0) beam_damping<P>(e, ns, bc, st, mb);
+        if (part == PART_MASS && valid && bc.props[13 * bc.E + e] > 0.0) beam_damping<double, S>(e, ns, bc, st, mb);
     }
 }
 

@@ -26,12 +26,14 @@ struct SphereDev { double c[3], R; };
 
 template <typename P> struct ContactArgs {
     int surface;  // EB_SURFACE_*
-    SphereDev sphere;
-    SDFDev<P> sdf;
+    const SphereDev* sphere;   // [ninst]
+    const SDFDev<P>* sdf;      // [ninst]
+    const int32_t* beam_inst;  // [E]
+    const uint8_t* active;     // [ninst]
     double kn, mu, eps_t, eta_n, alpha;
     const int32_t* beams;  // 0-based contact beam list
     int64_t n;
-    P* Tc;                 // [E][12] staging (written for every listed beam)
+    double* Tc;            // [E][12] staging (written for every listed beam), FP64
     double* energy;        // [E][3] staging, contact energy in slot 2
 };
 
@@ -42,8 +44,8 @@ EBD bool isinside_dom(V3<double> p, const double* dom) {
 
 // 27-coefficient stencil: value (and optionally gradient, Hessian) of the interpolant.
 // t = (x - x₀)/dx + 1, knot = round-half-up(t) clamped to [1, n], δ = t - knot.
-template <typename P>
-EBD void sdf_eval(const SDFDev<P>& s, V3<double> p, P& val, V3<P>* grad, M3<P>* hess) {
+template <typename P, typename C>
+EBD void sdf_eval(const SDFDev<C>& s, V3<double> p, P& val, V3<P>* grad, M3<P>* hess) {
     int ix[3];
     P w[3][3], wd[3][3], wdd[3][3];
     double pc[3] = {p.x, p.y, p.z};
@@ -68,10 +70,10 @@ EBD void sdf_eval(const SDFDev<P>& s, V3<double> p, P& val, V3<P>* grad, M3<P>* 
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-            const P* row = s.coef + ((int64_t)(ix[2] - 1 + c) * P1 + (ix[1] - 1 + b)) * P0 + (ix[0] - 1);
+            const C* row = s.coef + ((int64_t)(ix[2] - 1 + c) * P1 + (ix[1] - 1 + b)) * P0 + (ix[0] - 1);
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                P q = row[a];
+                P q = P(row[a]);
                 v += w[0][a] * w[1][b] * w[2][c] * q;
                 if (grad) {
                     gx += wd[0][a] * w[1][b] * w[2][c] * q;
@@ -94,24 +96,28 @@ EBD void sdf_eval(const SDFDev<P>& s, V3<double> p, P& val, V3<P>* grad, M3<P>* 
 }
 
 // incontact_beams (utils_compute_interactions_beams.jl:23-36, :67-75)
-template <typename P> EBD bool in_contact(const ContactArgs<P>& ca, V3<double> p, double rb) {
+template <typename P> EBD bool in_contact(const ContactArgs<P>& ca, int32_t inst, V3<double> p, double rb) {
     double gbar = rb / 4;
     if (ca.surface == 1) {
-        V3<double> aux = {p.x - ca.sphere.c[0], p.y - ca.sphere.c[1], p.z - ca.sphere.c[2]};
-        return (norm(aux) - ca.sphere.R) - rb <= gbar;
+        const SphereDev& sp = ca.sphere[inst];
+        V3<double> aux = {p.x - sp.c[0], p.y - sp.c[1], p.z - sp.c[2]};
+        return (norm(aux) - sp.R) - rb <= gbar;
     }
-    if (!isinside_dom(p, ca.sdf.dom)) return false;
-    P v;
-    sdf_eval<P>(ca.sdf, p, v, nullptr, nullptr);
-    return double(v) - rb <= gbar;
+    const SDFDev<P>& sdf = ca.sdf[inst];
+    if (!isinside_dom(p, sdf.dom)) return false;
+    double v;
+    sdf_eval<double, P>(sdf, p, v, nullptr, nullptr);
+    return v - rb <= gbar;
 }
 
 // contact_gap_beams (utils_compute_interactions_beams.jl:2-20, :50-64)
-template <typename P> EBD void contact_gap(const ContactArgs<P>& ca, V3<double> p, double rb, P& g, V3<P>& n, M3<P>& H) {
+template <typename P, typename C>
+EBD void contact_gap(const ContactArgs<C>& ca, int32_t inst, V3<double> p, double rb, P& g, V3<P>& n, M3<P>& H) {
     if (ca.surface == 1) {
-        V3<P> aux = {P(p.x - ca.sphere.c[0]), P(p.y - ca.sphere.c[1]), P(p.z - ca.sphere.c[2])};
+        const SphereDev& sp = ca.sphere[inst];
+        V3<P> aux = {P(p.x - sp.c[0]), P(p.y - sp.c[1]), P(p.z - sp.c[2])};
         P na = norm(aux);
-        g = (na - P(ca.sphere.R)) - P(rb);
+        g = (na - P(sp.R)) - P(rb);
         P inv = P(1) / na;
         n = inv * aux;
         H = inv * eye<P>() - ((inv * inv) * inv) * outer(aux, aux);
@@ -120,7 +126,7 @@ template <typename P> EBD void contact_gap(const ContactArgs<P>& ca, V3<double> 
     P v;
     V3<P> gr;
     M3<P> h;
-    sdf_eval<P>(ca.sdf, p, v, &gr, &h);
+    sdf_eval<P, C>(ca.sdf[inst], p, v, &gr, &h);
     P nn = dot(gr, gr);
     P nmaginv = P(1) / sqrt(nn);
     n = gr * nmaginv;
@@ -130,10 +136,14 @@ template <typename P> EBD void contact_gap(const ContactArgs<P>& ca, V3<double> 
 
 // Consistent contact tangent of one beam for block column B of Kᶜ and Cᶜ (rows a = 0..3), and
 // Tᶜ of node slot B; the beam has a Gauss point in contact (mask: the points in contact).
-template <typename P, int B>
-EBD void contact_column(int64_t e, unsigned mask, const NodeState& ns, const BeamConsts& bc, const BeamStage<P>& st,
-                        const GaussPts& gp, const ContactArgs<P>& ca) {
+enum { CONTACT_FORCE = 1, CONTACT_TANGENT = 2 };
+
+template <typename P, typename C, int B, int WHAT>
+EBD void contact_column(int64_t e, unsigned mask, const NodeState& ns, const BeamConsts& bc, const BeamStage<C>& st,
+                        const GaussPts& gp, const ContactArgs<C>& ca) {
+    constexpr bool FORCE = (WHAT & CONTACT_FORCE) != 0, TANGENT = (WHAT & CONTACT_TANGENT) != 0;
     const int64_t E = bc.E;
+    const int32_t inst = ca.beam_inst[e];
     const int32_t na = bc.n1[e], nb = bc.n2[e];
     const double rb = bc.props[0 * E + e];
     V3<double> x1 = ldv(ns.X0, na) + ldv(ns.u, na), x2 = ldv(ns.X0, nb) + ldv(ns.u, nb);
@@ -194,7 +204,7 @@ EBD void contact_column(int64_t e, unsigned mask, const NodeState& ns, const Bea
         P gn;
         V3<P> n;
         M3<P> Hg;
-        contact_gap<P>(ca, xG, rb, gn, n, Hg);
+        contact_gap<P, C>(ca, inst, xG, rb, gn, n, Hg);
         // regularize_gap_penalty_beams
         P gb = P(rb) / P(4), hh = gb / P(2), pn, dpn, Pi;
         if (gn <= P(0)) {
@@ -219,7 +229,7 @@ EBD void contact_column(int64_t e, unsigned mask, const NodeState& ns, const Bea
         V3<P> udn = udn_mag * n;
         V3<P> udt = ud0 - udn;
         P udt2 = dot(udt, udt);
-        energy += double((w * kn) * Pi);
+        if (FORCE) energy += double((w * kn) * Pi);
         V3<P> fn = (kn * pn) * n - eta_n * udn;
         P mureg = mu / sqrt(udt2 + eps_t);
         V3<P> ft = (((-kn) * pn) * mureg) * udt;
@@ -227,7 +237,8 @@ EBD void contact_column(int64_t e, unsigned mask, const NodeState& ns, const Bea
         V3<P> Fc = tmul(Re, fc);
         M3<P> RHR[4];
         for (int a = 0; a < 4; ++a) RHR[a] = (Re * H1[a]) * ReT;
-        Tcb += w * tmul(RHR[B], fc);
+        if (FORCE) Tcb += w * tmul(RHR[B], fc);
+        if (!TANGENT) continue;
         M3<P> A1F = rows(P(0), P(0), P(0), Fc.y * v1.x, Fc.y * v1.y, Fc.y * v1.z, Fc.z * v1.x, Fc.z * v1.y, Fc.z * v1.z);
         M3<P> SF = skew(Fc);
         M3<P> SFP1 = (SF * PP1) * ReT, SFP2 = (SF * PP2) * ReT, SFP4 = (SF * PP4) * ReT;
@@ -288,16 +299,17 @@ EBD void contact_column(int64_t e, unsigned mask, const NodeState& ns, const Bea
         }
     }
     const P l02 = P(l0 / 2);
-    if (B == 0) ca.energy[3 * e + 2] = double(l02) * energy;
+    if (FORCE) {
+        if (B == 0) ca.energy[3 * e + 2] = double(l02) * energy;
+        V3<P> t = l02 * Tcb;
+        ca.Tc[e * 12 + 3 * B] = double(t.x);
+        ca.Tc[e * 12 + 3 * B + 1] = double(t.y);
+        ca.Tc[e * 12 + 3 * B + 2] = double(t.z);
+    }
+    if (!TANGENT) return;
     const M3<P> TT1t = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, na)))));
     const M3<P> TT2t = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, nb)))));
     const P c1a = P(1 + ca.alpha);
-    {
-        V3<P> t = l02 * Tcb;
-        ca.Tc[e * 12 + 3 * B] = t.x;
-        ca.Tc[e * 12 + 3 * B + 1] = t.y;
-        ca.Tc[e * 12 + 3 * B + 2] = t.z;
-    }
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         M3<P> K = l02 * Kc[a];
@@ -309,8 +321,8 @@ EBD void contact_column(int64_t e, unsigned mask, const NodeState& ns, const Bea
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 int64_t idx = e * 144 + (3 * B + j) * 12 + 3 * a + i;
-                st.K[idx] += -K.a[i + 3 * j];
-                st.C[idx] += -(c1a * Cm.a[i + 3 * j]);
+                st.K[idx] += C(-K.a[i + 3 * j]);
+                st.C[idx] += C(-(c1a * Cm.a[i + 3 * j]));
             }
     }
 }
@@ -326,6 +338,8 @@ __global__ void __launch_bounds__(256) contact_detect(NodeState ns, BeamConsts b
     if (li >= ca.n) return;
     const int64_t E = bc.E;
     const int64_t e = ca.beams[li];
+    const int32_t inst = ca.beam_inst[e];
+    if (!ca.active[inst]) return;   // frozen instance: its staging is not gathered
     const int32_t na = bc.n1[e], nb = bc.n2[e];
     const double rb = bc.props[0 * E + e];
     V3<double> x1 = ldv(ns.X0, na) + ldv(ns.u, na), x2 = ldv(ns.X0, nb) + ldv(ns.u, nb);
@@ -342,10 +356,10 @@ __global__ void __launch_bounds__(256) contact_detect(NodeState ns, BeamConsts b
         ShapeF s = shape_fn(l0, gp.z[ig]);
         V3<double> ut = {0.0, s.N3 * Tb1.z + s.N4 * Tb2.z, (-s.N3) * Tb1.y + (-s.N4) * Tb2.y};
         V3<double> xG = (s.N1 * x1 + s.N2 * x2) + f.Re * ut;
-        if (in_contact<P>(ca, xG, rb)) mask |= 1u << ig;
+        if (in_contact<P>(ca, inst, xG, rb)) mask |= 1u << ig;
     }
     if (!mask) {
-        for (int k = 0; k < 12; ++k) ca.Tc[e * 12 + k] = P(0);
+        for (int k = 0; k < 12; ++k) ca.Tc[e * 12 + k] = 0.0;
         ca.energy[3 * e + 2] = 0.0;
         return;
     }
@@ -354,6 +368,19 @@ __global__ void __launch_bounds__(256) contact_detect(NodeState ns, BeamConsts b
 }
 
 __global__ void contact_reset(int32_t* count) { *count = 0; }
+
+// One block column: FP64 throughout for the FP64 form; for the precision form (C = float) the
+// contact force and energy still in FP64 (they enter the residual), the tangent in FP32.
+template <typename C, int B>
+EBD void contact_part(int64_t e, unsigned mask, const NodeState& ns, const BeamConsts& bc, const BeamStage<C>& st,
+                      const GaussPts& gp, const ContactArgs<C>& ca) {
+    if constexpr (sizeof(C) == sizeof(double)) {
+        contact_column<double, C, B, CONTACT_FORCE | CONTACT_TANGENT>(e, mask, ns, bc, st, gp, ca);
+    } else {
+        contact_column<double, C, B, CONTACT_FORCE>(e, mask, ns, bc, st, gp, ca);
+        contact_column<C, C, B, CONTACT_TANGENT>(e, mask, ns, bc, st, gp, ca);
+    }
+}
 
 // Pass 2: four wavefronts per 64 listed beams, wavefront b computing block column b. Launched for
 // the whole contact list; workgroups past the number of beams found in contact exit at once.
@@ -370,10 +397,10 @@ __global__ void __launch_bounds__(4 * CONTACT_TILE) contact_kernel(NodeState ns,
     const uint64_t ent = list[li];
     const int64_t e = (int64_t)(ent >> 8);
     const unsigned mask = (unsigned)(ent & 0xffu);
-    if (part == 0) contact_column<P, 0>(e, mask, ns, bc, st, gp, ca);
-    else if (part == 1) contact_column<P, 1>(e, mask, ns, bc, st, gp, ca);
-    else if (part == 2) contact_column<P, 2>(e, mask, ns, bc, st, gp, ca);
-    else contact_column<P, 3>(e, mask, ns, bc, st, gp, ca);
+    if (part == 0) contact_part<P, 0>(e, mask, ns, bc, st, gp, ca);
+    else if (part == 1) contact_part<P, 1>(e, mask, ns, bc, st, gp, ca);
+    else if (part == 2) contact_part<P, 2>(e, mask, ns, bc, st, gp, ca);
+    else contact_part<P, 3>(e, mask, ns, bc, st, gp, ca);
 }
 
 }  // namespace ebd

@@ -59,6 +59,61 @@ template <int NQ> EBD void finalize_sum(const double* partials, int grid, double
     for (int q = 0; q < NQ; ++q) out[q] = res[q];
 }
 
+// ---- per-instance (segmented) reductions ---------------------------------------------------
+// A context holds one or more independent structures (instances) laid out one after the other.
+// Reductions over dofs run over chunks that never straddle two instances: chunk c covers dofs
+// [lo[c], hi[c]) of instance inst[c], and instance s owns chunks cptr[s] .. cptr[s+1]-1. One
+// workgroup reduces one chunk; a finalize kernel sums each instance's chunk partials in chunk
+// order, so every sum is deterministic.
+constexpr int SEG_CHUNK = 1024;   // dofs per chunk
+constexpr int FIN_BLOCK = 1024;   // threads of the per-instance finalize kernels
+
+struct Segs {
+    const int64_t *lo, *hi;
+    const int32_t* inst;
+    const int32_t* cptr;
+    int32_t nchunks, ninst;
+};
+
+template <int NQ> EBD void seg_block_store(double (&v)[NQ], double* partials, int nchunks) {
+    __shared__ double sh[NQ][RED_BLOCK / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        double x = v[q];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) sh[q][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double t = 0;
+            for (int w = 0; w < RED_BLOCK / 64; ++w) t += sh[q][w];
+            partials[(int64_t)q * nchunks + blockIdx.x] = t;
+        }
+    }
+}
+
+template <int NQ> EBD void seg_sum(const double* partials, const Segs& sg, int s, double (&out)[NQ]) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) out[q] = 0;
+    for (int c = sg.cptr[s]; c < sg.cptr[s + 1]; ++c) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) out[q] += partials[(int64_t)q * sg.nchunks + c];
+    }
+}
+
+// Newton activity per instance: an instance that has converged is frozen (no assembly, solve or
+// update) until the step ends, so that it ends exactly where it would end alone.
+struct Active {
+    const uint8_t* flag;       // [ninst]
+    const int32_t* node_inst;  // [N]
+    const int32_t* beam_inst;  // [E]
+    const int32_t* blk_inst;   // [nnz/36]
+};
+
 // ---- assembly gathers (atomic-free, beam order = the reference's serial scatter order) ------
 
 // K, C, M (and optionally Ktan = (1+α)K + γ/(βΔt) C + M/(βΔt²)) for every block-row entry
@@ -67,10 +122,11 @@ __global__ void __launch_bounds__(256) gather_matrices(int64_t nnz, const int32_
                                                        const uint32_t* __restrict__ blk_src, const P* __restrict__ sK,
                                                        const P* __restrict__ sC, const P* __restrict__ sM, double* K,
                                                        double* C, double* M, double* Ktan, double c1, double c2,
-                                                       double c3, int with_tangent) {
+                                                       double c3, int with_tangent, Active act) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nnz) return;
     const int64_t b = q / 36;
+    if (!act.flag[act.blk_inst[b]]) return;
     const int r = (int)(q % 36);
     const int a = r / 6, c = r % 6;
     double k = 0, cc = 0, m = 0;
@@ -92,14 +148,15 @@ __global__ void __launch_bounds__(256) gather_matrices(int64_t nnz, const int32_
 // Tⁱⁿᵗ, Tᵏ, Tᶜ per dof, optionally followed by the corrector residual (utils_solver.jl:87)
 template <typename P>
 __global__ void __launch_bounds__(256) gather_vectors(int64_t ndofs, const int32_t* __restrict__ inc_ptr,
-                                                      const uint32_t* __restrict__ inc_src, const P* __restrict__ sTint,
-                                                      const P* __restrict__ sTk, const P* __restrict__ sTc, int with_contact,
+                                                      const uint32_t* __restrict__ inc_src, const double* __restrict__ sTint,
+                                                      const double* __restrict__ sTk, const double* __restrict__ sTc, int with_contact,
                                                       double* Tint, double* Tk, double* Tc, int with_residual, double alpha,
                                                       const double* fext, const double* fext_n, const double* Tc_n,
-                                                      const double* Tint_n, double* res) {
+                                                      const double* Tint_n, double* res, Active act) {
     int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= ndofs) return;
     const int64_t n = d / 6;
+    if (!act.flag[act.node_inst[n]]) return;
     const int c = (int)(d % 6);
     double ti = 0, tk = 0, tc = 0;
     for (int s = inc_ptr[n]; s < inc_ptr[n + 1]; ++s) {
@@ -107,9 +164,9 @@ __global__ void __launch_bounds__(256) gather_vectors(int64_t ndofs, const int32
         int64_t e = src >> 1;
         int slot = src & 1;
         int64_t idx = e * 12 + 6 * slot + c;
-        tk += double(sTk[idx]);
-        ti += double(sTint[idx]);
-        if (with_contact) tc += double(sTc[idx]);
+        tk += sTk[idx];
+        ti += sTint[idx];
+        if (with_contact) tc += sTc[idx];
     }
     Tint[d] = ti;
     Tk[d] = tk;
@@ -120,9 +177,11 @@ __global__ void __launch_bounds__(256) gather_vectors(int64_t ndofs, const int32
 
 // energies: Σ over beams of [E][3] staging, added to the running totals (never reset, as in
 // the reference's Energy accumulation, assemble_beams.jl:56-57)
-__global__ void __launch_bounds__(RED_BLOCK) energy_partials(int64_t E, const double* st, double* partials, int mask) {
+__global__ void __launch_bounds__(RED_BLOCK) energy_partials(int64_t E, const double* st, double* partials, int mask,
+                                                             Active act) {
     double v[3] = {0, 0, 0};
     for (int64_t e = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; e < E; e += (int64_t)gridDim.x * RED_BLOCK) {
+        if (!act.flag[act.beam_inst[e]]) continue;
         if (mask & 1) { v[0] += st[3 * e]; v[1] += st[3 * e + 1]; }
         if (mask & 2) v[2] += st[3 * e + 2];
     }
@@ -190,9 +249,10 @@ __global__ void __launch_bounds__(256) residual_predictor(int64_t ndofs, const i
 __global__ void __launch_bounds__(256) bc_lift(int64_t ndofs, const int32_t* __restrict__ bptr,
                                                const int32_t* __restrict__ bcol, const int32_t* __restrict__ fixidx,
                                                const double* imposed, const double* D, const double* Ktan, double* dD,
-                                               double* res) {
+                                               double* res, Active act) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ndofs) return;
+    if (!act.flag[act.node_inst[i / 6]]) return;
     if (fixidx[i] >= 0) dD[i] = imposed[fixidx[i]] - D[i];
     const int64_t n = i / 6;
     const int a = (int)(i % 6);
@@ -215,15 +275,21 @@ __global__ void __launch_bounds__(256) bc_lift(int64_t ndofs, const int32_t* __r
 
 // ---- masked block-row SpMV: y = A_ff x on free dofs, y = x on fixed dofs ---------------------
 // (x is zero on fixed dofs throughout the Krylov iteration). Optional dot products with y.
+// One workgroup per reduction chunk; chunks of instances whose solve is done are skipped.
+struct KrylovCtl {
+    double rho, rho_old, alpha, omega, bnorm2, rr2, ss2, tol2;
+    int done, iters, breakdown, pad;
+};
+
 template <int NQ>
-__global__ void __launch_bounds__(RED_BLOCK) spmv_masked(int64_t ndofs, const int32_t* __restrict__ bptr,
+__global__ void __launch_bounds__(RED_BLOCK) spmv_masked(Segs sg, const int32_t* __restrict__ bptr,
                                                          const int32_t* __restrict__ bcol, const uint8_t* __restrict__ fixed,
                                                          const double* __restrict__ A, const double* __restrict__ x,
                                                          double* __restrict__ y, const double* __restrict__ dotw,
-                                                         double* partials, const int* done) {
-    if (*done) return;
+                                                         double* partials, const KrylovCtl* kc) {
+    if (kc[sg.inst[blockIdx.x]].done) return;
     double v[NQ > 0 ? NQ : 1] = {0};
-    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < ndofs; i += (int64_t)gridDim.x * RED_BLOCK) {
+    for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
         double acc;
         if (fixed[i]) {
             acc = x[i];
@@ -242,7 +308,7 @@ __global__ void __launch_bounds__(RED_BLOCK) spmv_masked(int64_t ndofs, const in
         if (NQ >= 1) v[0] += dotw[i] * acc;
         if (NQ >= 2) v[NQ - 1] += acc * acc;
     }
-    if (NQ > 0) block_sum_store<(NQ > 0 ? NQ : 1)>(v, partials, gridDim.x);
+    if (NQ > 0) seg_block_store<(NQ > 0 ? NQ : 1)>(v, partials, sg.nchunks);
 }
 
 // ---- chain block-tridiagonal preconditioner ------------------------------------------------
@@ -272,6 +338,7 @@ struct ChainDev {
     double* hist;                  // 72 per slot: L then U of a kept position at one level
     double *bw, *yw;               // 6 per position: solve work vectors
     int32_t* rej;                  // per position: unpivoted inverse rejected (zero between calls)
+    const int32_t* inst;           // per chain: instance
     int64_t n_chains;
 };
 
@@ -559,8 +626,8 @@ __global__ void __launch_bounds__(CHAIN_THREADS) chain_factor(ChainDev cd, const
 // saved blocks), then solved back from position 0 (x_j = y_j - GL_j x_{j-h} - GU_j x_{j+h}).
 // Six lanes per position, one per row.
 __global__ void __launch_bounds__(CHAIN_THREADS) chain_solve(ChainDev cd, const double* __restrict__ b, double* __restrict__ x,
-                                                             const int* done) {
-    if (*done) return;
+                                                             const KrylovCtl* kc) {
+    if (kc[cd.inst[blockIdx.x]].done) return;
     const int64_t s0 = cd.ptr[blockIdx.x];
     const int32_t n = cd.ptr[blockIdx.x + 1] - (int32_t)s0;
     const int tid = threadIdx.x;
@@ -637,115 +704,136 @@ __global__ void __launch_bounds__(CHAIN_THREADS) chain_solve(ChainDev cd, const 
 }
 
 // ---- BiCGStab (van der Vorst), right-preconditioned, FP64 scalars on the device -------------
+// One independent solve per instance, run in lockstep: the scalars live in KrylovCtl[instance],
+// element-wise and reduction kernels work chunk by chunk, and the finalize kernels update every
+// instance's scalars and the all-done flag the host polls.
 
-struct KrylovCtl {
-    double rho, rho_old, alpha, omega, bnorm2, rr2, ss2, tol2;
-    int done, iters, breakdown, pad;
-};
+struct KrylovAll { int all_done, pad[3]; };
 
 // x = 0, r = r̂ = b (b = residual with fixed dofs zeroed), ‖b‖²
-__global__ void __launch_bounds__(RED_BLOCK) krylov_init(int64_t ndofs, const uint8_t* fixed, const double* res, double* x,
+__global__ void __launch_bounds__(RED_BLOCK) krylov_init(Segs sg, const uint8_t* fixed, const double* res, double* x,
                                                          double* r, double* rhat, double* p, double* v, double* partials) {
-    double s[1] = {0};
-    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < ndofs; i += (int64_t)gridDim.x * RED_BLOCK) {
+    double q[1] = {0};
+    for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
         double bi = fixed[i] ? 0.0 : res[i];
         x[i] = 0; r[i] = bi; rhat[i] = bi; p[i] = 0; v[i] = 0;
-        s[0] += bi * bi;
+        q[0] += bi * bi;
     }
-    block_sum_store<1>(s, partials, gridDim.x);
+    seg_block_store<1>(q, partials, sg.nchunks);
 }
-__global__ void __launch_bounds__(RED_BLOCK) krylov_init_fin(const double* partials, int grid, KrylovCtl* k, double tol) {
-    double o[1];
-    finalize_sum<1>(partials, grid, o);
-    if (threadIdx.x == 0) {
-        k->bnorm2 = o[0]; k->rho = o[0]; k->rho_old = 1; k->alpha = 1; k->omega = 1;
-        k->rr2 = o[0]; k->ss2 = 0; k->tol2 = tol * tol; k->iters = 0; k->breakdown = 0;
-        k->done = (o[0] == 0.0) ? 1 : 0;
+// instances that are not active in the Newton iteration start done
+__global__ void __launch_bounds__(FIN_BLOCK) krylov_init_fin(const double* partials, Segs sg, KrylovCtl* kc, double tol,
+                                                             const uint8_t* active, KrylovAll* all) {
+    int alldone = 1;
+    for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
+        double o[1];
+        seg_sum<1>(partials, sg, s, o);
+        KrylovCtl& k = kc[s];
+        k.bnorm2 = o[0]; k.rho = o[0]; k.rho_old = 1; k.alpha = 1; k.omega = 1;
+        k.rr2 = o[0]; k.ss2 = 0; k.tol2 = tol * tol; k.iters = 0; k.breakdown = 0;
+        k.done = (o[0] == 0.0 || !active[s]) ? 1 : 0;
+        alldone &= k.done;
     }
+    alldone = __syncthreads_and(alldone);
+    if (threadIdx.x == 0) all->all_done = alldone;
 }
 // p = r + β(p - ωv), β = (ρ/ρ_old)(α/ω); first iteration p = r
-__global__ void __launch_bounds__(256) krylov_p(int64_t ndofs, const KrylovCtl* k, const double* r, double* p, const double* v) {
-    if (k->done) return;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ndofs) return;
-    if (k->iters == 0) { p[i] = r[i]; return; }
-    double beta = (k->rho / k->rho_old) * (k->alpha / k->omega);
-    p[i] = r[i] + beta * (p[i] - k->omega * v[i]);
+__global__ void __launch_bounds__(RED_BLOCK) krylov_p(Segs sg, const KrylovCtl* kc, const double* r, double* p,
+                                                      const double* v) {
+    const KrylovCtl& k = kc[sg.inst[blockIdx.x]];
+    if (k.done) return;
+    const bool first = k.iters == 0;
+    const double beta = first ? 0.0 : (k.rho / k.rho_old) * (k.alpha / k.omega);
+    const double om = k.omega;
+    for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK)
+        p[i] = first ? r[i] : r[i] + beta * (p[i] - om * v[i]);
 }
-__global__ void __launch_bounds__(RED_BLOCK) krylov_alpha_fin(const double* partials, int grid, KrylovCtl* k) {
-    if (k->done) return;
-    double o[1];
-    finalize_sum<1>(partials, grid, o);
-    if (threadIdx.x == 0) {
-        if (o[0] == 0.0 || !isfinite(o[0])) { k->breakdown = 1; k->done = 1; return; }
-        k->alpha = k->rho / o[0];
+__global__ void __launch_bounds__(FIN_BLOCK) krylov_alpha_fin(const double* partials, Segs sg, KrylovCtl* kc) {
+    for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
+        KrylovCtl& k = kc[s];
+        if (k.done) continue;
+        double o[1];
+        seg_sum<1>(partials, sg, s, o);
+        if (o[0] == 0.0 || !isfinite(o[0])) { k.breakdown = 1; k.done = 1; continue; }
+        k.alpha = k.rho / o[0];
     }
 }
 // s = r - αv, x += αy, ‖s‖²
-__global__ void __launch_bounds__(RED_BLOCK) krylov_s(int64_t ndofs, const KrylovCtl* k, const double* r, const double* v,
+__global__ void __launch_bounds__(RED_BLOCK) krylov_s(Segs sg, const KrylovCtl* kc, const double* r, const double* v,
                                                       const double* y, double* s, double* x, double* partials) {
-    if (k->done) return;
-    const double al = k->alpha;
+    const KrylovCtl& k = kc[sg.inst[blockIdx.x]];
+    if (k.done) return;
+    const double al = k.alpha;
     double q[1] = {0};
-    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < ndofs; i += (int64_t)gridDim.x * RED_BLOCK) {
+    for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
         double si = r[i] - al * v[i];
         s[i] = si;
         x[i] += al * y[i];
         q[0] += si * si;
     }
-    block_sum_store<1>(q, partials, gridDim.x);
+    seg_block_store<1>(q, partials, sg.nchunks);
 }
-__global__ void __launch_bounds__(RED_BLOCK) krylov_s_fin(const double* partials, int grid, KrylovCtl* k) {
-    if (k->done) return;
-    double o[1];
-    finalize_sum<1>(partials, grid, o);
-    if (threadIdx.x == 0) {
-        k->ss2 = o[0];
-        if (o[0] <= k->tol2 * k->bnorm2) { k->rr2 = o[0]; k->iters += 1; k->done = 1; }
+__global__ void __launch_bounds__(FIN_BLOCK) krylov_s_fin(const double* partials, Segs sg, KrylovCtl* kc) {
+    for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
+        KrylovCtl& k = kc[s];
+        if (k.done) continue;
+        double o[1];
+        seg_sum<1>(partials, sg, s, o);
+        k.ss2 = o[0];
+        if (o[0] <= k.tol2 * k.bnorm2) { k.rr2 = o[0]; k.iters += 1; k.done = 1; }
     }
 }
-__global__ void __launch_bounds__(RED_BLOCK) krylov_omega_fin(const double* partials, int grid, KrylovCtl* k) {
-    if (k->done) return;
-    double o[2];
-    finalize_sum<2>(partials, grid, o);
-    if (threadIdx.x == 0) {
-        if (o[1] == 0.0 || !isfinite(o[1])) { k->breakdown = 1; k->done = 1; return; }
-        k->omega = o[0] / o[1];
+__global__ void __launch_bounds__(FIN_BLOCK) krylov_omega_fin(const double* partials, Segs sg, KrylovCtl* kc) {
+    for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
+        KrylovCtl& k = kc[s];
+        if (k.done) continue;
+        double o[2];
+        seg_sum<2>(partials, sg, s, o);
+        if (o[1] == 0.0 || !isfinite(o[1])) { k.breakdown = 1; k.done = 1; continue; }
+        k.omega = o[0] / o[1];
     }
 }
 // x += ωz, r = s - ωt, (r̂·r), ‖r‖²
-__global__ void __launch_bounds__(RED_BLOCK) krylov_x(int64_t ndofs, const KrylovCtl* k, const double* z, const double* s,
+__global__ void __launch_bounds__(RED_BLOCK) krylov_x(Segs sg, const KrylovCtl* kc, const double* z, const double* s,
                                                       const double* t, const double* rhat, double* x, double* r,
                                                       double* partials) {
-    if (k->done) return;
-    const double om = k->omega;
+    const KrylovCtl& k = kc[sg.inst[blockIdx.x]];
+    if (k.done) return;
+    const double om = k.omega;
     double q[2] = {0, 0};
-    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < ndofs; i += (int64_t)gridDim.x * RED_BLOCK) {
+    for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
         x[i] += om * z[i];
         double ri = s[i] - om * t[i];
         r[i] = ri;
         q[0] += rhat[i] * ri;
         q[1] += ri * ri;
     }
-    block_sum_store<2>(q, partials, gridDim.x);
+    seg_block_store<2>(q, partials, sg.nchunks);
 }
-__global__ void __launch_bounds__(RED_BLOCK) krylov_rho_fin(const double* partials, int grid, KrylovCtl* k) {
-    if (k->done) return;
-    double o[2];
-    finalize_sum<2>(partials, grid, o);
-    if (threadIdx.x == 0) {
-        k->rho_old = k->rho;
-        k->rho = o[0];
-        k->rr2 = o[1];
-        k->iters += 1;
-        if (o[1] <= k->tol2 * k->bnorm2) k->done = 1;
-        else if (o[0] == 0.0 || !isfinite(o[0])) { k->breakdown = 1; k->done = 1; }
+__global__ void __launch_bounds__(FIN_BLOCK) krylov_rho_fin(const double* partials, Segs sg, KrylovCtl* kc, KrylovAll* all) {
+    int alldone = 1;
+    for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
+        KrylovCtl& k = kc[s];
+        if (!k.done) {
+            double o[2];
+            seg_sum<2>(partials, sg, s, o);
+            k.rho_old = k.rho;
+            k.rho = o[0];
+            k.rr2 = o[1];
+            k.iters += 1;
+            if (o[1] <= k.tol2 * k.bnorm2) k.done = 1;
+            else if (o[0] == 0.0 || !isfinite(o[0])) { k.breakdown = 1; k.done = 1; }
+        }
+        alldone &= k.done;
     }
+    alldone = __syncthreads_and(alldone);
+    if (threadIdx.x == 0) all->all_done = alldone;
 }
-// ΔD[free] = x[free]
-__global__ void __launch_bounds__(256) scatter_solution(int64_t ndofs, const uint8_t* fixed, const double* x, double* dD) {
+// ΔD[free] = x[free] for the instances active in the Newton iteration
+__global__ void __launch_bounds__(256) scatter_solution(int64_t ndofs, const uint8_t* fixed, const double* x, double* dD,
+                                                        Active act) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ndofs && !fixed[i]) dD[i] = x[i];
+    if (i < ndofs && !fixed[i] && act.flag[act.node_inst[i / 6]]) dD[i] = x[i];
 }
 
 // ---- Newmark / rotation updates (src/utils_solver.jl:6-48, :157-245) -----------------------
@@ -771,9 +859,9 @@ __global__ void __launch_bounds__(256) init_global(int64_t N, NodeFields nf, dou
 
 // compute_global_corrector! per dof (all dofs, fixed included)
 __global__ void __launch_bounds__(256) global_corrector(int64_t ndofs, double gb, double inv, double* D, double* Dd,
-                                                        double* Ddd, const double* dD) {
+                                                        double* Ddd, const double* dD, Active act) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ndofs) return;
+    if (i >= ndofs || !act.flag[act.node_inst[i / 6]]) return;
     D[i] += dD[i]; Dd[i] += gb * dD[i]; Ddd[i] += inv * dD[i];
 }
 // compute_global_predictor!
@@ -791,9 +879,9 @@ struct HHTc { double c_w, c_wd, c_wdd, d_w, d_wd, d_wdd; };  // ẇ = ΔR(c_w w 
 
 // compute_local_corrector!: u, u̇, ü from D; ΔR ← exp(ΔD_θ)ΔR; w = log ΔR; ẇ, ẅ; R = ΔR Rⁿ
 __global__ void __launch_bounds__(256) local_corrector(int64_t N, NodeFields nf, const double* D, const double* Dd,
-                                                       const double* Ddd, const double* dD, HHTc h) {
+                                                       const double* Ddd, const double* dD, HHTc h, Active act) {
     int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+    if (n >= N || !act.flag[act.node_inst[n]]) return;
     stv(nf.u, n, {D[6 * n], D[6 * n + 1], D[6 * n + 2]});
     stv(nf.ud, n, {Dd[6 * n], Dd[6 * n + 1], Dd[6 * n + 2]});
     stv(nf.udd, n, {Ddd[6 * n], Ddd[6 * n + 1], Ddd[6 * n + 2]});
@@ -829,12 +917,12 @@ __global__ void __launch_bounds__(256) local_predictor(int64_t N, NodeFields nf,
 }
 
 // compute_norms_corrector (utils_solver.jl:126-151): Σ over free dofs of ΔD², r², (f+Tᶜ+Tᵏ)² and
-// Σ over fixed dofs of (f+Tᶜ+Tᵏ)²
-__global__ void __launch_bounds__(RED_BLOCK) norms_partials(int64_t ndofs, const uint8_t* fixed, const double* dD,
+// Σ over fixed dofs of (f+Tᶜ+Tᵏ)², per instance
+__global__ void __launch_bounds__(RED_BLOCK) norms_partials(Segs sg, const uint8_t* fixed, const double* dD,
                                                             const double* res, const double* fext, const double* Tc,
                                                             const double* Tk, double* partials) {
     double q[4] = {0, 0, 0, 0};
-    for (int64_t i = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; i < ndofs; i += (int64_t)gridDim.x * RED_BLOCK) {
+    for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
         double f = (fext[i] + Tc[i]) + Tk[i];
         if (fixed[i]) {
             q[3] += f * f;
@@ -844,16 +932,52 @@ __global__ void __launch_bounds__(RED_BLOCK) norms_partials(int64_t ndofs, const
             q[2] += f * f;
         }
     }
-    block_sum_store<4>(q, partials, gridDim.x);
+    seg_block_store<4>(q, partials, sg.nchunks);
 }
-__global__ void __launch_bounds__(RED_BLOCK) norms_finalize(const double* partials, int grid, double* out2) {
-    double o[4];
-    finalize_sum<4>(partials, grid, o);
-    if (threadIdx.x == 0) {
+
+struct NewtonCtl {            // corrector loop control (src/corrector.jl:17-36)
+    double tol_res, tol_disp;
+    int k, max_iterations;    // k: the iteration whose norms are being finalised (1-based)
+    int newton;               // 0: norms only (no convergence bookkeeping)
+};
+struct NewtonAll { int n_active, pad[3]; };
+
+// per-instance norms (relative residual, ‖ΔD‖) into out[2s], out[2s+1]; in a Newton iteration,
+// an active instance whose norms pass (or that has used its iterations) is frozen and its k
+// recorded as the reference's loop leaves it (iterations done + 1)
+__global__ void __launch_bounds__(FIN_BLOCK) norms_finalize(const double* partials, Segs sg, double* out, NewtonCtl nc,
+                                                            uint8_t* active, int32_t* kiter, NewtonAll* all) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
+        if (nc.newton && !active[s]) continue;
+        double o[4];
+        seg_sum<4>(partials, sg, s, o);
         double dn = sqrt(o[0]), rn = sqrt(o[1]), fn = sqrt(o[2]), en = sqrt(o[3]);
         if (fn + en > 1e-12) rn = rn / (fn + en);
-        out2[0] = rn;
-        out2[1] = dn;
+        out[2 * s] = rn;
+        out[2 * s + 1] = dn;
+        if (nc.newton) {
+            const bool more = (rn > nc.tol_res || dn > nc.tol_disp) && nc.k + 1 <= nc.max_iterations;
+            if (more) {
+                mine += 1;
+            } else {
+                active[s] = 0;
+                kiter[s] = nc.k + 1;
+            }
+        }
+    }
+    if (mine) atomicAdd(&cnt, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) all->n_active = cnt;
+}
+
+__global__ void newton_reset(int ninst, uint8_t* active, int32_t* kiter) {
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ninst; s += gridDim.x * blockDim.x) {
+        active[s] = 1;
+        kiter[s] = 0;
     }
 }
 

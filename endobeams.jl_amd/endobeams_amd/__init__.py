@@ -7,7 +7,7 @@ from . import abi
 from .model import (Beams, BeamElementSurface, BeamsConfiguration, BoundaryConditions, ConcentratedForce,
                     DiscreteSignedDistanceField, Encastre, ImposedDisplacement, InteractionProperties, Loads,
                     NodesBeams, RigidInteraction, SimulationParams, SphereSurface, read_vtk_sdf)
-from .engine import EBError, Engine, HipEngine, LIB_PATH, load_hip_library
+from .engine import BatchEngine, EBError, Engine, HipBatchEngine, HipEngine, LIB_PATH, load_hip_library
 from .simulation import RunStats, SimulationAborted, Stepper, run_simulation
 from . import configs
 
@@ -15,6 +15,6 @@ __all__ = [
     "abi", "Beams", "BeamElementSurface", "BeamsConfiguration", "BoundaryConditions", "ConcentratedForce",
     "DiscreteSignedDistanceField", "Encastre", "ImposedDisplacement", "InteractionProperties", "Loads",
     "NodesBeams", "RigidInteraction", "SimulationParams", "SphereSurface", "read_vtk_sdf",
-    "EBError", "Engine", "HipEngine", "LIB_PATH", "load_hip_library",
+    "BatchEngine", "EBError", "Engine", "HipBatchEngine", "HipEngine", "LIB_PATH", "load_hip_library",
     "RunStats", "SimulationAborted", "Stepper", "run_simulation", "configs",
 ]

@@ -102,6 +102,10 @@ PROTOTYPES = {
     "get_counters": (_i, [_vp, _lp]),
     "profile_enable": (_i, [_vp, C.c_uint32]),
     "profile_read": (_i, [_vp, _i, _dp, _lp]),
+    "create_batch": (_i, [C.POINTER(ModelDesc), C.c_int64, C.POINTER(_vp)]),
+    "get_instances": (_i, [_vp, _lp, _lp, _lp]),
+    "get_instance_iterations": (_i, [_vp, C.POINTER(C.c_int32)]),
+    "get_instance_norms": (_i, [_vp, _dp]),
 }
 
 KERNELS = {"beam": 0, "contact": 1, "gather": 2, "precond": 3, "krylov": 4, "update": 5}

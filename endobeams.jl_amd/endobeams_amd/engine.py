@@ -213,6 +213,107 @@ class Engine:
         self._check(self.fn["update_not_converged"](self.h), "update_not_converged")
 
 
+class BatchEngine(Engine):
+    """Many independent structures in one context (eb_create_batch): design sweeps advanced
+    together. Instance s keeps its own loads, boundary conditions and contact surface; the batch
+    shares the time step, the HHT/Newton parameters and the contact parameters. Fields are the
+    instances' fields one after the other (`offsets()` gives the node and beam offsets)."""
+
+    def __init__(self, lib: C.CDLL, instances, precision: int = abi.EB_PRECISION_FP64):
+        self.lib = lib
+        self.fn = abi.bind(lib, self.prefix)
+        self.instances = [(c, p, i) for (c, p, i) in instances]
+        conf0, params0, inter0 = self.instances[0]
+        self.conf, self.params, self.inter = conf0, params0, inter0
+        self.arrays_list = [ModelArrays(c, p, i, precision) for (c, p, i) in self.instances]
+        S = len(self.arrays_list)
+        descs = (abi.ModelDesc * S)(*[a.desc for a in self.arrays_list])
+        h = C.c_void_p()
+        st = self.fn["create_batch"](descs, S, C.byref(h))
+        if st != abi.EB_OK:
+            raise EBError(f"{self.prefix}create_batch failed with status {st}")
+        self.h = h
+        self._descs = descs
+        s = abi.Sizes()
+        self._check(self.fn["get_sizes"](self.h, C.byref(s)), "get_sizes")
+        self.sizes = s
+        self.S = S
+        noff = np.zeros(S + 1, np.int64)
+        boff = np.zeros(S + 1, np.int64)
+        n = C.c_int64(0)
+        lp = lambda a: a.ctypes.data_as(C.POINTER(C.c_int64))
+        self._check(self.fn["get_instances"](self.h, C.byref(n), lp(noff), lp(boff)), "get_instances")
+        self.node_off, self.beam_off = noff, boff
+        fields = [initial_fields(c) for (c, _, _) in self.instances]
+        for name in fields[0]:
+            self.set(name, np.concatenate([np.asarray(f[name], dtype=np.float64).reshape(-1) for f in fields]))
+        f0 = np.concatenate([self._forces(c, 0.0) for (c, _, _) in self.instances])
+        if f0.any():
+            self.set("FEXT", f0)
+            self.set("FEXT_N", f0)
+        self._np = params0.newton()
+
+    @staticmethod
+    def _forces(conf, t):
+        f = np.zeros(conf.ndofs)
+        ld = conf.loads
+        if ld is not None and ld.concentrated_force is not None:
+            dofs = np.asarray(ld.concentrated_force.loaded_dofs, dtype=np.int64)
+            f[dofs - 1] = ld.concentrated_force.values(t)
+        return f
+
+    def offsets(self):
+        return self.node_off.copy(), self.beam_off.copy()
+
+    def instance_slice(self, name: str, s: int) -> slice:
+        """Slice of field `name` that belongs to instance s."""
+        a, b = self.node_off[s], self.node_off[s + 1]
+        if name in abi.NODE_VEC_FIELDS:
+            return slice(3 * a, 3 * b)
+        if name in abi.NODE_MAT_FIELDS:
+            return slice(9 * a, 9 * b)
+        if name in abi.DOF_FIELDS:
+            return slice(6 * a, 6 * b)
+        raise KeyError(f"{name} has no per-instance slice")
+
+    def instance_iterations(self) -> np.ndarray:
+        k = np.zeros(self.S, np.int32)
+        self._check(self.fn["get_instance_iterations"](self.h, k.ctypes.data_as(C.POINTER(C.c_int32))),
+                    "get_instance_iterations")
+        return k
+
+    def update_loads(self, t):
+        dofs, vals = [], []
+        for s, (c, _, _) in enumerate(self.instances):
+            ld = c.loads
+            if ld is None or ld.concentrated_force is None:
+                continue
+            cf = ld.concentrated_force
+            dofs.append(np.asarray(cf.loaded_dofs, dtype=np.int64) + 6 * int(self.node_off[s]))
+            vals.append(np.asarray(cf.values(t), dtype=np.float64))
+        if not dofs:
+            return
+        d = np.ascontiguousarray(np.concatenate(dofs))
+        v = np.ascontiguousarray(np.concatenate(vals))
+        self._check(self.fn["update_loads"](self.h, d.size, d.ctypes.data_as(C.POINTER(C.c_int64)),
+                                            v.ctypes.data_as(C.POINTER(C.c_double))), "update_loads")
+
+    def update_boundary_conditions(self, t):
+        parts, any_fn = [], False
+        for (c, _, _) in self.instances:
+            bcs = c.bcs
+            if bcs.imposed_displacements_function is not None and len(bcs.displaced_indices) > 0:
+                any_fn = True
+                for i in bcs.displaced_indices:
+                    bcs.imposed_displacements[i] = bcs.imposed_displacements_function(t)
+            parts.append(np.asarray(bcs.imposed_displacements, dtype=np.float64).reshape(-1))
+        if not any_fn:
+            return
+        v = np.ascontiguousarray(np.concatenate(parts))
+        self._check(self.fn["update_boundary_conditions"](self.h, v.ctypes.data_as(C.POINTER(C.c_double)), v.size),
+                    "update_boundary_conditions")
+
+
 _HIP_LIB = None
 
 
@@ -232,3 +333,12 @@ class HipEngine(Engine):
 
     def __init__(self, conf, params, inter=None, precision=abi.EB_PRECISION_FP64, lib_path: str = LIB_PATH):
         super().__init__(load_hip_library(lib_path), conf, params, inter, precision)
+
+
+class HipBatchEngine(BatchEngine):
+    """Batch of independent structures on the MI355X path; `instances` is a list of
+    (BeamsConfiguration, SimulationParams, RigidInteraction or None)."""
+    prefix = "eb_"
+
+    def __init__(self, instances, precision=abi.EB_PRECISION_FP64, lib_path: str = LIB_PATH):
+        super().__init__(load_hip_library(lib_path), instances, precision)

@@ -210,6 +210,26 @@ int eb_solve_step_dynamics(eb_ctx* ctx, const eb_newton_params* p, double dt, in
 int eb_update_converged(eb_ctx* ctx);
 int eb_update_not_converged(eb_ctx* ctx);
 
+/* --- batches of independent structures (no counterpart in the reference) ------------------
+ * For design sweeps: one context advances many structures (instances) together, each as if it
+ * ran alone through run_simulation! (src/run_simulation.jl:1-119) with the batch's time step.
+ * Every instance is a complete single-structure descriptor; precision, Gauss points, contact
+ * surface kind and contact parameters must agree. Inside the context the instances are stored
+ * one after the other: node, beam and dof numbers of instance s are shifted by the sizes of
+ * instances 0..s-1, so every field, load dof and imposed-displacement list of the single-
+ * structure API addresses the batch through those offsets. The corrector iterates every
+ * instance until its own norms pass (corrector.jl:17-36); an instance that has converged is
+ * frozen until the step ends, so its trajectory is the one it has alone. eb_solve_step_dynamics
+ * returns the largest k of the instances (the step is repeated with Δt/2 for all when it exceeds
+ * max_iterations). */
+int eb_create_batch(const eb_model_desc* descs, int64_t n_instances, eb_ctx** out);
+/* node_offsets and beam_offsets receive n_instances+1 entries (either may be NULL) */
+int eb_get_instances(const eb_ctx* ctx, int64_t* n_instances, int64_t* node_offsets, int64_t* beam_offsets);
+/* k of every instance at the end of the last corrector call */
+int eb_get_instance_iterations(const eb_ctx* ctx, int32_t* k_out);
+/* relative residual and ‖ΔD‖ of every instance (2 per instance) from the last norms evaluation */
+int eb_get_instance_norms(const eb_ctx* ctx, double* rn_dn);
+
 /* --- instrumentation (no counterpart in the reference) ------------------------------------ */
 
 /* cumulative counters since eb_create: [0] Newton iterations, [1] Krylov iterations,

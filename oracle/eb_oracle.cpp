@@ -57,6 +57,8 @@ struct Oracle {
     BandLU lu;
     bool lu_ready = false;
     int64_t counters[4] = {0, 0, 0, 0};
+    int last_k = 0;
+    double last_rn = 0, last_dn = 0;
     std::string err;
 };
 
@@ -651,9 +653,12 @@ int ebo_corrector(void* h, const eb_newton_params* p, double dt, int* kout) {
         ebo_compute_global_corrector(h, dt, p->beta, p->gamma);
         ebo_compute_local_corrector(h, dt, p->beta, p->gamma);
         ebo_compute_norms_corrector(h, &rn, &dn);
+        o->last_rn = rn;
+        o->last_dn = dn;
         o->counters[0]++;
         k += 1;
     }
+    o->last_k = k;
     *kout = k;
     return EB_OK;
 }
@@ -680,6 +685,30 @@ int ebo_update_not_converged(void* h) {
     for (int f = 0; f < 8; ++f) o->node[EB_F_U + f] = o->node[EB_F_U_N + f];
     o->Tint = o->Tint_n; o->Tk = o->Tk_n; o->Tc = o->Tc_n; o->fext = o->fext_n;
     o->K = o->Kn; o->C = o->Cn; o->M = o->Mn;
+    return EB_OK;
+}
+
+// Batches: the oracle runs one structure per context; a batch is checked against each of its
+// instances run alone, so only a batch of one is accepted here.
+int ebo_create_batch(const eb_model_desc* d, int64_t n_instances, void** out) {
+    if (n_instances != 1) return EB_ERR_ARG;
+    return ebo_create(d, out);
+}
+int ebo_get_instances(void* h, int64_t* n_instances, int64_t* node_offsets, int64_t* beam_offsets) {
+    Oracle* o = (Oracle*)h;
+    if (n_instances) *n_instances = 1;
+    if (node_offsets) { node_offsets[0] = 0; node_offsets[1] = o->N; }
+    if (beam_offsets) { beam_offsets[0] = 0; beam_offsets[1] = o->E; }
+    return EB_OK;
+}
+int ebo_get_instance_iterations(void* h, int32_t* k_out) {
+    *k_out = ((Oracle*)h)->last_k;
+    return EB_OK;
+}
+int ebo_get_instance_norms(void* h, double* rn_dn) {
+    Oracle* o = (Oracle*)h;
+    rn_dn[0] = o->last_rn;
+    rn_dn[1] = o->last_dn;
     return EB_OK;
 }
 
