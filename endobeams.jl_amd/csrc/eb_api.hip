@@ -11,9 +11,12 @@
 #include <vector>
 
 #include "../../include/endobeams_hip.h"
+#include "../../include/endobeams_learned.h"
 #include "eb_beam_kernel.h"
 #include "eb_contact_kernel.h"
+#include "eb_learned.h"
 #include "eb_linalg.h"
+#include "eb_mlp.h"
 #include "eb_setup.h"
 
 using namespace ebd;
@@ -57,6 +60,17 @@ struct eb_ctx {
     int32_t* h_kiter = nullptr;
     void* d_sdf = nullptr;          // [S] SDFDev<P>
     SphereDev* d_sphere = nullptr;  // [S]
+    // learned warm start
+    eb_mlp* mlp = nullptr;
+    float *lin_scale = nullptr, *lout_scale = nullptr, *ones = nullptr, *lX = nullptr, *lT = nullptr;
+    double* hrate[3] = {};          // rates of the last three converged steps, newest first
+    double* pdD = nullptr;          // the reference predictor's increment of the current step
+    int hist_count = 0;
+    double last_dt = 0;
+    double* snap = nullptr;         // node fields U..DR, then D, Ḋ, D̈ of the reference predictor's start
+    double* rn_plain = nullptr;     // [2S]
+    uint8_t* use_plain = nullptr;   // [S]
+    int32_t* pred_counts = nullptr; // [2]
     // node state
     double* node[16] = {};
     double* X0 = nullptr;
@@ -295,16 +309,18 @@ template <typename P> int launch_gathers(eb_ctx* ctx, bool with_contact, bool ta
 }
 
 // assemble! [+ assemble_contact!] [+ Ktan and the corrector residual]
-int assemble_all(eb_ctx* ctx, bool contact, double alpha, bool fused, double dt, double b, double g) {
+int assemble_all(eb_ctx* ctx, bool contact, double alpha, bool fused, double dt, double b, double g,
+                 bool energy = true) {
     bool wc = contact && ctx->surface != EB_SURFACE_NONE;
+    const int emask = energy ? (wc ? 3 : 1) : 0;
     if (ctx->precision == EB_PRECISION_FP32) {
         EB_TRY(launch_beams<float>(ctx));
         if (wc) EB_TRY(launch_contact<float>(ctx, alpha));
-        return launch_gathers<float>(ctx, wc, fused, dt, alpha, b, g, fused, wc ? 3 : 1);
+        return launch_gathers<float>(ctx, wc, fused, dt, alpha, b, g, fused, emask);
     }
     EB_TRY(launch_beams<double>(ctx));
     if (wc) EB_TRY(launch_contact<double>(ctx, alpha));
-    return launch_gathers<double>(ctx, wc, fused, dt, alpha, b, g, fused, wc ? 3 : 1);
+    return launch_gathers<double>(ctx, wc, fused, dt, alpha, b, g, fused, emask);
 }
 
 int bc_lift_launch(eb_ctx* ctx) {
@@ -429,6 +445,92 @@ int newton_reset_launch(eb_ctx* ctx) {
     hipLaunchKernelGGL(newton_reset, dim3(nblk(ctx->S, 256)), dim3(256), 0, ctx->stream, (int)ctx->S, ctx->active,
                        ctx->kiter);
     return check_launch(ctx, "newton_reset");
+}
+
+int global_corrector_async(eb_ctx* ctx, double dt, double b, double g) {
+    hipLaunchKernelGGL(global_corrector, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs,
+                       g / (b * dt), 1 / (b * (dt * dt)), ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, active_dev(ctx));
+    return check_launch(ctx, "global_corrector");
+}
+int local_corrector_async(eb_ctx* ctx, double dt, double b, double g) {
+    hipLaunchKernelGGL(local_corrector, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx),
+                       ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, hht_consts(dt, b, g), active_dev(ctx));
+    return check_launch(ctx, "local_corrector");
+}
+int global_predictor_async(eb_ctx* ctx, double dt, double b, double g) {
+    hipLaunchKernelGGL(global_predictor, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs, dt, g / b,
+                       g / (b * dt), 1 / (b * (dt * dt)), dt * (2 * b - g) / (2 * b), ctx->D, ctx->Dd, ctx->Ddd, ctx->dD);
+    return check_launch(ctx, "global_predictor");
+}
+int local_predictor_async(eb_ctx* ctx, double dt, double b, double g) {
+    hipLaunchKernelGGL(local_predictor, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx),
+                       ctx->D, ctx->Dd, ctx->Ddd, hht_consts(dt, b, g));
+    return check_launch(ctx, "local_predictor");
+}
+
+// ---- learned warm start ---------------------------------------------------------------------
+
+const int NODE_FIELD_WIDTH[8] = {3, 3, 3, 3, 3, 3, 9, 9};   // U UDOT UDDOT W WDOT WDDOT R DR
+
+// copies the current start (node fields U..DR, D, Ḋ, D̈) into the snapshot, or back for the
+// instances flagged in `flag` (all when null)
+int snapshot_start(eb_ctx* ctx, bool restore, const uint8_t* flag) {
+    const int64_t N = ctx->N;
+    int64_t off = 0;
+    auto cp = [&](double* cur, int per) {
+        double* sn = ctx->snap + off;
+        hipLaunchKernelGGL(copy_node_fields, dim3(nblk(N * per, 256)), dim3(256), 0, ctx->stream, N, per,
+                           restore ? (const double*)sn : (const double*)cur, restore ? cur : sn, flag, ctx->node_inst);
+        off += per * N;
+    };
+    for (int f = 0; f < 8; ++f) cp(ctx->node[EB_F_U + f], NODE_FIELD_WIDTH[f]);
+    cp(ctx->D, 6);
+    cp(ctx->Dd, 6);
+    cp(ctx->Ddd, 6);
+    return check_launch(ctx, "snapshot_start");
+}
+
+// relative residual of every instance at the current start: assembly with contact, the corrector
+// residual and the boundary-condition lift, as the first Newton iteration sees them (energy not
+// accumulated); per-instance norms into ctx->norms
+int start_residual(eb_ctx* ctx, const eb_newton_params* p, double dt) {
+    EB_TRY(assemble_all(ctx, true, p->alpha, true, dt, p->beta, p->gamma, false));
+    EB_TRY(bc_lift_launch(ctx));
+    NewtonCtl nc{0, 0, 0, 0, 0};
+    return norms_launch(ctx, nc, nullptr);
+}
+
+// after the reference predictor: the network's start from step n, and per instance the better one
+int learned_predictor(eb_ctx* ctx, const eb_newton_params* p, double dt) {
+    const int64_t N = ctx->N;
+    EB_TRY(start_residual(ctx, p, dt));
+    EB_HIP(hipMemcpyAsync(ctx->rn_plain, ctx->norms, sizeof(double) * 2 * ctx->S, hipMemcpyDeviceToDevice, ctx->stream));
+    EB_TRY(snapshot_start(ctx, false, nullptr));
+    // back to step n (the predictor starts from it), then the network's increment
+    for (int f = 0; f < 8; ++f)
+        hipLaunchKernelGGL(copy_node_fields, dim3(nblk(N * NODE_FIELD_WIDTH[f], 256)), dim3(256), 0, ctx->stream, N,
+                           NODE_FIELD_WIDTH[f], (const double*)ctx->node[EB_F_U_N + f], ctx->node[EB_F_U + f],
+                           (const uint8_t*)nullptr, ctx->node_inst);
+    hipLaunchKernelGGL(init_global, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, node_fields(ctx), ctx->D, ctx->Dd,
+                       ctx->Ddd, ctx->dD, 1);
+    hipLaunchKernelGGL(learned_features, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, ctx->pdD, ctx->hrate[0],
+                       ctx->hrate[1], ctx->hrate[2], ctx->Tc_n, dt, ctx->lin_scale, ctx->lX);
+    EB_TRY(check_launch(ctx, "learned_features"));
+    const float* Y = nullptr;
+    if (ebm::mlp_infer(ctx->mlp, ctx->lX, N, ctx->stream, &Y) != EB_OK) {
+        ctx->err = std::string("network inference: ") + eb_mlp_last_error(ctx->mlp);
+        return EB_ERR_DEVICE;
+    }
+    hipLaunchKernelGGL(learned_apply, dim3(nblk(6 * N, 256)), dim3(256), 0, ctx->stream, N, Y, ctx->pdD,
+                       ctx->lout_scale, ctx->is_fixed, ctx->dD);
+    EB_TRY(check_launch(ctx, "learned_apply"));
+    EB_TRY(global_predictor_async(ctx, dt, p->beta, p->gamma));
+    EB_TRY(local_predictor_async(ctx, dt, p->beta, p->gamma));
+    EB_TRY(start_residual(ctx, p, dt));
+    hipLaunchKernelGGL(learned_choose, dim3(1), dim3(256), 0, ctx->stream, (int)ctx->S, ctx->rn_plain, ctx->norms,
+                       ctx->use_plain, ctx->pred_counts);
+    EB_TRY(check_launch(ctx, "learned_choose"));
+    return snapshot_start(ctx, true, ctx->use_plain);
 }
 
 int field_info(eb_ctx* ctx, int f, double*& ptr, int64_t& len, int& kind) {
@@ -700,6 +802,20 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
         CK(dalloc(ctx, ctx->nall, 1));
         CK(dalloc(ctx, ctx->kall, 1));
     }
+    for (int k = 0; k < 3; ++k) CK(dalloc(ctx, ctx->hrate[k], 6 * N));
+    CK(dalloc(ctx, ctx->pdD, 6 * N));
+    CK(dalloc(ctx, ctx->snap, 54 * N));
+    CK(dalloc(ctx, ctx->rn_plain, 2 * S));
+    CK(dalloc(ctx, ctx->use_plain, S));
+    CK(dalloc(ctx, ctx->pred_counts, 2));
+    CK(dalloc(ctx, ctx->lX, LEARN_IN * N));
+    CK(dalloc(ctx, ctx->lT, LEARN_OUT * N));
+    {
+        std::vector<float> one(LEARN_IN, 1.0f);
+        CK(upload_vec(ctx, ctx->ones, one));
+        CK(dalloc(ctx, ctx->lin_scale, LEARN_IN));
+        CK(dalloc(ctx, ctx->lout_scale, LEARN_OUT));
+    }
     CK(dalloc(ctx, ctx->partials, std::max<int64_t>(8 * (int64_t)ctx->nchunks, 8 * RED_MAX_GRID)));
     CK(dalloc(ctx, ctx->kctl, S));
     CK(dalloc(ctx, ctx->kstatus, 1));
@@ -970,27 +1086,6 @@ int eb_initialize_global_predictor(eb_ctx* ctx) {
     return EB_OK;
 }
 
-static int global_corrector_async(eb_ctx* ctx, double dt, double b, double g) {
-    hipLaunchKernelGGL(global_corrector, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs,
-                       g / (b * dt), 1 / (b * (dt * dt)), ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, active_dev(ctx));
-    return check_launch(ctx, "global_corrector");
-}
-static int local_corrector_async(eb_ctx* ctx, double dt, double b, double g) {
-    hipLaunchKernelGGL(local_corrector, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx),
-                       ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, hht_consts(dt, b, g), active_dev(ctx));
-    return check_launch(ctx, "local_corrector");
-}
-static int global_predictor_async(eb_ctx* ctx, double dt, double b, double g) {
-    hipLaunchKernelGGL(global_predictor, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs, dt, g / b,
-                       g / (b * dt), 1 / (b * (dt * dt)), dt * (2 * b - g) / (2 * b), ctx->D, ctx->Dd, ctx->Ddd, ctx->dD);
-    return check_launch(ctx, "global_predictor");
-}
-static int local_predictor_async(eb_ctx* ctx, double dt, double b, double g) {
-    hipLaunchKernelGGL(local_predictor, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx),
-                       ctx->D, ctx->Dd, ctx->Ddd, hht_consts(dt, b, g));
-    return check_launch(ctx, "local_predictor");
-}
-
 int eb_compute_global_corrector(eb_ctx* ctx, double dt, double b, double g) {
     if (!ctx) return EB_ERR_ARG;
     EB_TRY(global_corrector_async(ctx, dt, b, g));
@@ -1062,6 +1157,7 @@ int eb_update_boundary_conditions(eb_ctx* ctx, const double* values, int64_t n_f
 
 int eb_initialize_state_simulation(eb_ctx* ctx) {
     if (!ctx) return EB_ERR_ARG;
+    ctx->hist_count = 0;
     EB_TRY(assemble_all(ctx, false, 0.0, false, 1.0, 1.0, 1.0));
     size_t b = sizeof(double) * ctx->nnz;
     EB_HIP(hipMemcpyAsync(ctx->Kn, ctx->K, b, hipMemcpyDeviceToDevice, ctx->stream));
@@ -1073,6 +1169,7 @@ int eb_initialize_state_simulation(eb_ctx* ctx) {
 
 int eb_predictor(eb_ctx* ctx, const eb_newton_params* p, double dt) {
     if (!ctx || !p) return EB_ERR_ARG;
+    ctx->last_dt = dt;
     hipLaunchKernelGGL(init_global, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx), ctx->D,
                        ctx->Dd, ctx->Ddd, ctx->dD, 1);
     EB_TRY(tangent_residuals_predictor_async(ctx, dt, p->alpha, p->beta, p->gamma));
@@ -1080,6 +1177,9 @@ int eb_predictor(eb_ctx* ctx, const eb_newton_params* p, double dt) {
     EB_TRY(solve_free(ctx, p, nullptr));
     EB_TRY(global_predictor_async(ctx, dt, p->beta, p->gamma));
     EB_TRY(local_predictor_async(ctx, dt, p->beta, p->gamma));
+    // the reference predictor's increment: an input of the learned start and of its samples
+    EB_HIP(hipMemcpyAsync(ctx->pdD, ctx->dD, sizeof(double) * ctx->ndofs, hipMemcpyDeviceToDevice, ctx->stream));
+    if (ctx->mlp && ctx->hist_count >= 3) EB_TRY(learned_predictor(ctx, p, dt));
     EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
 }
@@ -1127,6 +1227,15 @@ int eb_solve_step_dynamics(eb_ctx* ctx, const eb_newton_params* p, double dt, in
 
 int eb_update_converged(eb_ctx* ctx) {
     if (!ctx) return EB_ERR_ARG;
+    // rate of the step that converged, newest first in the history of the learned warm start
+    if (ctx->last_dt > 0) {
+        std::swap(ctx->hrate[1], ctx->hrate[2]);
+        std::swap(ctx->hrate[0], ctx->hrate[1]);
+        hipLaunchKernelGGL(learned_record, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N,
+                           ctx->node[EB_F_U], ctx->node[EB_F_U_N], ctx->node[EB_F_W], 1.0 / ctx->last_dt, ctx->hrate[0]);
+        EB_TRY(check_launch(ctx, "learned_record"));
+        ctx->hist_count = std::min(ctx->hist_count + 1, 3);
+    }
     for (int f = 0; f < 8; ++f)
         EB_HIP(hipMemcpyAsync(ctx->node[EB_F_U_N + f], ctx->node[EB_F_U + f], sizeof(double) * (f >= 6 ? 9 : 3) * ctx->N,
                               hipMemcpyDeviceToDevice, ctx->stream));
@@ -1158,6 +1267,44 @@ int eb_update_not_converged(eb_ctx* ctx) {
     EB_HIP(hipMemcpyAsync(ctx->fext, ctx->fext_n, vb, hipMemcpyDeviceToDevice, ctx->stream));
     ctx->np1_alias = true;
     EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_learned_sample(eb_ctx* ctx, float* X, float* T) {
+    if (!ctx || !X || !T) return EB_ERR_ARG;
+    if (ctx->hist_count < 3 || ctx->last_dt <= 0) { ctx->err = "fewer than three converged steps recorded"; return EB_ERR_STATE; }
+    const int64_t N = ctx->N;
+    hipLaunchKernelGGL(learned_features, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, ctx->pdD, ctx->hrate[0],
+                       ctx->hrate[1], ctx->hrate[2], ctx->Tc_n, ctx->last_dt, ctx->ones, ctx->lX);
+    hipLaunchKernelGGL(learned_targets, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, ctx->node[EB_F_U],
+                       ctx->node[EB_F_U_N], ctx->node[EB_F_W], ctx->pdD, ctx->ones, ctx->lT);
+    EB_TRY(check_launch(ctx, "learned_sample"));
+    EB_HIP(hipMemcpyAsync(X, ctx->lX, sizeof(float) * LEARN_IN * N, hipMemcpyDeviceToHost, ctx->stream));
+    EB_HIP(hipMemcpyAsync(T, ctx->lT, sizeof(float) * LEARN_OUT * N, hipMemcpyDeviceToHost, ctx->stream));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_set_learned_predictor(eb_ctx* ctx, eb_mlp* mlp, const float* in_scale, const float* out_scale) {
+    if (!ctx) return EB_ERR_ARG;
+    if (!mlp) { ctx->mlp = nullptr; return EB_OK; }
+    if (!in_scale || !out_scale || ebm::mlp_width(mlp, 0) != LEARN_IN || ebm::mlp_width(mlp, 1) != LEARN_OUT) {
+        ctx->err = "the network must map 27 inputs to 6 outputs";
+        return EB_ERR_ARG;
+    }
+    EB_HIP(hipMemcpyAsync(ctx->lin_scale, in_scale, sizeof(float) * LEARN_IN, hipMemcpyHostToDevice, ctx->stream));
+    EB_HIP(hipMemcpyAsync(ctx->lout_scale, out_scale, sizeof(float) * LEARN_OUT, hipMemcpyHostToDevice, ctx->stream));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->mlp = mlp;
+    return EB_OK;
+}
+
+int eb_get_predictor_stats(const eb_ctx* ctx, int64_t* out2) {
+    if (!ctx || !out2) return EB_ERR_ARG;
+    int32_t c[2] = {0, 0};
+    if (hipMemcpy(c, ctx->pred_counts, sizeof c, hipMemcpyDeviceToHost) != hipSuccess) return EB_ERR_DEVICE;
+    out2[0] = c[0];
+    out2[1] = c[1];
     return EB_OK;
 }
 

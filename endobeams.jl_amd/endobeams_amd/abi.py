@@ -110,12 +110,34 @@ PROTOTYPES = {
 
 KERNELS = {"beam": 0, "contact": 1, "gather": 2, "precond": 3, "krylov": 4, "update": 5}
 
+# include/endobeams_learned.h (network handle is a void* here)
+_fp = C.POINTER(C.c_float)
+LEARNED_PROTOTYPES = {
+    "mlp_create": (_i, [C.POINTER(C.c_int32), C.c_int32, C.c_uint64, C.POINTER(_vp)]),
+    "mlp_destroy": (_i, [_vp]),
+    "mlp_last_error": (C.c_char_p, [_vp]),
+    "mlp_num_params": (_i, [_vp, _lp]),
+    "mlp_get_params": (_i, [_vp, _fp, C.c_int64]),
+    "mlp_set_params": (_i, [_vp, _fp, C.c_int64]),
+    "mlp_forward": (_i, [_vp, _fp, C.c_int64, _fp]),
+    "mlp_loss_gradient": (_i, [_vp, _fp, _fp, C.c_int64, _fp, _fp]),
+    "mlp_set_dataset": (_i, [_vp, _fp, _fp, C.c_int64]),
+    "mlp_train": (_i, [_vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _fp]),
+    "learned_sample": (_i, [_vp, _fp, _fp]),
+    "set_learned_predictor": (_i, [_vp, _vp, _fp, _fp]),
+    "get_predictor_stats": (_i, [_vp, _lp]),
+}
+LEARNED_INPUTS, LEARNED_OUTPUTS = 27, 6
+
 
 def bind(lib, prefix):
     """Attach restype/argtypes for every ABI function found in `lib` under `prefix`; returns
     {short_name: function}. Raises if any declared function is missing."""
     out = {}
-    for name, (res, args) in PROTOTYPES.items():
+    protos = dict(PROTOTYPES)
+    if hasattr(lib, prefix + "mlp_create"):
+        protos.update(LEARNED_PROTOTYPES)
+    for name, (res, args) in protos.items():
         fn = getattr(lib, prefix + name)  # AttributeError if the symbol is absent
         fn.restype = res
         fn.argtypes = args

@@ -120,17 +120,18 @@ def reference_net(positions, connectivity, sphere=True, sdf_n=None, simulation_e
 # benchmark structures
 # ---------------------------------------------------------------------------------------------
 
-def l_cantilever(tol=1e-8):
+def l_cantilever(tol=1e-8, load=50.0):
     """Config 1: L-shaped cantilever, two arms of length 10 (along x, then y), 10 beams per arm
     (21 nodes), r = 0.1, E = 1e6, ν = 0.3, ρ = 1, node 1 clamped, F_z at the tip = 50 t for
-    t <= 1 and 0 after, α = -0.05, Δt = 0.25 fixed, t_end = 30, tolerances 1e-8."""
+    t <= 1 and 0 after, α = -0.05, Δt = 0.25 fixed, t_end = 30, tolerances 1e-8. `load` replaces
+    the peak tip force 50 (variants for training the learned warm start)."""
     pts = [(float(i), 0.0, 0.0) for i in range(11)] + [(10.0, float(j), 0.0) for j in range(1, 11)]
     positions = np.array(pts)
     conn = _chain(21)
     nodes = NodesBeams(positions)
     beams = Beams(nodes, conn, 1e6, 0.3, 1.0, 0.1, 0.0)
     tip_z = 6 * 20 + 3
-    cf = ConcentratedForce(lambda t, i: 50.0 * t if t <= 1.0 + 1e-12 else 0.0, [tip_z])
+    cf = ConcentratedForce(lambda t, i: load * t if t <= 1.0 + 1e-12 else 0.0, [tip_z])
     bcs = BoundaryConditions(126, encastre=Encastre(range(1, 7)))
     conf = BeamsConfiguration(nodes, beams, Loads(cf), bcs)
     params = SimulationParams.hht(-0.05, initial_timestep=0.25, min_timestep=1e-6, max_timestep=0.25,

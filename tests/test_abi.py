@@ -9,12 +9,13 @@ import pytest
 from conftest import REPO
 
 HEADER = os.path.join(REPO, "include", "endobeams_hip.h")
+LEARNED = os.path.join(REPO, "include", "endobeams_learned.h")
 LIB = os.path.join(REPO, "endobeams.jl_amd", "lib", "libendobeams_hip.so")
 ORACLE = os.path.join(REPO, "oracle", "liboracle.so")
 
 
-def declared():
-    text = open(HEADER).read()
+def declared(path=HEADER):
+    text = open(path).read()
     return sorted(set(re.findall(r"^(?:int|const char\*)\s+(eb_\w+)\(", text, flags=re.M)))
 
 
@@ -29,7 +30,7 @@ def test_header_declares_the_entry_points():
 @pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(LIB)
-    missing = [n for n in declared() if not hasattr(lib, n)]
+    missing = [n for n in declared() + declared(LEARNED) if not hasattr(lib, n)]
     assert not missing, missing
 
 
@@ -46,3 +47,5 @@ def test_python_bindings_cover_the_abi():
     from endobeams_amd import abi
     bound = {"eb_" + k for k in abi.PROTOTYPES}
     assert bound == set(declared())
+    assert {"eb_" + k for k in abi.LEARNED_PROTOTYPES} == set(declared(LEARNED))
+    assert len(declared(LEARNED)) == 13

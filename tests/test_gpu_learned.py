@@ -1,0 +1,120 @@
+"""The learned warm start on the device: the network's forward pass and gradient against the
+numpy restatement (oracle/mlp.py), training on the device, and the solver with the network
+against the solver without it (same tolerance, fewer or equal Newton iterations)."""
+import numpy as np
+import pytest
+
+import mlp as M
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+SMALL = (27, 32, 16, 6)
+
+
+def test_forward_matches_numpy():
+    from endobeams_amd.learned import WIDTHS, Network
+    rng = np.random.default_rng(0)
+    for widths in (SMALL, WIDTHS):
+        net = Network(widths, seed=1)
+        p = rng.normal(0, 0.2, net.n_params).astype(np.float32)
+        net.params = p
+        X = rng.normal(size=(130, widths[0])).astype(np.float32)
+        Y = net.forward(X)
+        ref = M.forward(p.astype(np.float64), widths, X.astype(np.float64))[-1]
+        assert rel_err(Y, ref) <= 1e-5, rel_err(Y, ref)
+        net.close()
+
+
+def test_gradient_matches_numpy():
+    from endobeams_amd.learned import WIDTHS, Network
+    rng = np.random.default_rng(2)
+    for widths, rows in ((SMALL, 70), (WIDTHS, 300)):
+        net = Network(widths, seed=3)
+        p = rng.normal(0, 0.2, net.n_params).astype(np.float32)
+        net.params = p
+        X = rng.normal(size=(rows, widths[0])).astype(np.float32)
+        T = rng.normal(size=(rows, widths[-1])).astype(np.float32)
+        loss, g = net.loss_gradient(X, T)
+        lref, gref = M.loss_and_gradient(p.astype(np.float64), widths, X.astype(np.float64), T.astype(np.float64))
+        assert abs(loss - lref) <= 1e-5 * abs(lref)
+        assert rel_err(g, gref) <= 1e-4, rel_err(g, gref)
+        # every layer's block, not only the norm of the whole vector
+        o = 0
+        for i, j in zip(widths[:-1], widths[1:]):
+            n = i * j + j
+            assert rel_err(g[o:o + n], gref[o:o + n]) <= 1e-4
+            o += n
+        net.close()
+
+
+def test_training_fits_a_smooth_map():
+    from endobeams_amd.learned import Network
+    rng = np.random.default_rng(4)
+    X = rng.uniform(-1, 1, size=(8192, 27)).astype(np.float32)
+    A = rng.normal(size=(27, 6)) / np.sqrt(27)
+    T = np.tanh(X @ A).astype(np.float32) + 0.1 * (X[:, :6] ** 2).astype(np.float32)
+    net = Network(SMALL, seed=5)
+    losses = net.train(X, T, epochs=30, batch=512, lr=3e-3, seed=6)
+    print("loss per epoch", losses[[0, 9, 19, 29]])
+    assert losses[-1] < 0.1 * losses[0]
+    net.close()
+
+
+def small_stent(seed):
+    from endobeams_amd import configs
+    return configs.braided_stent(n_wires=6, el_per_wire=32, length=8e-3, sdf_n=64, seed=seed, dt=1.25e-4)
+
+
+def run(build, ws=None, steps=30, zero=False):
+    from endobeams_amd import HipEngine, Stepper
+    conf, params, inter = build
+    g = HipEngine(conf, params, inter)
+    if ws is not None:
+        ws.install(g)
+    st = Stepper(g, params)
+    st.initialize()
+    while st.stats.steps < steps:
+        st.advance()
+    return g, st.stats
+
+
+def test_zero_correction_reproduces_the_reference_predictor():
+    """A network whose output is zero makes the learned start the reference predictor's start
+    (ties keep the network's): the run, with its snapshot/restore and per-instance choice, must be
+    the run without a network, bit for bit."""
+    from endobeams_amd import learned as L
+    net = L.Network(L.WIDTHS, seed=0)
+    net.params = np.zeros(net.n_params, np.float32)
+    ws = L.WarmStart(net, np.ones(27, np.float32), np.ones(6, np.float32))
+    g0, s0 = run(small_stent(31), steps=12)
+    g1, s1 = run(small_stent(31), ws, steps=12)
+    assert s0.iterations_per_step == s1.iterations_per_step and s0.failed_steps == s1.failed_steps
+    for f in ("U_N", "R_N", "UDOT_N", "TINT_N", "TC_N"):
+        assert np.array_equal(g0.get(f), g1.get(f)), f
+    stats = L.predictor_stats(g1)
+    assert stats[1] == 0 and stats[0] == 12 + s1.failed_steps - 3
+
+
+def test_learned_warm_start_against_the_reference_predictor():
+    """Trained on three seeded geometries, run on a fourth: the solver with the network converges
+    every step to the same tolerance; what it gains in Newton iterations is measured and printed
+    (the reference predictor stays the fallback per step, so the loss is bounded)."""
+    from endobeams_amd import HipBatchEngine
+    from endobeams_amd import learned as L
+    train = [small_stent(s) for s in (11, 12, 13)]
+    eng = HipBatchEngine(train)
+    X, T = L.collect(eng, train[0][1], 30)
+    eng.close()
+    ws = L.WarmStart.train(X, T, epochs=40, batch=1024, lr=1e-3, seed=0)
+    print("samples", X.shape, "loss", ws.losses[[0, -1]])
+    assert ws.losses[-1] < 0.5 * ws.losses[0]
+    gp, sp = run(small_stent(21))
+    gl, sl = run(small_stent(21), ws)
+    stats = L.predictor_stats(gl)
+    ip, il = np.mean(sp.iterations_per_step[3:]), np.mean(sl.iterations_per_step[3:])
+    print(f"newton/step: reference predictor {ip:.2f} ({sp.failed_steps} halvings), with the network {il:.2f} "
+          f"({sl.failed_steps} halvings); network start kept {stats[0]} of {stats.sum()}")
+    assert sl.steps == sp.steps == 30
+    assert stats.sum() == 30 + sl.failed_steps - 3
+    assert il <= 1.25 * ip
