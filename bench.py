@@ -6,13 +6,16 @@ tolerance 1e-8, the reference's adaptive Δt), inputs resident on the GPU. Workl
 stent, 24 helical wires × 200 beams (4800 beams, 4824 nodes, 28 944 dofs), crimped to 1.5 mm and
 released in a rigid vessel given as a 256³ FP32 signed distance field.
 
-Two forms are timed in the same run:
-  plain   — FP64 element/contact kernels, one stent per context;
+Three forms are timed in the same run:
+  plain   — FP64 element/contact kernels, one stent per context, the reference predictor;
   fast    — the precision form (FP32 element matrices and contact tangent; forces, residuals,
             rotations, Krylov and updates FP64) on a batch of seeded stent variants advanced in
-            one context (each instance its own vessel SDF).
-`value` is the fast form's whole-job throughput; the plain form is reported beside it with the
-ratio. Also reported: the roofline of the fast form's element kernel (algorithmic flops per
+            one context (each instance its own vessel SDF);
+  learned — the fast form with the learned warm start: a 3×128 FP32 network trained in this run,
+            by the library's own kernels, on trajectories of 8 other seeded stents; the reference
+            predictor stays the per-instance fallback.
+`value` is the learned form's whole-job throughput; the plain and fast forms are reported beside
+it with the ratios. Also reported: the roofline of the fast form's element kernel (algorithmic flops per
 launch ÷ average launch time from HIP events in this run), the CPU restatement's throughput on a
 bounded sample of the plain workload (1 thread) and the fast form's trajectory error against
 that FP64 CPU run at the same step.
@@ -106,7 +109,9 @@ def main():
     ap.add_argument("--sdf-n", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["plain", "fast"], default=None, help="time one form only")
+    ap.add_argument("--only", choices=["plain", "fast", "learned"], default=None, help="time one form only")
+    ap.add_argument("--train-steps", type=int, default=40, help="converged steps recorded per training stent")
+    ap.add_argument("--epochs", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -115,7 +120,9 @@ def main():
 
     from endobeams_amd import HipBatchEngine, HipEngine, abi
 
-    plain = fast = None
+    from endobeams_amd import learned as L
+
+    plain = fast = lrn = None
     if args.only in (None, "plain"):
         conf, params, inter = stent(sdf_n=args.sdf_n)
         eng = HipEngine(conf, params, inter)
@@ -123,23 +130,51 @@ def main():
         print(f"# plain FP64: {plain['steps']} steps, {plain['ms_per_step']:.2f} ms/step, "
               f"{plain['value']:.0f} element-steps/s", file=sys.stderr, flush=True)
         eng.close()
-    if args.only in (None, "fast"):
+    inst = None
+    if args.only in (None, "fast", "learned"):
         t = time.perf_counter()
         inst = [stent(seed=None if s == 0 else s, sdf_n=args.sdf_n) for s in range(args.batch)]
+        print(f"# batch of {args.batch} stents built in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    if args.only in (None, "fast"):
         eng = HipBatchEngine(inst, precision=abi.EB_PRECISION_FP32)
         E = int(eng.beam_off[-1])
-        print(f"# fast form: {args.batch} stents, {E} beams, set up in {time.perf_counter() - t:.1f} s",
-              file=sys.stderr, flush=True)
         fast = timed(eng, inst[0][1], E, args.warmup, args.steps)
         print(f"# fast: {fast['steps']} steps, {fast['ms_per_step']:.2f} ms/step, {fast['value']:.0f} element-steps/s",
               file=sys.stderr, flush=True)
         eng.close()
+    training = None
+    if args.only in (None, "learned"):
+        # training trajectories: 8 seeded stents not in the timed batch, FP32 form, reference predictor
+        t = time.perf_counter()
+        tr = [stent(seed=1000 + s, sdf_n=args.sdf_n) for s in range(8)]
+        teng = HipBatchEngine(tr, precision=abi.EB_PRECISION_FP32)
+        X, T = L.collect(teng, tr[0][1], args.train_steps)
+        teng.close()
+        t_collect = time.perf_counter() - t
+        t = time.perf_counter()
+        ws = L.WarmStart.train(X, T, epochs=args.epochs, batch=4096, lr=1e-3, seed=0)
+        t_train = time.perf_counter() - t
+        training = {"samples": int(X.shape[0]), "epochs": args.epochs, "collect_s": t_collect, "train_s": t_train,
+                    "loss_first_epoch": float(ws.losses[0]), "loss_last_epoch": float(ws.losses[-1])}
+        print(f"# network trained: {X.shape[0]} samples, {t_train:.1f} s, loss {ws.losses[0]:.3f} -> {ws.losses[-1]:.3f}",
+              file=sys.stderr, flush=True)
+        eng = HipBatchEngine(inst, precision=abi.EB_PRECISION_FP32)
+        ws.install(eng)
+        E = int(eng.beam_off[-1])
+        lrn = timed(eng, inst[0][1], E, args.warmup, args.steps)
+        ch = L.predictor_stats(eng)
+        training["network_start_kept"] = int(ch[0])
+        training["fallback_to_reference_predictor"] = int(ch[1])
+        print(f"# learned: {lrn['steps']} steps, {lrn['ms_per_step']:.2f} ms/step, {lrn['value']:.0f} element-steps/s, "
+              f"choices {ch.tolist()}", file=sys.stderr, flush=True)
+        eng.close()
 
-    main_form = fast if fast is not None else plain
+    main_form = lrn if lrn is not None else (fast if fast is not None else plain)
     avg_s = main_form["kernel_ms"] * 1e-3 / max(main_form["kernel_launches"], 1)
     flops = FLOPS_PER_BEAM * main_form["E"]
     achieved = flops / avg_s / 1e12
-    peak = PEAK_FP32_TFLOPS if main_form is fast else PEAK_FP64_TFLOPS
+    reduced = main_form is fast or main_form is lrn
+    peak = PEAK_FP32_TFLOPS if reduced else PEAK_FP64_TFLOPS
     out = {
         "metric": METRIC,
         "value": main_form["value"],
@@ -151,19 +186,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if main_form is fast else "f64",
+        "dtype": "f32+f64" if reduced else "f64",
         "data": "synthetic",
         "config": {"workload": "braided stent: 24 helical wires x 200 beams (4800 beams, 4824 nodes), "
                                f"{args.sdf_n}^3 FP32 vessel SDF, Newton tolerance 1e-8",
-                   "form": (f"FP32 element matrices + FP64 forces/solver, batch of {args.batch} seeded stents"
-                            if main_form is fast else "plain FP64, one stent"),
+                   "form": ("learned warm start + " if main_form is lrn else "") +
+                           (f"FP32 element matrices + FP64 forces/solver, batch of {args.batch} seeded stents"
+                            if reduced else "plain FP64, one stent"),
                    "dt": 1e-3, "tolerance": 1e-8, "beams_per_step": main_form["E"]},
         "newton_iters_per_step": main_form["newton_iters_per_step"],
         "failed_steps": main_form["failed_steps"],
         "wall_s_per_simulated_s": main_form["wall_s_per_simulated_s"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
-                     "kernel": "beam_kernel<float>" if main_form is fast else "beam_kernel<double>",
+                     "kernel": "beam_kernel<float>" if reduced else "beam_kernel<double>",
                      "avg_launch_us": avg_s * 1e6, "launches": main_form["kernel_launches"],
                      "flops_per_launch": flops},
     }
@@ -172,8 +208,13 @@ def main():
                              "newton_iters_per_step": plain["newton_iters_per_step"],
                              "wall_s_per_simulated_s": plain["wall_s_per_simulated_s"],
                              "beam_kernel_us": 1e3 * plain["kernel_ms"] / max(plain["kernel_launches"], 1)}
-        if fast is not None:
-            out["speedup_vs_plain_fp64"] = fast["value"] / plain["value"]
+        out["speedup_vs_plain_fp64"] = main_form["value"] / plain["value"]
+    if fast is not None and main_form is not fast:
+        out["fp32_batch_without_network"] = {"value": fast["value"], "ms_per_step": fast["ms_per_step"],
+                                             "newton_iters_per_step": fast["newton_iters_per_step"],
+                                             "failed_steps": fast["failed_steps"]}
+    if training is not None:
+        out["learned_training"] = training
     if not args.no_cpu:
         cb, oracle_eng, n_cpu = cpu_baseline(args.cpu_seconds)
         out["cpu_baseline"] = cb

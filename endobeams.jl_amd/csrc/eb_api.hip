@@ -87,6 +87,7 @@ struct eb_ctx {
     // matrices (block-row order) and vectors
     double *K = nullptr, *C = nullptr, *M = nullptr, *Kn = nullptr, *Cn = nullptr, *Mn = nullptr, *Ktan = nullptr;
     bool np1_alias = false;  // K,C,M at n+1 currently equal the step-n matrices (after a swap)
+    bool kcm_pending = false; // K,C,M at n+1 not yet gathered from the element staging of the last assembly
     double *fext = nullptr, *Tint = nullptr, *Tk = nullptr, *Tc = nullptr;
     double *fext_n = nullptr, *Tint_n = nullptr, *Tk_n = nullptr, *Tc_n = nullptr;
     double *D = nullptr, *Dd = nullptr, *Ddd = nullptr, *res = nullptr, *dD = nullptr;
@@ -105,6 +106,7 @@ struct eb_ctx {
     int* h_status = nullptr;
     // contact
     int32_t* cbeams = nullptr;
+    uint8_t* is_contact = nullptr;  // [E]
     uint64_t* clist = nullptr;     // beams found in contact, (beam << 8 | Gauss point mask)
     int32_t* clist_n = nullptr;
     // chains
@@ -241,8 +243,23 @@ int check_launch(eb_ctx* ctx, const char* what) {
     return EB_OK;
 }
 
+// K, C, M at n+1 from the element staging of the last assembly, if that assembly left them pending
+template <typename P> int materialize_kcm_t(eb_ctx* ctx) {
+    hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz, ctx->blk_ptr,
+                       ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM, ctx->K, ctx->C, ctx->M,
+                       ctx->Ktan, 0.0, 0.0, 0.0, 0, 1, active_dev(ctx));
+    return check_launch(ctx, "gather_matrices");
+}
+int materialize_kcm(eb_ctx* ctx) {
+    if (!ctx->kcm_pending) return EB_OK;
+    ctx->kcm_pending = false;
+    ctx->np1_alias = false;
+    return ctx->precision == EB_PRECISION_FP32 ? materialize_kcm_t<float>(ctx) : materialize_kcm_t<double>(ctx);
+}
+
 // materialise K,C,M at n+1 if they are currently aliased to step n
 int unalias(eb_ctx* ctx) {
+    EB_TRY(materialize_kcm(ctx));
     if (!ctx->np1_alias) return EB_OK;
     size_t b = sizeof(double) * ctx->nnz;
     EB_HIP(hipMemcpyAsync(ctx->K, ctx->Kn, b, hipMemcpyDeviceToDevice, ctx->stream));
@@ -263,7 +280,7 @@ template <typename P> int launch_beams(eb_ctx* ctx) {
     return check_launch(ctx, "beam_kernel");
 }
 
-template <typename P> int launch_contact(eb_ctx* ctx, double alpha) {
+template <typename P> ContactArgs<P> contact_args(eb_ctx* ctx, double alpha) {
     ContactArgs<P> ca;
     ca.surface = ctx->surface;
     ca.sphere = ctx->d_sphere;
@@ -275,10 +292,10 @@ template <typename P> int launch_contact(eb_ctx* ctx, double alpha) {
     ca.n = ctx->n_contact;
     ca.Tc = (double*)ctx->sTc;
     ca.energy = ctx->senergy;
-    hipLaunchKernelGGL(contact_reset, dim3(1), dim3(1), 0, ctx->stream, ctx->clist_n);
-    hipLaunchKernelGGL(contact_detect<P>, dim3(nblk(ctx->n_contact, 256)), dim3(256), 0, ctx->stream, node_state(ctx),
-                       beam_consts(ctx), ctx->gauss, ca, ctx->clist, ctx->clist_n);
-    EB_TRY(check_launch(ctx, "contact_detect"));
+    return ca;
+}
+
+template <typename P> int launch_contact_tangent(eb_ctx* ctx, const ContactArgs<P>& ca) {
     prof_begin(ctx, EB_KERNEL_CONTACT);
     hipLaunchKernelGGL(contact_kernel<P>, dim3(nblk(ctx->n_contact, CONTACT_TILE)), dim3(4 * CONTACT_TILE), 0, ctx->stream,
                        node_state(ctx), beam_consts(ctx), beam_stage<P>(ctx), ctx->gauss, ca, ctx->clist, ctx->clist_n);
@@ -286,14 +303,41 @@ template <typename P> int launch_contact(eb_ctx* ctx, double alpha) {
     return check_launch(ctx, "contact_kernel");
 }
 
+// element kernel with the contact detection in its static part, then the contact tangent
+template <typename P> int launch_beams_contact(eb_ctx* ctx, double alpha) {
+    const ContactArgs<P> ca = contact_args<P>(ctx, alpha);
+    hipLaunchKernelGGL(contact_reset, dim3(1), dim3(1), 0, ctx->stream, ctx->clist_n);
+    prof_begin(ctx, EB_KERNEL_BEAM);
+    hipLaunchKernelGGL(beam_detect_kernel<P>, dim3(nblk(ctx->E, BEAM_TILE)), dim3(4 * BEAM_TILE), 0, ctx->stream,
+                       node_state(ctx), beam_consts(ctx), beam_stage<P>(ctx), ctx->gauss, ctx->active, ctx->beam_inst, ca,
+                       ctx->is_contact, ctx->clist, ctx->clist_n);
+    prof_end(ctx, EB_KERNEL_BEAM);
+    ctx->counters[3]++;
+    EB_TRY(check_launch(ctx, "beam_detect_kernel"));
+    return launch_contact_tangent<P>(ctx, ca);
+}
+
+// contact detection and tangent on the staging of the last element assembly
+template <typename P> int launch_contact(eb_ctx* ctx, double alpha) {
+    const ContactArgs<P> ca = contact_args<P>(ctx, alpha);
+    hipLaunchKernelGGL(contact_reset, dim3(1), dim3(1), 0, ctx->stream, ctx->clist_n);
+    hipLaunchKernelGGL(contact_detect<P>, dim3(nblk(ctx->n_contact, 256)), dim3(256), 0, ctx->stream, node_state(ctx),
+                       beam_consts(ctx), ctx->gauss, ca, ctx->clist, ctx->clist_n);
+    EB_TRY(check_launch(ctx, "contact_detect"));
+    return launch_contact_tangent<P>(ctx, ca);
+}
+
 template <typename P> int launch_gathers(eb_ctx* ctx, bool with_contact, bool tangent, double dt, double a, double b,
                                          double g, bool residual, int energy_mask) {
     double c1 = 1 + a, c2 = tangent ? g / (b * dt) : 0, c3 = tangent ? 1 / (b * (dt * dt)) : 0;
     prof_begin(ctx, EB_KERNEL_GATHER);
+    // inside the Newton iteration only Ktan is needed: K, C, M at n+1 are gathered from the element
+    // staging when they are first read or when the step is committed (materialize_kcm)
     hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz, ctx->blk_ptr,
                        ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM, ctx->K, ctx->C, ctx->M,
-                       ctx->Ktan, c1, c2, c3, tangent ? 1 : 0, active_dev(ctx));
+                       ctx->Ktan, c1, c2, c3, tangent ? 1 : 0, tangent ? 0 : 1, active_dev(ctx));
     EB_TRY(check_launch(ctx, "gather_matrices"));
+    ctx->kcm_pending = tangent;
     hipLaunchKernelGGL(gather_vectors<P>, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs,
                        ctx->inc_ptr, ctx->inc_src, (const double*)ctx->sTint, (const double*)ctx->sTk, (const double*)ctx->sTc,
                        with_contact ? 1 : 0, ctx->Tint, ctx->Tk, ctx->Tc, residual ? 1 : 0, a, ctx->fext, ctx->fext_n,
@@ -314,12 +358,10 @@ int assemble_all(eb_ctx* ctx, bool contact, double alpha, bool fused, double dt,
     bool wc = contact && ctx->surface != EB_SURFACE_NONE;
     const int emask = energy ? (wc ? 3 : 1) : 0;
     if (ctx->precision == EB_PRECISION_FP32) {
-        EB_TRY(launch_beams<float>(ctx));
-        if (wc) EB_TRY(launch_contact<float>(ctx, alpha));
+        EB_TRY(wc ? launch_beams_contact<float>(ctx, alpha) : launch_beams<float>(ctx));
         return launch_gathers<float>(ctx, wc, fused, dt, alpha, b, g, fused, emask);
     }
-    EB_TRY(launch_beams<double>(ctx));
-    if (wc) EB_TRY(launch_contact<double>(ctx, alpha));
+    EB_TRY(wc ? launch_beams_contact<double>(ctx, alpha) : launch_beams<double>(ctx));
     return launch_gathers<double>(ctx, wc, fused, dt, alpha, b, g, fused, emask);
 }
 
@@ -385,37 +427,33 @@ int solve_free(eb_ctx* ctx, const eb_newton_params* p, int* its_out) {
                        ctx->active, ctx->kall);
     EB_TRY(check_launch(ctx, "krylov_init"));
     int launched = 0, batch = 1;
-    while (launched < maxit) {
+    for (;;) {
+        // every poll brings the all-instances record and the preconditioner's pivot status
+        EB_HIP(hipMemcpyAsync(ctx->h_kall, ctx->kall, sizeof(KrylovAll), hipMemcpyDeviceToHost, ctx->stream));
+        EB_HIP(hipMemcpyAsync(ctx->h_status, ctx->kstatus, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        if (launched > 0 || maxit == 0) {
+            EB_HIP(hipStreamSynchronize(ctx->stream));
+            if (ctx->h_kall->all_done || launched >= maxit || *ctx->h_status) break;
+        }
         int m = std::min(batch, maxit - launched);
         for (int k = 0; k < m; ++k) EB_TRY(krylov_iteration(ctx));
         launched += m;
-        EB_HIP(hipMemcpyAsync(ctx->h_kall, ctx->kall, sizeof(KrylovAll), hipMemcpyDeviceToHost, ctx->stream));
-        EB_HIP(hipStreamSynchronize(ctx->stream));
-        if (ctx->h_kall->all_done) break;
         batch = std::min(batch * 2, 16);
     }
-    EB_HIP(hipMemcpyAsync(ctx->h_kctl, ctx->kctl, sizeof(KrylovCtl) * ctx->S, hipMemcpyDeviceToHost, ctx->stream));
-    EB_HIP(hipMemcpyAsync(ctx->h_status, ctx->kstatus, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    EB_HIP(hipStreamSynchronize(ctx->stream));
-    int its = 0;
-    for (int64_t s = 0; s < ctx->S; ++s) its = std::max(its, ctx->h_kctl[s].iters);
-    ctx->counters[1] += its;
+    const KrylovAll& ka = *ctx->h_kall;
+    ctx->counters[1] += ka.max_iters;
     ctx->counters[2] += 1;
-    if (its_out) *its_out = its;
+    if (its_out) *its_out = ka.max_iters;
     if (*ctx->h_status) { ctx->err = "singular 6x6 pivot block in the chain preconditioner"; return EB_ERR_SOLVE; }
-    for (int64_t s = 0; s < ctx->S; ++s) {
-        const KrylovCtl& k = ctx->h_kctl[s];
-        bool ok = k.done && (k.bnorm2 == 0.0 || k.rr2 <= k.tol2 * k.bnorm2 * 1.0000001 || k.ss2 <= k.tol2 * k.bnorm2);
-        if (ok || k.iters == 0) continue;   // iters == 0 and done: zero right-hand side or frozen instance
-        // accept a solve that stalled at rounding level; fail otherwise
-        double rel = k.bnorm2 > 0 ? std::sqrt(std::min(k.rr2, k.ss2 > 0 ? k.ss2 : k.rr2) / k.bnorm2) : 0.0;
-        if (!(rel < 1e-6) || !std::isfinite(rel)) {
-            char buf[200];
-            snprintf(buf, sizeof buf, "BiCGStab did not converge (instance %lld): %d iterations, relative residual %.3e%s",
-                     (long long)s, k.iters, rel, k.breakdown ? " (breakdown)" : "");
-            ctx->err = buf;
-            return EB_ERR_SOLVE;
-        }
+    if (!ka.all_done || ka.fail_inst >= 0) {
+        char buf[200];
+        if (ka.fail_inst >= 0)
+            snprintf(buf, sizeof buf, "BiCGStab did not converge (instance %d): relative residual %.3e%s", ka.fail_inst,
+                     ka.fail_rel, ka.breakdown ? " (breakdown)" : "");
+        else
+            snprintf(buf, sizeof buf, "BiCGStab did not converge in %d iterations", ka.max_iters);
+        ctx->err = buf;
+        return EB_ERR_SOLVE;
     }
     hipLaunchKernelGGL(scatter_solution, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs,
                        ctx->is_fixed, ctx->kx, ctx->dD, active_dev(ctx));
@@ -562,6 +600,7 @@ int field_info(eb_ctx* ctx, int f, double*& ptr, int64_t& len, int& kind) {
     }
     kind = 1;
     len = ctx->nnz;
+    if (f >= EB_F_K && f <= EB_F_M) EB_TRY(materialize_kcm(ctx));
     switch (f) {
         case EB_F_K: ptr = ctx->np1_alias ? ctx->Kn : ctx->K; return EB_OK;
         case EB_F_C: ptr = ctx->np1_alias ? ctx->Cn : ctx->C; return EB_OK;
@@ -842,6 +881,7 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
         }
         ctx->n_contact = d->n_contact_beams;
         CK(upload_vec(ctx, ctx->cbeams, cb));
+        CK(upload_vec(ctx, ctx->is_contact, seen));
         CK(dalloc(ctx, ctx->clist, ctx->n_contact));
         CK(dalloc(ctx, ctx->clist_n, 1));
         if (d->surface == EB_SURFACE_SPHERE) {
@@ -1244,6 +1284,7 @@ int eb_update_converged(eb_ctx* ctx) {
     EB_HIP(hipMemcpyAsync(ctx->Tk_n, ctx->Tk, vb, hipMemcpyDeviceToDevice, ctx->stream));
     EB_HIP(hipMemcpyAsync(ctx->Tc_n, ctx->Tc, vb, hipMemcpyDeviceToDevice, ctx->stream));
     EB_HIP(hipMemcpyAsync(ctx->fext_n, ctx->fext, vb, hipMemcpyDeviceToDevice, ctx->stream));
+    EB_TRY(materialize_kcm(ctx));
     if (!ctx->np1_alias) {
         // matricesⁿ = matricesⁿ⁺¹ by exchanging buffers; n+1 now aliases n until the next assembly
         std::swap(ctx->K, ctx->Kn);
@@ -1265,6 +1306,7 @@ int eb_update_not_converged(eb_ctx* ctx) {
     EB_HIP(hipMemcpyAsync(ctx->Tk, ctx->Tk_n, vb, hipMemcpyDeviceToDevice, ctx->stream));
     EB_HIP(hipMemcpyAsync(ctx->Tc, ctx->Tc_n, vb, hipMemcpyDeviceToDevice, ctx->stream));
     EB_HIP(hipMemcpyAsync(ctx->fext, ctx->fext_n, vb, hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->kcm_pending = false;
     ctx->np1_alias = true;
     EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;

@@ -201,20 +201,24 @@ EBD StaticOut beam_frame(int64_t e, const NodeState& ns, const BeamConsts& bc) {
 namespace ebd {
 
 // Parts of the dynamic computation, each run by its own wavefront of a workgroup
-// (PART_MASS: M, Tᵏ, kinetic energy; PART_C13: Cᵏ block columns 1 and 3; PART_C24: columns 2, 4).
+// (PART_MASS: Tᵏ, kinetic energy and, in the FP64 form, M; PART_C13: Cᵏ block columns 1 and 3 and,
+// in the precision form, M; PART_C24: columns 2, 4).
 // Every part evaluates the shared Gauss-point kinematics; the compiler keeps only what the part's
 // outputs need, so each wavefront carries a fraction of the accumulators of the whole element.
 enum { PART_STATIC = 0, PART_MASS = 1, PART_C13 = 2, PART_C24 = 3 };
+// outputs a dynamic part can be asked for
+enum { DYN_TK = 1, DYN_M = 2, DYN_C13 = 4, DYN_C24 = 8 };
 
 // M blocks after l₀/2 Rₑ(·)Rₑᵀ and before the Tₛ⁻¹ post-multiplication, kept by the mass part
 // for the damping term.
 template <typename P> struct MassBlocks { M3<P> M11, M12, M13, M14, M22, M23, M24, M33, M34, M44; };
 
 // Dynamic part (inertia, gyroscopic and damping forces, M and Cᵏ), compute_beams.jl:161-517.
-template <typename P, typename S, int PART>
+template <typename P, typename S, int WHAT>
 EBD void beam_dynamic(int64_t e, const NodeState& ns, const BeamConsts& bc, const BeamStage<S>& st,
                       const GaussPts& gp, const StaticOut& so, MassBlocks<P>& mb) {
-    constexpr bool MASS = PART == PART_MASS, CA = PART == PART_C13, CB = PART == PART_C24;
+    constexpr bool TK = (WHAT & DYN_TK) != 0, WM = (WHAT & DYN_M) != 0;
+    constexpr bool CA = (WHAT & DYN_C13) != 0, CB = (WHAT & DYN_C24) != 0;
     const int64_t E = bc.E;
     const int32_t na = bc.n1[e], nb = bc.n2[e];
     const M3<P> I = eye<P>();
@@ -307,7 +311,7 @@ EBD void beam_dynamic(int64_t e, const NodeState& ns, const BeamConsts& bc, cons
         M3<P> SW0Ir = skew(W0) * Ir;
         V3<P> X = Ir * ReTwdd0 + SW0Ir * W0;
         M3<P> AH11 = Ar * tr(H11), AH12 = Ar * tr(H12), AH14 = Ar * tr(H14);
-        if constexpr (MASS) {
+        if constexpr (TK) {
         V3<P> Tk1G = w * (AH11 * ReTudd0 + tmul(H21, X));
         Tk1 += Tk1G;
         Tk2 += w * (AH12 * ReTudd0 + tmul(H22, X));
@@ -321,6 +325,10 @@ EBD void beam_dynamic(int64_t e, const NodeState& ns, const BeamConsts& bc, cons
             Tk3 += (-Td1G) + ((w * Ar) * damping) * h1;
             Tk4 += (w * damping) * (AH14 * h1 + tmul(H24, Irh2));
         }
+        M3<P> Irg = (Re * Ir) * ReT;
+        kinetic += (w / P(2)) * (Ar * dot(ud0, ud0) + dot(wd0, Irg * wd0));
+        }
+        if constexpr (WM) {
         M3<P> tH21Ir = tr(H21) * Ir, tH22Ir = tr(H22) * Ir, tH24Ir = tr(H24) * Ir;
         M3<P> M11G = w * (AH11 * H11 + tH21Ir * H21);
         M3<P> M12G = w * (AH11 * H12 + tH21Ir * H22);
@@ -335,8 +343,6 @@ EBD void beam_dynamic(int64_t e, const NodeState& ns, const BeamConsts& bc, cons
         M33 += M11G + (w * Ar) * ((I - H11) - tr(H11));
         M34 += (-M14G) + (w * Ar) * H14;
         M44 += w * (AH14 * H14 + tH24Ir * H24);
-        M3<P> Irg = (Re * Ir) * ReT;
-        kinetic += (w / P(2)) * (Ar * dot(ud0, ud0) + dot(wd0, Irg * wd0));
         }
         M3<P> SW0IrmS = SW0Ir - skew(Ir * W0);
         M3<P> AC1 = Ar * (C1_1 + C3_1), AC2 = Ar * (C1_2 + C3_2), AC4 = Ar * (C1_4 + C3_4);
@@ -368,12 +374,14 @@ EBD void beam_dynamic(int64_t e, const NodeState& ns, const BeamConsts& bc, cons
     // exact Tₛ⁻¹(log ΔR)ᵀ post-multiplication of block rows 2 and 4 (compute_beams.jl:480-506)
     const M3<P> TT1t = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, na)))));
     const M3<P> TT2t = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, nb)))));
-    if constexpr (MASS) {
+    if constexpr (TK) {
         st.energy[3 * e + 1] = double(l02 * kinetic);
         st_vec(st.Tk, E, e, 0, l02Re * Tk1);
         st_vec(st.Tk, E, e, 1, l02Re * Tk2);
         st_vec(st.Tk, E, e, 2, l02Re * Tk3);
         st_vec(st.Tk, E, e, 3, l02Re * Tk4);
+    }
+    if constexpr (WM) {
         M11 = (l02Re * M11) * ReT; M12 = (l02Re * M12) * ReT; M13 = (l02Re * M13) * ReT; M14 = (l02Re * M14) * ReT;
         M22 = (l02Re * M22) * ReT; M23 = (l02Re * M23) * ReT; M24 = (l02Re * M24) * ReT;
         M33 = (l02Re * M33) * ReT; M34 = (l02Re * M34) * ReT; M44 = (l02Re * M44) * ReT;
@@ -436,32 +444,54 @@ constexpr int BEAM_TILE = 64;
 // S: storage type of the element matrices (double: the FP64 form; float: the precision form, whose
 // Coriolis parts also compute in FP32). The static part always computes in FP64 and the mass part
 // too, because it yields the inertia forces Tᵏ, which enter the FP64 residual.
-template <typename S>
-__global__ void __launch_bounds__(4 * BEAM_TILE) beam_kernel(NodeState ns, BeamConsts bc, BeamStage<S> st, GaussPts gp,
-                                                              const uint8_t* __restrict__ active,
-                                                              const int32_t* __restrict__ beam_inst) {
+struct NoStaticHook {
+    EBD void operator()(int64_t, const StaticOut&) const {}
+};
+
+// `hook(e, frame)` runs in the static part after the element is done (the contact detection of
+// the assembly with contact uses it)
+template <typename S, typename Hook>
+EBD void beam_kernel_body(NodeState ns, BeamConsts bc, BeamStage<S> st, GaussPts gp, const uint8_t* __restrict__ active,
+                          const int32_t* __restrict__ beam_inst, const Hook& hook) {
     const int part = __builtin_amdgcn_readfirstlane(threadIdx.x / BEAM_TILE);
     const int64_t e = (int64_t)blockIdx.x * BEAM_TILE + threadIdx.x % BEAM_TILE;
     // elements of instances frozen in the Newton iteration are skipped
     const bool valid = e < bc.E && active[beam_inst[e]];
-    MassBlocks<double> mb;
-    MassBlocks<S> unused;
+    // FP64 form: the mass part computes M with Tᵏ; precision form: Tᵏ alone in FP64 (it enters the
+    // residual), M in FP32 with the Coriolis columns 1 and 3
+    constexpr bool FP64 = sizeof(S) == sizeof(double);
+    constexpr int MASS_WHAT = FP64 ? (DYN_TK | DYN_M) : DYN_TK;
+    constexpr int C13_WHAT = FP64 ? DYN_C13 : (DYN_C13 | DYN_M);
+    const int m_part = FP64 ? PART_MASS : PART_C13;
+    MassBlocks<double> mbd;
+    MassBlocks<S> mbs;
     if (valid) {
         if (part == PART_STATIC) {
-            beam_static<S>(e, ns, bc, st);
+            const StaticOut so = beam_static<S>(e, ns, bc, st);
+            hook(e, so);
         } else {
             const StaticOut so = beam_frame(e, ns, bc);
-            if (part == PART_MASS) beam_dynamic<double, S, PART_MASS>(e, ns, bc, st, gp, so, mb);
-            else if (part == PART_C13) beam_dynamic<S, S, PART_C13>(e, ns, bc, st, gp, so, unused);
-            else beam_dynamic<S, S, PART_C24>(e, ns, bc, st, gp, so, unused);
+            if (part == PART_MASS) beam_dynamic<double, S, MASS_WHAT>(e, ns, bc, st, gp, so, mbd);
+            else if (part == PART_C13) beam_dynamic<S, S, C13_WHAT>(e, ns, bc, st, gp, so, mbs);
+            else beam_dynamic<S, S, DYN_C24>(e, ns, bc, st, gp, so, mbs);
         }
     }
     // every wavefront sees the same 64 elements, so this test is uniform over the workgroup
     const bool damped = __ballot(valid && bc.props[13 * bc.E + e] > 0.0) != 0;
     if (damped) {
         __syncthreads();
-        if (part == PART_MASS && valid && bc.props[13 * bc.E + e] > 0.0) beam_damping<double, S>(e, ns, bc, st, mb);
+        if (part == m_part && valid && bc.props[13 * bc.E + e] > 0.0) {
+            if constexpr (FP64) beam_damping<double, S>(e, ns, bc, st, mbd);
+            else beam_damping<S, S>(e, ns, bc, st, mbs);
+        }
     }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(4 * BEAM_TILE) beam_kernel(NodeState ns, BeamConsts bc, BeamStage<S> st, GaussPts gp,
+                                                              const uint8_t* __restrict__ active,
+                                                              const int32_t* __restrict__ beam_inst) {
+    beam_kernel_body<S>(ns, bc, st, gp, active, beam_inst, NoStaticHook{});
 }
 
 }  // namespace ebd

@@ -369,6 +369,50 @@ __global__ void __launch_bounds__(256) contact_detect(NodeState ns, BeamConsts b
 
 __global__ void contact_reset(int32_t* count) { *count = 0; }
 
+// The same detection as a hook of the element kernel's static part, which has the element's frame
+// (Rₑ, Θ̅₁, Θ̅₂) at hand: the assembly with contact needs no separate detection pass.
+template <typename P> struct DetectHook {
+    NodeState ns;
+    BeamConsts bc;
+    GaussPts gp;
+    ContactArgs<P> ca;
+    const uint8_t* is_contact;  // [E]
+    uint64_t* list;
+    int32_t* count;
+    EBD void operator()(int64_t e, const StaticOut& so) const {
+        if (!is_contact[e]) return;
+        const int64_t E = bc.E;
+        const int32_t na = bc.n1[e], nb = bc.n2[e];
+        const int32_t inst = ca.beam_inst[e];
+        const double rb = bc.props[0 * E + e];
+        const V3<double> x1 = ldv(ns.X0, na) + ldv(ns.u, na), x2 = ldv(ns.X0, nb) + ldv(ns.u, nb);
+        const double l0 = bc.l0[e];
+        unsigned mask = 0;
+        for (int ig = 0; ig < gp.n; ++ig) {
+            ShapeF s = shape_fn(l0, gp.z[ig]);
+            V3<double> ut = {0.0, s.N3 * so.Tb1.z + s.N4 * so.Tb2.z, (-s.N3) * so.Tb1.y + (-s.N4) * so.Tb2.y};
+            V3<double> xG = (s.N1 * x1 + s.N2 * x2) + so.Re * ut;
+            if (in_contact<P>(ca, inst, xG, rb)) mask |= 1u << ig;
+        }
+        if (!mask) {
+            for (int k = 0; k < 12; ++k) ca.Tc[e * 12 + k] = 0.0;
+            ca.energy[3 * e + 2] = 0.0;
+            return;
+        }
+        const int32_t slot = atomicAdd(count, 1);
+        list[slot] = (uint64_t(e) << 8) | mask;
+    }
+};
+
+template <typename S>
+__global__ void __launch_bounds__(4 * BEAM_TILE) beam_detect_kernel(NodeState ns, BeamConsts bc, BeamStage<S> st,
+                                                                     GaussPts gp, const uint8_t* __restrict__ active,
+                                                                     const int32_t* __restrict__ beam_inst,
+                                                                     ContactArgs<S> ca, const uint8_t* is_contact,
+                                                                     uint64_t* list, int32_t* count) {
+    beam_kernel_body<S>(ns, bc, st, gp, active, beam_inst, DetectHook<S>{ns, bc, gp, ca, is_contact, list, count});
+}
+
 // One block column: FP64 throughout for the FP64 form; for the precision form (C = float) the
 // contact force and energy still in FP64 (they enter the residual), the tangent in FP32.
 template <typename C, int B>

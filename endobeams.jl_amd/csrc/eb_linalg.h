@@ -116,13 +116,13 @@ struct Active {
 
 // ---- assembly gathers (atomic-free, beam order = the reference's serial scatter order) ------
 
-// K, C, M (and optionally Ktan = (1+α)K + γ/(βΔt) C + M/(βΔt²)) for every block-row entry
+// K, C, M and/or Ktan = (1+α)K + γ/(βΔt) C + M/(βΔt²) for every block-row entry
 template <typename P>
 __global__ void __launch_bounds__(256) gather_matrices(int64_t nnz, const int32_t* __restrict__ blk_ptr,
                                                        const uint32_t* __restrict__ blk_src, const P* __restrict__ sK,
                                                        const P* __restrict__ sC, const P* __restrict__ sM, double* K,
                                                        double* C, double* M, double* Ktan, double c1, double c2,
-                                                       double c3, int with_tangent, Active act) {
+                                                       double c3, int with_tangent, int write_kcm, Active act) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nnz) return;
     const int64_t b = q / 36;
@@ -139,9 +139,11 @@ __global__ void __launch_bounds__(256) gather_matrices(int64_t nnz, const int32_
         cc += double(sC[idx]);
         m += double(sM[idx]);
     }
-    K[q] = k;
-    C[q] = cc;
-    M[q] = m;
+    if (write_kcm) {
+        K[q] = k;
+        C[q] = cc;
+        M[q] = m;
+    }
     if (with_tangent) Ktan[q] = (c1 * k + c2 * cc) + c3 * m;
 }
 
@@ -708,7 +710,46 @@ __global__ void __launch_bounds__(CHAIN_THREADS) chain_solve(ChainDev cd, const 
 // element-wise and reduction kernels work chunk by chunk, and the finalize kernels update every
 // instance's scalars and the all-done flag the host polls.
 
-struct KrylovAll { int all_done, pad[3]; };
+// what the host polls: every instance done, the largest iteration count, and the first instance
+// whose finished solve is not accepted (relative residual above 1e-6, or a breakdown short of
+// the tolerance), with its relative residual
+struct KrylovAll {
+    int all_done, max_iters, fail_inst, breakdown;
+    double fail_rel;
+};
+
+// acceptance of a finished solve: the tolerance reached, or a stall at rounding level (< 1e-6)
+EBD bool krylov_failed(const KrylovCtl& k, double& rel) {
+    rel = 0;
+    if (!k.done || k.iters == 0) return false;
+    if (k.bnorm2 == 0.0 || k.rr2 <= k.tol2 * k.bnorm2 * 1.0000001 || k.ss2 <= k.tol2 * k.bnorm2) return false;
+    rel = k.bnorm2 > 0 ? sqrt(fmin(k.rr2, k.ss2 > 0 ? k.ss2 : k.rr2) / k.bnorm2) : 0.0;
+    return !(rel < 1e-6);
+}
+
+// folds every instance's state into the polled record (one FIN_BLOCK workgroup)
+EBD void krylov_summary(const KrylovCtl* kc, int ninst, KrylovAll* all) {
+    __shared__ int s_done, s_iters, s_fail;
+    if (threadIdx.x == 0) { s_done = 1; s_iters = 0; s_fail = 0x7fffffff; }
+    __syncthreads();
+    for (int s = threadIdx.x; s < ninst; s += FIN_BLOCK) {
+        const KrylovCtl& k = kc[s];
+        double rel;
+        if (!k.done) atomicAnd(&s_done, 0);
+        atomicMax(&s_iters, k.iters);
+        if (krylov_failed(k, rel)) atomicMin(&s_fail, s);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        all->all_done = s_done;
+        all->max_iters = s_iters;
+        all->fail_inst = s_fail == 0x7fffffff ? -1 : s_fail;
+        double rel = 0;
+        if (s_fail != 0x7fffffff) krylov_failed(kc[s_fail], rel);
+        all->fail_rel = rel;
+        all->breakdown = s_fail != 0x7fffffff ? kc[s_fail].breakdown : 0;
+    }
+}
 
 // x = 0, r = r̂ = b (b = residual with fixed dofs zeroed), ‖b‖²
 __global__ void __launch_bounds__(RED_BLOCK) krylov_init(Segs sg, const uint8_t* fixed, const double* res, double* x,
@@ -724,7 +765,6 @@ __global__ void __launch_bounds__(RED_BLOCK) krylov_init(Segs sg, const uint8_t*
 // instances that are not active in the Newton iteration start done
 __global__ void __launch_bounds__(FIN_BLOCK) krylov_init_fin(const double* partials, Segs sg, KrylovCtl* kc, double tol,
                                                              const uint8_t* active, KrylovAll* all) {
-    int alldone = 1;
     for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
         double o[1];
         seg_sum<1>(partials, sg, s, o);
@@ -732,10 +772,9 @@ __global__ void __launch_bounds__(FIN_BLOCK) krylov_init_fin(const double* parti
         k.bnorm2 = o[0]; k.rho = o[0]; k.rho_old = 1; k.alpha = 1; k.omega = 1;
         k.rr2 = o[0]; k.ss2 = 0; k.tol2 = tol * tol; k.iters = 0; k.breakdown = 0;
         k.done = (o[0] == 0.0 || !active[s]) ? 1 : 0;
-        alldone &= k.done;
     }
-    alldone = __syncthreads_and(alldone);
-    if (threadIdx.x == 0) all->all_done = alldone;
+    __syncthreads();
+    krylov_summary(kc, sg.ninst, all);
 }
 // p = r + β(p - ωv), β = (ρ/ρ_old)(α/ω); first iteration p = r
 __global__ void __launch_bounds__(RED_BLOCK) krylov_p(Segs sg, const KrylovCtl* kc, const double* r, double* p,
@@ -811,7 +850,6 @@ __global__ void __launch_bounds__(RED_BLOCK) krylov_x(Segs sg, const KrylovCtl* 
     seg_block_store<2>(q, partials, sg.nchunks);
 }
 __global__ void __launch_bounds__(FIN_BLOCK) krylov_rho_fin(const double* partials, Segs sg, KrylovCtl* kc, KrylovAll* all) {
-    int alldone = 1;
     for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
         KrylovCtl& k = kc[s];
         if (!k.done) {
@@ -824,10 +862,9 @@ __global__ void __launch_bounds__(FIN_BLOCK) krylov_rho_fin(const double* partia
             if (o[1] <= k.tol2 * k.bnorm2) k.done = 1;
             else if (o[0] == 0.0 || !isfinite(o[0])) { k.breakdown = 1; k.done = 1; }
         }
-        alldone &= k.done;
     }
-    alldone = __syncthreads_and(alldone);
-    if (threadIdx.x == 0) all->all_done = alldone;
+    __syncthreads();
+    krylov_summary(kc, sg.ninst, all);
 }
 // ΔD[free] = x[free] for the instances active in the Newton iteration
 __global__ void __launch_bounds__(256) scatter_solution(int64_t ndofs, const uint8_t* fixed, const double* x, double* dD,
