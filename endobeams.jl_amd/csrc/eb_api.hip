@@ -377,8 +377,12 @@ int bc_lift_launch(eb_ctx* ctx) {
 int precond_factor(eb_ctx* ctx) {
     int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
     prof_begin(ctx, EB_KERNEL_PRECOND);
-    hipLaunchKernelGGL(chain_factor, dim3((unsigned)nch), dim3(CHAIN_THREADS), 0, ctx->stream, chain_dev(ctx), ctx->Ktan,
-                       ctx->kstatus);
+    // the reduced chain stays in LDS once it fits in 48 KB (with the kernel's 12 KB of static LDS,
+    // under the 64 KB a launch gets without opting in, and two workgroups per CU)
+    const int L0 = chain_lds_level(ctx->chains.max_len, 48 * 1024);
+    const size_t smem = chain_lds_bytes(ctx->chains.max_len, L0);
+    hipLaunchKernelGGL(chain_factor, dim3((unsigned)nch), dim3(CHAIN_THREADS), smem, ctx->stream, chain_dev(ctx), ctx->Ktan,
+                       ctx->kstatus, L0);
     prof_end(ctx, EB_KERNEL_PRECOND);
     return check_launch(ctx, "chain_factor");
 }

@@ -452,9 +452,33 @@ EBD double masked_entry(const ChainDev& cd, const double* __restrict__ A, int64_
     return blk >= 0 ? A[36 * (int64_t)blk + k] : 0.0;
 }
 
-// Block of position g: from the tangent (first level) or from the working arrays
-template <bool FROM_A> EBD void chain_block(const ChainDev& cd, const double* __restrict__ A, int64_t g, int kind, double* out) {
-    if (FROM_A) {
+// Where a level reads its blocks and writes the kept positions' new blocks: the masked tangent
+// (first level only), the global working arrays, or the workgroup's LDS. Once positions are
+// 2^L0 apart (levels l ≥ L0) the chain's active blocks live in LDS, at slot p >> L0; the factors
+// the solve needs (D⁻¹, GL, GU at elimination, history) always go to the global arrays.
+enum { CH_A = 0, CH_G = 1, CH_S = 2 };
+
+struct ChainLDS {
+    double *D, *L, *U;  // [slots][36]
+    int L0;
+};
+
+template <int SRC>
+EBD const double* chain_src(const ChainDev& cd, const ChainLDS& sm, int64_t g, int32_t p, int kind) {
+    if (SRC == CH_S) return (kind == 0 ? sm.D : (kind == 1 ? sm.L : sm.U)) + 36 * (int64_t)(p >> sm.L0);
+    return (kind == 0 ? cd.Dm : (kind == 1 ? cd.Lm : cd.Um)) + 36 * g;
+}
+
+template <int DST> EBD double* chain_dst(const ChainDev& cd, const ChainLDS& sm, int64_t g, int32_t p, int kind) {
+    if (DST == CH_S) return (kind == 0 ? sm.D : (kind == 1 ? sm.L : sm.U)) + 36 * (int64_t)(p >> sm.L0);
+    return (kind == 0 ? cd.Dm : (kind == 1 ? cd.Lm : cd.Um)) + 36 * g;
+}
+
+// Block of position p (g = s0 + p)
+template <int SRC>
+EBD void chain_block(const ChainDev& cd, const ChainLDS& sm, const double* __restrict__ A, int64_t g, int32_t p, int kind,
+                     double* out) {
+    if (SRC == CH_A) {
         const uint64_t m = kind == 0 ? cd.mD[g] : (kind == 1 ? cd.mL[g] : cd.mU[g]);
         const int32_t blk = kind == 0 ? cd.diag_blk[g] : (kind == 1 ? cd.lo_blk[g] : cd.up_blk[g]);
         const double* src = A + 36 * (int64_t)(blk >= 0 ? blk : 0);
@@ -464,45 +488,56 @@ template <bool FROM_A> EBD void chain_block(const ChainDev& cd, const double* __
             out[k] = ((m >> k) & 1) ? ((kind == 0 && k / 6 == k % 6) ? 1.0 : 0.0) : (blk >= 0 ? v : 0.0);
         }
     } else {
-        const double* src = (kind == 0 ? cd.Dm : (kind == 1 ? cd.Lm : cd.Um)) + 36 * g;
+        const double* src = chain_src<SRC>(cd, sm, g, p, kind);
 #pragma unroll
         for (int k = 0; k < 36; ++k) out[k] = src[k];
     }
 }
 
-// phase A for one eliminated position g: D <- D^-1, L <- D^-1 L, U <- D^-1 U. Without PIVOT,
-// returns false when the unpivoted inverse was rejected (nothing is written then).
-template <bool PIVOT, bool FROM_A>
-EBD bool chain_eliminate(const ChainDev& cd, const double* __restrict__ A, int64_t g, bool& ok) {
+// phase A for one eliminated position: D <- D^-1, L <- D^-1 L, U <- D^-1 U into the global arrays
+// (and into LDS when the level works there). Without PIVOT, returns false when the unpivoted
+// inverse was rejected (nothing is written then).
+template <bool PIVOT, int SRC>
+EBD bool chain_eliminate(const ChainDev& cd, const ChainLDS& sm, const double* __restrict__ A, int64_t g, int32_t p,
+                         bool& ok) {
     double D[36], Lb[36], Ub[36];
-    chain_block<FROM_A>(cd, A, g, 0, D);
+    chain_block<SRC>(cd, sm, A, g, p, 0, D);
     if (!PIVOT) {  // all three blocks in flight together
-        chain_block<FROM_A>(cd, A, g, 1, Lb);
-        chain_block<FROM_A>(cd, A, g, 2, Ub);
+        chain_block<SRC>(cd, sm, A, g, p, 1, Lb);
+        chain_block<SRC>(cd, sm, A, g, p, 2, Ub);
         if (!inv6_nopiv(D)) return false;
     } else {
         ok = inv6(D) && ok;
-        chain_block<FROM_A>(cd, A, g, 1, Lb);
-        chain_block<FROM_A>(cd, A, g, 2, Ub);
+        chain_block<SRC>(cd, sm, A, g, p, 1, Lb);
+        chain_block<SRC>(cd, sm, A, g, p, 2, Ub);
     }
     mm6_inplace(D, Lb);
     mm6_inplace(D, Ub);
 #pragma unroll
     for (int k = 0; k < 36; ++k) { cd.Dm[36 * g + k] = D[k]; cd.Lm[36 * g + k] = Lb[k]; cd.Um[36 * g + k] = Ub[k]; }
+    if (SRC == CH_S) {
+        double* sd = sm.D + 36 * (int64_t)(p >> sm.L0);
+        double* sl = sm.L + 36 * (int64_t)(p >> sm.L0);
+        double* su = sm.U + 36 * (int64_t)(p >> sm.L0);
+#pragma unroll
+        for (int k = 0; k < 36; ++k) { sd[k] = D[k]; sl[k] = Lb[k]; su[k] = Ub[k]; }
+    }
     return true;
 }
 
-template <bool FROM_A>
-EBD void chain_level(const ChainDev& cd, const double* __restrict__ A, int64_t s0, int32_t n, int l, bool& ok,
-                     int& redo, double (*P2)[6]) {
+// one reduction level: SRC where its blocks are, DST where the kept positions' new blocks go
+template <int SRC, int DST>
+EBD void chain_level(const ChainDev& cd, const ChainLDS& sm, const double* __restrict__ A, int64_t s0, int32_t n, int l,
+                     bool& ok, int& redo, double (*P2)[6]) {
     const int tid = threadIdx.x;
     const int32_t h = 1 << l;
     // phase A: one lane per eliminated position; the rare rejected unpivoted inverses are
     // redone with partial pivoting in a second pass
     const int32_t ne = (n - h + 2 * h - 1) / (2 * h);
     for (int32_t m = tid; m < ne; m += CHAIN_THREADS) {
-        const int64_t g = s0 + h + 2 * (int64_t)h * m;
-        if (!chain_eliminate<false, FROM_A>(cd, A, g, ok)) {
+        const int32_t p = h + 2 * h * m;
+        const int64_t g = s0 + p;
+        if (!chain_eliminate<false, SRC>(cd, sm, A, g, p, ok)) {
             cd.rej[g] = 1;
             redo = 1;
         }
@@ -510,10 +545,11 @@ EBD void chain_level(const ChainDev& cd, const double* __restrict__ A, int64_t s
     __syncthreads();
     if (redo) {
         for (int32_t m = tid; m < ne; m += CHAIN_THREADS) {
-            const int64_t g = s0 + h + 2 * (int64_t)h * m;
+            const int32_t p = h + 2 * h * m;
+            const int64_t g = s0 + p;
             if (cd.rej[g]) {
                 cd.rej[g] = 0;
-                chain_eliminate<true, FROM_A>(cd, A, g, ok);
+                chain_eliminate<true, SRC>(cd, sm, A, g, p, ok);
             }
         }
         __syncthreads();
@@ -524,7 +560,9 @@ EBD void chain_level(const ChainDev& cd, const double* __restrict__ A, int64_t s
     //   s = 0: L_i GU_j1 (into D) and -L_i GL_j1 (new L)
     //   s = 1: U_i GL_j2 (into D) and -U_i GU_j2 (new U)
     // Each lane reads and writes only its own row of L_i, U_i or D_i; the side-1 part of D
-    // reaches the side-0 lane through LDS.
+    // reaches the side-0 lane through the P2 exchange. The eliminated neighbours' blocks come
+    // from where phase A put them (LDS for levels that work there, else the global arrays).
+    constexpr int NSRC = SRC == CH_S ? CH_S : CH_G;
     const int32_t nk = (n + 2 * h - 1) / (2 * h);
     constexpr int PER = (CHAIN_THREADS / 12) * 12;
     for (int32_t base = 0; base < 12 * nk; base += PER) {
@@ -536,14 +574,15 @@ EBD void chain_level(const ChainDev& cd, const double* __restrict__ A, int64_t s
         const bool has = act && (side == 0 ? i > 0 : i + h < n);
         double row[6] = {}, G1[36] = {}, G2[36] = {}, Dr[6] = {};
         if (has) {
-            const int64_t gj = side == 0 ? gi - h : gi + h;
-            const double* Gd = (side == 0 ? cd.Um : cd.Lm) + 36 * gj;   // into D
-            const double* Gn = (side == 0 ? cd.Lm : cd.Um) + 36 * gj;   // new L or U
-            if (FROM_A) {
+            const int32_t pj = side == 0 ? i - h : i + h;
+            const int64_t gj = s0 + pj;
+            const double* Gd = chain_src<NSRC>(cd, sm, gj, pj, side == 0 ? 2 : 1);   // into D
+            const double* Gn = chain_src<NSRC>(cd, sm, gj, pj, side == 0 ? 1 : 2);   // new L or U
+            if (SRC == CH_A) {
 #pragma unroll
                 for (int k = 0; k < 6; ++k) row[k] = masked_entry(cd, A, gi, side == 0 ? 1 : 2, 6 * r + k);
             } else {
-                const double* X = (side == 0 ? cd.Lm : cd.Um) + 36 * gi + 6 * r;
+                const double* X = chain_src<SRC>(cd, sm, gi, i, side == 0 ? 1 : 2) + 6 * r;
 #pragma unroll
                 for (int k = 0; k < 6; ++k) row[k] = X[k];
             }
@@ -551,12 +590,13 @@ EBD void chain_level(const ChainDev& cd, const double* __restrict__ A, int64_t s
             for (int k = 0; k < 36; ++k) { G1[k] = Gd[k]; G2[k] = Gn[k]; }
         }
         if (act && side == 0) {
-            if (FROM_A) {
+            if (SRC == CH_A) {
 #pragma unroll
                 for (int k = 0; k < 6; ++k) Dr[k] = masked_entry(cd, A, gi, 0, 6 * r + k);
             } else {
+                const double* X = chain_src<SRC>(cd, sm, gi, i, 0) + 6 * r;
 #pragma unroll
-                for (int k = 0; k < 6; ++k) Dr[k] = cd.Dm[36 * gi + 6 * r + k];
+                for (int k = 0; k < 6; ++k) Dr[k] = X[k];
             }
         }
         double pd[6], pn[6];
@@ -575,40 +615,55 @@ EBD void chain_level(const ChainDev& cd, const double* __restrict__ A, int64_t s
         __syncthreads();
         if (has) {
             double* hs = cd.hist + 72 * ((int64_t)cd.hbase[gi] + l) + 36 * side + 6 * r;
-            double* dst = (side == 0 ? cd.Lm : cd.Um) + 36 * gi + 6 * r;
+            double* dst = chain_dst<DST>(cd, sm, gi, i, side == 0 ? 1 : 2) + 6 * r;
 #pragma unroll
             for (int k = 0; k < 6; ++k) { hs[k] = row[k]; dst[k] = pn[k]; }
-        } else if (FROM_A && act) {
-            // first level, no neighbour on this side: the coupling block is zero
-            double* dst = (side == 0 ? cd.Lm : cd.Um) + 36 * gi + 6 * r;
+        } else if (act && (SRC == CH_A || DST != SRC)) {
+            // no neighbour on this side: the coupling block is zero (written where it now lives)
+            double* dst = chain_dst<DST>(cd, sm, gi, i, side == 0 ? 1 : 2) + 6 * r;
 #pragma unroll
             for (int k = 0; k < 6; ++k) dst[k] = 0.0;
         }
         if (act && side == 0) {
+            double* dst = chain_dst<DST>(cd, sm, gi, i, 0) + 6 * r;
 #pragma unroll
-            for (int c = 0; c < 6; ++c) cd.Dm[36 * gi + 6 * r + c] = Dr[c] - pd[c] - P2[tid + 6][c];
+            for (int c = 0; c < 6; ++c) dst[c] = Dr[c] - pd[c] - P2[tid + 6][c];
         }
         __syncthreads();
     }
 }
 
-__global__ void __launch_bounds__(CHAIN_THREADS) chain_factor(ChainDev cd, const double* __restrict__ A, int* status) {
+// Dynamic LDS: 3 × 36 doubles per slot, ceil(n / 2^L0) slots (chain_lds_bytes).
+__global__ void __launch_bounds__(CHAIN_THREADS) chain_factor(ChainDev cd, const double* __restrict__ A, int* status, int L0) {
+    extern __shared__ double chain_smem[];
     const int64_t s0 = cd.ptr[blockIdx.x];
     const int32_t n = cd.ptr[blockIdx.x + 1] - (int32_t)s0;
     const int tid = threadIdx.x;
     __shared__ double P2[CHAIN_THREADS][6];
     __shared__ int redo;
+    const int32_t slots = (n + (1 << L0) - 1) >> L0;
+    const ChainLDS sm{chain_smem, chain_smem + 36 * (int64_t)slots, chain_smem + 72 * (int64_t)slots, L0};
     if (tid == 0) redo = 0;
     __syncthreads();
     bool ok = true;
     const int nl = chain_levels(n);
-    // the first level reads the masked tangent blocks directly
-    if (nl > 0) chain_level<true>(cd, A, s0, n, 0, ok, redo, P2);
-    for (int l = 1; l < nl; ++l) chain_level<false>(cd, A, s0, n, l, ok, redo, P2);
+    for (int l = 0; l < nl; ++l) {
+        const bool to_lds = l + 1 >= L0;
+        if (l == 0) {
+            if (to_lds) chain_level<CH_A, CH_S>(cd, sm, A, s0, n, l, ok, redo, P2);
+            else chain_level<CH_A, CH_G>(cd, sm, A, s0, n, l, ok, redo, P2);
+        } else if (l < L0) {
+            if (to_lds) chain_level<CH_G, CH_S>(cd, sm, A, s0, n, l, ok, redo, P2);
+            else chain_level<CH_G, CH_G>(cd, sm, A, s0, n, l, ok, redo, P2);
+        } else {
+            chain_level<CH_S, CH_S>(cd, sm, A, s0, n, l, ok, redo, P2);
+        }
+    }
     if (tid == 0) {
         double D[36], C[36];
-        if (nl > 0) chain_block<false>(cd, A, s0, 0, D);
-        else chain_block<true>(cd, A, s0, 0, D);
+        if (nl == 0) chain_block<CH_A>(cd, sm, A, s0, 0, 0, D);
+        else if (nl >= L0) chain_block<CH_S>(cd, sm, A, s0, 0, 0, D);
+        else chain_block<CH_G>(cd, sm, A, s0, 0, 0, D);
 #pragma unroll
         for (int k = 0; k < 36; ++k) C[k] = D[k];
         if (inv6_nopiv(C)) {
@@ -621,6 +676,16 @@ __global__ void __launch_bounds__(CHAIN_THREADS) chain_factor(ChainDev cd, const
         for (int k = 0; k < 36; ++k) cd.Dm[36 * s0 + k] = D[k];
     }
     if (!ok) *status = 1;
+}
+
+// smallest L0 whose LDS share for the longest chain stays within `budget` bytes
+inline int chain_lds_level(int64_t max_len, int64_t budget) {
+    int L0 = 0;
+    while (((max_len + (int64_t(1) << L0) - 1) >> L0) * 3 * 36 * (int64_t)sizeof(double) > budget) ++L0;
+    return L0;
+}
+inline size_t chain_lds_bytes(int64_t max_len, int L0) {
+    return (size_t)(((max_len + (int64_t(1) << L0) - 1) >> L0) * 3 * 36 * (int64_t)sizeof(double));
 }
 
 // x = M^-1 b along each chain, with the factors of chain_factor: the right-hand side is reduced
