@@ -112,7 +112,19 @@ def main():
     ap.add_argument("--only", choices=["plain", "fast", "learned"], default=None, help="time one form only")
     ap.add_argument("--train-steps", type=int, default=40, help="converged steps recorded per training stent")
     ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES device to device 5 times (scales HBM counters of a profiler run)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 4 * n, "copies": 5}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(independent structures would shard across GPUs without communication)", file=sys.stderr)

@@ -68,6 +68,12 @@ struct StaticOut {
     double eta, eta11, eta12, eta21, eta22, ln;
 };
 
+// no work beside the element (the element kernel without contact)
+struct NoStaticHook {
+    EBD void operator()(int64_t, const NodeState&, const BeamConsts&, const GaussPts&, const V3<double>&,
+                        const V3<double>&, const M3<double>&, const V3<double>&, const V3<double>&) const {}
+};
+
 template <typename P>
 EBD StaticOut beam_static(int64_t e, const NodeState& ns, const BeamConsts& bc, const BeamStage<P>& st) {
     const int64_t E = bc.E;
@@ -444,12 +450,6 @@ constexpr int BEAM_TILE = 64;
 // S: storage type of the element matrices (double: the FP64 form; float: the precision form, whose
 // Coriolis parts also compute in FP32). The static part always computes in FP64 and the mass part
 // too, because it yields the inertia forces Tᵏ, which enter the FP64 residual.
-struct NoStaticHook {
-    EBD void operator()(int64_t, const StaticOut&) const {}
-};
-
-// `hook(e, frame)` runs in the static part after the element is done (the contact detection of
-// the assembly with contact uses it)
 template <typename S, typename Hook>
 EBD void beam_kernel_body(NodeState ns, BeamConsts bc, BeamStage<S> st, GaussPts gp, const uint8_t* __restrict__ active,
                           const int32_t* __restrict__ beam_inst, const Hook& hook) {
@@ -467,13 +467,17 @@ EBD void beam_kernel_body(NodeState ns, BeamConsts bc, BeamStage<S> st, GaussPts
     MassBlocks<S> mbs;
     if (valid) {
         if (part == PART_STATIC) {
-            const StaticOut so = beam_static<S>(e, ns, bc, st);
-            hook(e, so);
+            beam_static<S>(e, ns, bc, st);
         } else {
             const StaticOut so = beam_frame(e, ns, bc);
             if (part == PART_MASS) beam_dynamic<double, S, MASS_WHAT>(e, ns, bc, st, gp, so, mbd);
             else if (part == PART_C13) beam_dynamic<S, S, C13_WHAT>(e, ns, bc, st, gp, so, mbs);
-            else beam_dynamic<S, S, DYN_C24>(e, ns, bc, st, gp, so, mbs);
+            else {
+                const int32_t a = bc.n1[e], b = bc.n2[e];
+                // work beside the element, with its frame: hook(e, ns, bc, gp, x₁, x₂, Rₑ, Θ̅₁, Θ̅₂)
+                hook(e, ns, bc, gp, ldv(ns.X0, a) + ldv(ns.u, a), ldv(ns.X0, b) + ldv(ns.u, b), so.Re, so.Tb1, so.Tb2);
+                beam_dynamic<S, S, DYN_C24>(e, ns, bc, st, gp, so, mbs);
+            }
         }
     }
     // every wavefront sees the same 64 elements, so this test is uniform over the workgroup

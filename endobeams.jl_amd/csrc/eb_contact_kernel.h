@@ -369,29 +369,26 @@ __global__ void __launch_bounds__(256) contact_detect(NodeState ns, BeamConsts b
 
 __global__ void contact_reset(int32_t* count) { *count = 0; }
 
-// The same detection as a hook of the element kernel's static part, which has the element's frame
-// (Rₑ, Θ̅₁, Θ̅₂) at hand: the assembly with contact needs no separate detection pass.
+// The same detection as a hook of the element kernel: the wavefront of the Coriolis columns 2 and 4
+// runs it between the element's frame (x₁, x₂, Rₑ, Θ̅₁, Θ̅₂) and its Gauss loop, so the assembly
+// with contact needs no separate detection pass.
 template <typename P> struct DetectHook {
-    NodeState ns;
-    BeamConsts bc;
-    GaussPts gp;
-    ContactArgs<P> ca;
+    const ContactArgs<P>& ca;   // the kernel's arguments, by reference (no local copies)
     const uint8_t* is_contact;  // [E]
     uint64_t* list;
     int32_t* count;
-    EBD void operator()(int64_t e, const StaticOut& so) const {
+    EBD void operator()(int64_t e, const NodeState& ns, const BeamConsts& bc, const GaussPts& gp, const V3<double>& x1,
+                        const V3<double>& x2, const M3<double>& Re, const V3<double>& Tb1, const V3<double>& Tb2) const {
         if (!is_contact[e]) return;
         const int64_t E = bc.E;
-        const int32_t na = bc.n1[e], nb = bc.n2[e];
         const int32_t inst = ca.beam_inst[e];
         const double rb = bc.props[0 * E + e];
-        const V3<double> x1 = ldv(ns.X0, na) + ldv(ns.u, na), x2 = ldv(ns.X0, nb) + ldv(ns.u, nb);
         const double l0 = bc.l0[e];
         unsigned mask = 0;
         for (int ig = 0; ig < gp.n; ++ig) {
             ShapeF s = shape_fn(l0, gp.z[ig]);
-            V3<double> ut = {0.0, s.N3 * so.Tb1.z + s.N4 * so.Tb2.z, (-s.N3) * so.Tb1.y + (-s.N4) * so.Tb2.y};
-            V3<double> xG = (s.N1 * x1 + s.N2 * x2) + so.Re * ut;
+            V3<double> ut = {0.0, s.N3 * Tb1.z + s.N4 * Tb2.z, (-s.N3) * Tb1.y + (-s.N4) * Tb2.y};
+            V3<double> xG = (s.N1 * x1 + s.N2 * x2) + Re * ut;
             if (in_contact<P>(ca, inst, xG, rb)) mask |= 1u << ig;
         }
         if (!mask) {
@@ -410,7 +407,7 @@ __global__ void __launch_bounds__(4 * BEAM_TILE) beam_detect_kernel(NodeState ns
                                                                      const int32_t* __restrict__ beam_inst,
                                                                      ContactArgs<S> ca, const uint8_t* is_contact,
                                                                      uint64_t* list, int32_t* count) {
-    beam_kernel_body<S>(ns, bc, st, gp, active, beam_inst, DetectHook<S>{ns, bc, gp, ca, is_contact, list, count});
+    beam_kernel_body<S>(ns, bc, st, gp, active, beam_inst, DetectHook<S>{ca, is_contact, list, count});
 }
 
 // One block column: FP64 throughout for the FP64 form; for the precision form (C = float) the
