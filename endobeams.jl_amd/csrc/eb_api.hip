@@ -99,6 +99,7 @@ struct eb_ctx {
     double* partials = nullptr;
     KrylovCtl* kctl = nullptr;
     int* kstatus = nullptr;
+    int solve_id = 0;                 // number of the current linear solve (tags pivot failures)
     double* norms = nullptr;  // device (2)
     // host-pinned mirrors for per-iteration reads
     KrylovCtl* h_kctl = nullptr;
@@ -382,7 +383,7 @@ int precond_factor(eb_ctx* ctx) {
     const int L0 = chain_lds_level(ctx->chains.max_len, 48 * 1024);
     const size_t smem = chain_lds_bytes(ctx->chains.max_len, L0);
     hipLaunchKernelGGL(chain_factor, dim3((unsigned)nch), dim3(CHAIN_THREADS), smem, ctx->stream, chain_dev(ctx), ctx->Ktan,
-                       ctx->kstatus, L0);
+                       &ctx->kall->pivot_status, L0, ctx->active, ctx->solve_id);
     prof_end(ctx, EB_KERNEL_PRECOND);
     return check_launch(ctx, "chain_factor");
 }
@@ -417,38 +418,47 @@ int krylov_iteration(eb_ctx* ctx) {
     return check_launch(ctx, "krylov_iteration");
 }
 
-// Ktan (block-row order) and res must be current; solves the masked free-dof system of every
-// instance active in the Newton iteration, writes their ΔD[free]
-int solve_free(eb_ctx* ctx, const eb_newton_params* p, int* its_out) {
+// The linear solve in pieces: start (factorisation, BiCGStab initialisation), iterations, a poll
+// of the all-instances record (one small copy and one synchronisation), the acceptance check and
+// the scatter of the solution into ΔD. Ktan (block-row order) and res must be current.
+int solve_start(eb_ctx* ctx, const eb_newton_params* p) {
     const Segs sg = segs(ctx);
-    const int maxit = (p && p->krylov_maxit > 0) ? p->krylov_maxit : 200;
     const double tol = (p && p->krylov_tol > 0) ? p->krylov_tol : 1e-13;
-    EB_HIP(hipMemsetAsync(ctx->kstatus, 0, sizeof(int), ctx->stream));
+    ctx->solve_id += 1;
     EB_TRY(precond_factor(ctx));
     hipLaunchKernelGGL(krylov_init, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->is_fixed,
                        ctx->res, ctx->kx, ctx->kr, ctx->krhat, ctx->kp, ctx->kv, ctx->partials);
     hipLaunchKernelGGL(krylov_init_fin, dim3(1), dim3(FIN_BLOCK), 0, ctx->stream, ctx->partials, sg, ctx->kctl, tol,
                        ctx->active, ctx->kall);
-    EB_TRY(check_launch(ctx, "krylov_init"));
-    int launched = 0, batch = 1;
-    for (;;) {
-        // every poll brings the all-instances record and the preconditioner's pivot status
-        EB_HIP(hipMemcpyAsync(ctx->h_kall, ctx->kall, sizeof(KrylovAll), hipMemcpyDeviceToHost, ctx->stream));
-        EB_HIP(hipMemcpyAsync(ctx->h_status, ctx->kstatus, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        if (launched > 0 || maxit == 0) {
-            EB_HIP(hipStreamSynchronize(ctx->stream));
-            if (ctx->h_kall->all_done || launched >= maxit || *ctx->h_status) break;
-        }
-        int m = std::min(batch, maxit - launched);
+    return check_launch(ctx, "krylov_init");
+}
+
+int solve_poll(eb_ctx* ctx) {
+    EB_HIP(hipMemcpyAsync(ctx->h_kall, ctx->kall, sizeof(KrylovAll), hipMemcpyDeviceToHost, ctx->stream));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+// iterates (in doubling batches, polling after each) until done; `launched` iterations have run
+int solve_finish(eb_ctx* ctx, const eb_newton_params* p, int launched) {
+    const int maxit = (p && p->krylov_maxit > 0) ? p->krylov_maxit : 200;
+    int batch = std::max(1, launched);
+    while (!ctx->h_kall->all_done && launched < maxit && ctx->h_kall->pivot_status != ctx->solve_id) {
+        const int m = std::min(batch, maxit - launched);
         for (int k = 0; k < m; ++k) EB_TRY(krylov_iteration(ctx));
         launched += m;
         batch = std::min(batch * 2, 16);
+        EB_TRY(solve_poll(ctx));
     }
+    return EB_OK;
+}
+
+int solve_check(eb_ctx* ctx, int* its_out) {
     const KrylovAll& ka = *ctx->h_kall;
     ctx->counters[1] += ka.max_iters;
     ctx->counters[2] += 1;
     if (its_out) *its_out = ka.max_iters;
-    if (*ctx->h_status) { ctx->err = "singular 6x6 pivot block in the chain preconditioner"; return EB_ERR_SOLVE; }
+    if (ka.pivot_status == ctx->solve_id) { ctx->err = "singular 6x6 pivot block in the chain preconditioner"; return EB_ERR_SOLVE; }
     if (!ka.all_done || ka.fail_inst >= 0) {
         char buf[200];
         if (ka.fail_inst >= 0)
@@ -459,22 +469,42 @@ int solve_free(eb_ctx* ctx, const eb_newton_params* p, int* its_out) {
         ctx->err = buf;
         return EB_ERR_SOLVE;
     }
+    return EB_OK;
+}
+
+int scatter_launch(eb_ctx* ctx, const KrylovAll* gate) {
     hipLaunchKernelGGL(scatter_solution, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs,
-                       ctx->is_fixed, ctx->kx, ctx->dD, active_dev(ctx));
+                       ctx->is_fixed, ctx->kx, ctx->dD, active_dev(ctx), gate);
     return check_launch(ctx, "scatter_solution");
 }
 
-// per-instance norms; in a Newton iteration (nc.newton) also the convergence bookkeeping. Returns
-// the number of instances still iterating (Newton) through *n_active.
-int norms_launch(eb_ctx* ctx, const NewtonCtl& nc, int* n_active) {
+// solves the masked free-dof system of every instance active in the Newton iteration and writes
+// their ΔD[free]
+int solve_free(eb_ctx* ctx, const eb_newton_params* p, int* its_out) {
+    EB_TRY(solve_start(ctx, p));
+    EB_TRY(krylov_iteration(ctx));
+    EB_TRY(solve_poll(ctx));
+    EB_TRY(solve_finish(ctx, p, 1));
+    EB_TRY(solve_check(ctx, its_out));
+    return scatter_launch(ctx, nullptr);
+}
+
+// per-instance norms; in a Newton iteration (nc.newton) also the convergence bookkeeping. Kernels
+// only (no synchronisation); behind `gate` they wait for the solve on the device.
+int norms_kernels(eb_ctx* ctx, const NewtonCtl& nc, const KrylovAll* gate) {
     const Segs sg = segs(ctx);
     prof_begin(ctx, EB_KERNEL_UPDATE);
     hipLaunchKernelGGL(norms_partials, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->is_fixed,
                        ctx->dD, ctx->res, ctx->fext, ctx->Tc, ctx->Tk, ctx->partials);
     hipLaunchKernelGGL(norms_finalize, dim3(1), dim3(FIN_BLOCK), 0, ctx->stream, ctx->partials, sg, ctx->norms, nc,
-                       ctx->active, ctx->kiter, ctx->nall);
+                       ctx->active, ctx->kiter, ctx->nall, gate);
     prof_end(ctx, EB_KERNEL_UPDATE);
-    EB_TRY(check_launch(ctx, "norms"));
+    return check_launch(ctx, "norms");
+}
+
+// norms with the per-instance values copied to the host
+int norms_launch(eb_ctx* ctx, const NewtonCtl& nc, int* n_active) {
+    EB_TRY(norms_kernels(ctx, nc, nullptr));
     EB_HIP(hipMemcpyAsync(ctx->h_nall, ctx->nall, sizeof(NewtonAll), hipMemcpyDeviceToHost, ctx->stream));
     EB_HIP(hipMemcpyAsync(ctx->h_norms, ctx->norms, 2 * sizeof(double) * ctx->S, hipMemcpyDeviceToHost, ctx->stream));
     EB_HIP(hipStreamSynchronize(ctx->stream));
@@ -483,20 +513,41 @@ int norms_launch(eb_ctx* ctx, const NewtonCtl& nc, int* n_active) {
     return EB_OK;
 }
 
+// node state and force vectors, n+1 → n (commit) or n → n+1 (rollback), in one launch
+int copy_state(eb_ctx* ctx, bool commit) {
+    CopyList cl{};
+    int k = 0;
+    int64_t nmax = 0;
+    auto add = [&](double* cur, double* old, int64_t n) {
+        cl.src[k] = commit ? cur : old;
+        cl.dst[k] = commit ? old : cur;
+        cl.n[k] = n;
+        nmax = std::max(nmax, n);
+        ++k;
+    };
+    for (int f = 0; f < 8; ++f) add(ctx->node[EB_F_U + f], ctx->node[EB_F_U_N + f], (f >= 6 ? 9 : 3) * ctx->N);
+    add(ctx->Tint, ctx->Tint_n, ctx->ndofs);
+    add(ctx->Tk, ctx->Tk_n, ctx->ndofs);
+    add(ctx->Tc, ctx->Tc_n, ctx->ndofs);
+    add(ctx->fext, ctx->fext_n, ctx->ndofs);
+    hipLaunchKernelGGL(copy_many, dim3(std::min<unsigned>(nblk(nmax, 256), 4096), (unsigned)k), dim3(256), 0, ctx->stream, cl);
+    return check_launch(ctx, "copy_state");
+}
+
 int newton_reset_launch(eb_ctx* ctx) {
     hipLaunchKernelGGL(newton_reset, dim3(nblk(ctx->S, 256)), dim3(256), 0, ctx->stream, (int)ctx->S, ctx->active,
                        ctx->kiter);
     return check_launch(ctx, "newton_reset");
 }
 
-int global_corrector_async(eb_ctx* ctx, double dt, double b, double g) {
+int global_corrector_async(eb_ctx* ctx, double dt, double b, double g, const KrylovAll* gate = nullptr) {
     hipLaunchKernelGGL(global_corrector, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs,
-                       g / (b * dt), 1 / (b * (dt * dt)), ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, active_dev(ctx));
+                       g / (b * dt), 1 / (b * (dt * dt)), ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, active_dev(ctx), gate);
     return check_launch(ctx, "global_corrector");
 }
-int local_corrector_async(eb_ctx* ctx, double dt, double b, double g) {
+int local_corrector_async(eb_ctx* ctx, double dt, double b, double g, const KrylovAll* gate = nullptr) {
     hipLaunchKernelGGL(local_corrector, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx),
-                       ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, hht_consts(dt, b, g), active_dev(ctx));
+                       ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, hht_consts(dt, b, g), active_dev(ctx), gate);
     return check_launch(ctx, "local_corrector");
 }
 int global_predictor_async(eb_ctx* ctx, double dt, double b, double g) {
@@ -988,7 +1039,7 @@ int eb_get_instance_iterations(const eb_ctx* ctx, int32_t* k_out) {
 
 int eb_get_instance_norms(const eb_ctx* ctx, double* rn_dn) {
     if (!ctx || !rn_dn) return EB_ERR_ARG;
-    for (int64_t s = 0; s < 2 * ctx->S; ++s) rn_dn[s] = ctx->h_norms[s];
+    if (hipMemcpy(rn_dn, ctx->norms, 2 * sizeof(double) * ctx->S, hipMemcpyDeviceToHost) != hipSuccess) return EB_ERR_DEVICE;
     return EB_OK;
 }
 
@@ -1239,14 +1290,43 @@ int eb_corrector(eb_ctx* ctx, const eb_newton_params* p, double dt, int* kout) {
     int k = 1, n_active = (int)ctx->S;
     int st = EB_OK;
     while (n_active > 0 && k <= p->max_iterations) {
+        // assembly, lift, factorisation and one BiCGStab iteration, then the update and norms
+        // behind the solve's all-done flag: one synchronisation when one iteration solves (the
+        // usual case with the chain preconditioner); otherwise the solve continues and the update
+        // and norms are launched again after it
+        NewtonCtl nc{p->tol_res, p->tol_disp, k, p->max_iterations, 1};
         st = assemble_all(ctx, true, p->alpha, true, dt, p->beta, p->gamma);
         if (st == EB_OK) st = bc_lift_launch(ctx);
-        if (st == EB_OK) st = solve_free(ctx, p, nullptr);
-        if (st == EB_OK) st = global_corrector_async(ctx, dt, p->beta, p->gamma);
-        if (st == EB_OK) st = local_corrector_async(ctx, dt, p->beta, p->gamma);
-        NewtonCtl nc{p->tol_res, p->tol_disp, k, p->max_iterations, 1};
-        if (st == EB_OK) st = norms_launch(ctx, nc, &n_active);
+        if (st == EB_OK) st = solve_start(ctx, p);
+        if (st == EB_OK) st = krylov_iteration(ctx);
+        if (st == EB_OK) st = scatter_launch(ctx, ctx->kall);
+        if (st == EB_OK) st = global_corrector_async(ctx, dt, p->beta, p->gamma, ctx->kall);
+        if (st == EB_OK) st = local_corrector_async(ctx, dt, p->beta, p->gamma, ctx->kall);
+        if (st == EB_OK) st = norms_kernels(ctx, nc, ctx->kall);
+        if (st == EB_OK) {
+            if (hipMemcpyAsync(ctx->h_nall, ctx->nall, sizeof(NewtonAll), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+                st = EB_ERR_DEVICE;
+        }
+        if (st == EB_OK) st = solve_poll(ctx);
+        if (st == EB_OK && !ctx->h_kall->all_done) {
+            st = solve_finish(ctx, p, 1);
+            if (st == EB_OK) st = solve_check(ctx, nullptr);
+            if (st == EB_OK) st = scatter_launch(ctx, nullptr);
+            if (st == EB_OK) st = global_corrector_async(ctx, dt, p->beta, p->gamma);
+            if (st == EB_OK) st = local_corrector_async(ctx, dt, p->beta, p->gamma);
+            if (st == EB_OK) st = norms_kernels(ctx, nc, nullptr);
+            if (st == EB_OK) {
+                if (hipMemcpyAsync(ctx->h_nall, ctx->nall, sizeof(NewtonAll), hipMemcpyDeviceToHost, ctx->stream) !=
+                        hipSuccess ||
+                    hipStreamSynchronize(ctx->stream) != hipSuccess)
+                    st = EB_ERR_DEVICE;
+            }
+        } else if (st == EB_OK) {
+            st = solve_check(ctx, nullptr);
+        }
         if (st != EB_OK) break;
+        if (!ctx->ev_pending.empty()) prof_harvest(ctx);
+        n_active = ctx->h_nall->n_active;
         ctx->counters[0]++;
         k += 1;
     }
@@ -1280,14 +1360,7 @@ int eb_update_converged(eb_ctx* ctx) {
         EB_TRY(check_launch(ctx, "learned_record"));
         ctx->hist_count = std::min(ctx->hist_count + 1, 3);
     }
-    for (int f = 0; f < 8; ++f)
-        EB_HIP(hipMemcpyAsync(ctx->node[EB_F_U_N + f], ctx->node[EB_F_U + f], sizeof(double) * (f >= 6 ? 9 : 3) * ctx->N,
-                              hipMemcpyDeviceToDevice, ctx->stream));
-    size_t vb = sizeof(double) * ctx->ndofs;
-    EB_HIP(hipMemcpyAsync(ctx->Tint_n, ctx->Tint, vb, hipMemcpyDeviceToDevice, ctx->stream));
-    EB_HIP(hipMemcpyAsync(ctx->Tk_n, ctx->Tk, vb, hipMemcpyDeviceToDevice, ctx->stream));
-    EB_HIP(hipMemcpyAsync(ctx->Tc_n, ctx->Tc, vb, hipMemcpyDeviceToDevice, ctx->stream));
-    EB_HIP(hipMemcpyAsync(ctx->fext_n, ctx->fext, vb, hipMemcpyDeviceToDevice, ctx->stream));
+    EB_TRY(copy_state(ctx, true));
     EB_TRY(materialize_kcm(ctx));
     if (!ctx->np1_alias) {
         // matricesⁿ = matricesⁿ⁺¹ by exchanging buffers; n+1 now aliases n until the next assembly
@@ -1302,14 +1375,7 @@ int eb_update_converged(eb_ctx* ctx) {
 
 int eb_update_not_converged(eb_ctx* ctx) {
     if (!ctx) return EB_ERR_ARG;
-    for (int f = 0; f < 8; ++f)
-        EB_HIP(hipMemcpyAsync(ctx->node[EB_F_U + f], ctx->node[EB_F_U_N + f], sizeof(double) * (f >= 6 ? 9 : 3) * ctx->N,
-                              hipMemcpyDeviceToDevice, ctx->stream));
-    size_t vb = sizeof(double) * ctx->ndofs;
-    EB_HIP(hipMemcpyAsync(ctx->Tint, ctx->Tint_n, vb, hipMemcpyDeviceToDevice, ctx->stream));
-    EB_HIP(hipMemcpyAsync(ctx->Tk, ctx->Tk_n, vb, hipMemcpyDeviceToDevice, ctx->stream));
-    EB_HIP(hipMemcpyAsync(ctx->Tc, ctx->Tc_n, vb, hipMemcpyDeviceToDevice, ctx->stream));
-    EB_HIP(hipMemcpyAsync(ctx->fext, ctx->fext_n, vb, hipMemcpyDeviceToDevice, ctx->stream));
+    EB_TRY(copy_state(ctx, false));
     ctx->kcm_pending = false;
     ctx->np1_alias = true;
     EB_HIP(hipStreamSynchronize(ctx->stream));

@@ -634,8 +634,11 @@ EBD void chain_level(const ChainDev& cd, const ChainLDS& sm, const double* __res
 }
 
 // Dynamic LDS: 3 × 36 doubles per slot, ceil(n / 2^L0) slots (chain_lds_bytes).
-__global__ void __launch_bounds__(CHAIN_THREADS) chain_factor(ChainDev cd, const double* __restrict__ A, int* status, int L0) {
+// `status` receives `solve_id` when a pivot block is singular (no reset between solves is needed)
+__global__ void __launch_bounds__(CHAIN_THREADS) chain_factor(ChainDev cd, const double* __restrict__ A, int* status, int L0,
+                                                              const uint8_t* __restrict__ active, int solve_id) {
     extern __shared__ double chain_smem[];
+    if (!active[cd.inst[blockIdx.x]]) return;   // instance frozen in the Newton iteration
     const int64_t s0 = cd.ptr[blockIdx.x];
     const int32_t n = cd.ptr[blockIdx.x + 1] - (int32_t)s0;
     const int tid = threadIdx.x;
@@ -675,7 +678,7 @@ __global__ void __launch_bounds__(CHAIN_THREADS) chain_factor(ChainDev cd, const
 #pragma unroll
         for (int k = 0; k < 36; ++k) cd.Dm[36 * s0 + k] = D[k];
     }
-    if (!ok) *status = 1;
+    if (!ok) *status = solve_id;
 }
 
 // smallest L0 whose LDS share for the longest chain stays within `budget` bytes
@@ -781,7 +784,12 @@ __global__ void __launch_bounds__(CHAIN_THREADS) chain_solve(ChainDev cd, const 
 struct KrylovAll {
     int all_done, max_iters, fail_inst, breakdown;
     double fail_rel;
+    int pivot_status;   // solve number of the last singular pivot block (written by chain_factor)
+    int pad;
 };
+
+// true when the kernel may run: ungated, or the polled solve is done
+EBD bool gate_open(const KrylovAll* gate) { return !gate || gate->all_done; }
 
 // acceptance of a finished solve: the tolerance reached, or a stall at rounding level (< 1e-6)
 EBD bool krylov_failed(const KrylovCtl& k, double& rel) {
@@ -932,8 +940,12 @@ __global__ void __launch_bounds__(FIN_BLOCK) krylov_rho_fin(const double* partia
     krylov_summary(kc, sg.ninst, all);
 }
 // ΔD[free] = x[free] for the instances active in the Newton iteration
+// Kernels of the corrector's update that take a `gate` run only once the linear solve is done: the
+// corrector launches them behind a first Krylov iteration without waiting for the host, and
+// launches them again after further iterations when that one did not finish the solve.
 __global__ void __launch_bounds__(256) scatter_solution(int64_t ndofs, const uint8_t* fixed, const double* x, double* dD,
-                                                        Active act) {
+                                                        Active act, const KrylovAll* gate) {
+    if (!gate_open(gate)) return;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ndofs && !fixed[i] && act.flag[act.node_inst[i / 6]]) dD[i] = x[i];
 }
@@ -961,7 +973,8 @@ __global__ void __launch_bounds__(256) init_global(int64_t N, NodeFields nf, dou
 
 // compute_global_corrector! per dof (all dofs, fixed included)
 __global__ void __launch_bounds__(256) global_corrector(int64_t ndofs, double gb, double inv, double* D, double* Dd,
-                                                        double* Ddd, const double* dD, Active act) {
+                                                        double* Ddd, const double* dD, Active act, const KrylovAll* gate) {
+    if (!gate_open(gate)) return;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ndofs || !act.flag[act.node_inst[i / 6]]) return;
     D[i] += dD[i]; Dd[i] += gb * dD[i]; Ddd[i] += inv * dD[i];
@@ -981,7 +994,9 @@ struct HHTc { double c_w, c_wd, c_wdd, d_w, d_wd, d_wdd; };  // ẇ = ΔR(c_w w 
 
 // compute_local_corrector!: u, u̇, ü from D; ΔR ← exp(ΔD_θ)ΔR; w = log ΔR; ẇ, ẅ; R = ΔR Rⁿ
 __global__ void __launch_bounds__(256) local_corrector(int64_t N, NodeFields nf, const double* D, const double* Dd,
-                                                       const double* Ddd, const double* dD, HHTc h, Active act) {
+                                                       const double* Ddd, const double* dD, HHTc h, Active act,
+                                                       const KrylovAll* gate) {
+    if (!gate_open(gate)) return;
     int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N || !act.flag[act.node_inst[n]]) return;
     stv(nf.u, n, {D[6 * n], D[6 * n + 1], D[6 * n + 2]});
@@ -1048,7 +1063,9 @@ struct NewtonAll { int n_active, pad[3]; };
 // an active instance whose norms pass (or that has used its iterations) is frozen and its k
 // recorded as the reference's loop leaves it (iterations done + 1)
 __global__ void __launch_bounds__(FIN_BLOCK) norms_finalize(const double* partials, Segs sg, double* out, NewtonCtl nc,
-                                                            uint8_t* active, int32_t* kiter, NewtonAll* all) {
+                                                            uint8_t* active, int32_t* kiter, NewtonAll* all,
+                                                            const KrylovAll* gate) {
+    if (!gate_open(gate)) return;
     __shared__ int cnt;
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
@@ -1081,6 +1098,18 @@ __global__ void newton_reset(int ninst, uint8_t* active, int32_t* kiter) {
         active[s] = 1;
         kiter[s] = 0;
     }
+}
+
+// several device copies in one launch (the commit and the rollback of a step): list y = blockIdx.y
+struct CopyList {
+    const double* src[16];
+    double* dst[16];
+    int64_t n[16];
+};
+__global__ void __launch_bounds__(256) copy_many(CopyList cl) {
+    const int y = blockIdx.y;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cl.n[y]; i += (int64_t)gridDim.x * blockDim.x)
+        cl.dst[y][i] = cl.src[y][i];
 }
 
 // ---- layout conversions at the ABI ----------------------------------------------------------
