@@ -79,8 +79,9 @@ function beam_record(p)
     return [p.radius, p.E, K̄ū, diag(K̄Θ̅)..., diag(K̄Θ̅Θ̅)..., J..., p.Aᵨ, p.damping]
 end
 
-"""Creates the device context from an EndoBeams BeamsConfiguration (+ optional RigidInteraction)."""
-function Device(conf, params, inter=nothing; precision::Integer=0)
+"""Descriptor of one structure (an EndoBeams BeamsConfiguration + optional RigidInteraction) and
+the arrays it points at, which must stay alive until eb_create/eb_create_batch has returned."""
+function model_desc(conf, params, inter=nothing; precision::Integer=0)
     nodes, beams = conf.nodes, conf.beams
     X0 = flatten(nodes.X₀)
     conn = Int64[]; for b in beams; push!(conn, b.node1, b.node2); end
@@ -115,15 +116,42 @@ function Device(conf, params, inter=nothing; precision::Integer=0)
                   pointer(l0), pointer(Re0), pointer(props), length(fixed), pointer(fixed), Int32(length(gw)),
                   Int32(0), pointer(gw), pointer(gz), surface, Int32(surface == EB_SURFACE_SDF), kn, mu, epst, etan, length(cb),
                   pointer(cb), c, R, sdfn, sdfd, sdfx0, pointer(samples))
+    return d, keep
+end
+
+node_state(nodes) = ((EB_F_U, nodes.u), (EB_F_UDOT, nodes.u̇), (EB_F_UDDOT, nodes.ü), (EB_F_W, nodes.w),
+                     (EB_F_WDOT, nodes.ẇ), (EB_F_WDDOT, nodes.ẅ), (EB_F_R, nodes.R), (EB_F_DR, nodes.ΔR))
+
+"""Creates the device context from an EndoBeams BeamsConfiguration (+ optional RigidInteraction)."""
+function Device(conf, params, inter=nothing; precision::Integer=0)
+    d, keep = model_desc(conf, params, inter; precision=precision)
     ref = Ref{Ptr{Cvoid}}(C_NULL)
     st = GC.@preserve keep ccall((:eb_create, LIB), Cint, (Ref{ModelDesc}, Ref{Ptr{Cvoid}}), d, ref)
     st == 0 || error("eb_create failed ($st)")
     dev = Device(ref[], keep)
     finalizer(x -> ccall((:eb_destroy, LIB), Cint, (Ptr{Cvoid},), x.ctx), dev)
     # NodesBeams state: current and step-n copies, ΔR = I
-    for (f, v) in ((EB_F_U, nodes.u), (EB_F_UDOT, nodes.u̇), (EB_F_UDDOT, nodes.ü), (EB_F_W, nodes.w),
-                   (EB_F_WDOT, nodes.ẇ), (EB_F_WDDOT, nodes.ẅ), (EB_F_R, nodes.R), (EB_F_DR, nodes.ΔR))
+    for (f, v) in node_state(conf.nodes)
         set_field!(dev, f, flatten(v)); set_field!(dev, f + 8, flatten(v))
+    end
+    return dev
+end
+
+"""One context for many structures (eb_create_batch): `instances` is a vector of
+(conf, inter) pairs sharing `params`. Fields address the instances one after the other."""
+function DeviceBatch(instances, params; precision::Integer=0)
+    built = [model_desc(conf, params, inter; precision=precision) for (conf, inter) in instances]
+    descs = [b[1] for b in built]
+    keep = Any[descs, [b[2] for b in built]]
+    ref = Ref{Ptr{Cvoid}}(C_NULL)
+    st = GC.@preserve keep ccall((:eb_create_batch, LIB), Cint, (Ptr{ModelDesc}, Int64, Ref{Ptr{Cvoid}}),
+                                 descs, length(descs), ref)
+    st == 0 || error("eb_create_batch failed ($st)")
+    dev = Device(ref[], keep)
+    finalizer(x -> ccall((:eb_destroy, LIB), Cint, (Ptr{Cvoid},), x.ctx), dev)
+    for (f, _) in node_state(instances[1][1].nodes)
+        v = reduce(vcat, [flatten(Dict(node_state(conf.nodes))[f]) for (conf, _) in instances])
+        set_field!(dev, f, v); set_field!(dev, f + 8, v)
     end
     return dev
 end
