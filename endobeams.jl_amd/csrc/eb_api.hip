@@ -113,7 +113,7 @@ struct eb_ctx {
     // chains
     int32_t *ch_ptr = nullptr, *ch_node = nullptr, *ch_diag = nullptr, *ch_lo = nullptr, *ch_up = nullptr;
     uint64_t *ch_mD = nullptr, *ch_mL = nullptr, *ch_mU = nullptr;
-    int32_t *ch_hbase = nullptr, *ch_rej = nullptr;
+    int32_t* ch_hbase = nullptr;
     double *ch_D = nullptr, *ch_L = nullptr, *ch_U = nullptr, *ch_hist = nullptr, *ch_bw = nullptr, *ch_yw = nullptr;
     int precond_mode = 0;
     double* tmp = nullptr;  // nnz scratch for layout conversion
@@ -222,7 +222,7 @@ template <typename P> BeamStage<P> beam_stage(eb_ctx* c) {
 }
 ChainDev chain_dev(eb_ctx* c) {
     return ChainDev{c->ch_ptr, c->ch_node, c->ch_diag, c->ch_lo, c->ch_up, c->ch_mD, c->ch_mL, c->ch_mU,
-                    c->ch_hbase, c->ch_D, c->ch_L, c->ch_U, c->ch_hist, c->ch_bw, c->ch_yw, c->ch_rej, c->chain_inst,
+                    c->ch_hbase, c->ch_D, c->ch_L, c->ch_U, c->ch_hist, c->ch_bw, c->ch_yw, c->chain_inst,
                     (int64_t)c->chains.ptr.size() - 1};
 }
 Active active_dev(eb_ctx* c) { return Active{c->active, c->node_inst, c->beam_inst, c->blk_inst}; }
@@ -378,7 +378,7 @@ int bc_lift_launch(eb_ctx* ctx) {
 int precond_factor(eb_ctx* ctx) {
     int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
     prof_begin(ctx, EB_KERNEL_PRECOND);
-    // the reduced chain stays in LDS once it fits in 48 KB (with the kernel's 12 KB of static LDS,
+    // the reduced chain stays in LDS once it fits in 48 KB (with the kernel's 8 KB of static LDS,
     // under the 64 KB a launch gets without opting in, and two workgroups per CU)
     const int L0 = chain_lds_level(ctx->chains.max_len, 48 * 1024);
     const size_t smem = chain_lds_bytes(ctx->chains.max_len, L0);
@@ -390,8 +390,13 @@ int precond_factor(eb_ctx* ctx) {
 
 int precond_apply(eb_ctx* ctx, const double* b, double* x) {
     int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
-    hipLaunchKernelGGL(chain_solve, dim3((unsigned)nch), dim3(CHAIN_THREADS), 0, ctx->stream, chain_dev(ctx), b, x,
-                       ctx->kctl);
+    const size_t smem = chain_solve_lds_bytes(ctx->chains.max_len);
+    if (smem)
+        hipLaunchKernelGGL(chain_solve<true>, dim3((unsigned)nch), dim3(SOLVE_THREADS), smem, ctx->stream, chain_dev(ctx), b,
+                           x, ctx->kctl);
+    else
+        hipLaunchKernelGGL(chain_solve<false>, dim3((unsigned)nch), dim3(SOLVE_THREADS), 0, ctx->stream, chain_dev(ctx), b, x,
+                           ctx->kctl);
     return check_launch(ctx, "chain_solve");
 }
 
@@ -977,8 +982,6 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
         CK(dalloc(ctx, ctx->ch_hist, 72 * c.n_hist));
         CK(dalloc(ctx, ctx->ch_bw, 6 * N));
         CK(dalloc(ctx, ctx->ch_yw, 6 * N));
-        CK(dalloc(ctx, ctx->ch_rej, N));
-        if (hipMemsetAsync(ctx->ch_rej, 0, sizeof(int32_t) * N, ctx->stream) != hipSuccess) return bail(EB_ERR_DEVICE);
     }
 #undef CK
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return bail(EB_ERR_DEVICE);

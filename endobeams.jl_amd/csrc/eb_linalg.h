@@ -328,7 +328,10 @@ __global__ void __launch_bounds__(RED_BLOCK) spmv_masked(Segs sg, const int32_t*
 // are saved (hist) for the right-hand-side reduction of the solve. Depth: ceil(log2 n) levels
 // instead of n sequential 6x6 steps, and every level is parallel over the chain.
 
-constexpr int CHAIN_THREADS = 256;
+constexpr int CHAIN_THREADS = 256;                       // factorisation workgroup
+constexpr int CHAIN_GROUPS = (CHAIN_THREADS / 64) * 10;  // six-lane groups: ten per wavefront
+constexpr int GJ_BUF = 24;                               // doubles of LDS exchange per group
+constexpr int SOLVE_THREADS = 1024;                      // solve workgroup
 
 struct ChainDev {
     const int32_t* ptr;   // n_chains+1
@@ -338,8 +341,7 @@ struct ChainDev {
     const int32_t* hbase;          // per chain position: first history slot
     double *Dm, *Lm, *Um;          // 36 per position: working blocks, then D^-1, GL, GU
     double* hist;                  // 72 per slot: L then U of a kept position at one level
-    double *bw, *yw;               // 6 per position: solve work vectors
-    int32_t* rej;                  // per position: unpivoted inverse rejected (zero between calls)
+    double *bw, *yw;               // 6 per position: solve work vectors (chains too long for LDS)
     const int32_t* inst;           // per chain: instance
     int64_t n_chains;
 };
@@ -350,101 +352,135 @@ EBD int chain_levels(int32_t n) {
     return nl;
 }
 
-// In-place inverse of a 6x6 row-major block: Gauss-Jordan on [S | I] with partial pivoting.
-// Rows are exchanged by conditional swaps so that every index is static and the 6x12 working
-// matrix stays in registers of one lane.
-EBD bool inv6(double* S) {
-    double W[6][12];
+// A 6x6 block row-parallel: lane r of a six-lane group holds row r of [D | L | U] (NC = 18) or of
+// D (NC = 6); the group shares GJ_BUF doubles of LDS for the pivot row. All six lanes of a group
+// are in one wavefront, so the exchange needs no workgroup barrier, only the wavefront-scope
+// ordering of the LDS accesses.
+EBD void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct Lane6 {
+    int group;  // six-lane group of the workgroup (lanes 60-63 of a wavefront take none)
+    int r;      // row of the block this lane holds; 6 for the idle lanes
+    double* buf;
+};
+EBD Lane6 lane6(double* bufs) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    Lane6 l;
+    l.group = wave * 10 + (lane < 60 ? lane / 6 : 0);
+    l.r = lane < 60 ? lane % 6 : 6;
+    l.buf = bufs + GJ_BUF * l.group;
+    return l;
+}
+
+// In-place Gauss-Jordan without row exchanges: on return the group holds row r of
+// [D^-1 | D^-1 L | D^-1 U]. False (on the pivot row's lane) when its pivot has cancelled to below
+// 1e-12 of the diagonal entry it started from, or is not finite; the caller then redoes the
+// block with gj6_pivot. The test is invariant under row and column scaling, which matters here:
+// translational and rotational entries differ by many decades.
+template <int NC> EBD bool gj6_nopiv(double (&w)[NC], int r, double* buf) {
+    double d0 = 0;
 #pragma unroll
-    for (int r = 0; r < 6; ++r) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            W[r][c] = S[6 * r + c];
-            W[r][6 + c] = (r == c) ? 1.0 : 0.0;
-        }
-    }
+    for (int k = 0; k < 6; ++k) d0 = (r == k) ? fabs(w[k]) : d0;
     bool ok = true;
 #pragma unroll
     for (int p = 0; p < 6; ++p) {
+        if (r == p) {
+            const double piv = w[p];
+            ok = fabs(piv) > 1e-12 * d0 && fabs(piv) > 0.0 && fabs(piv) < INFINITY;
+            const double ip = 1.0 / piv;
+            w[p] = 1.0;
 #pragma unroll
-        for (int q = p + 1; q < 6; ++q) {
-            const bool sw = fabs(W[q][p]) > fabs(W[p][p]);
+            for (int c = 0; c < NC; ++c) w[c] *= ip;
 #pragma unroll
-            for (int c = p; c < 12; ++c) {
-                const double a = W[p][c], b = W[q][c];
-                W[p][c] = sw ? b : a;
-                W[q][c] = sw ? a : b;
-            }
+            for (int c = 0; c < NC; ++c) buf[c] = w[c];
         }
-        const double piv = W[p][p];
-        ok = ok && fabs(piv) > 0.0 && fabs(piv) < INFINITY;
-        const double ip = 1.0 / piv;
+        wave_sync();
+        if (r != p) {
+            const double f = w[p];
+            w[p] = 0.0;
 #pragma unroll
-        for (int c = p + 1; c < 12; ++c) W[p][c] *= ip;
-#pragma unroll
-        for (int r = 0; r < 6; ++r) {
-            if (r == p) continue;
-            const double f = W[r][p];
-#pragma unroll
-            for (int c = p + 1; c < 12; ++c) W[r][c] = fma(-f, W[p][c], W[r][c]);
+            for (int c = 0; c < NC; ++c) w[c] = fma(-f, buf[c], w[c]);
         }
-    }
-#pragma unroll
-    for (int r = 0; r < 6; ++r) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) S[6 * r + c] = W[r][6 + c];
+        wave_sync();
     }
     return ok;
 }
 
-// In-place inverse of a 6x6 row-major block by Gauss-Jordan without row exchanges. Returns false
-// when a pivot has cancelled to below 1e-12 of the diagonal entry it started from (or is not
-// finite); the caller then falls back to inv6. The test is invariant under row and column
-// scaling, which matters here: translational and rotational entries differ by many decades.
-EBD bool inv6_nopiv(double* a) {
-    double d0[6];
+// Gauss-Jordan with partial pivoting on [D | I | L | U], the pivot of step p being the largest
+// |entry| of column p among the rows not yet used (lowest row on ties). Rows are not moved: the
+// lane whose row is the pivot of step p ends with row p of the result, returned in `orow`.
+// False on that lane when its pivot is zero or not finite.
+template <int NC> EBD bool gj6_pivot(double (&w)[NC], int r, double* buf, int& orow) {
+    constexpr int NW = NC + 6;
+    double x[NW];
 #pragma unroll
-    for (int p = 0; p < 6; ++p) d0[p] = fabs(a[7 * p]);
-    bool ok = true;
+    for (int c = 0; c < 6; ++c) { x[c] = w[c]; x[6 + c] = (r == c) ? 1.0 : 0.0; }
+#pragma unroll
+    for (int c = 6; c < NC; ++c) x[6 + c] = w[c];
+    bool used = r >= 6, ok = true;
+    orow = 0;
 #pragma unroll
     for (int p = 0; p < 6; ++p) {
-        const double piv = a[7 * p];
-        ok = ok && fabs(piv) > 1e-12 * d0[p] && fabs(piv) > 0.0 && fabs(piv) < INFINITY;
-        const double ip = 1.0 / piv;
-        a[7 * p] = 1.0;
+        const double a = fabs(x[p]);
+        if (r < 6) buf[r] = used ? -1.0 : (a >= 0.0 ? a : 0.0);
+        wave_sync();
+        int q = 0;
+        double best = -0.5;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) a[6 * p + c] *= ip;
-#pragma unroll
-        for (int r = 0; r < 6; ++r) {
-            if (r == p) continue;
-            const double f = a[6 * r + p];
-            a[6 * r + p] = 0.0;
-#pragma unroll
-            for (int c = 0; c < 6; ++c) a[6 * r + c] = fma(-f, a[6 * p + c], a[6 * r + c]);
+        for (int k = 0; k < 6; ++k) {
+            const double v = buf[k];
+            if (v > best) { best = v; q = k; }
         }
+        wave_sync();
+        if (r == q) {
+            const double piv = x[p];
+            ok = fabs(piv) > 0.0 && fabs(piv) < INFINITY;
+            const double ip = 1.0 / piv;
+#pragma unroll
+            for (int c = 0; c < NW; ++c) x[c] *= ip;
+            x[p] = 1.0;
+#pragma unroll
+            for (int c = 0; c < NW; ++c) buf[c] = x[c];
+            used = true;
+            orow = p;
+        }
+        wave_sync();
+        if (r != q) {
+            const double f = x[p];
+#pragma unroll
+            for (int c = 0; c < NW; ++c) x[c] = fma(-f, buf[c], x[c]);
+        }
+        wave_sync();
     }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) w[c] = x[6 + c];
     return ok;
 }
 
-// B <- A B in place, column by column (6x6 row-major, both in registers)
-EBD void mm6_inplace(const double* A, double* B) {
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-        double t[6];
-#pragma unroll
-        for (int r = 0; r < 6; ++r) {
-            double acc = 0;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) acc = fma(A[6 * r + k], B[6 * k + c], acc);
-            t[r] = acc;
-        }
-#pragma unroll
-        for (int r = 0; r < 6; ++r) B[6 * r + c] = t[r];
+// Inverse of the group's block (and D^-1 L, D^-1 U for NC = 18): unpivoted, redone with pivoting
+// by the groups whose unpivoted elimination was rejected (load(w) fetches the rows again). `act`:
+// the group holds a block this pass (uniform over the group). Every lane of the wavefront calls
+// it. Returns the row of the result the lane holds.
+template <int NC, typename Load> EBD int gj6(double (&w)[NC], const Lane6& ln, bool act, bool& ok, const Load& load) {
+    const bool good = gj6_nopiv<NC>(w, ln.r, ln.buf);
+    const uint64_t bad = __ballot(act && ln.r < 6 && !good);
+    const int lane = threadIdx.x & 63;
+    const int first = lane < 60 ? lane - ln.r : 60;
+    int orow = ln.r;
+    if (act && ((bad >> first) & 0x3full)) {
+        load(w);
+        const bool nonsingular = gj6_pivot<NC>(w, ln.r, ln.buf, orow);
+        if (ln.r < 6) ok = nonsingular && ok;
     }
+    return orow;
 }
 
-// Masked block of position g (kind 0: D, 1: L, 2: U) read from the tangent: an entry in a fixed
-// row or column becomes the identity's; an absent block is zero.
+// Masked entry k of position g's block (kind 0: D, 1: L, 2: U) read from the tangent: an entry in
+// a fixed row or column becomes the identity's; an absent block is zero.
 EBD double masked_entry(const ChainDev& cd, const double* __restrict__ A, int64_t g, int kind, int k) {
     const uint64_t m = kind == 0 ? cd.mD[g] : (kind == 1 ? cd.mL[g] : cd.mU[g]);
     const int32_t blk = kind == 0 ? cd.diag_blk[g] : (kind == 1 ? cd.lo_blk[g] : cd.up_blk[g]);
@@ -474,209 +510,177 @@ template <int DST> EBD double* chain_dst(const ChainDev& cd, const ChainLDS& sm,
     return (kind == 0 ? cd.Dm : (kind == 1 ? cd.Lm : cd.Um)) + 36 * g;
 }
 
-// Block of position p (g = s0 + p)
+// row r of block `kind` of position p (g = s0 + p)
 template <int SRC>
-EBD void chain_block(const ChainDev& cd, const ChainLDS& sm, const double* __restrict__ A, int64_t g, int32_t p, int kind,
-                     double* out) {
+EBD void chain_row(const ChainDev& cd, const ChainLDS& sm, const double* __restrict__ A, int64_t g, int32_t p, int kind,
+                   int r, double* out) {
     if (SRC == CH_A) {
         const uint64_t m = kind == 0 ? cd.mD[g] : (kind == 1 ? cd.mL[g] : cd.mU[g]);
         const int32_t blk = kind == 0 ? cd.diag_blk[g] : (kind == 1 ? cd.lo_blk[g] : cd.up_blk[g]);
-        const double* src = A + 36 * (int64_t)(blk >= 0 ? blk : 0);
+        const double* src = A + 36 * (int64_t)(blk >= 0 ? blk : 0) + 6 * r;
 #pragma unroll
-        for (int k = 0; k < 36; ++k) {
-            const double v = src[k];
-            out[k] = ((m >> k) & 1) ? ((kind == 0 && k / 6 == k % 6) ? 1.0 : 0.0) : (blk >= 0 ? v : 0.0);
+        for (int c = 0; c < 6; ++c) {
+            const double v = src[c];
+            out[c] = ((m >> (6 * r + c)) & 1) ? ((kind == 0 && r == c) ? 1.0 : 0.0) : (blk >= 0 ? v : 0.0);
         }
     } else {
-        const double* src = chain_src<SRC>(cd, sm, g, p, kind);
+        const double* src = chain_src<SRC>(cd, sm, g, p, kind) + 6 * r;
 #pragma unroll
-        for (int k = 0; k < 36; ++k) out[k] = src[k];
+        for (int c = 0; c < 6; ++c) out[c] = src[c];
     }
 }
 
-// phase A for one eliminated position: D <- D^-1, L <- D^-1 L, U <- D^-1 U into the global arrays
-// (and into LDS when the level works there). Without PIVOT, returns false when the unpivoted
-// inverse was rejected (nothing is written then).
-template <bool PIVOT, int SRC>
-EBD bool chain_eliminate(const ChainDev& cd, const ChainLDS& sm, const double* __restrict__ A, int64_t g, int32_t p,
-                         bool& ok) {
-    double D[36], Lb[36], Ub[36];
-    chain_block<SRC>(cd, sm, A, g, p, 0, D);
-    if (!PIVOT) {  // all three blocks in flight together
-        chain_block<SRC>(cd, sm, A, g, p, 1, Lb);
-        chain_block<SRC>(cd, sm, A, g, p, 2, Ub);
-        if (!inv6_nopiv(D)) return false;
-    } else {
-        ok = inv6(D) && ok;
-        chain_block<SRC>(cd, sm, A, g, p, 1, Lb);
-        chain_block<SRC>(cd, sm, A, g, p, 2, Ub);
-    }
-    mm6_inplace(D, Lb);
-    mm6_inplace(D, Ub);
+// acc += row · G (6x6 row-major), the block's loads fenced off from the neighbouring code
+EBD void row_times_block(const double (&row)[6], const double* __restrict__ G, double (&acc)[6]) {
+    double g[36];
 #pragma unroll
-    for (int k = 0; k < 36; ++k) { cd.Dm[36 * g + k] = D[k]; cd.Lm[36 * g + k] = Lb[k]; cd.Um[36 * g + k] = Ub[k]; }
-    if (SRC == CH_S) {
-        double* sd = sm.D + 36 * (int64_t)(p >> sm.L0);
-        double* sl = sm.L + 36 * (int64_t)(p >> sm.L0);
-        double* su = sm.U + 36 * (int64_t)(p >> sm.L0);
+    for (int k = 0; k < 36; ++k) g[k] = G[k];
 #pragma unroll
-        for (int k = 0; k < 36; ++k) { sd[k] = D[k]; sl[k] = Lb[k]; su[k] = Ub[k]; }
+    for (int c = 0; c < 6; ++c) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[c] = fma(row[k], g[6 * k + c], acc[c]);
     }
-    return true;
+    __builtin_amdgcn_sched_barrier(0);
 }
 
 // one reduction level: SRC where its blocks are, DST where the kept positions' new blocks go
 template <int SRC, int DST>
 EBD void chain_level(const ChainDev& cd, const ChainLDS& sm, const double* __restrict__ A, int64_t s0, int32_t n, int l,
-                     bool& ok, int& redo, double (*P2)[6]) {
-    const int tid = threadIdx.x;
+                     bool& ok, const Lane6& ln) {
     const int32_t h = 1 << l;
-    // phase A: one lane per eliminated position; the rare rejected unpivoted inverses are
-    // redone with partial pivoting in a second pass
+    // phase A: a six-lane group per eliminated position j = h, 3h, 5h, ...:
+    //   D_j <- D_j^-1, GL_j = D_j^-1 L_j, GU_j = D_j^-1 U_j  (one Gauss-Jordan on [D | L | U])
     const int32_t ne = (n - h + 2 * h - 1) / (2 * h);
-    for (int32_t m = tid; m < ne; m += CHAIN_THREADS) {
-        const int32_t p = h + 2 * h * m;
+    for (int32_t base = 0; base < ne; base += CHAIN_GROUPS) {
+        const int32_t m = base + ln.group;
+        const bool act = m < ne;
+        const int32_t p = act ? h + 2 * h * m : 0;
         const int64_t g = s0 + p;
-        if (!chain_eliminate<false, SRC>(cd, sm, A, g, p, ok)) {
-            cd.rej[g] = 1;
-            redo = 1;
+        const int rr = ln.r < 6 ? ln.r : 0;
+        auto load = [&](double (&w)[18]) {
+            if (act) {
+                chain_row<SRC>(cd, sm, A, g, p, 0, rr, w);
+                chain_row<SRC>(cd, sm, A, g, p, 1, rr, w + 6);
+                chain_row<SRC>(cd, sm, A, g, p, 2, rr, w + 12);
+            } else {
+#pragma unroll
+                for (int c = 0; c < 18; ++c) w[c] = 0.0;
+            }
+        };
+        double w[18];
+        load(w);
+        const int orow = gj6<18>(w, ln, act, ok, load);
+        if (act && ln.r < 6) {
+            const int64_t o = 36 * g + 6 * orow;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) { cd.Dm[o + c] = w[c]; cd.Lm[o + c] = w[6 + c]; cd.Um[o + c] = w[12 + c]; }
+            if (SRC == CH_S) {
+                const int64_t so = 36 * (int64_t)(p >> sm.L0) + 6 * orow;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) { sm.D[so + c] = w[c]; sm.L[so + c] = w[6 + c]; sm.U[so + c] = w[12 + c]; }
+            }
         }
     }
     __syncthreads();
-    if (redo) {
-        for (int32_t m = tid; m < ne; m += CHAIN_THREADS) {
-            const int32_t p = h + 2 * h * m;
-            const int64_t g = s0 + p;
-            if (cd.rej[g]) {
-                cd.rej[g] = 0;
-                chain_eliminate<true, SRC>(cd, sm, A, g, p, ok);
-            }
-        }
-        __syncthreads();
-        if (tid == 0) redo = 0;
-        __syncthreads();
-    }
-    // phase B: twelve lanes per kept position i: lane (side s, row r) forms row r of
-    //   s = 0: L_i GU_j1 (into D) and -L_i GL_j1 (new L)
-    //   s = 1: U_i GL_j2 (into D) and -U_i GU_j2 (new U)
-    // Each lane reads and writes only its own row of L_i, U_i or D_i; the side-1 part of D
-    // reaches the side-0 lane through the P2 exchange. The eliminated neighbours' blocks come
-    // from where phase A put them (LDS for levels that work there, else the global arrays).
+    // phase B: a six-lane group per kept position i = 0, 2h, 4h, ... (j1 = i-h, j2 = i+h), lane r
+    // forming row r of
+    //   D_i <- D_i - L_i GU_j1 - U_i GL_j2,   L_i <- -L_i GL_j1,   U_i <- -U_i GU_j2
+    // The L_i, U_i it started from are saved (history) for the solve's right-hand-side reduction.
+    // The eliminated neighbours' blocks come from where phase A put them.
     constexpr int NSRC = SRC == CH_S ? CH_S : CH_G;
     const int32_t nk = (n + 2 * h - 1) / (2 * h);
-    constexpr int PER = (CHAIN_THREADS / 12) * 12;
-    for (int32_t base = 0; base < 12 * nk; base += PER) {
-        const int32_t t = base + tid;
-        const int32_t m = t / 12, side = (t % 12) / 6, r = t % 6;
-        const bool act = tid < PER && m < nk;
+    for (int32_t base = 0; base < nk; base += CHAIN_GROUPS) {
+        const int32_t m = base + ln.group;
+        if (m >= nk || ln.r >= 6) continue;
+        const int r = ln.r;
         const int32_t i = 2 * h * m;
         const int64_t gi = s0 + i;
-        const bool has = act && (side == 0 ? i > 0 : i + h < n);
-        double row[6] = {}, G1[36] = {}, G2[36] = {}, Dr[6] = {};
-        if (has) {
-            const int32_t pj = side == 0 ? i - h : i + h;
-            const int64_t gj = s0 + pj;
-            const double* Gd = chain_src<NSRC>(cd, sm, gj, pj, side == 0 ? 2 : 1);   // into D
-            const double* Gn = chain_src<NSRC>(cd, sm, gj, pj, side == 0 ? 1 : 2);   // new L or U
-            if (SRC == CH_A) {
-#pragma unroll
-                for (int k = 0; k < 6; ++k) row[k] = masked_entry(cd, A, gi, side == 0 ? 1 : 2, 6 * r + k);
-            } else {
-                const double* X = chain_src<SRC>(cd, sm, gi, i, side == 0 ? 1 : 2) + 6 * r;
-#pragma unroll
-                for (int k = 0; k < 6; ++k) row[k] = X[k];
-            }
-#pragma unroll
-            for (int k = 0; k < 36; ++k) { G1[k] = Gd[k]; G2[k] = Gn[k]; }
+        const bool left = i > 0, right = i + h < n;
+        double Lr[6], Ur[6], Dr[6];
+        chain_row<SRC>(cd, sm, A, gi, i, 0, r, Dr);
+        chain_row<SRC>(cd, sm, A, gi, i, 1, r, Lr);
+        chain_row<SRC>(cd, sm, A, gi, i, 2, r, Ur);
+        // one neighbour block at a time (its 36 entries in flight together), so that the lane's
+        // registers hold one block rather than four
+        double a1[6] = {}, a2[6] = {}, nl[6] = {}, nu[6] = {};
+        if (left) {
+            row_times_block(Lr, chain_src<NSRC>(cd, sm, gi - h, i - h, 2), a1);
+            row_times_block(Lr, chain_src<NSRC>(cd, sm, gi - h, i - h, 1), nl);
         }
-        if (act && side == 0) {
-            if (SRC == CH_A) {
-#pragma unroll
-                for (int k = 0; k < 6; ++k) Dr[k] = masked_entry(cd, A, gi, 0, 6 * r + k);
-            } else {
-                const double* X = chain_src<SRC>(cd, sm, gi, i, 0) + 6 * r;
-#pragma unroll
-                for (int k = 0; k < 6; ++k) Dr[k] = X[k];
-            }
+        if (right) {
+            row_times_block(Ur, chain_src<NSRC>(cd, sm, gi + h, i + h, 1), a2);
+            row_times_block(Ur, chain_src<NSRC>(cd, sm, gi + h, i + h, 2), nu);
         }
-        double pd[6], pn[6];
+        double* hs = cd.hist + 72 * ((int64_t)cd.hbase[gi] + l) + 6 * r;
+        if (left) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            double a1 = 0, a2 = 0;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) { a1 = fma(row[k], G1[6 * k + c], a1); a2 = fma(row[k], G2[6 * k + c], a2); }
-            pd[c] = has ? a1 : 0.0;
-            pn[c] = -a2;
+            for (int c = 0; c < 6; ++c) hs[c] = Lr[c];
         }
-        if (act && side == 1) {
+        if (right) {
 #pragma unroll
-            for (int c = 0; c < 6; ++c) P2[tid][c] = pd[c];
+            for (int c = 0; c < 6; ++c) hs[36 + c] = Ur[c];
         }
-        __syncthreads();
-        if (has) {
-            double* hs = cd.hist + 72 * ((int64_t)cd.hbase[gi] + l) + 36 * side + 6 * r;
-            double* dst = chain_dst<DST>(cd, sm, gi, i, side == 0 ? 1 : 2) + 6 * r;
+        // a missing neighbour leaves a zero coupling block (written where the blocks now live)
+        if (left || SRC == CH_A || DST != SRC) {
+            double* dst = chain_dst<DST>(cd, sm, gi, i, 1) + 6 * r;
 #pragma unroll
-            for (int k = 0; k < 6; ++k) { hs[k] = row[k]; dst[k] = pn[k]; }
-        } else if (act && (SRC == CH_A || DST != SRC)) {
-            // no neighbour on this side: the coupling block is zero (written where it now lives)
-            double* dst = chain_dst<DST>(cd, sm, gi, i, side == 0 ? 1 : 2) + 6 * r;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) dst[k] = 0.0;
+            for (int c = 0; c < 6; ++c) dst[c] = left ? -nl[c] : 0.0;
         }
-        if (act && side == 0) {
-            double* dst = chain_dst<DST>(cd, sm, gi, i, 0) + 6 * r;
+        if (right || SRC == CH_A || DST != SRC) {
+            double* dst = chain_dst<DST>(cd, sm, gi, i, 2) + 6 * r;
 #pragma unroll
-            for (int c = 0; c < 6; ++c) dst[c] = Dr[c] - pd[c] - P2[tid + 6][c];
+            for (int c = 0; c < 6; ++c) dst[c] = right ? -nu[c] : 0.0;
         }
-        __syncthreads();
+        double* dst = chain_dst<DST>(cd, sm, gi, i, 0) + 6 * r;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) dst[c] = (Dr[c] - a1[c]) - a2[c];
     }
+    __syncthreads();
 }
 
+// Cyclic reduction of each chain's block-tridiagonal system, one workgroup per chain.
 // Dynamic LDS: 3 × 36 doubles per slot, ceil(n / 2^L0) slots (chain_lds_bytes).
 // `status` receives `solve_id` when a pivot block is singular (no reset between solves is needed)
 __global__ void __launch_bounds__(CHAIN_THREADS) chain_factor(ChainDev cd, const double* __restrict__ A, int* status, int L0,
                                                               const uint8_t* __restrict__ active, int solve_id) {
     extern __shared__ double chain_smem[];
+    __shared__ double gj_buf[CHAIN_GROUPS * GJ_BUF];
     if (!active[cd.inst[blockIdx.x]]) return;   // instance frozen in the Newton iteration
     const int64_t s0 = cd.ptr[blockIdx.x];
     const int32_t n = cd.ptr[blockIdx.x + 1] - (int32_t)s0;
-    const int tid = threadIdx.x;
-    __shared__ double P2[CHAIN_THREADS][6];
-    __shared__ int redo;
     const int32_t slots = (n + (1 << L0) - 1) >> L0;
     const ChainLDS sm{chain_smem, chain_smem + 36 * (int64_t)slots, chain_smem + 72 * (int64_t)slots, L0};
-    if (tid == 0) redo = 0;
-    __syncthreads();
+    const Lane6 ln = lane6(gj_buf);
     bool ok = true;
     const int nl = chain_levels(n);
     for (int l = 0; l < nl; ++l) {
         const bool to_lds = l + 1 >= L0;
         if (l == 0) {
-            if (to_lds) chain_level<CH_A, CH_S>(cd, sm, A, s0, n, l, ok, redo, P2);
-            else chain_level<CH_A, CH_G>(cd, sm, A, s0, n, l, ok, redo, P2);
+            if (to_lds) chain_level<CH_A, CH_S>(cd, sm, A, s0, n, l, ok, ln);
+            else chain_level<CH_A, CH_G>(cd, sm, A, s0, n, l, ok, ln);
         } else if (l < L0) {
-            if (to_lds) chain_level<CH_G, CH_S>(cd, sm, A, s0, n, l, ok, redo, P2);
-            else chain_level<CH_G, CH_G>(cd, sm, A, s0, n, l, ok, redo, P2);
+            if (to_lds) chain_level<CH_G, CH_S>(cd, sm, A, s0, n, l, ok, ln);
+            else chain_level<CH_G, CH_G>(cd, sm, A, s0, n, l, ok, ln);
         } else {
-            chain_level<CH_S, CH_S>(cd, sm, A, s0, n, l, ok, redo, P2);
+            chain_level<CH_S, CH_S>(cd, sm, A, s0, n, l, ok, ln);
         }
     }
-    if (tid == 0) {
-        double D[36], C[36];
-        if (nl == 0) chain_block<CH_A>(cd, sm, A, s0, 0, 0, D);
-        else if (nl >= L0) chain_block<CH_S>(cd, sm, A, s0, 0, 0, D);
-        else chain_block<CH_G>(cd, sm, A, s0, 0, 0, D);
+    // position 0, left alone: its D inverted by the first group
+    if (threadIdx.x < 64) {
+        const bool act = ln.group == 0;
+        const int rr = ln.r < 6 ? ln.r : 0;
+        auto load = [&](double (&w)[6]) {
+            if (nl == 0) chain_row<CH_A>(cd, sm, A, s0, 0, 0, rr, w);
+            else if (nl >= L0) chain_row<CH_S>(cd, sm, A, s0, 0, 0, rr, w);
+            else chain_row<CH_G>(cd, sm, A, s0, 0, 0, rr, w);
+        };
+        double w[6];
+        load(w);
+        const int orow = gj6<6>(w, ln, act, ok, load);
+        if (act && ln.r < 6) {
 #pragma unroll
-        for (int k = 0; k < 36; ++k) C[k] = D[k];
-        if (inv6_nopiv(C)) {
-#pragma unroll
-            for (int k = 0; k < 36; ++k) D[k] = C[k];
-        } else {
-            ok = inv6(D) && ok;
+            for (int c = 0; c < 6; ++c) cd.Dm[36 * s0 + 6 * orow + c] = w[c];
         }
-#pragma unroll
-        for (int k = 0; k < 36; ++k) cd.Dm[36 * s0 + k] = D[k];
     }
     if (!ok) *status = solve_id;
 }
@@ -690,52 +694,57 @@ inline int chain_lds_level(int64_t max_len, int64_t budget) {
 inline size_t chain_lds_bytes(int64_t max_len, int L0) {
     return (size_t)(((max_len + (int64_t(1) << L0) - 1) >> L0) * 3 * 36 * (int64_t)sizeof(double));
 }
+// the solve keeps its two work vectors (12 doubles per position) in LDS up to this chain length
+constexpr int64_t SOLVE_LDS_MAX_LEN = 640;
+inline size_t chain_solve_lds_bytes(int64_t max_len) {
+    return max_len <= SOLVE_LDS_MAX_LEN ? (size_t)(12 * max_len * (int64_t)sizeof(double)) : 0;
+}
 
 // x = M^-1 b along each chain, with the factors of chain_factor: the right-hand side is reduced
 // level by level (y_j = D_j^-1 b_j for eliminated j; b_i -= L_i y_j1 + U_i y_j2 for kept i with the
 // saved blocks), then solved back from position 0 (x_j = y_j - GL_j x_{j-h} - GU_j x_{j+h}).
-// Six lanes per position, one per row.
-__global__ void __launch_bounds__(CHAIN_THREADS) chain_solve(ChainDev cd, const double* __restrict__ b, double* __restrict__ x,
-                                                             const KrylovCtl* kc) {
+// Six lanes per position, one per row; the work vectors in LDS (LDSV) or in the global arrays.
+template <bool LDSV>
+__global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const double* __restrict__ b,
+                                                             double* __restrict__ x, const KrylovCtl* kc) {
+    extern __shared__ double solve_smem[];
     if (kc[cd.inst[blockIdx.x]].done) return;
     const int64_t s0 = cd.ptr[blockIdx.x];
     const int32_t n = cd.ptr[blockIdx.x + 1] - (int32_t)s0;
     const int tid = threadIdx.x;
-    for (int64_t t = tid; t < 6 * (int64_t)n; t += CHAIN_THREADS) {
-        const int64_t g = s0 + t / 6;
-        cd.bw[6 * g + t % 6] = b[6 * (int64_t)cd.node[g] + t % 6];
-    }
+    double* bw = LDSV ? solve_smem : cd.bw + 6 * s0;
+    double* yw = LDSV ? solve_smem + 6 * (int64_t)n : cd.yw + 6 * s0;
+    for (int32_t t = tid; t < 6 * n; t += SOLVE_THREADS) bw[t] = b[6 * (int64_t)cd.node[s0 + t / 6] + t % 6];
     __syncthreads();
     const int nl = chain_levels(n);
     for (int l = 0; l < nl; ++l) {
         const int32_t h = 1 << l;
         const int32_t ne = (n - h + 2 * h - 1) / (2 * h);
-        for (int32_t t = tid; t < 6 * ne; t += CHAIN_THREADS) {
-            const int64_t g = s0 + h + 2 * (int64_t)h * (t / 6);
+        for (int32_t t = tid; t < 6 * ne; t += SOLVE_THREADS) {
+            const int32_t p = h + 2 * h * (t / 6);
             const int r = t % 6;
-            const double* Dr = cd.Dm + 36 * g + 6 * r;
+            const double* Dr = cd.Dm + 36 * (s0 + p) + 6 * r;
             double acc = 0;
 #pragma unroll
-            for (int c = 0; c < 6; ++c) acc = fma(Dr[c], cd.bw[6 * g + c], acc);
-            cd.yw[6 * g + r] = acc;
+            for (int c = 0; c < 6; ++c) acc = fma(Dr[c], bw[6 * p + c], acc);
+            yw[6 * p + r] = acc;
         }
         __syncthreads();
         const int32_t nk = (n + 2 * h - 1) / (2 * h);
-        for (int32_t t = tid; t < 6 * nk; t += CHAIN_THREADS) {
+        for (int32_t t = tid; t < 6 * nk; t += SOLVE_THREADS) {
             const int32_t i = 2 * h * (t / 6);
             const int r = t % 6;
-            const int64_t gi = s0 + i;
-            const double* hs = cd.hist + 72 * ((int64_t)cd.hbase[gi] + l) + 6 * r;
-            double v = cd.bw[6 * gi + r];
+            const double* hs = cd.hist + 72 * ((int64_t)cd.hbase[s0 + i] + l) + 6 * r;
+            double v = bw[6 * i + r];
             if (i > 0) {
 #pragma unroll
-                for (int c = 0; c < 6; ++c) v = fma(-hs[c], cd.yw[6 * (gi - h) + c], v);
+                for (int c = 0; c < 6; ++c) v = fma(-hs[c], yw[6 * (i - h) + c], v);
             }
             if (i + h < n) {
 #pragma unroll
-                for (int c = 0; c < 6; ++c) v = fma(-hs[36 + c], cd.yw[6 * (gi + h) + c], v);
+                for (int c = 0; c < 6; ++c) v = fma(-hs[36 + c], yw[6 * (i + h) + c], v);
             }
-            cd.bw[6 * gi + r] = v;
+            bw[6 * i + r] = v;
         }
         __syncthreads();
     }
@@ -743,34 +752,31 @@ __global__ void __launch_bounds__(CHAIN_THREADS) chain_solve(ChainDev cd, const 
         const double* Dr = cd.Dm + 36 * s0 + 6 * tid;
         double acc = 0;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) acc = fma(Dr[c], cd.bw[6 * s0 + c], acc);
-        cd.yw[6 * s0 + tid] = acc;
+        for (int c = 0; c < 6; ++c) acc = fma(Dr[c], bw[c], acc);
+        yw[tid] = acc;
     }
     __syncthreads();
     for (int l = nl - 1; l >= 0; --l) {
         const int32_t h = 1 << l;
         const int32_t ne = (n - h + 2 * h - 1) / (2 * h);
-        for (int32_t t = tid; t < 6 * ne; t += CHAIN_THREADS) {
+        for (int32_t t = tid; t < 6 * ne; t += SOLVE_THREADS) {
             const int32_t j = h + 2 * h * (t / 6);
             const int r = t % 6;
             const int64_t g = s0 + j;
             const double* GL = cd.Lm + 36 * g + 6 * r;
             const double* GU = cd.Um + 36 * g + 6 * r;
-            double v = cd.yw[6 * g + r];
+            double v = yw[6 * j + r];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) v = fma(-GL[c], cd.yw[6 * (g - h) + c], v);
+            for (int c = 0; c < 6; ++c) v = fma(-GL[c], yw[6 * (j - h) + c], v);
             if (j + h < n) {
 #pragma unroll
-                for (int c = 0; c < 6; ++c) v = fma(-GU[c], cd.yw[6 * (g + h) + c], v);
+                for (int c = 0; c < 6; ++c) v = fma(-GU[c], yw[6 * (j + h) + c], v);
             }
-            cd.yw[6 * g + r] = v;
+            yw[6 * j + r] = v;
         }
         __syncthreads();
     }
-    for (int64_t t = tid; t < 6 * (int64_t)n; t += CHAIN_THREADS) {
-        const int64_t g = s0 + t / 6;
-        x[6 * (int64_t)cd.node[g] + t % 6] = cd.yw[6 * g + t % 6];
-    }
+    for (int32_t t = tid; t < 6 * n; t += SOLVE_THREADS) x[6 * (int64_t)cd.node[s0 + t / 6] + t % 6] = yw[t];
 }
 
 // ---- BiCGStab (van der Vorst), right-preconditioned, FP64 scalars on the device -------------
