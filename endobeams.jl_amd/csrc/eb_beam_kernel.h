@@ -1,7 +1,8 @@
 // eb_beam_kernel.h — corotational beam element (compute_beams, src/beams/compute_beams.jl:1-538)
 // as a gfx950 kernel: one lane per element. Element constants are component-major ([k][E]) so a
-// wavefront's loads are contiguous rows; outputs are element-major so that the assembly gather
-// reads each 6x6 sub-block from one 576-byte span.
+// wavefront's loads are contiguous rows; outputs are element-major, each 12x12 matrix as its four
+// 6x6 node blocks, row-major (stage_idx), so that the assembly gather reads a block's 36 entries
+// as consecutive values.
 //
 // Precision: the local frame, R̅ = Rₑᵀ R Rₑ⁰, the log map Θ̅, Tₛ⁻¹(Θ̅), η/μ and K̄ₕ are always FP64;
 // everything downstream of them runs in the physics type P (double for the parity form, float
@@ -24,7 +25,7 @@ struct BeamConsts {  // component-major element constants
 };
 
 template <typename S> struct BeamStage {  // element-major outputs: beam e owns a contiguous record
-    S *K, *C, *M;        // [E][144], 12x12 column-major (local dof order u₁ θ₁ u₂ θ₂), storage type S
+    S *K, *C, *M;        // [E][144] in stage_idx order (local dof order u₁ θ₁ u₂ θ₂), storage type S
     double *Tint, *Tk;   // [E][12]: forces are always FP64 (they make the residual)
     double* energy;      // [E][3]: strain, kinetic, contact
 };
@@ -34,11 +35,15 @@ struct GaussPts { int n; double w[8], z[8]; };
 EBD V3<double> ldv(const double* a, int64_t i) { return {a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }
 EBD M3<double> ldm(const double* a, int64_t i) { M3<double> m; for (int k = 0; k < 9; ++k) m.a[k] = a[9 * i + k]; return m; }
 
+// entry (R, C) of beam e's 12x12 matrix in the staging: node block (R/6, C/6) at position
+// 2(R/6) + C/6 of the beam's record, each block row-major
+EBD int64_t stage_idx(int64_t e, int R, int C) { return e * 144 + (2 * (R / 6) + C / 6) * 36 + (R % 6) * 6 + C % 6; }
+
 template <typename S, typename P> EBD void st_block(S* base, int64_t E, int64_t e, int bi, int bj, const M3<P>& B) {
 #pragma unroll
     for (int j = 0; j < 3; ++j)
 #pragma unroll
-        for (int i = 0; i < 3; ++i) base[e * 144 + (3 * bj + j) * 12 + 3 * bi + i] = S(B.a[i + 3 * j]);
+        for (int i = 0; i < 3; ++i) base[stage_idx(e, 3 * bi + i, 3 * bj + j)] = S(B.a[i + 3 * j]);
 }
 template <typename P> EBD void st_vec(double* base, int64_t E, int64_t e, int b, V3<P> v) {
     base[e * 12 + 3 * b] = double(v.x); base[e * 12 + 3 * b + 1] = double(v.y); base[e * 12 + 3 * b + 2] = double(v.z);
@@ -433,7 +438,7 @@ EBD void beam_damping(int64_t e, const NodeState& ns, const BeamConsts& bc, cons
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
-            for (int i = 0; i < 3; ++i) st.C[e * 144 + (3 * bj + j) * 12 + 3 * bi + i] += S(B.a[i + 3 * j]);
+            for (int i = 0; i < 3; ++i) st.C[stage_idx(e, 3 * bi + i, 3 * bj + j)] += S(B.a[i + 3 * j]);
     };
     add(0, 0, d * m.M11); add(0, 1, d * m.M12); add(0, 2, d * m.M13); add(0, 3, d * m.M14);
     add(1, 0, (d * tr(m.M12)) * TT1t); add(1, 1, (d * m.M22) * TT1t);

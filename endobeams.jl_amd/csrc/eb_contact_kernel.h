@@ -320,7 +320,7 @@ EBD void contact_column(int64_t e, unsigned mask, const NodeState& ns, const Bea
         for (int j = 0; j < 3; ++j)
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                int64_t idx = e * 144 + (3 * B + j) * 12 + 3 * a + i;
+                const int64_t idx = stage_idx(e, 3 * a + i, 3 * B + j);
                 st.K[idx] += C(-K.a[i + 3 * j]);
                 st.C[idx] += C(-(c1a * Cm.a[i + 3 * j]));
             }

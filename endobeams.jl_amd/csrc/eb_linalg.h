@@ -9,6 +9,22 @@ namespace ebd {
 constexpr int RED_BLOCK = 256;
 constexpr int RED_MAX_GRID = 1024;
 
+// six doubles at a 16-byte aligned address, in three 16-byte accesses
+EBD void ld6(const double* p, double* out) {
+    const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double2 v = q[k];
+        out[2 * k] = v.x;
+        out[2 * k + 1] = v.y;
+    }
+}
+EBD void st6(double* p, const double* v) {
+    double2* q = reinterpret_cast<double2*>(p);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) q[k] = make_double2(v[2 * k], v[2 * k + 1]);
+}
+
 // ---- deterministic reductions ---------------------------------------------------------------
 
 template <int NQ> EBD void block_sum_store(double (&v)[NQ], double* partials, int grid) {
@@ -134,7 +150,7 @@ __global__ void __launch_bounds__(256) gather_matrices(int64_t nnz, const int32_
         uint32_t src = blk_src[s];
         int64_t e = src >> 2;
         int sr = (src >> 1) & 1, sc = src & 1;
-        int64_t idx = e * 144 + (6 * sc + c) * 12 + 6 * sr + a;
+        const int64_t idx = e * 144 + (2 * sr + sc) * 36 + 6 * a + c;   // stage_idx(e, 6 sr + a, 6 sc + c)
         k += double(sK[idx]);
         cc += double(sC[idx]);
         m += double(sM[idx]);
@@ -300,10 +316,11 @@ __global__ void __launch_bounds__(RED_BLOCK) spmv_masked(Segs sg, const int32_t*
             const int a = (int)(i % 6);
             acc = 0;
             for (int b = bptr[n]; b < bptr[n + 1]; ++b) {
-                const double* row = A + (int64_t)36 * b + 6 * a;
-                const double* xs = x + 6 * (int64_t)bcol[b];
+                double rv[6], xv[6];
+                ld6(A + (int64_t)36 * b + 6 * a, rv);
+                ld6(x + 6 * (int64_t)bcol[b], xv);
 #pragma unroll
-                for (int c = 0; c < 6; ++c) acc += row[c] * xs[c];
+                for (int c = 0; c < 6; ++c) acc += rv[c] * xv[c];
             }
         }
         y[i] = acc;
@@ -396,14 +413,17 @@ template <int NC> EBD bool gj6_nopiv(double (&w)[NC], int r, double* buf) {
 #pragma unroll
             for (int c = 0; c < NC; ++c) w[c] *= ip;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) buf[c] = w[c];
+            for (int c = 0; c < NC; c += 6) st6(buf + c, w + c);
         }
         wave_sync();
         if (r != p) {
+            double pr[NC];
+#pragma unroll
+            for (int c = 0; c < NC; c += 6) ld6(buf + c, pr + c);
             const double f = w[p];
             w[p] = 0.0;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) w[c] = fma(-f, buf[c], w[c]);
+            for (int c = 0; c < NC; ++c) w[c] = fma(-f, pr[c], w[c]);
         }
         wave_sync();
     }
@@ -444,15 +464,18 @@ template <int NC> EBD bool gj6_pivot(double (&w)[NC], int r, double* buf, int& o
             for (int c = 0; c < NW; ++c) x[c] *= ip;
             x[p] = 1.0;
 #pragma unroll
-            for (int c = 0; c < NW; ++c) buf[c] = x[c];
+            for (int c = 0; c < NW; c += 6) st6(buf + c, x + c);
             used = true;
             orow = p;
         }
         wave_sync();
         if (r != q) {
+            double pr[NW];
+#pragma unroll
+            for (int c = 0; c < NW; c += 6) ld6(buf + c, pr + c);
             const double f = x[p];
 #pragma unroll
-            for (int c = 0; c < NW; ++c) x[c] = fma(-f, buf[c], x[c]);
+            for (int c = 0; c < NW; ++c) x[c] = fma(-f, pr[c], x[c]);
         }
         wave_sync();
     }
@@ -517,16 +540,13 @@ EBD void chain_row(const ChainDev& cd, const ChainLDS& sm, const double* __restr
     if (SRC == CH_A) {
         const uint64_t m = kind == 0 ? cd.mD[g] : (kind == 1 ? cd.mL[g] : cd.mU[g]);
         const int32_t blk = kind == 0 ? cd.diag_blk[g] : (kind == 1 ? cd.lo_blk[g] : cd.up_blk[g]);
-        const double* src = A + 36 * (int64_t)(blk >= 0 ? blk : 0) + 6 * r;
+        double v[6];
+        ld6(A + 36 * (int64_t)(blk >= 0 ? blk : 0) + 6 * r, v);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            const double v = src[c];
-            out[c] = ((m >> (6 * r + c)) & 1) ? ((kind == 0 && r == c) ? 1.0 : 0.0) : (blk >= 0 ? v : 0.0);
-        }
+        for (int c = 0; c < 6; ++c)
+            out[c] = ((m >> (6 * r + c)) & 1) ? ((kind == 0 && r == c) ? 1.0 : 0.0) : (blk >= 0 ? v[c] : 0.0);
     } else {
-        const double* src = chain_src<SRC>(cd, sm, g, p, kind) + 6 * r;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) out[c] = src[c];
+        ld6(chain_src<SRC>(cd, sm, g, p, kind) + 6 * r, out);
     }
 }
 
@@ -534,7 +554,7 @@ EBD void chain_row(const ChainDev& cd, const ChainLDS& sm, const double* __restr
 EBD void row_times_block(const double (&row)[6], const double* __restrict__ G, double (&acc)[6]) {
     double g[36];
 #pragma unroll
-    for (int k = 0; k < 36; ++k) g[k] = G[k];
+    for (int k = 0; k < 36; k += 6) ld6(G + k, g + k);
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
 #pragma unroll
@@ -572,12 +592,10 @@ EBD void chain_level(const ChainDev& cd, const ChainLDS& sm, const double* __res
         const int orow = gj6<18>(w, ln, act, ok, load);
         if (act && ln.r < 6) {
             const int64_t o = 36 * g + 6 * orow;
-#pragma unroll
-            for (int c = 0; c < 6; ++c) { cd.Dm[o + c] = w[c]; cd.Lm[o + c] = w[6 + c]; cd.Um[o + c] = w[12 + c]; }
+            st6(cd.Dm + o, w); st6(cd.Lm + o, w + 6); st6(cd.Um + o, w + 12);
             if (SRC == CH_S) {
                 const int64_t so = 36 * (int64_t)(p >> sm.L0) + 6 * orow;
-#pragma unroll
-                for (int c = 0; c < 6; ++c) { sm.D[so + c] = w[c]; sm.L[so + c] = w[6 + c]; sm.U[so + c] = w[12 + c]; }
+                st6(sm.D + so, w); st6(sm.L + so, w + 6); st6(sm.U + so, w + 12);
             }
         }
     }
@@ -612,39 +630,38 @@ EBD void chain_level(const ChainDev& cd, const ChainLDS& sm, const double* __res
             row_times_block(Ur, chain_src<NSRC>(cd, sm, gi + h, i + h, 2), nu);
         }
         double* hs = cd.hist + 72 * ((int64_t)cd.hbase[gi] + l) + 6 * r;
-        if (left) {
-#pragma unroll
-            for (int c = 0; c < 6; ++c) hs[c] = Lr[c];
-        }
-        if (right) {
-#pragma unroll
-            for (int c = 0; c < 6; ++c) hs[36 + c] = Ur[c];
-        }
+        if (left) st6(hs, Lr);
+        if (right) st6(hs + 36, Ur);
         // a missing neighbour leaves a zero coupling block (written where the blocks now live)
+        double t[6];
         if (left || SRC == CH_A || DST != SRC) {
-            double* dst = chain_dst<DST>(cd, sm, gi, i, 1) + 6 * r;
 #pragma unroll
-            for (int c = 0; c < 6; ++c) dst[c] = left ? -nl[c] : 0.0;
+            for (int c = 0; c < 6; ++c) t[c] = left ? -nl[c] : 0.0;
+            st6(chain_dst<DST>(cd, sm, gi, i, 1) + 6 * r, t);
         }
         if (right || SRC == CH_A || DST != SRC) {
-            double* dst = chain_dst<DST>(cd, sm, gi, i, 2) + 6 * r;
 #pragma unroll
-            for (int c = 0; c < 6; ++c) dst[c] = right ? -nu[c] : 0.0;
+            for (int c = 0; c < 6; ++c) t[c] = right ? -nu[c] : 0.0;
+            st6(chain_dst<DST>(cd, sm, gi, i, 2) + 6 * r, t);
         }
-        double* dst = chain_dst<DST>(cd, sm, gi, i, 0) + 6 * r;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) dst[c] = (Dr[c] - a1[c]) - a2[c];
+        for (int c = 0; c < 6; ++c) t[c] = (Dr[c] - a1[c]) - a2[c];
+        st6(chain_dst<DST>(cd, sm, gi, i, 0) + 6 * r, t);
+
     }
     __syncthreads();
 }
 
 // Cyclic reduction of each chain's block-tridiagonal system, one workgroup per chain.
 // Dynamic LDS: 3 × 36 doubles per slot, ceil(n / 2^L0) slots (chain_lds_bytes).
-// `status` receives `solve_id` when a pivot block is singular (no reset between solves is needed)
-__global__ void __launch_bounds__(CHAIN_THREADS) chain_factor(ChainDev cd, const double* __restrict__ A, int* status, int L0,
-                                                              const uint8_t* __restrict__ active, int solve_id) {
-    extern __shared__ double chain_smem[];
-    __shared__ double gj_buf[CHAIN_GROUPS * GJ_BUF];
+// `status` receives `solve_id` when a pivot block is singular (no reset between solves is needed).
+// Two wavefronts per SIMD (at most 256 registers), so that two chains share a CU: a launch has
+// about 1.5 chains per CU.
+__global__ void __launch_bounds__(CHAIN_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
+chain_factor(ChainDev cd, const double* __restrict__ A, int* status, int L0, const uint8_t* __restrict__ active,
+             int solve_id) {
+    extern __shared__ __align__(16) double chain_smem[];
+    __shared__ __align__(16) double gj_buf[CHAIN_GROUPS * GJ_BUF];
     if (!active[cd.inst[blockIdx.x]]) return;   // instance frozen in the Newton iteration
     const int64_t s0 = cd.ptr[blockIdx.x];
     const int32_t n = cd.ptr[blockIdx.x + 1] - (int32_t)s0;
@@ -677,10 +694,7 @@ __global__ void __launch_bounds__(CHAIN_THREADS) chain_factor(ChainDev cd, const
         double w[6];
         load(w);
         const int orow = gj6<6>(w, ln, act, ok, load);
-        if (act && ln.r < 6) {
-#pragma unroll
-            for (int c = 0; c < 6; ++c) cd.Dm[36 * s0 + 6 * orow + c] = w[c];
-        }
+        if (act && ln.r < 6) st6(cd.Dm + 36 * s0 + 6 * orow, w);
     }
     if (!ok) *status = solve_id;
 }
@@ -707,7 +721,7 @@ inline size_t chain_solve_lds_bytes(int64_t max_len) {
 template <bool LDSV>
 __global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const double* __restrict__ b,
                                                              double* __restrict__ x, const KrylovCtl* kc) {
-    extern __shared__ double solve_smem[];
+    extern __shared__ __align__(16) double solve_smem[];
     if (kc[cd.inst[blockIdx.x]].done) return;
     const int64_t s0 = cd.ptr[blockIdx.x];
     const int32_t n = cd.ptr[blockIdx.x + 1] - (int32_t)s0;
@@ -723,10 +737,12 @@ __global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const 
         for (int32_t t = tid; t < 6 * ne; t += SOLVE_THREADS) {
             const int32_t p = h + 2 * h * (t / 6);
             const int r = t % 6;
-            const double* Dr = cd.Dm + 36 * (s0 + p) + 6 * r;
+            double Dr[6], bv[6];
+            ld6(cd.Dm + 36 * (s0 + p) + 6 * r, Dr);
+            ld6(bw + 6 * p, bv);
             double acc = 0;
 #pragma unroll
-            for (int c = 0; c < 6; ++c) acc = fma(Dr[c], bw[6 * p + c], acc);
+            for (int c = 0; c < 6; ++c) acc = fma(Dr[c], bv[c], acc);
             yw[6 * p + r] = acc;
         }
         __syncthreads();
@@ -737,22 +753,30 @@ __global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const 
             const double* hs = cd.hist + 72 * ((int64_t)cd.hbase[s0 + i] + l) + 6 * r;
             double v = bw[6 * i + r];
             if (i > 0) {
+                double hr[6], yv[6];
+                ld6(hs, hr);
+                ld6(yw + 6 * (i - h), yv);
 #pragma unroll
-                for (int c = 0; c < 6; ++c) v = fma(-hs[c], yw[6 * (i - h) + c], v);
+                for (int c = 0; c < 6; ++c) v = fma(-hr[c], yv[c], v);
             }
             if (i + h < n) {
+                double hr[6], yv[6];
+                ld6(hs + 36, hr);
+                ld6(yw + 6 * (i + h), yv);
 #pragma unroll
-                for (int c = 0; c < 6; ++c) v = fma(-hs[36 + c], yw[6 * (i + h) + c], v);
+                for (int c = 0; c < 6; ++c) v = fma(-hr[c], yv[c], v);
             }
             bw[6 * i + r] = v;
         }
         __syncthreads();
     }
     if (tid < 6) {
-        const double* Dr = cd.Dm + 36 * s0 + 6 * tid;
+        double Dr[6], bv[6];
+        ld6(cd.Dm + 36 * s0 + 6 * tid, Dr);
+        ld6(bw, bv);
         double acc = 0;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) acc = fma(Dr[c], bw[c], acc);
+        for (int c = 0; c < 6; ++c) acc = fma(Dr[c], bv[c], acc);
         yw[tid] = acc;
     }
     __syncthreads();
@@ -763,14 +787,20 @@ __global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const 
             const int32_t j = h + 2 * h * (t / 6);
             const int r = t % 6;
             const int64_t g = s0 + j;
-            const double* GL = cd.Lm + 36 * g + 6 * r;
-            const double* GU = cd.Um + 36 * g + 6 * r;
             double v = yw[6 * j + r];
+            {
+                double gr[6], yv[6];
+                ld6(cd.Lm + 36 * g + 6 * r, gr);
+                ld6(yw + 6 * (j - h), yv);
 #pragma unroll
-            for (int c = 0; c < 6; ++c) v = fma(-GL[c], yw[6 * (j - h) + c], v);
+                for (int c = 0; c < 6; ++c) v = fma(-gr[c], yv[c], v);
+            }
             if (j + h < n) {
+                double gr[6], yv[6];
+                ld6(cd.Um + 36 * g + 6 * r, gr);
+                ld6(yw + 6 * (j + h), yv);
 #pragma unroll
-                for (int c = 0; c < 6; ++c) v = fma(-GU[c], yw[6 * (j + h) + c], v);
+                for (int c = 0; c < 6; ++c) v = fma(-gr[c], yv[c], v);
             }
             yw[6 * j + r] = v;
         }
