@@ -68,9 +68,15 @@ struct eb_ctx {
     int hist_count = 0;
     double last_dt = 0;
     double* snap = nullptr;         // node fields U..DR, then D, Ḋ, D̈ of the reference predictor's start
-    double* rn_plain = nullptr;     // [2S]
+    double* snap_l = nullptr;       // the same of the network's start
+    double* rn_learned = nullptr;   // [2S]
     uint8_t* use_plain = nullptr;   // [S]
-    int32_t* pred_counts = nullptr; // [2]
+    uint8_t* use_learned = nullptr; // [S]
+    int32_t* pred_counts = nullptr; // [3]: starts kept, fallbacks (totals), starts kept by the last choice
+    int32_t* h_pred = nullptr;      // pinned copy of pred_counts
+    // the start of the Newton iteration has been assembled (with its tangent, without energies):
+    // the first iteration forms only the residual and the energies
+    bool start_assembled = false;
     // node state
     double* node[16] = {};
     double* X0 = nullptr;
@@ -88,6 +94,17 @@ struct eb_ctx {
     double *K = nullptr, *C = nullptr, *M = nullptr, *Kn = nullptr, *Cn = nullptr, *Mn = nullptr, *Ktan = nullptr;
     bool np1_alias = false;  // K,C,M at n+1 currently equal the step-n matrices (after a swap)
     bool kcm_pending = false; // K,C,M at n+1 not yet gathered from the element staging of the last assembly
+    // What the tangent, the chain factors and the matrices hold, so that a predictor whose tangent
+    // (1+α)Kⁿ + γ/(βΔt) Cⁿ + Mⁿ/(βΔt²) equals the corrector's last one reuses its factorisation:
+    // both are gathered from the same element staging with the same coefficients, bit for bit.
+    struct TanTag {
+        uint64_t ver = 0;           // staging version the tangent was gathered from (0: other)
+        double c1 = 0, c2 = 0, c3 = 0;
+        bool operator==(const TanTag& o) const { return ver == o.ver && c1 == o.c1 && c2 == o.c2 && c3 == o.c3; }
+    };
+    uint64_t stage_ver = 1;         // incremented by every write of the element staging
+    TanTag ktan_tag, factor_tag;    // Ktan's content; the tangent the chain factors are of
+    uint64_t kcm_ver = 0, kcm_n_ver = 0;  // staging versions of K, C, M at n+1 and at n (0: other)
     double *fext = nullptr, *Tint = nullptr, *Tk = nullptr, *Tc = nullptr;
     double *fext_n = nullptr, *Tint_n = nullptr, *Tk_n = nullptr, *Tc_n = nullptr;
     double *D = nullptr, *Dd = nullptr, *Ddd = nullptr, *res = nullptr, *dD = nullptr;
@@ -246,6 +263,7 @@ int check_launch(eb_ctx* ctx, const char* what) {
 
 // K, C, M at n+1 from the element staging of the last assembly, if that assembly left them pending
 template <typename P> int materialize_kcm_t(eb_ctx* ctx) {
+    ctx->kcm_ver = ctx->stage_ver;
     hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz, ctx->blk_ptr,
                        ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM, ctx->K, ctx->C, ctx->M,
                        ctx->Ktan, 0.0, 0.0, 0.0, 0, 1, active_dev(ctx));
@@ -267,12 +285,15 @@ int unalias(eb_ctx* ctx) {
     EB_HIP(hipMemcpyAsync(ctx->C, ctx->Cn, b, hipMemcpyDeviceToDevice, ctx->stream));
     EB_HIP(hipMemcpyAsync(ctx->M, ctx->Mn, b, hipMemcpyDeviceToDevice, ctx->stream));
     ctx->np1_alias = false;
+    ctx->kcm_ver = ctx->kcm_n_ver;
     return EB_OK;
 }
 
 // ---- assembly -------------------------------------------------------------------------------
 
 template <typename P> int launch_beams(eb_ctx* ctx) {
+    ctx->start_assembled = false;
+    ctx->stage_ver++;
     prof_begin(ctx, EB_KERNEL_BEAM);
     hipLaunchKernelGGL(beam_kernel<P>, dim3(nblk(ctx->E, BEAM_TILE)), dim3(4 * BEAM_TILE), 0, ctx->stream, node_state(ctx),
                        beam_consts(ctx), beam_stage<P>(ctx), ctx->gauss, ctx->active, ctx->beam_inst);
@@ -306,7 +327,9 @@ template <typename P> int launch_contact_tangent(eb_ctx* ctx, const ContactArgs<
 
 // element kernel with the contact detection in its static part, then the contact tangent
 template <typename P> int launch_beams_contact(eb_ctx* ctx, double alpha) {
+    ctx->start_assembled = false;
     const ContactArgs<P> ca = contact_args<P>(ctx, alpha);
+    ctx->stage_ver++;
     hipLaunchKernelGGL(contact_reset, dim3(1), dim3(1), 0, ctx->stream, ctx->clist_n);
     prof_begin(ctx, EB_KERNEL_BEAM);
     hipLaunchKernelGGL(beam_detect_kernel<P>, dim3(nblk(ctx->E, BEAM_TILE)), dim3(4 * BEAM_TILE), 0, ctx->stream,
@@ -320,7 +343,9 @@ template <typename P> int launch_beams_contact(eb_ctx* ctx, double alpha) {
 
 // contact detection and tangent on the staging of the last element assembly
 template <typename P> int launch_contact(eb_ctx* ctx, double alpha) {
+    ctx->start_assembled = false;
     const ContactArgs<P> ca = contact_args<P>(ctx, alpha);
+    ctx->stage_ver++;
     hipLaunchKernelGGL(contact_reset, dim3(1), dim3(1), 0, ctx->stream, ctx->clist_n);
     hipLaunchKernelGGL(contact_detect<P>, dim3(nblk(ctx->n_contact, 256)), dim3(256), 0, ctx->stream, node_state(ctx),
                        beam_consts(ctx), ctx->gauss, ca, ctx->clist, ctx->clist_n);
@@ -328,28 +353,35 @@ template <typename P> int launch_contact(eb_ctx* ctx, double alpha) {
     return launch_contact_tangent<P>(ctx, ca);
 }
 
+// matrices false: the vectors (and the residual) only, K, C, M and Ktan left as they are
 template <typename P> int launch_gathers(eb_ctx* ctx, bool with_contact, bool tangent, double dt, double a, double b,
-                                         double g, bool residual, int energy_mask) {
+                                         double g, bool residual, int energy_mask, bool matrices = true) {
     double c1 = 1 + a, c2 = tangent ? g / (b * dt) : 0, c3 = tangent ? 1 / (b * (dt * dt)) : 0;
     prof_begin(ctx, EB_KERNEL_GATHER);
-    // inside the Newton iteration only Ktan is needed: K, C, M at n+1 are gathered from the element
-    // staging when they are first read or when the step is committed (materialize_kcm)
-    hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz, ctx->blk_ptr,
-                       ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM, ctx->K, ctx->C, ctx->M,
-                       ctx->Ktan, c1, c2, c3, tangent ? 1 : 0, tangent ? 0 : 1, active_dev(ctx));
-    EB_TRY(check_launch(ctx, "gather_matrices"));
-    ctx->kcm_pending = tangent;
+    if (matrices) {
+        // inside the Newton iteration only Ktan is needed: K, C, M at n+1 are gathered from the
+        // element staging when they are first read or when the step is committed (materialize_kcm)
+        hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz,
+                           ctx->blk_ptr, ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM, ctx->K,
+                           ctx->C, ctx->M, ctx->Ktan, c1, c2, c3, tangent ? 1 : 0, tangent ? 0 : 1, active_dev(ctx));
+        EB_TRY(check_launch(ctx, "gather_matrices"));
+        ctx->kcm_pending = tangent;
+        if (tangent) ctx->ktan_tag = eb_ctx::TanTag{ctx->stage_ver, c1, c2, c3};
+        else ctx->kcm_ver = ctx->stage_ver;
+        ctx->np1_alias = false;
+    }
     hipLaunchKernelGGL(gather_vectors<P>, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs,
                        ctx->inc_ptr, ctx->inc_src, (const double*)ctx->sTint, (const double*)ctx->sTk, (const double*)ctx->sTc,
                        with_contact ? 1 : 0, ctx->Tint, ctx->Tk, ctx->Tc, residual ? 1 : 0, a, ctx->fext, ctx->fext_n,
                        ctx->Tc_n, ctx->Tint_n, ctx->res, active_dev(ctx));
     EB_TRY(check_launch(ctx, "gather_vectors"));
-    int grid = red_grid(ctx->E);
-    hipLaunchKernelGGL(energy_partials, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, ctx->E, ctx->senergy,
-                       ctx->partials, energy_mask, active_dev(ctx));
-    hipLaunchKernelGGL(energy_finalize, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->energy);
+    if (energy_mask) {
+        int grid = red_grid(ctx->E);
+        hipLaunchKernelGGL(energy_partials, dim3(grid), dim3(RED_BLOCK), 0, ctx->stream, ctx->E, ctx->senergy,
+                           ctx->partials, energy_mask, active_dev(ctx));
+        hipLaunchKernelGGL(energy_finalize, dim3(1), dim3(RED_BLOCK), 0, ctx->stream, ctx->partials, grid, ctx->energy);
+    }
     prof_end(ctx, EB_KERNEL_GATHER);
-    ctx->np1_alias = false;
     return check_launch(ctx, "energy");
 }
 
@@ -364,6 +396,27 @@ int assemble_all(eb_ctx* ctx, bool contact, double alpha, bool fused, double dt,
     }
     EB_TRY(wc ? launch_beams_contact<double>(ctx, alpha) : launch_beams<double>(ctx));
     return launch_gathers<double>(ctx, wc, fused, dt, alpha, b, g, fused, emask);
+}
+
+// the element kernels and the force gathers with the corrector residual, no matrices (a start
+// evaluated for its residual only)
+int assemble_residual(eb_ctx* ctx, double alpha, double dt, double b, double g) {
+    bool wc = ctx->surface != EB_SURFACE_NONE;
+    if (ctx->precision == EB_PRECISION_FP32) {
+        EB_TRY(wc ? launch_beams_contact<float>(ctx, alpha) : launch_beams<float>(ctx));
+        return launch_gathers<float>(ctx, wc, false, dt, alpha, b, g, true, 0, false);
+    }
+    EB_TRY(wc ? launch_beams_contact<double>(ctx, alpha) : launch_beams<double>(ctx));
+    return launch_gathers<double>(ctx, wc, false, dt, alpha, b, g, true, 0, false);
+}
+
+// the force gathers, corrector residual and energies of the element staging in place (its
+// assembly and tangent already done by start_residual)
+int residual_of_staging(eb_ctx* ctx, double alpha, double dt, double b, double g) {
+    bool wc = ctx->surface != EB_SURFACE_NONE;
+    const int emask = wc ? 3 : 1;
+    if (ctx->precision == EB_PRECISION_FP32) return launch_gathers<float>(ctx, wc, false, dt, alpha, b, g, true, emask, false);
+    return launch_gathers<double>(ctx, wc, false, dt, alpha, b, g, true, emask, false);
 }
 
 int bc_lift_launch(eb_ctx* ctx) {
@@ -384,6 +437,7 @@ int precond_factor(eb_ctx* ctx) {
     const size_t smem = chain_lds_bytes(ctx->chains.max_len, L0);
     hipLaunchKernelGGL(chain_factor, dim3((unsigned)nch), dim3(CHAIN_THREADS), smem, ctx->stream, chain_dev(ctx), ctx->Ktan,
                        &ctx->kall->pivot_status, L0, ctx->active, ctx->solve_id);
+    ctx->factor_tag = ctx->ktan_tag;
     prof_end(ctx, EB_KERNEL_PRECOND);
     return check_launch(ctx, "chain_factor");
 }
@@ -426,11 +480,11 @@ int krylov_iteration(eb_ctx* ctx) {
 // The linear solve in pieces: start (factorisation, BiCGStab initialisation), iterations, a poll
 // of the all-instances record (one small copy and one synchronisation), the acceptance check and
 // the scatter of the solution into ΔD. Ktan (block-row order) and res must be current.
-int solve_start(eb_ctx* ctx, const eb_newton_params* p) {
+int solve_start(eb_ctx* ctx, const eb_newton_params* p, bool factor = true) {
     const Segs sg = segs(ctx);
     const double tol = (p && p->krylov_tol > 0) ? p->krylov_tol : 1e-13;
     ctx->solve_id += 1;
-    EB_TRY(precond_factor(ctx));
+    if (factor) EB_TRY(precond_factor(ctx));
     hipLaunchKernelGGL(krylov_init, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->is_fixed,
                        ctx->res, ctx->kx, ctx->kr, ctx->krhat, ctx->kp, ctx->kv, ctx->partials);
     hipLaunchKernelGGL(krylov_init_fin, dim3(1), dim3(FIN_BLOCK), 0, ctx->stream, ctx->partials, sg, ctx->kctl, tol,
@@ -485,8 +539,8 @@ int scatter_launch(eb_ctx* ctx, const KrylovAll* gate) {
 
 // solves the masked free-dof system of every instance active in the Newton iteration and writes
 // their ΔD[free]
-int solve_free(eb_ctx* ctx, const eb_newton_params* p, int* its_out) {
-    EB_TRY(solve_start(ctx, p));
+int solve_free(eb_ctx* ctx, const eb_newton_params* p, int* its_out, bool factor = true) {
+    EB_TRY(solve_start(ctx, p, factor));
     EB_TRY(krylov_iteration(ctx));
     EB_TRY(solve_poll(ctx));
     EB_TRY(solve_finish(ctx, p, 1));
@@ -520,6 +574,7 @@ int norms_launch(eb_ctx* ctx, const NewtonCtl& nc, int* n_active) {
 
 // node state and force vectors, n+1 → n (commit) or n → n+1 (rollback), in one launch
 int copy_state(eb_ctx* ctx, bool commit) {
+    ctx->start_assembled = false;
     CopyList cl{};
     int k = 0;
     int64_t nmax = 0;
@@ -546,21 +601,25 @@ int newton_reset_launch(eb_ctx* ctx) {
 }
 
 int global_corrector_async(eb_ctx* ctx, double dt, double b, double g, const KrylovAll* gate = nullptr) {
+    ctx->start_assembled = false;
     hipLaunchKernelGGL(global_corrector, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs,
                        g / (b * dt), 1 / (b * (dt * dt)), ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, active_dev(ctx), gate);
     return check_launch(ctx, "global_corrector");
 }
 int local_corrector_async(eb_ctx* ctx, double dt, double b, double g, const KrylovAll* gate = nullptr) {
+    ctx->start_assembled = false;
     hipLaunchKernelGGL(local_corrector, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx),
                        ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, hht_consts(dt, b, g), active_dev(ctx), gate);
     return check_launch(ctx, "local_corrector");
 }
 int global_predictor_async(eb_ctx* ctx, double dt, double b, double g) {
+    ctx->start_assembled = false;
     hipLaunchKernelGGL(global_predictor, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs, dt, g / b,
                        g / (b * dt), 1 / (b * (dt * dt)), dt * (2 * b - g) / (2 * b), ctx->D, ctx->Dd, ctx->Ddd, ctx->dD);
     return check_launch(ctx, "global_predictor");
 }
 int local_predictor_async(eb_ctx* ctx, double dt, double b, double g) {
+    ctx->start_assembled = false;
     hipLaunchKernelGGL(local_predictor, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx),
                        ctx->D, ctx->Dd, ctx->Ddd, hht_consts(dt, b, g));
     return check_launch(ctx, "local_predictor");
@@ -572,11 +631,11 @@ const int NODE_FIELD_WIDTH[8] = {3, 3, 3, 3, 3, 3, 9, 9};   // U UDOT UDDOT W WD
 
 // copies the current start (node fields U..DR, D, Ḋ, D̈) into the snapshot, or back for the
 // instances flagged in `flag` (all when null)
-int snapshot_start(eb_ctx* ctx, bool restore, const uint8_t* flag) {
+int snapshot_start(eb_ctx* ctx, bool restore, const uint8_t* flag, double* snap) {
     const int64_t N = ctx->N;
     int64_t off = 0;
     auto cp = [&](double* cur, int per) {
-        double* sn = ctx->snap + off;
+        double* sn = snap + off;
         hipLaunchKernelGGL(copy_node_fields, dim3(nblk(N * per, 256)), dim3(256), 0, ctx->stream, N, per,
                            restore ? (const double*)sn : (const double*)cur, restore ? cur : sn, flag, ctx->node_inst);
         off += per * N;
@@ -590,20 +649,22 @@ int snapshot_start(eb_ctx* ctx, bool restore, const uint8_t* flag) {
 
 // relative residual of every instance at the current start: assembly with contact, the corrector
 // residual and the boundary-condition lift, as the first Newton iteration sees them (energy not
-// accumulated); per-instance norms into ctx->norms
-int start_residual(eb_ctx* ctx, const eb_newton_params* p, double dt) {
-    EB_TRY(assemble_all(ctx, true, p->alpha, true, dt, p->beta, p->gamma, false));
+// accumulated; with the tangent when `tangent`); per-instance norms into ctx->norms
+int start_residual(eb_ctx* ctx, const eb_newton_params* p, double dt, bool tangent) {
+    if (tangent) EB_TRY(assemble_all(ctx, true, p->alpha, true, dt, p->beta, p->gamma, false));
+    else EB_TRY(assemble_residual(ctx, p->alpha, dt, p->beta, p->gamma));
     EB_TRY(bc_lift_launch(ctx));
     NewtonCtl nc{0, 0, 0, 0, 0};
     return norms_launch(ctx, nc, nullptr);
 }
 
-// after the reference predictor: the network's start from step n, and per instance the better one
+// After the reference predictor: the network's start from step n, and per instance the start with
+// the smaller residual. The network's start is evaluated first (residual only), the reference
+// predictor's last with its tangent, so that when every instance keeps the reference start the
+// first Newton iteration finds its assembly done (start_assembled).
 int learned_predictor(eb_ctx* ctx, const eb_newton_params* p, double dt) {
     const int64_t N = ctx->N;
-    EB_TRY(start_residual(ctx, p, dt));
-    EB_HIP(hipMemcpyAsync(ctx->rn_plain, ctx->norms, sizeof(double) * 2 * ctx->S, hipMemcpyDeviceToDevice, ctx->stream));
-    EB_TRY(snapshot_start(ctx, false, nullptr));
+    EB_TRY(snapshot_start(ctx, false, nullptr, ctx->snap));
     // back to step n (the predictor starts from it), then the network's increment
     for (int f = 0; f < 8; ++f)
         hipLaunchKernelGGL(copy_node_fields, dim3(nblk(N * NODE_FIELD_WIDTH[f], 256)), dim3(256), 0, ctx->stream, N,
@@ -624,11 +685,23 @@ int learned_predictor(eb_ctx* ctx, const eb_newton_params* p, double dt) {
     EB_TRY(check_launch(ctx, "learned_apply"));
     EB_TRY(global_predictor_async(ctx, dt, p->beta, p->gamma));
     EB_TRY(local_predictor_async(ctx, dt, p->beta, p->gamma));
-    EB_TRY(start_residual(ctx, p, dt));
-    hipLaunchKernelGGL(learned_choose, dim3(1), dim3(256), 0, ctx->stream, (int)ctx->S, ctx->rn_plain, ctx->norms,
-                       ctx->use_plain, ctx->pred_counts);
+    EB_TRY(start_residual(ctx, p, dt, false));
+    EB_HIP(hipMemcpyAsync(ctx->rn_learned, ctx->norms, sizeof(double) * 2 * ctx->S, hipMemcpyDeviceToDevice, ctx->stream));
+    EB_TRY(snapshot_start(ctx, false, nullptr, ctx->snap_l));
+    // the reference predictor's start, assembled with its tangent
+    EB_TRY(snapshot_start(ctx, true, nullptr, ctx->snap));
+    EB_TRY(start_residual(ctx, p, dt, true));
+    hipLaunchKernelGGL(learned_choose, dim3(1), dim3(256), 0, ctx->stream, (int)ctx->S, ctx->norms, ctx->rn_learned,
+                       ctx->use_plain, ctx->use_learned, ctx->pred_counts);
     EB_TRY(check_launch(ctx, "learned_choose"));
-    return snapshot_start(ctx, true, ctx->use_plain);
+    EB_HIP(hipMemcpyAsync(ctx->h_pred, ctx->pred_counts, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->h_pred[2] == 0) {
+        ctx->start_assembled = true;
+        return EB_OK;
+    }
+    // some instances take the network's start: theirs back, and a full first assembly
+    return snapshot_start(ctx, true, ctx->use_learned, ctx->snap_l);
 }
 
 int field_info(eb_ctx* ctx, int f, double*& ptr, int64_t& len, int& kind) {
@@ -904,9 +977,11 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
     for (int k = 0; k < 3; ++k) CK(dalloc(ctx, ctx->hrate[k], 6 * N));
     CK(dalloc(ctx, ctx->pdD, 6 * N));
     CK(dalloc(ctx, ctx->snap, 54 * N));
-    CK(dalloc(ctx, ctx->rn_plain, 2 * S));
+    CK(dalloc(ctx, ctx->snap_l, 54 * N));
+    CK(dalloc(ctx, ctx->rn_learned, 2 * S));
     CK(dalloc(ctx, ctx->use_plain, S));
-    CK(dalloc(ctx, ctx->pred_counts, 2));
+    CK(dalloc(ctx, ctx->use_learned, S));
+    CK(dalloc(ctx, ctx->pred_counts, 3));
     CK(dalloc(ctx, ctx->lX, LEARN_IN * N));
     CK(dalloc(ctx, ctx->lT, LEARN_OUT * N));
     {
@@ -924,8 +999,10 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
         hipHostMalloc((void**)&ctx->h_status, sizeof(int), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_kall, sizeof(KrylovAll), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_nall, sizeof(NewtonAll), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&ctx->h_kiter, sizeof(int32_t) * S, hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc((void**)&ctx->h_kiter, sizeof(int32_t) * S, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_pred, 3 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
         return bail(EB_ERR_DEVICE);
+    for (int k = 0; k < 3; ++k) ctx->h_pred[k] = 0;
     // contact
     ctx->surface = d->surface;
     if (d->surface != EB_SURFACE_NONE) {
@@ -1011,6 +1088,7 @@ int eb_destroy(eb_ctx* ctx) {
     if (ctx->h_status) hipHostFree(ctx->h_status);
     if (ctx->h_kall) hipHostFree(ctx->h_kall);
     if (ctx->h_nall) hipHostFree(ctx->h_nall);
+    if (ctx->h_pred) hipHostFree(ctx->h_pred);
     if (ctx->h_kiter) hipHostFree(ctx->h_kiter);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1081,7 +1159,10 @@ int eb_get_field(eb_ctx* ctx, int f, double* out, int64_t count) {
 
 int eb_set_field(eb_ctx* ctx, int f, const double* in, int64_t count) {
     if (!ctx || (!in && count)) return EB_ERR_ARG;
+    ctx->start_assembled = false;
     if (f == EB_F_K || f == EB_F_C || f == EB_F_M) EB_TRY(unalias(ctx));
+    ctx->ktan_tag = ctx->factor_tag = eb_ctx::TanTag{};   // matrices set from outside
+    ctx->kcm_ver = ctx->kcm_n_ver = 0;
     double* ptr; int64_t len; int kind;
     EB_TRY(field_info(ctx, f, ptr, len, kind));
     if (count != len) { ctx->err = "field size mismatch"; return EB_ERR_ARG; }
@@ -1122,9 +1203,11 @@ int eb_assemble_contact(eb_ctx* ctx, double alpha) {
 
 int eb_tangent_residuals_corrector(eb_ctx* ctx, double dt, double a, double b, double g) {
     if (!ctx) return EB_ERR_ARG;
+    ctx->start_assembled = false;
     EB_TRY(unalias(ctx));
     hipLaunchKernelGGL(combine_tangent, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz, ctx->K, ctx->C,
                        ctx->M, ctx->Ktan, 1 + a, g / (b * dt), 1 / (b * (dt * dt)));
+    ctx->ktan_tag = eb_ctx::TanTag{};
     hipLaunchKernelGGL(residual_corrector, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs, a,
                        ctx->fext, ctx->Tc, ctx->Tint, ctx->fext_n, ctx->Tc_n, ctx->Tint_n, ctx->Tk, ctx->res);
     EB_TRY(check_launch(ctx, "tangent_residuals_corrector"));
@@ -1132,10 +1215,15 @@ int eb_tangent_residuals_corrector(eb_ctx* ctx, double dt, double a, double b, d
     return EB_OK;
 }
 
-static int tangent_residuals_predictor_async(eb_ctx* ctx, double dt, double a, double b, double g) {
+// `form_tangent` false: Ktan already holds this tangent (see predictor_reuses_factor)
+static int tangent_residuals_predictor_async(eb_ctx* ctx, double dt, double a, double b, double g,
+                                             bool form_tangent = true) {
     double c3 = 1 / (b * (dt * dt));
-    hipLaunchKernelGGL(combine_tangent, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz, ctx->Kn, ctx->Cn,
-                       ctx->Mn, ctx->Ktan, 1 + a, g / (b * dt), c3);
+    if (form_tangent) {
+        hipLaunchKernelGGL(combine_tangent, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz, ctx->Kn,
+                           ctx->Cn, ctx->Mn, ctx->Ktan, 1 + a, g / (b * dt), c3);
+        ctx->ktan_tag = eb_ctx::TanTag{};
+    }
     hipLaunchKernelGGL(residual_predictor, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs, ctx->bptr,
                        ctx->bcol, a, g / b, ((dt / 2) * (2 * b - g)) / b, dt, c3, ctx->fext, ctx->Tint_n, ctx->Tk_n,
                        ctx->fext_n, ctx->Dd, ctx->Ddd, ctx->Cn, ctx->Mn, ctx->res);
@@ -1144,6 +1232,7 @@ static int tangent_residuals_predictor_async(eb_ctx* ctx, double dt, double a, d
 
 int eb_tangent_residuals_predictor(eb_ctx* ctx, double dt, double a, double b, double g) {
     if (!ctx) return EB_ERR_ARG;
+    ctx->start_assembled = false;
     EB_TRY(tangent_residuals_predictor_async(ctx, dt, a, b, g));
     EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
@@ -1151,6 +1240,7 @@ int eb_tangent_residuals_predictor(eb_ctx* ctx, double dt, double a, double b, d
 
 int eb_apply_boundary_conditions(eb_ctx* ctx) {
     if (!ctx) return EB_ERR_ARG;
+    ctx->start_assembled = false;
     EB_TRY(bc_lift_launch(ctx));
     EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
@@ -1161,6 +1251,7 @@ int eb_extract_free_dofs(eb_ctx* ctx) { return ctx ? EB_OK : EB_ERR_ARG; }
 
 int eb_solve_free_dofs(eb_ctx* ctx, const eb_newton_params* p, int* its) {
     if (!ctx) return EB_ERR_ARG;
+    ctx->start_assembled = false;
     EB_TRY(solve_free(ctx, p, its));
     EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
@@ -1168,6 +1259,7 @@ int eb_solve_free_dofs(eb_ctx* ctx, const eb_newton_params* p, int* its) {
 
 int eb_initialize_global_corrector(eb_ctx* ctx) {
     if (!ctx) return EB_ERR_ARG;
+    ctx->start_assembled = false;
     hipLaunchKernelGGL(init_global, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx), ctx->D,
                        ctx->Dd, ctx->Ddd, ctx->dD, 0);
     EB_TRY(check_launch(ctx, "init_global"));
@@ -1177,6 +1269,7 @@ int eb_initialize_global_corrector(eb_ctx* ctx) {
 
 int eb_initialize_global_predictor(eb_ctx* ctx) {
     if (!ctx) return EB_ERR_ARG;
+    ctx->start_assembled = false;
     hipLaunchKernelGGL(init_global, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx), ctx->D,
                        ctx->Dd, ctx->Ddd, ctx->dD, 1);
     EB_TRY(check_launch(ctx, "init_global"));
@@ -1226,6 +1319,7 @@ int eb_compute_norms_corrector(eb_ctx* ctx, double* rn, double* dn) {
 
 int eb_update_loads(eb_ctx* ctx, int64_t n, const int64_t* dofs, const double* values) {
     if (!ctx) return EB_ERR_ARG;
+    ctx->start_assembled = false;
     if (n == 0) return EB_OK;
     // loads are few or dense: upload the touched dofs through the scratch buffers
     std::vector<int32_t> idx((size_t)n);
@@ -1246,6 +1340,7 @@ int eb_update_loads(eb_ctx* ctx, int64_t n, const int64_t* dofs, const double* v
 
 int eb_update_boundary_conditions(eb_ctx* ctx, const double* values, int64_t n_fixed) {
     if (!ctx) return EB_ERR_ARG;
+    ctx->start_assembled = false;
     if (n_fixed != ctx->n_fixed) { ctx->err = "imposed displacement count mismatch"; return EB_ERR_ARG; }
     if (n_fixed == 0) return EB_OK;
     EB_HIP(hipMemcpyAsync(ctx->imposed, values, sizeof(double) * n_fixed, hipMemcpyHostToDevice, ctx->stream));
@@ -1255,6 +1350,7 @@ int eb_update_boundary_conditions(eb_ctx* ctx, const double* values, int64_t n_f
 
 int eb_initialize_state_simulation(eb_ctx* ctx) {
     if (!ctx) return EB_ERR_ARG;
+    ctx->start_assembled = false;
     ctx->hist_count = 0;
     EB_TRY(assemble_all(ctx, false, 0.0, false, 1.0, 1.0, 1.0));
     size_t b = sizeof(double) * ctx->nnz;
@@ -1265,14 +1361,23 @@ int eb_initialize_state_simulation(eb_ctx* ctx) {
     return EB_OK;
 }
 
+// The predictor's tangent from Kⁿ, Cⁿ, Mⁿ equals the tangent of the last corrector iteration when
+// both were gathered from the same element staging with the same HHT coefficients (the step
+// size is unchanged): then Ktan and the chain factors of that iteration are reused as they are.
+bool predictor_reuses_factor(const eb_ctx* ctx, double dt, double a, double b, double g) {
+    const eb_ctx::TanTag want{ctx->kcm_n_ver, 1 + a, g / (b * dt), 1 / (b * (dt * dt))};
+    return ctx->kcm_n_ver != 0 && ctx->ktan_tag == want && ctx->factor_tag == want;
+}
+
 int eb_predictor(eb_ctx* ctx, const eb_newton_params* p, double dt) {
     if (!ctx || !p) return EB_ERR_ARG;
     ctx->last_dt = dt;
     hipLaunchKernelGGL(init_global, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx), ctx->D,
                        ctx->Dd, ctx->Ddd, ctx->dD, 1);
-    EB_TRY(tangent_residuals_predictor_async(ctx, dt, p->alpha, p->beta, p->gamma));
+    const bool reuse = predictor_reuses_factor(ctx, dt, p->alpha, p->beta, p->gamma);
+    EB_TRY(tangent_residuals_predictor_async(ctx, dt, p->alpha, p->beta, p->gamma, !reuse));
     EB_TRY(bc_lift_launch(ctx));
-    EB_TRY(solve_free(ctx, p, nullptr));
+    EB_TRY(solve_free(ctx, p, nullptr, !reuse));
     EB_TRY(global_predictor_async(ctx, dt, p->beta, p->gamma));
     EB_TRY(local_predictor_async(ctx, dt, p->beta, p->gamma));
     // the reference predictor's increment: an input of the learned start and of its samples
@@ -1298,7 +1403,9 @@ int eb_corrector(eb_ctx* ctx, const eb_newton_params* p, double dt, int* kout) {
         // usual case with the chain preconditioner); otherwise the solve continues and the update
         // and norms are launched again after it
         NewtonCtl nc{p->tol_res, p->tol_disp, k, p->max_iterations, 1};
-        st = assemble_all(ctx, true, p->alpha, true, dt, p->beta, p->gamma);
+        if (k == 1 && ctx->start_assembled) st = residual_of_staging(ctx, p->alpha, dt, p->beta, p->gamma);
+        else st = assemble_all(ctx, true, p->alpha, true, dt, p->beta, p->gamma);
+        ctx->start_assembled = false;
         if (st == EB_OK) st = bc_lift_launch(ctx);
         if (st == EB_OK) st = solve_start(ctx, p);
         if (st == EB_OK) st = krylov_iteration(ctx);
@@ -1371,6 +1478,7 @@ int eb_update_converged(eb_ctx* ctx) {
         std::swap(ctx->C, ctx->Cn);
         std::swap(ctx->M, ctx->Mn);
         ctx->np1_alias = true;
+        ctx->kcm_n_ver = ctx->kcm_ver;
     }
     EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
@@ -1381,6 +1489,7 @@ int eb_update_not_converged(eb_ctx* ctx) {
     EB_TRY(copy_state(ctx, false));
     ctx->kcm_pending = false;
     ctx->np1_alias = true;
+    ctx->kcm_ver = ctx->kcm_n_ver;
     EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
 }

@@ -64,14 +64,28 @@ __global__ void __launch_bounds__(256) copy_node_fields(int64_t N, int per_node,
     dst[i] = src[i];
 }
 
-// per instance: keep the network's start when its residual is not larger (and finite)
-__global__ void learned_choose(int ninst, const double* rn_plain, const double* rn_learned, uint8_t* use_plain,
-                               int32_t* counts) {
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ninst; s += gridDim.x * blockDim.x) {
+// one workgroup: per instance the start with the smaller relative residual (the network's when
+// it is not larger, and finite); counts[0], counts[1]: running totals of the two choices,
+// counts[2]: instances taking the network's start in this call
+__global__ void __launch_bounds__(256) learned_choose(int ninst, const double* rn_plain, const double* rn_learned,
+                                                      uint8_t* use_plain, uint8_t* use_learned, int32_t* counts) {
+    __shared__ int kept;
+    if (threadIdx.x == 0) kept = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int s = threadIdx.x; s < ninst; s += blockDim.x) {
         const double a = rn_plain[2 * s], b = rn_learned[2 * s];
         const bool learned = b <= a && b == b;
         use_plain[s] = learned ? 0 : 1;
-        atomicAdd(&counts[learned ? 0 : 1], 1);
+        use_learned[s] = learned ? 1 : 0;
+        mine += learned ? 1 : 0;
+    }
+    atomicAdd(&kept, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        counts[0] += kept;
+        counts[1] += ninst - kept;
+        counts[2] = kept;
     }
 }
 

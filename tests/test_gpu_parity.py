@@ -229,3 +229,37 @@ def test_full_size_assembly_parity(builder):
         e.assemble_contact(params.alpha)
     for f in ("TINT", "TK", "TC", "K", "C", "M"):
         assert rel_err(g.get(f), o.get(f), floor=1e-300) <= TOL, f
+
+
+def test_predictor_reuses_last_factorisation():
+    """With Δt unchanged, the predictor's tangent (1+α)Kⁿ + γ/(βΔt)Cⁿ + Mⁿ/(βΔt²) is the last
+    corrector iteration's tangent bit for bit, so its chain factors are reused: one
+    factorisation fewer than linear solves per such step, and the trajectory still follows the
+    oracle step for step."""
+    from endobeams_amd import Stepper
+    conf, params, inter = SETUPS["l_cantilever"]()
+    g = HipEngine(conf, params, inter)
+    o = O.OracleEngine(conf, params, inter)
+    sg, so = Stepper(g, params), Stepper(o, params)
+    sg.initialize()
+    so.initialize()
+    g.profile_enable(["precond"])
+    solves0 = int(g.counters()[2])
+    # a predictor can reuse when the attempt before it converged with the same step size
+    eligible, last = 0, None
+    while sg.stats.steps < 30:
+        dt = sg.dt
+        assert so.dt == dt
+        if last is not None and last == (True, dt):
+            eligible += 1
+        ok_g, ok_o = sg.advance(), so.advance()
+        assert ok_g == ok_o
+        last = (ok_g, dt)
+    _, factorisations = g.profile_read("precond")
+    solves = int(g.counters()[2]) - solves0
+    assert sg.stats.iterations_per_step == so.stats.iterations_per_step
+    assert sg.stats.failed_steps == so.stats.failed_steps
+    assert eligible >= 5, eligible
+    assert factorisations == solves - eligible, (factorisations, solves, eligible)
+    for f in ("U_N", "R_N"):
+        assert rel_err(g.get(f), o.get(f), floor=1e-30) <= 1e-10, f
