@@ -57,11 +57,14 @@ def run_steps(stepper, n):
     return done, attempts
 
 
-def timed(eng, params, E, warmup, steps, kernel="beam"):
+def timed(eng, params, E, warmup, steps, kernel="beam", graph=True, autotune=True):
+    """Warm-up steps (then the autotuner, on the live state), then `steps` timed converged steps."""
     from endobeams_amd import Stepper
+    eng.set_exec_mode(graph)
     st = Stepper(eng, params)
     st.initialize()
     run_steps(st, warmup)
+    tuning = eng.autotune() if autotune else (eng.tuning, None)
     eng.profile_enable([kernel])
     n0, it0, t0sim, f0 = st.stats.steps, st.stats.newton_iterations, st.tn, st.stats.failed_steps
     t0 = time.perf_counter()
@@ -73,7 +76,9 @@ def timed(eng, params, E, warmup, steps, kernel="beam"):
             "newton_iters_per_step": (st.stats.newton_iterations - it0) / max(done, 1),
             "failed_steps": st.stats.failed_steps - f0,
             "wall_s_per_simulated_s": wall / (st.tn - t0sim) if st.tn > t0sim else None,
-            "kernel_ms": ms, "kernel_launches": launches, "stepper": st}
+            "kernel_ms": ms, "kernel_launches": launches, "stepper": st,
+            "exec": {"mode": "graph" if graph else "eager", "graph_captures_replays": list(eng.exec_stats()[1:]),
+                     "tuning": tuning[0], "tuning_ms": tuning[1]}}
 
 
 def cpu_baseline(seconds):
@@ -100,6 +105,93 @@ def cpu_baseline(seconds):
     return out, eng, st.stats.steps
 
 
+def sweep(args):
+    """The design sweep: `--instances` braided stents of uneven sizes (1 200 to 14 400 beams each),
+    each in its own vessel SDF, FP32 element matrices, advanced in batch contexts whose size the
+    batch-shape autotuner picks (each context then autotunes its launch shapes); the learned warm
+    start (trained on 8 seeded stents) with the reference predictor as the fallback."""
+    from endobeams_amd import HipBatchEngine, Stepper, abi, autotune, configs
+    from endobeams_amd import learned as L
+    t = time.perf_counter()
+    inst = configs.stent_sweep(args.instances, sdf_n=args.sweep_sdf_n)
+    E_all = sum(len(c.beams) for (c, _, _) in inst)
+    print(f"# sweep of {len(inst)} stents, {E_all} beams, built in {time.perf_counter() - t:.1f} s",
+          file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    tr = [stent(seed=1000 + s, sdf_n=args.sweep_sdf_n) for s in range(8)]
+    teng = HipBatchEngine(tr, precision=abi.EB_PRECISION_FP32)
+    X, T = L.collect(teng, tr[0][1], args.train_steps)
+    teng.close()
+    ws = L.WarmStart.train(X, T, epochs=args.epochs, batch=4096, lr=1e-3, seed=0)
+    print(f"# network trained in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    if args.sub_batch > 0:
+        size, rates, shapes = min(args.sub_batch, len(inst)), {}, {}
+    else:
+        t = time.perf_counter()
+        size, rates, shapes = autotune.pick_batch_size(
+            inst, [c for c in (64, 128, 256, 512, 1024)], warmup=1, steps=2,
+            log=lambda c, r: print(f"#   {c} instances per context: {r:.0f} element-steps/s", file=sys.stderr, flush=True))
+        print(f"# batch shape: {size} instances per context ({ {k: round(v) for k, v in rates.items()} }), "
+              f"tuned in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    groups = autotune.split(inst, size)
+    engs, steppers = [], []
+    for gi, g in enumerate(groups):
+        print(f"# context {gi + 1}/{len(groups)}: {len(g)} instances", file=sys.stderr, flush=True)
+        e = HipBatchEngine(g, precision=abi.EB_PRECISION_FP32)
+        e.set_exec_mode(args.exec == "graph")
+        ws.install(e)
+        st = Stepper(e, g[0][1])
+        st.initialize()
+        run_steps(st, args.warmup)
+        e.autotune()
+        e.profile_enable(["beam"])
+        engs.append(e)
+        steppers.append(st)
+    base = [(st.stats.steps, st.stats.newton_iterations, st.stats.failed_steps, st.tn) for st in steppers]
+    t0 = time.perf_counter()
+    done = [0] * len(engs)
+    for _ in range(args.steps):
+        for i, st in enumerate(steppers):
+            d, _ = run_steps(st, 1)
+            done[i] += d
+    wall = time.perf_counter() - t0
+    E_g = [int(e.beam_off[-1]) for e in engs]
+    value = sum(E * d for E, d in zip(E_g, done)) / wall
+    its = sum(st.stats.newton_iterations - b[1] for st, b in zip(steppers, base)) / max(sum(done), 1)
+    failed = sum(st.stats.failed_steps - b[2] for st, b in zip(steppers, base))
+    sim = min(st.tn - b[3] for st, b in zip(steppers, base))
+    kms = kn = 0.0
+    for e in engs:
+        ms, n = e.profile_read("beam")
+        kms += ms
+        kn += n
+    chosen = [L.predictor_stats(e).tolist() for e in engs]
+    avg_s = (kms / max(kn, 1)) * 1e-3
+    E_launch = sum(E_g) / len(E_g)
+    achieved = FLOPS_PER_BEAM * E_launch / avg_s / 1e12
+    out = {
+        "metric": METRIC, "value": value, "unit": "element-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"stent design sweep: {len(inst)} braided stents, {E_all} beams, "
+                               f"{args.sweep_sdf_n}^3 FP32 vessel SDF each, Newton tolerance 1e-8",
+                   "form": "learned warm start + FP32 element matrices + FP64 forces/solver, "
+                           f"{len(groups)} batch contexts of {size} instances",
+                   "exec": args.exec, "dt": 1e-3, "tolerance": 1e-8, "beams": E_all},
+        "newton_iters_per_step": its, "failed_steps": failed,
+        "wall_s_per_simulated_s": wall / sim if sim > 0 else None,
+        "batch_autotune": {"instances_per_context": size, "element_steps_per_s": rates,
+                           "launch_shapes": shapes.get(size)},
+        "learned_choices_per_context": chosen,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "beam_detect_kernel<float>",
+                     "avg_launch_us": avg_s * 1e6, "launches": int(kn), "flops_per_launch": FLOPS_PER_BEAM * E_launch},
+    }
+    for e in engs:
+        e.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +204,14 @@ def main():
     ap.add_argument("--only", choices=["plain", "fast", "learned"], default=None, help="time one form only")
     ap.add_argument("--train-steps", type=int, default=40, help="converged steps recorded per training stent")
     ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--exec", choices=["graph", "eager"], default="eager",
+                    help="Newton iterations launched one by one (default) or as HIP graph replays")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch shapes")
+    ap.add_argument("--workload", choices=["stent", "sweep"], default="stent",
+                    help="stent: the braided stent (default); sweep: the stent design sweep")
+    ap.add_argument("--instances", type=int, default=512, help="stents in the design sweep")
+    ap.add_argument("--sweep-sdf-n", type=int, default=128)
+    ap.add_argument("--sub-batch", type=int, default=0, help="instances per context in the sweep (0: autotuned)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device to device 5 times (scales HBM counters of a profiler run)")
     args = ap.parse_args()
@@ -130,15 +230,20 @@ def main():
               "(independent structures would shard across GPUs without communication)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "sweep":
+        sweep(args)
+        return
+
     from endobeams_amd import HipBatchEngine, HipEngine, abi
 
     from endobeams_amd import learned as L
 
     plain = fast = lrn = None
+    xo = {"graph": args.exec == "graph", "autotune": not args.no_autotune}
     if args.only in (None, "plain"):
         conf, params, inter = stent(sdf_n=args.sdf_n)
         eng = HipEngine(conf, params, inter)
-        plain = timed(eng, params, len(conf.beams), args.warmup, args.steps)
+        plain = timed(eng, params, len(conf.beams), args.warmup, args.steps, **xo)
         print(f"# plain FP64: {plain['steps']} steps, {plain['ms_per_step']:.2f} ms/step, "
               f"{plain['value']:.0f} element-steps/s", file=sys.stderr, flush=True)
         eng.close()
@@ -150,7 +255,7 @@ def main():
     if args.only in (None, "fast"):
         eng = HipBatchEngine(inst, precision=abi.EB_PRECISION_FP32)
         E = int(eng.beam_off[-1])
-        fast = timed(eng, inst[0][1], E, args.warmup, args.steps)
+        fast = timed(eng, inst[0][1], E, args.warmup, args.steps, **xo)
         print(f"# fast: {fast['steps']} steps, {fast['ms_per_step']:.2f} ms/step, {fast['value']:.0f} element-steps/s",
               file=sys.stderr, flush=True)
         eng.close()
@@ -173,10 +278,11 @@ def main():
         eng = HipBatchEngine(inst, precision=abi.EB_PRECISION_FP32)
         ws.install(eng)
         E = int(eng.beam_off[-1])
-        lrn = timed(eng, inst[0][1], E, args.warmup, args.steps)
+        lrn = timed(eng, inst[0][1], E, args.warmup, args.steps, **xo)
         ch = L.predictor_stats(eng)
         training["network_start_kept"] = int(ch[0])
         training["fallback_to_reference_predictor"] = int(ch[1])
+        training["network_skipped_by_backoff"] = int(ch[2])
         print(f"# learned: {lrn['steps']} steps, {lrn['ms_per_step']:.2f} ms/step, {lrn['value']:.0f} element-steps/s, "
               f"choices {ch.tolist()}", file=sys.stderr, flush=True)
         eng.close()
@@ -209,6 +315,7 @@ def main():
         "newton_iters_per_step": main_form["newton_iters_per_step"],
         "failed_steps": main_form["failed_steps"],
         "wall_s_per_simulated_s": main_form["wall_s_per_simulated_s"],
+        "exec": main_form["exec"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,
                      "kernel": "beam_kernel<float>" if reduced else "beam_kernel<double>",

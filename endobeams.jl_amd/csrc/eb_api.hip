@@ -8,6 +8,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/endobeams_hip.h"
@@ -50,6 +51,7 @@ struct eb_ctx {
     int32_t *node_inst = nullptr, *beam_inst = nullptr, *blk_inst = nullptr, *chain_inst = nullptr;
     uint8_t* active = nullptr;                        // [S] Newton activity
     int32_t* kiter = nullptr;                         // [S] k at the end of the corrector loop
+    int32_t* kdone = nullptr;                         // [1] Newton iterations finalised since newton_reset
     NewtonAll* nall = nullptr;
     KrylovAll* kall = nullptr;
     int64_t *seg_lo = nullptr, *seg_hi = nullptr;
@@ -74,6 +76,9 @@ struct eb_ctx {
     uint8_t* use_learned = nullptr; // [S]
     int32_t* pred_counts = nullptr; // [3]: starts kept, fallbacks (totals), starts kept by the last choice
     int32_t* h_pred = nullptr;      // pinned copy of pred_counts
+    int32_t learn_max_skip = 16;    // backoff of a network whose start keeps losing
+    int32_t learn_backoff = 0, learn_wait = 0;
+    int64_t learn_skipped = 0;
     // the start of the Newton iteration has been assembled (with its tangent, without energies):
     // the first iteration forms only the residual and the energies
     bool start_assembled = false;
@@ -116,7 +121,6 @@ struct eb_ctx {
     double* partials = nullptr;
     KrylovCtl* kctl = nullptr;
     int* kstatus = nullptr;
-    int solve_id = 0;                 // number of the current linear solve (tags pivot failures)
     double* norms = nullptr;  // device (2)
     // host-pinned mirrors for per-iteration reads
     KrylovCtl* h_kctl = nullptr;
@@ -142,6 +146,22 @@ struct eb_ctx {
     double prof_ms[EB_KERNEL_COUNT] = {};
     int64_t prof_n[EB_KERNEL_COUNT] = {};
     hipEvent_t ev_open[EB_KERNEL_COUNT] = {};
+    // One Newton iteration (assembly, lift, factorisation, one BiCGStab iteration, update and
+    // norms behind the solve's done flag, the two status copies) captured as a HIP graph and
+    // replayed while its parameters stay the same. Its launch timings come from event nodes
+    // recorded at capture (one pair per timed kernel, read after each replay).
+    struct NewtonGraph {
+        hipGraphExec_t exec = nullptr;
+        double key[9] = {};
+        hipEvent_t ev[EB_KERNEL_COUNT][2] = {};
+        bool timed[EB_KERNEL_COUNT] = {};
+    };
+    int exec_mode = EB_EXEC_EAGER;
+    bool capturing = false;
+    NewtonGraph graph;
+    int64_t graph_stats[2] = {0, 0};   // captures, replays
+    // launch shapes (eb_autotune)
+    int32_t tune[EB_TUNE_COUNT] = {};
 };
 
 namespace {
@@ -171,12 +191,26 @@ hipEvent_t pool_event(eb_ctx* ctx) {
     if (hipEventCreate(&e) != hipSuccess) return nullptr;
     return e;
 }
+// while a Newton iteration is captured, the timing events become event nodes of the graph (the
+// events are created before the capture begins)
+bool graph_event(eb_ctx* ctx, int id, int which) {
+    hipEvent_t e = ctx->graph.ev[id][which];
+    return e && hipEventRecordWithFlags(e, ctx->stream, hipEventRecordExternal) == hipSuccess;
+}
 inline void prof_begin(eb_ctx* ctx, int id) {
     if (!(ctx->prof_mask & (1u << id))) return;
+    if (ctx->capturing) {
+        ctx->graph.timed[id] = graph_event(ctx, id, 0);
+        return;
+    }
     hipEvent_t e = pool_event(ctx);
     if (e && hipEventRecord(e, ctx->stream) == hipSuccess) ctx->ev_open[id] = e;
 }
 inline void prof_end(eb_ctx* ctx, int id) {
+    if (ctx->capturing) {
+        if ((ctx->prof_mask & (1u << id)) && ctx->graph.timed[id]) ctx->graph.timed[id] = graph_event(ctx, id, 1);
+        return;
+    }
     if (!(ctx->prof_mask & (1u << id)) || !ctx->ev_open[id]) return;
     hipEvent_t e = pool_event(ctx);
     if (e && hipEventRecord(e, ctx->stream) == hipSuccess) ctx->ev_pending.push_back({id, {ctx->ev_open[id], e}});
@@ -195,8 +229,24 @@ void prof_harvest(eb_ctx* ctx) {
     }
     ctx->ev_pending.clear();
 }
+// folds the timings of the last graph replay into the totals (stream synchronised)
+void prof_harvest_graph(eb_ctx* ctx) {
+    for (int id = 0; id < EB_KERNEL_COUNT; ++id) {
+        if (!ctx->graph.timed[id] || !(ctx->prof_mask & (1u << id))) continue;
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, ctx->graph.ev[id][0], ctx->graph.ev[id][1]) == hipSuccess) {
+            ctx->prof_ms[id] += ms;
+            ctx->prof_n[id] += 1;
+        }
+    }
+}
 
 inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// launch shapes chosen by eb_autotune (0: the built-in default)
+constexpr int32_t TUNE_DEFAULT[EB_TUNE_COUNT] = {48, 1, 256};
+inline int32_t tuned(const eb_ctx* ctx, int id) { return ctx->tune[id] > 0 ? ctx->tune[id] : TUNE_DEFAULT[id]; }
+inline int gather_block(const eb_ctx* ctx) { return tuned(ctx, EB_TUNE_GATHER_BLOCK); }
 inline int red_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(RED_MAX_GRID, (n + RED_BLOCK - 1) / RED_BLOCK)); }
 
 template <typename T> int dalloc(eb_ctx* ctx, T*& out, int64_t count) {
@@ -264,7 +314,8 @@ int check_launch(eb_ctx* ctx, const char* what) {
 // K, C, M at n+1 from the element staging of the last assembly, if that assembly left them pending
 template <typename P> int materialize_kcm_t(eb_ctx* ctx) {
     ctx->kcm_ver = ctx->stage_ver;
-    hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz, ctx->blk_ptr,
+    const int bs = gather_block(ctx);
+    hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz, bs)), dim3(bs), 0, ctx->stream, ctx->nnz, ctx->blk_ptr,
                        ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM, ctx->K, ctx->C, ctx->M,
                        ctx->Ktan, 0.0, 0.0, 0.0, 0, 1, active_dev(ctx));
     return check_launch(ctx, "gather_matrices");
@@ -361,9 +412,12 @@ template <typename P> int launch_gathers(eb_ctx* ctx, bool with_contact, bool ta
     if (matrices) {
         // inside the Newton iteration only Ktan is needed: K, C, M at n+1 are gathered from the
         // element staging when they are first read or when the step is committed (materialize_kcm)
-        hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz,
-                           ctx->blk_ptr, ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM, ctx->K,
-                           ctx->C, ctx->M, ctx->Ktan, c1, c2, c3, tangent ? 1 : 0, tangent ? 0 : 1, active_dev(ctx));
+        // (K, C, M are passed only when written: their buffers are exchanged when a step is committed)
+        const int bs = gather_block(ctx);
+        hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz, bs)), dim3(bs), 0, ctx->stream, ctx->nnz,
+                           ctx->blk_ptr, ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM,
+                           tangent ? nullptr : ctx->K, tangent ? nullptr : ctx->C, tangent ? nullptr : ctx->M, ctx->Ktan,
+                           c1, c2, c3, tangent ? 1 : 0, tangent ? 0 : 1, active_dev(ctx));
         EB_TRY(check_launch(ctx, "gather_matrices"));
         ctx->kcm_pending = tangent;
         if (tangent) ctx->ktan_tag = eb_ctx::TanTag{ctx->stage_ver, c1, c2, c3};
@@ -431,12 +485,12 @@ int bc_lift_launch(eb_ctx* ctx) {
 int precond_factor(eb_ctx* ctx) {
     int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
     prof_begin(ctx, EB_KERNEL_PRECOND);
-    // the reduced chain stays in LDS once it fits in 48 KB (with the kernel's 8 KB of static LDS,
-    // under the 64 KB a launch gets without opting in, and two workgroups per CU)
-    const int L0 = chain_lds_level(ctx->chains.max_len, 48 * 1024);
+    // the reduced chain stays in LDS once it fits in the budget (48 KB by default: with the
+    // kernel's 8 KB of static LDS, under the 64 KB of a workgroup, two workgroups per CU)
+    const int L0 = chain_lds_level(ctx->chains.max_len, 1024 * (int64_t)tuned(ctx, EB_TUNE_CHAIN_LDS_KB));
     const size_t smem = chain_lds_bytes(ctx->chains.max_len, L0);
     hipLaunchKernelGGL(chain_factor, dim3((unsigned)nch), dim3(CHAIN_THREADS), smem, ctx->stream, chain_dev(ctx), ctx->Ktan,
-                       &ctx->kall->pivot_status, L0, ctx->active, ctx->solve_id);
+                       &ctx->kall->pivot_status, L0, ctx->active);
     ctx->factor_tag = ctx->ktan_tag;
     prof_end(ctx, EB_KERNEL_PRECOND);
     return check_launch(ctx, "chain_factor");
@@ -444,7 +498,7 @@ int precond_factor(eb_ctx* ctx) {
 
 int precond_apply(eb_ctx* ctx, const double* b, double* x) {
     int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
-    const size_t smem = chain_solve_lds_bytes(ctx->chains.max_len);
+    const size_t smem = tuned(ctx, EB_TUNE_CHAIN_SOLVE_LDS) == 1 ? chain_solve_lds_bytes(ctx->chains.max_len) : 0;
     if (smem)
         hipLaunchKernelGGL(chain_solve<true>, dim3((unsigned)nch), dim3(SOLVE_THREADS), smem, ctx->stream, chain_dev(ctx), b,
                            x, ctx->kctl);
@@ -483,7 +537,7 @@ int krylov_iteration(eb_ctx* ctx) {
 int solve_start(eb_ctx* ctx, const eb_newton_params* p, bool factor = true) {
     const Segs sg = segs(ctx);
     const double tol = (p && p->krylov_tol > 0) ? p->krylov_tol : 1e-13;
-    ctx->solve_id += 1;
+    hipLaunchKernelGGL(solve_begin, dim3(1), dim3(1), 0, ctx->stream, ctx->kall);
     if (factor) EB_TRY(precond_factor(ctx));
     hipLaunchKernelGGL(krylov_init, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->is_fixed,
                        ctx->res, ctx->kx, ctx->kr, ctx->krhat, ctx->kp, ctx->kv, ctx->partials);
@@ -502,7 +556,7 @@ int solve_poll(eb_ctx* ctx) {
 int solve_finish(eb_ctx* ctx, const eb_newton_params* p, int launched) {
     const int maxit = (p && p->krylov_maxit > 0) ? p->krylov_maxit : 200;
     int batch = std::max(1, launched);
-    while (!ctx->h_kall->all_done && launched < maxit && ctx->h_kall->pivot_status != ctx->solve_id) {
+    while (!ctx->h_kall->all_done && launched < maxit && ctx->h_kall->pivot_status == 0) {
         const int m = std::min(batch, maxit - launched);
         for (int k = 0; k < m; ++k) EB_TRY(krylov_iteration(ctx));
         launched += m;
@@ -517,7 +571,7 @@ int solve_check(eb_ctx* ctx, int* its_out) {
     ctx->counters[1] += ka.max_iters;
     ctx->counters[2] += 1;
     if (its_out) *its_out = ka.max_iters;
-    if (ka.pivot_status == ctx->solve_id) { ctx->err = "singular 6x6 pivot block in the chain preconditioner"; return EB_ERR_SOLVE; }
+    if (ka.pivot_status != 0) { ctx->err = "singular 6x6 pivot block in the chain preconditioner"; return EB_ERR_SOLVE; }
     if (!ka.all_done || ka.fail_inst >= 0) {
         char buf[200];
         if (ka.fail_inst >= 0)
@@ -556,7 +610,7 @@ int norms_kernels(eb_ctx* ctx, const NewtonCtl& nc, const KrylovAll* gate) {
     hipLaunchKernelGGL(norms_partials, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->is_fixed,
                        ctx->dD, ctx->res, ctx->fext, ctx->Tc, ctx->Tk, ctx->partials);
     hipLaunchKernelGGL(norms_finalize, dim3(1), dim3(FIN_BLOCK), 0, ctx->stream, ctx->partials, sg, ctx->norms, nc,
-                       ctx->active, ctx->kiter, ctx->nall, gate);
+                       ctx->active, ctx->kiter, ctx->nall, gate, ctx->kdone);
     prof_end(ctx, EB_KERNEL_UPDATE);
     return check_launch(ctx, "norms");
 }
@@ -596,7 +650,7 @@ int copy_state(eb_ctx* ctx, bool commit) {
 
 int newton_reset_launch(eb_ctx* ctx) {
     hipLaunchKernelGGL(newton_reset, dim3(nblk(ctx->S, 256)), dim3(256), 0, ctx->stream, (int)ctx->S, ctx->active,
-                       ctx->kiter);
+                       ctx->kiter, ctx->kdone);
     return check_launch(ctx, "newton_reset");
 }
 
@@ -623,6 +677,204 @@ int local_predictor_async(eb_ctx* ctx, double dt, double b, double g) {
     hipLaunchKernelGGL(local_predictor, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx),
                        ctx->D, ctx->Dd, ctx->Ddd, hht_consts(dt, b, g));
     return check_launch(ctx, "local_predictor");
+}
+
+// ---- one Newton iteration ------------------------------------------------------------------
+
+// The launches of one corrector iteration (src/corrector.jl:18-51): the assembly with contact and
+// the tangent (or, on the first iteration of a start already assembled, its residual only), the
+// boundary-condition lift, the factorisation and one BiCGStab iteration, then the update and the
+// norms behind the solve's all-done flag, and the copies of the two status records. Nothing here
+// synchronises: the caller waits on the stream.
+int newton_body(eb_ctx* ctx, const eb_newton_params* p, double dt, bool start_done) {
+    const NewtonCtl nc{p->tol_res, p->tol_disp, 0, p->max_iterations, 1};
+    if (start_done) EB_TRY(residual_of_staging(ctx, p->alpha, dt, p->beta, p->gamma));
+    else EB_TRY(assemble_all(ctx, true, p->alpha, true, dt, p->beta, p->gamma));
+    ctx->start_assembled = false;
+    EB_TRY(bc_lift_launch(ctx));
+    EB_TRY(solve_start(ctx, p));
+    EB_TRY(krylov_iteration(ctx));
+    EB_TRY(scatter_launch(ctx, ctx->kall));
+    EB_TRY(global_corrector_async(ctx, dt, p->beta, p->gamma, ctx->kall));
+    EB_TRY(local_corrector_async(ctx, dt, p->beta, p->gamma, ctx->kall));
+    EB_TRY(norms_kernels(ctx, nc, ctx->kall));
+    EB_HIP(hipMemcpyAsync(ctx->h_nall, ctx->nall, sizeof(NewtonAll), hipMemcpyDeviceToHost, ctx->stream));
+    EB_HIP(hipMemcpyAsync(ctx->h_kall, ctx->kall, sizeof(KrylovAll), hipMemcpyDeviceToHost, ctx->stream));
+    return EB_OK;
+}
+
+// What the host side of newton_body(start_done = false) records about the device buffers; a
+// graph replay applies it without running the body.
+struct BodyNotes {
+    bool start_assembled, kcm_pending, np1_alias;
+    uint64_t stage_ver;
+    eb_ctx::TanTag ktan_tag, factor_tag;
+    int64_t beam_launches;
+    bool operator==(const BodyNotes& o) const {
+        return start_assembled == o.start_assembled && kcm_pending == o.kcm_pending && np1_alias == o.np1_alias &&
+               stage_ver == o.stage_ver && ktan_tag == o.ktan_tag && factor_tag == o.factor_tag &&
+               beam_launches == o.beam_launches;
+    }
+};
+BodyNotes body_notes(const eb_ctx* ctx) {
+    return BodyNotes{ctx->start_assembled, ctx->kcm_pending, ctx->np1_alias, ctx->stage_ver,
+                     ctx->ktan_tag,        ctx->factor_tag,  ctx->counters[3]};
+}
+void set_body_notes(eb_ctx* ctx, const BodyNotes& n) {
+    ctx->start_assembled = n.start_assembled;
+    ctx->kcm_pending = n.kcm_pending;
+    ctx->np1_alias = n.np1_alias;
+    ctx->stage_ver = n.stage_ver;
+    ctx->ktan_tag = n.ktan_tag;
+    ctx->factor_tag = n.factor_tag;
+    ctx->counters[3] = n.beam_launches;
+}
+void apply_body_notes(eb_ctx* ctx, const eb_newton_params* p, double dt) {
+    ctx->start_assembled = false;
+    ctx->stage_ver++;
+    ctx->counters[3]++;
+    ctx->kcm_pending = true;
+    ctx->ktan_tag = eb_ctx::TanTag{ctx->stage_ver, 1 + p->alpha, p->gamma / (p->beta * dt), 1 / (p->beta * (dt * dt))};
+    ctx->np1_alias = false;
+    ctx->factor_tag = ctx->ktan_tag;
+}
+
+void graph_release(eb_ctx* ctx) {
+    if (ctx->graph.exec) hipGraphExecDestroy(ctx->graph.exec);
+    ctx->graph.exec = nullptr;
+}
+
+// newton_body(start_done = false) as a replay of the captured graph, captured first when there is
+// none for these parameters. Every buffer the body touches is fixed for the context's lifetime
+// (K, C, M, whose buffers are exchanged at commits, are not passed to it).
+int graph_iteration(eb_ctx* ctx, const eb_newton_params* p, double dt) {
+    eb_ctx::NewtonGraph& g = ctx->graph;
+    const double key[9] = {dt, p->alpha, p->beta, p->gamma, p->tol_res, p->tol_disp, (double)p->max_iterations,
+                           p->krylov_tol, (double)ctx->prof_mask};
+    bool captured = false;
+    BodyNotes after_capture{};
+    const BodyNotes before = body_notes(ctx);
+    if (!g.exec || std::memcmp(key, g.key, sizeof key) != 0) {
+        graph_release(ctx);
+        for (int id = 0; id < EB_KERNEL_COUNT; ++id) {
+            g.timed[id] = false;
+            if (!(ctx->prof_mask & (1u << id))) continue;
+            for (hipEvent_t& e : g.ev[id])
+                if (!e) EB_HIP(hipEventCreate(&e));
+        }
+        EB_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        ctx->capturing = true;
+        const int st = newton_body(ctx, p, dt, false);
+        ctx->capturing = false;
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+        after_capture = body_notes(ctx);
+        set_body_notes(ctx, before);
+        if (st != EB_OK || e != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            if (st == EB_OK) ctx->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
+            return st != EB_OK ? st : EB_ERR_DEVICE;
+        }
+        const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            g.exec = nullptr;
+            ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(ei);
+            return EB_ERR_DEVICE;
+        }
+        std::memcpy(g.key, key, sizeof key);
+        ctx->graph_stats[0]++;
+        captured = true;
+    }
+    EB_HIP(hipGraphLaunch(g.exec, ctx->stream));
+    ctx->graph_stats[1]++;
+    apply_body_notes(ctx, p, dt);
+    if (captured && !(body_notes(ctx) == after_capture)) {
+        ctx->err = "captured Newton iteration: host bookkeeping differs from the replay's";
+        return EB_ERR_STATE;
+    }
+    return EB_OK;
+}
+
+// ---- launch-shape autotuning ----------------------------------------------------------------
+
+// average milliseconds per launch of `launch` over `reps` launches after an untimed one
+template <typename F> int time_launches(eb_ctx* ctx, int reps, F&& launch, double* ms_out) {
+    hipEvent_t a = pool_event(ctx), b = pool_event(ctx);
+    if (!a || !b) { ctx->err = "autotune: hipEventCreate failed"; return EB_ERR_DEVICE; }
+    int st = launch();
+    float ms = 0;
+    if (st == EB_OK && hipEventRecord(a, ctx->stream) != hipSuccess) st = EB_ERR_DEVICE;
+    for (int r = 0; r < reps && st == EB_OK; ++r) st = launch();
+    if (st == EB_OK && (hipEventRecord(b, ctx->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                        hipEventElapsedTime(&ms, a, b) != hipSuccess))
+        st = EB_ERR_DEVICE;
+    ctx->ev_pool.push_back(a);
+    ctx->ev_pool.push_back(b);
+    if (st == EB_ERR_DEVICE && ctx->err.empty()) ctx->err = "autotune: event timing failed";
+    *ms_out = ms / reps;
+    return st;
+}
+
+// Times the candidate launch shapes of the chain factorisation (LDS budget of the reduced
+// chain), the chain solve (work vectors in LDS or in global memory) and the matrix gather
+// (workgroup size) on the context's own sizes and its current tangent, and keeps the fastest.
+// The results of the timed launches are discarded: the gathers write a scratch buffer, the solve
+// a Krylov work vector, and the factors are marked stale.
+int autotune_impl(eb_ctx* ctx, int32_t* choice, double* ms_out) {
+    if (ctx->ktan_tag.ver == 0) { ctx->err = "eb_autotune needs a tangent: call it after a time step"; return EB_ERR_STATE; }
+    const int reps = 5;
+    static const int32_t cand_lds[EB_TUNE_CANDIDATES] = {8, 16, 32, 48};
+    static const int32_t cand_solve[EB_TUNE_CANDIDATES] = {1, 2, 0, 0};
+    static const int32_t cand_gather[EB_TUNE_CANDIDATES] = {64, 128, 256, 0};
+    const int32_t* cands[EB_TUNE_COUNT] = {cand_lds, cand_solve, cand_gather};
+    for (int i = 0; i < EB_TUNE_COUNT * EB_TUNE_CANDIDATES; ++i) ms_out[i] = -1;
+    EB_TRY(materialize_kcm(ctx));   // the gather timing reads the staging; K, C, M stay as they are
+    graph_release(ctx);
+    const int32_t saved[EB_TUNE_COUNT] = {ctx->tune[0], ctx->tune[1], ctx->tune[2]};
+    int st = EB_OK;
+    for (int id = 0; id < EB_TUNE_COUNT && st == EB_OK; ++id) {
+        double best = 1e30;
+        int32_t pick = TUNE_DEFAULT[id];
+        for (int c = 0; c < EB_TUNE_CANDIDATES && st == EB_OK; ++c) {
+            const int32_t v = cands[id][c];
+            if (v == 0) continue;
+            if (id == EB_TUNE_CHAIN_SOLVE_LDS && v == 1 && chain_solve_lds_bytes(ctx->chains.max_len) == 0) continue;
+            ctx->tune[id] = v;
+            double ms = 0;
+            if (id == EB_TUNE_CHAIN_LDS_KB) {
+                st = time_launches(ctx, reps, [&] { return precond_factor(ctx); }, &ms);
+            } else if (id == EB_TUNE_CHAIN_SOLVE_LDS) {
+                st = time_launches(ctx, reps, [&] { return precond_apply(ctx, ctx->res, ctx->ky); }, &ms);
+            } else {
+                const int bs = v;
+                const eb_ctx::TanTag t = ctx->ktan_tag;
+                st = time_launches(ctx, reps, [&] {
+                    if (ctx->precision == EB_PRECISION_FP32)
+                        hipLaunchKernelGGL(gather_matrices<float>, dim3(nblk(ctx->nnz, bs)), dim3(bs), 0, ctx->stream, ctx->nnz,
+                                           ctx->blk_ptr, ctx->blk_src, (const float*)ctx->sK, (const float*)ctx->sC,
+                                           (const float*)ctx->sM, nullptr, nullptr, nullptr, ctx->tmp, t.c1, t.c2, t.c3, 1, 0,
+                                           active_dev(ctx));
+                    else
+                        hipLaunchKernelGGL(gather_matrices<double>, dim3(nblk(ctx->nnz, bs)), dim3(bs), 0, ctx->stream, ctx->nnz,
+                                           ctx->blk_ptr, ctx->blk_src, (const double*)ctx->sK, (const double*)ctx->sC,
+                                           (const double*)ctx->sM, nullptr, nullptr, nullptr, ctx->tmp, t.c1, t.c2, t.c3, 1, 0,
+                                           active_dev(ctx));
+                    return check_launch(ctx, "gather_matrices");
+                }, &ms);
+            }
+            ms_out[id * EB_TUNE_CANDIDATES + c] = ms;
+            if (st == EB_OK && ms < best) { best = ms; pick = v; }
+        }
+        ctx->tune[id] = pick;
+    }
+    ctx->factor_tag = eb_ctx::TanTag{};   // the factors were overwritten by the timed launches
+    if (st != EB_OK) {
+        for (int id = 0; id < EB_TUNE_COUNT; ++id) ctx->tune[id] = saved[id];
+        return st;
+    }
+    for (int id = 0; id < EB_TUNE_COUNT; ++id) choice[id] = ctx->tune[id];
+    return EB_OK;
 }
 
 // ---- learned warm start ---------------------------------------------------------------------
@@ -832,16 +1084,30 @@ template <typename P> int upload_sdfs(eb_ctx* ctx, const eb_model_desc* ds, int6
             f.dom[2 * a] = d->sdf_x0[a];
             f.dom[2 * a + 1] = d->sdf_x0[a] + (d->sdf_n[a] - 1) * d->sdf_d[a];
         }
-        std::vector<double> coef;
-        if (d->sdf_prefiltered) {
-            coef.assign(d->sdf_samples, d->sdf_samples + (d->sdf_n[0] + 2) * (d->sdf_n[1] + 2) * (d->sdf_n[2] + 2));
-        } else {
-            ebh::prefilter_sdf(d->sdf_n, d->sdf_samples, coef);
+    }
+    // prefilter (or copy) the instances' volumes on host threads, a bounded number in flight, and
+    // upload each in the storage type as it is done
+    const int64_t workers = std::max<int64_t>(1, std::min<int64_t>(S, std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()))));
+    for (int64_t s0 = 0; s0 < S; s0 += workers) {
+        const int64_t s1 = std::min(S, s0 + workers);
+        std::vector<std::vector<P>> cps((size_t)(s1 - s0));
+        std::vector<std::thread> th;
+        for (int64_t s = s0; s < s1; ++s)
+            th.emplace_back([&, s] {
+                const eb_model_desc* d = ds + s;
+                std::vector<double> coef;
+                if (d->sdf_prefiltered)
+                    coef.assign(d->sdf_samples, d->sdf_samples + (d->sdf_n[0] + 2) * (d->sdf_n[1] + 2) * (d->sdf_n[2] + 2));
+                else
+                    ebh::prefilter_sdf(d->sdf_n, d->sdf_samples, coef);
+                cps[(size_t)(s - s0)].assign(coef.begin(), coef.end());
+            });
+        for (auto& t : th) t.join();
+        for (int64_t s = s0; s < s1; ++s) {
+            P* dp;
+            EB_TRY(upload_vec(ctx, dp, cps[(size_t)(s - s0)]));
+            h[s].coef = dp;
         }
-        std::vector<P> cp(coef.begin(), coef.end());
-        P* dp;
-        EB_TRY(upload_vec(ctx, dp, cp));
-        f.coef = dp;
     }
     SDFDev<P>* dptr;
     EB_TRY(upload_vec(ctx, dptr, h));
@@ -971,6 +1237,7 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
         std::vector<uint8_t> on((size_t)S, 1);
         CK(upload_vec(ctx, ctx->active, on));
         CK(dalloc(ctx, ctx->kiter, S));
+        CK(dalloc(ctx, ctx->kdone, 1));
         CK(dalloc(ctx, ctx->nall, 1));
         CK(dalloc(ctx, ctx->kall, 1));
     }
@@ -1082,6 +1349,10 @@ int eb_destroy(eb_ctx* ctx) {
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     for (auto& p : ctx->ev_pending) { ctx->ev_pool.push_back(p.second.first); ctx->ev_pool.push_back(p.second.second); }
     for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
+    graph_release(ctx);
+    for (auto& pair : ctx->graph.ev)
+        for (hipEvent_t e : pair)
+            if (e) hipEventDestroy(e);
     for (void* p : ctx->allocs) hipFree(p);
     if (ctx->h_kctl) hipHostFree(ctx->h_kctl);
     if (ctx->h_norms) hipHostFree(ctx->h_norms);
@@ -1382,7 +1653,20 @@ int eb_predictor(eb_ctx* ctx, const eb_newton_params* p, double dt) {
     EB_TRY(local_predictor_async(ctx, dt, p->beta, p->gamma));
     // the reference predictor's increment: an input of the learned start and of its samples
     EB_HIP(hipMemcpyAsync(ctx->pdD, ctx->dD, sizeof(double) * ctx->ndofs, hipMemcpyDeviceToDevice, ctx->stream));
-    if (ctx->mlp && ctx->hist_count >= 3) EB_TRY(learned_predictor(ctx, p, dt));
+    if (ctx->mlp && ctx->hist_count >= 3) {
+        if (ctx->learn_wait > 0) {
+            ctx->learn_wait--;
+            ctx->learn_skipped++;
+        } else {
+            EB_TRY(learned_predictor(ctx, p, dt));
+            if (ctx->h_pred[2] == 0) {
+                ctx->learn_backoff = std::min(std::max(1, 2 * ctx->learn_backoff), ctx->learn_max_skip);
+            } else {
+                ctx->learn_backoff = 0;
+            }
+            ctx->learn_wait = ctx->learn_backoff;
+        }
+    }
     EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
 }
@@ -1397,27 +1681,20 @@ int eb_corrector(eb_ctx* ctx, const eb_newton_params* p, double dt, int* kout) {
     // iterations are used up; the others keep iterating with it frozen
     int k = 1, n_active = (int)ctx->S;
     int st = EB_OK;
+    const NewtonCtl nc{p->tol_res, p->tol_disp, 0, p->max_iterations, 1};
     while (n_active > 0 && k <= p->max_iterations) {
         // assembly, lift, factorisation and one BiCGStab iteration, then the update and norms
         // behind the solve's all-done flag: one synchronisation when one iteration solves (the
         // usual case with the chain preconditioner); otherwise the solve continues and the update
-        // and norms are launched again after it
-        NewtonCtl nc{p->tol_res, p->tol_disp, k, p->max_iterations, 1};
-        if (k == 1 && ctx->start_assembled) st = residual_of_staging(ctx, p->alpha, dt, p->beta, p->gamma);
-        else st = assemble_all(ctx, true, p->alpha, true, dt, p->beta, p->gamma);
-        ctx->start_assembled = false;
-        if (st == EB_OK) st = bc_lift_launch(ctx);
-        if (st == EB_OK) st = solve_start(ctx, p);
-        if (st == EB_OK) st = krylov_iteration(ctx);
-        if (st == EB_OK) st = scatter_launch(ctx, ctx->kall);
-        if (st == EB_OK) st = global_corrector_async(ctx, dt, p->beta, p->gamma, ctx->kall);
-        if (st == EB_OK) st = local_corrector_async(ctx, dt, p->beta, p->gamma, ctx->kall);
-        if (st == EB_OK) st = norms_kernels(ctx, nc, ctx->kall);
-        if (st == EB_OK) {
-            if (hipMemcpyAsync(ctx->h_nall, ctx->nall, sizeof(NewtonAll), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
-                st = EB_ERR_DEVICE;
+        // and norms are launched again after it. In graph mode the body is one graph replay.
+        const bool start_done = k == 1 && ctx->start_assembled;
+        const bool graph = ctx->exec_mode == EB_EXEC_GRAPH && !start_done;
+        st = graph ? graph_iteration(ctx, p, dt) : newton_body(ctx, p, dt, start_done);
+        if (st == EB_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            ctx->err = "Newton iteration: stream synchronisation failed";
+            st = EB_ERR_DEVICE;
         }
-        if (st == EB_OK) st = solve_poll(ctx);
+        if (st == EB_OK && graph) prof_harvest_graph(ctx);
         if (st == EB_OK && !ctx->h_kall->all_done) {
             st = solve_finish(ctx, p, 1);
             if (st == EB_OK) st = solve_check(ctx, nullptr);
@@ -1520,15 +1797,24 @@ int eb_set_learned_predictor(eb_ctx* ctx, eb_mlp* mlp, const float* in_scale, co
     EB_HIP(hipMemcpyAsync(ctx->lout_scale, out_scale, sizeof(float) * LEARN_OUT, hipMemcpyHostToDevice, ctx->stream));
     EB_HIP(hipStreamSynchronize(ctx->stream));
     ctx->mlp = mlp;
+    ctx->learn_backoff = ctx->learn_wait = 0;
     return EB_OK;
 }
 
-int eb_get_predictor_stats(const eb_ctx* ctx, int64_t* out2) {
-    if (!ctx || !out2) return EB_ERR_ARG;
+int eb_set_learned_backoff(eb_ctx* ctx, int32_t max_skip) {
+    if (!ctx || max_skip < 0) return EB_ERR_ARG;
+    ctx->learn_max_skip = max_skip;
+    ctx->learn_backoff = std::min(ctx->learn_backoff, max_skip);
+    ctx->learn_wait = std::min(ctx->learn_wait, max_skip);
+    return EB_OK;
+}
+int eb_get_predictor_stats(const eb_ctx* ctx, int64_t* out3) {
+    if (!ctx || !out3) return EB_ERR_ARG;
     int32_t c[2] = {0, 0};
     if (hipMemcpy(c, ctx->pred_counts, sizeof c, hipMemcpyDeviceToHost) != hipSuccess) return EB_ERR_DEVICE;
-    out2[0] = c[0];
-    out2[1] = c[1];
+    out3[0] = c[0];
+    out3[1] = c[1];
+    out3[2] = ctx->learn_skipped;
     return EB_OK;
 }
 
@@ -1544,6 +1830,47 @@ int eb_profile_enable(eb_ctx* ctx, uint32_t mask) {
     prof_harvest(ctx);
     for (int k = 0; k < EB_KERNEL_COUNT; ++k) { ctx->prof_ms[k] = 0; ctx->prof_n[k] = 0; ctx->ev_open[k] = nullptr; }
     ctx->prof_mask = mask;
+    return EB_OK;
+}
+
+int eb_set_exec_mode(eb_ctx* ctx, int mode) {
+    if (!ctx || (mode != EB_EXEC_EAGER && mode != EB_EXEC_GRAPH)) return EB_ERR_ARG;
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->exec_mode = mode;
+    if (mode == EB_EXEC_EAGER) graph_release(ctx);
+    return EB_OK;
+}
+
+int eb_get_exec_stats(const eb_ctx* ctx, int32_t* mode, int64_t* captures, int64_t* replays) {
+    if (!ctx) return EB_ERR_ARG;
+    if (mode) *mode = ctx->exec_mode;
+    if (captures) *captures = ctx->graph_stats[0];
+    if (replays) *replays = ctx->graph_stats[1];
+    return EB_OK;
+}
+
+int eb_autotune(eb_ctx* ctx, int32_t* choice, double* ms_per_candidate) {
+    if (!ctx || !choice) return EB_ERR_ARG;
+    double scratch[EB_TUNE_COUNT * EB_TUNE_CANDIDATES];
+    return autotune_impl(ctx, choice, ms_per_candidate ? ms_per_candidate : scratch);
+}
+
+int eb_set_tuning(eb_ctx* ctx, const int32_t* choice) {
+    if (!ctx || !choice) return EB_ERR_ARG;
+    if (choice[EB_TUNE_CHAIN_LDS_KB] != 0 && (choice[EB_TUNE_CHAIN_LDS_KB] < 1 || choice[EB_TUNE_CHAIN_LDS_KB] > 48))
+        return EB_ERR_ARG;
+    if (choice[EB_TUNE_CHAIN_SOLVE_LDS] < 0 || choice[EB_TUNE_CHAIN_SOLVE_LDS] > 2) return EB_ERR_ARG;
+    const int32_t g = choice[EB_TUNE_GATHER_BLOCK];
+    if (g != 0 && g != 64 && g != 128 && g != 256) return EB_ERR_ARG;
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    for (int id = 0; id < EB_TUNE_COUNT; ++id) ctx->tune[id] = choice[id];
+    graph_release(ctx);
+    return EB_OK;
+}
+
+int eb_get_tuning(const eb_ctx* ctx, int32_t* choice) {
+    if (!ctx || !choice) return EB_ERR_ARG;
+    for (int id = 0; id < EB_TUNE_COUNT; ++id) choice[id] = tuned(ctx, id);
     return EB_OK;
 }
 

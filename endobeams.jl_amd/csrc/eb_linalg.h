@@ -654,12 +654,11 @@ EBD void chain_level(const ChainDev& cd, const ChainLDS& sm, const double* __res
 
 // Cyclic reduction of each chain's block-tridiagonal system, one workgroup per chain.
 // Dynamic LDS: 3 × 36 doubles per slot, ceil(n / 2^L0) slots (chain_lds_bytes).
-// `status` receives `solve_id` when a pivot block is singular (no reset between solves is needed).
+// `status` is set to 1 when a pivot block is singular (solve_begin clears it before each solve).
 // Two wavefronts per SIMD (at most 256 registers), so that two chains share a CU: a launch has
 // about 1.5 chains per CU.
 __global__ void __launch_bounds__(CHAIN_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
-chain_factor(ChainDev cd, const double* __restrict__ A, int* status, int L0, const uint8_t* __restrict__ active,
-             int solve_id) {
+chain_factor(ChainDev cd, const double* __restrict__ A, int* status, int L0, const uint8_t* __restrict__ active) {
     extern __shared__ __align__(16) double chain_smem[];
     __shared__ __align__(16) double gj_buf[CHAIN_GROUPS * GJ_BUF];
     if (!active[cd.inst[blockIdx.x]]) return;   // instance frozen in the Newton iteration
@@ -696,7 +695,7 @@ chain_factor(ChainDev cd, const double* __restrict__ A, int* status, int L0, con
         const int orow = gj6<6>(w, ln, act, ok, load);
         if (act && ln.r < 6) st6(cd.Dm + 36 * s0 + 6 * orow, w);
     }
-    if (!ok) *status = solve_id;
+    if (!ok) *status = 1;
 }
 
 // smallest L0 whose LDS share for the longest chain stays within `budget` bytes
@@ -820,9 +819,12 @@ __global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const 
 struct KrylovAll {
     int all_done, max_iters, fail_inst, breakdown;
     double fail_rel;
-    int pivot_status;   // solve number of the last singular pivot block (written by chain_factor)
+    int pivot_status;   // 1: a pivot block of the current solve's factorisation is singular
     int pad;
 };
+
+// start of a linear solve: clears the pivot flag the factorisation may set
+__global__ void solve_begin(KrylovAll* all) { all->pivot_status = 0; }
 
 // true when the kernel may run: ungated, or the polled solve is done
 EBD bool gate_open(const KrylovAll* gate) { return !gate || gate->all_done; }
@@ -1090,7 +1092,7 @@ __global__ void __launch_bounds__(RED_BLOCK) norms_partials(Segs sg, const uint8
 
 struct NewtonCtl {            // corrector loop control (src/corrector.jl:17-36)
     double tol_res, tol_disp;
-    int k, max_iterations;    // k: the iteration whose norms are being finalised (1-based)
+    int k, max_iterations;    // k: set on the device from the count of finalised iterations
     int newton;               // 0: norms only (no convergence bookkeeping)
 };
 struct NewtonAll { int n_active, pad[3]; };
@@ -1098,12 +1100,15 @@ struct NewtonAll { int n_active, pad[3]; };
 // per-instance norms (relative residual, ‖ΔD‖) into out[2s], out[2s+1]; in a Newton iteration,
 // an active instance whose norms pass (or that has used its iterations) is frozen and its k
 // recorded as the reference's loop leaves it (iterations done + 1)
+// In a Newton iteration the iteration number is kept on the device: `kdone` counts the
+// iterations finalised since newton_reset, so that one launch sequence serves every iteration.
 __global__ void __launch_bounds__(FIN_BLOCK) norms_finalize(const double* partials, Segs sg, double* out, NewtonCtl nc,
                                                             uint8_t* active, int32_t* kiter, NewtonAll* all,
-                                                            const KrylovAll* gate) {
+                                                            const KrylovAll* gate, int32_t* kdone) {
     if (!gate_open(gate)) return;
     __shared__ int cnt;
     if (threadIdx.x == 0) cnt = 0;
+    if (nc.newton) nc.k = *kdone + 1;
     __syncthreads();
     int mine = 0;
     for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
@@ -1126,10 +1131,14 @@ __global__ void __launch_bounds__(FIN_BLOCK) norms_finalize(const double* partia
     }
     if (mine) atomicAdd(&cnt, mine);
     __syncthreads();
-    if (threadIdx.x == 0) all->n_active = cnt;
+    if (threadIdx.x == 0) {
+        all->n_active = cnt;
+        if (nc.newton) *kdone = nc.k;
+    }
 }
 
-__global__ void newton_reset(int ninst, uint8_t* active, int32_t* kiter) {
+__global__ void newton_reset(int ninst, uint8_t* active, int32_t* kiter, int32_t* kdone) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *kdone = 0;
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ninst; s += gridDim.x * blockDim.x) {
         active[s] = 1;
         kiter[s] = 0;

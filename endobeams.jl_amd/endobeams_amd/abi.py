@@ -106,7 +106,17 @@ PROTOTYPES = {
     "get_instances": (_i, [_vp, _lp, _lp, _lp]),
     "get_instance_iterations": (_i, [_vp, C.POINTER(C.c_int32)]),
     "get_instance_norms": (_i, [_vp, _dp]),
+    "set_exec_mode": (_i, [_vp, _i]),
+    "get_exec_stats": (_i, [_vp, C.POINTER(C.c_int32), _lp, _lp]),
+    "autotune": (_i, [_vp, C.POINTER(C.c_int32), _dp]),
+    "set_tuning": (_i, [_vp, C.POINTER(C.c_int32)]),
+    "get_tuning": (_i, [_vp, C.POINTER(C.c_int32)]),
 }
+
+EB_EXEC_EAGER, EB_EXEC_GRAPH = 0, 1
+TUNE_NAMES = ("chain_lds_kb", "chain_solve_lds", "gather_block")
+EB_TUNE_COUNT, EB_TUNE_CANDIDATES = 3, 4
+TUNE_CANDIDATES = ((8, 16, 32, 48), (1, 2), (64, 128, 256))
 
 KERNELS = {"beam": 0, "contact": 1, "gather": 2, "precond": 3, "krylov": 4, "update": 5}
 
@@ -125,6 +135,7 @@ LEARNED_PROTOTYPES = {
     "mlp_train": (_i, [_vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _fp]),
     "learned_sample": (_i, [_vp, _fp, _fp]),
     "set_learned_predictor": (_i, [_vp, _vp, _fp, _fp]),
+    "set_learned_backoff": (_i, [_vp, C.c_int32]),
     "get_predictor_stats": (_i, [_vp, _lp]),
 }
 LEARNED_INPUTS, LEARNED_OUTPUTS = 27, 6

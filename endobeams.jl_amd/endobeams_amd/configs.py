@@ -260,9 +260,30 @@ def braided_stent(n_wires=24, radius=4e-3, pitch=8e-3, length=40e-3, el_per_wire
     return conf, params, inter
 
 
+def stent_sweep_instance(i, sdf_n=128, length=40e-3):
+    """Config 5, instance i: a braided stent drawn with numpy's PCG64 seeded 1000 + i: wire count
+    W ~ U{12, 16, ..., 48}, radius R ~ U[3, 5] mm, pitch ~ U[6, 10] mm, beams per wire
+    ~ U{100, ..., 300} (1 200 to 14 400 beams), in its own cylindrical vessel of radius 0.7 R
+    (FP32 SDF on sdf_n³). The stent is crimped to 0.375 R (config 4's 1.5 mm of 4 mm) and released;
+    length 40 mm and every material, contact and time-integration value as config 4."""
+    rng = np.random.default_rng(1000 + i)
+    W = int(rng.choice(np.arange(12, 49, 4)))
+    R = float(rng.uniform(3e-3, 5e-3))
+    pitch = float(rng.uniform(6e-3, 10e-3))
+    m = int(rng.integers(100, 301))
+    return braided_stent(n_wires=W, radius=R, pitch=pitch, length=length, el_per_wire=m, crimp_radius=0.375 * R,
+                         vessel_radius=0.7 * R, sdf_n=sdf_n)
+
+
+def stent_sweep(n_instances=512, sdf_n=128, first=0):
+    """Config 5: instances first .. first + n_instances - 1 of the design sweep."""
+    return [stent_sweep_instance(first + i, sdf_n=sdf_n) for i in range(n_instances)]
+
+
 WORKLOADS = {
     "l_cantilever": l_cantilever,
     "ring_drop": ring_drop,
     "net_on_sphere": net_on_sphere,
     "braided_stent": braided_stent,
+    "stent_sweep": stent_sweep,
 }

@@ -119,6 +119,42 @@ class Engine:
         self._check(self.fn["profile_read"](self.h, abi.KERNELS[kernel], C.byref(ms), C.byref(n)), "profile_read")
         return ms.value, n.value
 
+    # -- execution mode and launch shapes (no reference counterpart) --------------------------------
+    def set_exec_mode(self, graph: bool):
+        """Newton iterations as replays of a captured HIP graph (True) or launched one by one."""
+        mode = abi.EB_EXEC_GRAPH if graph else abi.EB_EXEC_EAGER
+        self._check(self.fn["set_exec_mode"](self.h, mode), "set_exec_mode")
+
+    def exec_stats(self):
+        """(mode, graph captures, graph replays)"""
+        m, c, r = C.c_int32(0), C.c_int64(0), C.c_int64(0)
+        self._check(self.fn["get_exec_stats"](self.h, C.byref(m), C.byref(c), C.byref(r)), "get_exec_stats")
+        return m.value, c.value, r.value
+
+    def autotune(self):
+        """Times the candidate launch shapes on this context and keeps the fastest; returns
+        ({name: choice}, {name: {candidate: ms}})."""
+        ch = np.zeros(abi.EB_TUNE_COUNT, np.int32)
+        ms = np.zeros(abi.EB_TUNE_COUNT * abi.EB_TUNE_CANDIDATES, np.float64)
+        self._check(self.fn["autotune"](self.h, ch.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        ms.ctypes.data_as(C.POINTER(C.c_double))), "autotune")
+        times = {}
+        for i, name in enumerate(abi.TUNE_NAMES):
+            times[name] = {c: float(ms[i * abi.EB_TUNE_CANDIDATES + j])
+                           for j, c in enumerate(abi.TUNE_CANDIDATES[i]) if ms[i * abi.EB_TUNE_CANDIDATES + j] >= 0}
+        return dict(zip(abi.TUNE_NAMES, (int(v) for v in ch))), times
+
+    @property
+    def tuning(self):
+        ch = np.zeros(abi.EB_TUNE_COUNT, np.int32)
+        self._check(self.fn["get_tuning"](self.h, ch.ctypes.data_as(C.POINTER(C.c_int32))), "get_tuning")
+        return dict(zip(abi.TUNE_NAMES, (int(v) for v in ch)))
+
+    @tuning.setter
+    def tuning(self, choice):
+        ch = np.array([int(choice.get(n, 0)) for n in abi.TUNE_NAMES], np.int32)
+        self._check(self.fn["set_tuning"](self.h, ch.ctypes.data_as(C.POINTER(C.c_int32))), "set_tuning")
+
     # -- reference functions ----------------------------------------------------------------------
     def assemble(self):
         self._check(self.fn["assemble"](self.h), "assemble")

@@ -168,8 +168,15 @@ def uninstall(engine):
     engine._check(engine.fn["set_learned_predictor"](engine.h, None, None, None), "set_learned_predictor")
 
 
+def set_backoff(engine, max_skip: int):
+    """Longest run of predictions that skip a network whose start keeps losing (0: evaluate it
+    at every prediction)."""
+    engine._check(engine.fn["set_learned_backoff"](engine.h, int(max_skip)), "set_learned_backoff")
+
+
 def predictor_stats(engine) -> np.ndarray:
-    out = np.zeros(2, np.int64)
+    """[network start kept, reference start kept after evaluating both, network skipped]"""
+    out = np.zeros(3, np.int64)
     engine._check(engine.fn["get_predictor_stats"](engine.h, out.ctypes.data_as(C.POINTER(C.c_int64))),
                   "get_predictor_stats")
     return out

@@ -62,6 +62,27 @@ def get_Re0(X1, X2):
     return (np.eye(3) + Sv) + ((1 - c) / s**2 * Sv) @ Sv
 
 
+def get_Re0_many(X1, X2):
+    """get_Re0 for rows of chord endpoints at once (the same formula per row)."""
+    X1 = np.asarray(X1, dtype=np.float64).reshape(-1, 3)
+    X2 = np.asarray(X2, dtype=np.float64).reshape(-1, 3)
+    d = X2 - X1
+    l0 = np.sqrt(np.einsum("ij,ij->i", d, d))
+    E10 = d / l0[:, None]
+    # v = e₁ × E10, c = e₁ · E10
+    v = np.column_stack([np.zeros_like(l0), -E10[:, 2], E10[:, 1]])
+    s2 = v[:, 1] ** 2 + v[:, 2] ** 2
+    c = E10[:, 0]
+    Sv = np.zeros((len(l0), 3, 3))
+    Sv[:, 0, 1], Sv[:, 0, 2] = -v[:, 2], v[:, 1]
+    Sv[:, 1, 0], Sv[:, 1, 2] = v[:, 2], -v[:, 0]
+    Sv[:, 2, 0], Sv[:, 2, 1] = -v[:, 1], v[:, 0]
+    out = np.eye(3) + Sv + ((1 - c) / np.where(s2 > 0, s2, 1.0))[:, None, None] * (Sv @ Sv)
+    out[c < -(1 - 2 * EPS)] = np.diag([-1.0, 1.0, -1.0])
+    out[c > (1 - 2 * EPS)] = np.eye(3)
+    return out
+
+
 def skew(v):
     return np.array([[0.0, -v[2], v[1]], [v[2], 0.0, -v[0]], [-v[1], v[0], 0.0]])
 
@@ -81,6 +102,18 @@ def beam_material_properties(l0, E, nu, rho, radius, damping):
     return np.array([radius, E, Ku, *KT, *KTT, *J, rho * A, damping], dtype=np.float64)
 
 
+def beam_material_properties_many(l0, E, nu, rho, radius, damping):
+    """beam_material_properties for arrays of beams (rows of the same 14-double record)."""
+    G = E / (2 * (1 + nu))
+    A = math.pi * radius**2
+    I22 = math.pi * radius**4 / 4
+    I33 = I22
+    Io = I22 + I33
+    return np.column_stack([radius, E, A * E / l0, G * Io / l0, 4 * E * I33 / l0, 4 * E * I22 / l0,
+                            -G * Io / l0, 2 * E * I33 / l0, 2 * E * I22 / l0, rho * Io, rho * I22, rho * I33,
+                            rho * A, damping]).astype(np.float64)
+
+
 class Beams:
     """Beams(nodes, connectivity, E, ν, ρ, radius, damping, Rₑ⁰=nothing)
     (src/beams/def_beam.jl:32-72). Scalars or per-beam arrays; connectivity is (E, 2), 1-based."""
@@ -92,14 +125,11 @@ class Beams:
         E, nu, rho, radius, damping = map(per, (E, nu, rho, radius, damping))
         X = nodes.X0
         self.conn = conn
-        self.l0 = np.empty(ne)
-        self.Re0 = np.empty((ne, 3, 3))
-        self.props = np.empty((ne, abi.EB_NPROPS))
-        for e in range(ne):
-            X1, X2 = X[conn[e, 0] - 1], X[conn[e, 1] - 1]
-            self.l0[e] = np.linalg.norm(X1 - X2)
-            self.Re0[e] = get_Re0(X1, X2) if Re0 is None else np.asarray(Re0[e], dtype=np.float64)
-            self.props[e] = beam_material_properties(self.l0[e], E[e], nu[e], rho[e], radius[e], damping[e])
+        X1, X2 = X[conn[:, 0] - 1], X[conn[:, 1] - 1]
+        d = X1 - X2
+        self.l0 = np.sqrt(np.einsum("ij,ij->i", d, d))
+        self.Re0 = get_Re0_many(X1, X2) if Re0 is None else np.asarray(Re0, dtype=np.float64).reshape(ne, 3, 3).copy()
+        self.props = beam_material_properties_many(self.l0, E, nu, rho, radius, damping)
 
     def __len__(self):
         return self.conn.shape[0]

@@ -249,6 +249,36 @@ int eb_profile_enable(eb_ctx* ctx, uint32_t mask);
 /* total milliseconds and number of timed launches of kernel `id` since enabling */
 int eb_profile_read(eb_ctx* ctx, int id, double* total_ms, int64_t* launches);
 
+/* --- execution mode and launch shapes (no counterpart in the reference) --------------------- */
+
+/* How eb_corrector issues a Newton iteration. EAGER: the host launches every kernel.
+ * GRAPH: the iteration (assembly, lift, factorisation, one BiCGStab iteration, the update and
+ * norms gated on the solve's done flag, the two status copies) is captured once as a HIP graph
+ * per set of step parameters (Δt, α, β, γ, tolerances) and replayed; results are identical. */
+#define EB_EXEC_EAGER 0
+#define EB_EXEC_GRAPH 1
+int eb_set_exec_mode(eb_ctx* ctx, int mode);
+/* mode, graph captures and graph replays since eb_create (any pointer may be NULL) */
+int eb_get_exec_stats(const eb_ctx* ctx, int32_t* mode, int64_t* captures, int64_t* replays);
+
+/* Launch shapes chosen by eb_autotune (0 in eb_set_tuning: the built-in default):
+ *   EB_TUNE_CHAIN_LDS_KB    LDS budget (KB) of the reduced chain in the chain factorisation
+ *                           (candidates 8, 16, 32, 48; default 48)
+ *   EB_TUNE_CHAIN_SOLVE_LDS chain solve work vectors in LDS (1, default) or global memory (2)
+ *   EB_TUNE_GATHER_BLOCK    workgroup size of the matrix gather (64, 128, 256; default 256) */
+#define EB_TUNE_CHAIN_LDS_KB 0
+#define EB_TUNE_CHAIN_SOLVE_LDS 1
+#define EB_TUNE_GATHER_BLOCK 2
+#define EB_TUNE_COUNT 3
+#define EB_TUNE_CANDIDATES 4
+/* Times every candidate on this context's sizes and current tangent (call after a time step)
+ * and keeps the fastest of each; choice[EB_TUNE_COUNT] receives them and, when not NULL,
+ * ms_per_candidate[EB_TUNE_COUNT * EB_TUNE_CANDIDATES] the average launch time of each
+ * candidate (-1: not a candidate). Results of later calls are unchanged by the choice. */
+int eb_autotune(eb_ctx* ctx, int32_t* choice, double* ms_per_candidate);
+int eb_set_tuning(eb_ctx* ctx, const int32_t* choice);
+int eb_get_tuning(const eb_ctx* ctx, int32_t* choice);
+
 #ifdef __cplusplus
 }
 #endif

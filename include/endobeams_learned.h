@@ -54,8 +54,15 @@ int eb_mlp_train(eb_mlp* mlp, int32_t epochs, int32_t batch, float lr, uint64_t 
 int eb_learned_sample(eb_ctx* ctx, float* X, float* T);
 /* installs the network (NULL: reference predictor only); the context keeps the pointer */
 int eb_set_learned_predictor(eb_ctx* ctx, eb_mlp* mlp, const float* in_scale, const float* out_scale);
-/* instance-predictions that kept the network's start [0] and that fell back to the reference's [1] */
-int eb_get_predictor_stats(const eb_ctx* ctx, int64_t* out2);
+/* A network whose start keeps losing is not evaluated at every step: after an evaluation in
+ * which no instance kept the network's start, the next 1, 2, 4, ... (at most max_skip)
+ * predictions use the reference predictor alone; an evaluation in which some instance keeps it
+ * resets the wait. max_skip = 0 evaluates the network at every prediction; the default is 16. */
+int eb_set_learned_backoff(eb_ctx* ctx, int32_t max_skip);
+/* instance-predictions that kept the network's start [0], that fell back to the reference's after
+ * evaluating both [1], and predictions that used the reference's without evaluating the network
+ * under the backoff [2] */
+int eb_get_predictor_stats(const eb_ctx* ctx, int64_t* out3);
 
 #ifdef __cplusplus
 }

@@ -722,6 +722,27 @@ int ebo_get_counters(void* h, int64_t* out4) {
 int ebo_profile_enable(void*, uint32_t) { return EB_OK; }
 int ebo_profile_read(void*, int, double* ms, int64_t* n) { *ms = 0; *n = 0; return EB_OK; }
 
+// execution mode and launch shapes have no CPU counterpart: the oracle accepts them and computes
+// as always
+int ebo_set_exec_mode(void*, int mode) { return (mode == EB_EXEC_EAGER || mode == EB_EXEC_GRAPH) ? EB_OK : EB_ERR_ARG; }
+int ebo_get_exec_stats(const void*, int32_t* mode, int64_t* captures, int64_t* replays) {
+    if (mode) *mode = EB_EXEC_EAGER;
+    if (captures) *captures = 0;
+    if (replays) *replays = 0;
+    return EB_OK;
+}
+int ebo_autotune(void*, int32_t* choice, double* ms) {
+    for (int i = 0; i < EB_TUNE_COUNT; ++i) choice[i] = 0;
+    if (ms)
+        for (int i = 0; i < EB_TUNE_COUNT * EB_TUNE_CANDIDATES; ++i) ms[i] = -1;
+    return EB_OK;
+}
+int ebo_set_tuning(void*, const int32_t*) { return EB_OK; }
+int ebo_get_tuning(const void*, int32_t* choice) {
+    for (int i = 0; i < EB_TUNE_COUNT; ++i) choice[i] = 0;
+    return EB_OK;
+}
+
 // ---- single-element probes for the oracle's own invariant tests ---------------------------
 
 // in: X1,X2,u1,u2,ud1,ud2,udd1,udd2,wd1,wd2,wdd1,wdd2 (12×3), R1,R2,dR1,dR2 (4×9), l0, Re0 (9)

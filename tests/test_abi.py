@@ -48,4 +48,4 @@ def test_python_bindings_cover_the_abi():
     bound = {"eb_" + k for k in abi.PROTOTYPES}
     assert bound == set(declared())
     assert {"eb_" + k for k in abi.LEARNED_PROTOTYPES} == set(declared(LEARNED))
-    assert len(declared(LEARNED)) == 13
+    assert len(declared(LEARNED)) == 14
