@@ -192,6 +192,67 @@ def sweep(args):
     print(json.dumps(out))
 
 
+def small_config(args):
+    """One of the smaller structures: plain FP64 and the precision form, one structure per
+    context, each timed over `--steps` converged steps; the precision form's error against the
+    FP64 CPU restatement after `--compare-steps` steps (nodal positions, rotations and contact
+    forces); the restatement's own throughput on those steps."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    from endobeams_amd import HipEngine, Stepper, abi, configs
+    build = {"cantilever": configs.l_cantilever, "ring": configs.ring_drop, "net": configs.net_on_sphere}[args.workload]
+    forms = {}
+    for name, prec in (("fp64", abi.EB_PRECISION_FP64), ("fp32", abi.EB_PRECISION_FP32)):
+        conf, params, inter = build()
+        eng = HipEngine(conf, params, inter, precision=prec)
+        forms[name] = timed(eng, params, len(conf.beams), args.warmup, args.steps, graph=args.exec == "graph",
+                            autotune=not args.no_autotune)
+        eng.close()
+        print(f"# {args.workload} {name}: {forms[name]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    conf, params, inter = build()
+    ref = O.OracleEngine(conf, params, inter)
+    st = Stepper(ref, params)
+    st.initialize()
+    t0 = time.perf_counter()
+    run_steps(st, args.compare_steps)
+    cpu_wall = time.perf_counter() - t0
+    g = HipEngine(conf, params, inter, precision=abi.EB_PRECISION_FP32)
+    sg = Stepper(g, params)
+    sg.initialize()
+    run_steps(sg, args.compare_steps)
+    err = {}
+    for f in ("U_N", "R_N", "TC_N"):
+        a, b = g.get(f), ref.get(f)
+        nb = float(np.linalg.norm(b))
+        err[f] = float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+    E = len(conf.beams)
+    f32, f64 = forms["fp32"], forms["fp64"]
+    avg_s = f32["kernel_ms"] * 1e-3 / max(f32["kernel_launches"], 1)
+    achieved = FLOPS_PER_BEAM * E / avg_s / 1e12
+    out = {
+        "metric": METRIC, "value": f32["value"], "unit": "element-steps/s", "n_gpus": 1, "steps": f32["steps"],
+        "warmup": args.warmup, "ms_per_step": f32["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": {"cantilever": "L-shaped cantilever, 20 beams",
+                                "ring": "ring of 60 beams dropped on a plane (64x64x32 FP32 SDF)",
+                                "net": "40x40 net of 3120 beams dropped on a sphere (128^3 FP32 SDF)"}[args.workload],
+                   "form": "FP32 element matrices + FP64 forces/solver, one structure", "exec": args.exec,
+                   "dt": params.initial_timestep, "tolerance": params.tolerance_residual, "beams": E},
+        "newton_iters_per_step": f32["newton_iters_per_step"], "failed_steps": f32["failed_steps"],
+        "wall_s_per_simulated_s": f32["wall_s_per_simulated_s"],
+        "plain_fp64": {"value": f64["value"], "ms_per_step": f64["ms_per_step"],
+                       "newton_iters_per_step": f64["newton_iters_per_step"],
+                       "wall_s_per_simulated_s": f64["wall_s_per_simulated_s"]},
+        "fp32_error_vs_fp64_oracle": {"steps": args.compare_steps, **err},
+        "cpu_baseline": {"value": E * args.compare_steps / cpu_wall, "unit": "element-steps/s", "cores": 1,
+                         "kind": "port", "sample": f"first {args.compare_steps} steps, C++ FP64 restatement, 1 thread"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "beam_kernel<float>",
+                     "avg_launch_us": avg_s * 1e6, "launches": f32["kernel_launches"], "flops_per_launch": FLOPS_PER_BEAM * E},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,8 +268,11 @@ def main():
     ap.add_argument("--exec", choices=["graph", "eager"], default="eager",
                     help="Newton iterations launched one by one (default) or as HIP graph replays")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch shapes")
-    ap.add_argument("--workload", choices=["stent", "sweep"], default="stent",
-                    help="stent: the braided stent (default); sweep: the stent design sweep")
+    ap.add_argument("--workload", choices=["stent", "sweep", "cantilever", "ring", "net"], default="stent",
+                    help="stent: the braided stent (default); sweep: the stent design sweep; "
+                         "cantilever, ring, net: the smaller structures, plain FP64 and FP32")
+    ap.add_argument("--compare-steps", type=int, default=10,
+                    help="steps of the FP32-vs-FP64-restatement comparison of the smaller structures")
     ap.add_argument("--instances", type=int, default=512, help="stents in the design sweep")
     ap.add_argument("--sweep-sdf-n", type=int, default=128)
     ap.add_argument("--sub-batch", type=int, default=0, help="instances per context in the sweep (0: autotuned)")
@@ -232,6 +296,9 @@ def main():
 
     if args.workload == "sweep":
         sweep(args)
+        return
+    if args.workload in ("cantilever", "ring", "net"):
+        small_config(args)
         return
 
     from endobeams_amd import HipBatchEngine, HipEngine, abi

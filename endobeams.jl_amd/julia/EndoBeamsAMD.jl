@@ -199,6 +199,26 @@ end
 update_converged!(dev::Device) = check(ccall((:eb_update_converged, LIB), Cint, (Ptr{Cvoid},), dev.ctx), dev.ctx, "eb_update_converged")
 update_not_converged!(dev::Device) = check(ccall((:eb_update_not_converged, LIB), Cint, (Ptr{Cvoid},), dev.ctx), dev.ctx, "eb_update_not_converged")
 
+# execution mode and launch shapes (no counterpart in the reference)
+const EB_EXEC_EAGER = 0
+const EB_EXEC_GRAPH = 1
+"""Newton iterations as replays of a captured HIP graph (`graph = true`) or launched one by one."""
+set_exec_mode!(dev, graph::Bool) =
+    check(ccall((:eb_set_exec_mode, LIB), Cint, (Ptr{Cvoid}, Cint), dev.ctx, graph ? EB_EXEC_GRAPH : EB_EXEC_EAGER),
+          dev.ctx, "eb_set_exec_mode")
+"""Times the candidate launch shapes on the live context (after a time step) and keeps the
+fastest; returns (chain LDS budget in KB, chain-solve LDS mode, gather workgroup size)."""
+function autotune!(dev)
+    choice = zeros(Int32, 3)
+    check(ccall((:eb_autotune, LIB), Cint, (Ptr{Cvoid}, Ptr{Int32}, Ptr{Float64}), dev.ctx, choice, C_NULL),
+          dev.ctx, "eb_autotune")
+    return Tuple(choice)
+end
+"""Longest run of predictions that skip a learned start that keeps losing (0: evaluate always)."""
+set_learned_backoff!(dev, max_skip::Integer) =
+    check(ccall((:eb_set_learned_backoff, LIB), Cint, (Ptr{Cvoid}, Int32), dev.ctx, max_skip), dev.ctx,
+          "eb_set_learned_backoff")
+
 """run_simulation!(conf, params, inter) with the hot path on the GPU (src/run_simulation.jl:1-119)."""
 function run_simulation_gpu!(conf, params, inter=nothing)
     dev = Device(conf, params, inter)
