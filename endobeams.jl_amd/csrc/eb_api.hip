@@ -18,6 +18,7 @@
 #include "eb_learned.h"
 #include "eb_linalg.h"
 #include "eb_mlp.h"
+#include "eb_sdf_prefilter.h"
 #include "eb_setup.h"
 
 using namespace ebd;
@@ -1070,8 +1071,12 @@ void concat_descs(const eb_model_desc* ds, int64_t S, Concat& c, std::vector<int
     c.d.contact_beams = c.cbeams.data();
 }
 
+// The instances' volumes, prefiltered on the device (or taken as given when the descriptor
+// holds prefiltered coefficients), stored in the storage type P. One pair of FP64 scratch
+// buffers, sized for the largest volume, serves every instance.
 template <typename P> int upload_sdfs(eb_ctx* ctx, const eb_model_desc* ds, int64_t S) {
     std::vector<SDFDev<P>> h((size_t)S);
+    int64_t max_samples = 0, max_coef = 0;
     for (int64_t s = 0; s < S; ++s) {
         const eb_model_desc* d = ds + s;
         SDFDev<P>& f = h[s];
@@ -1084,30 +1089,76 @@ template <typename P> int upload_sdfs(eb_ctx* ctx, const eb_model_desc* ds, int6
             f.dom[2 * a] = d->sdf_x0[a];
             f.dom[2 * a + 1] = d->sdf_x0[a] + (d->sdf_n[a] - 1) * d->sdf_d[a];
         }
+        max_samples = std::max(max_samples, d->sdf_n[0] * d->sdf_n[1] * d->sdf_n[2]);
+        max_coef = std::max(max_coef, (d->sdf_n[0] + 2) * (d->sdf_n[1] + 2) * (d->sdf_n[2] + 2));
     }
-    // prefilter (or copy) the instances' volumes on host threads, a bounded number in flight, and
-    // upload each in the storage type as it is done
-    const int64_t workers = std::max<int64_t>(1, std::min<int64_t>(S, std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()))));
-    for (int64_t s0 = 0; s0 < S; s0 += workers) {
-        const int64_t s1 = std::min(S, s0 + workers);
-        std::vector<std::vector<P>> cps((size_t)(s1 - s0));
-        std::vector<std::thread> th;
-        for (int64_t s = s0; s < s1; ++s)
-            th.emplace_back([&, s] {
-                const eb_model_desc* d = ds + s;
-                std::vector<double> coef;
-                if (d->sdf_prefiltered)
-                    coef.assign(d->sdf_samples, d->sdf_samples + (d->sdf_n[0] + 2) * (d->sdf_n[1] + 2) * (d->sdf_n[2] + 2));
-                else
-                    ebh::prefilter_sdf(d->sdf_n, d->sdf_samples, coef);
-                cps[(size_t)(s - s0)].assign(coef.begin(), coef.end());
-            });
-        for (auto& t : th) t.join();
-        for (int64_t s = s0; s < s1; ++s) {
-            P* dp;
-            EB_TRY(upload_vec(ctx, dp, cps[(size_t)(s - s0)]));
+    double *d_samples = nullptr, *d_coef = nullptr, *d_tri = nullptr;
+    auto release = [&] {
+        for (double* q : {d_samples, d_coef, d_tri})
+            if (q) (void)hipFree(q);
+    };
+    // LU factors of every axis length in use, concatenated: [cp | inv] per length
+    std::vector<int64_t> lens;
+    for (int64_t s = 0; s < S; ++s)
+        for (int a = 0; a < 3; ++a) lens.push_back(ds[s].sdf_n[a]);
+    std::sort(lens.begin(), lens.end());
+    lens.erase(std::unique(lens.begin(), lens.end()), lens.end());
+    std::vector<double> tri;
+    std::vector<int64_t> tri_off;
+    for (int64_t n : lens) {
+        std::vector<double> cp, inv;
+        ebh::tri_factors(n, cp, inv);
+        tri_off.push_back((int64_t)tri.size());
+        tri.insert(tri.end(), cp.begin(), cp.end());
+        tri.insert(tri.end(), inv.begin(), inv.end());
+    }
+    if (hipMalloc(&d_samples, sizeof(double) * max_samples) != hipSuccess ||
+        hipMalloc(&d_coef, sizeof(double) * max_coef) != hipSuccess ||
+        hipMalloc(&d_tri, sizeof(double) * tri.size()) != hipSuccess ||
+        hipMemcpy(d_tri, tri.data(), sizeof(double) * tri.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        release();
+        ctx->err = "SDF prefilter: device scratch allocation failed";
+        return EB_ERR_DEVICE;
+    }
+    auto lu = [&](int64_t n) {
+        const size_t k = (size_t)(std::lower_bound(lens.begin(), lens.end(), n) - lens.begin());
+        return TriLU{d_tri + tri_off[k], d_tri + tri_off[k] + n, n};
+    };
+    int st = EB_OK;
+    for (int64_t s = 0; s < S && st == EB_OK; ++s) {
+        const eb_model_desc* d = ds + s;
+        const SdfGrid g{d->sdf_n[0], d->sdf_n[1], d->sdf_n[2]};
+        const int64_t P0 = g.n0 + 2, P1 = g.n1 + 2, ncoef = P0 * P1 * (g.n2 + 2);
+        if (d->sdf_prefiltered) {
+            if (hipMemcpyAsync(d_coef, d->sdf_samples, sizeof(double) * ncoef, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+                st = EB_ERR_DEVICE;
+        } else {
+            if (hipMemcpyAsync(d_samples, d->sdf_samples, sizeof(double) * g.n0 * g.n1 * g.n2, hipMemcpyHostToDevice,
+                               ctx->stream) != hipSuccess)
+                st = EB_ERR_DEVICE;
+            if (st == EB_OK) {
+                hipLaunchKernelGGL(sdf_prefilter_x, dim3(nblk(g.n1 * g.n2, 256)), dim3(256), 0, ctx->stream, g, lu(g.n0),
+                                   (const double*)d_samples, d_coef);
+                hipLaunchKernelGGL(sdf_prefilter_y, dim3(nblk(P0 * g.n2, 256)), dim3(256), 0, ctx->stream, g, lu(g.n1), d_coef);
+                hipLaunchKernelGGL(sdf_prefilter_z, dim3(nblk(P0 * P1, 256)), dim3(256), 0, ctx->stream, g, lu(g.n2), d_coef);
+                st = check_launch(ctx, "sdf_prefilter");
+            }
+        }
+        P* dp = nullptr;
+        if (st == EB_OK) st = dalloc(ctx, dp, ncoef);
+        if (st == EB_OK) {
+            hipLaunchKernelGGL(sdf_to_storage<P>, dim3(std::min<unsigned>(nblk(ncoef, 256), 4096)), dim3(256), 0, ctx->stream,
+                               ncoef, (const double*)d_coef, dp);
+            st = check_launch(ctx, "sdf_to_storage");
             h[s].coef = dp;
         }
+        // the scratch is reused by the next instance
+        if (st == EB_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) st = EB_ERR_DEVICE;
+    }
+    release();
+    if (st != EB_OK) {
+        if (ctx->err.empty()) ctx->err = "SDF prefilter on the device failed";
+        return st;
     }
     SDFDev<P>* dptr;
     EB_TRY(upload_vec(ctx, dptr, h));
@@ -1830,6 +1881,26 @@ int eb_profile_enable(eb_ctx* ctx, uint32_t mask) {
     prof_harvest(ctx);
     for (int k = 0; k < EB_KERNEL_COUNT; ++k) { ctx->prof_ms[k] = 0; ctx->prof_n[k] = 0; ctx->ev_open[k] = nullptr; }
     ctx->prof_mask = mask;
+    return EB_OK;
+}
+
+int eb_get_sdf_coefficients(eb_ctx* ctx, int64_t instance, double* out, int64_t count) {
+    if (!ctx || !out || instance < 0 || instance >= ctx->S) return EB_ERR_ARG;
+    if (ctx->surface != EB_SURFACE_SDF) { ctx->err = "no signed distance field"; return EB_ERR_STATE; }
+    const bool f32 = ctx->precision == EB_PRECISION_FP32;
+    const size_t rec = f32 ? sizeof(SDFDev<float>) : sizeof(SDFDev<double>);
+    std::vector<char> hdr(rec);
+    EB_HIP(hipMemcpy(hdr.data(), (const char*)ctx->d_sdf + rec * instance, rec, hipMemcpyDeviceToHost));
+    const int32_t* n = f32 ? ((const SDFDev<float>*)hdr.data())->n : ((const SDFDev<double>*)hdr.data())->n;
+    const int64_t ncoef = (int64_t)(n[0] + 2) * (n[1] + 2) * (n[2] + 2);
+    if (count != ncoef) { ctx->err = "coefficient count mismatch"; return EB_ERR_ARG; }
+    if (f32) {
+        std::vector<float> tmp((size_t)ncoef);
+        EB_HIP(hipMemcpy(tmp.data(), ((const SDFDev<float>*)hdr.data())->coef, sizeof(float) * ncoef, hipMemcpyDeviceToHost));
+        for (int64_t q = 0; q < ncoef; ++q) out[q] = tmp[q];
+    } else {
+        EB_HIP(hipMemcpy(out, ((const SDFDev<double>*)hdr.data())->coef, sizeof(double) * ncoef, hipMemcpyDeviceToHost));
+    }
     return EB_OK;
 }
 

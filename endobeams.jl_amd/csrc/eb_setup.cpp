@@ -216,21 +216,25 @@ void build_chains(Chains& c, const Pattern& p, const double* X0, int mode) {
 // Separable solve of  c₀ = c₁, c_{i-1}/8 + 3c_i/4 + c_{i+1}/8 = y_i, c_{n+1} = c_n  along each axis
 // (the Reflect(OnCell()) quadratic B-spline system, src/interactions/def_interactions.jl:77).
 // The tridiagonal factor is computed once per axis length and reused for every line.
+void tri_factors(int64_t n, std::vector<double>& cp, std::vector<double>& inv) {
+    cp.assign((size_t)n, 0.0);
+    inv.assign((size_t)n, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+        double a = i == 0 ? 0.0 : 0.125;
+        double b = (i == 0 || i == n - 1) ? 0.875 : 0.75;
+        if (n == 1) b = 1.0;
+        double c = i == n - 1 ? 0.0 : 0.125;
+        double den = b - (i > 0 ? a * cp[i - 1] : 0.0);
+        inv[i] = 1.0 / den;
+        cp[i] = c * inv[i];
+    }
+}
+
 namespace {
 struct Tri {
     int64_t n;
     std::vector<double> cp, inv;  // modified super-diagonal and reciprocal pivots
-    explicit Tri(int64_t n_) : n(n_), cp(n_), inv(n_) {
-        for (int64_t i = 0; i < n; ++i) {
-            double a = i == 0 ? 0.0 : 0.125;
-            double b = (i == 0 || i == n - 1) ? 0.875 : 0.75;
-            if (n == 1) b = 1.0;
-            double c = i == n - 1 ? 0.0 : 0.125;
-            double den = b - (i > 0 ? a * cp[i - 1] : 0.0);
-            inv[i] = 1.0 / den;
-            cp[i] = c * inv[i];
-        }
-    }
+    explicit Tri(int64_t n_) : n(n_) { tri_factors(n, cp, inv); }
     // y (stride sy, n values) -> padded c (stride sc, n+2 values)
     void solve(const double* y, int64_t sy, double* c, int64_t sc, double* tmp) const {
         for (int64_t i = 0; i < n; ++i) tmp[i] = (y[i * sy] - (i > 0 ? 0.125 * tmp[i - 1] : 0.0)) * inv[i];

@@ -55,5 +55,8 @@ void build_chains(Chains& c, const Pattern& p, const double* X0, int mode);
 
 // Quadratic B-spline Reflect(OnCell()) coefficients, padded by one cell per side
 void prefilter_sdf(const int64_t n[3], const double* samples, std::vector<double>& coef);
+// LU factors of the n×n system of prefilter_sdf along one axis: modified super-diagonal cp and
+// reciprocal pivots inv (the device prefilter uses the same factors)
+void tri_factors(int64_t n, std::vector<double>& cp, std::vector<double>& inv);
 
 }  // namespace ebh

@@ -119,6 +119,16 @@ class Engine:
         self._check(self.fn["profile_read"](self.h, abi.KERNELS[kernel], C.byref(ms), C.byref(n)), "profile_read")
         return ms.value, n.value
 
+    def sdf_coefficients(self, instance: int = 0) -> np.ndarray:
+        """The padded quadratic B-spline coefficients of an instance's SDF as the engine holds them
+        ((n2+2, n1+2, n0+2), x fastest)."""
+        conf, _, inter = self.instances[instance] if hasattr(self, "instances") else (self.conf, None, self.inter)
+        n = inter.master.n
+        out = np.empty((n[2] + 2) * (n[1] + 2) * (n[0] + 2))
+        self._check(self.fn["get_sdf_coefficients"](self.h, instance, out.ctypes.data_as(C.POINTER(C.c_double)), out.size),
+                    "get_sdf_coefficients")
+        return out.reshape(n[2] + 2, n[1] + 2, n[0] + 2)
+
     # -- execution mode and launch shapes (no reference counterpart) --------------------------------
     def set_exec_mode(self, graph: bool):
         """Newton iterations as replays of a captured HIP graph (True) or launched one by one."""

@@ -249,6 +249,12 @@ int eb_profile_enable(eb_ctx* ctx, uint32_t mask);
 /* total milliseconds and number of timed launches of kernel `id` since enabling */
 int eb_profile_read(eb_ctx* ctx, int id, double* total_ms, int64_t* launches);
 
+/* Prefiltered quadratic B-spline coefficients of instance `instance`'s signed distance field as
+ * the device holds them ((n0+2)(n1+2)(n2+2), x fastest, converted to double) — the coefficient
+ * array of the reference's interpolant (src/interactions/def_interactions.jl:77). The device
+ * computes them at eb_create from the samples (unless the descriptor passes them prefiltered). */
+int eb_get_sdf_coefficients(eb_ctx* ctx, int64_t instance, double* out, int64_t count);
+
 /* --- execution mode and launch shapes (no counterpart in the reference) --------------------- */
 
 /* How eb_corrector issues a Newton iteration. EAGER: the host launches every kernel.

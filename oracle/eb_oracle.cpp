@@ -793,6 +793,15 @@ int ebo_sdf_eval(void* h, const double* p, double* out13) {
     return EB_OK;
 }
 
+int ebo_get_sdf_coefficients(void* h, int64_t instance, double* out, int64_t count) {
+    Oracle* o = (Oracle*)h;
+    if (instance != 0) return fail(o, EB_ERR_ARG, "one instance");
+    if (o->surf.kind != EB_SURFACE_SDF) return fail(o, EB_ERR_STATE, "no SDF");
+    if ((int64_t)o->surf.sdf.coef.size() != count) return fail(o, EB_ERR_ARG, "size mismatch");
+    std::memcpy(out, o->surf.sdf.coef.data(), sizeof(double) * count);
+    return EB_OK;
+}
+
 // prefiltered coefficients ((n0+2)(n1+2)(n2+2), x fastest)
 int ebo_sdf_coefs(void* h, double* out, int64_t count) {
     Oracle* o = (Oracle*)h;
