@@ -59,23 +59,25 @@ def run_steps(stepper, n):
 
 def timed(eng, params, E, warmup, steps, kernel="beam", graph=True, autotune=True):
     """Warm-up steps (then the autotuner, on the live state), then `steps` timed converged steps."""
-    from endobeams_amd import Stepper
+    from endobeams_amd import Stepper, abi
     eng.set_exec_mode(graph)
     st = Stepper(eng, params)
     st.initialize()
     run_steps(st, warmup)
     tuning = eng.autotune() if autotune else (eng.tuning, None)
     eng.profile_enable([kernel])
-    n0, it0, t0sim, f0 = st.stats.steps, st.stats.newton_iterations, st.tn, st.stats.failed_steps
+    # the timed steps: the library's own time loop (eb_run_steps), continuing the warm-up's
+    state = abi.RunState(t=st.tn, dt=st.dt, successive=st.successive)
     t0 = time.perf_counter()
-    done, attempts = run_steps(st, steps)
+    state = eng.run_steps(steps, state)
     wall = time.perf_counter() - t0
+    done = state.steps
     ms, launches = eng.profile_read(kernel)
-    return {"steps": done, "attempts": attempts, "wall": wall, "E": E,
+    return {"steps": done, "attempts": done + state.failed_steps, "wall": wall, "E": E,
             "value": E * done / wall, "ms_per_step": 1e3 * wall / max(done, 1),
-            "newton_iters_per_step": (st.stats.newton_iterations - it0) / max(done, 1),
-            "failed_steps": st.stats.failed_steps - f0,
-            "wall_s_per_simulated_s": wall / (st.tn - t0sim) if st.tn > t0sim else None,
+            "newton_iters_per_step": state.newton_iterations / max(done, 1),
+            "failed_steps": state.failed_steps,
+            "wall_s_per_simulated_s": wall / (state.t - st.tn) if state.t > st.tn else None,
             "kernel_ms": ms, "kernel_launches": launches, "stepper": st,
             "exec": {"mode": "graph" if graph else "eager", "graph_captures_replays": list(eng.exec_stats()[1:]),
                      "tuning": tuning[0], "tuning_ms": tuning[1]}}
@@ -147,19 +149,19 @@ def sweep(args):
         e.profile_enable(["beam"])
         engs.append(e)
         steppers.append(st)
-    base = [(st.stats.steps, st.stats.newton_iterations, st.stats.failed_steps, st.tn) for st in steppers]
+    states = [abi.RunState(t=st.tn, dt=st.dt, successive=st.successive) for st in steppers]
+    t_start = [st.tn for st in steppers]
     t0 = time.perf_counter()
-    done = [0] * len(engs)
-    for _ in range(args.steps):
-        for i, st in enumerate(steppers):
-            d, _ = run_steps(st, 1)
-            done[i] += d
+    for _ in range(args.steps):   # the contexts in turn, one step each (eb_run_steps)
+        for i, e in enumerate(engs):
+            states[i] = e.run_steps(1, states[i])
     wall = time.perf_counter() - t0
     E_g = [int(e.beam_off[-1]) for e in engs]
+    done = [s.steps for s in states]
     value = sum(E * d for E, d in zip(E_g, done)) / wall
-    its = sum(st.stats.newton_iterations - b[1] for st, b in zip(steppers, base)) / max(sum(done), 1)
-    failed = sum(st.stats.failed_steps - b[2] for st, b in zip(steppers, base))
-    sim = min(st.tn - b[3] for st, b in zip(steppers, base))
+    its = sum(s.newton_iterations for s in states) / max(sum(done), 1)
+    failed = sum(s.failed_steps for s in states)
+    sim = min(s.t - t for s, t in zip(states, t_start))
     kms = kn = 0.0
     for e in engs:
         ms, n = e.profile_read("beam")

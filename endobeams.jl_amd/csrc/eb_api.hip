@@ -1787,6 +1787,51 @@ int eb_solve_step_dynamics(eb_ctx* ctx, const eb_newton_params* p, double dt, in
     return eb_corrector(ctx, p, dt, k);
 }
 
+int eb_run_steps(eb_ctx* ctx, const eb_newton_params* p, const eb_time_params* tp, int64_t max_steps,
+                 eb_step_callback callback, void* user, eb_run_state* state) {
+    if (!ctx || !p || !tp || !state || max_steps < 0) return EB_ERR_ARG;
+    int64_t done = 0;
+    while (done < max_steps && tp->simulation_end_time - state->t > 1.0e-10 && !state->stopped) {
+        const double tnp1 = state->t + state->dt;
+        if (callback && callback(ctx, tnp1, user) != 0) {
+            ctx->err = "step callback failed at t = " + std::to_string(tnp1);
+            return EB_ERR_ARG;
+        }
+        int k = 0;
+        EB_TRY(eb_solve_step_dynamics(ctx, p, state->dt, &k));
+        state->last_k = k;
+        if (k > p->max_iterations) {
+            state->dt /= 2;
+            state->failed_steps += 1;
+            if (state->dt < tp->min_timestep) {
+                char buf[96];
+                snprintf(buf, sizeof buf, "Δt < %.2e: simulation aborted", tp->min_timestep);
+                ctx->err = buf;
+                return EB_ERR_STATE;
+            }
+            EB_TRY(eb_update_not_converged(ctx));
+            state->successive = 0;
+            continue;
+        }
+        state->successive += 1;
+        state->steps += 1;
+        state->newton_iterations += k - 1;
+        done += 1;
+        if (state->successive >= tp->accelerate_after_success_it && k <= 8) {
+            state->dt = std::min(state->dt * 1.5, tp->max_timestep);
+            state->successive = 0;
+        }
+        EB_TRY(eb_update_converged(ctx));
+        state->t = tnp1;
+        if (tp->stop_on_energy_threshold) {
+            double en[3];
+            EB_HIP(hipMemcpy(en, ctx->energy, sizeof en, hipMemcpyDeviceToHost));
+            if (en[1] < tp->energy_threshold) state->stopped = 1;
+        }
+    }
+    return EB_OK;
+}
+
 int eb_update_converged(eb_ctx* ctx) {
     if (!ctx) return EB_ERR_ARG;
     // rate of the step that converged, newest first in the history of the learned warm start

@@ -58,6 +58,26 @@ class NewtonParams(C.Structure):
     ]
 
 
+class TimeParams(C.Structure):
+    _fields_ = [
+        ("initial_timestep", C.c_double), ("min_timestep", C.c_double), ("max_timestep", C.c_double),
+        ("simulation_end_time", C.c_double), ("accelerate_after_success_it", C.c_int32),
+        ("stop_on_energy_threshold", C.c_int32), ("energy_threshold", C.c_double),
+    ]
+
+
+class RunState(C.Structure):
+    _fields_ = [
+        ("t", C.c_double), ("dt", C.c_double), ("successive", C.c_int32), ("stopped", C.c_int32),
+        ("steps", C.c_int64), ("failed_steps", C.c_int64), ("newton_iterations", C.c_int64),
+        ("last_k", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
+# int (*)(eb_ctx*, double t, void* user)
+STEP_CALLBACK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_void_p)
+
+
 class Sizes(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("n_nodes", "n_beams", "ndofs", "nnz", "n_free", "n_fixed", "nnz_free")]
 
@@ -99,6 +119,7 @@ PROTOTYPES = {
     "solve_step_dynamics": (_i, [_vp, _NP, _d, _IP]),
     "update_converged": (_i, [_vp]),
     "update_not_converged": (_i, [_vp]),
+    "run_steps": (_i, [_vp, _NP, C.POINTER(TimeParams), C.c_int64, STEP_CALLBACK, _vp, C.POINTER(RunState)]),
     "get_counters": (_i, [_vp, _lp]),
     "profile_enable": (_i, [_vp, C.c_uint32]),
     "profile_read": (_i, [_vp, _i, _dp, _lp]),

@@ -252,6 +252,33 @@ class Engine:
         self._check(self.fn["solve_step_dynamics"](self.h, C.byref(self._np), dt, C.byref(k)), "solve_step_dynamics")
         return k.value
 
+    def run_steps(self, max_steps: int, state: Optional[abi.RunState] = None) -> abi.RunState:
+        """run_simulation!'s time loop inside the library (eb_run_steps): up to `max_steps` more
+        converged steps from `state` (a fresh one: t = 0, Δt = initial_timestep). Loads and
+        imposed displacements of each attempt's time are set by a callback into this engine."""
+        p = self.params
+        if state is None:
+            state = abi.RunState(t=0.0, dt=p.initial_timestep)
+        tp = abi.TimeParams(p.initial_timestep, p.min_timestep, p.max_timestep, p.simulation_end_time,
+                            int(p.accelerate_after_success_it), int(bool(p.stop_on_energy_threshold)),
+                            float(p.energy_threshold))
+        errors = []
+
+        def on_step(ctx, t, user):
+            try:
+                self.update_loads(t)
+                self.update_boundary_conditions(t)
+                return 0
+            except Exception as e:  # reported after eb_run_steps returns
+                errors.append(e)
+                return 1
+        cb = abi.STEP_CALLBACK(on_step)
+        st = self.fn["run_steps"](self.h, C.byref(self._np), C.byref(tp), int(max_steps), cb, None, C.byref(state))
+        if errors:
+            raise errors[0]
+        self._check(st, "run_steps")
+        return state
+
     def update_converged(self):
         self._check(self.fn["update_converged"](self.h), "update_converged")
 

@@ -92,13 +92,25 @@ class Stepper:
 
 def run_simulation(conf: BeamsConfiguration, params: SimulationParams, inter: Optional[RigidInteraction] = None,
                    engine: Optional[Engine] = None, on_converged: Optional[Callable] = None,
-                   max_steps: Optional[int] = None) -> RunStats:
+                   max_steps: Optional[int] = None, driver: str = "host") -> RunStats:
     """run_simulation!(conf, params, inter) (src/run_simulation.jl:1-119).
 
     `engine` defaults to the HIP engine. `on_converged(step, t, engine)` is called after each
     converged step, before update_converged! (where the reference writes VTK output).
-    `max_steps` bounds the number of converged steps (None: run to simulation_end_time)."""
+    `max_steps` bounds the number of converged steps (None: run to simulation_end_time).
+    driver="device": the time loop runs inside the library (eb_run_steps), whole steps without a
+    return to Python; `on_converged` is then not available."""
     eng = engine if engine is not None else HipEngine(conf, params, inter)
+    if driver == "device":
+        if on_converged is not None:
+            raise ValueError("on_converged needs the host driver")
+        start = time.perf_counter()
+        eng.initialize_state_simulation()
+        state = eng.run_steps(max_steps if max_steps is not None else 2**62)
+        stats = RunStats(steps=state.steps, failed_steps=state.failed_steps,
+                         newton_iterations=state.newton_iterations, t_end=state.t)
+        stats.wall_s = time.perf_counter() - start
+        return stats
     st = Stepper(eng, params)
     st.initialize()
     while not st.finished and not st.stop:

@@ -210,6 +210,44 @@ int eb_solve_step_dynamics(eb_ctx* ctx, const eb_newton_params* p, double dt, in
 int eb_update_converged(eb_ctx* ctx);
 int eb_update_not_converged(eb_ctx* ctx);
 
+/* --- the time loop ---------------------------------------------------------------------------- */
+
+/* Δt control and stop criterion of run_simulation! (SimulationParams, src/def_params.jl:1-35) */
+typedef struct eb_time_params {
+    double initial_timestep, min_timestep, max_timestep, simulation_end_time;
+    int32_t accelerate_after_success_it;  /* Δt × 1.5 after this many successive steps with k <= 8 */
+    int32_t stop_on_energy_threshold;     /* stop once the kinetic energy is below energy_threshold */
+    double energy_threshold;
+} eb_time_params;
+
+/* Progress of the time loop, read and updated by eb_run_steps: before the first call t = 0,
+ * dt = initial_timestep and everything else 0; later calls continue where the last one ended. */
+typedef struct eb_run_state {
+    double t, dt;                   /* tⁿ and the next step's Δt */
+    int32_t successive;             /* successive converged steps since the last Δt change */
+    int32_t stopped;                /* 1: the energy criterion ended the run */
+    int64_t steps, failed_steps;    /* converged steps, attempts repeated with Δt/2 */
+    int64_t newton_iterations;      /* Σ (k - 1) over converged steps */
+    int32_t last_k, pad;            /* k of the last attempt */
+} eb_run_state;
+
+/* Called before every step attempt with the attempt's time tⁿ⁺¹: sets the loads and imposed
+ * displacements of that time (update_loads!, update_boundary_conditions!,
+ * src/utils_bcs_and_loads.jl:42-64) through eb_update_loads / eb_update_boundary_conditions and
+ * returns 0 (nonzero stops eb_run_steps with EB_ERR_ARG). NULL: no time-dependent loads. */
+typedef int (*eb_step_callback)(eb_ctx* ctx, double t, void* user);
+
+/* run_simulation!'s time loop (src/run_simulation.jl:38-112) inside the library, so that whole
+ * time steps stay on the device: until max_steps more steps have converged, the end time is
+ * reached or the energy criterion stops the run. Each attempt: callback(tⁿ + Δt), predictor!,
+ * corrector!; k > max_iterations ⇒ Δt / 2 and update_not_converged! (EB_ERR_STATE when Δt falls
+ * below min_timestep: the reference's "Simulation aborted."); otherwise Δt grows to
+ * min(1.5 Δt, max_timestep) after accelerate_after_success_it successive steps with k <= 8, and
+ * the step is committed (update_converged!). Wall-clock limits and output are the caller's (it
+ * calls eb_run_steps in chunks). eb_initialize_state_simulation must have run. */
+int eb_run_steps(eb_ctx* ctx, const eb_newton_params* p, const eb_time_params* tp, int64_t max_steps,
+                 eb_step_callback callback, void* user, eb_run_state* state);
+
 /* --- batches of independent structures (no counterpart in the reference) ------------------
  * For design sweeps: one context advances many structures (instances) together, each as if it
  * ran alone through run_simulation! (src/run_simulation.jl:1-119) with the batch's time step.

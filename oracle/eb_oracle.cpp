@@ -690,6 +690,49 @@ int ebo_update_not_converged(void* h) {
 
 // Batches: the oracle runs one structure per context; a batch is checked against each of its
 // instances run alone, so only a batch of one is accepted here.
+// run_simulation!'s time loop (src/run_simulation.jl:38-112) over the oracle's own step functions
+int ebo_run_steps(void* h, const eb_newton_params* p, const eb_time_params* tp, int64_t max_steps,
+                  int (*callback)(void*, double, void*), void* user, eb_run_state* state) {
+    Oracle* o = (Oracle*)h;
+    if (!p || !tp || !state || max_steps < 0) return EB_ERR_ARG;
+    int64_t done = 0;
+    while (done < max_steps && tp->simulation_end_time - state->t > 1.0e-10 && !state->stopped) {
+        const double tnp1 = state->t + state->dt;
+        if (callback && callback(h, tnp1, user) != 0) return fail(o, EB_ERR_ARG, "step callback failed");
+        int k = 0;
+        int st = ebo_solve_step_dynamics(h, p, state->dt, &k);
+        if (st != EB_OK) return st;
+        state->last_k = k;
+        if (k > p->max_iterations) {
+            state->dt = state->dt / 2;
+            state->failed_steps += 1;
+            if (state->dt < tp->min_timestep) return fail(o, EB_ERR_STATE, "time step below min_timestep: simulation aborted");
+            st = ebo_update_not_converged(h);
+            if (st != EB_OK) return st;
+            state->successive = 0;
+            continue;
+        }
+        state->successive += 1;
+        state->steps += 1;
+        state->newton_iterations += k - 1;
+        done += 1;
+        if (state->successive >= tp->accelerate_after_success_it && k <= 8) {
+            state->dt = std::min(state->dt * 1.5, tp->max_timestep);
+            state->successive = 0;
+        }
+        st = ebo_update_converged(h);
+        if (st != EB_OK) return st;
+        state->t = tnp1;
+        if (tp->stop_on_energy_threshold) {
+            double en[3];
+            st = ebo_get_field(h, EB_F_ENERGY, en, 3);
+            if (st != EB_OK) return st;
+            if (en[1] < tp->energy_threshold) state->stopped = 1;
+        }
+    }
+    return EB_OK;
+}
+
 int ebo_create_batch(const eb_model_desc* d, int64_t n_instances, void** out) {
     if (n_instances != 1) return EB_ERR_ARG;
     return ebo_create(d, out);
