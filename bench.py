@@ -357,6 +357,15 @@ def main():
         eng.close()
 
     main_form = lrn if lrn is not None else (fast if fast is not None else plain)
+    traffic = None
+    if main_form is not plain and args.batch == 16 and args.sdf_n == 256:
+        # HBM bytes per launch of this kernel on this workload, from the committed FETCH_SIZE /
+        # WRITE_SIZE profiler runs (scaled by a device copy of known size)
+        tf = os.path.join(REPO, "profiles", "braided_stent_batch16_fp32_hbm_traffic.json")
+        if os.path.exists(tf):
+            k = json.load(open(tf))["kernels"].get("ebd::beam_detect_kernel<float>")
+            if k:
+                traffic = k["fetch_bytes"] + k["write_bytes"]
     avg_s = main_form["kernel_ms"] * 1e-3 / max(main_form["kernel_launches"], 1)
     flops = FLOPS_PER_BEAM * main_form["E"]
     achieved = flops / avg_s / 1e12
@@ -386,8 +395,8 @@ def main():
         "wall_s_per_simulated_s": main_form["wall_s_per_simulated_s"],
         "exec": main_form["exec"],
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": "beam_kernel<float>" if reduced else "beam_kernel<double>",
+                     "frac": achieved / peak, "traffic": traffic,
+                     "kernel": "beam_detect_kernel<float>" if reduced else "beam_detect_kernel<double>",
                      "avg_launch_us": avg_s * 1e6, "launches": main_form["kernel_launches"],
                      "flops_per_launch": flops},
     }

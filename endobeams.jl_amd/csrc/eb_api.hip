@@ -18,6 +18,7 @@
 #include "eb_learned.h"
 #include "eb_linalg.h"
 #include "eb_mlp.h"
+#include "eb_output.h"
 #include "eb_sdf_prefilter.h"
 #include "eb_setup.h"
 
@@ -1927,6 +1928,36 @@ int eb_profile_enable(eb_ctx* ctx, uint32_t mask) {
     for (int k = 0; k < EB_KERNEL_COUNT; ++k) { ctx->prof_ms[k] = 0; ctx->prof_n[k] = 0; ctx->ev_open[k] = nullptr; }
     ctx->prof_mask = mask;
     return EB_OK;
+}
+
+int eb_output_points(eb_ctx* ctx, int32_t n_points, double* position, double* displacement, double* velocity,
+                     double* strain, double* stress) {
+    if (!ctx || n_points < 1) return EB_ERR_ARG;
+    const int64_t np = (int64_t)n_points * ctx->E;
+    double* outs[5] = {position, displacement, velocity, strain, stress};
+    const int64_t width[5] = {3, 3, 3, 1, 1};
+    double* dev[5] = {};
+    auto release = [&] {
+        for (double* q : dev)
+            if (q) (void)hipFree(q);
+    };
+    for (int k = 0; k < 5; ++k)
+        if (outs[k] && hipMalloc(&dev[k], sizeof(double) * width[k] * std::max<int64_t>(np, 1)) != hipSuccess) {
+            release();
+            ctx->err = "eb_output_points: device allocation failed";
+            return EB_ERR_DEVICE;
+        }
+    hipLaunchKernelGGL(beam_output_points, dim3(nblk(ctx->E, 256)), dim3(256), 0, ctx->stream, node_state(ctx),
+                       beam_consts(ctx), (int)n_points, dev[0], dev[1], dev[2], dev[3], dev[4]);
+    int st = check_launch(ctx, "beam_output_points");
+    for (int k = 0; k < 5 && st == EB_OK; ++k)
+        if (outs[k] && hipMemcpyAsync(outs[k], dev[k], sizeof(double) * width[k] * np, hipMemcpyDeviceToHost, ctx->stream) !=
+                           hipSuccess)
+            st = EB_ERR_DEVICE;
+    if (st == EB_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) st = EB_ERR_DEVICE;
+    release();
+    if (st != EB_OK && ctx->err.empty()) ctx->err = "eb_output_points failed";
+    return st;
 }
 
 int eb_get_sdf_coefficients(eb_ctx* ctx, int64_t instance, double* out, int64_t count) {

@@ -127,6 +127,7 @@ PROTOTYPES = {
     "get_instances": (_i, [_vp, _lp, _lp, _lp]),
     "get_instance_iterations": (_i, [_vp, C.POINTER(C.c_int32)]),
     "get_instance_norms": (_i, [_vp, _dp]),
+    "output_points": (_i, [_vp, C.c_int32, _dp, _dp, _dp, _dp, _dp]),
     "get_sdf_coefficients": (_i, [_vp, C.c_int64, _dp, C.c_int64]),
     "set_exec_mode": (_i, [_vp, _i]),
     "get_exec_stats": (_i, [_vp, C.POINTER(C.c_int32), _lp, _lp]),

@@ -119,6 +119,18 @@ class Engine:
         self._check(self.fn["profile_read"](self.h, abi.KERNELS[kernel], C.byref(ms), C.byref(n)), "profile_read")
         return ms.value, n.value
 
+    def output_points(self, n_points: int = 10) -> dict:
+        """recompute_at_gausspts_beams! (src/IO/visualization_beams.jl:132-181): position,
+        displacement, velocity (n_beams·n_points × 3), strain and stress (n_beams·n_points) at
+        n_points points along every beam, from the current node state."""
+        m = self.sizes.n_beams * n_points
+        out = {"position": np.empty((m, 3)), "displacement": np.empty((m, 3)), "velocity": np.empty((m, 3)),
+               "strain": np.empty(m), "stress": np.empty(m)}
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        self._check(self.fn["output_points"](self.h, int(n_points), dp(out["position"]), dp(out["displacement"]),
+                                             dp(out["velocity"]), dp(out["strain"]), dp(out["stress"])), "output_points")
+        return out
+
     def sdf_coefficients(self, instance: int = 0) -> np.ndarray:
         """The padded quadratic B-spline coefficients of an instance's SDF as the engine holds them
         ((n2+2, n1+2, n0+2), x fastest)."""

@@ -112,11 +112,28 @@ def run_simulation(conf: BeamsConfiguration, params: SimulationParams, inter: Op
         stats.wall_s = time.perf_counter() - start
         return stats
     st = Stepper(eng, params)
+    hook = on_converged
+    if params.output_dir:
+        # write_VTK of the initial state, then at the output cadence, before update_converged!
+        # (run_simulation.jl:30-32, :86-90); the file carries tⁿ, as the reference passes it
+        from .vtk import VTKDataBeams
+        vtk = VTKDataBeams(conf, params.output_dir)
+        vtk.write(0, st.step_index, 0.0, eng)
+        out = {"counter": 1, "t": 0.0}
+
+        def hook(step, tnp1, e):
+            due = out["t"] + params.output_timestep
+            if tnp1 > due or abs(tnp1 - due) <= 1.4901161193847656e-08 * max(abs(tnp1), abs(due)):
+                vtk.write(out["counter"], step, st.tn, e)
+                out["counter"] += 1
+                out["t"] = tnp1
+            if on_converged is not None:
+                on_converged(step, tnp1, e)
     st.initialize()
     while not st.finished and not st.stop:
         if max_steps is not None and st.stats.steps >= max_steps:
             break
-        st.advance(on_converged)
+        st.advance(hook)
     st.stats.t_end = st.tn
     st.stats.wall_s = time.perf_counter() - st.start
     return st.stats

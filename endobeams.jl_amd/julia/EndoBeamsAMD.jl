@@ -171,7 +171,8 @@ end
 newton(p) = NewtonParams(p.α, p.β, p.γ, p.tolerance_residual, p.tolerance_displacement, Int32(p.max_iterations), Int32(200), 1e-13)
 
 """solve_step_dynamics!(conf, state, params, inter, tⁿ⁺¹, Δt, solver) (src/solve_step_dynamics.jl:1-14)."""
-function solve_step_dynamics!(dev::Device, conf, params, tⁿ⁺¹, Δt)
+"""update_loads! and update_boundary_conditions! (src/utils_bcs_and_loads.jl:42-64) at time tⁿ⁺¹."""
+function update_loads_and_bcs!(dev::Device, conf, tⁿ⁺¹)
     if conf.loads !== nothing && conf.loads.concentrated_force !== nothing
         dofs = collect(Int64, conf.loads.concentrated_force.loaded_dofs)
         vals = [conf.loads.concentrated_force.force_function(tⁿ⁺¹, i) for i in dofs]
@@ -189,6 +190,11 @@ function solve_step_dynamics!(dev::Device, conf, params, tⁿ⁺¹, Δt)
                                   dev.ctx, v, length(v))
         check(st, dev.ctx, "eb_update_boundary_conditions")
     end
+    return nothing
+end
+
+function solve_step_dynamics!(dev::Device, conf, params, tⁿ⁺¹, Δt)
+    update_loads_and_bcs!(dev, conf, tⁿ⁺¹)
     k = Ref{Cint}(0)
     st = ccall((:eb_solve_step_dynamics, LIB), Cint, (Ptr{Cvoid}, Ref{NewtonParams}, Float64, Ref{Cint}),
                dev.ctx, newton(params), Δt, k)
@@ -218,6 +224,66 @@ end
 set_learned_backoff!(dev, max_skip::Integer) =
     check(ccall((:eb_set_learned_backoff, LIB), Cint, (Ptr{Cvoid}, Int32), dev.ctx, max_skip), dev.ctx,
           "eb_set_learned_backoff")
+
+# the time loop inside the library (eb_run_steps)
+struct TimeParams
+    initial_timestep::Float64
+    min_timestep::Float64
+    max_timestep::Float64
+    simulation_end_time::Float64
+    accelerate_after_success_it::Int32
+    stop_on_energy_threshold::Int32
+    energy_threshold::Float64
+end
+mutable struct RunState
+    t::Float64
+    dt::Float64
+    successive::Int32
+    stopped::Int32
+    steps::Int64
+    failed_steps::Int64
+    newton_iterations::Int64
+    last_k::Int32
+    pad::Int32
+end
+RunState(params) = RunState(0.0, params.initial_timestep, 0, 0, 0, 0, 0, 0, 0)
+
+# loads and imposed displacements of time t, called by the library before every step attempt
+function step_callback(ctx::Ptr{Cvoid}, t::Float64, user::Ptr{Cvoid})::Cint
+    conf = unsafe_pointer_to_objref(user)
+    dev = Device(ctx, Any[])
+    try
+        update_loads_and_bcs!(dev, conf, t)
+        return Cint(0)
+    catch
+        return Cint(1)
+    end
+end
+
+"""run_simulation!'s time loop (src/run_simulation.jl:38-112) inside the library: up to
+`max_steps` more converged steps from `state`."""
+function run_steps!(dev, conf, params, state::RunState, max_steps::Integer)
+    tp = TimeParams(params.initial_timestep, params.min_timestep, params.max_timestep, params.simulation_end_time,
+                    params.accelerate_after_success_it, params.stop_on_energy_threshold, params.energy_threshold)
+    np = newton(params)
+    cb = @cfunction(step_callback, Cint, (Ptr{Cvoid}, Float64, Ptr{Cvoid}))
+    st = GC.@preserve conf ccall((:eb_run_steps, LIB), Cint,
+                                 (Ptr{Cvoid}, Ref{NewtonParams}, Ref{TimeParams}, Int64, Ptr{Cvoid}, Any, Ref{RunState}),
+                                 dev.ctx, np, tp, max_steps, cb, conf, state)
+    check(st, dev.ctx, "eb_run_steps")
+    return state
+end
+
+"""recompute_at_gausspts_beams! (src/IO/visualization_beams.jl:132-181) on the device: positions,
+displacements, velocities (3 × n per beam), strain and stress at n points along every beam."""
+function output_points(dev, nbeams::Integer, n::Integer=10)
+    m = nbeams * n
+    pos, disp, vel = zeros(3m), zeros(3m), zeros(3m)
+    strain, stress = zeros(m), zeros(m)
+    check(ccall((:eb_output_points, LIB), Cint, (Ptr{Cvoid}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                Ptr{Float64}, Ptr{Float64}), dev.ctx, n, pos, disp, vel, strain, stress), dev.ctx, "eb_output_points")
+    return pos, disp, vel, strain, stress
+end
 
 """run_simulation!(conf, params, inter) with the hot path on the GPU (src/run_simulation.jl:1-119)."""
 function run_simulation_gpu!(conf, params, inter=nothing)

@@ -287,6 +287,17 @@ int eb_profile_enable(eb_ctx* ctx, uint32_t mask);
 /* total milliseconds and number of timed launches of kernel `id` since enabling */
 int eb_profile_read(eb_ctx* ctx, int id, double* total_ms, int64_t* launches);
 
+/* --- output ----------------------------------------------------------------------------------- */
+
+/* recompute_at_gausspts_beams! (src/IO/visualization_beams.jl:132-181), the data of write_VTK:
+ * per beam, n_points points evenly spaced along it (range(-1, 1, length = n_points); the
+ * reference uses 10), from the current node state: position N₁x₁ + N₂x₂ + Rₑuᵗ (with the cubic
+ * bending offset uᵗ), displacement N₁u₁ + N₂u₂, velocity N₁u̇₁ + N₂u̇₂, strain 1 - lₙ/l₀ and stress
+ * E(1 - lₙ/l₀). Point i of beam b (both 0-based) is entry b·n_points + i; vectors take 3 doubles
+ * per point. Any output pointer may be NULL. */
+int eb_output_points(eb_ctx* ctx, int32_t n_points, double* position, double* displacement, double* velocity,
+                     double* strain, double* stress);
+
 /* Prefiltered quadratic B-spline coefficients of instance `instance`'s signed distance field as
  * the device holds them ((n0+2)(n1+2)(n2+2), x fastest, converted to double) — the coefficient
  * array of the reference's interpolant (src/interactions/def_interactions.jl:77). The device

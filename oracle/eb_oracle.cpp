@@ -836,6 +836,42 @@ int ebo_sdf_eval(void* h, const double* p, double* out13) {
     return EB_OK;
 }
 
+// recompute_at_gausspts_beams! (src/IO/visualization_beams.jl:132-181)
+int ebo_output_points(void* h, int32_t n_points, double* position, double* displacement, double* velocity,
+                      double* strain, double* stress) {
+    Oracle* o = (Oracle*)h;
+    if (n_points < 1) return fail(o, EB_ERR_ARG, "n_points < 1");
+    for (int64_t e = 0; e < o->E; ++e) {
+        ElemIn in = gather_elem(o, e);
+        V3 x1 = in.X1 + in.u1, x2 = in.X2 + in.u2;
+        real ln = norm(x2 - x1);
+        LocalFrame f = local_Re_and_aux(x1, x2, in.R1, in.R2, col(in.Re0, 1), ln);
+        M3 ReT = tr(f.Re);
+        V3 Tb1 = toangle((ReT * in.R1) * in.Re0), Tb2 = toangle((ReT * in.R2) * in.Re0);
+        real Emod = o->props[(size_t)(EB_NPROPS * e + 1)];
+        for (int i = 0; i < n_points; ++i) {
+            int64_t k = e * n_points + i;
+            real z = n_points > 1 ? -1.0 + (2.0 * i) / (n_points - 1) : -1.0;
+            real xi = in.l0 * (z + 1) / 2;
+            real N1 = 1 - xi / in.l0, N2 = 1 - N1;
+            real c = 1 - xi / in.l0;
+            real N3 = xi * (c * c);
+            real N4 = -(1 - xi / in.l0) * ((xi * xi) / in.l0);
+            V3 ut = v3(0.0, N3 * Tb1[2] + N4 * Tb2[2], -N3 * Tb1[1] - N4 * Tb2[1]);
+            V3 xg = (N1 * x1 + N2 * x2) + f.Re * ut;
+            V3 d = N1 * in.u1 + N2 * in.u2, w = N1 * in.ud1 + N2 * in.ud2;
+            for (int a = 0; a < 3; ++a) {
+                if (position) position[3 * k + a] = xg[a];
+                if (displacement) displacement[3 * k + a] = d[a];
+                if (velocity) velocity[3 * k + a] = w[a];
+            }
+            if (strain) strain[k] = 1 - ln / in.l0;
+            if (stress) stress[k] = Emod * (1 - ln / in.l0);
+        }
+    }
+    return EB_OK;
+}
+
 int ebo_get_sdf_coefficients(void* h, int64_t instance, double* out, int64_t count) {
     Oracle* o = (Oracle*)h;
     if (instance != 0) return fail(o, EB_ERR_ARG, "one instance");
