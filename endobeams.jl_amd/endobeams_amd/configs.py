@@ -140,9 +140,10 @@ def l_cantilever(tol=1e-8, load=50.0):
     return conf, params, None
 
 
-def ring_drop(n_el=60, sdf_dtype=np.float32, tol=1e-8):
+def ring_drop(n_el=60, sdf_dtype=np.float32, tol=1e-8, height=0.5):
     """Config 2: ring of radius 1 (60 beams, closed), wire radius 0.05, E = 1e5, ν = 0.3,
-    ρ = 1000, gravity 9.81 as nodal loads, tilted 30° about x with its lowest point at z = 0.5,
+    ρ = 1000, gravity 9.81 as nodal loads, tilted 30° about x with its lowest point at z = 0.5
+    (`height`: a lower start reaches the plane within the first steps),
     falling onto the plane z = 0 given as φ = z sampled (FP32) on a 64×64×32 grid of spacing
     0.05 with origin (-1.575, -1.575, -0.25). kₙ = 1e4, μ = 0.3, εᵗ = 10, ηₙ = 0, α = -0.05,
     Δt = 1e-3, 2000 steps, no boundary conditions."""
@@ -151,7 +152,7 @@ def ring_drop(n_el=60, sdf_dtype=np.float32, tol=1e-8):
     a = math.radians(30.0)
     Rx = np.array([[1, 0, 0], [0, math.cos(a), -math.sin(a)], [0, math.sin(a), math.cos(a)]])
     P = P @ Rx.T
-    P[:, 2] += 0.5 - P[:, 2].min()
+    P[:, 2] += height - P[:, 2].min()
     conn = np.array([(i, i % n_el + 1) for i in range(1, n_el + 1)], dtype=np.int64)
     nodes = NodesBeams(P)
     rho, r = 1000.0, 0.05
@@ -166,15 +167,15 @@ def ring_drop(n_el=60, sdf_dtype=np.float32, tol=1e-8):
     return conf, params, inter
 
 
-def net_on_sphere(n=40, sdf_n=128, sdf_dtype=np.float32, tol=1e-8):
+def net_on_sphere(n=40, sdf_n=128, sdf_dtype=np.float32, tol=1e-8, gap=0.1):
     """Config 3: square net of n×n wire crossings at spacing 0.05 (2n(n-1) beams), r = 0.005,
-    E = 1e7, ν = 0.3, ρ = 1000, gravity as nodal loads, centred 0.1 above the top of a rigid
+    E = 1e7, ν = 0.3, ρ = 1000, gravity as nodal loads, centred `gap` = 0.1 above the top of a rigid
     sphere of radius 0.5 at the origin, given as φ = |x| - 0.5 sampled (FP32) on sdf_n³ over
     [-1.1, 1.1]² × [-0.6, 1.6]. μ = 0.2, kₙ = 1e5, εᵗ = 10, ηₙ = 0, Δt = 5e-4, 4000 steps."""
     h = 0.05
     xs = (np.arange(n) - (n - 1) / 2) * h
     X, Y = np.meshgrid(xs, xs, indexing="xy")
-    P = np.column_stack([X.reshape(-1), Y.reshape(-1), np.full(n * n, 0.6)])
+    P = np.column_stack([X.reshape(-1), Y.reshape(-1), np.full(n * n, 0.5 + gap)])
     idx = lambda i, j: j * n + i + 1
     conn = [(idx(i, j), idx(i + 1, j)) for j in range(n) for i in range(n - 1)]
     conn += [(idx(i, j), idx(i, j + 1)) for j in range(n - 1) for i in range(n)]

@@ -20,6 +20,9 @@ CASES = {
     "ring_sdf": (configs.ring_drop, 60),
     "contact_sphere": (lambda: configs.reference_test_contact_2(0.2), 20),
     "stent_sdf": (small_stent, 15),
+    # started at the surface: the contact forces are compared too
+    "ring_on_plane": (lambda: configs.ring_drop(height=0.06), 30),
+    "net_on_sphere": (lambda: configs.net_on_sphere(n=12, sdf_n=64, gap=0.006), 20),
 }
 
 
@@ -34,9 +37,12 @@ def test_fp32_form_trajectory_within_1e5_of_fp64_oracle(name):
     so = run_simulation(conf, params, inter, engine=o, max_steps=steps)
     assert sg.steps == so.steps == steps
     assert sg.failed_steps == so.failed_steps
-    errs = {f: rel_err(g.get(f), o.get(f), floor=1e-30) for f in ("U_N", "R_N", "UDOT_N")}
+    errs = {f: rel_err(g.get(f), o.get(f), floor=1e-30) for f in ("U_N", "R_N", "UDOT_N", "TC_N")}
     print(name, steps, errs, "newton/step fp32", sg.newton_per_step, "fp64 oracle", so.newton_per_step)
     for f in ("U_N", "R_N"):
         assert errs[f] <= 1e-5, (f, errs[f])
+    if name in ("ring_on_plane", "net_on_sphere"):
+        assert np.linalg.norm(o.get("TC_N")) > 0, "no contact"
+        assert errs["TC_N"] <= 1e-4, errs["TC_N"]
     # the FP32 tangent only changes the Newton path: the iteration counts stay close
     assert sg.newton_per_step <= so.newton_per_step + 2
