@@ -9,13 +9,16 @@ import numpy as np
 import pytest
 from scipy.spatial.transform import Rotation
 
+import os
+
 import oracle as O
-from conftest import rel_err
+from conftest import REPO, rel_err
 from endobeams_amd import HipEngine, configs, run_simulation
 
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
+GOLDEN = os.path.join(REPO, "tests", "golden")
 
 
 def small_stent():
@@ -33,6 +36,16 @@ def clamped_stent():
     return conf, params, inter
 
 
+def reference_net(sdf_n=None, drop=0.0):
+    """The reference's net (tests/golden: test/input/pos_net.txt, conn_net.txt) as its contact
+    tests set it up (test/test_beams2rigid_contact_1_{analytical,discrete}.jl), on the analytical
+    sphere or an SDF of it; `drop` lowers it towards the sphere (0.0049: touching)."""
+    pos = np.loadtxt(os.path.join(GOLDEN, "pos_net.txt"))
+    conn = np.loadtxt(os.path.join(GOLDEN, "conn_net.txt"), dtype=np.int64)
+    pos[:, 1] -= drop
+    return configs.reference_net(pos, conn, sdf_n=sdf_n)
+
+
 SETUPS = {
     "test_beams": lambda: configs.reference_test_beams(0.15),
     "contact_sphere": lambda: configs.reference_test_contact_2(0.2),
@@ -40,6 +53,8 @@ SETUPS = {
     "ring_sdf": configs.ring_drop,
     "stent_sdf": small_stent,
     "stent_clamped": clamped_stent,
+    "reference_net_sphere": lambda: reference_net(drop=0.0049),
+    "reference_net_sdf": lambda: reference_net(sdf_n=40, drop=0.0049),
 }
 
 
@@ -187,7 +202,8 @@ def test_updates_and_norms_parity(name):
 
 
 @pytest.mark.parametrize("name,steps", [("test_beams", 60), ("contact_sphere", 20), ("l_cantilever", 40),
-                                        ("ring_sdf", 60), ("stent_sdf", 15), ("stent_clamped", 10)])
+                                        ("ring_sdf", 60), ("stent_sdf", 15), ("stent_clamped", 10),
+                                        ("reference_net_sphere", 15), ("reference_net_sdf", 15)])
 def test_trajectory_parity(name, steps):
     conf, params, inter = SETUPS[name]()
     g = HipEngine(conf, params, inter)

@@ -40,3 +40,22 @@ def test_reference_contact_script_runs():
     # kₙ = 10 is a soft penalty: the driven wire indents the sphere by ~1.5 mm but is stopped well
     # before crossing it (the sphere radius is 40 mm)
     assert -0.005 < seen["gap"] < 0.0
+
+
+def test_reference_net_fixture_runs_on_the_oracle():
+    """The reference's net (tests/golden, from test/input/) in its contact-test setup, lowered onto
+    the sphere: the oracle converges every step and the contact forces are active."""
+    import os
+    import numpy as np
+    import oracle as O
+    from conftest import REPO
+    from endobeams_amd import configs, run_simulation
+    pos = np.loadtxt(os.path.join(REPO, "tests", "golden", "pos_net.txt"))
+    conn = np.loadtxt(os.path.join(REPO, "tests", "golden", "conn_net.txt"), dtype=np.int64)
+    assert pos.shape == (667, 3) and conn.shape == (786, 2) and conn.min() == 1 and conn.max() == 667
+    pos[:, 1] -= 0.0049
+    conf, params, inter = configs.reference_net(pos, conn)
+    o = O.OracleEngine(conf, params, inter)
+    st = run_simulation(conf, params, inter, engine=o, max_steps=5)
+    assert st.steps == 5 and st.failed_steps == 0
+    assert np.linalg.norm(o.get("TC_N")) > 0
