@@ -213,7 +213,7 @@ def small_config(args):
         print(f"# {args.workload} {name}: {forms[name]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     # the comparison starts the ring and the net at the surface, so that contact forces are compared
     near = {"cantilever": configs.l_cantilever, "ring": lambda: configs.ring_drop(height=0.06),
-            "net": lambda: configs.net_on_sphere(gap=0.003)}[args.workload]
+            "net": lambda: configs.net_on_sphere(gap=0.005)}[args.workload]
     conf, params, inter = near()
     ref = O.OracleEngine(conf, params, inter)
     st = Stepper(ref, params)
