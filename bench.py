@@ -265,7 +265,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=16, help="stents advanced together by the fast form")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="stents advanced together by the batched forms (0: picked by the batch-shape autotuner)")
+    ap.add_argument("--batch-candidates", default="32,64,128,256", help="sizes the batch-shape autotuner times")
     ap.add_argument("--sdf-n", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -313,6 +315,7 @@ def main():
     from endobeams_amd import learned as L
 
     plain = fast = lrn = None
+    batch_tuning = None
     xo = {"graph": args.exec == "graph", "autotune": not args.no_autotune}
     if args.only in (None, "plain"):
         conf, params, inter = stent(sdf_n=args.sdf_n)
@@ -324,8 +327,20 @@ def main():
     inst = None
     if args.only in (None, "fast", "learned"):
         t = time.perf_counter()
-        inst = [stent(seed=None if s == 0 else s, sdf_n=args.sdf_n) for s in range(args.batch)]
-        print(f"# batch of {args.batch} stents built in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+        cands = [int(c) for c in args.batch_candidates.split(",")] if args.batch == 0 else [args.batch]
+        inst = [stent(seed=None if s == 0 else s, sdf_n=args.sdf_n) for s in range(max(cands))]
+        print(f"# {len(inst)} seeded stents built in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+        if args.batch == 0:
+            from endobeams_amd import autotune
+            t = time.perf_counter()
+            size, rates, shapes = autotune.pick_batch_size(
+                inst, cands, warmup=args.warmup, steps=3,
+                log=lambda c, r: print(f"#   batch of {c}: {r:.0f} element-steps/s", file=sys.stderr, flush=True))
+            batch_tuning = {"candidates": {int(k): v for k, v in rates.items()}, "chosen": size,
+                            "tuning_s": time.perf_counter() - t}
+            args.batch = size
+            inst = inst[:size]
+        print(f"# batch of {args.batch} stents", file=sys.stderr, flush=True)
     if args.only in (None, "fast"):
         eng = HipBatchEngine(inst, precision=abi.EB_PRECISION_FP32)
         E = int(eng.beam_off[-1])
@@ -363,14 +378,14 @@ def main():
 
     main_form = lrn if lrn is not None else (fast if fast is not None else plain)
     traffic = None
-    if main_form is not plain and args.batch == 16 and args.sdf_n == 256:
-        # HBM bytes per launch of this kernel on this workload, from the committed FETCH_SIZE /
-        # WRITE_SIZE profiler runs (scaled by a device copy of known size)
+    if main_form is not plain and args.sdf_n == 256:
+        # HBM bytes per launch of this kernel, from the committed FETCH_SIZE / WRITE_SIZE profiler
+        # runs on 16 of these stents (scaled by a device copy of known size), per beam × beams
         tf = os.path.join(REPO, "profiles", "braided_stent_batch16_fp32_hbm_traffic.json")
         if os.path.exists(tf):
             k = json.load(open(tf))["kernels"].get("ebd::beam_detect_kernel<float>")
             if k:
-                traffic = k["fetch_bytes"] + k["write_bytes"]
+                traffic = (k["fetch_bytes"] + k["write_bytes"]) / (16 * 4800) * main_form["E"]
     avg_s = main_form["kernel_ms"] * 1e-3 / max(main_form["kernel_launches"], 1)
     flops = FLOPS_PER_BEAM * main_form["E"]
     achieved = flops / avg_s / 1e12
@@ -417,6 +432,8 @@ def main():
                                              "failed_steps": fast["failed_steps"]}
     if training is not None:
         out["learned_training"] = training
+    if batch_tuning is not None:
+        out["batch_autotune"] = batch_tuning
     if not args.no_cpu:
         cb, oracle_eng, n_cpu = cpu_baseline(args.cpu_seconds)
         out["cpu_baseline"] = cb

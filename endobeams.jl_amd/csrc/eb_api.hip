@@ -742,7 +742,7 @@ void apply_body_notes(eb_ctx* ctx, const eb_newton_params* p, double dt) {
 }
 
 void graph_release(eb_ctx* ctx) {
-    if (ctx->graph.exec) hipGraphExecDestroy(ctx->graph.exec);
+    if (ctx->graph.exec) (void)hipGraphExecDestroy(ctx->graph.exec);
     ctx->graph.exec = nullptr;
 }
 
@@ -773,12 +773,12 @@ int graph_iteration(eb_ctx* ctx, const eb_newton_params* p, double dt) {
         after_capture = body_notes(ctx);
         set_body_notes(ctx, before);
         if (st != EB_OK || e != hipSuccess) {
-            if (graph) hipGraphDestroy(graph);
+            if (graph) (void)hipGraphDestroy(graph);
             if (st == EB_OK) ctx->err = std::string("hipStreamEndCapture: ") + hipGetErrorString(e);
             return st != EB_OK ? st : EB_ERR_DEVICE;
         }
         const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         if (ei != hipSuccess) {
             g.exec = nullptr;
             ctx->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(ei);
@@ -1398,22 +1398,22 @@ int eb_create_batch(const eb_model_desc* descs, int64_t n_instances, eb_ctx** ou
 
 int eb_destroy(eb_ctx* ctx) {
     if (!ctx) return EB_OK;
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& p : ctx->ev_pending) { ctx->ev_pool.push_back(p.second.first); ctx->ev_pool.push_back(p.second.second); }
-    for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     graph_release(ctx);
     for (auto& pair : ctx->graph.ev)
         for (hipEvent_t e : pair)
-            if (e) hipEventDestroy(e);
-    for (void* p : ctx->allocs) hipFree(p);
-    if (ctx->h_kctl) hipHostFree(ctx->h_kctl);
-    if (ctx->h_norms) hipHostFree(ctx->h_norms);
-    if (ctx->h_status) hipHostFree(ctx->h_status);
-    if (ctx->h_kall) hipHostFree(ctx->h_kall);
-    if (ctx->h_nall) hipHostFree(ctx->h_nall);
-    if (ctx->h_pred) hipHostFree(ctx->h_pred);
-    if (ctx->h_kiter) hipHostFree(ctx->h_kiter);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+            if (e) (void)hipEventDestroy(e);
+    for (void* p : ctx->allocs) (void)hipFree(p);
+    if (ctx->h_kctl) (void)hipHostFree(ctx->h_kctl);
+    if (ctx->h_norms) (void)hipHostFree(ctx->h_norms);
+    if (ctx->h_status) (void)hipHostFree(ctx->h_status);
+    if (ctx->h_kall) (void)hipHostFree(ctx->h_kall);
+    if (ctx->h_nall) (void)hipHostFree(ctx->h_nall);
+    if (ctx->h_pred) (void)hipHostFree(ctx->h_pred);
+    if (ctx->h_kiter) (void)hipHostFree(ctx->h_kiter);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return EB_OK;
 }
