@@ -1153,6 +1153,17 @@ template <typename P> int upload_sdfs(eb_ctx* ctx, const eb_model_desc* ds, int6
             st = check_launch(ctx, "sdf_to_storage");
             h[s].coef = dp;
         }
+        // coarse lower bounds of the stored coefficients (the contact detection's early out)
+        P* cm = nullptr;
+        for (int a = 0; a < 3; ++a) h[s].cn[a] = (int32_t)((d->sdf_n[a] + SDF_BLOCK - 1) / SDF_BLOCK);
+        const int64_t ncm = (int64_t)h[s].cn[0] * h[s].cn[1] * h[s].cn[2];
+        if (st == EB_OK) st = dalloc(ctx, cm, ncm);
+        if (st == EB_OK) {
+            hipLaunchKernelGGL(sdf_block_min<P>, dim3(nblk(ncm, 256)), dim3(256), 0, ctx->stream, g, SDF_BLOCK,
+                               (const P*)dp, h[s].cn[0], h[s].cn[1], h[s].cn[2], cm);
+            st = check_launch(ctx, "sdf_block_min");
+            h[s].cmin = cm;
+        }
         // the scratch is reused by the next instance
         if (st == EB_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) st = EB_ERR_DEVICE;
     }

@@ -15,12 +15,29 @@ namespace ebd {
 
 // Quadratic B-spline SDF with Reflect(OnCell()) prefiltered coefficients, padded by one cell
 // (src/interactions/def_interactions.jl:67-82).
+// Coarse lower bound of the interpolant: cell c of axis a holds the stencil centres (1-based
+// knots) B·c + 1 … B·c + B, and its entry the minimum of the coefficients any of their stencils
+// reads (padded indices B·c … B·c + B + 1), so that φ(x) ≥ lower bound for every x whose stencil
+// centre falls in the cell (the quadratic B-spline weights are non-negative and sum to one).
+constexpr int SDF_BLOCK = 8;
+
 template <typename P> struct SDFDev {
     const P* coef;       // (n0+2)(n1+2)(n2+2), x fastest
     int32_t n[3];
     double d[3], x0[3];
     double dom[6];
+    const P* cmin;       // coarse lower bounds, cn0·cn1·cn2, x fastest
+    int32_t cn[3];       // ceil(n / SDF_BLOCK)
 };
+
+// stencil centre (1-based knot) of coordinate x along an axis, as sdf_eval computes it
+EBD int sdf_knot(double x, double x0, double d, int32_t n) {
+    double t = (x - x0) / d + 1.0;
+    double r = floor(t + 0.5);
+    r = r < 1.0 ? 1.0 : r;
+    r = r > double(n) ? double(n) : r;
+    return int(r);
+}
 
 struct SphereDev { double c[3], R; };
 
@@ -105,6 +122,13 @@ template <typename P> EBD bool in_contact(const ContactArgs<P>& ca, int32_t inst
     }
     const SDFDev<P>& sdf = ca.sdf[inst];
     if (!isinside_dom(p, sdf.dom)) return false;
+    // far from the surface: the coarse lower bound already exceeds the contact threshold (with a
+    // margin far above the rounding of the 27-term sum), so the full stencil would say the same
+    const int c0 = (sdf_knot(p.x, sdf.x0[0], sdf.d[0], sdf.n[0]) - 1) / SDF_BLOCK;
+    const int c1 = (sdf_knot(p.y, sdf.x0[1], sdf.d[1], sdf.n[1]) - 1) / SDF_BLOCK;
+    const int c2 = (sdf_knot(p.z, sdf.x0[2], sdf.d[2], sdf.n[2]) - 1) / SDF_BLOCK;
+    const double lb = double(sdf.cmin[((int64_t)c2 * sdf.cn[1] + c1) * sdf.cn[0] + c0]);
+    if (lb > 0 && lb * (1 - 1e-12) - rb > gbar + 1e-12 * (lb + rb)) return false;
     double v;
     sdf_eval<double, P>(sdf, p, v, nullptr, nullptr);
     return v - rb <= gbar;

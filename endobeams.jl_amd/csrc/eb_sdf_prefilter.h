@@ -62,6 +62,29 @@ __global__ void __launch_bounds__(256) sdf_prefilter_z(SdfGrid g, TriLU t, doubl
     tri_solve_line(t, base + P1 * P0, P1 * P0, base, P1 * P0);
 }
 
+// coarse lower bounds: one lane per coarse cell, the minimum of the stored coefficients over the
+// padded indices block·c … block·c + block + 1 of each axis (clipped to the padded grid)
+template <typename P>
+__global__ void __launch_bounds__(256) sdf_block_min(SdfGrid g, int block, const P* __restrict__ coef, int32_t cn0,
+                                                     int32_t cn1, int32_t cn2, P* cmin) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (int64_t)cn0 * cn1 * cn2) return;
+    const int64_t c0 = q % cn0, c1 = (q / cn0) % cn1, c2 = q / ((int64_t)cn0 * cn1);
+    const int64_t P0 = g.n0 + 2, P1 = g.n1 + 2, P2 = g.n2 + 2;
+    const int64_t i0 = block * c0, j0 = block * c1, k0 = block * c2;
+    const int64_t i1 = i0 + block + 1 < P0 - 1 ? i0 + block + 1 : P0 - 1;
+    const int64_t j1 = j0 + block + 1 < P1 - 1 ? j0 + block + 1 : P1 - 1;
+    const int64_t k1 = k0 + block + 1 < P2 - 1 ? k0 + block + 1 : P2 - 1;
+    P m = coef[(k0 * P1 + j0) * P0 + i0];
+    for (int64_t k = k0; k <= k1; ++k)
+        for (int64_t j = j0; j <= j1; ++j)
+            for (int64_t i = i0; i <= i1; ++i) {
+                const P v = coef[(k * P1 + j) * P0 + i];
+                m = v < m ? v : m;
+            }
+    cmin[q] = m;
+}
+
 template <typename P>
 __global__ void __launch_bounds__(256) sdf_to_storage(int64_t n, const double* __restrict__ src, P* dst) {
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
