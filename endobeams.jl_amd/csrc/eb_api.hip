@@ -317,7 +317,7 @@ int check_launch(eb_ctx* ctx, const char* what) {
 template <typename P> int materialize_kcm_t(eb_ctx* ctx) {
     ctx->kcm_ver = ctx->stage_ver;
     const int bs = gather_block(ctx);
-    hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz, bs)), dim3(bs), 0, ctx->stream, ctx->nnz, ctx->blk_ptr,
+    hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz / 4, bs)), dim3(bs), 0, ctx->stream, ctx->nnz, ctx->blk_ptr,
                        ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM, ctx->K, ctx->C, ctx->M,
                        ctx->Ktan, 0.0, 0.0, 0.0, 0, 1, active_dev(ctx));
     return check_launch(ctx, "gather_matrices");
@@ -416,7 +416,7 @@ template <typename P> int launch_gathers(eb_ctx* ctx, bool with_contact, bool ta
         // element staging when they are first read or when the step is committed (materialize_kcm)
         // (K, C, M are passed only when written: their buffers are exchanged when a step is committed)
         const int bs = gather_block(ctx);
-        hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz, bs)), dim3(bs), 0, ctx->stream, ctx->nnz,
+        hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz / 4, bs)), dim3(bs), 0, ctx->stream, ctx->nnz,
                            ctx->blk_ptr, ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM,
                            tangent ? nullptr : ctx->K, tangent ? nullptr : ctx->C, tangent ? nullptr : ctx->M, ctx->Ktan,
                            c1, c2, c3, tangent ? 1 : 0, tangent ? 0 : 1, active_dev(ctx));
@@ -853,12 +853,12 @@ int autotune_impl(eb_ctx* ctx, int32_t* choice, double* ms_out) {
                 const eb_ctx::TanTag t = ctx->ktan_tag;
                 st = time_launches(ctx, reps, [&] {
                     if (ctx->precision == EB_PRECISION_FP32)
-                        hipLaunchKernelGGL(gather_matrices<float>, dim3(nblk(ctx->nnz, bs)), dim3(bs), 0, ctx->stream, ctx->nnz,
+                        hipLaunchKernelGGL(gather_matrices<float>, dim3(nblk(ctx->nnz / 4, bs)), dim3(bs), 0, ctx->stream, ctx->nnz,
                                            ctx->blk_ptr, ctx->blk_src, (const float*)ctx->sK, (const float*)ctx->sC,
                                            (const float*)ctx->sM, nullptr, nullptr, nullptr, ctx->tmp, t.c1, t.c2, t.c3, 1, 0,
                                            active_dev(ctx));
                     else
-                        hipLaunchKernelGGL(gather_matrices<double>, dim3(nblk(ctx->nnz, bs)), dim3(bs), 0, ctx->stream, ctx->nnz,
+                        hipLaunchKernelGGL(gather_matrices<double>, dim3(nblk(ctx->nnz / 4, bs)), dim3(bs), 0, ctx->stream, ctx->nnz,
                                            ctx->blk_ptr, ctx->blk_src, (const double*)ctx->sK, (const double*)ctx->sC,
                                            (const double*)ctx->sM, nullptr, nullptr, nullptr, ctx->tmp, t.c1, t.c2, t.c3, 1, 0,
                                            active_dev(ctx));

@@ -132,35 +132,63 @@ struct Active {
 
 // ---- assembly gathers (atomic-free, beam order = the reference's serial scatter order) ------
 
-// K, C, M and/or Ktan = (1+α)K + γ/(βΔt) C + M/(βΔt²) for every block-row entry
+// four consecutive staging values (16-byte aligned: a block's 36 entries start on a multiple of
+// four) widened to double
+EBD void ld4w(const float* __restrict__ p, double (&o)[4]) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    o[0] = double(v.x); o[1] = double(v.y); o[2] = double(v.z); o[3] = double(v.w);
+}
+EBD void ld4w(const double* __restrict__ p, double (&o)[4]) {
+    const double2 v0 = *reinterpret_cast<const double2*>(p), v1 = *reinterpret_cast<const double2*>(p + 2);
+    o[0] = v0.x; o[1] = v0.y; o[2] = v1.x; o[3] = v1.y;
+}
+EBD void st4(double* p, const double (&v)[4]) {
+    *reinterpret_cast<double2*>(p) = double2{v[0], v[1]};
+    *reinterpret_cast<double2*>(p + 2) = double2{v[2], v[3]};
+}
+
+// K, C, M and/or Ktan = (1+α)K + γ/(βΔt) C + M/(βΔt²) for every block-row entry; one lane per
+// four consecutive entries of a 6×6 block (nnz/4 lanes), each entry summed over the block's
+// contributions in beam order
 template <typename P>
 __global__ void __launch_bounds__(256) gather_matrices(int64_t nnz, const int32_t* __restrict__ blk_ptr,
                                                        const uint32_t* __restrict__ blk_src, const P* __restrict__ sK,
                                                        const P* __restrict__ sC, const P* __restrict__ sM, double* K,
                                                        double* C, double* M, double* Ktan, double c1, double c2,
                                                        double c3, int with_tangent, int write_kcm, Active act) {
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t q = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= nnz) return;
     const int64_t b = q / 36;
     if (!act.flag[act.blk_inst[b]]) return;
-    const int r = (int)(q % 36);
-    const int a = r / 6, c = r % 6;
-    double k = 0, cc = 0, m = 0;
+    const int r = (int)(q % 36);   // row-major position 6a + c in the block, a multiple of four
+    double k[4] = {0, 0, 0, 0}, cc[4] = {0, 0, 0, 0}, m[4] = {0, 0, 0, 0};
     for (int s = blk_ptr[b]; s < blk_ptr[b + 1]; ++s) {
-        uint32_t src = blk_src[s];
-        int64_t e = src >> 2;
-        int sr = (src >> 1) & 1, sc = src & 1;
-        const int64_t idx = e * 144 + (2 * sr + sc) * 36 + 6 * a + c;   // stage_idx(e, 6 sr + a, 6 sc + c)
-        k += double(sK[idx]);
-        cc += double(sC[idx]);
-        m += double(sM[idx]);
+        const uint32_t src = blk_src[s];
+        const int64_t e = src >> 2;
+        const int sr = (src >> 1) & 1, sc = src & 1;
+        const int64_t idx = e * 144 + (2 * sr + sc) * 36 + r;   // stage_idx(e, 6 sr + a, 6 sc + c)
+        double vk[4], vc[4], vm[4];
+        ld4w(sK + idx, vk);
+        ld4w(sC + idx, vc);
+        ld4w(sM + idx, vm);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            k[j] += vk[j];
+            cc[j] += vc[j];
+            m[j] += vm[j];
+        }
     }
     if (write_kcm) {
-        K[q] = k;
-        C[q] = cc;
-        M[q] = m;
+        st4(K + q, k);
+        st4(C + q, cc);
+        st4(M + q, m);
     }
-    if (with_tangent) Ktan[q] = (c1 * k + c2 * cc) + c3 * m;
+    if (with_tangent) {
+        double t[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t[j] = (c1 * k[j] + c2 * cc[j]) + c3 * m[j];
+        st4(Ktan + q, t);
+    }
 }
 
 // Tⁱⁿᵗ, Tᵏ, Tᶜ per dof, optionally followed by the corrector residual (utils_solver.jl:87)
