@@ -20,12 +20,16 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include")]
 
-SOURCES = ["eb_api.hip", "eb_mlp.hip", "eb_setup.cpp"]
+SOURCES = ["eb_api.hip", "eb_element.hip", "eb_mlp.hip", "eb_setup.cpp"]
+
+# the element and contact kernels fold their structurally zero terms (why the results stay the same
+# is explained at the top of eb_element.hip); no reassociation, no contraction
+EXTRA = {"eb_element.hip": ["-fno-signed-zeros", "-ffinite-math-only"]}
 
 
 def _compile(src: str) -> str:
     obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *COMMON, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *COMMON, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     if src.endswith(".cpp"):
         cmd = [HIPCC, "-x", "hip", *COMMON, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -15,6 +15,7 @@
 #include "../../include/endobeams_learned.h"
 #include "eb_beam_kernel.h"
 #include "eb_contact_kernel.h"
+#include "eb_element.h"
 #include "eb_learned.h"
 #include "eb_linalg.h"
 #include "eb_mlp.h"
@@ -348,8 +349,8 @@ template <typename P> int launch_beams(eb_ctx* ctx) {
     ctx->start_assembled = false;
     ctx->stage_ver++;
     prof_begin(ctx, EB_KERNEL_BEAM);
-    hipLaunchKernelGGL(beam_kernel<P>, dim3(nblk(ctx->E, BEAM_TILE)), dim3(4 * BEAM_TILE), 0, ctx->stream, node_state(ctx),
-                       beam_consts(ctx), beam_stage<P>(ctx), ctx->gauss, ctx->active, ctx->beam_inst);
+    launch_beam_kernel<P>(nblk(ctx->E, BEAM_TILE), ctx->stream, node_state(ctx), beam_consts(ctx), beam_stage<P>(ctx),
+                          ctx->gauss, ctx->active, ctx->beam_inst);
     prof_end(ctx, EB_KERNEL_BEAM);
     ctx->counters[3]++;
     return check_launch(ctx, "beam_kernel");
@@ -372,11 +373,13 @@ template <typename P> ContactArgs<P> contact_args(eb_ctx* ctx, double alpha) {
 
 template <typename P> int launch_contact_tangent(eb_ctx* ctx, const ContactArgs<P>& ca) {
     prof_begin(ctx, EB_KERNEL_CONTACT);
-    hipLaunchKernelGGL(contact_kernel<P>, dim3(nblk(ctx->n_contact, CONTACT_TILE)), dim3(4 * CONTACT_TILE), 0, ctx->stream,
-                       node_state(ctx), beam_consts(ctx), beam_stage<P>(ctx), ctx->gauss, ca, ctx->clist, ctx->clist_n);
+    launch_contact_kernel<P>(nblk(ctx->n_contact, CONTACT_TILE), ctx->stream, node_state(ctx), beam_consts(ctx),
+                             beam_stage<P>(ctx), ctx->gauss, ca, ctx->clist, ctx->clist_n);
     prof_end(ctx, EB_KERNEL_CONTACT);
     return check_launch(ctx, "contact_kernel");
 }
+
+__global__ void contact_reset(int32_t* count) { *count = 0; }
 
 // element kernel with the contact detection in its static part, then the contact tangent
 template <typename P> int launch_beams_contact(eb_ctx* ctx, double alpha) {
@@ -385,9 +388,9 @@ template <typename P> int launch_beams_contact(eb_ctx* ctx, double alpha) {
     ctx->stage_ver++;
     hipLaunchKernelGGL(contact_reset, dim3(1), dim3(1), 0, ctx->stream, ctx->clist_n);
     prof_begin(ctx, EB_KERNEL_BEAM);
-    hipLaunchKernelGGL(beam_detect_kernel<P>, dim3(nblk(ctx->E, BEAM_TILE)), dim3(4 * BEAM_TILE), 0, ctx->stream,
-                       node_state(ctx), beam_consts(ctx), beam_stage<P>(ctx), ctx->gauss, ctx->active, ctx->beam_inst, ca,
-                       ctx->is_contact, ctx->clist, ctx->clist_n);
+    launch_beam_detect_kernel<P>(nblk(ctx->E, BEAM_TILE), ctx->stream, node_state(ctx), beam_consts(ctx),
+                                 beam_stage<P>(ctx), ctx->gauss, ctx->active, ctx->beam_inst, ca, ctx->is_contact,
+                                 ctx->clist, ctx->clist_n);
     prof_end(ctx, EB_KERNEL_BEAM);
     ctx->counters[3]++;
     EB_TRY(check_launch(ctx, "beam_detect_kernel"));
@@ -400,8 +403,8 @@ template <typename P> int launch_contact(eb_ctx* ctx, double alpha) {
     const ContactArgs<P> ca = contact_args<P>(ctx, alpha);
     ctx->stage_ver++;
     hipLaunchKernelGGL(contact_reset, dim3(1), dim3(1), 0, ctx->stream, ctx->clist_n);
-    hipLaunchKernelGGL(contact_detect<P>, dim3(nblk(ctx->n_contact, 256)), dim3(256), 0, ctx->stream, node_state(ctx),
-                       beam_consts(ctx), ctx->gauss, ca, ctx->clist, ctx->clist_n);
+    launch_contact_detect<P>(nblk(ctx->n_contact, 256), ctx->stream, node_state(ctx), beam_consts(ctx), ctx->gauss, ca,
+                             ctx->clist, ctx->clist_n);
     EB_TRY(check_launch(ctx, "contact_detect"));
     return launch_contact_tangent<P>(ctx, ca);
 }

@@ -391,8 +391,6 @@ __global__ void __launch_bounds__(256) contact_detect(NodeState ns, BeamConsts b
     list[slot] = (uint64_t(e) << 8) | mask;
 }
 
-__global__ void contact_reset(int32_t* count) { *count = 0; }
-
 // The same detection as a hook of the element kernel: the wavefront of the Coriolis columns 2 and 4
 // runs it between the element's frame (x₁, x₂, Rₑ, Θ̅₁, Θ̅₂) and its Gauss loop, so the assembly
 // with contact needs no separate detection pass.

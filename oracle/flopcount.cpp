@@ -2,14 +2,16 @@
 //
 // Instantiates eb_elements.h with a scalar that counts every +, -, ×, ÷ (FP64 arithmetic of the
 // restated expression DAG, so also products with the structural zeros that dense 3x3 algebra
-// carries) and, separately, sqrt and the transcendentals. The beam count is the algorithmic
+// carries) and, separately, sqrt and the transcendentals. It also counts the operations left
+// once exact zeros are folded (x·0, x ± 0, 0 / x, the work of a kernel compiled to drop them): on
+// a generic element state every exact zero is a structural one. That count is the algorithmic
 // flop figure the benchmark's roofline divides by the element kernel's launch time.
 #include <cmath>
 #include <cstdint>
 #include <type_traits>
 
 namespace flopcount {
-struct Counts { long long flops = 0, sqrt = 0, trans = 0; };
+struct Counts { long long flops = 0, sqrt = 0, trans = 0, nonzero = 0; };
 static Counts g;
 
 struct C {
@@ -20,12 +22,13 @@ struct C {
     explicit operator double() const { return v; }
     explicit operator int64_t() const { return (int64_t)v; }
     C operator-() const { return C(-v); }
-    C& operator+=(const C& o) { ++g.flops; v += o.v; return *this; }
+    C& operator+=(const C& o) { ++g.flops; g.nonzero += (v != 0 && o.v != 0); v += o.v; return *this; }
 };
-inline C operator+(const C& a, const C& b) { ++g.flops; return C(a.v + b.v); }
-inline C operator-(const C& a, const C& b) { ++g.flops; return C(a.v - b.v); }
-inline C operator*(const C& a, const C& b) { ++g.flops; return C(a.v * b.v); }
-inline C operator/(const C& a, const C& b) { ++g.flops; return C(a.v / b.v); }
+inline void both(const C& a, const C& b) { ++g.flops; g.nonzero += (a.v != 0 && b.v != 0); }
+inline C operator+(const C& a, const C& b) { both(a, b); return C(a.v + b.v); }
+inline C operator-(const C& a, const C& b) { both(a, b); return C(a.v - b.v); }
+inline C operator*(const C& a, const C& b) { both(a, b); return C(a.v * b.v); }
+inline C operator/(const C& a, const C& b) { ++g.flops; g.nonzero += (a.v != 0); return C(a.v / b.v); }
 template <typename T> inline C operator+(const C& a, T b) { return a + C(b); }
 template <typename T> inline C operator+(T a, const C& b) { return C(a) + b; }
 template <typename T> inline C operator-(const C& a, T b) { return a - C(b); }
@@ -68,9 +71,9 @@ ElemIn unpack(const double* x) {
 }  // namespace
 
 extern "C" {
-// counts for one compute_beams call: out3 = {flops, sqrt, transcendentals}
+// counts for one compute_beams call: out4 = {flops, sqrt, transcendentals, flops without exact zeros}
 int ebo_count_beam(const double* elem82, const double* props14, int n_gauss, const double* gw, const double* gz,
-                   long long* out3) {
+                   long long* out4) {
     ElemIn in = unpack(elem82);
     flopcount::C p[14];
     for (int k = 0; k < 14; ++k) p[k] = props14[k];
@@ -81,7 +84,8 @@ int ebo_count_beam(const double* elem82, const double* props14, int n_gauss, con
     BeamOut o;
     flopcount::g = flopcount::Counts();
     compute_beams(in, bp, g, o);
-    out3[0] = flopcount::g.flops; out3[1] = flopcount::g.sqrt; out3[2] = flopcount::g.trans;
+    out4[0] = flopcount::g.flops; out4[1] = flopcount::g.sqrt; out4[2] = flopcount::g.trans;
+    out4[3] = flopcount::g.nonzero;
     return 0;
 }
 }
