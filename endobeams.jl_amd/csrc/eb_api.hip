@@ -504,12 +504,16 @@ int precond_factor(eb_ctx* ctx) {
 int precond_apply(eb_ctx* ctx, const double* b, double* x) {
     int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
     const size_t smem = tuned(ctx, EB_TUNE_CHAIN_SOLVE_LDS) == 1 ? chain_solve_lds_bytes(ctx->chains.max_len) : 0;
-    if (smem)
-        hipLaunchKernelGGL(chain_solve<true>, dim3((unsigned)nch), dim3(SOLVE_THREADS), smem, ctx->stream, chain_dev(ctx), b,
-                           x, ctx->kctl);
+    const bool many = nch >= SOLVE_MANY_CHAINS;
+    const dim3 grid((unsigned)nch), block(many ? SOLVE_THREADS_MANY : SOLVE_THREADS_FEW);
+    if (smem && many)
+        hipLaunchKernelGGL((chain_solve<true, SOLVE_THREADS_MANY>), grid, block, smem, ctx->stream, chain_dev(ctx), b, x, ctx->kctl);
+    else if (smem)
+        hipLaunchKernelGGL((chain_solve<true, SOLVE_THREADS_FEW>), grid, block, smem, ctx->stream, chain_dev(ctx), b, x, ctx->kctl);
+    else if (many)
+        hipLaunchKernelGGL((chain_solve<false, SOLVE_THREADS_MANY>), grid, block, 0, ctx->stream, chain_dev(ctx), b, x, ctx->kctl);
     else
-        hipLaunchKernelGGL(chain_solve<false>, dim3((unsigned)nch), dim3(SOLVE_THREADS), 0, ctx->stream, chain_dev(ctx), b, x,
-                           ctx->kctl);
+        hipLaunchKernelGGL((chain_solve<false, SOLVE_THREADS_FEW>), grid, block, 0, ctx->stream, chain_dev(ctx), b, x, ctx->kctl);
     return check_launch(ctx, "chain_solve");
 }
 
@@ -837,6 +841,12 @@ int autotune_impl(eb_ctx* ctx, int32_t* choice, double* ms_out) {
     EB_TRY(materialize_kcm(ctx));   // the gather timing reads the staging; K, C, M stay as they are
     graph_release(ctx);
     const int32_t saved[EB_TUNE_COUNT] = {ctx->tune[0], ctx->tune[1], ctx->tune[2]};
+    // the chain solve skips the chains of finished solves, and after a step every solve is
+    // finished: the solve is timed with every instance's control record cleared (not done), and
+    // the records are put back afterwards
+    std::vector<KrylovCtl> kc_saved((size_t)ctx->S);
+    EB_HIP(hipMemcpyAsync(kc_saved.data(), ctx->kctl, sizeof(KrylovCtl) * ctx->S, hipMemcpyDeviceToHost, ctx->stream));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
     int st = EB_OK;
     for (int id = 0; id < EB_TUNE_COUNT && st == EB_OK; ++id) {
         double best = 1e30;
@@ -850,6 +860,7 @@ int autotune_impl(eb_ctx* ctx, int32_t* choice, double* ms_out) {
             if (id == EB_TUNE_CHAIN_LDS_KB) {
                 st = time_launches(ctx, reps, [&] { return precond_factor(ctx); }, &ms);
             } else if (id == EB_TUNE_CHAIN_SOLVE_LDS) {
+                EB_HIP(hipMemsetAsync(ctx->kctl, 0, sizeof(KrylovCtl) * ctx->S, ctx->stream));
                 st = time_launches(ctx, reps, [&] { return precond_apply(ctx, ctx->res, ctx->ky); }, &ms);
             } else {
                 const int bs = v;
@@ -874,6 +885,8 @@ int autotune_impl(eb_ctx* ctx, int32_t* choice, double* ms_out) {
         ctx->tune[id] = pick;
     }
     ctx->factor_tag = eb_ctx::TanTag{};   // the factors were overwritten by the timed launches
+    EB_HIP(hipMemcpyAsync(ctx->kctl, kc_saved.data(), sizeof(KrylovCtl) * ctx->S, hipMemcpyHostToDevice, ctx->stream));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
     if (st != EB_OK) {
         for (int id = 0; id < EB_TUNE_COUNT; ++id) ctx->tune[id] = saved[id];
         return st;

@@ -376,7 +376,10 @@ __global__ void __launch_bounds__(RED_BLOCK) spmv_masked(Segs sg, const int32_t*
 constexpr int CHAIN_THREADS = 256;                       // factorisation workgroup
 constexpr int CHAIN_GROUPS = (CHAIN_THREADS / 64) * 10;  // six-lane groups: ten per wavefront
 constexpr int GJ_BUF = 24;                               // doubles of LDS exchange per group
-constexpr int SOLVE_THREADS = 1024;                      // solve workgroup
+// solve workgroup: 1024 threads while the chains are few (each chain's latency sets the time), 256
+// once there are enough chains to fill the chip several times over (more chains per CU)
+constexpr int SOLVE_THREADS_FEW = 1024, SOLVE_THREADS_MANY = 256;
+constexpr int64_t SOLVE_MANY_CHAINS = 1024;
 
 struct ChainDev {
     const int32_t* ptr;   // n_chains+1
@@ -745,7 +748,7 @@ inline size_t chain_solve_lds_bytes(int64_t max_len) {
 // level by level (y_j = D_j^-1 b_j for eliminated j; b_i -= L_i y_j1 + U_i y_j2 for kept i with the
 // saved blocks), then solved back from position 0 (x_j = y_j - GL_j x_{j-h} - GU_j x_{j+h}).
 // Six lanes per position, one per row; the work vectors in LDS (LDSV) or in the global arrays.
-template <bool LDSV>
+template <bool LDSV, int SOLVE_THREADS>
 __global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const double* __restrict__ b,
                                                              double* __restrict__ x, const KrylovCtl* kc) {
     extern __shared__ __align__(16) double solve_smem[];

@@ -400,15 +400,23 @@ class BatchEngine(Engine):
 
 
 _HIP_LIB = None
+_HIP_LIB_PATH = None
 
 
 def load_hip_library(path: str = LIB_PATH) -> C.CDLL:
-    """Loads libendobeams_hip.so; raises if it has not been built (there is no CPU fallback)."""
-    global _HIP_LIB
+    """Loads libendobeams_hip.so; raises if it has not been built (there is no CPU fallback).
+    One build per process: two builds of the library export the same kernel and launcher
+    symbols, and the dynamic linker would bind the second one's launches to the first one's
+    kernels, so asking for a second path raises instead of silently running the first."""
+    global _HIP_LIB, _HIP_LIB_PATH
+    real = os.path.realpath(path)
     if _HIP_LIB is None:
         if not os.path.exists(path):
             raise EBError(f"{path} is missing: build it with __graft_entry__.build() (no CPU fallback exists)")
         _HIP_LIB = C.CDLL(path, mode=C.RTLD_GLOBAL)
+        _HIP_LIB_PATH = real
+    elif real != _HIP_LIB_PATH:
+        raise EBError(f"{path}: this process already runs {_HIP_LIB_PATH}; load another build in another process")
     return _HIP_LIB
 
 

@@ -49,3 +49,22 @@ def test_python_bindings_cover_the_abi():
     assert bound == set(declared())
     assert {"eb_" + k for k in abi.LEARNED_PROTOTYPES} == set(declared(LEARNED))
     assert len(declared(LEARNED)) == 14
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_a_second_build_in_one_process_is_refused(tmp_path):
+    # two builds export the same kernel and launcher symbols; the loader refuses the second path
+    # instead of silently binding it to the first (run in a child so this process stays clean)
+    import shutil
+    import subprocess
+    import sys
+    other = tmp_path / "libendobeams_hip.so"
+    shutil.copy(LIB, other)
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from endobeams_amd.engine import load_hip_library, EBError\n"
+            "a = load_hip_library(); assert load_hip_library() is a\n"
+            "try:\n    load_hip_library(%r)\nexcept EBError as e:\n    print('refused', e)\n") % (
+        os.path.join(REPO, "endobeams.jl_amd"), str(other))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "refused" in out.stdout
