@@ -100,6 +100,7 @@ struct eb_ctx {
     uint8_t* is_fixed = nullptr;
     // matrices (block-row order) and vectors
     double *K = nullptr, *C = nullptr, *M = nullptr, *Kn = nullptr, *Cn = nullptr, *Mn = nullptr, *Ktan = nullptr;
+    int64_t chain_lds_max = 0;   // dynamic LDS the chain factorisation may take (bytes)
     bool np1_alias = false;  // K,C,M at n+1 currently equal the step-n matrices (after a swap)
     bool kcm_pending = false; // K,C,M at n+1 not yet gathered from the element staging of the last assembly
     // What the tangent, the chain factors and the matrices hold, so that a predictor whose tangent
@@ -491,9 +492,14 @@ int precond_factor(eb_ctx* ctx) {
     int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
     prof_begin(ctx, EB_KERNEL_PRECOND);
     // the reduced chain stays in LDS once it fits in the budget (48 KB by default: with the
-    // kernel's 8 KB of static LDS, under the 64 KB of a workgroup, two workgroups per CU)
+    // kernel's 8 KB of static LDS, two workgroups per CU; 96 KB: one workgroup per CU, the chain
+    // in LDS one level earlier). Budgets beyond 64 KB of dynamic LDS were enabled at eb_create.
     const int L0 = chain_lds_level(ctx->chains.max_len, 1024 * (int64_t)tuned(ctx, EB_TUNE_CHAIN_LDS_KB));
     const size_t smem = chain_lds_bytes(ctx->chains.max_len, L0);
+    if (smem > (size_t)ctx->chain_lds_max) {
+        ctx->err = "chain factorisation: the LDS budget exceeds what the device allows";
+        return EB_ERR_ARG;
+    }
     hipLaunchKernelGGL(chain_factor, dim3((unsigned)nch), dim3(CHAIN_THREADS), smem, ctx->stream, chain_dev(ctx), ctx->Ktan,
                        &ctx->kall->pivot_status, L0, ctx->active);
     ctx->factor_tag = ctx->ktan_tag;
@@ -833,7 +839,7 @@ template <typename F> int time_launches(eb_ctx* ctx, int reps, F&& launch, doubl
 int autotune_impl(eb_ctx* ctx, int32_t* choice, double* ms_out) {
     if (ctx->ktan_tag.ver == 0) { ctx->err = "eb_autotune needs a tangent: call it after a time step"; return EB_ERR_STATE; }
     const int reps = 5;
-    static const int32_t cand_lds[EB_TUNE_CANDIDATES] = {8, 16, 32, 48};
+    static const int32_t cand_lds[EB_TUNE_CANDIDATES] = {16, 32, 48, 96};
     static const int32_t cand_solve[EB_TUNE_CANDIDATES] = {1, 2, 0, 0};
     static const int32_t cand_gather[EB_TUNE_CANDIDATES] = {64, 128, 256, 0};
     const int32_t* cands[EB_TUNE_COUNT] = {cand_lds, cand_solve, cand_gather};
@@ -855,6 +861,10 @@ int autotune_impl(eb_ctx* ctx, int32_t* choice, double* ms_out) {
             const int32_t v = cands[id][c];
             if (v == 0) continue;
             if (id == EB_TUNE_CHAIN_SOLVE_LDS && v == 1 && chain_solve_lds_bytes(ctx->chains.max_len) == 0) continue;
+            if (id == EB_TUNE_CHAIN_LDS_KB &&
+                chain_lds_bytes(ctx->chains.max_len, chain_lds_level(ctx->chains.max_len, 1024 * (int64_t)v)) >
+                    (size_t)ctx->chain_lds_max)
+                continue;
             ctx->tune[id] = v;
             double ms = 0;
             if (id == EB_TUNE_CHAIN_LDS_KB) {
@@ -1216,6 +1226,23 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
     auto bail = [&](int code) { eb_destroy(ctx); return code; };
     if (hipGetDevice(&ctx->device) != hipSuccess) return bail(EB_ERR_DEVICE);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return bail(EB_ERR_DEVICE);
+    {
+        // dynamic LDS for the chain factorisation: what a workgroup may hold beside the kernel's
+        // static LDS, asked for explicitly beyond the 64 KB every device grants
+        int max_lds = 0;
+        hipFuncAttributes fa;
+        if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess ||
+            hipFuncGetAttributes(&fa, (const void*)chain_factor) != hipSuccess)
+            return bail(EB_ERR_DEVICE);
+        const int64_t dflt = 65536 - (int64_t)fa.sharedSizeBytes;
+        ctx->chain_lds_max = std::max<int64_t>(dflt, (int64_t)max_lds - (int64_t)fa.sharedSizeBytes);
+        if (ctx->chain_lds_max > dflt &&
+            hipFuncSetAttribute((const void*)chain_factor, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)ctx->chain_lds_max) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->chain_lds_max = dflt;
+        }
+    }
     ctx->precision = d->precision;
     std::string err;
     if (!ebh::build_pattern(ctx->pat, d->n_nodes, d->n_beams, d->conn, d->n_fixed, d->fixed_dofs, err)) {
@@ -2031,7 +2058,7 @@ int eb_autotune(eb_ctx* ctx, int32_t* choice, double* ms_per_candidate) {
 
 int eb_set_tuning(eb_ctx* ctx, const int32_t* choice) {
     if (!ctx || !choice) return EB_ERR_ARG;
-    if (choice[EB_TUNE_CHAIN_LDS_KB] != 0 && (choice[EB_TUNE_CHAIN_LDS_KB] < 1 || choice[EB_TUNE_CHAIN_LDS_KB] > 48))
+    if (choice[EB_TUNE_CHAIN_LDS_KB] != 0 && (choice[EB_TUNE_CHAIN_LDS_KB] < 1 || choice[EB_TUNE_CHAIN_LDS_KB] > 96))
         return EB_ERR_ARG;
     if (choice[EB_TUNE_CHAIN_SOLVE_LDS] < 0 || choice[EB_TUNE_CHAIN_SOLVE_LDS] > 2) return EB_ERR_ARG;
     const int32_t g = choice[EB_TUNE_GATHER_BLOCK];

@@ -318,7 +318,8 @@ int eb_get_exec_stats(const eb_ctx* ctx, int32_t* mode, int64_t* captures, int64
 
 /* Launch shapes chosen by eb_autotune (0 in eb_set_tuning: the built-in default):
  *   EB_TUNE_CHAIN_LDS_KB    LDS budget (KB) of the reduced chain in the chain factorisation
- *                           (candidates 8, 16, 32, 48; default 48)
+ *                           (candidates 16, 32, 48, 96; default 48; at most 96, and at most what
+ *                           the device grants a workgroup: a larger budget fails the factorisation)
  *   EB_TUNE_CHAIN_SOLVE_LDS chain solve work vectors in LDS (1, default) or global memory (2)
  *   EB_TUNE_GATHER_BLOCK    workgroup size of the matrix gather (64, 128, 256; default 256) */
 #define EB_TUNE_CHAIN_LDS_KB 0
