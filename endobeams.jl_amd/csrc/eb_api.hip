@@ -551,7 +551,7 @@ int krylov_iteration(eb_ctx* ctx) {
 // the scatter of the solution into ΔD. Ktan (block-row order) and res must be current.
 int solve_start(eb_ctx* ctx, const eb_newton_params* p, bool factor = true) {
     const Segs sg = segs(ctx);
-    const double tol = (p && p->krylov_tol > 0) ? p->krylov_tol : 1e-13;
+    const double tol = (p && p->krylov_tol > 0) ? p->krylov_tol : 1e-11;
     hipLaunchKernelGGL(solve_begin, dim3(1), dim3(1), 0, ctx->stream, ctx->kall);
     if (factor) EB_TRY(precond_factor(ctx));
     hipLaunchKernelGGL(krylov_init, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->is_fixed,

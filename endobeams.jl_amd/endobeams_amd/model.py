@@ -236,7 +236,7 @@ class SimulationParams:
     n_gauss_beams: int = 3
     w_gauss_beams: tuple = (5 / 9, 8 / 9, 5 / 9)
     z_gauss_beams: tuple = (-math.sqrt(3 / 5), 0.0, math.sqrt(3 / 5))
-    krylov_tol: float = 1e-13
+    krylov_tol: float = 1e-11
     krylov_maxit: int = 200
 
     @staticmethod

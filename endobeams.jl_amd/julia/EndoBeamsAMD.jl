@@ -168,7 +168,7 @@ function get_field(dev::Device, f::Integer, n::Integer)
     return v
 end
 
-newton(p) = NewtonParams(p.α, p.β, p.γ, p.tolerance_residual, p.tolerance_displacement, Int32(p.max_iterations), Int32(200), 1e-13)
+newton(p) = NewtonParams(p.α, p.β, p.γ, p.tolerance_residual, p.tolerance_displacement, Int32(p.max_iterations), Int32(200), 1e-11)
 
 """solve_step_dynamics!(conf, state, params, inter, tⁿ⁺¹, Δt, solver) (src/solve_step_dynamics.jl:1-14)."""
 """update_loads! and update_boundary_conditions! (src/utils_bcs_and_loads.jl:42-64) at time tⁿ⁺¹."""

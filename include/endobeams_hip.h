@@ -89,7 +89,7 @@ typedef struct eb_newton_params {
     double tol_res, tol_disp;     /* tolerance_residual, tolerance_displacement */
     int32_t max_iterations;
     int32_t krylov_maxit;         /* BiCGStab iteration cap per linear solve (<= 0: default 200) */
-    double krylov_tol;            /* BiCGStab relative residual target (<= 0: default 1e-13) */
+    double krylov_tol;            /* BiCGStab relative residual target (<= 0: default 1e-11) */
 } eb_newton_params;
 
 /* state fields addressable by eb_get_field / eb_set_field */

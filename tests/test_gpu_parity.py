@@ -2,7 +2,7 @@
 
 Every reference function on the hot path is called on both engines from the same randomised
 state and its outputs compared norm-wise (‖a-b‖/‖b‖): 1e-10 for the FP64 kernels (bar: north
-star), 1e-8 on the linear-solve output (BiCGStab to 1e-13 vs a direct LU). Trajectories are
+star), 1e-8 on the linear-solve output (BiCGStab to 1e-11 vs a direct LU). Trajectories are
 compared after many steps with identical Newton iteration counts.
 """
 import numpy as np
@@ -158,7 +158,8 @@ def test_linear_solve_parity(name):
     Kf = K[free - 1][:, free - 1]
     x = g.get("DELTA_D_FREE")
     r = o.get("RES_FREE")
-    assert np.linalg.norm(Kf @ x - r) <= 1e-11 * np.linalg.norm(r)
+    # BiCGStab's relative-residual target (its recursive residual; the true one may differ by rounding)
+    assert np.linalg.norm(Kf @ x - r) <= 2 * params.krylov_tol * np.linalg.norm(r)
     assert rel_err(x, o.get("DELTA_D_FREE")) <= 1e-8
     assert rel_err(g.get("DELTA_D"), o.get("DELTA_D")) <= 1e-8
     assert its >= 1
