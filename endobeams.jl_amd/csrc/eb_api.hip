@@ -139,6 +139,12 @@ struct eb_ctx {
     int32_t *ch_ptr = nullptr, *ch_node = nullptr, *ch_diag = nullptr, *ch_lo = nullptr, *ch_up = nullptr;
     uint64_t *ch_mD = nullptr, *ch_mL = nullptr, *ch_mU = nullptr;
     int32_t* ch_hbase = nullptr;
+    // closed chains (rings): their chain, coupling blocks and masks, T⁻¹U columns and H
+    int32_t *cl_chain = nullptr, *cl_fl = nullptr, *cl_lf = nullptr;
+    uint64_t *cl_mfl = nullptr, *cl_mlf = nullptr;
+    int64_t* cl_zoff = nullptr;
+    double *cl_Z = nullptr, *cl_H = nullptr;
+    int32_t n_closures = 0;
     double *ch_D = nullptr, *ch_L = nullptr, *ch_U = nullptr, *ch_hist = nullptr, *ch_bw = nullptr, *ch_yw = nullptr;
     int precond_mode = 0;
     double* tmp = nullptr;  // nnz scratch for layout conversion
@@ -290,6 +296,9 @@ NodeFields node_fields(eb_ctx* c) {
 BeamConsts beam_consts(eb_ctx* c) { return BeamConsts{c->n1, c->n2, c->l0, c->Re0c, c->propsc, c->E}; }
 template <typename P> BeamStage<P> beam_stage(eb_ctx* c) {
     return BeamStage<P>{(P*)c->sK, (P*)c->sC, (P*)c->sM, (double*)c->sTint, (double*)c->sTk, c->senergy};
+}
+ClosureDev closure_dev(eb_ctx* c) {
+    return ClosureDev{c->cl_chain, c->cl_fl, c->cl_lf, c->cl_mfl, c->cl_mlf, c->cl_zoff, c->cl_Z, c->cl_H, c->n_closures};
 }
 ChainDev chain_dev(eb_ctx* c) {
     return ChainDev{c->ch_ptr, c->ch_node, c->ch_diag, c->ch_lo, c->ch_up, c->ch_mD, c->ch_mL, c->ch_mU,
@@ -502,6 +511,14 @@ int precond_factor(eb_ctx* ctx) {
     }
     hipLaunchKernelGGL(chain_factor, dim3((unsigned)nch), dim3(CHAIN_THREADS), smem, ctx->stream, chain_dev(ctx), ctx->Ktan,
                        &ctx->kall->pivot_status, L0, ctx->active);
+    if (ctx->n_closures > 0) {
+        // rings: the 12 columns of T⁻¹U (LDS: 12 doubles per position, at most 60 KB), then H
+        const size_t cl_smem = (size_t)(12 * ctx->chains.cl_max_len) * sizeof(double);
+        hipLaunchKernelGGL(closure_columns<SOLVE_THREADS_MANY>, dim3((unsigned)ctx->n_closures, 12),
+                           dim3(SOLVE_THREADS_MANY), cl_smem, ctx->stream, chain_dev(ctx), closure_dev(ctx), ctx->active);
+        hipLaunchKernelGGL(closure_setup, dim3((unsigned)ctx->n_closures), dim3(64), 0, ctx->stream, chain_dev(ctx),
+                           closure_dev(ctx), ctx->Ktan, ctx->active);
+    }
     ctx->factor_tag = ctx->ktan_tag;
     prof_end(ctx, EB_KERNEL_PRECOND);
     return check_launch(ctx, "chain_factor");
@@ -520,6 +537,9 @@ int precond_apply(eb_ctx* ctx, const double* b, double* x) {
         hipLaunchKernelGGL((chain_solve<false, SOLVE_THREADS_MANY>), grid, block, 0, ctx->stream, chain_dev(ctx), b, x, ctx->kctl);
     else
         hipLaunchKernelGGL((chain_solve<false, SOLVE_THREADS_FEW>), grid, block, 0, ctx->stream, chain_dev(ctx), b, x, ctx->kctl);
+    if (ctx->n_closures > 0)
+        hipLaunchKernelGGL(closure_apply, dim3((unsigned)ctx->n_closures), dim3(256), 0, ctx->stream, chain_dev(ctx),
+                           closure_dev(ctx), x, ctx->kctl);
     return check_launch(ctx, "chain_solve");
 }
 
@@ -1432,6 +1452,17 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
         CK(dalloc(ctx, ctx->ch_hist, 72 * c.n_hist));
         CK(dalloc(ctx, ctx->ch_bw, 6 * N));
         CK(dalloc(ctx, ctx->ch_yw, 6 * N));
+        ctx->n_closures = (int32_t)c.cl_chain.size();
+        if (ctx->n_closures > 0) {
+            CK(upload_vec(ctx, ctx->cl_chain, c.cl_chain));
+            CK(upload_vec(ctx, ctx->cl_fl, c.cl_fl));
+            CK(upload_vec(ctx, ctx->cl_lf, c.cl_lf));
+            CK(upload_vec(ctx, ctx->cl_mfl, c.cl_mfl));
+            CK(upload_vec(ctx, ctx->cl_mlf, c.cl_mlf));
+            CK(upload_vec(ctx, ctx->cl_zoff, c.cl_zoff));
+            CK(dalloc(ctx, ctx->cl_Z, c.cl_zoff.back()));
+            CK(dalloc(ctx, ctx->cl_H, 144 * (int64_t)ctx->n_closures));
+        }
     }
 #undef CK
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return bail(EB_ERR_DEVICE);

@@ -748,18 +748,13 @@ inline size_t chain_solve_lds_bytes(int64_t max_len) {
 // level by level (y_j = D_j^-1 b_j for eliminated j; b_i -= L_i y_j1 + U_i y_j2 for kept i with the
 // saved blocks), then solved back from position 0 (x_j = y_j - GL_j x_{j-h} - GU_j x_{j+h}).
 // Six lanes per position, one per row; the work vectors in LDS (LDSV) or in the global arrays.
-template <bool LDSV, int SOLVE_THREADS>
-__global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const double* __restrict__ b,
-                                                             double* __restrict__ x, const KrylovCtl* kc) {
-    extern __shared__ __align__(16) double solve_smem[];
-    if (kc[cd.inst[blockIdx.x]].done) return;
-    const int64_t s0 = cd.ptr[blockIdx.x];
-    const int32_t n = cd.ptr[blockIdx.x + 1] - (int32_t)s0;
+// The solve of one chain (s0, n) on its right-hand side in bw (6 per position, overwritten): the
+// reduction level by level, the root, the back-substitution; the solution ends in yw. Every
+// thread of the workgroup calls it (it holds workgroup barriers).
+template <int NT>
+EBD void chain_solve_body(const ChainDev& cd, int64_t s0, int32_t n, double* bw, double* yw) {
     const int tid = threadIdx.x;
-    double* bw = LDSV ? solve_smem : cd.bw + 6 * s0;
-    double* yw = LDSV ? solve_smem + 6 * (int64_t)n : cd.yw + 6 * s0;
-    for (int32_t t = tid; t < 6 * n; t += SOLVE_THREADS) bw[t] = b[6 * (int64_t)cd.node[s0 + t / 6] + t % 6];
-    __syncthreads();
+    constexpr int SOLVE_THREADS = NT;
     const int nl = chain_levels(n);
     for (int l = 0; l < nl; ++l) {
         const int32_t h = 1 << l;
@@ -836,7 +831,154 @@ __global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const 
         }
         __syncthreads();
     }
+}
+
+template <bool LDSV, int SOLVE_THREADS>
+__global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const double* __restrict__ b,
+                                                             double* __restrict__ x, const KrylovCtl* kc) {
+    extern __shared__ __align__(16) double solve_smem[];
+    if (kc[cd.inst[blockIdx.x]].done) return;
+    const int64_t s0 = cd.ptr[blockIdx.x];
+    const int32_t n = cd.ptr[blockIdx.x + 1] - (int32_t)s0;
+    const int tid = threadIdx.x;
+    double* bw = LDSV ? solve_smem : cd.bw + 6 * s0;
+    double* yw = LDSV ? solve_smem + 6 * (int64_t)n : cd.yw + 6 * s0;
+    for (int32_t t = tid; t < 6 * n; t += SOLVE_THREADS) bw[t] = b[6 * (int64_t)cd.node[s0 + t / 6] + t % 6];
+    __syncthreads();
+    chain_solve_body<SOLVE_THREADS>(cd, s0, n, bw, yw);
     for (int32_t t = tid; t < 6 * n; t += SOLVE_THREADS) x[6 * (int64_t)cd.node[s0 + t / 6] + t % 6] = yw[t];
+}
+
+// ---- closed chains (Sherman-Morrison-Woodbury) ----------------------------------------------
+// A chain whose two ends are coupled (a ring) has the matrix A = T + U M Uᵀ: T the block-
+// tridiagonal matrix the factorisation inverts, U the 12 unit columns of the first and last
+// positions' dofs, M = [[0, A_fl], [A_lf, 0]] the two coupling blocks (entries touching a fixed
+// dof zeroed, as in T). With Z = T⁻¹U and G = UᵀZ, A⁻¹ = T⁻¹ − Z H Uᵀ T⁻¹ with
+// H = (I + M G)⁻¹ M, so the preconditioner stays exact on a ring: one chain solve y = T⁻¹ b,
+// then y −= Z (H (Uᵀ y)).
+struct ClosureDev {
+    const int32_t *chain, *fl, *lf;   // per closure: chain, blocks (first, last) and (last, first)
+    const uint64_t *mfl, *mlf;        // their fixed-dof masks
+    const int64_t* zoff;              // per closure: offset of its Z (12 columns × 6 n, chain order)
+    double* Z;
+    double* H;                        // 144 per closure, row-major
+    int32_t n;
+};
+
+EBD int32_t closure_unit(int j, int32_t n) { return j < 6 ? j : 6 * (n - 1) + (j - 6); }
+
+// column j = blockIdx.y of Z for closure blockIdx.x: the chain solve of U's column j (in LDS:
+// 12 doubles per position)
+template <int NT>
+__global__ void __launch_bounds__(NT) closure_columns(ChainDev cd, ClosureDev cl, const uint8_t* __restrict__ active) {
+    extern __shared__ __align__(16) double cl_smem[];
+    const int c = blockIdx.x, j = blockIdx.y;
+    const int32_t ch = cl.chain[c];
+    if (!active[cd.inst[ch]]) return;   // frozen instance: its factors were not refreshed
+    const int64_t s0 = cd.ptr[ch];
+    const int32_t n = cd.ptr[ch + 1] - (int32_t)s0;
+    double* bw = cl_smem;
+    double* yw = cl_smem + 6 * (int64_t)n;
+    const int32_t unit = closure_unit(j, n);
+    for (int32_t t = threadIdx.x; t < 6 * n; t += NT) bw[t] = t == unit ? 1.0 : 0.0;
+    __syncthreads();
+    chain_solve_body<NT>(cd, s0, n, bw, yw);
+    double* z = cl.Z + cl.zoff[c] + (int64_t)j * 6 * n;
+    for (int32_t t = threadIdx.x; t < 6 * n; t += NT) z[t] = yw[t];
+}
+
+// H = (I + M G)⁻¹ M per closure, one wavefront each, in LDS: Gauss-Jordan elimination with
+// partial pivoting on [I + M G | M]; a singular or non-finite system leaves H = 0 (no correction:
+// the plain chain preconditioner)
+__global__ void __launch_bounds__(64) closure_setup(ChainDev cd, ClosureDev cl, const double* __restrict__ A,
+                                                    const uint8_t* __restrict__ active) {
+    __shared__ double M[12][12];
+    __shared__ double W[12][24];
+    __shared__ double fcol[12];
+    __shared__ int piv_row, ok;
+    const int c = blockIdx.x, t = threadIdx.x;
+    const int32_t ch = cl.chain[c];
+    if (!active[cd.inst[ch]]) return;
+    const int32_t n = cd.ptr[ch + 1] - cd.ptr[ch];
+    const double* Z = cl.Z + cl.zoff[c];
+    for (int q = t; q < 144; q += 64) {
+        const int r = q / 12, k = q % 12;
+        double v = 0.0;
+        if (r < 6 && k >= 6) {
+            const int e = 6 * r + (k - 6);
+            v = ((cl.mfl[c] >> e) & 1) ? 0.0 : A[36 * (int64_t)cl.fl[c] + e];
+        } else if (r >= 6 && k < 6) {
+            const int e = 6 * (r - 6) + k;
+            v = ((cl.mlf[c] >> e) & 1) ? 0.0 : A[36 * (int64_t)cl.lf[c] + e];
+        }
+        M[r][k] = v;
+    }
+    if (t == 0) ok = 1;
+    __syncthreads();
+    for (int q = t; q < 144; q += 64) {
+        const int r = q / 12, jc = q % 12;
+        double v = r == jc ? 1.0 : 0.0;
+        for (int k = 0; k < 12; ++k) v += M[r][k] * Z[(int64_t)jc * 6 * n + closure_unit(k, n)];
+        W[r][jc] = v;
+        W[r][12 + jc] = M[r][jc];
+    }
+    __syncthreads();
+    for (int p = 0; p < 12; ++p) {
+        if (t == 0) {
+            int pr = p;
+            for (int r = p + 1; r < 12; ++r)
+                if (fabs(W[r][p]) > fabs(W[pr][p])) pr = r;
+            piv_row = pr;
+            if (!(fabs(W[pr][p]) > 0.0) || !isfinite(W[pr][p])) ok = 0;
+        }
+        __syncthreads();
+        if (!ok) break;   // uniform: read after the barrier by every lane
+        const int pr = piv_row;
+        if (pr != p && t < 24) { const double a = W[p][t]; W[p][t] = W[pr][t]; W[pr][t] = a; }
+        __syncthreads();
+        const double inv = 1.0 / W[p][p];
+        __syncthreads();
+        if (t < 24) W[p][t] *= inv;
+        if (t < 12) fcol[t] = W[t][p];   // the column's factors, read before any row changes
+        __syncthreads();
+        for (int q = t; q < 12 * 24; q += 64) {
+            const int r = q / 24, k = q % 24;
+            if (r != p) W[r][k] -= fcol[r] * W[p][k];
+        }
+        __syncthreads();
+    }
+    double* H = cl.H + 144 * (int64_t)c;
+    for (int q = t; q < 144; q += 64) {
+        const double v = W[q / 12][12 + q % 12];
+        H[q] = ok && isfinite(v) ? v : 0.0;
+    }
+}
+
+// y −= Z (H (Uᵀ y)) on each closed chain of an unfinished solve, after the chain solve
+__global__ void __launch_bounds__(256) closure_apply(ChainDev cd, ClosureDev cl, double* __restrict__ y,
+                                                     const KrylovCtl* __restrict__ kc) {
+    __shared__ double sv[12];
+    const int c = blockIdx.x;
+    const int32_t ch = cl.chain[c];
+    if (kc[cd.inst[ch]].done) return;
+    const int64_t s0 = cd.ptr[ch];
+    const int32_t n = cd.ptr[ch + 1] - (int32_t)s0;
+    if (threadIdx.x < 12) {
+        const int32_t nf = cd.node[s0], nl = cd.node[s0 + n - 1];
+        const double* H = cl.H + 144 * (int64_t)c + 12 * threadIdx.x;
+        double acc = 0;
+        for (int k = 0; k < 6; ++k) acc += H[k] * y[6 * (int64_t)nf + k];
+        for (int k = 0; k < 6; ++k) acc += H[6 + k] * y[6 * (int64_t)nl + k];
+        sv[threadIdx.x] = acc;
+    }
+    __syncthreads();
+    const double* Z = cl.Z + cl.zoff[c];
+    for (int32_t q = threadIdx.x; q < 6 * n; q += 256) {
+        double acc = 0;
+#pragma unroll
+        for (int j = 0; j < 12; ++j) acc += Z[(int64_t)j * 6 * n + q] * sv[j];
+        y[6 * (int64_t)cd.node[s0 + q / 6] + q % 6] -= acc;
+    }
 }
 
 // ---- BiCGStab (van der Vorst), right-preconditioned, FP64 scalars on the device -------------

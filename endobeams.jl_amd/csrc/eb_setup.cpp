@@ -211,6 +211,24 @@ void build_chains(Chains& c, const Pattern& p, const double* X0, int mode) {
         }
     }
     c.n_hist = slot;
+    // couplings between a chain's two ends, which the block-tridiagonal factorisation leaves out
+    c.cl_chain.clear(); c.cl_fl.clear(); c.cl_lf.clear(); c.cl_mfl.clear(); c.cl_mlf.clear();
+    c.cl_zoff.assign(1, 0);
+    c.cl_max_len = 0;
+    for (size_t ch = 0; ch + 1 < c.ptr.size(); ++ch) {
+        const int64_t s = c.ptr[ch], e = c.ptr[ch + 1], n = e - s;
+        if (n < 3 || n > CLOSURE_MAX_LEN) continue;
+        const int32_t f = c.node[s], l = c.node[e - 1];
+        const int32_t fl = blk(f, l), lf = blk(l, f);
+        if (fl < 0 || lf < 0) continue;
+        c.cl_chain.push_back((int32_t)ch);
+        c.cl_fl.push_back(fl);
+        c.cl_lf.push_back(lf);
+        c.cl_mfl.push_back(mask(f, l));
+        c.cl_mlf.push_back(mask(l, f));
+        c.cl_zoff.push_back(c.cl_zoff.back() + 72 * n);
+        c.cl_max_len = std::max(c.cl_max_len, n);
+    }
 }
 
 // Separable solve of  c₀ = c₁, c_{i-1}/8 + 3c_i/4 + c_{i+1}/8 = y_i, c_{n+1} = c_n  along each axis

@@ -44,7 +44,16 @@ struct Chains {
     std::vector<int32_t> hbase;                  // per chain position: first cyclic-reduction history slot
     int64_t n_hist = 0;
     int64_t max_len = 0;
+    // closures: chains of at least three positions whose first and last nodes are coupled (a
+    // ring), up to CLOSURE_MAX_LEN positions. Per closure: its chain, the coupling blocks
+    // (first, last) and (last, first), their fixed-dof masks, and the offset of its 12 columns of
+    // T^-1 U (72 doubles per position)
+    std::vector<int32_t> cl_chain, cl_fl, cl_lf;
+    std::vector<uint64_t> cl_mfl, cl_mlf;
+    std::vector<int64_t> cl_zoff;   // n_closures+1
+    int64_t cl_max_len = 0;
 };
+constexpr int64_t CLOSURE_MAX_LEN = 640;
 
 bool build_pattern(Pattern& p, int64_t N, int64_t E, const int64_t* conn1b, int64_t n_fixed, const int64_t* fixed1b,
                    std::string& err);
