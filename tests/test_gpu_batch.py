@@ -82,3 +82,30 @@ def test_batch_instance_matches_oracle():
         print(f, e)
         assert e <= t, (f, e)
     batch.close()
+
+
+def rings():
+    from endobeams_amd import configs
+    # rings of 24, 60 and 40 beams, starting near the plane: each is one closed chain whose
+    # preconditioner carries the closure correction
+    return [configs.ring_drop(n_el=n, height=h) for (n, h) in [(24, 0.06), (60, 0.5), (40, 0.06)]]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_of_rings_follows_the_standalone_rings(precision):
+    from endobeams_amd import HipBatchEngine, HipEngine, abi
+    prec = abi.EB_PRECISION_FP64 if precision == "fp64" else abi.EB_PRECISION_FP32
+    batch = HipBatchEngine(rings(), precision=prec)
+    sched = batch_schedule(batch, 6)
+    print("attempts (dt, accepted, k per instance):", [(dt, ok, k.tolist()) for (_, dt, ok, k) in sched], flush=True)
+    assert sum(ok for (_, _, ok, _) in sched) >= 3
+    for s, (conf, params, inter) in enumerate(rings()):
+        alone = HipEngine(conf, params, inter, precision=prec)
+        ka = replay(alone, sched)
+        assert ka == [int(k[s]) for (_, _, _, k) in sched], (s, ka)
+        for f in FIELDS:
+            a = alone.get(f)
+            b = batch.get(f)[batch.instance_slice(f, s)]
+            assert rel_err(b, a) <= 1e-13, (s, f, rel_err(b, a))
+        alone.close()
+    batch.close()
