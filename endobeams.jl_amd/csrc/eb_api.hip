@@ -145,6 +145,11 @@ struct eb_ctx {
     int64_t* cl_zoff = nullptr;
     double *cl_Z = nullptr, *cl_H = nullptr;
     int32_t n_closures = 0;
+    // the crossing chain cover (the other lines of a net): the couplings the first cover leaves
+    // out, corrected after it, x += T₂⁻¹ (b − A x); dev2.n_chains = 0 when the first cover holds all
+    ebh::Chains chains2;
+    ChainDev dev2{};
+    double* cross_r = nullptr;   // [ndofs] b − A x
     double *ch_D = nullptr, *ch_L = nullptr, *ch_U = nullptr, *ch_hist = nullptr, *ch_bw = nullptr, *ch_yw = nullptr;
     int precond_mode = 0;
     double* tmp = nullptr;  // nnz scratch for layout conversion
@@ -519,6 +524,16 @@ int precond_factor(eb_ctx* ctx) {
         hipLaunchKernelGGL(closure_setup, dim3((unsigned)ctx->n_closures), dim3(64), 0, ctx->stream, chain_dev(ctx),
                            closure_dev(ctx), ctx->Ktan, ctx->active);
     }
+    if (ctx->dev2.n_chains > 0) {
+        const int L02 = chain_lds_level(ctx->chains2.max_len, 1024 * (int64_t)tuned(ctx, EB_TUNE_CHAIN_LDS_KB));
+        const size_t smem2 = chain_lds_bytes(ctx->chains2.max_len, L02);
+        if (smem2 > (size_t)ctx->chain_lds_max) {
+            ctx->err = "chain factorisation: the LDS budget exceeds what the device allows";
+            return EB_ERR_ARG;
+        }
+        hipLaunchKernelGGL(chain_factor, dim3((unsigned)ctx->dev2.n_chains), dim3(CHAIN_THREADS), smem2, ctx->stream,
+                           ctx->dev2, ctx->Ktan, &ctx->kall->pivot_status, L02, ctx->active);
+    }
     ctx->factor_tag = ctx->ktan_tag;
     prof_end(ctx, EB_KERNEL_PRECOND);
     return check_launch(ctx, "chain_factor");
@@ -540,6 +555,28 @@ int precond_apply(eb_ctx* ctx, const double* b, double* x) {
     if (ctx->n_closures > 0)
         hipLaunchKernelGGL(closure_apply, dim3((unsigned)ctx->n_closures), dim3(256), 0, ctx->stream, chain_dev(ctx),
                            closure_dev(ctx), x, ctx->kctl);
+    if (ctx->dev2.n_chains > 0) {
+        // the crossing cover: x += T₂⁻¹ (b − A x)
+        const Segs sg = segs(ctx);
+        hipLaunchKernelGGL(residual_masked, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->bptr,
+                           ctx->bcol, ctx->is_fixed, ctx->Ktan, x, b, ctx->cross_r, ctx->kctl);
+        const int64_t n2 = ctx->dev2.n_chains;
+        const size_t smem2 = tuned(ctx, EB_TUNE_CHAIN_SOLVE_LDS) == 1 ? chain_solve_lds_bytes(ctx->chains2.max_len) : 0;
+        const bool many2 = n2 >= SOLVE_MANY_CHAINS;
+        const dim3 grid2((unsigned)n2), block2(many2 ? SOLVE_THREADS_MANY : SOLVE_THREADS_FEW);
+        if (smem2 && many2)
+            hipLaunchKernelGGL((chain_solve<true, SOLVE_THREADS_MANY, true>), grid2, block2, smem2, ctx->stream, ctx->dev2,
+                               ctx->cross_r, x, ctx->kctl);
+        else if (smem2)
+            hipLaunchKernelGGL((chain_solve<true, SOLVE_THREADS_FEW, true>), grid2, block2, smem2, ctx->stream, ctx->dev2,
+                               ctx->cross_r, x, ctx->kctl);
+        else if (many2)
+            hipLaunchKernelGGL((chain_solve<false, SOLVE_THREADS_MANY, true>), grid2, block2, 0, ctx->stream, ctx->dev2,
+                               ctx->cross_r, x, ctx->kctl);
+        else
+            hipLaunchKernelGGL((chain_solve<false, SOLVE_THREADS_FEW, true>), grid2, block2, 0, ctx->stream, ctx->dev2,
+                               ctx->cross_r, x, ctx->kctl);
+    }
     return check_launch(ctx, "chain_solve");
 }
 
@@ -1453,6 +1490,37 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
         CK(dalloc(ctx, ctx->ch_bw, 6 * N));
         CK(dalloc(ctx, ctx->ch_yw, 6 * N));
         ctx->n_closures = (int32_t)c.cl_chain.size();
+        if (ebh::chains_left_out(c, P) > 0) {
+            ebh::build_chains(ctx->chains2, P, d->X0, 0, &c);
+            const ebh::Chains& c2 = ctx->chains2;
+            std::vector<int32_t> chi2(c2.ptr.size() - 1);
+            for (size_t k = 0; k + 1 < c2.ptr.size(); ++k) {
+                const int64_t n0 = c2.node[c2.ptr[k]];
+                chi2[k] = (int32_t)(std::upper_bound(noff.begin(), noff.end(), n0) - noff.begin() - 1);
+            }
+            int32_t *ptr2, *node2, *diag2, *lo2, *up2, *hb2, *inst2;
+            uint64_t *mD2, *mL2, *mU2;
+            double *D2, *L2, *U2, *hist2, *bw2, *yw2;
+            CK(upload_vec(ctx, inst2, chi2));
+            CK(upload_vec(ctx, ptr2, narrow<int32_t>(c2.ptr)));
+            CK(upload_vec(ctx, node2, c2.node));
+            CK(upload_vec(ctx, diag2, c2.diag_blk));
+            CK(upload_vec(ctx, lo2, c2.lo_blk));
+            CK(upload_vec(ctx, up2, c2.up_blk));
+            CK(upload_vec(ctx, mD2, c2.mD));
+            CK(upload_vec(ctx, mL2, c2.mL));
+            CK(upload_vec(ctx, mU2, c2.mU));
+            CK(upload_vec(ctx, hb2, c2.hbase));
+            CK(dalloc(ctx, D2, 36 * N));
+            CK(dalloc(ctx, L2, 36 * N));
+            CK(dalloc(ctx, U2, 36 * N));
+            CK(dalloc(ctx, hist2, 72 * c2.n_hist));
+            CK(dalloc(ctx, bw2, 6 * N));
+            CK(dalloc(ctx, yw2, 6 * N));
+            CK(dalloc(ctx, ctx->cross_r, 6 * N));
+            ctx->dev2 = ChainDev{ptr2, node2, diag2, lo2, up2, mD2, mL2, mU2, hb2, D2, L2, U2, hist2, bw2, yw2, inst2,
+                                 (int64_t)c2.ptr.size() - 1};
+        }
         if (ctx->n_closures > 0) {
             CK(upload_vec(ctx, ctx->cl_chain, c.cl_chain));
             CK(upload_vec(ctx, ctx->cl_fl, c.cl_fl));

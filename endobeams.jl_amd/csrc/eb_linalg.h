@@ -358,6 +358,35 @@ __global__ void __launch_bounds__(RED_BLOCK) spmv_masked(Segs sg, const int32_t*
     if (NQ > 0) seg_block_store<(NQ > 0 ? NQ : 1)>(v, partials, sg.nchunks);
 }
 
+// r = b − A x with the solve's operator (the identity on fixed dofs), on the chunks of unfinished
+// solves: the right-hand side of the crossing chain cover's correction
+__global__ void __launch_bounds__(RED_BLOCK) residual_masked(Segs sg, const int32_t* __restrict__ bptr,
+                                                             const int32_t* __restrict__ bcol,
+                                                             const uint8_t* __restrict__ fixed,
+                                                             const double* __restrict__ A, const double* __restrict__ x,
+                                                             const double* __restrict__ b, double* __restrict__ r,
+                                                             const KrylovCtl* kc) {
+    if (kc[sg.inst[blockIdx.x]].done) return;
+    for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
+        double acc;
+        if (fixed[i]) {
+            acc = x[i];
+        } else {
+            const int64_t n = i / 6;
+            const int a = (int)(i % 6);
+            acc = 0;
+            for (int k = bptr[n]; k < bptr[n + 1]; ++k) {
+                double rv[6], xv[6];
+                ld6(A + (int64_t)36 * k + 6 * a, rv);
+                ld6(x + 6 * (int64_t)bcol[k], xv);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) acc += rv[c] * xv[c];
+            }
+        }
+        r[i] = b[i] - acc;
+    }
+}
+
 // ---- chain block-tridiagonal preconditioner ------------------------------------------------
 // The node graph is covered by chains (paths); along a chain the masked tangent restricted to
 // the chain is block-tridiagonal with 6x6 blocks (row i: L_i x_{i-1} + D_i x_i + U_i x_{i+1}) and
@@ -833,7 +862,8 @@ EBD void chain_solve_body(const ChainDev& cd, int64_t s0, int32_t n, double* bw,
     }
 }
 
-template <bool LDSV, int SOLVE_THREADS>
+// ACCUM: x += M⁻¹ b (the crossing cover's correction) instead of x = M⁻¹ b
+template <bool LDSV, int SOLVE_THREADS, bool ACCUM = false>
 __global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const double* __restrict__ b,
                                                              double* __restrict__ x, const KrylovCtl* kc) {
     extern __shared__ __align__(16) double solve_smem[];
@@ -846,7 +876,10 @@ __global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const 
     for (int32_t t = tid; t < 6 * n; t += SOLVE_THREADS) bw[t] = b[6 * (int64_t)cd.node[s0 + t / 6] + t % 6];
     __syncthreads();
     chain_solve_body<SOLVE_THREADS>(cd, s0, n, bw, yw);
-    for (int32_t t = tid; t < 6 * n; t += SOLVE_THREADS) x[6 * (int64_t)cd.node[s0 + t / 6] + t % 6] = yw[t];
+    for (int32_t t = tid; t < 6 * n; t += SOLVE_THREADS) {
+        double& xi = x[6 * (int64_t)cd.node[s0 + t / 6] + t % 6];
+        xi = ACCUM ? xi + yw[t] : yw[t];
+    }
 }
 
 // ---- closed chains (Sherman-Morrison-Woodbury) ----------------------------------------------

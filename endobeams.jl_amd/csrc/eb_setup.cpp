@@ -107,8 +107,40 @@ bool build_pattern(Pattern& p, int64_t N, int64_t E, const int64_t* conn1b, int6
     return true;
 }
 
-void build_chains(Chains& c, const Pattern& p, const double* X0, int mode) {
+// per node, the sorted neighbours it is joined to by a chain of c (consecutive positions, and the
+// two ends of a closure)
+static std::vector<std::vector<int32_t>> chain_links(const Chains& c, int64_t N) {
+    std::vector<std::vector<int32_t>> l((size_t)N);
+    for (size_t ch = 0; ch + 1 < c.ptr.size(); ++ch)
+        for (int64_t i = c.ptr[ch]; i + 1 < c.ptr[ch + 1]; ++i) {
+            l[c.node[i]].push_back(c.node[i + 1]);
+            l[c.node[i + 1]].push_back(c.node[i]);
+        }
+    for (size_t k = 0; k < c.cl_chain.size(); ++k) {
+        const int32_t f = c.node[c.ptr[c.cl_chain[k]]], e = c.node[c.ptr[c.cl_chain[k] + 1] - 1];
+        l[f].push_back(e);
+        l[e].push_back(f);
+    }
+    for (auto& v : l) std::sort(v.begin(), v.end());
+    return l;
+}
+
+int64_t chains_left_out(const Chains& c, const Pattern& p) {
+    const auto links = chain_links(c, p.N);
+    int64_t n = 0;
+    for (int32_t a = 0; a < p.N; ++a)
+        for (int32_t m : p.adj[a])
+            if (m > a && !std::binary_search(links[a].begin(), links[a].end(), m)) ++n;
+    return n;
+}
+
+void build_chains(Chains& c, const Pattern& p, const double* X0, int mode, const Chains* across) {
     const int64_t N = p.N;
+    const std::vector<std::vector<int32_t>> taken = across ? chain_links(*across, N) : std::vector<std::vector<int32_t>>();
+    // a coupling the walk may follow: not a self-coupling, and not already held by `across`
+    auto free_link = [&](int32_t n, int32_t m) {
+        return m != n && (!across || !std::binary_search(taken[n].begin(), taken[n].end(), m));
+    };
     c.ptr.assign(1, 0);
     c.node.clear();
     std::vector<uint8_t> used(N, 0);
@@ -120,7 +152,7 @@ void build_chains(Chains& c, const Pattern& p, const double* X0, int mode) {
             int32_t best = -1;
             double bestdot = -1e300;
             for (int32_t m : p.adj[cur]) {
-                if (m == cur || used[m]) continue;
+                if (!free_link(cur, m) || used[m]) continue;
                 double d;
                 if (prev < 0) {
                     d = -(double)m;  // first step: smallest index
@@ -152,7 +184,7 @@ void build_chains(Chains& c, const Pattern& p, const double* X0, int mode) {
             for (int32_t n = 0; n < N; ++n) {
                 if (used[n]) continue;
                 int free_nb = 0;
-                for (int32_t m : p.adj[n]) if (m != n && !used[m]) ++free_nb;
+                for (int32_t m : p.adj[n]) if (free_link(n, m) && !used[m]) ++free_nb;
                 if (free_nb <= 1) { walk(n); progress = true; }
             }
         }
@@ -220,7 +252,7 @@ void build_chains(Chains& c, const Pattern& p, const double* X0, int mode) {
         if (n < 3 || n > CLOSURE_MAX_LEN) continue;
         const int32_t f = c.node[s], l = c.node[e - 1];
         const int32_t fl = blk(f, l), lf = blk(l, f);
-        if (fl < 0 || lf < 0) continue;
+        if (fl < 0 || lf < 0 || !free_link(f, l)) continue;
         c.cl_chain.push_back((int32_t)ch);
         c.cl_fl.push_back(fl);
         c.cl_lf.push_back(lf);

@@ -59,8 +59,14 @@ bool build_pattern(Pattern& p, int64_t N, int64_t E, const int64_t* conn1b, int6
                    std::string& err);
 
 // mode 0: path cover preferring straight continuation (exact for chains of beams);
-// mode 1: every node alone (6x6 block-Jacobi)
-void build_chains(Chains& c, const Pattern& p, const double* X0, int mode);
+// mode 1: every node alone (6x6 block-Jacobi).
+// With `across`, the walk uses only the couplings that none of across's chains holds between
+// consecutive positions (a second, crossing cover: the other lines of a net); a node with no such
+// coupling left is a chain of its own.
+void build_chains(Chains& c, const Pattern& p, const double* X0, int mode, const Chains* across = nullptr);
+// couplings (node pairs with a matrix block) that no chain of c holds between consecutive
+// positions, counted once per pair; closures count as held
+int64_t chains_left_out(const Chains& c, const Pattern& p);
 
 // Quadratic B-spline Reflect(OnCell()) coefficients, padded by one cell per side
 void prefilter_sdf(const int64_t n[3], const double* samples, std::vector<double>& coef);
