@@ -101,6 +101,7 @@ struct eb_ctx {
     // matrices (block-row order) and vectors
     double *K = nullptr, *C = nullptr, *M = nullptr, *Kn = nullptr, *Cn = nullptr, *Mn = nullptr, *Ktan = nullptr;
     int64_t chain_lds_max = 0;   // dynamic LDS the chain factorisation may take (bytes)
+    int64_t solve_lds_max = 0;   // dynamic LDS the chain solve may take (bytes)
     bool np1_alias = false;  // K,C,M at n+1 currently equal the step-n matrices (after a swap)
     bool kcm_pending = false; // K,C,M at n+1 not yet gathered from the element staging of the last assembly
     // What the tangent, the chain factors and the matrices hold, so that a predictor whose tangent
@@ -541,7 +542,7 @@ int precond_factor(eb_ctx* ctx) {
 
 int precond_apply(eb_ctx* ctx, const double* b, double* x) {
     int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
-    const size_t smem = tuned(ctx, EB_TUNE_CHAIN_SOLVE_LDS) == 1 ? chain_solve_lds_bytes(ctx->chains.max_len) : 0;
+    const size_t smem = tuned(ctx, EB_TUNE_CHAIN_SOLVE_LDS) == 1 ? chain_solve_lds_bytes(ctx->chains.max_len, ctx->solve_lds_max) : 0;
     const bool many = nch >= SOLVE_MANY_CHAINS;
     const dim3 grid((unsigned)nch), block(many ? SOLVE_THREADS_MANY : SOLVE_THREADS_FEW);
     if (smem && many)
@@ -561,7 +562,7 @@ int precond_apply(eb_ctx* ctx, const double* b, double* x) {
         hipLaunchKernelGGL(residual_masked, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->bptr,
                            ctx->bcol, ctx->is_fixed, ctx->Ktan, x, b, ctx->cross_r, ctx->kctl);
         const int64_t n2 = ctx->dev2.n_chains;
-        const size_t smem2 = tuned(ctx, EB_TUNE_CHAIN_SOLVE_LDS) == 1 ? chain_solve_lds_bytes(ctx->chains2.max_len) : 0;
+        const size_t smem2 = tuned(ctx, EB_TUNE_CHAIN_SOLVE_LDS) == 1 ? chain_solve_lds_bytes(ctx->chains2.max_len, ctx->solve_lds_max) : 0;
         const bool many2 = n2 >= SOLVE_MANY_CHAINS;
         const dim3 grid2((unsigned)n2), block2(many2 ? SOLVE_THREADS_MANY : SOLVE_THREADS_FEW);
         if (smem2 && many2)
@@ -917,7 +918,7 @@ int autotune_impl(eb_ctx* ctx, int32_t* choice, double* ms_out) {
         for (int c = 0; c < EB_TUNE_CANDIDATES && st == EB_OK; ++c) {
             const int32_t v = cands[id][c];
             if (v == 0) continue;
-            if (id == EB_TUNE_CHAIN_SOLVE_LDS && v == 1 && chain_solve_lds_bytes(ctx->chains.max_len) == 0) continue;
+            if (id == EB_TUNE_CHAIN_SOLVE_LDS && v == 1 && chain_solve_lds_bytes(ctx->chains.max_len, ctx->solve_lds_max) == 0) continue;
             if (id == EB_TUNE_CHAIN_LDS_KB &&
                 chain_lds_bytes(ctx->chains.max_len, chain_lds_level(ctx->chains.max_len, 1024 * (int64_t)v)) >
                     (size_t)ctx->chain_lds_max)
@@ -1299,6 +1300,18 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
             (void)hipGetLastError();
             ctx->chain_lds_max = dflt;
         }
+        // the chain solve (no static LDS): every instantiation that takes its work vectors in LDS
+        ctx->solve_lds_max = std::max<int64_t>(65536, max_lds);
+        const void* solves[4] = {(const void*)chain_solve<true, SOLVE_THREADS_MANY>,
+                                 (const void*)chain_solve<true, SOLVE_THREADS_FEW>,
+                                 (const void*)chain_solve<true, SOLVE_THREADS_MANY, true>,
+                                 (const void*)chain_solve<true, SOLVE_THREADS_FEW, true>};
+        for (const void* f : solves)
+            if (ctx->solve_lds_max > 65536 &&
+                hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->solve_lds_max) != hipSuccess) {
+                (void)hipGetLastError();
+                ctx->solve_lds_max = 65536;
+            }
     }
     ctx->precision = d->precision;
     std::string err;

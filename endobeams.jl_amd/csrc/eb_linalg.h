@@ -767,10 +767,11 @@ inline int chain_lds_level(int64_t max_len, int64_t budget) {
 inline size_t chain_lds_bytes(int64_t max_len, int L0) {
     return (size_t)(((max_len + (int64_t(1) << L0) - 1) >> L0) * 3 * 36 * (int64_t)sizeof(double));
 }
-// the solve keeps its two work vectors (12 doubles per position) in LDS up to this chain length
-constexpr int64_t SOLVE_LDS_MAX_LEN = 640;
-inline size_t chain_solve_lds_bytes(int64_t max_len) {
-    return max_len <= SOLVE_LDS_MAX_LEN ? (size_t)(12 * max_len * (int64_t)sizeof(double)) : 0;
+// the solve keeps its two work vectors (12 doubles per position) in LDS while they fit in `cap`
+// bytes (what the device grants a workgroup beside the kernel's static LDS); 0: global memory
+inline size_t chain_solve_lds_bytes(int64_t max_len, int64_t cap) {
+    const int64_t b = 12 * max_len * (int64_t)sizeof(double);
+    return b <= cap ? (size_t)b : 0;
 }
 
 // x = M^-1 b along each chain, with the factors of chain_factor: the right-hand side is reduced
