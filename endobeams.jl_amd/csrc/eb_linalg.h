@@ -327,6 +327,24 @@ struct KrylovCtl {
     int done, iters, breakdown, pad;
 };
 
+// row i of the solve's operator times x: the identity on fixed dofs, the tangent's block row
+// (node i / 6, row i % 6) elsewhere, summed block by block in column order
+EBD double masked_row(int64_t i, const int32_t* __restrict__ bptr, const int32_t* __restrict__ bcol,
+                      const uint8_t* __restrict__ fixed, const double* __restrict__ A, const double* __restrict__ x) {
+    if (fixed[i]) return x[i];
+    const int64_t n = i / 6;
+    const int a = (int)(i % 6);
+    double acc = 0;
+    for (int b = bptr[n]; b < bptr[n + 1]; ++b) {
+        double rv[6], xv[6];
+        ld6(A + (int64_t)36 * b + 6 * a, rv);
+        ld6(x + 6 * (int64_t)bcol[b], xv);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc += rv[c] * xv[c];
+    }
+    return acc;
+}
+
 template <int NQ>
 __global__ void __launch_bounds__(RED_BLOCK) spmv_masked(Segs sg, const int32_t* __restrict__ bptr,
                                                          const int32_t* __restrict__ bcol, const uint8_t* __restrict__ fixed,
@@ -336,21 +354,7 @@ __global__ void __launch_bounds__(RED_BLOCK) spmv_masked(Segs sg, const int32_t*
     if (kc[sg.inst[blockIdx.x]].done) return;
     double v[NQ > 0 ? NQ : 1] = {0};
     for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
-        double acc;
-        if (fixed[i]) {
-            acc = x[i];
-        } else {
-            const int64_t n = i / 6;
-            const int a = (int)(i % 6);
-            acc = 0;
-            for (int b = bptr[n]; b < bptr[n + 1]; ++b) {
-                double rv[6], xv[6];
-                ld6(A + (int64_t)36 * b + 6 * a, rv);
-                ld6(x + 6 * (int64_t)bcol[b], xv);
-#pragma unroll
-                for (int c = 0; c < 6; ++c) acc += rv[c] * xv[c];
-            }
-        }
+        const double acc = masked_row(i, bptr, bcol, fixed, A, x);
         y[i] = acc;
         if (NQ >= 1) v[0] += dotw[i] * acc;
         if (NQ >= 2) v[NQ - 1] += acc * acc;
@@ -368,21 +372,7 @@ __global__ void __launch_bounds__(RED_BLOCK) residual_masked(Segs sg, const int3
                                                              const KrylovCtl* kc) {
     if (kc[sg.inst[blockIdx.x]].done) return;
     for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
-        double acc;
-        if (fixed[i]) {
-            acc = x[i];
-        } else {
-            const int64_t n = i / 6;
-            const int a = (int)(i % 6);
-            acc = 0;
-            for (int k = bptr[n]; k < bptr[n + 1]; ++k) {
-                double rv[6], xv[6];
-                ld6(A + (int64_t)36 * k + 6 * a, rv);
-                ld6(x + 6 * (int64_t)bcol[k], xv);
-#pragma unroll
-                for (int c = 0; c < 6; ++c) acc += rv[c] * xv[c];
-            }
-        }
+        const double acc = masked_row(i, bptr, bcol, fixed, A, x);
         r[i] = b[i] - acc;
     }
 }
