@@ -5,9 +5,9 @@
 // Activations of a minibatch are row-major [rows][width] per layer.
 //
 // Kernels (gfx950, wave64):
-//   forward  — one workgroup per 64 rows; the input tile sits in LDS and every wavefront computes
-//              one output neuron for the 64 rows (lane = row), so the weight it reads is the same
-//              for the whole wavefront (a scalar load) and the activations are LDS reads;
+//   forward  — one workgroup per 32 rows on the FP32 matrix cores: the input tile sits in LDS and
+//              wavefront w computes outputs 32w .. 32w+31 of the 32 rows with
+//              v_mfma_f32_32x32x2f32, two input features per instruction;
 //   backward — one workgroup per 32 rows: ∂L/∂W and ∂L/∂b of the tile into per-tile partials
 //              (summed afterwards in tile order, so training is deterministic), and ∂L/∂Z of the
 //              layer below with the same wave-uniform weight scheme;
@@ -27,10 +27,16 @@
 
 namespace ebm {
 
-constexpr int FWD_TILE = 64;
+constexpr int FWD_TILE = 32;
 constexpr int BWD_TILE = 32;
 constexpr int MAXW = 128;
 
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+// Z = Hin Wᵀ + b for 32 rows, wavefront w taking output columns 32w .. 32w+31 (out ≤ 128: four
+// wavefronts). v_mfma_f32_32x32x2f32 multiplies a 32×2 by a 2×32 block and accumulates 32×32:
+// lane l supplies A[l % 32][l / 32] and B[l / 32][l % 32], and holds column l % 32 of the result,
+// rows 8⌊r/4⌋ + 4⌊l/32⌋ + r mod 4 in its register r.
 __global__ void __launch_bounds__(256) mlp_forward_layer(int64_t rows, int in, int out, const float* __restrict__ W,
                                                          const float* __restrict__ b, const float* __restrict__ Hin,
                                                          float* __restrict__ Hout, int act) {
@@ -44,12 +50,27 @@ __global__ void __launch_bounds__(256) mlp_forward_layer(int64_t rows, int in, i
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    for (int j = wave; j < out; j += blockDim.x >> 6) {
-        const int jj = __builtin_amdgcn_readfirstlane(j);
-        const float* w = W + (int64_t)jj * in;
-        float z = b[jj];
-        for (int k = 0; k < in; ++k) z = fmaf(w[k], X[lane][k], z);
-        if (lane < nr) Hout[(r0 + lane) * out + jj] = act ? tanhf(z) : z;
+    if (32 * wave >= out) return;   // wave-uniform; no barrier follows
+    const int j = 32 * wave + (lane & 31);
+    const int kh = lane >> 5;
+    v16f acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    for (int k0 = 0; k0 < in; k0 += 2) {
+        const int k = k0 + kh;
+        const float a = k < in ? X[lane & 31][k] : 0.0f;
+        const float w = (k < in && j < out) ? W[(int64_t)j * in + k] : 0.0f;
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w, acc, 0, 0, 0);
+    }
+    if (j >= out) return;
+    const float bj = b[j];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = 8 * (r / 4) + 4 * kh + (r % 4);
+        if (row < nr) {
+            const float z = acc[r] + bj;
+            Hout[(r0 + row) * out + j] = act ? tanhf(z) : z;
+        }
     }
 }
 
