@@ -2059,6 +2059,10 @@ int eb_set_learned_predictor(eb_ctx* ctx, eb_mlp* mlp, const float* in_scale, co
     EB_HIP(hipMemcpyAsync(ctx->lin_scale, in_scale, sizeof(float) * LEARN_IN, hipMemcpyHostToDevice, ctx->stream));
     EB_HIP(hipMemcpyAsync(ctx->lout_scale, out_scale, sizeof(float) * LEARN_OUT, hipMemcpyHostToDevice, ctx->stream));
     EB_HIP(hipStreamSynchronize(ctx->stream));
+    if (ebm::mlp_reserve(mlp, ctx->N) != EB_OK) {
+        ctx->err = "learned predictor: the network's inference workspace could not be allocated";
+        return EB_ERR_DEVICE;
+    }
     ctx->mlp = mlp;
     ctx->learn_backoff = ctx->learn_wait = 0;
     return EB_OK;

@@ -192,6 +192,9 @@ struct eb_mlp {
     float *x = nullptr, *t = nullptr, *dz0 = nullptr, *dz1 = nullptr, *partial = nullptr, *loss_part = nullptr;
     int32_t* idx = nullptr;
     int64_t partial_cap = 0;
+    // inference workspace (the solver's whole node set): per layer outputs only
+    int64_t infer_cap = 0;
+    std::vector<float*> iact;
     // dataset
     float *dX = nullptr, *dT = nullptr;
     int64_t n_data = 0, data_cap = 0;
@@ -243,7 +246,7 @@ int ensure_cap(eb_mlp* mlp, int64_t rows) {
     return EB_OK;
 }
 
-int forward_dev(eb_mlp* mlp, const float* X, int64_t rows) {
+int forward_dev(eb_mlp* mlp, const float* X, int64_t rows, const std::vector<float*>& act) {
     const int L = layers(mlp);
     const float* in = X;
     const unsigned grid = (unsigned)((rows + FWD_TILE - 1) / FWD_TILE);
@@ -251,8 +254,8 @@ int forward_dev(eb_mlp* mlp, const float* X, int64_t rows) {
         const float* W = mlp->w + mlp->off[l];
         const float* b = W + (int64_t)mlp->width[l] * mlp->width[l + 1];
         hipLaunchKernelGGL(mlp_forward_layer, dim3(grid), dim3(256), 0, mlp->stream, rows, mlp->width[l],
-                           mlp->width[l + 1], W, b, in, mlp->act[l], l + 1 < L ? 1 : 0);
-        in = mlp->act[l];
+                           mlp->width[l + 1], W, b, in, act[l], l + 1 < L ? 1 : 0);
+        in = act[l];
     }
     MLP_HIP(hipGetLastError());
     return EB_OK;
@@ -261,7 +264,7 @@ int forward_dev(eb_mlp* mlp, const float* X, int64_t rows) {
 // forward + loss + backward on the rows in mlp->x / mlp->t; gradient into mlp->g
 int loss_grad_dev(eb_mlp* mlp, int64_t rows, float* loss_out) {
     const int L = layers(mlp);
-    if (forward_dev(mlp, mlp->x, rows)) return EB_ERR_DEVICE;
+    if (forward_dev(mlp, mlp->x, rows, mlp->act)) return EB_ERR_DEVICE;
     const int outw = mlp->width[L];
     const int lgrid = (int)std::min<int64_t>(1024, (rows * outw + 255) / 256);
     hipLaunchKernelGGL(mlp_loss, dim3(lgrid), dim3(256), 0, mlp->stream, rows, outw, mlp->act[L - 1], mlp->t,
@@ -296,13 +299,24 @@ int loss_grad_dev(eb_mlp* mlp, int64_t rows, float* loss_out) {
 
 namespace ebm {
 
+int mlp_reserve(eb_mlp* mlp, int64_t rows) {
+    if (rows <= mlp->infer_cap) return EB_OK;
+    const int L = layers(mlp);
+    mlp->iact.assign(L, nullptr);
+    for (int l = 0; l < L; ++l)
+        if (mlp_alloc(mlp, mlp->iact[l], rows * mlp->width[l + 1])) return EB_ERR_DEVICE;
+    MLP_HIP(hipStreamSynchronize(mlp->stream));
+    mlp->infer_cap = rows;
+    return EB_OK;
+}
+
 int mlp_infer(eb_mlp* mlp, const float* X_dev, int64_t rows, hipStream_t stream, const float** Y_dev) {
-    if (ensure_cap(mlp, rows)) return EB_ERR_DEVICE;
+    if (mlp_reserve(mlp, rows)) return EB_ERR_DEVICE;
     hipStream_t keep = mlp->stream;
     mlp->stream = stream;
-    int st = forward_dev(mlp, X_dev, rows);
+    int st = forward_dev(mlp, X_dev, rows, mlp->iact);
     mlp->stream = keep;
-    *Y_dev = mlp->act[layers(mlp) - 1];
+    *Y_dev = mlp->iact[layers(mlp) - 1];
     return st;
 }
 
@@ -382,7 +396,7 @@ int eb_mlp_forward(eb_mlp* mlp, const float* X, int64_t rows, float* Y) {
     if (!mlp || !X || !Y || rows < 1) return EB_ERR_ARG;
     if (ensure_cap(mlp, rows)) return EB_ERR_DEVICE;
     MLP_HIP(hipMemcpyAsync(mlp->x, X, sizeof(float) * rows * mlp->width[0], hipMemcpyHostToDevice, mlp->stream));
-    if (forward_dev(mlp, mlp->x, rows)) return EB_ERR_DEVICE;
+    if (forward_dev(mlp, mlp->x, rows, mlp->act)) return EB_ERR_DEVICE;
     MLP_HIP(hipMemcpyAsync(Y, mlp->act[layers(mlp) - 1], sizeof(float) * rows * mlp->width.back(),
                            hipMemcpyDeviceToHost, mlp->stream));
     MLP_HIP(hipStreamSynchronize(mlp->stream));
