@@ -52,7 +52,8 @@ int eb_mlp_train(eb_mlp* mlp, int32_t epochs, int32_t batch, float lr, uint64_t 
  * X (n_nodes × 27) unscaled inputs as the predictor saw them, T (n_nodes × 6) the unscaled target
  * correction ΔD* - p. Fails until three steps have converged since eb_initialize_state_simulation. */
 int eb_learned_sample(eb_ctx* ctx, float* X, float* T);
-/* installs the network (NULL: reference predictor only); the context keeps the pointer */
+/* installs the network (NULL: reference predictor only); the context keeps the pointer, and the
+ * network's inference workspace is sized for the context's nodes here (device memory) */
 int eb_set_learned_predictor(eb_ctx* ctx, eb_mlp* mlp, const float* in_scale, const float* out_scale);
 /* A network whose start keeps losing is not evaluated at every step: after an evaluation in
  * which no instance kept the network's start, the next 1, 2, 4, ... (at most max_skip)
