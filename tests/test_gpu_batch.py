@@ -109,3 +109,28 @@ def test_batch_of_rings_follows_the_standalone_rings(precision):
             assert rel_err(b, a) <= 1e-13, (s, f, rel_err(b, a))
         alone.close()
     batch.close()
+
+
+def nets():
+    from endobeams_amd import configs
+    # nets of 8×8, 12×12 and 10×10 crossings, started in contact with the sphere: each has a first
+    # chain cover and a crossing cover, both per instance
+    return [configs.net_on_sphere(n=n, sdf_n=48, gap=0.005) for n in (8, 12, 10)]
+
+
+def test_batch_of_nets_follows_the_standalone_nets():
+    from endobeams_amd import HipBatchEngine, HipEngine, abi
+    batch = HipBatchEngine(nets(), precision=abi.EB_PRECISION_FP64)
+    sched = batch_schedule(batch, 5)
+    print("attempts (dt, accepted, k per instance):", [(dt, ok, k.tolist()) for (_, dt, ok, k) in sched], flush=True)
+    assert sum(ok for (_, _, ok, _) in sched) >= 3
+    for s, (conf, params, inter) in enumerate(nets()):
+        alone = HipEngine(conf, params, inter)
+        ka = replay(alone, sched)
+        assert ka == [int(k[s]) for (_, _, _, k) in sched], (s, ka)
+        for f in FIELDS:
+            a = alone.get(f)
+            b = batch.get(f)[batch.instance_slice(f, s)]
+            assert rel_err(b, a) <= 1e-12, (s, f, rel_err(b, a))
+        alone.close()
+    batch.close()
