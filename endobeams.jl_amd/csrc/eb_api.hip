@@ -68,10 +68,13 @@ struct eb_ctx {
     // learned warm start
     eb_mlp* mlp = nullptr;
     float *lin_scale = nullptr, *lout_scale = nullptr, *ones = nullptr, *lX = nullptr, *lT = nullptr;
-    double* hrate[3] = {};          // rates of the last three converged steps, newest first
+    double* hist = nullptr;         // rates of the last three converged steps of each instance (3 slots of 6N)
+    int32_t* newest = nullptr;      // [S] slot each instance wrote last
+    int32_t* h_newest = nullptr;    // pinned copy
+    std::vector<int32_t> hcount;    // [S] converged steps recorded (at most 3)
+    uint8_t* eligible = nullptr;    // [S] taking part in the step with three steps recorded
+    uint8_t* h_eligible = nullptr;  // pinned copy
     double* pdD = nullptr;          // the reference predictor's increment of the current step
-    int hist_count = 0;
-    double last_dt = 0;
     double* snap = nullptr;         // node fields U..DR, then D, Ḋ, D̈ of the reference predictor's start
     double* snap_l = nullptr;       // the same of the network's start
     double* rn_learned = nullptr;   // [2S]
@@ -85,6 +88,22 @@ struct eb_ctx {
     // the start of the Newton iteration has been assembled (with its tangent, without energies):
     // the first iteration forms only the residual and the energies
     bool start_assembled = false;
+    // per-instance step: coefficients of each instance's Δt, the instances taking part in the
+    // step, the predictor's solve failures, the factorisation's singular pivots, final k
+    StepCoef* coef = nullptr;       // [S] device
+    StepCoef* h_coef = nullptr;     // [S] pinned copy of what coef holds
+    std::vector<double> dts;        // [S] Δt the coefficients were computed for
+    double coef_b = 0, coef_g = 0;
+    bool coef_set = false;
+    uint64_t coef_ver = 0;          // incremented by every change of the coefficients
+    uint8_t* part = nullptr;        // [S] device
+    uint8_t* h_part = nullptr;      // [S] pinned copy
+    bool all_part = true;           // every instance takes part
+    uint8_t* pfail = nullptr;       // [S] the predictor's linear solve failed
+    int32_t* pivot_inst = nullptr;  // [S] singular pivot block in the instance's factorisation
+    uint8_t* sel = nullptr;         // [S] instance selection of commits and copy-backs
+    uint8_t* h_sel = nullptr;       // pinned copy
+    std::vector<int32_t> k_inst;    // [S] k of each instance at the end of the last corrector (0: not taking part)
     // node state
     double* node[16] = {};
     double* X0 = nullptr;
@@ -109,8 +128,9 @@ struct eb_ctx {
     // both are gathered from the same element staging with the same coefficients, bit for bit.
     struct TanTag {
         uint64_t ver = 0;           // staging version the tangent was gathered from (0: other)
-        double c1 = 0, c2 = 0, c3 = 0;
-        bool operator==(const TanTag& o) const { return ver == o.ver && c1 == o.c1 && c2 == o.c2 && c3 == o.c3; }
+        double c1 = 0;              // 1 + α
+        uint64_t cver = 0;          // version of the per-instance step coefficients (Δt, β, γ)
+        bool operator==(const TanTag& o) const { return ver == o.ver && c1 == o.c1 && cver == o.cver; }
     };
     uint64_t stage_ver = 1;         // incremented by every write of the element staging
     TanTag ktan_tag, factor_tag;    // Ktan's content; the tangent the chain factors are of
@@ -324,6 +344,40 @@ HHTc hht_consts(double dt, double b, double g) {
     return h;
 }
 
+// the coefficients of one instance's step, with the expressions the kernels took as arguments
+// when every instance shared one Δt
+StepCoef step_coef(double dt, double b, double g) {
+    StepCoef c;
+    c.dt = dt;
+    c.gbdt = g / (b * dt);
+    c.inv = 1 / (b * (dt * dt));
+    c.gob = g / b;
+    c.dt2b = dt * (2 * b - g) / (2 * b);
+    c.t2c = ((dt / 2) * (2 * b - g)) / b;
+    c.h = hht_consts(dt, b, g);
+    return c;
+}
+
+// Sets the step coefficients of every instance: Δt = dt_inst[s] (one per instance), or `dt` for
+// all when dt_inst is null. Uploads and bumps coef_ver only when something changed.
+int set_coefs(eb_ctx* ctx, const double* dt_inst, double dt, double b, double g) {
+    bool same = ctx->coef_set && b == ctx->coef_b && g == ctx->coef_g;
+    for (int64_t s = 0; s < ctx->S && same; ++s) same = ctx->dts[s] == (dt_inst ? dt_inst[s] : dt);
+    if (same) return EB_OK;
+    // the pinned table may still feed a copy in flight
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    for (int64_t s = 0; s < ctx->S; ++s) {
+        ctx->dts[s] = dt_inst ? dt_inst[s] : dt;
+        ctx->h_coef[s] = step_coef(ctx->dts[s], b, g);
+    }
+    ctx->coef_b = b;
+    ctx->coef_g = g;
+    ctx->coef_set = true;
+    ctx->coef_ver++;
+    EB_HIP(hipMemcpyAsync(ctx->coef, ctx->h_coef, sizeof(StepCoef) * ctx->S, hipMemcpyHostToDevice, ctx->stream));
+    return EB_OK;
+}
+
 int check_launch(eb_ctx* ctx, const char* what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { ctx->err = std::string(what) + ": " + hipGetErrorString(e); return EB_ERR_DEVICE; }
@@ -336,7 +390,7 @@ template <typename P> int materialize_kcm_t(eb_ctx* ctx) {
     const int bs = gather_block(ctx);
     hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz / 4, bs)), dim3(bs), 0, ctx->stream, ctx->nnz, ctx->blk_ptr,
                        ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM, ctx->K, ctx->C, ctx->M,
-                       ctx->Ktan, 0.0, 0.0, 0.0, 0, 1, active_dev(ctx));
+                       ctx->Ktan, 0.0, (const StepCoef*)ctx->coef, 0, 1, active_dev(ctx));
     return check_launch(ctx, "gather_matrices");
 }
 int materialize_kcm(eb_ctx* ctx) {
@@ -428,7 +482,9 @@ template <typename P> int launch_contact(eb_ctx* ctx, double alpha) {
 // matrices false: the vectors (and the residual) only, K, C, M and Ktan left as they are
 template <typename P> int launch_gathers(eb_ctx* ctx, bool with_contact, bool tangent, double dt, double a, double b,
                                          double g, bool residual, int energy_mask, bool matrices = true) {
-    double c1 = 1 + a, c2 = tangent ? g / (b * dt) : 0, c3 = tangent ? 1 / (b * (dt * dt)) : 0;
+    // the tangent's Δt-dependent factors come from the per-instance step coefficients
+    (void)dt; (void)b; (void)g;
+    const double c1 = 1 + a;
     prof_begin(ctx, EB_KERNEL_GATHER);
     if (matrices) {
         // inside the Newton iteration only Ktan is needed: K, C, M at n+1 are gathered from the
@@ -438,10 +494,10 @@ template <typename P> int launch_gathers(eb_ctx* ctx, bool with_contact, bool ta
         hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz / 4, bs)), dim3(bs), 0, ctx->stream, ctx->nnz,
                            ctx->blk_ptr, ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM,
                            tangent ? nullptr : ctx->K, tangent ? nullptr : ctx->C, tangent ? nullptr : ctx->M, ctx->Ktan,
-                           c1, c2, c3, tangent ? 1 : 0, tangent ? 0 : 1, active_dev(ctx));
+                           c1, (const StepCoef*)ctx->coef, tangent ? 1 : 0, tangent ? 0 : 1, active_dev(ctx));
         EB_TRY(check_launch(ctx, "gather_matrices"));
         ctx->kcm_pending = tangent;
-        if (tangent) ctx->ktan_tag = eb_ctx::TanTag{ctx->stage_ver, c1, c2, c3};
+        if (tangent) ctx->ktan_tag = eb_ctx::TanTag{ctx->stage_ver, c1, ctx->coef_ver};
         else ctx->kcm_ver = ctx->stage_ver;
         ctx->np1_alias = false;
     }
@@ -516,7 +572,7 @@ int precond_factor(eb_ctx* ctx) {
         return EB_ERR_ARG;
     }
     hipLaunchKernelGGL(chain_factor, dim3((unsigned)nch), dim3(CHAIN_THREADS), smem, ctx->stream, chain_dev(ctx), ctx->Ktan,
-                       &ctx->kall->pivot_status, L0, ctx->active);
+                       ctx->pivot_inst, L0, ctx->active);
     if (ctx->n_closures > 0) {
         // rings: the 12 columns of T⁻¹U (LDS: 12 doubles per position, at most 60 KB), then H
         const size_t cl_smem = (size_t)(12 * ctx->chains.cl_max_len) * sizeof(double);
@@ -533,7 +589,7 @@ int precond_factor(eb_ctx* ctx) {
             return EB_ERR_ARG;
         }
         hipLaunchKernelGGL(chain_factor, dim3((unsigned)ctx->dev2.n_chains), dim3(CHAIN_THREADS), smem2, ctx->stream,
-                           ctx->dev2, ctx->Ktan, &ctx->kall->pivot_status, L02, ctx->active);
+                           ctx->dev2, ctx->Ktan, ctx->pivot_inst, L02, ctx->active);
     }
     ctx->factor_tag = ctx->ktan_tag;
     prof_end(ctx, EB_KERNEL_PRECOND);
@@ -610,12 +666,15 @@ int krylov_iteration(eb_ctx* ctx) {
 int solve_start(eb_ctx* ctx, const eb_newton_params* p, bool factor = true) {
     const Segs sg = segs(ctx);
     const double tol = (p && p->krylov_tol > 0) ? p->krylov_tol : 1e-11;
-    hipLaunchKernelGGL(solve_begin, dim3(1), dim3(1), 0, ctx->stream, ctx->kall);
-    if (factor) EB_TRY(precond_factor(ctx));
+    // the pivot flags are those of the factorisation in use: cleared only when it is redone
+    if (factor) {
+        hipLaunchKernelGGL(solve_begin, dim3(1), dim3(256), 0, ctx->stream, (int)ctx->S, ctx->pivot_inst, ctx->kall);
+        EB_TRY(precond_factor(ctx));
+    }
     hipLaunchKernelGGL(krylov_init, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->is_fixed,
                        ctx->res, ctx->kx, ctx->kr, ctx->krhat, ctx->kp, ctx->kv, ctx->partials);
     hipLaunchKernelGGL(krylov_init_fin, dim3(1), dim3(FIN_BLOCK), 0, ctx->stream, ctx->partials, sg, ctx->kctl, tol,
-                       ctx->active, ctx->kall);
+                       ctx->active, ctx->pivot_inst, ctx->kall);
     return check_launch(ctx, "krylov_init");
 }
 
@@ -629,7 +688,7 @@ int solve_poll(eb_ctx* ctx) {
 int solve_finish(eb_ctx* ctx, const eb_newton_params* p, int launched) {
     const int maxit = (p && p->krylov_maxit > 0) ? p->krylov_maxit : 200;
     int batch = std::max(1, launched);
-    while (!ctx->h_kall->all_done && launched < maxit && ctx->h_kall->pivot_status == 0) {
+    while (!ctx->h_kall->all_done && launched < maxit) {
         const int m = std::min(batch, maxit - launched);
         for (int k = 0; k < m; ++k) EB_TRY(krylov_iteration(ctx));
         launched += m;
@@ -639,11 +698,18 @@ int solve_finish(eb_ctx* ctx, const eb_newton_params* p, int launched) {
     return EB_OK;
 }
 
-int solve_check(eb_ctx* ctx, int* its_out) {
+// counts the finished solve (Krylov iterations and solves)
+void solve_account(eb_ctx* ctx, int* its_out) {
     const KrylovAll& ka = *ctx->h_kall;
     ctx->counters[1] += ka.max_iters;
     ctx->counters[2] += 1;
     if (its_out) *its_out = ka.max_iters;
+}
+
+// the stand-alone solve (eb_solve_free_dofs) fails when any instance's solve failed
+int solve_check(eb_ctx* ctx, int* its_out) {
+    const KrylovAll& ka = *ctx->h_kall;
+    solve_account(ctx, its_out);
     if (ka.pivot_status != 0) { ctx->err = "singular 6x6 pivot block in the chain preconditioner"; return EB_ERR_SOLVE; }
     if (!ka.all_done || ka.fail_inst >= 0) {
         char buf[200];
@@ -656,6 +722,14 @@ int solve_check(eb_ctx* ctx, int* its_out) {
         return EB_ERR_SOLVE;
     }
     return EB_OK;
+}
+
+// Inside the Newton loops a failed solve ends the Newton iteration of its instance only: the
+// corrector records k = max_iterations + 1 for it (pfail null), the predictor flags it in pfail.
+int screen_launch(eb_ctx* ctx, int32_t kfail, uint8_t* pfail, const KrylovAll* gate) {
+    hipLaunchKernelGGL(solve_screen, dim3(1), dim3(256), 0, ctx->stream, (int)ctx->S, ctx->kctl, ctx->active, ctx->kiter,
+                       kfail, pfail, gate);
+    return check_launch(ctx, "solve_screen");
 }
 
 int scatter_launch(eb_ctx* ctx, const KrylovAll* gate) {
@@ -672,6 +746,17 @@ int solve_free(eb_ctx* ctx, const eb_newton_params* p, int* its_out, bool factor
     EB_TRY(solve_poll(ctx));
     EB_TRY(solve_finish(ctx, p, 1));
     EB_TRY(solve_check(ctx, its_out));
+    return scatter_launch(ctx, nullptr);
+}
+
+// the predictor's solve: instances whose solve fails are flagged in pfail (and leave `active`)
+int solve_free_screened(eb_ctx* ctx, const eb_newton_params* p, bool factor) {
+    EB_TRY(solve_start(ctx, p, factor));
+    EB_TRY(krylov_iteration(ctx));
+    EB_TRY(solve_poll(ctx));
+    EB_TRY(solve_finish(ctx, p, 1));
+    solve_account(ctx, nullptr);
+    EB_TRY(screen_launch(ctx, 0, ctx->pfail, nullptr));
     return scatter_launch(ctx, nullptr);
 }
 
@@ -699,57 +784,88 @@ int norms_launch(eb_ctx* ctx, const NewtonCtl& nc, int* n_active) {
     return EB_OK;
 }
 
-// node state and force vectors, n+1 → n (commit) or n → n+1 (rollback), in one launch
-int copy_state(eb_ctx* ctx, bool commit) {
+// node state and force vectors, n+1 → n (commit) or n → n+1 (rollback), in one launch; with
+// `sel` (device, per instance) only the selected instances
+int copy_state(eb_ctx* ctx, bool commit, const uint8_t* sel = nullptr) {
     ctx->start_assembled = false;
     CopyList cl{};
+    cl.sel = sel;
     int k = 0;
     int64_t nmax = 0;
-    auto add = [&](double* cur, double* old, int64_t n) {
+    auto add = [&](double* cur, double* old, int64_t n, int32_t per) {
         cl.src[k] = commit ? cur : old;
         cl.dst[k] = commit ? old : cur;
         cl.n[k] = n;
+        cl.map[k] = ctx->node_inst;
+        cl.per[k] = per;
         nmax = std::max(nmax, n);
         ++k;
     };
-    for (int f = 0; f < 8; ++f) add(ctx->node[EB_F_U + f], ctx->node[EB_F_U_N + f], (f >= 6 ? 9 : 3) * ctx->N);
-    add(ctx->Tint, ctx->Tint_n, ctx->ndofs);
-    add(ctx->Tk, ctx->Tk_n, ctx->ndofs);
-    add(ctx->Tc, ctx->Tc_n, ctx->ndofs);
-    add(ctx->fext, ctx->fext_n, ctx->ndofs);
+    for (int f = 0; f < 8; ++f) {
+        const int32_t per = f >= 6 ? 9 : 3;
+        add(ctx->node[EB_F_U + f], ctx->node[EB_F_U_N + f], per * ctx->N, per);
+    }
+    add(ctx->Tint, ctx->Tint_n, ctx->ndofs, 6);
+    add(ctx->Tk, ctx->Tk_n, ctx->ndofs, 6);
+    add(ctx->Tc, ctx->Tc_n, ctx->ndofs, 6);
+    add(ctx->fext, ctx->fext_n, ctx->ndofs, 6);
     hipLaunchKernelGGL(copy_many, dim3(std::min<unsigned>(nblk(nmax, 256), 4096), (unsigned)k), dim3(256), 0, ctx->stream, cl);
     return check_launch(ctx, "copy_state");
 }
 
-int newton_reset_launch(eb_ctx* ctx) {
-    hipLaunchKernelGGL(newton_reset, dim3(nblk(ctx->S, 256)), dim3(256), 0, ctx->stream, (int)ctx->S, ctx->active,
-                       ctx->kiter, ctx->kdone);
+// active = the instances taking part in the step; at the start of a corrector loop (`start`)
+// an instance whose predictor solve failed stays frozen with k = kfail
+int newton_reset_launch(eb_ctx* ctx, bool start = false, int32_t kfail = 0) {
+    hipLaunchKernelGGL(newton_reset, dim3(nblk(ctx->S, 256)), dim3(256), 0, ctx->stream, (int)ctx->S, ctx->part,
+                       start ? ctx->pfail : nullptr, ctx->active, ctx->kiter, ctx->kdone, kfail);
     return check_launch(ctx, "newton_reset");
 }
 
-int global_corrector_async(eb_ctx* ctx, double dt, double b, double g, const KrylovAll* gate = nullptr) {
+// Which instances take part in the next step (all when `part` is null); instances that do not
+// are frozen by every kernel of the predictor, the corrector and the step's commit.
+int set_participation(eb_ctx* ctx, const uint8_t* part) {
+    bool all = true, same = true;
+    for (int64_t s = 0; s < ctx->S; ++s) {
+        const uint8_t v = part ? (part[s] ? 1 : 0) : 1;
+        all = all && v;
+        same = same && ctx->h_part[s] == v;
+    }
+    if (same) return EB_OK;
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    for (int64_t s = 0; s < ctx->S; ++s) ctx->h_part[s] = part ? (part[s] ? 1 : 0) : 1;
+    ctx->all_part = all;
+    EB_HIP(hipMemcpyAsync(ctx->part, ctx->h_part, ctx->S, hipMemcpyHostToDevice, ctx->stream));
+    return newton_reset_launch(ctx);
+}
+
+int global_corrector_async(eb_ctx* ctx, const KrylovAll* gate = nullptr) {
     ctx->start_assembled = false;
     hipLaunchKernelGGL(global_corrector, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs,
-                       g / (b * dt), 1 / (b * (dt * dt)), ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, active_dev(ctx), gate);
+                       (const StepCoef*)ctx->coef, ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, active_dev(ctx), gate);
     return check_launch(ctx, "global_corrector");
 }
-int local_corrector_async(eb_ctx* ctx, double dt, double b, double g, const KrylovAll* gate = nullptr) {
+int local_corrector_async(eb_ctx* ctx, const KrylovAll* gate = nullptr) {
     ctx->start_assembled = false;
     hipLaunchKernelGGL(local_corrector, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx),
-                       ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, hht_consts(dt, b, g), active_dev(ctx), gate);
+                       ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, (const StepCoef*)ctx->coef, active_dev(ctx), gate);
     return check_launch(ctx, "local_corrector");
 }
-int global_predictor_async(eb_ctx* ctx, double dt, double b, double g) {
+int global_predictor_async(eb_ctx* ctx) {
     ctx->start_assembled = false;
-    hipLaunchKernelGGL(global_predictor, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs, dt, g / b,
-                       g / (b * dt), 1 / (b * (dt * dt)), dt * (2 * b - g) / (2 * b), ctx->D, ctx->Dd, ctx->Ddd, ctx->dD);
+    hipLaunchKernelGGL(global_predictor, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs,
+                       (const StepCoef*)ctx->coef, ctx->D, ctx->Dd, ctx->Ddd, ctx->dD, active_dev(ctx));
     return check_launch(ctx, "global_predictor");
 }
-int local_predictor_async(eb_ctx* ctx, double dt, double b, double g) {
+int local_predictor_async(eb_ctx* ctx) {
     ctx->start_assembled = false;
     hipLaunchKernelGGL(local_predictor, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx),
-                       ctx->D, ctx->Dd, ctx->Ddd, hht_consts(dt, b, g));
+                       ctx->D, ctx->Dd, ctx->Ddd, (const StepCoef*)ctx->coef, active_dev(ctx));
     return check_launch(ctx, "local_predictor");
+}
+int init_global_launch(eb_ctx* ctx, int predictor) {
+    hipLaunchKernelGGL(init_global, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx), ctx->D,
+                       ctx->Dd, ctx->Ddd, ctx->dD, predictor, active_dev(ctx));
+    return check_launch(ctx, "init_global");
 }
 
 // ---- one Newton iteration ------------------------------------------------------------------
@@ -767,9 +883,10 @@ int newton_body(eb_ctx* ctx, const eb_newton_params* p, double dt, bool start_do
     EB_TRY(bc_lift_launch(ctx));
     EB_TRY(solve_start(ctx, p));
     EB_TRY(krylov_iteration(ctx));
+    EB_TRY(screen_launch(ctx, p->max_iterations + 1, nullptr, ctx->kall));
     EB_TRY(scatter_launch(ctx, ctx->kall));
-    EB_TRY(global_corrector_async(ctx, dt, p->beta, p->gamma, ctx->kall));
-    EB_TRY(local_corrector_async(ctx, dt, p->beta, p->gamma, ctx->kall));
+    EB_TRY(global_corrector_async(ctx, ctx->kall));
+    EB_TRY(local_corrector_async(ctx, ctx->kall));
     EB_TRY(norms_kernels(ctx, nc, ctx->kall));
     EB_HIP(hipMemcpyAsync(ctx->h_nall, ctx->nall, sizeof(NewtonAll), hipMemcpyDeviceToHost, ctx->stream));
     EB_HIP(hipMemcpyAsync(ctx->h_kall, ctx->kall, sizeof(KrylovAll), hipMemcpyDeviceToHost, ctx->stream));
@@ -807,7 +924,7 @@ void apply_body_notes(eb_ctx* ctx, const eb_newton_params* p, double dt) {
     ctx->stage_ver++;
     ctx->counters[3]++;
     ctx->kcm_pending = true;
-    ctx->ktan_tag = eb_ctx::TanTag{ctx->stage_ver, 1 + p->alpha, p->gamma / (p->beta * dt), 1 / (p->beta * (dt * dt))};
+    ctx->ktan_tag = eb_ctx::TanTag{ctx->stage_ver, 1 + p->alpha, ctx->coef_ver};
     ctx->np1_alias = false;
     ctx->factor_tag = ctx->ktan_tag;
 }
@@ -822,7 +939,10 @@ void graph_release(eb_ctx* ctx) {
 // (K, C, M, whose buffers are exchanged at commits, are not passed to it).
 int graph_iteration(eb_ctx* ctx, const eb_newton_params* p, double dt) {
     eb_ctx::NewtonGraph& g = ctx->graph;
-    const double key[9] = {dt, p->alpha, p->beta, p->gamma, p->tol_res, p->tol_disp, (double)p->max_iterations,
+    // Δt enters through the per-instance coefficient table in device memory, not through
+    // kernel arguments: a captured iteration serves every Δt
+    (void)dt;
+    const double key[9] = {0.0, p->alpha, p->beta, p->gamma, p->tol_res, p->tol_disp, (double)p->max_iterations,
                            p->krylov_tol, (double)ctx->prof_mask};
     bool captured = false;
     BodyNotes after_capture{};
@@ -937,13 +1057,13 @@ int autotune_impl(eb_ctx* ctx, int32_t* choice, double* ms_out) {
                     if (ctx->precision == EB_PRECISION_FP32)
                         hipLaunchKernelGGL(gather_matrices<float>, dim3(nblk(ctx->nnz / 4, bs)), dim3(bs), 0, ctx->stream, ctx->nnz,
                                            ctx->blk_ptr, ctx->blk_src, (const float*)ctx->sK, (const float*)ctx->sC,
-                                           (const float*)ctx->sM, nullptr, nullptr, nullptr, ctx->tmp, t.c1, t.c2, t.c3, 1, 0,
-                                           active_dev(ctx));
+                                           (const float*)ctx->sM, nullptr, nullptr, nullptr, ctx->tmp, t.c1,
+                                           (const StepCoef*)ctx->coef, 1, 0, active_dev(ctx));
                     else
                         hipLaunchKernelGGL(gather_matrices<double>, dim3(nblk(ctx->nnz / 4, bs)), dim3(bs), 0, ctx->stream, ctx->nnz,
                                            ctx->blk_ptr, ctx->blk_src, (const double*)ctx->sK, (const double*)ctx->sC,
-                                           (const double*)ctx->sM, nullptr, nullptr, nullptr, ctx->tmp, t.c1, t.c2, t.c3, 1, 0,
-                                           active_dev(ctx));
+                                           (const double*)ctx->sM, nullptr, nullptr, nullptr, ctx->tmp, t.c1,
+                                           (const StepCoef*)ctx->coef, 1, 0, active_dev(ctx));
                     return check_launch(ctx, "gather_matrices");
                 }, &ms);
             }
@@ -1000,7 +1120,8 @@ int start_residual(eb_ctx* ctx, const eb_newton_params* p, double dt, bool tange
 // the smaller residual. The network's start is evaluated first (residual only), the reference
 // predictor's last with its tangent, so that when every instance keeps the reference start the
 // first Newton iteration finds its assembly done (start_assembled).
-int learned_predictor(eb_ctx* ctx, const eb_newton_params* p, double dt) {
+int learned_predictor(eb_ctx* ctx, const eb_newton_params* p) {
+    const double dt = 0.0;   // Δt enters through the per-instance step coefficients
     const int64_t N = ctx->N;
     EB_TRY(snapshot_start(ctx, false, nullptr, ctx->snap));
     // back to step n (the predictor starts from it), then the network's increment
@@ -1008,10 +1129,9 @@ int learned_predictor(eb_ctx* ctx, const eb_newton_params* p, double dt) {
         hipLaunchKernelGGL(copy_node_fields, dim3(nblk(N * NODE_FIELD_WIDTH[f], 256)), dim3(256), 0, ctx->stream, N,
                            NODE_FIELD_WIDTH[f], (const double*)ctx->node[EB_F_U_N + f], ctx->node[EB_F_U + f],
                            (const uint8_t*)nullptr, ctx->node_inst);
-    hipLaunchKernelGGL(init_global, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, node_fields(ctx), ctx->D, ctx->Dd,
-                       ctx->Ddd, ctx->dD, 1);
-    hipLaunchKernelGGL(learned_features, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, ctx->pdD, ctx->hrate[0],
-                       ctx->hrate[1], ctx->hrate[2], ctx->Tc_n, dt, ctx->lin_scale, ctx->lX);
+    EB_TRY(init_global_launch(ctx, 1));
+    hipLaunchKernelGGL(learned_features, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, ctx->pdD, ctx->hist,
+                       ctx->newest, ctx->node_inst, ctx->Tc_n, (const StepCoef*)ctx->coef, ctx->lin_scale, ctx->lX);
     EB_TRY(check_launch(ctx, "learned_features"));
     const float* Y = nullptr;
     if (ebm::mlp_infer(ctx->mlp, ctx->lX, N, ctx->stream, &Y) != EB_OK) {
@@ -1021,8 +1141,8 @@ int learned_predictor(eb_ctx* ctx, const eb_newton_params* p, double dt) {
     hipLaunchKernelGGL(learned_apply, dim3(nblk(6 * N, 256)), dim3(256), 0, ctx->stream, N, Y, ctx->pdD,
                        ctx->lout_scale, ctx->is_fixed, ctx->dD);
     EB_TRY(check_launch(ctx, "learned_apply"));
-    EB_TRY(global_predictor_async(ctx, dt, p->beta, p->gamma));
-    EB_TRY(local_predictor_async(ctx, dt, p->beta, p->gamma));
+    EB_TRY(global_predictor_async(ctx));
+    EB_TRY(local_predictor_async(ctx));
     EB_TRY(start_residual(ctx, p, dt, false));
     EB_HIP(hipMemcpyAsync(ctx->rn_learned, ctx->norms, sizeof(double) * 2 * ctx->S, hipMemcpyDeviceToDevice, ctx->stream));
     EB_TRY(snapshot_start(ctx, false, nullptr, ctx->snap_l));
@@ -1030,7 +1150,7 @@ int learned_predictor(eb_ctx* ctx, const eb_newton_params* p, double dt) {
     EB_TRY(snapshot_start(ctx, true, nullptr, ctx->snap));
     EB_TRY(start_residual(ctx, p, dt, true));
     hipLaunchKernelGGL(learned_choose, dim3(1), dim3(256), 0, ctx->stream, (int)ctx->S, ctx->norms, ctx->rn_learned,
-                       ctx->use_plain, ctx->use_learned, ctx->pred_counts);
+                       ctx->eligible, ctx->active, ctx->use_plain, ctx->use_learned, ctx->pred_counts);
     EB_TRY(check_launch(ctx, "learned_choose"));
     EB_HIP(hipMemcpyAsync(ctx->h_pred, ctx->pred_counts, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     EB_HIP(hipStreamSynchronize(ctx->stream));
@@ -1412,12 +1532,22 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
         CK(upload_vec(ctx, ctx->seg_cptr, cp));
         std::vector<uint8_t> on((size_t)S, 1);
         CK(upload_vec(ctx, ctx->active, on));
+        CK(upload_vec(ctx, ctx->part, on));
+        CK(dalloc(ctx, ctx->pfail, S));
+        CK(dalloc(ctx, ctx->pivot_inst, S));
+        CK(dalloc(ctx, ctx->sel, S));
+        CK(dalloc(ctx, ctx->coef, S));
+        CK(dalloc(ctx, ctx->newest, S));
+        CK(dalloc(ctx, ctx->eligible, S));
+        ctx->dts.assign((size_t)S, 0.0);
+        ctx->k_inst.assign((size_t)S, 0);
+        ctx->hcount.assign((size_t)S, 0);
         CK(dalloc(ctx, ctx->kiter, S));
         CK(dalloc(ctx, ctx->kdone, 1));
         CK(dalloc(ctx, ctx->nall, 1));
         CK(dalloc(ctx, ctx->kall, 1));
     }
-    for (int k = 0; k < 3; ++k) CK(dalloc(ctx, ctx->hrate[k], 6 * N));
+    CK(dalloc(ctx, ctx->hist, 18 * N));
     CK(dalloc(ctx, ctx->pdD, 6 * N));
     CK(dalloc(ctx, ctx->snap, 54 * N));
     CK(dalloc(ctx, ctx->snap_l, 54 * N));
@@ -1443,9 +1573,20 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
         hipHostMalloc((void**)&ctx->h_kall, sizeof(KrylovAll), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_nall, sizeof(NewtonAll), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_kiter, sizeof(int32_t) * S, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&ctx->h_pred, 3 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc((void**)&ctx->h_pred, 3 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_coef, sizeof(StepCoef) * S, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_part, S, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_sel, S, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_eligible, S, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_newest, sizeof(int32_t) * S, hipHostMallocDefault) != hipSuccess)
         return bail(EB_ERR_DEVICE);
     for (int k = 0; k < 3; ++k) ctx->h_pred[k] = 0;
+    for (int64_t s = 0; s < S; ++s) {
+        ctx->h_part[s] = 1;
+        ctx->h_sel[s] = 0;
+        ctx->h_eligible[s] = 0;
+        ctx->h_newest[s] = 0;
+    }
     // contact
     ctx->surface = d->surface;
     if (d->surface != EB_SURFACE_NONE) {
@@ -1504,7 +1645,15 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
         CK(dalloc(ctx, ctx->ch_yw, 6 * N));
         ctx->n_closures = (int32_t)c.cl_chain.size();
         if (ebh::chains_left_out(c, P) > 0) {
-            ebh::build_chains(ctx->chains2, P, d->X0, 0, &c);
+            // the crossing cover holds the nodes of the instances that have couplings the first
+            // cover leaves out (nets), and no node of the others, whose solve is complete without it
+            const std::vector<uint8_t> left = ebh::nodes_left_out(c, P);
+            std::vector<uint8_t> inst_left((size_t)S, 0), cover((size_t)N, 0);
+            for (int64_t s = 0; s < S; ++s)
+                for (int64_t n = noff[s]; n < noff[s + 1]; ++n) inst_left[s] |= left[n];
+            for (int64_t s = 0; s < S; ++s)
+                for (int64_t n = noff[s]; n < noff[s + 1]; ++n) cover[n] = inst_left[s];
+            ebh::build_chains(ctx->chains2, P, d->X0, 0, &c, &cover);
             const ebh::Chains& c2 = ctx->chains2;
             std::vector<int32_t> chi2(c2.ptr.size() - 1);
             for (size_t k = 0; k + 1 < c2.ptr.size(); ++k) {
@@ -1579,6 +1728,8 @@ int eb_destroy(eb_ctx* ctx) {
     if (ctx->h_nall) (void)hipHostFree(ctx->h_nall);
     if (ctx->h_pred) (void)hipHostFree(ctx->h_pred);
     if (ctx->h_kiter) (void)hipHostFree(ctx->h_kiter);
+    for (void* q : {(void*)ctx->h_coef, (void*)ctx->h_part, (void*)ctx->h_sel, (void*)ctx->h_eligible, (void*)ctx->h_newest})
+        if (q) (void)hipHostFree(q);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return EB_OK;
@@ -1603,7 +1754,7 @@ int eb_get_instances(const eb_ctx* ctx, int64_t* n_instances, int64_t* node_offs
 
 int eb_get_instance_iterations(const eb_ctx* ctx, int32_t* k_out) {
     if (!ctx || !k_out) return EB_ERR_ARG;
-    for (int64_t s = 0; s < ctx->S; ++s) k_out[s] = ctx->h_kiter[s];
+    for (int64_t s = 0; s < ctx->S; ++s) k_out[s] = ctx->k_inst[s];
     return EB_OK;
 }
 
@@ -1694,8 +1845,9 @@ int eb_tangent_residuals_corrector(eb_ctx* ctx, double dt, double a, double b, d
     if (!ctx) return EB_ERR_ARG;
     ctx->start_assembled = false;
     EB_TRY(unalias(ctx));
+    EB_TRY(set_coefs(ctx, nullptr, dt, b, g));
     hipLaunchKernelGGL(combine_tangent, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz, ctx->K, ctx->C,
-                       ctx->M, ctx->Ktan, 1 + a, g / (b * dt), 1 / (b * (dt * dt)));
+                       ctx->M, ctx->Ktan, 1 + a, (const StepCoef*)ctx->coef, ctx->blk_inst);
     ctx->ktan_tag = eb_ctx::TanTag{};
     hipLaunchKernelGGL(residual_corrector, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs, a,
                        ctx->fext, ctx->Tc, ctx->Tint, ctx->fext_n, ctx->Tc_n, ctx->Tint_n, ctx->Tk, ctx->res);
@@ -1704,17 +1856,16 @@ int eb_tangent_residuals_corrector(eb_ctx* ctx, double dt, double a, double b, d
     return EB_OK;
 }
 
-// `form_tangent` false: Ktan already holds this tangent (see predictor_reuses_factor)
-static int tangent_residuals_predictor_async(eb_ctx* ctx, double dt, double a, double b, double g,
-                                             bool form_tangent = true) {
-    double c3 = 1 / (b * (dt * dt));
+// the predictor's tangent and residual with the step coefficients in place; `form_tangent`
+// false: Ktan already holds this tangent (see predictor_reuses_factor)
+static int tangent_residuals_predictor_async(eb_ctx* ctx, double a, bool form_tangent = true) {
     if (form_tangent) {
         hipLaunchKernelGGL(combine_tangent, dim3(nblk(ctx->nnz, 256)), dim3(256), 0, ctx->stream, ctx->nnz, ctx->Kn,
-                           ctx->Cn, ctx->Mn, ctx->Ktan, 1 + a, g / (b * dt), c3);
+                           ctx->Cn, ctx->Mn, ctx->Ktan, 1 + a, (const StepCoef*)ctx->coef, ctx->blk_inst);
         ctx->ktan_tag = eb_ctx::TanTag{};
     }
     hipLaunchKernelGGL(residual_predictor, dim3(nblk(ctx->ndofs, 256)), dim3(256), 0, ctx->stream, ctx->ndofs, ctx->bptr,
-                       ctx->bcol, a, g / b, ((dt / 2) * (2 * b - g)) / b, dt, c3, ctx->fext, ctx->Tint_n, ctx->Tk_n,
+                       ctx->bcol, a, (const StepCoef*)ctx->coef, ctx->node_inst, ctx->fext, ctx->Tint_n, ctx->Tk_n,
                        ctx->fext_n, ctx->Dd, ctx->Ddd, ctx->Cn, ctx->Mn, ctx->res);
     return check_launch(ctx, "tangent_residuals_predictor");
 }
@@ -1722,7 +1873,8 @@ static int tangent_residuals_predictor_async(eb_ctx* ctx, double dt, double a, d
 int eb_tangent_residuals_predictor(eb_ctx* ctx, double dt, double a, double b, double g) {
     if (!ctx) return EB_ERR_ARG;
     ctx->start_assembled = false;
-    EB_TRY(tangent_residuals_predictor_async(ctx, dt, a, b, g));
+    EB_TRY(set_coefs(ctx, nullptr, dt, b, g));
+    EB_TRY(tangent_residuals_predictor_async(ctx, a));
     EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
 }
@@ -1749,9 +1901,7 @@ int eb_solve_free_dofs(eb_ctx* ctx, const eb_newton_params* p, int* its) {
 int eb_initialize_global_corrector(eb_ctx* ctx) {
     if (!ctx) return EB_ERR_ARG;
     ctx->start_assembled = false;
-    hipLaunchKernelGGL(init_global, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx), ctx->D,
-                       ctx->Dd, ctx->Ddd, ctx->dD, 0);
-    EB_TRY(check_launch(ctx, "init_global"));
+    EB_TRY(init_global_launch(ctx, 0));
     EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
 }
@@ -1759,34 +1909,36 @@ int eb_initialize_global_corrector(eb_ctx* ctx) {
 int eb_initialize_global_predictor(eb_ctx* ctx) {
     if (!ctx) return EB_ERR_ARG;
     ctx->start_assembled = false;
-    hipLaunchKernelGGL(init_global, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx), ctx->D,
-                       ctx->Dd, ctx->Ddd, ctx->dD, 1);
-    EB_TRY(check_launch(ctx, "init_global"));
+    EB_TRY(init_global_launch(ctx, 1));
     EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
 }
 
 int eb_compute_global_corrector(eb_ctx* ctx, double dt, double b, double g) {
     if (!ctx) return EB_ERR_ARG;
-    EB_TRY(global_corrector_async(ctx, dt, b, g));
+    EB_TRY(set_coefs(ctx, nullptr, dt, b, g));
+    EB_TRY(global_corrector_async(ctx));
     EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
 }
 int eb_compute_global_predictor(eb_ctx* ctx, double dt, double b, double g) {
     if (!ctx) return EB_ERR_ARG;
-    EB_TRY(global_predictor_async(ctx, dt, b, g));
+    EB_TRY(set_coefs(ctx, nullptr, dt, b, g));
+    EB_TRY(global_predictor_async(ctx));
     EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
 }
 int eb_compute_local_corrector(eb_ctx* ctx, double dt, double b, double g) {
     if (!ctx) return EB_ERR_ARG;
-    EB_TRY(local_corrector_async(ctx, dt, b, g));
+    EB_TRY(set_coefs(ctx, nullptr, dt, b, g));
+    EB_TRY(local_corrector_async(ctx));
     EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
 }
 int eb_compute_local_predictor(eb_ctx* ctx, double dt, double b, double g) {
     if (!ctx) return EB_ERR_ARG;
-    EB_TRY(local_predictor_async(ctx, dt, b, g));
+    EB_TRY(set_coefs(ctx, nullptr, dt, b, g));
+    EB_TRY(local_predictor_async(ctx));
     EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
 }
@@ -1840,7 +1992,7 @@ int eb_update_boundary_conditions(eb_ctx* ctx, const double* values, int64_t n_f
 int eb_initialize_state_simulation(eb_ctx* ctx) {
     if (!ctx) return EB_ERR_ARG;
     ctx->start_assembled = false;
-    ctx->hist_count = 0;
+    for (int64_t s = 0; s < ctx->S; ++s) ctx->hcount[s] = 0;
     EB_TRY(assemble_all(ctx, false, 0.0, false, 1.0, 1.0, 1.0));
     size_t b = sizeof(double) * ctx->nnz;
     EB_HIP(hipMemcpyAsync(ctx->Kn, ctx->K, b, hipMemcpyDeviceToDevice, ctx->stream));
@@ -1851,32 +2003,55 @@ int eb_initialize_state_simulation(eb_ctx* ctx) {
 }
 
 // The predictor's tangent from Kⁿ, Cⁿ, Mⁿ equals the tangent of the last corrector iteration when
-// both were gathered from the same element staging with the same HHT coefficients (the step
-// size is unchanged): then Ktan and the chain factors of that iteration are reused as they are.
-bool predictor_reuses_factor(const eb_ctx* ctx, double dt, double a, double b, double g) {
-    const eb_ctx::TanTag want{ctx->kcm_n_ver, 1 + a, g / (b * dt), 1 / (b * (dt * dt))};
-    return ctx->kcm_n_ver != 0 && ctx->ktan_tag == want && ctx->factor_tag == want;
+// both were gathered from the same element staging with the same HHT coefficients (every
+// instance's step size unchanged): then Ktan and the chain factors of that iteration are reused
+// as they are. Only when every instance took part in that iteration (the factors of the others
+// are older) and committed its step (kcm_n_ver is 0 after a step some instances repeated).
+static bool predictor_reuses_factor(const eb_ctx* ctx, double a) {
+    const eb_ctx::TanTag want{ctx->kcm_n_ver, 1 + a, ctx->coef_ver};
+    return ctx->all_part && ctx->kcm_n_ver != 0 && ctx->ktan_tag == want && ctx->factor_tag == want;
 }
 
-int eb_predictor(eb_ctx* ctx, const eb_newton_params* p, double dt) {
-    if (!ctx || !p) return EB_ERR_ARG;
-    ctx->last_dt = dt;
-    hipLaunchKernelGGL(init_global, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx), ctx->D,
-                       ctx->Dd, ctx->Ddd, ctx->dD, 1);
-    const bool reuse = predictor_reuses_factor(ctx, dt, p->alpha, p->beta, p->gamma);
-    EB_TRY(tangent_residuals_predictor_async(ctx, dt, p->alpha, p->beta, p->gamma, !reuse));
+// uploads which instances may take the network's start (taking part, three steps recorded);
+// false when none may
+static int learned_eligibility(eb_ctx* ctx, bool* any) {
+    *any = false;
+    bool same = true;
+    for (int64_t s = 0; s < ctx->S; ++s) {
+        const uint8_t e = (ctx->h_part[s] && ctx->hcount[s] >= 3) ? 1 : 0;
+        *any = *any || e;
+        same = same && ctx->h_eligible[s] == e;
+    }
+    if (same || !*any) return EB_OK;
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    for (int64_t s = 0; s < ctx->S; ++s) ctx->h_eligible[s] = (ctx->h_part[s] && ctx->hcount[s] >= 3) ? 1 : 0;
+    EB_HIP(hipMemcpyAsync(ctx->eligible, ctx->h_eligible, ctx->S, hipMemcpyHostToDevice, ctx->stream));
+    return EB_OK;
+}
+
+// predictor! (src/predictor.jl:1-27) for the instances taking part, with their step coefficients
+// in place; an instance whose linear solve fails is flagged in pfail (its corrector reports
+// k = max_iterations + 1)
+static int predictor_impl(eb_ctx* ctx, const eb_newton_params* p) {
+    EB_HIP(hipMemsetAsync(ctx->pfail, 0, ctx->S, ctx->stream));
+    EB_TRY(newton_reset_launch(ctx));   // active = the instances taking part
+    EB_TRY(init_global_launch(ctx, 1));
+    const bool reuse = predictor_reuses_factor(ctx, p->alpha);
+    EB_TRY(tangent_residuals_predictor_async(ctx, p->alpha, !reuse));
     EB_TRY(bc_lift_launch(ctx));
-    EB_TRY(solve_free(ctx, p, nullptr, !reuse));
-    EB_TRY(global_predictor_async(ctx, dt, p->beta, p->gamma));
-    EB_TRY(local_predictor_async(ctx, dt, p->beta, p->gamma));
+    EB_TRY(solve_free_screened(ctx, p, !reuse));
+    EB_TRY(global_predictor_async(ctx));
+    EB_TRY(local_predictor_async(ctx));
     // the reference predictor's increment: an input of the learned start and of its samples
     EB_HIP(hipMemcpyAsync(ctx->pdD, ctx->dD, sizeof(double) * ctx->ndofs, hipMemcpyDeviceToDevice, ctx->stream));
-    if (ctx->mlp && ctx->hist_count >= 3) {
+    bool any = false;
+    if (ctx->mlp) EB_TRY(learned_eligibility(ctx, &any));
+    if (any) {
         if (ctx->learn_wait > 0) {
             ctx->learn_wait--;
             ctx->learn_skipped++;
         } else {
-            EB_TRY(learned_predictor(ctx, p, dt));
+            EB_TRY(learned_predictor(ctx, p));
             if (ctx->h_pred[2] == 0) {
                 ctx->learn_backoff = std::min(std::max(1, 2 * ctx->learn_backoff), ctx->learn_max_skip);
             } else {
@@ -1889,17 +2064,26 @@ int eb_predictor(eb_ctx* ctx, const eb_newton_params* p, double dt) {
     return EB_OK;
 }
 
-int eb_corrector(eb_ctx* ctx, const eb_newton_params* p, double dt, int* kout) {
-    if (!ctx || !p || !kout) return EB_ERR_ARG;
-    hipLaunchKernelGGL(init_global, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, node_fields(ctx), ctx->D,
-                       ctx->Dd, ctx->Ddd, ctx->dD, 0);
-    EB_TRY(check_launch(ctx, "init_global"));
-    EB_TRY(newton_reset_launch(ctx));
+int eb_predictor(eb_ctx* ctx, const eb_newton_params* p, double dt) {
+    if (!ctx || !p) return EB_ERR_ARG;
+    EB_TRY(set_participation(ctx, nullptr));
+    EB_TRY(set_coefs(ctx, nullptr, dt, p->beta, p->gamma));
+    return predictor_impl(ctx, p);
+}
+
+// corrector! (src/corrector.jl:1-59) for the instances taking part; ctx->k_inst receives each
+// instance's k (0 for instances not taking part), *kout the largest
+static int corrector_impl(eb_ctx* ctx, const eb_newton_params* p, int* kout) {
+    const int32_t kfail = p->max_iterations + 1;
+    EB_TRY(init_global_launch(ctx, 0));
+    EB_TRY(newton_reset_launch(ctx, true, kfail));
     // corrector.jl:17-36 per instance: an instance leaves the loop when its norms pass or its
     // iterations are used up; the others keep iterating with it frozen
-    int k = 1, n_active = (int)ctx->S;
+    int k = 1, n_active = 0;
+    for (int64_t s = 0; s < ctx->S; ++s) n_active += ctx->h_part[s] ? 1 : 0;
     int st = EB_OK;
     const NewtonCtl nc{p->tol_res, p->tol_disp, 0, p->max_iterations, 1};
+    const double dt = ctx->dts.empty() ? 0.0 : ctx->dts[0];
     while (n_active > 0 && k <= p->max_iterations) {
         // assembly, lift, factorisation and one BiCGStab iteration, then the update and norms
         // behind the solve's all-done flag: one synchronisation when one iteration solves (the
@@ -1915,10 +2099,11 @@ int eb_corrector(eb_ctx* ctx, const eb_newton_params* p, double dt, int* kout) {
         if (st == EB_OK && graph) prof_harvest_graph(ctx);
         if (st == EB_OK && !ctx->h_kall->all_done) {
             st = solve_finish(ctx, p, 1);
-            if (st == EB_OK) st = solve_check(ctx, nullptr);
+            if (st == EB_OK) solve_account(ctx, nullptr);
+            if (st == EB_OK) st = screen_launch(ctx, kfail, nullptr, nullptr);
             if (st == EB_OK) st = scatter_launch(ctx, nullptr);
-            if (st == EB_OK) st = global_corrector_async(ctx, dt, p->beta, p->gamma);
-            if (st == EB_OK) st = local_corrector_async(ctx, dt, p->beta, p->gamma);
+            if (st == EB_OK) st = global_corrector_async(ctx);
+            if (st == EB_OK) st = local_corrector_async(ctx);
             if (st == EB_OK) st = norms_kernels(ctx, nc, nullptr);
             if (st == EB_OK) {
                 if (hipMemcpyAsync(ctx->h_nall, ctx->nall, sizeof(NewtonAll), hipMemcpyDeviceToHost, ctx->stream) !=
@@ -1927,7 +2112,7 @@ int eb_corrector(eb_ctx* ctx, const eb_newton_params* p, double dt, int* kout) {
                     st = EB_ERR_DEVICE;
             }
         } else if (st == EB_OK) {
-            st = solve_check(ctx, nullptr);
+            solve_account(ctx, nullptr);
         }
         if (st != EB_OK) break;
         if (!ctx->ev_pending.empty()) prof_harvest(ctx);
@@ -1939,14 +2124,26 @@ int eb_corrector(eb_ctx* ctx, const eb_newton_params* p, double dt, int* kout) {
         EB_HIP(hipMemcpyAsync(ctx->h_kiter, ctx->kiter, sizeof(int32_t) * ctx->S, hipMemcpyDeviceToHost, ctx->stream));
         EB_HIP(hipStreamSynchronize(ctx->stream));
     }
-    // every instance takes part in whatever runs outside the corrector loop
+    // every instance taking part in the step takes part in whatever runs outside the corrector
+    // loop; the predictor's failure flags are spent
     int st2 = newton_reset_launch(ctx);
+    if (st2 == EB_OK && hipMemsetAsync(ctx->pfail, 0, ctx->S, ctx->stream) != hipSuccess) st2 = EB_ERR_DEVICE;
     if (st != EB_OK) return st;
     EB_TRY(st2);
     int kmax = 0;
-    for (int64_t s = 0; s < ctx->S; ++s) kmax = std::max(kmax, ctx->h_kiter[s] > 0 ? ctx->h_kiter[s] : k);
+    for (int64_t s = 0; s < ctx->S; ++s) {
+        ctx->k_inst[s] = !ctx->h_part[s] ? 0 : (ctx->h_kiter[s] > 0 ? ctx->h_kiter[s] : k);
+        kmax = std::max(kmax, ctx->k_inst[s]);
+    }
     *kout = kmax;
     return EB_OK;
+}
+
+int eb_corrector(eb_ctx* ctx, const eb_newton_params* p, double dt, int* kout) {
+    if (!ctx || !p || !kout) return EB_ERR_ARG;
+    EB_TRY(set_participation(ctx, nullptr));
+    EB_TRY(set_coefs(ctx, nullptr, dt, p->beta, p->gamma));
+    return corrector_impl(ctx, p, kout);
 }
 
 int eb_solve_step_dynamics(eb_ctx* ctx, const eb_newton_params* p, double dt, int* k) {
@@ -1999,17 +2196,27 @@ int eb_run_steps(eb_ctx* ctx, const eb_newton_params* p, const eb_time_params* t
     return EB_OK;
 }
 
+// The rates of the step just converged into the learned start's history of the instances in
+// `sel_host` (all when null); `sel_dev` is its device copy
+static int record_history(eb_ctx* ctx, const uint8_t* sel_host, const uint8_t* sel_dev) {
+    if (!ctx->coef_set) return EB_OK;
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    for (int64_t s = 0; s < ctx->S; ++s) {
+        if (sel_host && !sel_host[s]) continue;
+        ctx->h_newest[s] = (ctx->h_newest[s] + 1) % 3;
+        ctx->hcount[s] = std::min(ctx->hcount[s] + 1, 3);
+    }
+    EB_HIP(hipMemcpyAsync(ctx->newest, ctx->h_newest, sizeof(int32_t) * ctx->S, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(learned_record, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, ctx->node[EB_F_U],
+                       ctx->node[EB_F_U_N], ctx->node[EB_F_W], (const StepCoef*)ctx->coef, ctx->node_inst, sel_dev,
+                       ctx->newest, ctx->hist);
+    return check_launch(ctx, "learned_record");
+}
+
 int eb_update_converged(eb_ctx* ctx) {
     if (!ctx) return EB_ERR_ARG;
     // rate of the step that converged, newest first in the history of the learned warm start
-    if (ctx->last_dt > 0) {
-        std::swap(ctx->hrate[1], ctx->hrate[2]);
-        std::swap(ctx->hrate[0], ctx->hrate[1]);
-        hipLaunchKernelGGL(learned_record, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N,
-                           ctx->node[EB_F_U], ctx->node[EB_F_U_N], ctx->node[EB_F_W], 1.0 / ctx->last_dt, ctx->hrate[0]);
-        EB_TRY(check_launch(ctx, "learned_record"));
-        ctx->hist_count = std::min(ctx->hist_count + 1, 3);
-    }
+    EB_TRY(record_history(ctx, nullptr, nullptr));
     EB_TRY(copy_state(ctx, true));
     EB_TRY(materialize_kcm(ctx));
     if (!ctx->np1_alias) {
@@ -2034,12 +2241,146 @@ int eb_update_not_converged(eb_ctx* ctx) {
     return EB_OK;
 }
 
+// End of a batch step in which the instances flagged in `commit` converged and the others taking
+// part repeat it: update_converged! for the first, update_not_converged! for the second
+// (src/utils_solver.jl:252-304), instance by instance. Instances not taking part are unchanged.
+static int step_end_mixed(eb_ctx* ctx, const uint8_t* commit) {
+    int64_t n_commit = 0, n_part = 0;
+    for (int64_t s = 0; s < ctx->S; ++s) {
+        n_commit += commit[s] ? 1 : 0;
+        n_part += ctx->h_part[s] ? 1 : 0;
+    }
+    if (n_commit == ctx->S) return eb_update_converged(ctx);
+    if (n_commit == 0) {
+        // only instances taking part changed their n+1 state
+        EB_TRY(copy_state(ctx, false, ctx->part));
+        ctx->kcm_pending = false;
+        ctx->np1_alias = true;
+        ctx->kcm_ver = ctx->kcm_n_ver;
+        EB_HIP(hipStreamSynchronize(ctx->stream));
+        return EB_OK;
+    }
+    (void)n_part;
+    // selection: the committing instances
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    for (int64_t s = 0; s < ctx->S; ++s) ctx->h_sel[s] = commit[s] ? 1 : 0;
+    EB_HIP(hipMemcpyAsync(ctx->sel, ctx->h_sel, ctx->S, hipMemcpyHostToDevice, ctx->stream));
+    EB_TRY(record_history(ctx, ctx->h_sel, ctx->sel));
+    EB_TRY(copy_state(ctx, true, ctx->sel));
+    // K, C, M at n+1 of the committing instances; the others take their step-n matrices back
+    // before the exchange of buffers
+    EB_TRY(materialize_kcm(ctx));
+    if (ctx->np1_alias) EB_TRY(unalias(ctx));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    for (int64_t s = 0; s < ctx->S; ++s) ctx->h_sel[s] = commit[s] ? 0 : 1;
+    EB_HIP(hipMemcpyAsync(ctx->sel, ctx->h_sel, ctx->S, hipMemcpyHostToDevice, ctx->stream));
+    EB_TRY(copy_state(ctx, false, ctx->sel));   // rollback (a no-op for instances not taking part)
+    {
+        CopyList cl{};
+        cl.sel = ctx->sel;
+        double* src[3] = {ctx->Kn, ctx->Cn, ctx->Mn};
+        double* dst[3] = {ctx->K, ctx->C, ctx->M};
+        for (int k = 0; k < 3; ++k) {
+            cl.src[k] = src[k];
+            cl.dst[k] = dst[k];
+            cl.n[k] = ctx->nnz;
+            cl.map[k] = ctx->blk_inst;
+            cl.per[k] = 36;
+        }
+        hipLaunchKernelGGL(copy_many, dim3(std::min<unsigned>(nblk(ctx->nnz, 256), 4096), 3), dim3(256), 0, ctx->stream, cl);
+        EB_TRY(check_launch(ctx, "copy_back_matrices"));
+    }
+    std::swap(ctx->K, ctx->Kn);
+    std::swap(ctx->C, ctx->Cn);
+    std::swap(ctx->M, ctx->Mn);
+    ctx->np1_alias = true;
+    ctx->kcm_pending = false;
+    // the step-n matrices now come from two element stagings: no predictor reuses a factorisation
+    ctx->kcm_n_ver = 0;
+    ctx->kcm_ver = 0;
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
+}
+
+int eb_run_steps_batch(eb_ctx* ctx, const eb_newton_params* p, const eb_time_params* tp, int64_t max_steps,
+                       eb_instance_callback callback, void* user, eb_run_state* states) {
+    if (!ctx || !p || !tp || !states || max_steps < 0) return EB_ERR_ARG;
+    if (tp->stop_on_energy_threshold) {
+        ctx->err = "eb_run_steps_batch: the energy stop needs per-instance kinetic energies, which the batch does not keep";
+        return EB_ERR_ARG;
+    }
+    const int64_t S = ctx->S;
+    std::vector<int64_t> done((size_t)S, 0);
+    std::vector<uint8_t> part((size_t)S, 0), commit((size_t)S, 0);
+    std::vector<double> dt_inst((size_t)S, 0.0);
+    int st = EB_OK;
+    for (;;) {
+        int64_t n_part = 0;
+        for (int64_t s = 0; s < S; ++s) {
+            const eb_run_state& r = states[s];
+            part[s] = (!r.stopped && done[s] < max_steps && tp->simulation_end_time - r.t > 1.0e-10) ? 1 : 0;
+            n_part += part[s];
+            dt_inst[s] = r.dt > 0 ? r.dt : tp->initial_timestep;
+        }
+        if (n_part == 0) break;
+        for (int64_t s = 0; s < S && callback; ++s) {
+            if (!part[s]) continue;
+            const double tnp1 = states[s].t + states[s].dt;
+            if (callback(ctx, s, tnp1, user) != 0) {
+                ctx->err = "step callback failed for instance " + std::to_string(s) + " at t = " + std::to_string(tnp1);
+                st = EB_ERR_ARG;
+                break;
+            }
+        }
+        if (st != EB_OK) break;
+        if ((st = set_participation(ctx, part.data())) != EB_OK) break;
+        if ((st = set_coefs(ctx, dt_inst.data(), 0.0, p->beta, p->gamma)) != EB_OK) break;
+        int kmax = 0;
+        if ((st = predictor_impl(ctx, p)) != EB_OK) break;
+        if ((st = corrector_impl(ctx, p, &kmax)) != EB_OK) break;
+        // run_simulation.jl:53-84 per instance
+        for (int64_t s = 0; s < S; ++s) {
+            commit[s] = 0;
+            if (!part[s]) continue;
+            eb_run_state& r = states[s];
+            const int k = ctx->k_inst[s];
+            r.last_k = k;
+            if (k > p->max_iterations) {
+                r.dt /= 2;
+                r.failed_steps += 1;
+                r.successive = 0;
+                if (r.dt < tp->min_timestep) r.stopped = 2;   // Δt below min_timestep: this instance aborts
+                continue;
+            }
+            commit[s] = 1;
+            const double tnp1 = r.t + r.dt;
+            r.successive += 1;
+            r.steps += 1;
+            r.newton_iterations += k - 1;
+            done[s] += 1;
+            if (r.successive >= tp->accelerate_after_success_it && k <= 8) {
+                r.dt = std::min(r.dt * 1.5, tp->max_timestep);
+                r.successive = 0;
+            }
+            r.t = tnp1;
+        }
+        if ((st = step_end_mixed(ctx, commit.data())) != EB_OK) break;
+    }
+    // every instance takes part in what the caller runs next
+    int st2 = set_participation(ctx, nullptr);
+    return st != EB_OK ? st : st2;
+}
+
 int eb_learned_sample(eb_ctx* ctx, float* X, float* T) {
     if (!ctx || !X || !T) return EB_ERR_ARG;
-    if (ctx->hist_count < 3 || ctx->last_dt <= 0) { ctx->err = "fewer than three converged steps recorded"; return EB_ERR_STATE; }
+    for (int64_t s = 0; s < ctx->S; ++s)
+        if (ctx->hcount[s] < 3 || !ctx->coef_set) {
+            ctx->err = "fewer than three converged steps recorded";
+            return EB_ERR_STATE;
+        }
     const int64_t N = ctx->N;
-    hipLaunchKernelGGL(learned_features, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, ctx->pdD, ctx->hrate[0],
-                       ctx->hrate[1], ctx->hrate[2], ctx->Tc_n, ctx->last_dt, ctx->ones, ctx->lX);
+    hipLaunchKernelGGL(learned_features, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, ctx->pdD, ctx->hist,
+                       ctx->newest, ctx->node_inst, ctx->Tc_n, (const StepCoef*)ctx->coef, ctx->ones, ctx->lX);
     hipLaunchKernelGGL(learned_targets, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, ctx->node[EB_F_U],
                        ctx->node[EB_F_U_N], ctx->node[EB_F_W], ctx->pdD, ctx->ones, ctx->lT);
     EB_TRY(check_launch(ctx, "learned_sample"));

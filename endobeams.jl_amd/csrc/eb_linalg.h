@@ -130,6 +130,21 @@ struct Active {
     const int32_t* blk_inst;   // [nnz/36]
 };
 
+// Per-instance coefficients of the HHT tangent and the Newmark/rotation updates with the
+// instance's own Δt (β, γ shared by the batch), computed on the host with the expressions of
+// utils_solver.jl:54-245 and read through the instance tables, so that instances of one batch may
+// advance with different time steps.
+struct HHTc { double c_w, c_wd, c_wdd, d_w, d_wd, d_wdd; };  // ẇ = ΔR(c_w w + c_wd ẇⁿ + c_wdd ẅⁿ), ẅ likewise
+struct StepCoef {
+    double dt;
+    double gbdt;   // γ/(βΔt): the tangent's C factor, Ḋ update
+    double inv;    // 1/(βΔt²): the tangent's M factor, D̈ update
+    double gob;    // γ/β: predictor velocity terms
+    double dt2b;   // Δt(2β - γ)/(2β): global predictor
+    double t2c;    // ((Δt/2)(2β - γ))/β: predictor residual
+    HHTc h;        // local (rotation) updates
+};
+
 // ---- assembly gathers (atomic-free, beam order = the reference's serial scatter order) ------
 
 // four consecutive staging values (16-byte aligned: a block's 36 entries start on a multiple of
@@ -154,8 +169,9 @@ template <typename P>
 __global__ void __launch_bounds__(256) gather_matrices(int64_t nnz, const int32_t* __restrict__ blk_ptr,
                                                        const uint32_t* __restrict__ blk_src, const P* __restrict__ sK,
                                                        const P* __restrict__ sC, const P* __restrict__ sM, double* K,
-                                                       double* C, double* M, double* Ktan, double c1, double c2,
-                                                       double c3, int with_tangent, int write_kcm, Active act) {
+                                                       double* C, double* M, double* Ktan, double c1,
+                                                       const StepCoef* __restrict__ coef, int with_tangent, int write_kcm,
+                                                       Active act) {
     const int64_t q = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= nnz) return;
     const int64_t b = q / 36;
@@ -184,6 +200,8 @@ __global__ void __launch_bounds__(256) gather_matrices(int64_t nnz, const int32_
         st4(M + q, m);
     }
     if (with_tangent) {
+        const StepCoef& sc = coef[act.blk_inst[b]];
+        const double c2 = sc.gbdt, c3 = sc.inv;
         double t[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) t[j] = (c1 * k[j] + c2 * cc[j]) + c3 * m[j];
@@ -242,9 +260,12 @@ __global__ void __launch_bounds__(RED_BLOCK) energy_finalize(const double* parti
 // ---- HHT combine and residuals (src/utils_solver.jl:54-89) ----------------------------------
 
 __global__ void __launch_bounds__(256) combine_tangent(int64_t nnz, const double* K, const double* C, const double* M,
-                                                       double* Ktan, double c1, double c2, double c3) {
+                                                       double* Ktan, double c1, const StepCoef* __restrict__ coef,
+                                                       const int32_t* __restrict__ blk_inst) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < nnz) Ktan[q] = (c1 * K[q] + c2 * C[q]) + c3 * M[q];
+    if (q >= nnz) return;
+    const StepCoef& sc = coef[blk_inst[q / 36]];
+    Ktan[q] = (c1 * K[q] + sc.gbdt * C[q]) + sc.inv * M[q];
 }
 
 __global__ void __launch_bounds__(256) residual_corrector(int64_t ndofs, double alpha, const double* fext,
@@ -259,8 +280,9 @@ __global__ void __launch_bounds__(256) residual_corrector(int64_t ndofs, double 
 // predictor residual: r = (1+α)f - Tⁱⁿᵗⁿ - Tᵏⁿ - αfⁿ + Cⁿ·t₁ + (1/(βΔt²))·Mⁿ·t₂, the two products
 // accumulated row-wise in increasing column order (what the CSC column sweep of mul! yields per row)
 __global__ void __launch_bounds__(256) residual_predictor(int64_t ndofs, const int32_t* __restrict__ bptr,
-                                                          const int32_t* __restrict__ bcol, double alpha, double t1c,
-                                                          double t2c, double dt, double c3, const double* fext,
+                                                          const int32_t* __restrict__ bcol, double alpha,
+                                                          const StepCoef* __restrict__ coef,
+                                                          const int32_t* __restrict__ node_inst, const double* fext,
                                                           const double* Tint_n, const double* Tk_n,
                                                           const double* fext_n, const double* Dd, const double* Ddd,
                                                           const double* Cn, const double* Mn, double* res) {
@@ -268,6 +290,8 @@ __global__ void __launch_bounds__(256) residual_predictor(int64_t ndofs, const i
     if (i >= ndofs) return;
     const int64_t n = i / 6;
     const int a = (int)(i % 6);
+    const StepCoef& sc = coef[node_inst[n]];
+    const double t1c = sc.gob, t2c = sc.t2c, dt = sc.dt, c3 = sc.inv;
     double r = (((1 + alpha) * fext[i] - Tint_n[i]) - Tk_n[i]) - alpha * fext_n[i];
     for (int b = bptr[n]; b < bptr[n + 1]; ++b) {
         const int64_t m = bcol[b];
@@ -324,7 +348,8 @@ __global__ void __launch_bounds__(256) bc_lift(int64_t ndofs, const int32_t* __r
 // One workgroup per reduction chunk; chunks of instances whose solve is done are skipped.
 struct KrylovCtl {
     double rho, rho_old, alpha, omega, bnorm2, rr2, ss2, tol2;
-    int done, iters, breakdown, pad;
+    int done, iters, breakdown;
+    int pivot;     // 1: the instance's factorisation met a singular pivot block (its solve failed)
 };
 
 // row i of the solve's operator times x: the identity on fixed dofs, the tangent's block row
@@ -704,7 +729,8 @@ EBD void chain_level(const ChainDev& cd, const ChainLDS& sm, const double* __res
 
 // Cyclic reduction of each chain's block-tridiagonal system, one workgroup per chain.
 // Dynamic LDS: 3 × 36 doubles per slot, ceil(n / 2^L0) slots (chain_lds_bytes).
-// `status` is set to 1 when a pivot block is singular (solve_begin clears it before each solve).
+// status[instance] is set to 1 when a pivot block of one of its chains is singular (solve_begin
+// clears the flags before each solve).
 // Two wavefronts per SIMD (at most 256 registers), so that two chains share a CU: a launch has
 // about 1.5 chains per CU.
 __global__ void __launch_bounds__(CHAIN_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
@@ -745,7 +771,7 @@ chain_factor(ChainDev cd, const double* __restrict__ A, int* status, int L0, con
         const int orow = gj6<6>(w, ln, act, ok, load);
         if (act && ln.r < 6) st6(cd.Dm + 36 * s0 + 6 * orow, w);
     }
-    if (!ok) *status = 1;
+    if (!ok) status[cd.inst[blockIdx.x]] = 1;
 }
 
 // smallest L0 whose LDS share for the longest chain stays within `budget` bytes
@@ -1016,12 +1042,15 @@ __global__ void __launch_bounds__(256) closure_apply(ChainDev cd, ClosureDev cl,
 struct KrylovAll {
     int all_done, max_iters, fail_inst, breakdown;
     double fail_rel;
-    int pivot_status;   // 1: a pivot block of the current solve's factorisation is singular
-    int pad;
+    int pivot_status;   // 1: a pivot block of some instance's factorisation is singular
+    int n_failed;       // instances whose finished solve is not accepted
 };
 
-// start of a linear solve: clears the pivot flag the factorisation may set
-__global__ void solve_begin(KrylovAll* all) { all->pivot_status = 0; }
+// start of a linear solve: clears the per-instance pivot flags the factorisation may set
+__global__ void __launch_bounds__(256) solve_begin(int ninst, int32_t* pivot_inst, KrylovAll* all) {
+    for (int s = threadIdx.x; s < ninst; s += blockDim.x) pivot_inst[s] = 0;
+    if (threadIdx.x == 0) all->pivot_status = 0;
+}
 
 // true when the kernel may run: ungated, or the polled solve is done
 EBD bool gate_open(const KrylovAll* gate) { return !gate || gate->all_done; }
@@ -1029,6 +1058,7 @@ EBD bool gate_open(const KrylovAll* gate) { return !gate || gate->all_done; }
 // acceptance of a finished solve: the tolerance reached, or a stall at rounding level (< 1e-6)
 EBD bool krylov_failed(const KrylovCtl& k, double& rel) {
     rel = 0;
+    if (k.pivot) { rel = 1; return true; }
     if (!k.done || k.iters == 0) return false;
     if (k.bnorm2 == 0.0 || k.rr2 <= k.tol2 * k.bnorm2 * 1.0000001 || k.ss2 <= k.tol2 * k.bnorm2) return false;
     rel = k.bnorm2 > 0 ? sqrt(fmin(k.rr2, k.ss2 > 0 ? k.ss2 : k.rr2) / k.bnorm2) : 0.0;
@@ -1037,20 +1067,23 @@ EBD bool krylov_failed(const KrylovCtl& k, double& rel) {
 
 // folds every instance's state into the polled record (one FIN_BLOCK workgroup)
 EBD void krylov_summary(const KrylovCtl* kc, int ninst, KrylovAll* all) {
-    __shared__ int s_done, s_iters, s_fail;
-    if (threadIdx.x == 0) { s_done = 1; s_iters = 0; s_fail = 0x7fffffff; }
+    __shared__ int s_done, s_iters, s_fail, s_nfail, s_piv;
+    if (threadIdx.x == 0) { s_done = 1; s_iters = 0; s_fail = 0x7fffffff; s_nfail = 0; s_piv = 0; }
     __syncthreads();
     for (int s = threadIdx.x; s < ninst; s += FIN_BLOCK) {
         const KrylovCtl& k = kc[s];
         double rel;
         if (!k.done) atomicAnd(&s_done, 0);
         atomicMax(&s_iters, k.iters);
-        if (krylov_failed(k, rel)) atomicMin(&s_fail, s);
+        if (krylov_failed(k, rel)) { atomicMin(&s_fail, s); atomicAdd(&s_nfail, 1); }
+        if (k.pivot) atomicOr(&s_piv, 1);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         all->all_done = s_done;
         all->max_iters = s_iters;
+        all->n_failed = s_nfail;
+        all->pivot_status = s_piv;
         all->fail_inst = s_fail == 0x7fffffff ? -1 : s_fail;
         double rel = 0;
         if (s_fail != 0x7fffffff) krylov_failed(kc[s_fail], rel);
@@ -1071,15 +1104,18 @@ __global__ void __launch_bounds__(RED_BLOCK) krylov_init(Segs sg, const uint8_t*
     seg_block_store<1>(q, partials, sg.nchunks);
 }
 // instances that are not active in the Newton iteration start done
+// instances whose factorisation met a singular pivot block start done and failed
 __global__ void __launch_bounds__(FIN_BLOCK) krylov_init_fin(const double* partials, Segs sg, KrylovCtl* kc, double tol,
-                                                             const uint8_t* active, KrylovAll* all) {
+                                                             const uint8_t* active, const int32_t* pivot_inst,
+                                                             KrylovAll* all) {
     for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
         double o[1];
         seg_sum<1>(partials, sg, s, o);
         KrylovCtl& k = kc[s];
         k.bnorm2 = o[0]; k.rho = o[0]; k.rho_old = 1; k.alpha = 1; k.omega = 1;
         k.rr2 = o[0]; k.ss2 = 0; k.tol2 = tol * tol; k.iters = 0; k.breakdown = 0;
-        k.done = (o[0] == 0.0 || !active[s]) ? 1 : 0;
+        k.pivot = (active[s] && pivot_inst[s]) ? 1 : 0;
+        k.done = (o[0] == 0.0 || !active[s] || k.pivot) ? 1 : 0;
     }
     __syncthreads();
     krylov_summary(kc, sg.ninst, all);
@@ -1185,6 +1221,25 @@ __global__ void __launch_bounds__(256) scatter_solution(int64_t ndofs, const uin
     if (i < ndofs && !fixed[i] && act.flag[act.node_inst[i / 6]]) dD[i] = x[i];
 }
 
+// Instances whose solve failed (a singular pivot block, a breakdown or a stall short of the
+// tolerance, or the iteration cap reached) leave the Newton iteration: the corrector records
+// k = kfail (max_iterations + 1, a step that did not converge: Δt is halved and the step
+// repeated, run_simulation.jl:55-71), the predictor flags them in pfail. The other instances
+// carry on as if alone.
+__global__ void __launch_bounds__(256) solve_screen(int ninst, const KrylovCtl* kc, uint8_t* active, int32_t* kiter,
+                                                    int32_t kfail, uint8_t* pfail, const KrylovAll* gate) {
+    if (!gate_open(gate)) return;
+    for (int s = threadIdx.x; s < ninst; s += blockDim.x) {
+        if (!active[s]) continue;
+        double rel;
+        if (!kc[s].done || krylov_failed(kc[s], rel)) {
+            active[s] = 0;
+            if (pfail) pfail[s] = 1;
+            else kiter[s] = kfail;
+        }
+    }
+}
+
 // ---- Newmark / rotation updates (src/utils_solver.jl:6-48, :157-245) -----------------------
 
 struct NodeFields { double *u, *ud, *udd, *w, *wd, *wdd, *R, *dR, *u_n, *ud_n, *udd_n, *w_n, *wd_n, *wdd_n, *R_n, *dR_n; };
@@ -1196,9 +1251,9 @@ EBD void stm(double* a, int64_t n, const M3<double>& m) { for (int k = 0; k < 9;
 
 // initialize_global_corrector! (predictor=0) / initialize_global_predictor! (predictor=1)
 __global__ void __launch_bounds__(256) init_global(int64_t N, NodeFields nf, double* D, double* Dd, double* Ddd, double* dD,
-                                                   int predictor) {
+                                                   int predictor, Active act) {
     int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+    if (n >= N || !act.flag[act.node_inst[n]]) return;
     for (int k = 0; k < 3; ++k) {
         dD[6 * n + k] = 0; D[6 * n + k] = nf.u[3 * n + k]; Dd[6 * n + k] = nf.ud[3 * n + k]; Ddd[6 * n + k] = nf.udd[3 * n + k];
         dD[6 * n + 3 + k] = 0; D[6 * n + 3 + k] = predictor ? 0.0 : nf.w[3 * n + k];
@@ -1207,33 +1262,41 @@ __global__ void __launch_bounds__(256) init_global(int64_t N, NodeFields nf, dou
 }
 
 // compute_global_corrector! per dof (all dofs, fixed included)
-__global__ void __launch_bounds__(256) global_corrector(int64_t ndofs, double gb, double inv, double* D, double* Dd,
-                                                        double* Ddd, const double* dD, Active act, const KrylovAll* gate) {
+__global__ void __launch_bounds__(256) global_corrector(int64_t ndofs, const StepCoef* __restrict__ coef, double* D,
+                                                        double* Dd, double* Ddd, const double* dD, Active act,
+                                                        const KrylovAll* gate) {
     if (!gate_open(gate)) return;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ndofs || !act.flag[act.node_inst[i / 6]]) return;
+    if (i >= ndofs) return;
+    const int32_t s = act.node_inst[i / 6];
+    if (!act.flag[s]) return;
+    const double gb = coef[s].gbdt, inv = coef[s].inv;
     D[i] += dD[i]; Dd[i] += gb * dD[i]; Ddd[i] += inv * dD[i];
 }
 // compute_global_predictor!
-__global__ void __launch_bounds__(256) global_predictor(int64_t ndofs, double dt, double gob, double gbdt, double inv,
-                                                        double dt2b, double* D, double* Dd, double* Ddd, const double* dD) {
+__global__ void __launch_bounds__(256) global_predictor(int64_t ndofs, const StepCoef* __restrict__ coef, double* D,
+                                                        double* Dd, double* Ddd, const double* dD, Active act) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ndofs) return;
+    const int32_t s = act.node_inst[i / 6];
+    if (!act.flag[s]) return;
+    const StepCoef& sc = coef[s];
+    const double dt = sc.dt, gob = sc.gob, gbdt = sc.gbdt, inv = sc.inv, dt2b = sc.dt2b;
     double Ddn = Dd[i];
     D[i] += dD[i];
     Dd[i] += (gbdt * dD[i] - gob * Ddn) + dt2b * Ddd[i];
     Ddd[i] += inv * ((dD[i] - dt * Ddn) - ((dt * dt) / 2) * Ddd[i]);
 }
 
-struct HHTc { double c_w, c_wd, c_wdd, d_w, d_wd, d_wdd; };  // ẇ = ΔR(c_w w + c_wd ẇⁿ + c_wdd ẅⁿ), ẅ likewise
-
 // compute_local_corrector!: u, u̇, ü from D; ΔR ← exp(ΔD_θ)ΔR; w = log ΔR; ẇ, ẅ; R = ΔR Rⁿ
 __global__ void __launch_bounds__(256) local_corrector(int64_t N, NodeFields nf, const double* D, const double* Dd,
-                                                       const double* Ddd, const double* dD, HHTc h, Active act,
+                                                       const double* Ddd, const double* dD,
+                                                       const StepCoef* __restrict__ coef, Active act,
                                                        const KrylovAll* gate) {
     if (!gate_open(gate)) return;
     int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N || !act.flag[act.node_inst[n]]) return;
+    const HHTc h = coef[act.node_inst[n]].h;
     stv(nf.u, n, {D[6 * n], D[6 * n + 1], D[6 * n + 2]});
     stv(nf.ud, n, {Dd[6 * n], Dd[6 * n + 1], Dd[6 * n + 2]});
     stv(nf.udd, n, {Ddd[6 * n], Ddd[6 * n + 1], Ddd[6 * n + 2]});
@@ -1250,9 +1313,10 @@ __global__ void __launch_bounds__(256) local_corrector(int64_t N, NodeFields nf,
 
 // compute_local_predictor!: w = θ̃ = D_θ, ΔR = exp(θ̃), R = ΔR Rⁿ, ẇ, ẅ from θ̃
 __global__ void __launch_bounds__(256) local_predictor(int64_t N, NodeFields nf, const double* D, const double* Dd,
-                                                       const double* Ddd, HHTc h) {
+                                                       const double* Ddd, const StepCoef* __restrict__ coef, Active act) {
     int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+    if (n >= N || !act.flag[act.node_inst[n]]) return;
+    const HHTc h = coef[act.node_inst[n]].h;
     stv(nf.u, n, {D[6 * n], D[6 * n + 1], D[6 * n + 2]});
     stv(nf.ud, n, {Dd[6 * n], Dd[6 * n + 1], Dd[6 * n + 2]});
     stv(nf.udd, n, {Ddd[6 * n], Ddd[6 * n + 1], Ddd[6 * n + 2]});
@@ -1334,24 +1398,32 @@ __global__ void __launch_bounds__(FIN_BLOCK) norms_finalize(const double* partia
     }
 }
 
-__global__ void newton_reset(int ninst, uint8_t* active, int32_t* kiter, int32_t* kdone) {
+// Start (and end) of a corrector loop: the instances taking part in the step become active. At
+// the start (`pfail` given) an instance whose predictor solve failed stays frozen with k = kfail.
+__global__ void newton_reset(int ninst, const uint8_t* part, const uint8_t* pfail, uint8_t* active, int32_t* kiter,
+                             int32_t* kdone, int32_t kfail) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *kdone = 0;
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ninst; s += gridDim.x * blockDim.x) {
-        active[s] = 1;
-        kiter[s] = 0;
+        const bool failed = pfail && part[s] && pfail[s];
+        active[s] = (part[s] && !failed) ? 1 : 0;
+        kiter[s] = failed ? kfail : 0;
     }
 }
 
-// several device copies in one launch (the commit and the rollback of a step): list y = blockIdx.y
+// several device copies in one launch (the commit and the rollback of a step): list y = blockIdx.y.
+// With `sel`, element i of list y is copied only when its instance, map[y][i / per[y]], is selected.
 struct CopyList {
     const double* src[16];
     double* dst[16];
     int64_t n[16];
+    const int32_t* map[16];
+    int32_t per[16];
+    const uint8_t* sel;
 };
 __global__ void __launch_bounds__(256) copy_many(CopyList cl) {
     const int y = blockIdx.y;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cl.n[y]; i += (int64_t)gridDim.x * blockDim.x)
-        cl.dst[y][i] = cl.src[y][i];
+        if (!cl.sel || cl.sel[cl.map[y][i / cl.per[y]]]) cl.dst[y][i] = cl.src[y][i];
 }
 
 // ---- layout conversions at the ABI ----------------------------------------------------------

@@ -134,7 +134,17 @@ int64_t chains_left_out(const Chains& c, const Pattern& p) {
     return n;
 }
 
-void build_chains(Chains& c, const Pattern& p, const double* X0, int mode, const Chains* across) {
+std::vector<uint8_t> nodes_left_out(const Chains& c, const Pattern& p) {
+    const auto links = chain_links(c, p.N);
+    std::vector<uint8_t> out((size_t)p.N, 0);
+    for (int32_t a = 0; a < p.N; ++a)
+        for (int32_t m : p.adj[a])
+            if (m != a && !std::binary_search(links[a].begin(), links[a].end(), m)) out[a] = 1;
+    return out;
+}
+
+void build_chains(Chains& c, const Pattern& p, const double* X0, int mode, const Chains* across,
+                  const std::vector<uint8_t>* cover) {
     const int64_t N = p.N;
     const std::vector<std::vector<int32_t>> taken = across ? chain_links(*across, N) : std::vector<std::vector<int32_t>>();
     // a coupling the walk may follow: not a self-coupling, and not already held by `across`
@@ -143,7 +153,10 @@ void build_chains(Chains& c, const Pattern& p, const double* X0, int mode, const
     };
     c.ptr.assign(1, 0);
     c.node.clear();
+    // nodes outside `cover` start as used: no chain holds them
     std::vector<uint8_t> used(N, 0);
+    if (cover)
+        for (int64_t n = 0; n < N; ++n) used[n] = (*cover)[n] ? 0 : 1;
     auto push = [&](int32_t n) { c.node.push_back(n); used[n] = 1; };
     auto walk = [&](int32_t start) {
         int32_t prev = -1, cur = start;
@@ -175,7 +188,11 @@ void build_chains(Chains& c, const Pattern& p, const double* X0, int mode, const
         c.ptr.push_back((int64_t)c.node.size());
     };
     if (mode == 1) {
-        for (int32_t n = 0; n < N; ++n) { push(n); c.ptr.push_back((int64_t)c.node.size()); }
+        for (int32_t n = 0; n < N; ++n) {
+            if (used[n]) continue;
+            push(n);
+            c.ptr.push_back((int64_t)c.node.size());
+        }
     } else {
         // start from path ends (one free neighbour), then from whatever is left (closed loops)
         bool progress = true;

@@ -63,7 +63,11 @@ bool build_pattern(Pattern& p, int64_t N, int64_t E, const int64_t* conn1b, int6
 // With `across`, the walk uses only the couplings that none of across's chains holds between
 // consecutive positions (a second, crossing cover: the other lines of a net); a node with no such
 // coupling left is a chain of its own.
-void build_chains(Chains& c, const Pattern& p, const double* X0, int mode, const Chains* across = nullptr);
+// `cover` (per node, may be null: all): the nodes the chains may hold
+void build_chains(Chains& c, const Pattern& p, const double* X0, int mode, const Chains* across = nullptr,
+                  const std::vector<uint8_t>* cover = nullptr);
+// per node: 1 when it has a coupling that no chain of `c` holds
+std::vector<uint8_t> nodes_left_out(const Chains& c, const Pattern& p);
 // couplings (node pairs with a matrix block) that no chain of c holds between consecutive
 // positions, counted once per pair; closures count as held
 int64_t chains_left_out(const Chains& c, const Pattern& p);

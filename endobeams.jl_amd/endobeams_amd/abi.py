@@ -76,6 +76,8 @@ class RunState(C.Structure):
 
 # int (*)(eb_ctx*, double t, void* user)
 STEP_CALLBACK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_void_p)
+# int (*)(eb_ctx*, int64_t instance, double t, void* user)
+INSTANCE_CALLBACK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.c_double, C.c_void_p)
 
 
 class Sizes(C.Structure):
@@ -120,6 +122,7 @@ PROTOTYPES = {
     "update_converged": (_i, [_vp]),
     "update_not_converged": (_i, [_vp]),
     "run_steps": (_i, [_vp, _NP, C.POINTER(TimeParams), C.c_int64, STEP_CALLBACK, _vp, C.POINTER(RunState)]),
+    "run_steps_batch": (_i, [_vp, _NP, C.POINTER(TimeParams), C.c_int64, INSTANCE_CALLBACK, _vp, C.POINTER(RunState)]),
     "get_counters": (_i, [_vp, _lp]),
     "profile_enable": (_i, [_vp, C.c_uint32]),
     "profile_read": (_i, [_vp, _i, _dp, _lp]),

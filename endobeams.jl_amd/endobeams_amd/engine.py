@@ -291,6 +291,53 @@ class Engine:
         self._check(st, "run_steps")
         return state
 
+    def _time_params(self):
+        p = self.params
+        return abi.TimeParams(p.initial_timestep, p.min_timestep, p.max_timestep, p.simulation_end_time,
+                              int(p.accelerate_after_success_it), int(bool(p.stop_on_energy_threshold)),
+                              float(p.energy_threshold))
+
+    def _has_time_functions(self) -> bool:
+        confs = [c for (c, _, _) in self.instances] if hasattr(self, "instances") else [self.conf]
+        for c in confs:
+            if c.loads is not None and c.loads.concentrated_force is not None:
+                return True
+            if c.bcs.imposed_displacements_function is not None and len(c.bcs.displaced_indices) > 0:
+                return True
+        return False
+
+    def update_instance(self, s: int, t: float):
+        """Loads and imposed displacements of instance s at its time t (a single structure: s = 0)."""
+        self.update_loads(t)
+        self.update_boundary_conditions(t)
+
+    def run_steps_batch(self, max_steps: int, states=None):
+        """eb_run_steps_batch: every instance advances with its own time and Δt (run_simulation!'s
+        loop per instance) until it has `max_steps` more converged steps, reaches the end time or
+        aborts (stopped = 2). `states` is a ctypes array of RunState, one per instance (fresh
+        ones: t = 0, Δt = initial_timestep); returns it."""
+        S = getattr(self, "S", 1)
+        if states is None:
+            states = (abi.RunState * S)()
+            for r in states:
+                r.t, r.dt = 0.0, self.params.initial_timestep
+        tp = self._time_params()
+        errors = []
+
+        def on_step(ctx, s, t, user):
+            try:
+                self.update_instance(int(s), t)
+                return 0
+            except Exception as e:  # reported after eb_run_steps_batch returns
+                errors.append(e)
+                return 1
+        cb = abi.INSTANCE_CALLBACK(on_step) if self._has_time_functions() else abi.INSTANCE_CALLBACK()
+        st = self.fn["run_steps_batch"](self.h, C.byref(self._np), C.byref(tp), int(max_steps), cb, None, states)
+        if errors:
+            raise errors[0]
+        self._check(st, "run_steps_batch")
+        return states
+
     def update_converged(self):
         self._check(self.fn["update_converged"](self.h), "update_converged")
 
@@ -382,6 +429,25 @@ class BatchEngine(Engine):
         v = np.ascontiguousarray(np.concatenate(vals))
         self._check(self.fn["update_loads"](self.h, d.size, d.ctypes.data_as(C.POINTER(C.c_int64)),
                                             v.ctypes.data_as(C.POINTER(C.c_double))), "update_loads")
+
+    def update_instance(self, s: int, t: float):
+        """Loads and imposed displacements of instance s alone at its own time t."""
+        c = self.instances[s][0]
+        ld = c.loads
+        if ld is not None and ld.concentrated_force is not None:
+            cf = ld.concentrated_force
+            d = np.ascontiguousarray(np.asarray(cf.loaded_dofs, dtype=np.int64) + 6 * int(self.node_off[s]))
+            v = np.ascontiguousarray(cf.values(t), dtype=np.float64)
+            self._check(self.fn["update_loads"](self.h, d.size, d.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                v.ctypes.data_as(C.POINTER(C.c_double))), "update_loads")
+        bcs = c.bcs
+        if bcs.imposed_displacements_function is not None and len(bcs.displaced_indices) > 0:
+            for i in bcs.displaced_indices:
+                bcs.imposed_displacements[i] = bcs.imposed_displacements_function(t)
+            v = np.ascontiguousarray(np.concatenate([np.asarray(ci.bcs.imposed_displacements, dtype=np.float64).reshape(-1)
+                                                     for (ci, _, _) in self.instances]))
+            self._check(self.fn["update_boundary_conditions"](self.h, v.ctypes.data_as(C.POINTER(C.c_double)), v.size),
+                        "update_boundary_conditions")
 
     def update_boundary_conditions(self, t):
         parts, any_fn = [], False

@@ -225,7 +225,8 @@ typedef struct eb_time_params {
 typedef struct eb_run_state {
     double t, dt;                   /* tⁿ and the next step's Δt */
     int32_t successive;             /* successive converged steps since the last Δt change */
-    int32_t stopped;                /* 1: the energy criterion ended the run */
+    int32_t stopped;                /* 1: the energy criterion ended the run; 2 (eb_run_steps_batch
+                                       only): Δt fell below min_timestep, the instance aborted */
     int64_t steps, failed_steps;    /* converged steps, attempts repeated with Δt/2 */
     int64_t newton_iterations;      /* Σ (k - 1) over converged steps */
     int32_t last_k, pad;            /* k of the last attempt */
@@ -261,9 +262,32 @@ int eb_run_steps(eb_ctx* ctx, const eb_newton_params* p, const eb_time_params* t
  * returns the largest k of the instances (the step is repeated with Δt/2 for all when it exceeds
  * max_iterations). */
 int eb_create_batch(const eb_model_desc* descs, int64_t n_instances, eb_ctx** out);
+/* Called before every step attempt of instance `instance` with the attempt's time tⁿ⁺¹ (its own
+ * time: instances of a batch advance with their own Δt); sets that instance's loads and imposed
+ * displacements through eb_update_loads / eb_update_boundary_conditions (its dofs and fixed-dof
+ * entries shifted by the sizes of the instances before it) and returns 0. NULL: no
+ * time-dependent loads. */
+typedef int (*eb_instance_callback)(eb_ctx* ctx, int64_t instance, double t, void* user);
+
+/* run_simulation!'s time loop (src/run_simulation.jl:38-112) for every instance of a batch with
+ * its own time and Δt: states[s] (n_instances records, as eb_run_steps keeps them) holds instance
+ * s's progress. Each round, the instances that have fewer than max_steps new converged steps in
+ * this call, have not reached simulation_end_time and have not stopped attempt one step
+ * together, each with its own Δt; per instance, k > max_iterations halves Δt and repeats its step
+ * (update_not_converged! for that instance alone), Δt below min_timestep ends that instance
+ * (stopped = 2) without stopping the others, and a converged step is committed for that instance
+ * (update_converged!) with the ×1.5 growth after accelerate_after_success_it easy steps.
+ * Instances that do not attempt a step stay frozen. A linear solve that fails for one instance
+ * (singular pivot block, breakdown, iteration cap) counts as that instance's Newton failure.
+ * Each instance follows the trajectory it has alone under eb_run_steps. The energy stop is not
+ * available (EB_ERR_ARG): the batch keeps one energy total. */
+int eb_run_steps_batch(eb_ctx* ctx, const eb_newton_params* p, const eb_time_params* tp, int64_t max_steps,
+                       eb_instance_callback callback, void* user, eb_run_state* states);
+
 /* node_offsets and beam_offsets receive n_instances+1 entries (either may be NULL) */
 int eb_get_instances(const eb_ctx* ctx, int64_t* n_instances, int64_t* node_offsets, int64_t* beam_offsets);
-/* k of every instance at the end of the last corrector call */
+/* k of every instance at the end of the last corrector call (0: the instance took no part in
+ * the step; max_iterations + 1: its Newton iteration or one of its linear solves failed) */
 int eb_get_instance_iterations(const eb_ctx* ctx, int32_t* k_out);
 /* relative residual and ‖ΔD‖ of every instance (2 per instance) from the last norms evaluation */
 int eb_get_instance_norms(const eb_ctx* ctx, double* rn_dn);

@@ -733,6 +733,45 @@ int ebo_run_steps(void* h, const eb_newton_params* p, const eb_time_params* tp, 
     return EB_OK;
 }
 
+// the per-instance time loop of a batch (a batch of one here): run_simulation!'s loop with an
+// abort recorded as stopped = 2 instead of an error
+int ebo_run_steps_batch(void* h, const eb_newton_params* p, const eb_time_params* tp, int64_t max_steps,
+                        int (*callback)(void*, int64_t, double, void*), void* user, eb_run_state* state) {
+    Oracle* o = (Oracle*)h;
+    if (!p || !tp || !state || max_steps < 0) return EB_ERR_ARG;
+    if (tp->stop_on_energy_threshold) return fail(o, EB_ERR_ARG, "the energy stop is not available per instance");
+    int64_t done = 0;
+    while (done < max_steps && tp->simulation_end_time - state->t > 1.0e-10 && !state->stopped) {
+        const double tnp1 = state->t + state->dt;
+        if (callback && callback(h, 0, tnp1, user) != 0) return fail(o, EB_ERR_ARG, "step callback failed");
+        int k = 0;
+        int st = ebo_solve_step_dynamics(h, p, state->dt, &k);
+        if (st != EB_OK) return st;
+        state->last_k = k;
+        if (k > p->max_iterations) {
+            state->dt = state->dt / 2;
+            state->failed_steps += 1;
+            state->successive = 0;
+            st = ebo_update_not_converged(h);
+            if (st != EB_OK) return st;
+            if (state->dt < tp->min_timestep) state->stopped = 2;
+            continue;
+        }
+        state->successive += 1;
+        state->steps += 1;
+        state->newton_iterations += k - 1;
+        done += 1;
+        if (state->successive >= tp->accelerate_after_success_it && k <= 8) {
+            state->dt = std::min(state->dt * 1.5, tp->max_timestep);
+            state->successive = 0;
+        }
+        st = ebo_update_converged(h);
+        if (st != EB_OK) return st;
+        state->t = tnp1;
+    }
+    return EB_OK;
+}
+
 int ebo_create_batch(const eb_model_desc* d, int64_t n_instances, void** out) {
     if (n_instances != 1) return EB_ERR_ARG;
     return ebo_create(d, out);
