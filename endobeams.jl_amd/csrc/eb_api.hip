@@ -8,6 +8,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -25,7 +26,18 @@
 
 using namespace ebd;
 
+// Device staging of eb_output_points_noalloc: a file-scope array of EB_NOALLOC_SCRATCH_DOUBLES
+// doubles, one copy on every GPU the library runs on, held until the process ends. In host code
+// `scratch` is a host-side stand-in: its device address on the current GPU comes from
+// hipGetSymbolAddress (scratch_setup, once per device at eb_create), and only that address is
+// passed to kernels and copies.
+static __device__ __align__(256) double scratch[EB_NOALLOC_SCRATCH_DOUBLES];
+
 namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
 
 struct DevBuf {
     void* p = nullptr;
@@ -198,6 +210,8 @@ struct eb_ctx {
     int64_t graph_stats[2] = {0, 0};   // captures, replays
     // launch shapes (eb_autotune)
     int32_t tune[EB_TUNE_COUNT] = {};
+    // this device's copy of the file-scope staging array (eb_output_points_noalloc)
+    double* scratch = nullptr;
 };
 
 namespace {
@@ -375,6 +389,26 @@ int set_coefs(eb_ctx* ctx, const double* dt_inst, double dt, double b, double g)
     ctx->coef_set = true;
     ctx->coef_ver++;
     EB_HIP(hipMemcpyAsync(ctx->coef, ctx->h_coef, sizeof(StepCoef) * ctx->S, hipMemcpyHostToDevice, ctx->stream));
+    return EB_OK;
+}
+
+// the device address of `scratch` on the context's device, looked up once per device
+int scratch_setup(eb_ctx* ctx) {
+    if (ctx->device < 0 || ctx->device >= SCRATCH_MAX_DEVICES) {
+        ctx->err = "device index beyond the staging table";
+        return EB_ERR_DEVICE;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            ctx->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return EB_ERR_DEVICE;
+        }
+        g_scratch_ptr[ctx->device] = (double*)p;
+    }
+    ctx->scratch = g_scratch_ptr[ctx->device];
     return EB_OK;
 }
 
@@ -1404,6 +1438,10 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
     auto bail = [&](int code) { eb_destroy(ctx); return code; };
     if (hipGetDevice(&ctx->device) != hipSuccess) return bail(EB_ERR_DEVICE);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return bail(EB_ERR_DEVICE);
+    if (scratch_setup(ctx) != EB_OK) {
+        fprintf(stderr, "eb_create: %s\n", ctx->err.c_str());
+        return bail(EB_ERR_DEVICE);
+    }
     {
         // dynamic LDS for the chain factorisation: what a workgroup may hold beside the kernel's
         // static LDS, asked for explicitly beyond the 64 KB every device grants
@@ -2469,6 +2507,36 @@ int eb_output_points(eb_ctx* ctx, int32_t n_points, double* position, double* di
     release();
     if (st != EB_OK && ctx->err.empty()) ctx->err = "eb_output_points failed";
     return st;
+}
+
+int eb_output_points_noalloc(eb_ctx* ctx, int32_t n_points, double* position, double* displacement, double* velocity,
+                             double* strain, double* stress) {
+    if (!ctx || n_points < 1) return EB_ERR_ARG;
+    if (!ctx->scratch) { ctx->err = "eb_output_points_noalloc: no staging on this device"; return EB_ERR_STATE; }
+    const int64_t np = (int64_t)n_points * ctx->E;
+    double* outs[5] = {position, displacement, velocity, strain, stress};
+    const int64_t width[5] = {3, 3, 3, 1, 1};
+    int64_t need = 0;
+    for (int k = 0; k < 5; ++k)
+        if (outs[k]) need += width[k] * np;
+    if (need > EB_NOALLOC_SCRATCH_DOUBLES) {
+        ctx->err = "eb_output_points_noalloc: " + std::to_string(need) + " doubles of output exceed the staging of " +
+                   std::to_string((long long)EB_NOALLOC_SCRATCH_DOUBLES) + " (use eb_output_points)";
+        return EB_ERR_ARG;
+    }
+    // every requested output is written in full by the kernel: nothing to clear
+    double* dev[5] = {};
+    int64_t off = 0;
+    for (int k = 0; k < 5; ++k)
+        if (outs[k]) { dev[k] = ctx->scratch + off; off += width[k] * np; }
+    hipLaunchKernelGGL(beam_output_points, dim3(nblk(ctx->E, 256)), dim3(256), 0, ctx->stream, node_state(ctx),
+                       beam_consts(ctx), (int)n_points, dev[0], dev[1], dev[2], dev[3], dev[4]);
+    EB_TRY(check_launch(ctx, "beam_output_points"));
+    for (int k = 0; k < 5; ++k)
+        if (outs[k] && np > 0)
+            EB_HIP(hipMemcpyAsync(outs[k], dev[k], sizeof(double) * width[k] * np, hipMemcpyDeviceToHost, ctx->stream));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    return EB_OK;
 }
 
 int eb_get_sdf_coefficients(eb_ctx* ctx, int64_t instance, double* out, int64_t count) {

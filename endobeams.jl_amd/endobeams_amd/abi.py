@@ -9,6 +9,8 @@ EB_ABI_VERSION = 1
 EB_NPROPS = 14
 
 EB_OK, EB_ERR_ARG, EB_ERR_DEVICE, EB_ERR_SOLVE, EB_ERR_STATE = 0, -1, -2, -3, -4
+# device staging of eb_output_points_noalloc, in doubles (include/endobeams_hip.h)
+EB_NOALLOC_SCRATCH_DOUBLES = 1 << 22
 
 EB_SURFACE_NONE, EB_SURFACE_SPHERE, EB_SURFACE_SDF = 0, 1, 2
 EB_PRECISION_FP64, EB_PRECISION_FP32 = 0, 1
@@ -131,6 +133,7 @@ PROTOTYPES = {
     "get_instance_iterations": (_i, [_vp, C.POINTER(C.c_int32)]),
     "get_instance_norms": (_i, [_vp, _dp]),
     "output_points": (_i, [_vp, C.c_int32, _dp, _dp, _dp, _dp, _dp]),
+    "output_points_noalloc": (_i, [_vp, C.c_int32, _dp, _dp, _dp, _dp, _dp]),
     "get_sdf_coefficients": (_i, [_vp, C.c_int64, _dp, C.c_int64]),
     "set_exec_mode": (_i, [_vp, _i]),
     "get_exec_stats": (_i, [_vp, C.POINTER(C.c_int32), _lp, _lp]),

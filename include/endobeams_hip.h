@@ -322,6 +322,18 @@ int eb_profile_read(eb_ctx* ctx, int id, double* total_ms, int64_t* launches);
 int eb_output_points(eb_ctx* ctx, int32_t n_points, double* position, double* displacement, double* velocity,
                      double* strain, double* stress);
 
+/* eb_output_points without any allocation: no counterpart in the reference. For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight output loops, harnesses that
+ * forbid hipMalloc/hipFree): the same points and fields, staged in a fixed device array of
+ * EB_NOALLOC_SCRATCH_DOUBLES doubles that every GPU the library runs on holds until the process
+ * ends. The requested outputs may take at most that many doubles together (n_points × n_beams ×
+ * 3 per requested vector field, × 1 per requested scalar field); larger requests fail with
+ * EB_ERR_ARG. Calls on contexts of one GPU share that GPU's array: calls that overlap in time on
+ * one GPU (several host threads) race; calls in sequence do not. */
+#define EB_NOALLOC_SCRATCH_DOUBLES (1 << 22)
+int eb_output_points_noalloc(eb_ctx* ctx, int32_t n_points, double* position, double* displacement, double* velocity,
+                             double* strain, double* stress);
+
 /* Prefiltered quadratic B-spline coefficients of instance `instance`'s signed distance field as
  * the device holds them ((n0+2)(n1+2)(n2+2), x fastest, converted to double) — the coefficient
  * array of the reference's interpolant (src/interactions/def_interactions.jl:77). The device

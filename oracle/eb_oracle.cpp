@@ -911,6 +911,12 @@ int ebo_output_points(void* h, int32_t n_points, double* position, double* displ
     return EB_OK;
 }
 
+// the oracle allocates nothing on a device: the same as ebo_output_points
+int ebo_output_points_noalloc(void* h, int32_t n_points, double* position, double* displacement, double* velocity,
+                              double* strain, double* stress) {
+    return ebo_output_points(h, n_points, position, displacement, velocity, strain, stress);
+}
+
 int ebo_get_sdf_coefficients(void* h, int64_t instance, double* out, int64_t count) {
     Oracle* o = (Oracle*)h;
     if (instance != 0) return fail(o, EB_ERR_ARG, "one instance");
