@@ -6,19 +6,22 @@ tolerance 1e-8, the reference's adaptive Δt), inputs resident on the GPU. Workl
 stent, 24 helical wires × 200 beams (4800 beams, 4824 nodes, 28 944 dofs), crimped to 1.5 mm and
 released in a rigid vessel given as a 256³ FP32 signed distance field.
 
-Three forms are timed in the same run:
-  plain   — FP64 element/contact kernels, one stent per context, the reference predictor;
-  fast    — the precision form (FP32 element matrices and contact tangent; forces, residuals,
-            rotations, Krylov and updates FP64) on a batch of seeded stent variants advanced in
-            one context (each instance its own vessel SDF);
-  learned — the fast form with the learned warm start: a 3×128 FP32 network trained in this run,
-            by the library's own kernels, on trajectories of 8 other seeded stents; the reference
-            predictor stays the per-instance fallback.
-`value` is the learned form's whole-job throughput; the plain and fast forms are reported beside
-it with the ratios. Also reported: the roofline of the fast form's element kernel (algorithmic flops per
-launch ÷ average launch time from HIP events in this run), the CPU restatement's throughput on a
-bounded sample of the plain workload (1 thread) and the fast form's trajectory error against
-that FP64 CPU run at the same step.
+Four forms are timed in the same run:
+  plain       — FP64 element/contact kernels, one stent per context, the reference predictor;
+  fp64_batch  — the same FP64 kernels on a batch of seeded stent variants advanced in one context
+                (each instance its own vessel SDF and its own time step);
+  fast        — the precision form (FP32 element matrices and contact tangent; forces, residuals,
+                rotations, Krylov and updates FP64) on the same batch;
+  learned     — the fast form with the learned warm start: a 3×128 FP32 network trained in this
+                run, by the library's own kernels, on trajectories of 8 other seeded stents; the
+                reference predictor stays the per-instance fallback.
+`value` is the learned form's whole-job throughput; the other forms are reported beside it with
+the ratios (the FP64 batch separates batching from the FP32 element matrices and the learned
+start). Also reported: the roofline of the element kernel (algorithmic flops per launch ÷ average
+launch time from HIP events in this run), HBM roofline records of the matrix gather and the
+contact kernel (algorithmic bytes per launch ÷ average launch time), the CPU restatement's
+throughput on a bounded sample of the plain workload (one thread and all OpenMP threads) and the
+fast form's trajectory error against that FP64 CPU run at the same step.
 """
 import argparse
 import json
@@ -46,16 +49,53 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector
 PEAK_HBM_GBS = 8000.0
 
 
-def roofline(kernel, avg_s, beams, peak, traffic=None):
+def roofline(kernel, avg_s, beams, peak, traffic=None, peak_fp64=PEAK_FP64_TFLOPS):
     """The element kernel's roofline record: the dense algorithmic flops per launch (and, beside
-    them, the flops left once structural zeros are folded) over the measured average launch."""
+    them, the flops left once structural zeros are folded) over the measured average launch. The
+    kernel runs on the vector ALUs (no matrix cores), part of it in FP64 in the precision form:
+    the fraction is given against the FP32 vector peak and, beside it, against the FP64 one."""
     achieved = FLOPS_PER_BEAM * beams / avg_s / 1e12
     nonzero = FLOPS_PER_BEAM_NONZERO * beams / avg_s / 1e12
-    return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-            "traffic": traffic, "kernel": kernel, "avg_launch_us": avg_s * 1e6,
-            "flops_per_launch": FLOPS_PER_BEAM * beams,
+    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "frac_of_fp64_peak": achieved / peak_fp64, "traffic": traffic, "kernel": kernel,
+            "avg_launch_us": avg_s * 1e6, "flops_per_launch": FLOPS_PER_BEAM * beams,
             "nonzero_flops_per_launch": FLOPS_PER_BEAM_NONZERO * beams, "nonzero_achieved": nonzero,
             "nonzero_frac": nonzero / peak}
+
+
+# Algorithmic HBM bytes of the HBM-bound kernels (s: bytes of the staging type, 4 in the precision
+# form, 8 in FP64).
+def gather_bytes(E, nnz, s):
+    """gather_matrices forming the tangent: each beam's three 144-entry staging records read once
+    (plus its four block-source entries), every tangent entry written once in FP64, and per 6×6
+    block its pointer and instance entries."""
+    return E * (3 * 144 * s + 4 * 4) + nnz * 8 + (nnz // 36) * 8
+
+
+def contact_beam_bytes(s):
+    """contact_kernel per listed beam: two nodes' X₀, u, u̇, ẇ, R, ΔR (30 doubles each), the beam's
+    l₀, Rₑ⁰, radius and node indices, its list entry, read-modify-write of its K and C staging
+    records (2 × 144 entries), Tᶜ (12 doubles) and the contact energy."""
+    return 2 * 30 * 8 + (11 * 8 + 8) + 8 + 2 * 144 * s * 2 + 12 * 8 + 8
+
+
+def hbm_record(kernel, bytes_per_launch, avg_s, **extra):
+    achieved = bytes_per_launch / avg_s / 1e9
+    return {"kernel": kernel, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS, "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6,
+            **extra}
+
+
+def cpu_info():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
 
 
 def stent(seed=None, sdf_n=256):
@@ -73,39 +113,113 @@ def run_steps(stepper, n):
     return done, attempts
 
 
-def timed(eng, params, E, warmup, steps, kernel="beam", graph=True, autotune=True):
-    """Warm-up steps (then the autotuner, on the live state), then `steps` timed converged steps."""
+PROFILED = ("beam", "gather_matrices", "contact")
+
+
+def _kernel_times(eng, contact0):
+    out = {k: eng.profile_read(k) for k in PROFILED}
+    c1 = eng.contact_stats()
+    return out, (c1[0] - contact0[0], c1[1] - contact0[1])
+
+
+def timed(eng, params, E, warmup, steps, graph=True, autotune=True):
+    """One structure: warm-up steps (then the autotuner, on the live state), then `steps` timed
+    converged steps of the library's own time loop (eb_run_steps)."""
     from endobeams_amd import Stepper, abi
     eng.set_exec_mode(graph)
     st = Stepper(eng, params)
     st.initialize()
     run_steps(st, warmup)
     tuning = eng.autotune() if autotune else (eng.tuning, None)
-    eng.profile_enable([kernel])
-    # the timed steps: the library's own time loop (eb_run_steps), continuing the warm-up's
+    eng.profile_enable(list(PROFILED))
+    c0 = eng.contact_stats()
     state = abi.RunState(t=st.tn, dt=st.dt, successive=st.successive)
     t0 = time.perf_counter()
     state = eng.run_steps(steps, state)
     wall = time.perf_counter() - t0
     done = state.steps
-    ms, launches = eng.profile_read(kernel)
-    return {"steps": done, "attempts": done + state.failed_steps, "wall": wall, "E": E,
+    kt, contact = _kernel_times(eng, c0)
+    its = state.newton_iterations / max(done, 1)
+    return {"steps": done, "attempts": done + state.failed_steps, "wall": wall, "E": E, "nnz": eng.sizes.nnz,
             "value": E * done / wall, "ms_per_step": 1e3 * wall / max(done, 1),
-            "newton_iters_per_step": state.newton_iterations / max(done, 1),
+            "newton_iters_per_step": its, "newton_iters_per_step_max": its,
             "failed_steps": state.failed_steps,
             "wall_s_per_simulated_s": wall / (state.t - st.tn) if state.t > st.tn else None,
-            "kernel_ms": ms, "kernel_launches": launches, "stepper": st,
+            "kernels": kt, "contact": contact,
             "exec": {"mode": "graph" if graph else "eager", "graph_captures_replays": list(eng.exec_stats()[1:]),
                      "tuning": tuning[0], "tuning_ms": tuning[1]}}
 
 
-def cpu_baseline(seconds):
+def timed_batch(eng, warmup, steps, graph=True, autotune=True):
+    """A batch context: every instance with its own Δt (eb_run_steps_batch): `warmup` converged
+    steps each (then the autotuner), then `steps` timed converged steps each."""
+    eng.set_exec_mode(graph)
+    eng.initialize_state_simulation()
+    states = eng.run_steps_batch(warmup)
+    tuning = eng.autotune() if autotune else (eng.tuning, None)
+    eng.profile_enable(list(PROFILED))
+    E_s = [int(eng.beam_off[s + 1] - eng.beam_off[s]) for s in range(eng.S)]
+    n0 = [r.steps for r in states]
+    i0 = [r.newton_iterations for r in states]
+    f0 = [r.failed_steps for r in states]
+    t_0 = [r.t for r in states]
+    c0 = eng.contact_stats()
+    t0 = time.perf_counter()
+    states = eng.run_steps_batch(steps, states)
+    wall = time.perf_counter() - t0
+    dn = [r.steps - n for r, n in zip(states, n0)]
+    di = [r.newton_iterations - i for r, i in zip(states, i0)]
+    sim = [r.t - t for r, t in zip(states, t_0)]
+    kt, contact = _kernel_times(eng, c0)
+    return {"steps": min(dn), "attempts": sum(dn) + sum(r.failed_steps - f for r, f in zip(states, f0)),
+            "wall": wall, "E": sum(E_s), "nnz": eng.sizes.nnz,
+            "value": sum(e * d for e, d in zip(E_s, dn)) / wall, "ms_per_step": 1e3 * wall / max(min(dn), 1),
+            "newton_iters_per_step": sum(di) / max(sum(dn), 1),
+            "newton_iters_per_step_max": max(i / max(d, 1) for i, d in zip(di, dn)),
+            "failed_steps": sum(r.failed_steps - f for r, f in zip(states, f0)),
+            "wall_s_per_simulated_s": wall / (sum(sim) / len(sim)) if sum(sim) > 0 else None,
+            "kernels": kt, "contact": contact,
+            "exec": {"mode": "graph" if graph else "eager", "graph_captures_replays": list(eng.exec_stats()[1:]),
+                     "tuning": tuning[0], "tuning_ms": tuning[1]}}
+
+
+def kernel_avg_s(form, k):
+    ms, n = form["kernels"][k]
+    return ms * 1e-3 / n if n else None
+
+
+def hbm_records(form, s):
+    """The HBM roofline records of the matrix gather and the contact kernel of a timed form."""
+    out = []
+    g = kernel_avg_s(form, "gather_matrices")
+    if g:
+        out.append(hbm_record("gather_matrices", gather_bytes(form["E"], form["nnz"], s), g,
+                              launches=form["kernels"]["gather_matrices"][1]))
+    c = kernel_avg_s(form, "contact")
+    n = form["kernels"]["contact"][1]
+    if c and n and form["contact"][0] > 0:
+        listed, gps = form["contact"][0] / n, form["contact"][1] / n
+        out.append(hbm_record("contact_kernel", listed * contact_beam_bytes(s) + gps * 27 * s, c, launches=n,
+                              beams_in_contact_per_launch=listed, gauss_points_in_contact_per_launch=gps))
+    return out
+
+
+def summary(form):
+    keys = ("value", "ms_per_step", "newton_iters_per_step", "newton_iters_per_step_max", "failed_steps",
+            "wall_s_per_simulated_s")
+    out = {k: form[k] for k in keys}
+    b = kernel_avg_s(form, "beam")
+    out["beam_kernel_us"] = b * 1e6 if b else None
+    return out
+
+
+def _cpu_leg(seconds, timing):
     """The FP64 CPU restatement on the plain workload: its first step, then steps for `seconds`."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
     from endobeams_amd import Stepper
     conf, params, inter = stent()
-    eng = O.OracleEngine(conf, params, inter)
+    eng = O.OracleEngine(conf, params, inter, timing=timing)
     st = Stepper(eng, params)
     st.initialize()
     run_steps(st, 1)
@@ -115,12 +229,25 @@ def cpu_baseline(seconds):
         run_steps(st, 1)
     wall = time.perf_counter() - t0
     steps = st.stats.steps - n0
-    E = len(conf.beams)
-    out = {"value": E * steps / wall, "unit": "element-steps/s", "cores": 1, "kind": "port",
-           "sample": f"braided stent (4800 beams), {steps} converged steps after its first step, "
-                     f"{st.stats.newton_iterations - it0} Newton iterations, {wall:.1f} s, C++ FP64 "
-                     f"restatement with banded-LU direct solve, 1 thread"}
-    return out, eng, st.stats.steps
+    return len(conf.beams) * steps / wall, steps, st.stats.newton_iterations - it0, wall, eng, st.stats.steps
+
+
+def cpu_baseline(seconds):
+    """The C++ FP64 restatement (a port, not Julia, which is not available): one thread (the
+    parity build), then all the cores OpenMP is given (the timing build: element and contact loops
+    over threads; the scatter in beam order and the banded-LU direct solve stay serial)."""
+    v1, n1, i1, w1, eng, n_done = _cpu_leg(seconds * 0.6, False)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    vn, nn, i_n, wn, _, _ = _cpu_leg(seconds * 0.4, True)
+    info = cpu_info()
+    out = {"value": vn, "unit": "element-steps/s", "cores": threads, "kind": "port",
+           "sample": f"braided stent (4800 beams), {nn} converged steps after its first step, {i_n} Newton "
+                     f"iterations, {wn:.1f} s, C++ FP64 restatement with banded-LU direct solve, element and "
+                     f"contact loops on {threads} OpenMP threads",
+           **info,
+           "one_thread": {"value": v1, "cores": 1,
+                          "sample": f"{n1} converged steps after the first, {i1} Newton iterations, {w1:.1f} s"}}
+    return out, eng, n_done
 
 
 def sweep(args):
@@ -152,32 +279,36 @@ def sweep(args):
         print(f"# batch shape: {size} instances per context ({ {k: round(v) for k, v in rates.items()} }), "
               f"tuned in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     groups = autotune.split(inst, size)
-    engs, steppers = [], []
+    engs, states = [], []
     for gi, g in enumerate(groups):
         print(f"# context {gi + 1}/{len(groups)}: {len(g)} instances", file=sys.stderr, flush=True)
         e = HipBatchEngine(g, precision=abi.EB_PRECISION_FP32)
         e.set_exec_mode(args.exec == "graph")
         ws.install(e)
-        st = Stepper(e, g[0][1])
-        st.initialize()
-        run_steps(st, args.warmup)
+        e.initialize_state_simulation()
+        states.append(e.run_steps_batch(args.warmup))
         e.autotune()
         e.profile_enable(["beam"])
         engs.append(e)
-        steppers.append(st)
-    states = [abi.RunState(t=st.tn, dt=st.dt, successive=st.successive) for st in steppers]
-    t_start = [st.tn for st in steppers]
+    base = [[(r.steps, r.newton_iterations, r.failed_steps, r.t) for r in st] for st in states]
     t0 = time.perf_counter()
-    for _ in range(args.steps):   # the contexts in turn, one step each (eb_run_steps)
+    for _ in range(args.steps):   # the contexts in turn, one step of each instance (eb_run_steps_batch)
         for i, e in enumerate(engs):
-            states[i] = e.run_steps(1, states[i])
+            states[i] = e.run_steps_batch(1, states[i])
     wall = time.perf_counter() - t0
-    E_g = [int(e.beam_off[-1]) for e in engs]
-    done = [s.steps for s in states]
-    value = sum(E * d for E, d in zip(E_g, done)) / wall
-    its = sum(s.newton_iterations for s in states) / max(sum(done), 1)
-    failed = sum(s.failed_steps for s in states)
-    sim = min(s.t - t for s, t in zip(states, t_start))
+    value = steps_sum = its_sum = failed = 0
+    sims = []
+    for e, st, b0 in zip(engs, states, base):
+        for sidx, (r, (n0, i0, f0, t_0)) in enumerate(zip(st, b0)):
+            E_s = int(e.beam_off[sidx + 1] - e.beam_off[sidx])
+            value += E_s * (r.steps - n0)
+            steps_sum += r.steps - n0
+            its_sum += r.newton_iterations - i0
+            failed += r.failed_steps - f0
+            sims.append(r.t - t_0)
+    value /= wall
+    its = its_sum / max(steps_sum, 1)
+    sim = sum(sims) / len(sims)
     kms = kn = 0.0
     for e in engs:
         ms, n = e.profile_read("beam")
@@ -185,7 +316,7 @@ def sweep(args):
         kn += n
     chosen = [L.predictor_stats(e).tolist() for e in engs]
     avg_s = (kms / max(kn, 1)) * 1e-3
-    E_launch = sum(E_g) / len(E_g)
+    E_launch = sum(int(e.beam_off[-1]) for e in engs) / len(engs)
     out = {
         "metric": METRIC, "value": value, "unit": "element-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
@@ -197,6 +328,7 @@ def sweep(args):
                    "exec": args.exec, "dt": 1e-3, "tolerance": 1e-8, "beams": E_all},
         "newton_iters_per_step": its, "failed_steps": failed,
         "wall_s_per_simulated_s": wall / sim if sim > 0 else None,
+        "time_steps": "each stent its own Δt (eb_run_steps_batch)",
         "batch_autotune": {"instances_per_context": size, "element_steps_per_s": rates,
                            "launch_shapes": shapes.get(size)},
         "learned_choices_per_context": chosen,
@@ -245,7 +377,7 @@ def small_config(args):
         err[f] = float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
     E = len(conf.beams)
     f32, f64 = forms["fp32"], forms["fp64"]
-    avg_s = f32["kernel_ms"] * 1e-3 / max(f32["kernel_launches"], 1)
+    avg_s = kernel_avg_s(f32, "beam")
     out = {
         "metric": METRIC, "value": f32["value"], "unit": "element-steps/s", "n_gpus": 1, "steps": f32["steps"],
         "warmup": args.warmup, "ms_per_step": f32["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -265,7 +397,9 @@ def small_config(args):
                                       **err},
         "cpu_baseline": {"value": E * args.compare_steps / cpu_wall, "unit": "element-steps/s", "cores": 1,
                          "kind": "port", "sample": f"first {args.compare_steps} steps, C++ FP64 restatement, 1 thread"},
-        "roofline": {**roofline("beam_kernel<float>", avg_s, E, PEAK_FP32_TFLOPS), "launches": f32["kernel_launches"]},
+        "roofline": {**roofline("beam_kernel<float>" if inter is None else "beam_detect_kernel<float>", avg_s, E,
+                                PEAK_FP32_TFLOPS), "launches": f32["kernels"]["beam"][1]},
+        "hbm_rooflines": hbm_records(f32, 4),
     }
     print(json.dumps(out))
 
@@ -281,7 +415,8 @@ def main():
     ap.add_argument("--sdf-n", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["plain", "fast", "learned"], default=None, help="time one form only")
+    ap.add_argument("--only", default=None,
+                    help="comma-separated forms to time: plain, fp64_batch, fast, learned (default: all)")
     ap.add_argument("--train-steps", type=int, default=40, help="converged steps recorded per training stent")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--exec", choices=["graph", "eager"], default="eager",
@@ -324,18 +459,21 @@ def main():
 
     from endobeams_amd import learned as L
 
-    plain = fast = lrn = None
+    plain = f64b = fast = lrn = None
     batch_tuning = None
     xo = {"graph": args.exec == "graph", "autotune": not args.no_autotune}
-    if args.only in (None, "plain"):
+    log = lambda name, f: print(f"# {name}: {f['steps']} steps, {f['ms_per_step']:.2f} ms/step, {f['value']:.0f} "
+                                f"element-steps/s, {f['newton_iters_per_step']:.2f} Newton it./step",
+                                file=sys.stderr, flush=True)
+    forms = set(args.only.split(",")) if args.only else {"plain", "fp64_batch", "fast", "learned"}
+    if "plain" in forms:
         conf, params, inter = stent(sdf_n=args.sdf_n)
         eng = HipEngine(conf, params, inter)
         plain = timed(eng, params, len(conf.beams), args.warmup, args.steps, **xo)
-        print(f"# plain FP64: {plain['steps']} steps, {plain['ms_per_step']:.2f} ms/step, "
-              f"{plain['value']:.0f} element-steps/s", file=sys.stderr, flush=True)
+        log("plain FP64, one stent", plain)
         eng.close()
     inst = None
-    if args.only in (None, "fast", "learned"):
+    if forms & {"fp64_batch", "fast", "learned"}:
         t = time.perf_counter()
         cands = [int(c) for c in args.batch_candidates.split(",")] if args.batch == 0 else [args.batch]
         inst = [stent(seed=None if s == 0 else s, sdf_n=args.sdf_n) for s in range(max(cands))]
@@ -351,15 +489,18 @@ def main():
             args.batch = size
             inst = inst[:size]
         print(f"# batch of {args.batch} stents", file=sys.stderr, flush=True)
-    if args.only in (None, "fast"):
+    if "fp64_batch" in forms:
+        eng = HipBatchEngine(inst, precision=abi.EB_PRECISION_FP64)
+        f64b = timed_batch(eng, args.warmup, args.steps, **xo)
+        log("FP64 batch", f64b)
+        eng.close()
+    if "fast" in forms:
         eng = HipBatchEngine(inst, precision=abi.EB_PRECISION_FP32)
-        E = int(eng.beam_off[-1])
-        fast = timed(eng, inst[0][1], E, args.warmup, args.steps, **xo)
-        print(f"# fast: {fast['steps']} steps, {fast['ms_per_step']:.2f} ms/step, {fast['value']:.0f} element-steps/s",
-              file=sys.stderr, flush=True)
+        fast = timed_batch(eng, args.warmup, args.steps, **xo)
+        log("FP32 batch", fast)
         eng.close()
     training = None
-    if args.only in (None, "learned"):
+    if "learned" in forms:
         # training trajectories: 8 seeded stents not in the timed batch, FP32 form, reference predictor
         t = time.perf_counter()
         tr = [stent(seed=1000 + s, sdf_n=args.sdf_n) for s in range(8)]
@@ -376,19 +517,18 @@ def main():
               file=sys.stderr, flush=True)
         eng = HipBatchEngine(inst, precision=abi.EB_PRECISION_FP32)
         ws.install(eng)
-        E = int(eng.beam_off[-1])
-        lrn = timed(eng, inst[0][1], E, args.warmup, args.steps, **xo)
+        lrn = timed_batch(eng, args.warmup, args.steps, **xo)
         ch = L.predictor_stats(eng)
         training["network_start_kept"] = int(ch[0])
         training["fallback_to_reference_predictor"] = int(ch[1])
         training["network_skipped_by_backoff"] = int(ch[2])
-        print(f"# learned: {lrn['steps']} steps, {lrn['ms_per_step']:.2f} ms/step, {lrn['value']:.0f} element-steps/s, "
-              f"choices {ch.tolist()}", file=sys.stderr, flush=True)
+        log(f"learned + FP32 batch (choices {ch.tolist()})", lrn)
         eng.close()
 
-    main_form = lrn if lrn is not None else (fast if fast is not None else plain)
+    main_form = next(f for f in (lrn, fast, f64b, plain) if f is not None)
+    reduced = main_form is fast or main_form is lrn
     traffic = None
-    if main_form is not plain and args.sdf_n == 256:
+    if reduced and args.sdf_n == 256:
         # HBM bytes per launch of this kernel, from the committed FETCH_SIZE / WRITE_SIZE profiler
         # runs on 16 of these stents (scaled by a device copy of known size), per beam × beams
         tf = os.path.join(REPO, "profiles", "braided_stent_batch16_fp32_hbm_traffic.json")
@@ -396,9 +536,9 @@ def main():
             k = json.load(open(tf))["kernels"].get("ebd::beam_detect_kernel<float>")
             if k:
                 traffic = (k["fetch_bytes"] + k["write_bytes"]) / (16 * 4800) * main_form["E"]
-    avg_s = main_form["kernel_ms"] * 1e-3 / max(main_form["kernel_launches"], 1)
-    reduced = main_form is fast or main_form is lrn
+    avg_s = kernel_avg_s(main_form, "beam")
     peak = PEAK_FP32_TFLOPS if reduced else PEAK_FP64_TFLOPS
+    s_bytes = 4 if reduced else 8
     out = {
         "metric": METRIC,
         "value": main_form["value"],
@@ -416,25 +556,26 @@ def main():
                                f"{args.sdf_n}^3 FP32 vessel SDF, Newton tolerance 1e-8",
                    "form": ("learned warm start + " if main_form is lrn else "") +
                            (f"FP32 element matrices + FP64 forces/solver, batch of {args.batch} seeded stents"
-                            if reduced else "plain FP64, one stent"),
+                            if reduced else "plain FP64") +
+                           (", each stent with its own time step" if main_form is not plain else ", one stent"),
                    "dt": 1e-3, "tolerance": 1e-8, "beams_per_step": main_form["E"]},
         "newton_iters_per_step": main_form["newton_iters_per_step"],
+        "newton_iters_per_step_max": main_form["newton_iters_per_step_max"],
         "failed_steps": main_form["failed_steps"],
         "wall_s_per_simulated_s": main_form["wall_s_per_simulated_s"],
         "exec": main_form["exec"],
         "roofline": {**roofline("beam_detect_kernel<float>" if reduced else "beam_detect_kernel<double>", avg_s,
-                                main_form["E"], peak, traffic), "launches": main_form["kernel_launches"]},
+                                main_form["E"], peak, traffic), "launches": main_form["kernels"]["beam"][1]},
+        "hbm_rooflines": hbm_records(main_form, s_bytes),
     }
     if plain is not None:
-        out["plain_fp64"] = {"value": plain["value"], "ms_per_step": plain["ms_per_step"],
-                             "newton_iters_per_step": plain["newton_iters_per_step"],
-                             "wall_s_per_simulated_s": plain["wall_s_per_simulated_s"],
-                             "beam_kernel_us": 1e3 * plain["kernel_ms"] / max(plain["kernel_launches"], 1)}
+        out["plain_fp64"] = summary(plain)
         out["speedup_vs_plain_fp64"] = main_form["value"] / plain["value"]
+    if f64b is not None and main_form is not f64b:
+        out["fp64_batch"] = {**summary(f64b), "hbm_rooflines": hbm_records(f64b, 8)}
+        out["speedup_vs_fp64_batch"] = main_form["value"] / f64b["value"]
     if fast is not None and main_form is not fast:
-        out["fp32_batch_without_network"] = {"value": fast["value"], "ms_per_step": fast["ms_per_step"],
-                                             "newton_iters_per_step": fast["newton_iters_per_step"],
-                                             "failed_steps": fast["failed_steps"]}
+        out["fp32_batch_without_network"] = summary(fast)
     if training is not None:
         out["learned_training"] = training
     if batch_tuning is not None:

@@ -212,6 +212,7 @@ struct eb_ctx {
     int32_t tune[EB_TUNE_COUNT] = {};
     // this device's copy of the file-scope staging array (eb_output_points_noalloc)
     double* scratch = nullptr;
+    unsigned long long* contact_stats = nullptr;   // [2] listed beams, Gauss points in contact
 };
 
 namespace {
@@ -472,6 +473,7 @@ template <typename P> ContactArgs<P> contact_args(eb_ctx* ctx, double alpha) {
     ca.n = ctx->n_contact;
     ca.Tc = (double*)ctx->sTc;
     ca.energy = ctx->senergy;
+    ca.stats = ctx->contact_stats;
     return ca;
 }
 
@@ -525,10 +527,12 @@ template <typename P> int launch_gathers(eb_ctx* ctx, bool with_contact, bool ta
         // element staging when they are first read or when the step is committed (materialize_kcm)
         // (K, C, M are passed only when written: their buffers are exchanged when a step is committed)
         const int bs = gather_block(ctx);
+        if (tangent) prof_begin(ctx, EB_KERNEL_GATHER_MATRICES);
         hipLaunchKernelGGL(gather_matrices<P>, dim3(nblk(ctx->nnz / 4, bs)), dim3(bs), 0, ctx->stream, ctx->nnz,
                            ctx->blk_ptr, ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM,
                            tangent ? nullptr : ctx->K, tangent ? nullptr : ctx->C, tangent ? nullptr : ctx->M, ctx->Ktan,
                            c1, (const StepCoef*)ctx->coef, tangent ? 1 : 0, tangent ? 0 : 1, active_dev(ctx));
+        if (tangent) prof_end(ctx, EB_KERNEL_GATHER_MATRICES);
         EB_TRY(check_launch(ctx, "gather_matrices"));
         ctx->kcm_pending = tangent;
         if (tangent) ctx->ktan_tag = eb_ctx::TanTag{ctx->stage_ver, c1, ctx->coef_ver};
@@ -1643,6 +1647,7 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
         CK(upload_vec(ctx, ctx->is_contact, seen));
         CK(dalloc(ctx, ctx->clist, ctx->n_contact));
         CK(dalloc(ctx, ctx->clist_n, 1));
+        CK(dalloc(ctx, ctx->contact_stats, 2));
         if (d->surface == EB_SURFACE_SPHERE) {
             std::vector<SphereDev> sp((size_t)S);
             for (int64_t s = 0; s < S; ++s) {
@@ -2353,14 +2358,17 @@ int eb_run_steps_batch(eb_ctx* ctx, const eb_newton_params* p, const eb_time_par
     std::vector<double> dt_inst((size_t)S, 0.0);
     int st = EB_OK;
     for (;;) {
-        int64_t n_part = 0;
+        // every instance that has not stopped or reached the end time steps in every round, until
+        // none of them is short of max_steps: an instance that repeats a step with Δt/2 does not
+        // hold the others up
+        int64_t n_short = 0;
         for (int64_t s = 0; s < S; ++s) {
             const eb_run_state& r = states[s];
-            part[s] = (!r.stopped && done[s] < max_steps && tp->simulation_end_time - r.t > 1.0e-10) ? 1 : 0;
-            n_part += part[s];
+            part[s] = (!r.stopped && tp->simulation_end_time - r.t > 1.0e-10) ? 1 : 0;
+            n_short += (part[s] && done[s] < max_steps) ? 1 : 0;
             dt_inst[s] = r.dt > 0 ? r.dt : tp->initial_timestep;
         }
-        if (n_part == 0) break;
+        if (n_short == 0) break;
         for (int64_t s = 0; s < S && callback; ++s) {
             if (!part[s]) continue;
             const double tnp1 = states[s].t + states[s].dt;
@@ -2507,6 +2515,18 @@ int eb_output_points(eb_ctx* ctx, int32_t n_points, double* position, double* di
     release();
     if (st != EB_OK && ctx->err.empty()) ctx->err = "eb_output_points failed";
     return st;
+}
+
+int eb_get_contact_stats(eb_ctx* ctx, int64_t* out2) {
+    if (!ctx || !out2) return EB_ERR_ARG;
+    unsigned long long v[2] = {0, 0};
+    if (ctx->contact_stats) {
+        EB_HIP(hipStreamSynchronize(ctx->stream));
+        EB_HIP(hipMemcpy(v, ctx->contact_stats, sizeof v, hipMemcpyDeviceToHost));
+    }
+    out2[0] = (int64_t)v[0];
+    out2[1] = (int64_t)v[1];
+    return EB_OK;
 }
 
 int eb_output_points_noalloc(eb_ctx* ctx, int32_t n_points, double* position, double* displacement, double* velocity,

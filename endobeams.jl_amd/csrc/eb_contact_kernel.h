@@ -52,6 +52,7 @@ template <typename P> struct ContactArgs {
     int64_t n;
     double* Tc;            // [E][12] staging (written for every listed beam), FP64
     double* energy;        // [E][3] staging, contact energy in slot 2
+    unsigned long long* stats;   // [2]: listed beams and their Gauss points in contact, summed over launches
 };
 
 EBD bool isinside_dom(V3<double> p, const double* dom) {
@@ -456,7 +457,22 @@ __global__ void __launch_bounds__(4 * CONTACT_TILE) contact_kernel(NodeState ns,
                                                                    const int32_t* __restrict__ count) {
     const int part = __builtin_amdgcn_readfirstlane(threadIdx.x / CONTACT_TILE);
     const int64_t li = (int64_t)blockIdx.x * CONTACT_TILE + threadIdx.x % CONTACT_TILE;
-    if (li >= *count) return;
+    const int32_t nlisted = *count;
+    if (part == 0 && ca.stats) {
+        // the work of this launch: listed beams and Gauss points in contact (one wavefront's sums)
+        int nb = li < nlisted ? 1 : 0;
+        int ng = li < nlisted ? __popc((unsigned)(list[li] & 0xffu)) : 0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            nb += __shfl_down(nb, off, 64);
+            ng += __shfl_down(ng, off, 64);
+        }
+        if (threadIdx.x == 0 && nb > 0) {
+            atomicAdd(&ca.stats[0], (unsigned long long)nb);
+            atomicAdd(&ca.stats[1], (unsigned long long)ng);
+        }
+    }
+    if (li >= nlisted) return;
     const uint64_t ent = list[li];
     const int64_t e = (int64_t)(ent >> 8);
     const unsigned mask = (unsigned)(ent & 0xffu);

@@ -134,6 +134,7 @@ PROTOTYPES = {
     "get_instance_norms": (_i, [_vp, _dp]),
     "output_points": (_i, [_vp, C.c_int32, _dp, _dp, _dp, _dp, _dp]),
     "output_points_noalloc": (_i, [_vp, C.c_int32, _dp, _dp, _dp, _dp, _dp]),
+    "get_contact_stats": (_i, [_vp, _lp]),
     "get_sdf_coefficients": (_i, [_vp, C.c_int64, _dp, C.c_int64]),
     "set_exec_mode": (_i, [_vp, _i]),
     "get_exec_stats": (_i, [_vp, C.POINTER(C.c_int32), _lp, _lp]),
@@ -147,7 +148,7 @@ TUNE_NAMES = ("chain_lds_kb", "chain_solve_lds", "gather_block")
 EB_TUNE_COUNT, EB_TUNE_CANDIDATES = 3, 4
 TUNE_CANDIDATES = ((16, 32, 48, 96), (1, 2), (64, 128, 256))
 
-KERNELS = {"beam": 0, "contact": 1, "gather": 2, "precond": 3, "krylov": 4, "update": 5}
+KERNELS = {"beam": 0, "contact": 1, "gather": 2, "precond": 3, "krylov": 4, "update": 5, "gather_matrices": 6}
 
 # include/endobeams_learned.h (network handle is a void* here)
 _fp = C.POINTER(C.c_float)

@@ -2,7 +2,7 @@
 
 A sweep of independent structures can be advanced as one batch context or as several smaller
 ones. A larger batch fills the GPU better; a smaller one iterates only until its own slowest
-instance has converged and halves Δt only for its own instances. `pick_batch_size` measures
+instance has converged (each instance keeps its own Δt either way). `pick_batch_size` measures
 instead of guessing: for each candidate size it builds a context of that many instances (the
 first ones of the sweep), runs warm-up steps, lets the library's own autotuner (eb_autotune) pick
 the launch shapes on the live state, and times converged steps; the candidate with the highest
@@ -15,31 +15,22 @@ from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 from . import abi
 from .engine import HipBatchEngine
-from .simulation import Stepper
-
-
-def _advance(st: Stepper, n: int) -> Tuple[int, int]:
-    done = attempts = 0
-    while done < n and not st.finished:
-        attempts += 1
-        if st.advance():
-            done += 1
-    return done, attempts
 
 
 def time_batch(instances, precision=abi.EB_PRECISION_FP32, warmup=1, steps=2, tune=True):
-    """Element-steps/s of one context holding `instances`, with its launch shapes."""
+    """Element-steps/s of one context holding `instances` (each instance with its own Δt,
+    eb_run_steps_batch), with its launch shapes."""
     eng = HipBatchEngine(instances, precision=precision)
     try:
-        st = Stepper(eng, instances[0][1])
-        st.initialize()
-        _advance(st, warmup)
+        eng.initialize_state_simulation()
+        states = eng.run_steps_batch(warmup)
         shapes = eng.autotune()[0] if tune else eng.tuning
-        E = int(eng.beam_off[-1])
+        E = [int(eng.beam_off[s + 1] - eng.beam_off[s]) for s in range(eng.S)]
+        n0 = [r.steps for r in states]
         t0 = time.perf_counter()
-        done, _ = _advance(st, steps)
+        states = eng.run_steps_batch(steps, states)
         wall = time.perf_counter() - t0
-        return E * done / wall, shapes
+        return sum(e * (r.steps - n) for e, r, n in zip(E, states, n0)) / wall, shapes
     finally:
         eng.close()
 

@@ -108,6 +108,13 @@ class Engine:
         self._check(self.fn["get_counters"](self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "get_counters")
         return out
 
+    def contact_stats(self):
+        """(beams listed in contact, their Gauss points in contact), summed over the contact-tangent
+        launches since the context was created."""
+        out = np.zeros(2, np.int64)
+        self._check(self.fn["get_contact_stats"](self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "get_contact_stats")
+        return int(out[0]), int(out[1])
+
     def profile_enable(self, kernels=()):
         mask = 0
         for k in kernels:
@@ -313,9 +320,10 @@ class Engine:
 
     def run_steps_batch(self, max_steps: int, states=None):
         """eb_run_steps_batch: every instance advances with its own time and Δt (run_simulation!'s
-        loop per instance) until it has `max_steps` more converged steps, reaches the end time or
-        aborts (stopped = 2). `states` is a ctypes array of RunState, one per instance (fresh
-        ones: t = 0, Δt = initial_timestep); returns it."""
+        loop per instance), all of them stepping together until none that has not reached the end
+        time or aborted (stopped = 2) has fewer than `max_steps` more converged steps (some may end
+        with more). `states` is a ctypes array of RunState, one per instance (fresh ones: t = 0,
+        Δt = initial_timestep); returns it."""
         S = getattr(self, "S", 1)
         if states is None:
             states = (abi.RunState * S)()

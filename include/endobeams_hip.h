@@ -271,9 +271,10 @@ typedef int (*eb_instance_callback)(eb_ctx* ctx, int64_t instance, double t, voi
 
 /* run_simulation!'s time loop (src/run_simulation.jl:38-112) for every instance of a batch with
  * its own time and Δt: states[s] (n_instances records, as eb_run_steps keeps them) holds instance
- * s's progress. Each round, the instances that have fewer than max_steps new converged steps in
- * this call, have not reached simulation_end_time and have not stopped attempt one step
- * together, each with its own Δt; per instance, k > max_iterations halves Δt and repeats its step
+ * s's progress. Each round, every instance that has not reached simulation_end_time and has not
+ * stopped attempts one step, with its own Δt, until none of them has fewer than max_steps new
+ * converged steps in this call (an instance that repeats a step does not hold the others up, so
+ * some end with more than max_steps new steps); per instance, k > max_iterations halves Δt and repeats its step
  * (update_not_converged! for that instance alone), Δt below min_timestep ends that instance
  * (stopped = 2) without stopping the others, and a converged step is committed for that instance
  * (update_converged!) with the ×1.5 growth after accelerate_after_success_it easy steps.
@@ -306,10 +307,15 @@ int eb_get_counters(const eb_ctx* ctx, int64_t* out4);
 #define EB_KERNEL_PRECOND 3       /* chain factorisation */
 #define EB_KERNEL_KRYLOV 4        /* one BiCGStab iteration (all its kernels) */
 #define EB_KERNEL_UPDATE 5        /* Newmark/rotation update and norms */
-#define EB_KERNEL_COUNT 6
+#define EB_KERNEL_GATHER_MATRICES 6   /* the tangent's gather alone (part of EB_KERNEL_GATHER) */
+#define EB_KERNEL_COUNT 7
 int eb_profile_enable(eb_ctx* ctx, uint32_t mask);
 /* total milliseconds and number of timed launches of kernel `id` since enabling */
 int eb_profile_read(eb_ctx* ctx, int id, double* total_ms, int64_t* launches);
+
+/* beams listed in contact by the detection and their Gauss points in contact, summed over the
+ * contact-tangent launches since eb_create (out2[0], out2[1]): the work of those launches */
+int eb_get_contact_stats(eb_ctx* ctx, int64_t* out2);
 
 /* --- output ----------------------------------------------------------------------------------- */
 

@@ -375,6 +375,11 @@ int ebo_set_field(void* h, int f, const double* in, int64_t count) {
 }
 
 // assemble!, src/beams/assemble_beams.jl:1-70 (serial loop: the scatter order is beam order)
+// The element computations of a chunk of beams run in parallel (OpenMP in the timing build,
+// liboracle_omp.so), then their contributions are added in beam order, as the serial loop of
+// assemble! adds them: the results do not depend on the thread count.
+constexpr int64_t ORACLE_CHUNK = 1024;
+
 int ebo_assemble(void* h) {
     Oracle* o = (Oracle*)h;
     std::fill(o->K.begin(), o->K.end(), 0.0);
@@ -383,21 +388,30 @@ int ebo_assemble(void* h) {
     std::fill(o->Tint.begin(), o->Tint.end(), 0.0);
     std::fill(o->Tk.begin(), o->Tk.end(), 0.0);
     std::fill(o->Tc.begin(), o->Tc.end(), 0.0);
-    BeamOut out;
-    for (int64_t e = 0; e < o->E; ++e) {
-        ElemIn in = gather_elem(o, e);
-        BeamProps bp = props_from(&o->props[EB_NPROPS * e]);
-        compute_beams(in, bp, o->g, out);
-        o->energy[0] += out.strain;
-        o->energy[1] += out.kinetic;
-        int64_t dofs[12];
-        for (int k = 0; k < 6; ++k) { dofs[k] = 6 * o->n1[e] + k; dofs[6 + k] = 6 * o->n2[e] + k; }
-        for (int k = 0; k < 12; ++k) o->Tk[dofs[k]] += out.Tk[k];
-        for (int k = 0; k < 12; ++k) o->Tint[dofs[k]] += out.Tint[k];
-        const int64_t* mp = &o->smap[144 * e];
-        for (int k = 0; k < 144; ++k) o->K[mp[k]] += out.K.a[k];
-        for (int k = 0; k < 144; ++k) o->C[mp[k]] += out.C.a[k];
-        for (int k = 0; k < 144; ++k) o->M[mp[k]] += out.M.a[k];
+    std::vector<BeamOut> outs((size_t)std::min<int64_t>(ORACLE_CHUNK, o->E));
+    for (int64_t e0 = 0; e0 < o->E; e0 += ORACLE_CHUNK) {
+        const int64_t n = std::min<int64_t>(ORACLE_CHUNK, o->E - e0);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t e = e0 + i;
+            ElemIn in = gather_elem(o, e);
+            BeamProps bp = props_from(&o->props[EB_NPROPS * e]);
+            compute_beams(in, bp, o->g, outs[i]);
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t e = e0 + i;
+            const BeamOut& out = outs[i];
+            o->energy[0] += out.strain;
+            o->energy[1] += out.kinetic;
+            int64_t dofs[12];
+            for (int k = 0; k < 6; ++k) { dofs[k] = 6 * o->n1[e] + k; dofs[6 + k] = 6 * o->n2[e] + k; }
+            for (int k = 0; k < 12; ++k) o->Tk[dofs[k]] += out.Tk[k];
+            for (int k = 0; k < 12; ++k) o->Tint[dofs[k]] += out.Tint[k];
+            const int64_t* mp = &o->smap[144 * e];
+            for (int k = 0; k < 144; ++k) o->K[mp[k]] += out.K.a[k];
+            for (int k = 0; k < 144; ++k) o->C[mp[k]] += out.C.a[k];
+            for (int k = 0; k < 144; ++k) o->M[mp[k]] += out.M.a[k];
+        }
     }
     o->counters[3]++;
     return EB_OK;
@@ -407,18 +421,28 @@ int ebo_assemble(void* h) {
 int ebo_assemble_contact(void* h, double alpha) {
     Oracle* o = (Oracle*)h;
     if (o->surf.kind == EB_SURFACE_NONE) return EB_OK;
-    ContactOut out;
-    for (int64_t e : o->contact_beams) {
-        ElemIn in = gather_elem(o, e);
-        double rb = o->props[EB_NPROPS * e + 0];
-        compute_contact_beams(in, rb, o->cp, o->surf, o->g, out);
-        o->energy[2] += out.energy;
-        int64_t dofs[12];
-        for (int k = 0; k < 6; ++k) { dofs[k] = 6 * o->n1[e] + k; dofs[6 + k] = 6 * o->n2[e] + k; }
-        for (int k = 0; k < 12; ++k) o->Tc[dofs[k]] += out.Tc[k];
-        const int64_t* mp = &o->smap[144 * e];
-        for (int k = 0; k < 144; ++k) o->K[mp[k]] += -out.K.a[k];
-        for (int k = 0; k < 144; ++k) o->C[mp[k]] += -((1 + alpha) * out.C.a[k]);
+    const int64_t nc = (int64_t)o->contact_beams.size();
+    std::vector<ContactOut> outs((size_t)std::min<int64_t>(ORACLE_CHUNK, nc));
+    for (int64_t c0 = 0; c0 < nc; c0 += ORACLE_CHUNK) {
+        const int64_t n = std::min<int64_t>(ORACLE_CHUNK, nc - c0);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t e = o->contact_beams[c0 + i];
+            ElemIn in = gather_elem(o, e);
+            double rb = o->props[EB_NPROPS * e + 0];
+            compute_contact_beams(in, rb, o->cp, o->surf, o->g, outs[i]);
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t e = o->contact_beams[c0 + i];
+            const ContactOut& out = outs[i];
+            o->energy[2] += out.energy;
+            int64_t dofs[12];
+            for (int k = 0; k < 6; ++k) { dofs[k] = 6 * o->n1[e] + k; dofs[6 + k] = 6 * o->n2[e] + k; }
+            for (int k = 0; k < 12; ++k) o->Tc[dofs[k]] += out.Tc[k];
+            const int64_t* mp = &o->smap[144 * e];
+            for (int k = 0; k < 144; ++k) o->K[mp[k]] += -out.K.a[k];
+            for (int k = 0; k < 144; ++k) o->C[mp[k]] += -((1 + alpha) * out.C.a[k]);
+        }
     }
     return EB_OK;
 }
@@ -769,6 +793,12 @@ int ebo_run_steps_batch(void* h, const eb_newton_params* p, const eb_time_params
         if (st != EB_OK) return st;
         state->t = tnp1;
     }
+    return EB_OK;
+}
+
+// the oracle has no contact-tangent launches to count
+int ebo_get_contact_stats(void* h, int64_t* out2) {
+    out2[0] = out2[1] = 0;
     return EB_OK;
 }
 

@@ -20,19 +20,22 @@ from endobeams_amd import abi  # noqa: E402
 from endobeams_amd.engine import Engine  # noqa: E402
 
 LIB = os.path.join(_HERE, "liboracle.so")
-_lib = None
+# the same restatement with its element and contact loops over OpenMP threads (timing only)
+LIB_OMP = os.path.join(_HERE, "liboracle_omp.so")
+_libs = {}
 
 
 def build():
     subprocess.run(["make", "-C", _HERE, "-s"], check=True)
 
 
-def load():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def load(timing=False):
+    path = LIB_OMP if timing else LIB
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        _lib = C.CDLL(LIB)
+        _lib = C.CDLL(path)
+        _libs[path] = _lib
         dp, lp = C.POINTER(C.c_double), C.POINTER(C.c_int64)
         _lib.ebo_compute_beam.argtypes = [dp, dp, C.c_int, dp, dp, dp]
         _lib.ebo_compute_contact_beam.argtypes = [C.c_void_p, dp, C.c_double, dp]
@@ -42,7 +45,7 @@ def load():
         _lib.ebo_rot_util.argtypes = [C.c_int, dp, dp]
         _lib.ebo_regularization.argtypes = [C.c_double, C.c_double, C.c_double, dp]
         _lib.ebo_band_info.argtypes = [C.c_void_p, lp, lp]
-    return _lib
+    return _libs[path]
 
 
 def _dp(a):
@@ -53,8 +56,8 @@ class OracleEngine(Engine):
     """Same interface as endobeams_amd.HipEngine, computed on the CPU by the restatement."""
     prefix = "ebo_"
 
-    def __init__(self, conf, params, inter=None, precision=abi.EB_PRECISION_FP64):
-        super().__init__(load(), conf, params, inter, precision)
+    def __init__(self, conf, params, inter=None, precision=abi.EB_PRECISION_FP64, timing=False):
+        super().__init__(load(timing), conf, params, inter, precision)
 
     def band_info(self):
         kl, ku = C.c_int64(0), C.c_int64(0)

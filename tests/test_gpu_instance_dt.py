@@ -49,8 +49,9 @@ def alone(case, precision, steps):
 @pytest.mark.parametrize("precision", [abi.EB_PRECISION_FP64, abi.EB_PRECISION_FP32])
 def test_batch_instances_keep_their_own_time_step(precision):
     """Three L-cantilevers with tip loads 5, 50 and 500 (max_iterations 8): the heavier loads
-    halve Δt at different steps. In one batch each instance ends where it ends alone: the same
-    time, Δt, converged and repeated steps, Newton iterations, and state, bit for bit."""
+    halve Δt at different steps. In one batch each instance ends where it ends alone after as many
+    converged steps: the same Newton iterations, time, Δt and repeated steps, and state, bit for
+    bit."""
     loads = (5.0, 50.0, 500.0)
     steps = 12
     cases = [cantilever(f) for f in loads]
@@ -59,11 +60,15 @@ def test_batch_instances_keep_their_own_time_step(precision):
     st = b.run_steps_batch(steps, states_of(b, len(cases)))
     failed = []
     for s, case in enumerate(cases):
-        e, ref = alone(case, precision, steps)
         r = st[s]
-        assert (r.steps, r.failed_steps, r.newton_iterations, r.successive, r.stopped) == \
-               (ref.steps, ref.failed_steps, ref.newton_iterations, ref.successive, ref.stopped), s
-        assert r.t == ref.t and r.dt == ref.dt, (s, r.t, ref.t, r.dt, ref.dt)
+        # instances keep stepping until the slowest has its quota: some have more steps
+        assert r.steps >= steps
+        e, ref = alone(case, precision, r.steps)
+        assert (r.steps, r.newton_iterations) == (ref.steps, ref.newton_iterations), s
+        if r.last_k <= case[1].max_iterations:
+            # the last attempt converged (no repeated attempt after the last converged step)
+            assert (r.failed_steps, r.successive) == (ref.failed_steps, ref.successive), s
+            assert r.t == ref.t and r.dt == ref.dt, (s, r.t, ref.t, r.dt, ref.dt)
         for f in FIELDS:
             np.testing.assert_array_equal(b.get(f)[b.instance_slice(f, s)], e.get(f), err_msg=f"{s} {f}")
         failed.append(r.failed_steps)
@@ -87,6 +92,7 @@ def test_batch_instance_with_halvings_matches_the_oracle():
     ref = o.run_steps(steps)
     r = st[1]
     assert ref.failed_steps >= 2
+    # the hard instance is the slowest: it ends at its quota, like the oracle's run
     assert (r.steps, r.failed_steps, r.newton_iterations) == (ref.steps, ref.failed_steps, ref.newton_iterations)
     assert r.t == pytest.approx(ref.t, rel=1e-15)
     for f in FIELDS:
@@ -142,7 +148,7 @@ def test_a_failed_linear_solve_is_a_newton_failure_of_its_instance():
     w, n = st[0], st[1]
     assert n.stopped == 2 and n.steps == 0 and n.failed_steps == 4 and n.last_k == 6
     assert w.stopped == 0 and w.steps == 6 and w.failed_steps == 0
-    e, ref = alone(wire, abi.EB_PRECISION_FP64, 6)
+    e, ref = alone(wire, abi.EB_PRECISION_FP64, w.steps)
     assert (w.steps, w.newton_iterations) == (ref.steps, ref.newton_iterations) and w.t == ref.t
     for f in FIELDS:
         np.testing.assert_array_equal(b.get(f)[b.instance_slice(f, 0)], e.get(f), err_msg=f)

@@ -252,6 +252,46 @@ def test_full_size_assembly_parity(builder):
         assert rel_err(g.get(f), o.get(f), floor=1e-300) <= TOL, f
 
 
+IN_CONTACT = {
+    # the stent expanded to the vessel wall (radius 2.96 mm in the 3 mm vessel, wire radius 0.05 mm)
+    "braided_stent": (lambda: configs.braided_stent(crimp_radius=2.96e-3), 0.9),
+    # the 40×40 net pressed 2 cm into the sphere at its centre (a disc of about 0.16 m radius in contact)
+    "net_on_sphere": (lambda: configs.net_on_sphere(gap=-0.02), 0.01),
+}
+
+
+@pytest.mark.parametrize("builder", list(IN_CONTACT))
+def test_full_size_assembly_parity_in_contact(builder):
+    """Full-size structures in contact (4800 and 3120 beams): assembly + contact from the
+    configured state with small random perturbations of positions (1e-3 of the wire radius),
+    velocities and rotations; the contact path runs on most of the stent's beams and on the
+    net's centre."""
+    build, min_share = IN_CONTACT[builder]
+    conf, params, inter = build()
+    g, o = HipEngine(conf, params, inter), O.OracleEngine(conf, params, inter)
+    rng = np.random.default_rng(17)
+    N = len(conf.nodes)
+    rb = float(conf.beams.props[0][0])
+    vals = {"U": g.get("U") + rng.standard_normal(3 * N) * 1e-3 * rb,
+            "UDOT": rng.standard_normal(3 * N) * 1e-3, "WDOT": rng.standard_normal(3 * N) * 1e-2}
+    R0 = g.get("R").reshape(N, 3, 3).transpose(0, 2, 1)
+    dR = Rotation.from_rotvec(rng.standard_normal((N, 3)) * 1e-4).as_matrix()
+    vals["DR"] = np.transpose(dR, (0, 2, 1)).reshape(-1)
+    vals["R"] = np.transpose(dR @ R0, (0, 2, 1)).reshape(-1)
+    for e in (g, o):
+        for k, v in vals.items():
+            e.set(k, v)
+        e.assemble()
+        e.assemble_contact(params.alpha)
+    listed, points = g.contact_stats()
+    assert listed >= min_share * len(conf.beams), (listed, len(conf.beams))
+    for f in ("TINT", "TK", "TC", "K", "C", "M"):
+        assert rel_err(g.get(f), o.get(f), floor=1e-300) <= TOL, f
+    assert np.linalg.norm(o.get("TC")) > 0
+    e_g, e_o = g.get("ENERGY"), o.get("ENERGY")
+    assert e_g[2] == pytest.approx(e_o[2], rel=1e-10)
+
+
 def test_predictor_reuses_last_factorisation():
     """With Δt unchanged, the predictor's tangent (1+α)Kⁿ + γ/(βΔt)Cⁿ + Mⁿ/(βΔt²) is the last
     corrector iteration's tangent bit for bit, so its chain factors are reused: one
