@@ -79,6 +79,8 @@ struct eb_ctx {
     SphereDev* d_sphere = nullptr;  // [S]
     // learned warm start
     eb_mlp* mlp = nullptr;
+    std::vector<float*> lact;       // the network's layer outputs for this context's nodes (owned here)
+    double* ldD = nullptr;          // the increment of the network's start at its last evaluation
     float *lin_scale = nullptr, *lout_scale = nullptr, *ones = nullptr, *lX = nullptr, *lT = nullptr;
     double* hist = nullptr;         // rates of the last three converged steps of each instance (3 slots of 6N)
     int32_t* newest = nullptr;      // [S] slot each instance wrote last
@@ -1172,13 +1174,14 @@ int learned_predictor(eb_ctx* ctx, const eb_newton_params* p) {
                        ctx->newest, ctx->node_inst, ctx->Tc_n, (const StepCoef*)ctx->coef, ctx->lin_scale, ctx->lX);
     EB_TRY(check_launch(ctx, "learned_features"));
     const float* Y = nullptr;
-    if (ebm::mlp_infer(ctx->mlp, ctx->lX, N, ctx->stream, &Y) != EB_OK) {
-        ctx->err = std::string("network inference: ") + eb_mlp_last_error(ctx->mlp);
+    if (ebm::mlp_infer(ctx->mlp, ctx->lX, N, ctx->stream, ctx->lact.data(), &Y) != EB_OK) {
+        ctx->err = "network inference: forward pass launch failed";
         return EB_ERR_DEVICE;
     }
     hipLaunchKernelGGL(learned_apply, dim3(nblk(6 * N, 256)), dim3(256), 0, ctx->stream, N, Y, ctx->pdD,
                        ctx->lout_scale, ctx->is_fixed, ctx->dD);
     EB_TRY(check_launch(ctx, "learned_apply"));
+    EB_HIP(hipMemcpyAsync(ctx->ldD, ctx->dD, sizeof(double) * ctx->ndofs, hipMemcpyDeviceToDevice, ctx->stream));
     EB_TRY(global_predictor_async(ctx));
     EB_TRY(local_predictor_async(ctx));
     EB_TRY(start_residual(ctx, p, dt, false));
@@ -1591,6 +1594,7 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
     }
     CK(dalloc(ctx, ctx->hist, 18 * N));
     CK(dalloc(ctx, ctx->pdD, 6 * N));
+    CK(dalloc(ctx, ctx->ldD, 6 * N));
     CK(dalloc(ctx, ctx->snap, 54 * N));
     CK(dalloc(ctx, ctx->snap_l, 54 * N));
     CK(dalloc(ctx, ctx->rn_learned, 2 * S));
@@ -1771,6 +1775,7 @@ int eb_destroy(eb_ctx* ctx) {
     if (ctx->h_nall) (void)hipHostFree(ctx->h_nall);
     if (ctx->h_pred) (void)hipHostFree(ctx->h_pred);
     if (ctx->h_kiter) (void)hipHostFree(ctx->h_kiter);
+    for (float* q : ctx->lact) (void)hipFree(q);
     for (void* q : {(void*)ctx->h_coef, (void*)ctx->h_part, (void*)ctx->h_sel, (void*)ctx->h_eligible, (void*)ctx->h_newest})
         if (q) (void)hipHostFree(q);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -2446,12 +2451,28 @@ int eb_set_learned_predictor(eb_ctx* ctx, eb_mlp* mlp, const float* in_scale, co
     EB_HIP(hipMemcpyAsync(ctx->lin_scale, in_scale, sizeof(float) * LEARN_IN, hipMemcpyHostToDevice, ctx->stream));
     EB_HIP(hipMemcpyAsync(ctx->lout_scale, out_scale, sizeof(float) * LEARN_OUT, hipMemcpyHostToDevice, ctx->stream));
     EB_HIP(hipStreamSynchronize(ctx->stream));
-    if (ebm::mlp_reserve(mlp, ctx->N) != EB_OK) {
-        ctx->err = "learned predictor: the network's inference workspace could not be allocated";
-        return EB_ERR_DEVICE;
+    // this context's layer outputs for its nodes (replacing those of a network installed before)
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    for (float* q : ctx->lact) (void)hipFree(q);
+    ctx->lact.assign((size_t)ebm::mlp_layers(mlp), nullptr);
+    for (int l = 0; l < ebm::mlp_layers(mlp); ++l) {
+        void* q = nullptr;
+        if (hipMalloc(&q, sizeof(float) * (size_t)std::max<int64_t>(ctx->N, 1) * ebm::mlp_layer_width(mlp, l)) != hipSuccess) {
+            ctx->err = "learned predictor: the network's layer outputs could not be allocated";
+            return EB_ERR_DEVICE;
+        }
+        ctx->lact[l] = (float*)q;
     }
     ctx->mlp = mlp;
     ctx->learn_backoff = ctx->learn_wait = 0;
+    return EB_OK;
+}
+
+int eb_get_learned_increment(eb_ctx* ctx, double* out, int64_t count) {
+    if (!ctx || !out) return EB_ERR_ARG;
+    if (count != ctx->ndofs) { ctx->err = "increment size mismatch"; return EB_ERR_ARG; }
+    EB_HIP(hipMemcpyAsync(out, ctx->ldD, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
 }
 

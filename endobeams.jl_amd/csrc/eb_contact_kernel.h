@@ -161,12 +161,17 @@ EBD void contact_gap(const ContactArgs<C>& ca, int32_t inst, V3<double> p, doubl
 
 // Consistent contact tangent of one beam for block column B of Kᶜ and Cᶜ (rows a = 0..3), and
 // Tᶜ of node slot B; the beam has a Gauss point in contact (mask: the points in contact).
+// B = CONTACT_ALL_SLOTS (force only): Tᶜ of all four slots and the contact energy in one pass.
 enum { CONTACT_FORCE = 1, CONTACT_TANGENT = 2 };
+constexpr int CONTACT_ALL_SLOTS = 4;
 
 template <typename P, typename C, int B, int WHAT>
 EBD void contact_column(int64_t e, unsigned mask, const NodeState& ns, const BeamConsts& bc, const BeamStage<C>& st,
                         const GaussPts& gp, const ContactArgs<C>& ca) {
     constexpr bool FORCE = (WHAT & CONTACT_FORCE) != 0, TANGENT = (WHAT & CONTACT_TANGENT) != 0;
+    constexpr bool ALL = B == CONTACT_ALL_SLOTS;
+    static_assert(!(ALL && TANGENT), "the tangent is computed one block column at a time");
+    constexpr int BC = ALL ? 0 : B;   // the block column of the tangent
     const int64_t E = bc.E;
     const int32_t inst = ca.beam_inst[e];
     const int32_t na = bc.n1[e], nb = bc.n2[e];
@@ -198,7 +203,8 @@ EBD void contact_column(int64_t e, unsigned mask, const NodeState& ns, const Bea
     const V3<P> tb1 = cvt<P>(Tb1), tb2 = cvt<P>(Tb2);
     P eta_n = P(ca.eta_n);  // overwritten by the smoothstep at each contacting point (:100)
     double energy = 0;
-    V3<P> Tcb = zv<P>();
+    V3<P> Tcs[ALL ? 4 : 1];
+    for (auto& t : Tcs) t = zv<P>();
     M3<P> Kc[4], Cc[4];
     for (int a = 0; a < 4; ++a) { Kc[a] = zm<P>(); Cc[a] = zm<P>(); }
 #pragma unroll 1
@@ -262,7 +268,13 @@ EBD void contact_column(int64_t e, unsigned mask, const NodeState& ns, const Bea
         V3<P> Fc = tmul(Re, fc);
         M3<P> RHR[4];
         for (int a = 0; a < 4; ++a) RHR[a] = (Re * H1[a]) * ReT;
-        if (FORCE) Tcb += w * tmul(RHR[B], fc);
+        if constexpr (FORCE) {
+            if constexpr (ALL) {
+                for (int a = 0; a < 4; ++a) Tcs[a] += w * tmul(RHR[a], fc);
+            } else {
+                Tcs[0] += w * tmul(RHR[BC], fc);
+            }
+        }
         if (!TANGENT) continue;
         M3<P> A1F = rows(P(0), P(0), P(0), Fc.y * v1.x, Fc.y * v1.y, Fc.y * v1.z, Fc.z * v1.x, Fc.z * v1.y, Fc.z * v1.z);
         M3<P> SF = skew(Fc);
@@ -282,7 +294,7 @@ EBD void contact_column(int64_t e, unsigned mask, const NodeState& ns, const Bea
         M3<P> auxn = ((kn * dpn) * nn + (kn * pn) * Hg) - outer(deta * udn, n);
         M3<P> aux1 = outer((((-mureg) * kn) * dpn) * udt, n);
         M3<P> aux2 = (((-mureg) * kn) * pn) * (I - outer((P(1) / (udt2 + eps_t)) * udt, udt));
-        constexpr int b = B;
+        constexpr int b = BC;
         M3<P> dundd = aux * RHR[b] + nn * du0dd[b];
         M3<P> dutdd = du0dd[b] - dundd;
         M3<P> dutdv = RHR[b] - nn * RHR[b];
@@ -325,11 +337,14 @@ EBD void contact_column(int64_t e, unsigned mask, const NodeState& ns, const Bea
     }
     const P l02 = P(l0 / 2);
     if (FORCE) {
-        if (B == 0) ca.energy[3 * e + 2] = double(l02) * energy;
-        V3<P> t = l02 * Tcb;
-        ca.Tc[e * 12 + 3 * B] = double(t.x);
-        ca.Tc[e * 12 + 3 * B + 1] = double(t.y);
-        ca.Tc[e * 12 + 3 * B + 2] = double(t.z);
+        if (B == 0 || ALL) ca.energy[3 * e + 2] = double(l02) * energy;
+        for (int k = 0; k < (ALL ? 4 : 1); ++k) {
+            const int slot = ALL ? k : BC;
+            V3<P> t = l02 * Tcs[k];
+            ca.Tc[e * 12 + 3 * slot] = double(t.x);
+            ca.Tc[e * 12 + 3 * slot + 1] = double(t.y);
+            ca.Tc[e * 12 + 3 * slot + 2] = double(t.z);
+        }
     }
     if (!TANGENT) return;
     const M3<P> TT1t = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, na)))));
@@ -345,7 +360,7 @@ EBD void contact_column(int64_t e, unsigned mask, const NodeState& ns, const Bea
         for (int j = 0; j < 3; ++j)
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                const int64_t idx = stage_idx(e, 3 * a + i, 3 * B + j);
+                const int64_t idx = stage_idx(e, 3 * a + i, 3 * BC + j);
                 st.K[idx] += C(-K.a[i + 3 * j]);
                 st.C[idx] += C(-(c1a * Cm.a[i + 3 * j]));
             }
@@ -434,16 +449,29 @@ __global__ void __launch_bounds__(4 * BEAM_TILE) beam_detect_kernel(NodeState ns
 }
 
 // One block column: FP64 throughout for the FP64 form; for the precision form (C = float) the
-// contact force and energy still in FP64 (they enter the residual), the tangent in FP32.
+// tangent in FP32, its contact force and energy (FP64: they enter the residual) computed by
+// contact_force_kernel before.
 template <typename C, int B>
 EBD void contact_part(int64_t e, unsigned mask, const NodeState& ns, const BeamConsts& bc, const BeamStage<C>& st,
                       const GaussPts& gp, const ContactArgs<C>& ca) {
     if constexpr (sizeof(C) == sizeof(double)) {
         contact_column<double, C, B, CONTACT_FORCE | CONTACT_TANGENT>(e, mask, ns, bc, st, gp, ca);
     } else {
-        contact_column<double, C, B, CONTACT_FORCE>(e, mask, ns, bc, st, gp, ca);
         contact_column<C, C, B, CONTACT_TANGENT>(e, mask, ns, bc, st, gp, ca);
     }
+}
+
+// Precision form: the FP64 contact force (Tᶜ of all four node slots) and energy of every listed
+// beam, one lane per beam, with the arithmetic of the FP64 pass of each block column.
+template <typename P>
+__global__ void __launch_bounds__(256) contact_force_kernel(NodeState ns, BeamConsts bc, BeamStage<P> st, GaussPts gp,
+                                                            ContactArgs<P> ca, const uint64_t* __restrict__ list,
+                                                            const int32_t* __restrict__ count) {
+    const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= *count) return;
+    const uint64_t ent = list[li];
+    contact_column<double, P, CONTACT_ALL_SLOTS, CONTACT_FORCE>((int64_t)(ent >> 8), (unsigned)(ent & 0xffu), ns, bc, st,
+                                                                gp, ca);
 }
 
 // Pass 2: four wavefronts per 64 listed beams, wavefront b computing block column b. Launched for

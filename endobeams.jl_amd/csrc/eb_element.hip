@@ -37,6 +37,12 @@ void launch_contact_detect(unsigned grid, hipStream_t stream, NodeState ns, Beam
 template <typename P>
 void launch_contact_kernel(unsigned grid, hipStream_t stream, NodeState ns, BeamConsts bc, BeamStage<P> st,
                                  GaussPts gp, ContactArgs<P> ca, const uint64_t* list, const int32_t* count) {
+    if constexpr (sizeof(P) == sizeof(float)) {
+        // the FP64 contact forces first, one lane per listed beam (grid * CONTACT_TILE lanes cover
+        // the list), then the FP32 tangent, four wavefronts per 64 beams
+        hipLaunchKernelGGL(contact_force_kernel<P>, dim3((unsigned)((grid * (int64_t)CONTACT_TILE + 255) / 256)), dim3(256),
+                           0, stream, ns, bc, st, gp, ca, list, count);
+    }
     hipLaunchKernelGGL(contact_kernel<P>, dim3(grid), dim3(4 * CONTACT_TILE), 0, stream, ns, bc, st, gp, ca, list,
                        count);
 }

@@ -417,7 +417,10 @@ __global__ void __launch_bounds__(RED_BLOCK) residual_masked(Segs sg, const int3
 // are saved (hist) for the right-hand-side reduction of the solve. Depth: ceil(log2 n) levels
 // instead of n sequential 6x6 steps, and every level is parallel over the chain.
 
-constexpr int CHAIN_THREADS = 256;                       // factorisation workgroup
+#ifndef EB_CHAIN_THREADS
+#define EB_CHAIN_THREADS 256
+#endif
+constexpr int CHAIN_THREADS = EB_CHAIN_THREADS;          // factorisation workgroup
 constexpr int CHAIN_GROUPS = (CHAIN_THREADS / 64) * 10;  // six-lane groups: ten per wavefront
 constexpr int GJ_BUF = 24;                               // doubles of LDS exchange per group
 // solve workgroup: 1024 threads while the chains are few (each chain's latency sets the time), 256

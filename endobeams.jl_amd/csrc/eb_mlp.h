@@ -7,12 +7,14 @@
 struct eb_mlp;
 
 namespace ebm {
-// forward pass of `rows` rows of X (device, row-major) on `stream`; *Y points at the network's
-// output buffer (rows × output width), valid until the next call on this network
-int mlp_infer(eb_mlp* mlp, const float* X_dev, int64_t rows, hipStream_t stream, const float** Y_dev);
-// the inference workspace for `rows` rows (layer outputs only), allocated now rather than at the
-// first inference
-int mlp_reserve(eb_mlp* mlp, int64_t rows);
+// forward pass of `rows` rows of X (device, row-major) on `stream`, layer l's output into the
+// caller's act[l] (rows × mlp_layer_width(l)); *Y_dev = the last one. The network's parameters
+// are only read: contexts may infer with one network concurrently, each with its own buffers
+// and stream, as long as nobody trains it meanwhile.
+int mlp_infer(eb_mlp* mlp, const float* X_dev, int64_t rows, hipStream_t stream, float* const* act,
+              const float** Y_dev);
+int mlp_layers(const eb_mlp* mlp);
+int mlp_layer_width(const eb_mlp* mlp, int l);   // outputs of layer l
 // which = 0: input width, otherwise output width
 int mlp_width(const eb_mlp* mlp, int which);
 }  // namespace ebm

@@ -192,9 +192,6 @@ struct eb_mlp {
     float *x = nullptr, *t = nullptr, *dz0 = nullptr, *dz1 = nullptr, *partial = nullptr, *loss_part = nullptr;
     int32_t* idx = nullptr;
     int64_t partial_cap = 0;
-    // inference workspace (the solver's whole node set): per layer outputs only
-    int64_t infer_cap = 0;
-    std::vector<float*> iact;
     // dataset
     float *dX = nullptr, *dT = nullptr;
     int64_t n_data = 0, data_cap = 0;
@@ -246,25 +243,29 @@ int ensure_cap(eb_mlp* mlp, int64_t rows) {
     return EB_OK;
 }
 
-int forward_dev(eb_mlp* mlp, const float* X, int64_t rows, const std::vector<float*>& act) {
+int forward_dev(eb_mlp* mlp, const float* X, int64_t rows, float* const* act, hipStream_t stream) {
     const int L = layers(mlp);
     const float* in = X;
     const unsigned grid = (unsigned)((rows + FWD_TILE - 1) / FWD_TILE);
     for (int l = 0; l < L; ++l) {
         const float* W = mlp->w + mlp->off[l];
         const float* b = W + (int64_t)mlp->width[l] * mlp->width[l + 1];
-        hipLaunchKernelGGL(mlp_forward_layer, dim3(grid), dim3(256), 0, mlp->stream, rows, mlp->width[l],
+        hipLaunchKernelGGL(mlp_forward_layer, dim3(grid), dim3(256), 0, stream, rows, mlp->width[l],
                            mlp->width[l + 1], W, b, in, act[l], l + 1 < L ? 1 : 0);
         in = act[l];
     }
-    MLP_HIP(hipGetLastError());
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return EB_ERR_DEVICE;
     return EB_OK;
 }
 
 // forward + loss + backward on the rows in mlp->x / mlp->t; gradient into mlp->g
 int loss_grad_dev(eb_mlp* mlp, int64_t rows, float* loss_out) {
     const int L = layers(mlp);
-    if (forward_dev(mlp, mlp->x, rows, mlp->act)) return EB_ERR_DEVICE;
+    if (forward_dev(mlp, mlp->x, rows, mlp->act.data(), mlp->stream)) {
+        mlp->err = "forward pass launch failed";
+        return EB_ERR_DEVICE;
+    }
     const int outw = mlp->width[L];
     const int lgrid = (int)std::min<int64_t>(1024, (rows * outw + 255) / 256);
     hipLaunchKernelGGL(mlp_loss, dim3(lgrid), dim3(256), 0, mlp->stream, rows, outw, mlp->act[L - 1], mlp->t,
@@ -299,26 +300,15 @@ int loss_grad_dev(eb_mlp* mlp, int64_t rows, float* loss_out) {
 
 namespace ebm {
 
-int mlp_reserve(eb_mlp* mlp, int64_t rows) {
-    if (rows <= mlp->infer_cap) return EB_OK;
-    const int L = layers(mlp);
-    mlp->iact.assign(L, nullptr);
-    for (int l = 0; l < L; ++l)
-        if (mlp_alloc(mlp, mlp->iact[l], rows * mlp->width[l + 1])) return EB_ERR_DEVICE;
-    MLP_HIP(hipStreamSynchronize(mlp->stream));
-    mlp->infer_cap = rows;
-    return EB_OK;
-}
-
-int mlp_infer(eb_mlp* mlp, const float* X_dev, int64_t rows, hipStream_t stream, const float** Y_dev) {
-    if (mlp_reserve(mlp, rows)) return EB_ERR_DEVICE;
-    hipStream_t keep = mlp->stream;
-    mlp->stream = stream;
-    int st = forward_dev(mlp, X_dev, rows, mlp->iact);
-    mlp->stream = keep;
-    *Y_dev = mlp->iact[layers(mlp) - 1];
+int mlp_infer(eb_mlp* mlp, const float* X_dev, int64_t rows, hipStream_t stream, float* const* act,
+              const float** Y_dev) {
+    const int st = forward_dev(mlp, X_dev, rows, act, stream);
+    *Y_dev = act[layers(mlp) - 1];
     return st;
 }
+
+int mlp_layers(const eb_mlp* mlp) { return layers(mlp); }
+int mlp_layer_width(const eb_mlp* mlp, int l) { return mlp->width[l + 1]; }
 
 int mlp_width(const eb_mlp* mlp, int which) { return which == 0 ? mlp->width.front() : mlp->width.back(); }
 
@@ -396,7 +386,10 @@ int eb_mlp_forward(eb_mlp* mlp, const float* X, int64_t rows, float* Y) {
     if (!mlp || !X || !Y || rows < 1) return EB_ERR_ARG;
     if (ensure_cap(mlp, rows)) return EB_ERR_DEVICE;
     MLP_HIP(hipMemcpyAsync(mlp->x, X, sizeof(float) * rows * mlp->width[0], hipMemcpyHostToDevice, mlp->stream));
-    if (forward_dev(mlp, mlp->x, rows, mlp->act)) return EB_ERR_DEVICE;
+    if (forward_dev(mlp, mlp->x, rows, mlp->act.data(), mlp->stream)) {
+        mlp->err = "forward pass launch failed";
+        return EB_ERR_DEVICE;
+    }
     MLP_HIP(hipMemcpyAsync(Y, mlp->act[layers(mlp) - 1], sizeof(float) * rows * mlp->width.back(),
                            hipMemcpyDeviceToHost, mlp->stream));
     MLP_HIP(hipStreamSynchronize(mlp->stream));

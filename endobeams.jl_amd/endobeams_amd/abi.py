@@ -167,6 +167,7 @@ LEARNED_PROTOTYPES = {
     "set_learned_predictor": (_i, [_vp, _vp, _fp, _fp]),
     "set_learned_backoff": (_i, [_vp, C.c_int32]),
     "get_predictor_stats": (_i, [_vp, _lp]),
+    "get_learned_increment": (_i, [_vp, _dp, C.c_int64]),
 }
 LEARNED_INPUTS, LEARNED_OUTPUTS = 27, 6
 

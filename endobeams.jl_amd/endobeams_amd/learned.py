@@ -174,6 +174,14 @@ def set_backoff(engine, max_skip: int):
     engine._check(engine.fn["set_learned_backoff"](engine.h, int(max_skip)), "set_learned_backoff")
 
 
+def learned_increment(engine) -> np.ndarray:
+    """The increment ΔD of the network's start at the last prediction that evaluated it."""
+    out = np.zeros(engine.sizes.ndofs)
+    engine._check(engine.fn["get_learned_increment"](engine.h, out.ctypes.data_as(C.POINTER(C.c_double)), out.size),
+                  "get_learned_increment")
+    return out
+
+
 def predictor_stats(engine) -> np.ndarray:
     """[network start kept, reference start kept after evaluating both, network skipped]"""
     out = np.zeros(3, np.int64)

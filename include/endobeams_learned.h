@@ -52,8 +52,10 @@ int eb_mlp_train(eb_mlp* mlp, int32_t epochs, int32_t batch, float lr, uint64_t 
  * X (n_nodes × 27) unscaled inputs as the predictor saw them, T (n_nodes × 6) the unscaled target
  * correction ΔD* - p. Fails until three steps have converged since eb_initialize_state_simulation. */
 int eb_learned_sample(eb_ctx* ctx, float* X, float* T);
-/* installs the network (NULL: reference predictor only); the context keeps the pointer, and the
- * network's inference workspace is sized for the context's nodes here (device memory) */
+/* installs the network (NULL: reference predictor only); the context keeps the pointer and
+ * allocates here its own buffers for the network's layer outputs over its nodes (device memory,
+ * replacing those of a network installed before). Several contexts may use one network, also
+ * from different host threads; it must not be trained meanwhile. */
 int eb_set_learned_predictor(eb_ctx* ctx, eb_mlp* mlp, const float* in_scale, const float* out_scale);
 /* A network whose start keeps losing is not evaluated at every step: after an evaluation in
  * which no instance kept the network's start, the next 1, 2, 4, ... (at most max_skip)
@@ -64,6 +66,9 @@ int eb_set_learned_backoff(eb_ctx* ctx, int32_t max_skip);
  * evaluating both [1], and predictions that used the reference's without evaluating the network
  * under the backoff [2] */
 int eb_get_predictor_stats(const eb_ctx* ctx, int64_t* out3);
+/* for checks: the increment ΔD of the network's start at the last prediction that evaluated the
+ * network (n_dofs doubles: p + out_scale ⊙ Y on free dofs, p on fixed dofs) */
+int eb_get_learned_increment(eb_ctx* ctx, double* out, int64_t count);
 
 #ifdef __cplusplus
 }

@@ -48,7 +48,7 @@ def test_python_bindings_cover_the_abi():
     bound = {"eb_" + k for k in abi.PROTOTYPES}
     assert bound == set(declared())
     assert {"eb_" + k for k in abi.LEARNED_PROTOTYPES} == set(declared(LEARNED))
-    assert len(declared(LEARNED)) == 14
+    assert len(declared(LEARNED)) == 15
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
