@@ -6,7 +6,8 @@
 //
 // Precision: the local frame, R̅ = Rₑᵀ R Rₑ⁰, the log map Θ̅, Tₛ⁻¹(Θ̅), η/μ and K̄ₕ are always FP64;
 // everything downstream of them runs in the physics type P (double for the parity form, float
-// for the precision form) and is stored as P.
+// for the precision form) and is stored as P, except the forces Tⁱⁿᵗ and Tᵏ, which enter the
+// residual and stay FP64.
 #pragma once
 #include "eb_dmath.h"
 
@@ -79,6 +80,69 @@ struct NoStaticHook {
                         const V3<double>&, const M3<double>&, const V3<double>&, const V3<double>&) const {}
 };
 
+// The static part's tangent Kⁱⁿᵗ = BᵀK̄B + Kₘ + K̃ (compute_beams.jl:76-159) from the frame, Tₛ⁻¹,
+// T̄ⁱⁿᵗ, K̄ₕ and B, all given in the physics type P; stored in the storage type of the staging.
+template <typename P, typename S>
+EBD void beam_static_tangent(int64_t e, int64_t E, const BeamStage<S>& st, const M3<P>& Re, const V3<P>& v1, P eta, P ln,
+                             P Tu, P Ku, const M3<P>& Ti1, const M3<P>& Ti2, const V3<P>& TT1, const V3<P>& TT2,
+                             const M3<P>& Kh1, const M3<P>& Kh2, const M3<P>& G1, const M3<P>& G2, const M3<P>& G4,
+                             const V3<P>& B1, const M3<P>& B11, const M3<P>& B12, const M3<P>& B14, const M3<P>& B21,
+                             const M3<P>& B22, const M3<P>& B24, P KT0, P KT1, P KT2, P KTT0, P KTT1, P KTT2) {
+    const M3<P> I = eye<P>();
+    const M3<P> ReT = tr(Re);
+    M3<P> P11 = -G1, P12 = I - G2, P22 = -G2, P14 = -G4, P24 = I - G4;
+    M3<P> Bp11 = P11 * ReT, Bp12 = P12 * ReT, Bp22 = P22 * ReT, Bp14 = P14 * ReT, Bp24 = P24 * ReT;
+    const M3<P>& Bp21 = Bp11;
+    V3<P> Mp1 = tmul(Ti1, TT1), Mp2 = tmul(Ti2, TT2);
+    M3<P> Q1 = skew(tmul(P11, Mp1) + tmul(P11, Mp2));
+    M3<P> Q2 = skew(tmul(P12, Mp1) + tmul(P22, Mp2));
+    M3<P> Q4 = skew(tmul(P14, Mp1) + tmul(P24, Mp2));
+    V3<P> av = {P(0), eta * (Mp1.x + Mp2.x) / ln + (Mp1.y + Mp2.y) / ln, (Mp1.z + Mp2.z) / ln};
+    M3<P> D3 = (I - outer(v1, v1)) / ln;
+    V3<P> EGa1 = Re * tmul(G1, av);
+    M3<P> Km11 = (D3 * Tu - (Re * (Q1 * G1)) * ReT) + outer(EGa1, B1);
+    M3<P> Km12 = -((Re * (Q1 * G2)) * ReT);
+    M3<P> Km14 = -((Re * (Q1 * G4)) * ReT);
+    M3<P> Km22 = -((Re * (Q2 * G2)) * ReT);
+    M3<P> Km24 = -((Re * (Q2 * G4)) * ReT);
+    M3<P> Km44 = -((Re * (Q4 * G4)) * ReT);
+    M3<P> tB11 = tr(Bp11), tB12 = tr(Bp12), tB14 = tr(Bp14), tB22 = tr(Bp22), tB24 = tr(Bp24);
+    M3<P> Kt11 = ((tB11 * Kh1) * Bp11 + (tB11 * Kh2) * Bp21) + Km11;
+    M3<P> Kt12 = ((tB11 * Kh1) * Bp12 + (tB11 * Kh2) * Bp22) + Km12;
+    M3<P> Kt14 = ((tB11 * Kh1) * Bp14 + (tB11 * Kh2) * Bp24) + Km14;
+    M3<P> Kt22 = ((tB12 * Kh1) * Bp12 + (tB22 * Kh2) * Bp22) + Km22;
+    M3<P> Kt24 = ((tB12 * Kh1) * Bp14 + (tB22 * Kh2) * Bp24) + Km24;
+    M3<P> Kt44 = ((tB14 * Kh1) * Bp14 + (tB24 * Kh2) * Bp24) + Km44;
+    M3<P> KT = diag(KT0, KT1, KT2), KTT = diag(KTT0, KTT1, KTT2);
+    M3<P> XB12 = KT * B12 + KTT * B22, YB12 = KTT * B12 + KT * B22;
+    M3<P> XB14 = KT * B14 + KTT * B24, YB14 = KTT * B14 + KT * B24;
+    M3<P> tb11 = tr(B11), tb21 = tr(B21), tb12 = tr(B12), tb22 = tr(B22), tb14 = tr(B14), tb24 = tr(B24);
+    M3<P> K11 = ((outer(B1 * Ku, B1) + tb11 * (KT * B11 + KTT * B21)) + tb21 * (KTT * B11 + KT * B21)) + Kt11;
+    M3<P> K12 = (tb11 * XB12 + tb21 * YB12) + Kt12;
+    M3<P> K14 = (tb11 * XB14 + tb21 * YB14) + Kt14;
+    M3<P> K22 = (tb12 * XB12 + tb22 * YB12) + Kt22;
+    M3<P> K24 = (tb12 * XB14 + tb22 * YB14) + Kt24;
+    M3<P> K44 = (tb14 * XB14 + tb24 * YB14) + Kt44;
+    // block layout of Kⁱⁿᵗ (compute_beams.jl:148-159)
+    M3<P> tK12 = tr(K12), tK14 = tr(K14);
+    st_block<S>(st.K, E, e, 0, 0, cvt<S>(K11));
+    st_block<S>(st.K, E, e, 1, 0, cvt<S>(tK12));
+    st_block<S>(st.K, E, e, 2, 0, cvt<S>(-tr(K11)));      // K¹³ᵀ with K¹³ = -K¹¹
+    st_block<S>(st.K, E, e, 3, 0, cvt<S>(tK14));
+    st_block<S>(st.K, E, e, 0, 1, cvt<S>(K12));
+    st_block<S>(st.K, E, e, 1, 1, cvt<S>(K22));
+    st_block<S>(st.K, E, e, 2, 1, cvt<S>(-K12));          // K²³ᵀ = (-K¹²ᵀ)ᵀ
+    st_block<S>(st.K, E, e, 3, 1, cvt<S>(tr(K24)));
+    st_block<S>(st.K, E, e, 0, 2, cvt<S>(-K11));
+    st_block<S>(st.K, E, e, 1, 2, cvt<S>(-tK12));
+    st_block<S>(st.K, E, e, 2, 2, cvt<S>(K11));
+    st_block<S>(st.K, E, e, 3, 2, cvt<S>(-tK14));         // K³⁴ᵀ = (-K¹⁴)ᵀ
+    st_block<S>(st.K, E, e, 0, 3, cvt<S>(K14));
+    st_block<S>(st.K, E, e, 1, 3, cvt<S>(K24));
+    st_block<S>(st.K, E, e, 2, 3, cvt<S>(-K14));
+    st_block<S>(st.K, E, e, 3, 3, cvt<S>(K44));
+}
+
 template <typename P>
 EBD StaticOut beam_static(int64_t e, const NodeState& ns, const BeamConsts& bc, const BeamStage<P>& st) {
     const int64_t E = bc.E;
@@ -130,55 +194,12 @@ EBD StaticOut beam_static(int64_t e, const NodeState& ns, const BeamConsts& bc, 
     st_vec(st.Tint, E, e, 2, -T1);
     st_vec(st.Tint, E, e, 3, T4);
 
-    // Kₘ, K̃, BᵀKB
-    V3<double> Mp1 = tmul(Ti1, TT1), Mp2 = tmul(Ti2, TT2);
-    M3<double> Q1 = skew(tmul(P11, Mp1) + tmul(P11, Mp2));
-    M3<double> Q2 = skew(tmul(P12, Mp1) + tmul(P22, Mp2));
-    M3<double> Q4 = skew(tmul(P14, Mp1) + tmul(P24, Mp2));
-    V3<double> av = {0.0, f.eta * (Mp1.x + Mp2.x) / ln + (Mp1.y + Mp2.y) / ln, (Mp1.z + Mp2.z) / ln};
-    M3<double> D3 = (I - outer(f.v1, f.v1)) / ln;
-    V3<double> EGa1 = f.Re * tmul(G1, av);
-    M3<double> Km11 = (D3 * Tu - (f.Re * (Q1 * G1)) * ReT) + outer(EGa1, B1);
-    M3<double> Km12 = -((f.Re * (Q1 * G2)) * ReT);
-    M3<double> Km14 = -((f.Re * (Q1 * G4)) * ReT);
-    M3<double> Km22 = -((f.Re * (Q2 * G2)) * ReT);
-    M3<double> Km24 = -((f.Re * (Q2 * G4)) * ReT);
-    M3<double> Km44 = -((f.Re * (Q4 * G4)) * ReT);
-    M3<double> tB11 = tr(Bp11), tB12 = tr(Bp12), tB14 = tr(Bp14), tB22 = tr(Bp22), tB24 = tr(Bp24);
-    M3<double> Kt11 = ((tB11 * Kh1) * Bp11 + (tB11 * Kh2) * Bp21) + Km11;
-    M3<double> Kt12 = ((tB11 * Kh1) * Bp12 + (tB11 * Kh2) * Bp22) + Km12;
-    M3<double> Kt14 = ((tB11 * Kh1) * Bp14 + (tB11 * Kh2) * Bp24) + Km14;
-    M3<double> Kt22 = ((tB12 * Kh1) * Bp12 + (tB22 * Kh2) * Bp22) + Km22;
-    M3<double> Kt24 = ((tB12 * Kh1) * Bp14 + (tB22 * Kh2) * Bp24) + Km24;
-    M3<double> Kt44 = ((tB14 * Kh1) * Bp14 + (tB24 * Kh2) * Bp24) + Km44;
-    M3<double> KT = diag(KT0, KT1, KT2), KTT = diag(KTT0, KTT1, KTT2);
-    M3<double> XB12 = KT * B12 + KTT * B22, YB12 = KTT * B12 + KT * B22;
-    M3<double> XB14 = KT * B14 + KTT * B24, YB14 = KTT * B14 + KT * B24;
-    M3<double> tb11 = tr(B11), tb21 = tr(B21), tb12 = tr(B12), tb22 = tr(B22), tb14 = tr(B14), tb24 = tr(B24);
-    M3<double> K11 = ((outer(B1 * Ku, B1) + tb11 * (KT * B11 + KTT * B21)) + tb21 * (KTT * B11 + KT * B21)) + Kt11;
-    M3<double> K12 = (tb11 * XB12 + tb21 * YB12) + Kt12;
-    M3<double> K14 = (tb11 * XB14 + tb21 * YB14) + Kt14;
-    M3<double> K22 = (tb12 * XB12 + tb22 * YB12) + Kt22;
-    M3<double> K24 = (tb12 * XB14 + tb22 * YB14) + Kt24;
-    M3<double> K44 = (tb14 * XB14 + tb24 * YB14) + Kt44;
-    // block layout of Kⁱⁿᵗ (compute_beams.jl:148-159)
-    M3<double> tK12 = tr(K12), tK14 = tr(K14);
-    st_block<P>(st.K, E, e, 0, 0, cvt<P>(K11));
-    st_block<P>(st.K, E, e, 1, 0, cvt<P>(tK12));
-    st_block<P>(st.K, E, e, 2, 0, cvt<P>(-tr(K11)));      // K¹³ᵀ with K¹³ = -K¹¹
-    st_block<P>(st.K, E, e, 3, 0, cvt<P>(tK14));
-    st_block<P>(st.K, E, e, 0, 1, cvt<P>(K12));
-    st_block<P>(st.K, E, e, 1, 1, cvt<P>(K22));
-    st_block<P>(st.K, E, e, 2, 1, cvt<P>(-K12));          // K²³ᵀ = (-K¹²ᵀ)ᵀ
-    st_block<P>(st.K, E, e, 3, 1, cvt<P>(tr(K24)));
-    st_block<P>(st.K, E, e, 0, 2, cvt<P>(-K11));
-    st_block<P>(st.K, E, e, 1, 2, cvt<P>(-tK12));
-    st_block<P>(st.K, E, e, 2, 2, cvt<P>(K11));
-    st_block<P>(st.K, E, e, 3, 2, cvt<P>(-tK14));         // K³⁴ᵀ = (-K¹⁴)ᵀ
-    st_block<P>(st.K, E, e, 0, 3, cvt<P>(K14));
-    st_block<P>(st.K, E, e, 1, 3, cvt<P>(K24));
-    st_block<P>(st.K, E, e, 2, 3, cvt<P>(-K14));
-    st_block<P>(st.K, E, e, 3, 3, cvt<P>(K44));
+    // Kₘ, K̃, BᵀKB downstream of K̄ₕ: in the physics type P (FP64 in the parity form, the same
+    // expressions in FP32 in the precision form)
+    beam_static_tangent<P>(e, E, st, cvt<P>(f.Re), cvt<P>(f.v1), P(f.eta), P(ln), P(Tu), P(Ku), cvt<P>(Ti1), cvt<P>(Ti2),
+                           cvt<P>(TT1), cvt<P>(TT2), cvt<P>(Kh1), cvt<P>(Kh2), cvt<P>(G1), cvt<P>(G2), cvt<P>(G4),
+                           cvt<P>(B1), cvt<P>(B11), cvt<P>(B12), cvt<P>(B14), cvt<P>(B21), cvt<P>(B22), cvt<P>(B24),
+                           P(KT0), P(KT1), P(KT2), P(KTT0), P(KTT1), P(KTT2));
 
     StaticOut so;
     so.Re = f.Re; so.Tb1 = Tb1; so.Tb2 = Tb2; so.v1 = f.v1;
@@ -453,8 +474,9 @@ EBD void beam_damping(int64_t e, const NodeState& ns, const BeamConsts& bc, cons
 constexpr int BEAM_TILE = 64;
 
 // S: storage type of the element matrices (double: the FP64 form; float: the precision form, whose
-// Coriolis parts also compute in FP32). The static part always computes in FP64 and the mass part
-// too, because it yields the inertia forces Tᵏ, which enter the FP64 residual.
+// Coriolis parts and the static part's tangent also compute in FP32). The static part's frame and
+// Tⁱⁿᵗ compute in FP64, and the mass part too, because it yields the inertia forces Tᵏ: both enter
+// the FP64 residual.
 template <typename S, typename Hook>
 EBD void beam_kernel_body(NodeState ns, BeamConsts bc, BeamStage<S> st, GaussPts gp, const uint8_t* __restrict__ active,
                           const int32_t* __restrict__ beam_inst, const Hook& hook) {
