@@ -113,7 +113,7 @@ def run_steps(stepper, n):
     return done, attempts
 
 
-PROFILED = ("beam", "gather_matrices", "contact")
+PROFILED = ("beam", "contact", "gather", "gather_matrices", "precond", "krylov", "update")
 
 
 def _kernel_times(eng, contact0):
@@ -204,12 +204,22 @@ def hbm_records(form, s):
     return out
 
 
+def breakdown(form):
+    """Device time per time step of each kernel group (HIP events around every launch; the matrix
+    gather is part of `gather`), and its share of the wall time."""
+    steps = max(form["steps"], 1)
+    out = {k: form["kernels"][k][0] / steps for k in PROFILED}
+    out["wall"] = form["ms_per_step"]
+    return out
+
+
 def summary(form):
     keys = ("value", "ms_per_step", "newton_iters_per_step", "newton_iters_per_step_max", "failed_steps",
             "wall_s_per_simulated_s")
     out = {k: form[k] for k in keys}
     b = kernel_avg_s(form, "beam")
     out["beam_kernel_us"] = b * 1e6 if b else None
+    out["ms_per_step_by_kernel"] = breakdown(form)
     return out
 
 
@@ -567,6 +577,7 @@ def main():
         "roofline": {**roofline("beam_detect_kernel<float>" if reduced else "beam_detect_kernel<double>", avg_s,
                                 main_form["E"], peak, traffic), "launches": main_form["kernels"]["beam"][1]},
         "hbm_rooflines": hbm_records(main_form, s_bytes),
+        "ms_per_step_by_kernel": breakdown(main_form),
     }
     if plain is not None:
         out["plain_fp64"] = summary(plain)
