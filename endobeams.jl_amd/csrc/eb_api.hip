@@ -297,7 +297,7 @@ void prof_harvest_graph(eb_ctx* ctx) {
 inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // launch shapes chosen by eb_autotune (0: the built-in default)
-constexpr int32_t TUNE_DEFAULT[EB_TUNE_COUNT] = {48, 1, 256};
+constexpr int32_t TUNE_DEFAULT[EB_TUNE_COUNT] = {48, 1, 256, EB_CHAIN_BLOCK_LU};
 inline int32_t tuned(const eb_ctx* ctx, int id) { return ctx->tune[id] > 0 ? ctx->tune[id] : TUNE_DEFAULT[id]; }
 inline int gather_block(const eb_ctx* ctx) { return tuned(ctx, EB_TUNE_GATHER_BLOCK); }
 inline int red_grid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(RED_MAX_GRID, (n + RED_BLOCK - 1) / RED_BLOCK)); }
@@ -599,9 +599,29 @@ int bc_lift_launch(eb_ctx* ctx) {
 
 // ---- linear solve ------------------------------------------------------------------------
 
+// the block-LU chain kernels: five chains per 64-lane workgroup
+inline unsigned lu_grid(int64_t items) { return (unsigned)((items + LU_CHAINS - 1) / LU_CHAINS); }
+inline bool chain_lu(const eb_ctx* ctx) { return tuned(ctx, EB_TUNE_CHAIN_ALGO) == EB_CHAIN_BLOCK_LU; }
+
 int precond_factor(eb_ctx* ctx) {
     int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
     prof_begin(ctx, EB_KERNEL_PRECOND);
+    if (chain_lu(ctx)) {
+        hipLaunchKernelGGL(chain_lu_factor, dim3(lu_grid(nch)), dim3(64), 0, ctx->stream, chain_dev(ctx), ctx->Ktan,
+                           ctx->pivot_inst, ctx->active);
+        if (ctx->n_closures > 0) {
+            hipLaunchKernelGGL(closure_columns_lu, dim3(lu_grid(12 * (int64_t)ctx->n_closures)), dim3(64), 0, ctx->stream,
+                               chain_dev(ctx), closure_dev(ctx), ctx->active);
+            hipLaunchKernelGGL(closure_setup, dim3((unsigned)ctx->n_closures), dim3(64), 0, ctx->stream, chain_dev(ctx),
+                               closure_dev(ctx), ctx->Ktan, ctx->active);
+        }
+        if (ctx->dev2.n_chains > 0)
+            hipLaunchKernelGGL(chain_lu_factor, dim3(lu_grid(ctx->dev2.n_chains)), dim3(64), 0, ctx->stream, ctx->dev2,
+                               ctx->Ktan, ctx->pivot_inst, ctx->active);
+        ctx->factor_tag = ctx->ktan_tag;
+        prof_end(ctx, EB_KERNEL_PRECOND);
+        return check_launch(ctx, "chain_lu_factor");
+    }
     // the reduced chain stays in LDS once it fits in the budget (48 KB by default: with the
     // kernel's 8 KB of static LDS, two workgroups per CU; 96 KB: one workgroup per CU, the chain
     // in LDS one level earlier). Budgets beyond 64 KB of dynamic LDS were enabled at eb_create.
@@ -638,6 +658,21 @@ int precond_factor(eb_ctx* ctx) {
 
 int precond_apply(eb_ctx* ctx, const double* b, double* x) {
     int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
+    if (chain_lu(ctx)) {
+        hipLaunchKernelGGL(chain_lu_solve<false>, dim3(lu_grid(nch)), dim3(64), 0, ctx->stream, chain_dev(ctx), b, x,
+                           ctx->kctl);
+        if (ctx->n_closures > 0)
+            hipLaunchKernelGGL(closure_apply, dim3((unsigned)ctx->n_closures), dim3(256), 0, ctx->stream, chain_dev(ctx),
+                               closure_dev(ctx), x, ctx->kctl);
+        if (ctx->dev2.n_chains > 0) {
+            const Segs sg = segs(ctx);
+            hipLaunchKernelGGL(residual_masked, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->bptr,
+                               ctx->bcol, ctx->is_fixed, ctx->Ktan, x, b, ctx->cross_r, ctx->kctl);
+            hipLaunchKernelGGL(chain_lu_solve<true>, dim3(lu_grid(ctx->dev2.n_chains)), dim3(64), 0, ctx->stream, ctx->dev2,
+                               ctx->cross_r, x, ctx->kctl);
+        }
+        return check_launch(ctx, "chain_lu_solve");
+    }
     const size_t smem = tuned(ctx, EB_TUNE_CHAIN_SOLVE_LDS) == 1 ? chain_solve_lds_bytes(ctx->chains.max_len, ctx->solve_lds_max) : 0;
     const bool many = nch >= SOLVE_MANY_CHAINS;
     const dim3 grid((unsigned)nch), block(many ? SOLVE_THREADS_MANY : SOLVE_THREADS_FEW);
@@ -1060,11 +1095,13 @@ int autotune_impl(eb_ctx* ctx, int32_t* choice, double* ms_out) {
     static const int32_t cand_lds[EB_TUNE_CANDIDATES] = {16, 32, 48, 96};
     static const int32_t cand_solve[EB_TUNE_CANDIDATES] = {1, 2, 0, 0};
     static const int32_t cand_gather[EB_TUNE_CANDIDATES] = {64, 128, 256, 0};
-    const int32_t* cands[EB_TUNE_COUNT] = {cand_lds, cand_solve, cand_gather};
+    static const int32_t cand_algo[EB_TUNE_CANDIDATES] = {EB_CHAIN_CYCLIC_REDUCTION, EB_CHAIN_BLOCK_LU, 0, 0};
+    const int32_t* cands[EB_TUNE_COUNT] = {cand_lds, cand_solve, cand_gather, cand_algo};
     for (int i = 0; i < EB_TUNE_COUNT * EB_TUNE_CANDIDATES; ++i) ms_out[i] = -1;
     EB_TRY(materialize_kcm(ctx));   // the gather timing reads the staging; K, C, M stay as they are
     graph_release(ctx);
-    const int32_t saved[EB_TUNE_COUNT] = {ctx->tune[0], ctx->tune[1], ctx->tune[2]};
+    int32_t saved[EB_TUNE_COUNT];
+    for (int id = 0; id < EB_TUNE_COUNT; ++id) saved[id] = ctx->tune[id];
     // the chain solve skips the chains of finished solves, and after a step every solve is
     // finished: the solve is timed with every instance's control record cleared (not done), and
     // the records are put back afterwards
@@ -1085,8 +1122,19 @@ int autotune_impl(eb_ctx* ctx, int32_t* choice, double* ms_out) {
                 continue;
             ctx->tune[id] = v;
             double ms = 0;
+            if (id == EB_TUNE_CHAIN_LDS_KB || id == EB_TUNE_CHAIN_SOLVE_LDS) {
+                // shapes of the cyclic-reduction kernels: timed with that algorithm
+                ctx->tune[EB_TUNE_CHAIN_ALGO] = EB_CHAIN_CYCLIC_REDUCTION;
+            }
             if (id == EB_TUNE_CHAIN_LDS_KB) {
                 st = time_launches(ctx, reps, [&] { return precond_factor(ctx); }, &ms);
+            } else if (id == EB_TUNE_CHAIN_ALGO) {
+                // a factorisation and one application, the pair a Newton iteration's solve needs
+                EB_HIP(hipMemsetAsync(ctx->kctl, 0, sizeof(KrylovCtl) * ctx->S, ctx->stream));
+                st = time_launches(ctx, reps, [&] {
+                    EB_TRY(precond_factor(ctx));
+                    return precond_apply(ctx, ctx->res, ctx->ky);
+                }, &ms);
             } else if (id == EB_TUNE_CHAIN_SOLVE_LDS) {
                 EB_HIP(hipMemsetAsync(ctx->kctl, 0, sizeof(KrylovCtl) * ctx->S, ctx->stream));
                 st = time_launches(ctx, reps, [&] { return precond_apply(ctx, ctx->res, ctx->ky); }, &ms);
@@ -2627,6 +2675,7 @@ int eb_set_tuning(eb_ctx* ctx, const int32_t* choice) {
     if (choice[EB_TUNE_CHAIN_LDS_KB] != 0 && (choice[EB_TUNE_CHAIN_LDS_KB] < 1 || choice[EB_TUNE_CHAIN_LDS_KB] > 96))
         return EB_ERR_ARG;
     if (choice[EB_TUNE_CHAIN_SOLVE_LDS] < 0 || choice[EB_TUNE_CHAIN_SOLVE_LDS] > 2) return EB_ERR_ARG;
+    if (choice[EB_TUNE_CHAIN_ALGO] < 0 || choice[EB_TUNE_CHAIN_ALGO] > 2) return EB_ERR_ARG;
     const int32_t g = choice[EB_TUNE_GATHER_BLOCK];
     if (g != 0 && g != 64 && g != 128 && g != 256) return EB_ERR_ARG;
     EB_HIP(hipStreamSynchronize(ctx->stream));

@@ -417,10 +417,7 @@ __global__ void __launch_bounds__(RED_BLOCK) residual_masked(Segs sg, const int3
 // are saved (hist) for the right-hand-side reduction of the solve. Depth: ceil(log2 n) levels
 // instead of n sequential 6x6 steps, and every level is parallel over the chain.
 
-#ifndef EB_CHAIN_THREADS
-#define EB_CHAIN_THREADS 256
-#endif
-constexpr int CHAIN_THREADS = EB_CHAIN_THREADS;          // factorisation workgroup
+constexpr int CHAIN_THREADS = 256;                       // factorisation workgroup
 constexpr int CHAIN_GROUPS = (CHAIN_THREADS / 64) * 10;  // six-lane groups: ten per wavefront
 constexpr int GJ_BUF = 24;                               // doubles of LDS exchange per group
 // solve workgroup: 1024 threads while the chains are few (each chain's latency sets the time), 256
@@ -456,6 +453,19 @@ EBD void wave_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+// The same ordering of a wavefront's LDS accesses kept at compiler level only: the LDS executes a
+// wavefront's instructions in issue order, so the lanes of a group see each other's writes. The
+// fences of wave_sync also make the wavefront wait until its own global stores, and with them
+// every load issued before those stores, have completed, which the block-LU kernels, whose loads
+// run several steps ahead, must not do.
+EBD void wave_lds_order() {
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+template <bool F> EBD void wsync() {
+    if (F) wave_sync();
+    else wave_lds_order();
+}
 
 struct Lane6 {
     int group;  // six-lane group of the workgroup (lanes 60-63 of a wavefront take none)
@@ -476,7 +486,7 @@ EBD Lane6 lane6(double* bufs) {
 // 1e-12 of the diagonal entry it started from, or is not finite; the caller then redoes the
 // block with gj6_pivot. The test is invariant under row and column scaling, which matters here:
 // translational and rotational entries differ by many decades.
-template <int NC> EBD bool gj6_nopiv(double (&w)[NC], int r, double* buf) {
+template <int NC, bool F = true> EBD bool gj6_nopiv(double (&w)[NC], int r, double* buf) {
     double d0 = 0;
 #pragma unroll
     for (int k = 0; k < 6; ++k) d0 = (r == k) ? fabs(w[k]) : d0;
@@ -493,7 +503,7 @@ template <int NC> EBD bool gj6_nopiv(double (&w)[NC], int r, double* buf) {
 #pragma unroll
             for (int c = 0; c < NC; c += 6) st6(buf + c, w + c);
         }
-        wave_sync();
+        wsync<F>();
         if (r != p) {
             double pr[NC];
 #pragma unroll
@@ -503,7 +513,7 @@ template <int NC> EBD bool gj6_nopiv(double (&w)[NC], int r, double* buf) {
 #pragma unroll
             for (int c = 0; c < NC; ++c) w[c] = fma(-f, pr[c], w[c]);
         }
-        wave_sync();
+        wsync<F>();
     }
     return ok;
 }
@@ -512,7 +522,7 @@ template <int NC> EBD bool gj6_nopiv(double (&w)[NC], int r, double* buf) {
 // |entry| of column p among the rows not yet used (lowest row on ties). Rows are not moved: the
 // lane whose row is the pivot of step p ends with row p of the result, returned in `orow`.
 // False on that lane when its pivot is zero or not finite.
-template <int NC> EBD bool gj6_pivot(double (&w)[NC], int r, double* buf, int& orow) {
+template <int NC, bool F = true> EBD bool gj6_pivot(double (&w)[NC], int r, double* buf, int& orow) {
     constexpr int NW = NC + 6;
     double x[NW];
 #pragma unroll
@@ -525,7 +535,7 @@ template <int NC> EBD bool gj6_pivot(double (&w)[NC], int r, double* buf, int& o
     for (int p = 0; p < 6; ++p) {
         const double a = fabs(x[p]);
         if (r < 6) buf[r] = used ? -1.0 : (a >= 0.0 ? a : 0.0);
-        wave_sync();
+        wsync<F>();
         int q = 0;
         double best = -0.5;
 #pragma unroll
@@ -533,7 +543,7 @@ template <int NC> EBD bool gj6_pivot(double (&w)[NC], int r, double* buf, int& o
             const double v = buf[k];
             if (v > best) { best = v; q = k; }
         }
-        wave_sync();
+        wsync<F>();
         if (r == q) {
             const double piv = x[p];
             ok = fabs(piv) > 0.0 && fabs(piv) < INFINITY;
@@ -546,7 +556,7 @@ template <int NC> EBD bool gj6_pivot(double (&w)[NC], int r, double* buf, int& o
             used = true;
             orow = p;
         }
-        wave_sync();
+        wsync<F>();
         if (r != q) {
             double pr[NW];
 #pragma unroll
@@ -555,7 +565,7 @@ template <int NC> EBD bool gj6_pivot(double (&w)[NC], int r, double* buf, int& o
 #pragma unroll
             for (int c = 0; c < NW; ++c) x[c] = fma(-f, pr[c], x[c]);
         }
-        wave_sync();
+        wsync<F>();
     }
 #pragma unroll
     for (int c = 0; c < NC; ++c) w[c] = x[6 + c];
@@ -566,15 +576,16 @@ template <int NC> EBD bool gj6_pivot(double (&w)[NC], int r, double* buf, int& o
 // by the groups whose unpivoted elimination was rejected (load(w) fetches the rows again). `act`:
 // the group holds a block this pass (uniform over the group). Every lane of the wavefront calls
 // it. Returns the row of the result the lane holds.
-template <int NC, typename Load> EBD int gj6(double (&w)[NC], const Lane6& ln, bool act, bool& ok, const Load& load) {
-    const bool good = gj6_nopiv<NC>(w, ln.r, ln.buf);
+template <int NC, typename Load, bool F = true>
+EBD int gj6(double (&w)[NC], const Lane6& ln, bool act, bool& ok, const Load& load) {
+    const bool good = gj6_nopiv<NC, F>(w, ln.r, ln.buf);
     const uint64_t bad = __ballot(act && ln.r < 6 && !good);
     const int lane = threadIdx.x & 63;
     const int first = lane < 60 ? lane - ln.r : 60;
     int orow = ln.r;
     if (act && ((bad >> first) & 0x3full)) {
         load(w);
-        const bool nonsingular = gj6_pivot<NC>(w, ln.r, ln.buf, orow);
+        const bool nonsingular = gj6_pivot<NC, F>(w, ln.r, ln.buf, orow);
         if (ln.r < 6) ok = nonsingular && ok;
     }
     return orow;
@@ -902,6 +913,382 @@ __global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const 
     }
 }
 
+// ---- block LU along each chain, from both ends ----------------------------------------------
+// The same block-tridiagonal system factored position by position (block Thomas), from both ends
+// of the chain at once and meeting in the middle, m = (n − 1) / 2:
+//   top    (p = 0 .. m−1):   D'_p = D_p − L_p G_{p−1},   G_p = D'_p⁻¹ U_p
+//   bottom (p = n−1 .. m+1): D'_p = D_p − U_p H_{p+1},   H_p = D'_p⁻¹ L_p
+//   middle:                  D'_m = D_m − L_m G_{m−1} − U_m H_{m+1}
+// kept as Dm = D'_p⁻¹ and, above the middle, Um = G_p, Lm = L_p, below it Lm = H_p, Um = U_p
+// (at m: Lm = L_m, Um = U_m), and solved by
+//   y_p = D'_p⁻¹ (b_p − L_p y_{p−1}) (top),   y_p = D'_p⁻¹ (b_p − U_p y_{p+1}) (bottom),
+//   x_m = D'_m⁻¹ (b_m − L_m y_{m−1} − U_m y_{m+1}),
+//   x_p = y_p − G_p x_{p+1} (top),   x_p = y_p − H_p x_{p−1} (bottom).
+// About n/2 dependent steps per chain instead of cyclic reduction's ceil(log2 n) levels, but half
+// its flops, no workgroup barrier and no level history: the form for batches of many chains,
+// where the chains, not the positions of one chain, fill the chip. One six-lane group per end
+// (lane r holds row r), the two groups of a chain in one wavefront, five chains per 64-lane
+// workgroup.
+//
+// Loads run in two stages ahead of the step that uses them: the indices of the step LU_AHEAD
+// further on (block numbers, fixed-dof masks, node numbers) two stages ahead, the data at those
+// indices one stage ahead. No load waits on a load issued in the same step, so a step waits only
+// for data requested LU_AHEAD steps earlier. Steps past the end of a half read position 0 (any
+// valid position): they are inactive and store nothing. The wavefront runs its longest half's
+// steps, rounded up to a multiple of LU_AHEAD; the step body has no branch but masked stores.
+constexpr int LU_GROUPS = 10;
+constexpr int LU_CHAINS = LU_GROUPS / 2;
+constexpr int LU_AHEAD = 4;
+
+EBD int32_t wave_max_i32(int32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+// the lane's half of its chain: group 2c + h of the workgroup holds chain c's half h (0: top)
+struct LuHalf {
+    int64_t s0;      // the chain's first position
+    int32_t n, m;    // positions, middle
+    int32_t steps;   // steps of this half (0 when the chain takes no part)
+    bool top;
+    EBD int32_t at(int32_t q) const {   // position of step q, any valid position past the end
+        return q < steps ? (top ? q : n - 1 - q) : 0;
+    }
+};
+EBD LuHalf lu_half(const ChainDev& cd, int64_t ch, bool take, int group) {
+    LuHalf h{0, 0, 0, 0, (group & 1) == 0};
+    if (take) {
+        h.s0 = cd.ptr[ch];
+        h.n = cd.ptr[ch + 1] - (int32_t)h.s0;
+        h.m = (h.n - 1) / 2;
+        h.steps = h.top ? h.m : h.n - 1 - h.m;
+    }
+    return h;
+}
+
+// block numbers and fixed-dof masks of D_p, L_p, U_p
+struct LuIx {
+    int32_t b[3];
+    uint64_t m[3];
+};
+EBD void lu_ix(const ChainDev& cd, int64_t g, LuIx& ix) {
+    ix.b[0] = cd.diag_blk[g];
+    ix.b[1] = cd.lo_blk[g];
+    ix.b[2] = cd.up_blk[g];
+    ix.m[0] = cd.mD[g];
+    ix.m[1] = cd.mL[g];
+    ix.m[2] = cd.mU[g];
+}
+// row r of the three blocks as the tangent holds them (an absent block reads block 0: masked off)
+EBD void lu_raw(const double* __restrict__ A, const LuIx& ix, int r, double (&v)[18]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ld6(A + 36 * (int64_t)(ix.b[k] >= 0 ? ix.b[k] : 0) + 6 * r, v + 6 * k);
+}
+// the masked rows: an entry in a fixed row or column becomes the identity's, an absent block zero
+EBD void lu_mask(const LuIx& ix, int r, double (&v)[18]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            v[6 * k + c] = ((ix.m[k] >> (6 * r + c)) & 1) ? ((k == 0 && r == c) ? 1.0 : 0.0)
+                                                           : (ix.b[k] >= 0 ? v[6 * k + c] : 0.0);
+    }
+}
+// w −= row · G (6x6 row-major in LDS)
+EBD void lu_sub_row_block(double* w, const double (&row)[6], const double* G) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double gj[6];
+        ld6(G + 6 * j, gj);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) w[c] = fma(-row[j], gj[c], w[c]);
+    }
+}
+
+// status[instance] is set to 1 when a pivot block of one of its chains is singular
+__global__ void __launch_bounds__(64) chain_lu_factor(ChainDev cd, const double* __restrict__ A, int* status,
+                                                      const uint8_t* __restrict__ active) {
+    __shared__ __align__(16) double gj_buf[LU_GROUPS * GJ_BUF];
+    __shared__ __align__(16) double g_prev[LU_GROUPS * 36];
+    const Lane6 ln = lane6(gj_buf);
+    const int64_t ch = (int64_t)blockIdx.x * LU_CHAINS + (ln.group >> 1);
+    const bool mine = ln.r < 6 && ch < cd.n_chains;
+    const bool on = mine && active[cd.inst[ch]];
+    const LuHalf h = lu_half(cd, ch, on, ln.group);
+    const int32_t smax = wave_max_i32(h.steps);
+    const int r = ln.r < 6 ? ln.r : 0;
+    double* gp = g_prev + 36 * ln.group;
+    // G_{−1} = H_n = 0, so that the first step of each half is like the others
+    if (ln.r < 6) {
+        const double z[6] = {0, 0, 0, 0, 0, 0};
+        st6(gp + 6 * ln.r, z);
+    }
+    wave_lds_order();
+    bool ok = true;
+    LuIx far[LU_AHEAD], near[LU_AHEAD];
+    double raw[LU_AHEAD][18];
+#pragma unroll
+    for (int k = 0; k < LU_AHEAD; ++k) lu_ix(cd, h.s0 + h.at(k), near[k]);
+#pragma unroll
+    for (int k = 0; k < LU_AHEAD; ++k) lu_raw(A, near[k], r, raw[k]);
+#pragma unroll
+    for (int k = 0; k < LU_AHEAD; ++k) lu_ix(cd, h.s0 + h.at(LU_AHEAD + k), far[k]);
+    for (int32_t q0 = 0; q0 < smax; q0 += LU_AHEAD) {
+#pragma unroll
+        for (int k = 0; k < LU_AHEAD; ++k) {
+            const int32_t q = q0 + k;
+            double v[18];
+#pragma unroll
+            for (int c = 0; c < 18; ++c) v[c] = raw[k][c];
+            const LuIx ix = near[k];
+            near[k] = far[k];
+            lu_raw(A, near[k], r, raw[k]);                        // step q + LU_AHEAD
+            lu_ix(cd, h.s0 + h.at(q + 2 * LU_AHEAD), far[k]);     // step q + 2 LU_AHEAD
+            lu_mask(ix, r, v);
+            const bool act = q < h.steps;
+            // the block towards the finished side (lo) and towards the middle (up)
+            double lo[6], up[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                lo[c] = h.top ? v[6 + c] : v[12 + c];
+                up[c] = h.top ? v[12 + c] : v[6 + c];
+            }
+            // row r of [D'_p | up], D'_p = D_p − lo · (the previous step's G or H, in LDS)
+            double w[12];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                w[c] = v[c];
+                w[6 + c] = up[c];
+            }
+            lu_sub_row_block(w, lo, gp);
+            double w0[12];
+#pragma unroll
+            for (int c = 0; c < 12; ++c) w0[c] = w[c];
+            auto reload = [&](double (&x)[12]) {
+#pragma unroll
+                for (int c = 0; c < 12; ++c) x[c] = w0[c];
+            };
+            const int orow = gj6<12, decltype(reload), false>(w, ln, act, ok, reload);
+            if (act) {
+                const int64_t g = h.s0 + h.at(q);
+                st6(cd.Dm + 36 * g + 6 * orow, w);
+                st6((h.top ? cd.Um : cd.Lm) + 36 * g + 6 * orow, w + 6);
+                st6((h.top ? cd.Lm : cd.Um) + 36 * g + 6 * r, lo);
+                st6(gp + 6 * orow, w + 6);
+            }
+            wave_lds_order();
+        }
+    }
+    // the middle, by the top group: D'_m = D_m − L_m G_{m−1} − U_m H_{m+1}
+    {
+        const bool act = on && h.top;
+        const int64_t g = h.s0 + (act ? h.m : 0);
+        LuIx ix;
+        lu_ix(cd, g, ix);
+        double v[18];
+        lu_raw(A, ix, r, v);
+        lu_mask(ix, r, v);
+        double w[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) w[c] = v[c];
+        double Lr[6], Ur[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            Lr[c] = v[6 + c];
+            Ur[c] = v[12 + c];
+        }
+        lu_sub_row_block(w, Lr, gp);        // G_{m−1} (zero when the top half is empty)
+        lu_sub_row_block(w, Ur, gp + 36);   // H_{m+1}, the bottom group's (zero when it is empty)
+        double w0[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) w0[c] = w[c];
+        auto reload = [&](double (&x)[6]) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) x[c] = w0[c];
+        };
+        const int orow = gj6<6, decltype(reload), false>(w, ln, act, ok, reload);
+        if (act) {
+            st6(cd.Dm + 36 * g + 6 * orow, w);
+            st6(cd.Lm + 36 * g + 6 * r, Lr);
+            st6(cd.Um + 36 * g + 6 * r, Ur);
+        }
+    }
+    if (mine && !ok) status[cd.inst[ch]] = 1;
+}
+
+// The solve of the lane's half (see LuHalf; smax: the wavefront's longest half) with the factors
+// of chain_lu_factor. io.node(p) is the index io.b and io.put address position p by (loaded two
+// stages ahead), io.b(node, p) row r of b_p, io.old(node, p) what io.put(node, p, old, x, w) adds
+// row r of x_p to (loaded one stage ahead); w points at position p's row r in wk, which holds 6
+// doubles per position (y). v6: the group's 6-double LDS exchange, t6 its scratch; the two groups
+// of a chain hold theirs side by side (the top group's first). Every lane of the wavefront calls it;
+// `live`: the lane holds a row of a participating chain.
+template <class IO>
+EBD void chain_lu_solve_group(const ChainDev& cd, const LuHalf& h, int32_t smax, int r, bool live, double* wk,
+                              double* v6, double* t6, const IO& io) {
+    const double* lo_m = h.top ? cd.Lm : cd.Um;   // the block towards the finished side
+    const double* up_m = h.top ? cd.Um : cd.Lm;   // G or H
+    if (live) v6[r] = 0.0;                        // y_{−1} = y_n = 0
+    wave_lds_order();
+    // forward: y_p = D'_p⁻¹ (b_p − lo_p y_prev)
+    {
+        int64_t far[LU_AHEAD];
+        double L[LU_AHEAD][6], D[LU_AHEAD][6], bb[LU_AHEAD];
+        auto near_load = [&](int k, int32_t q, int64_t node) {
+            const int32_t p = h.at(q);
+            ld6(lo_m + 36 * (h.s0 + p) + 6 * r, L[k]);
+            ld6(cd.Dm + 36 * (h.s0 + p) + 6 * r, D[k]);
+            bb[k] = io.b(node, p);
+        };
+#pragma unroll
+        for (int k = 0; k < LU_AHEAD; ++k) near_load(k, k, io.node(h.at(k)));
+#pragma unroll
+        for (int k = 0; k < LU_AHEAD; ++k) far[k] = io.node(h.at(LU_AHEAD + k));
+        for (int32_t q0 = 0; q0 < smax; q0 += LU_AHEAD) {
+#pragma unroll
+            for (int k = 0; k < LU_AHEAD; ++k) {
+                const int32_t q = q0 + k;
+                double cL[6], cD[6];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    cL[c] = L[k][c];
+                    cD[c] = D[k][c];
+                }
+                double t = bb[k];
+                near_load(k, q + LU_AHEAD, far[k]);
+                far[k] = io.node(h.at(q + 2 * LU_AHEAD));
+                {
+                    double yv[6];
+                    ld6(v6, yv);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) t = fma(-cL[c], yv[c], t);
+                }
+                if (live) t6[r] = t;
+                wave_lds_order();
+                double tv[6];
+                ld6(t6, tv);
+                double y = 0.0;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) y = fma(cD[c], tv[c], y);
+                wave_lds_order();
+                if (live && q < h.steps) {   // past its half's end a group keeps its last y
+                    wk[6 * h.at(q) + r] = y;
+                    v6[r] = y;
+                }
+                wave_lds_order();
+            }
+        }
+    }
+    // the middle, by the top group: x_m = D'_m⁻¹ (b_m − L_m y_{m−1} − U_m y_{m+1}); then both
+    // groups start their back-substitution from x_m
+    {
+        const bool act = live && h.top;
+        const int32_t m = act ? h.m : 0;
+        const int64_t node = io.node(m);
+        double Lr[6], Ur[6], Dr[6];
+        ld6(cd.Lm + 36 * (h.s0 + m) + 6 * r, Lr);
+        ld6(cd.Um + 36 * (h.s0 + m) + 6 * r, Ur);
+        ld6(cd.Dm + 36 * (h.s0 + m) + 6 * r, Dr);
+        double t = io.b(node, m);
+        const double xo = io.old(node, m);
+        double ya[6], yb[6];
+        ld6(v6, ya);        // y_{m−1} (top group)
+        ld6(v6 + 12, yb);   // y_{m+1} (its bottom group, 12 doubles on)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) t = fma(-Lr[c], ya[c], fma(-Ur[c], yb[c], t));
+        if (act) t6[r] = t;
+        wave_lds_order();
+        double tv[6];
+        ld6(t6, tv);
+        double x = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) x = fma(Dr[c], tv[c], x);
+        wave_lds_order();
+        if (act) {
+            v6[r] = x;
+            v6[12 + r] = x;
+            io.put(node, m, xo, x, wk + 6 * m + r);
+        }
+        wave_lds_order();
+    }
+    // backward: x_p = y_p − up_p x_next, from the middle outwards (step q of the half: q counted
+    // from the middle, the reverse of the forward order)
+    {
+        auto bat = [&](int32_t q) { return h.at(h.steps - 1 - q); };   // valid for q < steps
+        auto bpos = [&](int32_t q) { return q < h.steps ? bat(q) : 0; };
+        int64_t far[LU_AHEAD], nd[LU_AHEAD];
+        double G[LU_AHEAD][6], yy[LU_AHEAD], old[LU_AHEAD];
+        auto near_load = [&](int k, int32_t q, int64_t node) {
+            const int32_t p = bpos(q);
+            ld6(up_m + 36 * (h.s0 + p) + 6 * r, G[k]);
+            yy[k] = wk[6 * p + r];
+            old[k] = io.old(node, p);
+            nd[k] = node;
+        };
+#pragma unroll
+        for (int k = 0; k < LU_AHEAD; ++k) near_load(k, k, io.node(bpos(k)));
+#pragma unroll
+        for (int k = 0; k < LU_AHEAD; ++k) far[k] = io.node(bpos(LU_AHEAD + k));
+        for (int32_t q0 = 0; q0 < smax; q0 += LU_AHEAD) {
+#pragma unroll
+            for (int k = 0; k < LU_AHEAD; ++k) {
+                const int32_t q = q0 + k;
+                double cG[6];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) cG[c] = G[k][c];
+                double x = yy[k];
+                const double xo = old[k];
+                const int64_t node = nd[k];
+                near_load(k, q + LU_AHEAD, far[k]);
+                far[k] = io.node(bpos(q + 2 * LU_AHEAD));
+                {
+                    double xv[6];
+                    ld6(v6, xv);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) x = fma(-cG[c], xv[c], x);
+                }
+                wave_lds_order();
+                if (live && q < h.steps) {
+                    const int32_t p = bat(q);
+                    v6[r] = x;
+                    io.put(node, p, xo, x, wk + 6 * p + r);
+                }
+                wave_lds_order();
+            }
+        }
+    }
+}
+
+// x = M⁻¹ b (ACCUM: x += M⁻¹ b) along each chain of an unfinished solve
+template <bool ACCUM>
+__global__ void __launch_bounds__(64) chain_lu_solve(ChainDev cd, const double* __restrict__ b, double* __restrict__ x,
+                                                     const KrylovCtl* kc) {
+    __shared__ __align__(16) double ex[LU_GROUPS * 12];
+    const Lane6 ln = lane6(nullptr);
+    const int64_t ch = (int64_t)blockIdx.x * LU_CHAINS + (ln.group >> 1);
+    const bool live = ln.r < 6 && ch < cd.n_chains && !kc[cd.inst[ch]].done;
+    const LuHalf h = lu_half(cd, ch, live, ln.group);
+    if (wave_max_i32(live ? h.n : 0) == 0) return;
+    const int32_t smax = wave_max_i32(h.steps);
+    const int r = ln.r < 6 ? ln.r : 0;
+    struct Io {
+        const ChainDev* cd;
+        const double* bv;
+        double* xv;
+        int64_t s0;
+        int r;
+        EBD int64_t node(int32_t p) const { return cd->node[s0 + p]; }
+        EBD double b(int64_t nd, int32_t) const { return bv[6 * nd + r]; }
+        EBD double old(int64_t nd, int32_t) const { return ACCUM ? xv[6 * nd + r] : 0.0; }
+        EBD void put(int64_t nd, int32_t, double xo, double v, double*) const { xv[6 * nd + r] = ACCUM ? xo + v : v; }
+    };
+    const Io io{&cd, b, x, h.s0, r};
+    // the top group's exchange at ex + 24 c, its bottom group's 12 doubles on
+    double* e = ex + 12 * ln.group;
+    chain_lu_solve_group(cd, h, smax, r, live, cd.yw + 6 * h.s0, e, e + 6, io);
+}
+
 // ---- closed chains (Sherman-Morrison-Woodbury) ----------------------------------------------
 // A chain whose two ends are coupled (a ring) has the matrix A = T + U M Uᵀ: T the block-
 // tridiagonal matrix the factorisation inverts, U the 12 unit columns of the first and last
@@ -938,6 +1325,34 @@ __global__ void __launch_bounds__(NT) closure_columns(ChainDev cd, ClosureDev cl
     chain_solve_body<NT>(cd, s0, n, bw, yw);
     double* z = cl.Z + cl.zoff[c] + (int64_t)j * 6 * n;
     for (int32_t t = threadIdx.x; t < 6 * n; t += NT) z[t] = yw[t];
+}
+
+// the columns of Z with the block-LU factors: one chain's two six-lane groups per (closure,
+// column), the column solved in Z
+__global__ void __launch_bounds__(64) closure_columns_lu(ChainDev cd, ClosureDev cl, const uint8_t* __restrict__ active) {
+    __shared__ __align__(16) double ex[LU_GROUPS * 12];
+    const Lane6 ln = lane6(nullptr);
+    const int64_t item = (int64_t)blockIdx.x * LU_CHAINS + (ln.group >> 1);
+    const bool mine = ln.r < 6 && item < 12 * (int64_t)cl.n;
+    const int c = mine ? (int)(item / 12) : 0, j = (int)(item % 12);
+    const bool live = mine && active[cd.inst[cl.chain[c]]];
+    const LuHalf h = lu_half(cd, live ? cl.chain[c] : 0, live, ln.group);
+    if (wave_max_i32(live ? h.n : 0) == 0) return;
+    const int32_t smax = wave_max_i32(h.steps);
+    const int r = ln.r < 6 ? ln.r : 0;
+    // lanes without a column read (never write) the start of Z
+    double* z = live ? cl.Z + cl.zoff[c] + (int64_t)j * 6 * h.n : cl.Z;
+    struct Io {
+        int32_t unit;
+        int r;
+        EBD int64_t node(int32_t p) const { return p; }
+        EBD double b(int64_t, int32_t p) const { return 6 * p + r == unit ? 1.0 : 0.0; }
+        EBD double old(int64_t, int32_t) const { return 0.0; }
+        EBD void put(int64_t, int32_t, double, double v, double* w) const { *w = v; }
+    };
+    const Io io{live ? closure_unit(j, h.n) : -1, r};
+    double* e = ex + 12 * ln.group;
+    chain_lu_solve_group(cd, h, smax, r, live, z, e, e + 6, io);
 }
 
 // H = (I + M G)⁻¹ M per closure, one wavefront each, in LDS: Gauss-Jordan elimination with

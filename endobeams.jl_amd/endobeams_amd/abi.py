@@ -144,9 +144,10 @@ PROTOTYPES = {
 }
 
 EB_EXEC_EAGER, EB_EXEC_GRAPH = 0, 1
-TUNE_NAMES = ("chain_lds_kb", "chain_solve_lds", "gather_block")
-EB_TUNE_COUNT, EB_TUNE_CANDIDATES = 3, 4
-TUNE_CANDIDATES = ((16, 32, 48, 96), (1, 2), (64, 128, 256))
+TUNE_NAMES = ("chain_lds_kb", "chain_solve_lds", "gather_block", "chain_algo")
+EB_TUNE_COUNT, EB_TUNE_CANDIDATES = 4, 4
+EB_CHAIN_CYCLIC_REDUCTION, EB_CHAIN_BLOCK_LU = 1, 2
+TUNE_CANDIDATES = ((16, 32, 48, 96), (1, 2), (64, 128, 256), (EB_CHAIN_CYCLIC_REDUCTION, EB_CHAIN_BLOCK_LU))
 
 KERNELS = {"beam": 0, "contact": 1, "gather": 2, "precond": 3, "krylov": 4, "update": 5, "gather_matrices": 6}
 

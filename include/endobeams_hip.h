@@ -363,16 +363,27 @@ int eb_get_exec_stats(const eb_ctx* ctx, int32_t* mode, int64_t* captures, int64
  *                           (candidates 16, 32, 48, 96; default 48; at most 96, and at most what
  *                           the device grants a workgroup: a larger budget fails the factorisation)
  *   EB_TUNE_CHAIN_SOLVE_LDS chain solve work vectors in LDS (1, default) or global memory (2)
- *   EB_TUNE_GATHER_BLOCK    workgroup size of the matrix gather (64, 128, 256; default 256) */
+ *   EB_TUNE_GATHER_BLOCK    workgroup size of the matrix gather (64, 128, 256; default 256)
+ *   EB_TUNE_CHAIN_ALGO      how the chain preconditioner factors and solves each chain:
+ *                           EB_CHAIN_CYCLIC_REDUCTION (one workgroup per chain, ceil(log2 n)
+ *                           parallel levels) or EB_CHAIN_BLOCK_LU (default: six lanes per chain,
+ *                           n sequential block steps, ten chains per wavefront)
+ * The first three only change launch shapes: results are bit-identical whatever they are. The
+ * chain algorithm changes the preconditioner's rounding: results agree to the Krylov tolerance
+ * and the Newton iteration counts are the same, but they are not bit-identical. */
 #define EB_TUNE_CHAIN_LDS_KB 0
 #define EB_TUNE_CHAIN_SOLVE_LDS 1
 #define EB_TUNE_GATHER_BLOCK 2
-#define EB_TUNE_COUNT 3
+#define EB_TUNE_CHAIN_ALGO 3
+#define EB_TUNE_COUNT 4
 #define EB_TUNE_CANDIDATES 4
+#define EB_CHAIN_CYCLIC_REDUCTION 1
+#define EB_CHAIN_BLOCK_LU 2
 /* Times every candidate on this context's sizes and current tangent (call after a time step)
  * and keeps the fastest of each; choice[EB_TUNE_COUNT] receives them and, when not NULL,
  * ms_per_candidate[EB_TUNE_COUNT * EB_TUNE_CANDIDATES] the average launch time of each
- * candidate (-1: not a candidate). Results of later calls are unchanged by the choice. */
+ * candidate (-1: not a candidate; the chain algorithm is timed as one factorisation plus one
+ * application). */
 int eb_autotune(eb_ctx* ctx, int32_t* choice, double* ms_per_candidate);
 int eb_set_tuning(eb_ctx* ctx, const int32_t* choice);
 int eb_get_tuning(const eb_ctx* ctx, int32_t* choice);
