@@ -173,6 +173,7 @@ struct eb_ctx {
     // chains
     int32_t *ch_ptr = nullptr, *ch_node = nullptr, *ch_diag = nullptr, *ch_lo = nullptr, *ch_up = nullptr;
     uint64_t *ch_mD = nullptr, *ch_mL = nullptr, *ch_mU = nullptr;
+    uint32_t* ch_mrow = nullptr;   // per position and row: the row's fixed-dof bits of D, L, U
     int32_t* ch_hbase = nullptr;
     // closed chains (rings): their chain, coupling blocks and masks, T⁻¹U columns and H
     int32_t *cl_chain = nullptr, *cl_fl = nullptr, *cl_lf = nullptr;
@@ -346,7 +347,7 @@ ClosureDev closure_dev(eb_ctx* c) {
 ChainDev chain_dev(eb_ctx* c) {
     return ChainDev{c->ch_ptr, c->ch_node, c->ch_diag, c->ch_lo, c->ch_up, c->ch_mD, c->ch_mL, c->ch_mU,
                     c->ch_hbase, c->ch_D, c->ch_L, c->ch_U, c->ch_hist, c->ch_bw, c->ch_yw, c->chain_inst,
-                    (int64_t)c->chains.ptr.size() - 1};
+                    (int64_t)c->chains.ptr.size() - 1, c->ch_mrow};
 }
 Active active_dev(eb_ctx* c) { return Active{c->active, c->node_inst, c->beam_inst, c->blk_inst}; }
 Segs segs(eb_ctx* c) { return Segs{c->seg_lo, c->seg_hi, c->seg_inst, c->seg_cptr, c->nchunks, (int32_t)c->S}; }
@@ -598,6 +599,17 @@ int bc_lift_launch(eb_ctx* ctx) {
 }
 
 // ---- linear solve ------------------------------------------------------------------------
+
+// the fixed-dof bits of each position's row r in D, L and U (6 bits each, D lowest), the form the
+// block-LU kernels read: one word per lane instead of the three 64-bit block masks
+std::vector<uint32_t> row_masks(const ebh::Chains& c) {
+    std::vector<uint32_t> m(6 * c.mD.size());
+    for (size_t g = 0; g < c.mD.size(); ++g)
+        for (int r = 0; r < 6; ++r)
+            m[6 * g + r] = (uint32_t)((c.mD[g] >> (6 * r)) & 63) | (uint32_t)((c.mL[g] >> (6 * r)) & 63) << 6 |
+                           (uint32_t)((c.mU[g] >> (6 * r)) & 63) << 12;
+    return m;
+}
 
 // the block-LU chain kernels: five chains per 64-lane workgroup
 inline unsigned lu_grid(int64_t items) { return (unsigned)((items + LU_CHAINS - 1) / LU_CHAINS); }
@@ -1732,6 +1744,7 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
         CK(upload_vec(ctx, ctx->ch_mL, c.mL));
         CK(upload_vec(ctx, ctx->ch_mU, c.mU));
         CK(upload_vec(ctx, ctx->ch_hbase, c.hbase));
+        CK(upload_vec(ctx, ctx->ch_mrow, row_masks(c)));
         CK(dalloc(ctx, ctx->ch_D, 36 * N));
         CK(dalloc(ctx, ctx->ch_L, 36 * N));
         CK(dalloc(ctx, ctx->ch_U, 36 * N));
@@ -1757,6 +1770,7 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
             }
             int32_t *ptr2, *node2, *diag2, *lo2, *up2, *hb2, *inst2;
             uint64_t *mD2, *mL2, *mU2;
+            uint32_t* mrow2;
             double *D2, *L2, *U2, *hist2, *bw2, *yw2;
             CK(upload_vec(ctx, inst2, chi2));
             CK(upload_vec(ctx, ptr2, narrow<int32_t>(c2.ptr)));
@@ -1768,6 +1782,7 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
             CK(upload_vec(ctx, mL2, c2.mL));
             CK(upload_vec(ctx, mU2, c2.mU));
             CK(upload_vec(ctx, hb2, c2.hbase));
+            CK(upload_vec(ctx, mrow2, row_masks(c2)));
             CK(dalloc(ctx, D2, 36 * N));
             CK(dalloc(ctx, L2, 36 * N));
             CK(dalloc(ctx, U2, 36 * N));
@@ -1776,7 +1791,7 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
             CK(dalloc(ctx, yw2, 6 * N));
             CK(dalloc(ctx, ctx->cross_r, 6 * N));
             ctx->dev2 = ChainDev{ptr2, node2, diag2, lo2, up2, mD2, mL2, mU2, hb2, D2, L2, U2, hist2, bw2, yw2, inst2,
-                                 (int64_t)c2.ptr.size() - 1};
+                                 (int64_t)c2.ptr.size() - 1, mrow2};
         }
         if (ctx->n_closures > 0) {
             CK(upload_vec(ctx, ctx->cl_chain, c.cl_chain));

@@ -436,6 +436,8 @@ struct ChainDev {
     double *bw, *yw;               // 6 per position: solve work vectors (chains too long for LDS)
     const int32_t* inst;           // per chain: instance
     int64_t n_chains;
+    const uint32_t* mrow;          // per position and row r (6 g + r): bits c, 6 + c, 12 + c: entry
+                                   // (r, c) of D, L, U touches a fixed dof (mD, mL, mU restated)
 };
 
 EBD int chain_levels(int32_t n) {
@@ -576,9 +578,49 @@ template <int NC, bool F = true> EBD bool gj6_pivot(double (&w)[NC], int r, doub
 // by the groups whose unpivoted elimination was rejected (load(w) fetches the rows again). `act`:
 // the group holds a block this pass (uniform over the group). Every lane of the wavefront calls
 // it. Returns the row of the result the lane holds.
+// gj6_nopiv in one instruction stream for every lane of the group: the pivot row goes through
+// LDS to all six lanes, and the pivot lane's scaling and the others' elimination become selects,
+// so that no branch divides the wavefront and no register is copied where branches meet. The
+// reciprocal of the pivot is v_rcp_f64 refined by two Newton steps (within a few ulp; the
+// preconditioner does not need a correctly rounded division). Without fences (wave_lds_order).
+template <int NC> EBD bool gj6_flat(double (&w)[NC], int r, double* buf) {
+    double d0 = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) d0 = (r == k) ? fabs(w[k]) : d0;
+    bool ok = true;
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+        if (r == p) {
+#pragma unroll
+            for (int c = 0; c < NC; c += 6) st6(buf + c, w + c);
+        }
+        wave_lds_order();
+        double pr[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c += 6) ld6(buf + c, pr + c);
+        const double piv = pr[p];
+        double ip = __builtin_amdgcn_rcp(piv);
+        ip = fma(ip, fma(-piv, ip, 1.0), ip);
+        ip = fma(ip, fma(-piv, ip, 1.0), ip);
+        const bool pivot = r == p;
+        ok = pivot ? (fabs(piv) > 1e-12 * d0 && fabs(piv) > 0.0 && fabs(piv) < INFINITY) : ok;
+        const double f = w[p];
+        pr[p] = 1.0;   // column p becomes the inverse's column
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const double sc = pr[c] * ip;
+            w[c] = pivot ? sc : fma(-f, sc, c == p ? 0.0 : w[c]);
+        }
+        wave_lds_order();
+    }
+    return ok;
+}
+
 template <int NC, typename Load, bool F = true>
 EBD int gj6(double (&w)[NC], const Lane6& ln, bool act, bool& ok, const Load& load) {
-    const bool good = gj6_nopiv<NC, F>(w, ln.r, ln.buf);
+    bool good;
+    if constexpr (F) good = gj6_nopiv<NC, F>(w, ln.r, ln.buf);
+    else good = gj6_flat<NC>(w, ln.r, ln.buf);
     const uint64_t bad = __ballot(act && ln.r < 6 && !good);
     const int lane = threadIdx.x & 63;
     const int first = lane < 60 ? lane - ln.r : 60;
@@ -967,18 +1009,16 @@ EBD LuHalf lu_half(const ChainDev& cd, int64_t ch, bool take, int group) {
     return h;
 }
 
-// block numbers and fixed-dof masks of D_p, L_p, U_p
+// block numbers of D_p, L_p, U_p and the fixed-dof bits of their row r
 struct LuIx {
     int32_t b[3];
-    uint64_t m[3];
+    uint32_t m;
 };
-EBD void lu_ix(const ChainDev& cd, int64_t g, LuIx& ix) {
+EBD void lu_ix(const ChainDev& cd, int64_t g, int r, LuIx& ix) {
     ix.b[0] = cd.diag_blk[g];
     ix.b[1] = cd.lo_blk[g];
     ix.b[2] = cd.up_blk[g];
-    ix.m[0] = cd.mD[g];
-    ix.m[1] = cd.mL[g];
-    ix.m[2] = cd.mU[g];
+    ix.m = cd.mrow[6 * g + r];
 }
 // row r of the three blocks as the tangent holds them (an absent block reads block 0: masked off)
 EBD void lu_raw(const double* __restrict__ A, const LuIx& ix, int r, double (&v)[18]) {
@@ -991,8 +1031,8 @@ EBD void lu_mask(const LuIx& ix, int r, double (&v)[18]) {
     for (int k = 0; k < 3; ++k) {
 #pragma unroll
         for (int c = 0; c < 6; ++c)
-            v[6 * k + c] = ((ix.m[k] >> (6 * r + c)) & 1) ? ((k == 0 && r == c) ? 1.0 : 0.0)
-                                                           : (ix.b[k] >= 0 ? v[6 * k + c] : 0.0);
+            v[6 * k + c] = ((ix.m >> (6 * k + c)) & 1) ? ((k == 0 && r == c) ? 1.0 : 0.0)
+                                                       : (ix.b[k] >= 0 ? v[6 * k + c] : 0.0);
     }
 }
 // w −= row · G (6x6 row-major in LDS)
@@ -1028,12 +1068,15 @@ __global__ void __launch_bounds__(64) chain_lu_factor(ChainDev cd, const double*
     bool ok = true;
     LuIx far[LU_AHEAD], near[LU_AHEAD];
     double raw[LU_AHEAD][18];
+    // the first steps' loads in the order of the loop's own (data of step k, then the indices of
+    // step k + LU_AHEAD), so that the loop starts with the waits of its steady state
 #pragma unroll
-    for (int k = 0; k < LU_AHEAD; ++k) lu_ix(cd, h.s0 + h.at(k), near[k]);
+    for (int k = 0; k < LU_AHEAD; ++k) lu_ix(cd, h.s0 + h.at(k), r, near[k]);
 #pragma unroll
-    for (int k = 0; k < LU_AHEAD; ++k) lu_raw(A, near[k], r, raw[k]);
-#pragma unroll
-    for (int k = 0; k < LU_AHEAD; ++k) lu_ix(cd, h.s0 + h.at(LU_AHEAD + k), far[k]);
+    for (int k = 0; k < LU_AHEAD; ++k) {
+        lu_raw(A, near[k], r, raw[k]);
+        lu_ix(cd, h.s0 + h.at(LU_AHEAD + k), r, far[k]);
+    }
     for (int32_t q0 = 0; q0 < smax; q0 += LU_AHEAD) {
 #pragma unroll
         for (int k = 0; k < LU_AHEAD; ++k) {
@@ -1044,7 +1087,7 @@ __global__ void __launch_bounds__(64) chain_lu_factor(ChainDev cd, const double*
             const LuIx ix = near[k];
             near[k] = far[k];
             lu_raw(A, near[k], r, raw[k]);                        // step q + LU_AHEAD
-            lu_ix(cd, h.s0 + h.at(q + 2 * LU_AHEAD), far[k]);     // step q + 2 LU_AHEAD
+            lu_ix(cd, h.s0 + h.at(q + 2 * LU_AHEAD), r, far[k]);     // step q + 2 LU_AHEAD
             lu_mask(ix, r, v);
             const bool act = q < h.steps;
             // the block towards the finished side (lo) and towards the middle (up)
@@ -1085,7 +1128,7 @@ __global__ void __launch_bounds__(64) chain_lu_factor(ChainDev cd, const double*
         const bool act = on && h.top;
         const int64_t g = h.s0 + (act ? h.m : 0);
         LuIx ix;
-        lu_ix(cd, g, ix);
+        lu_ix(cd, g, r, ix);
         double v[18];
         lu_raw(A, ix, r, v);
         lu_mask(ix, r, v);
@@ -1141,10 +1184,15 @@ EBD void chain_lu_solve_group(const ChainDev& cd, const LuHalf& h, int32_t smax,
             ld6(cd.Dm + 36 * (h.s0 + p) + 6 * r, D[k]);
             bb[k] = io.b(node, p);
         };
+        // the first steps' loads in the loop's own order (see chain_lu_factor)
+        int64_t first[LU_AHEAD];
 #pragma unroll
-        for (int k = 0; k < LU_AHEAD; ++k) near_load(k, k, io.node(h.at(k)));
+        for (int k = 0; k < LU_AHEAD; ++k) first[k] = io.node(h.at(k));
 #pragma unroll
-        for (int k = 0; k < LU_AHEAD; ++k) far[k] = io.node(h.at(LU_AHEAD + k));
+        for (int k = 0; k < LU_AHEAD; ++k) {
+            near_load(k, k, first[k]);
+            far[k] = io.node(h.at(LU_AHEAD + k));
+        }
         for (int32_t q0 = 0; q0 < smax; q0 += LU_AHEAD) {
 #pragma unroll
             for (int k = 0; k < LU_AHEAD; ++k) {
@@ -1156,8 +1204,6 @@ EBD void chain_lu_solve_group(const ChainDev& cd, const LuHalf& h, int32_t smax,
                     cD[c] = D[k][c];
                 }
                 double t = bb[k];
-                near_load(k, q + LU_AHEAD, far[k]);
-                far[k] = io.node(h.at(q + 2 * LU_AHEAD));
                 {
                     double yv[6];
                     ld6(v6, yv);
@@ -1177,6 +1223,9 @@ EBD void chain_lu_solve_group(const ChainDev& cd, const LuHalf& h, int32_t smax,
                     v6[r] = y;
                 }
                 wave_lds_order();
+                // the slot's refill once its values are used: the loads go to the same registers
+                near_load(k, q + LU_AHEAD, far[k]);
+                far[k] = io.node(h.at(q + 2 * LU_AHEAD));
             }
         }
     }
@@ -1226,10 +1275,14 @@ EBD void chain_lu_solve_group(const ChainDev& cd, const LuHalf& h, int32_t smax,
             old[k] = io.old(node, p);
             nd[k] = node;
         };
+        int64_t first[LU_AHEAD];
 #pragma unroll
-        for (int k = 0; k < LU_AHEAD; ++k) near_load(k, k, io.node(bpos(k)));
+        for (int k = 0; k < LU_AHEAD; ++k) first[k] = io.node(bpos(k));
 #pragma unroll
-        for (int k = 0; k < LU_AHEAD; ++k) far[k] = io.node(bpos(LU_AHEAD + k));
+        for (int k = 0; k < LU_AHEAD; ++k) {
+            near_load(k, k, first[k]);
+            far[k] = io.node(bpos(LU_AHEAD + k));
+        }
         for (int32_t q0 = 0; q0 < smax; q0 += LU_AHEAD) {
 #pragma unroll
             for (int k = 0; k < LU_AHEAD; ++k) {
@@ -1240,8 +1293,6 @@ EBD void chain_lu_solve_group(const ChainDev& cd, const LuHalf& h, int32_t smax,
                 double x = yy[k];
                 const double xo = old[k];
                 const int64_t node = nd[k];
-                near_load(k, q + LU_AHEAD, far[k]);
-                far[k] = io.node(bpos(q + 2 * LU_AHEAD));
                 {
                     double xv[6];
                     ld6(v6, xv);
@@ -1255,6 +1306,8 @@ EBD void chain_lu_solve_group(const ChainDev& cd, const LuHalf& h, int32_t smax,
                     io.put(node, p, xo, x, wk + 6 * p + r);
                 }
                 wave_lds_order();
+                near_load(k, q + LU_AHEAD, far[k]);
+                far[k] = io.node(bpos(q + 2 * LU_AHEAD));
             }
         }
     }
