@@ -407,9 +407,9 @@ __global__ void __launch_bounds__(256) contact_detect(NodeState ns, BeamConsts b
     list[slot] = (uint64_t(e) << 8) | mask;
 }
 
-// The same detection as a hook of the element kernel: the wavefront of the Coriolis columns 2 and 4
-// runs it between the element's frame (x₁, x₂, Rₑ, Θ̅₁, Θ̅₂) and its Gauss loop, so the assembly
-// with contact needs no separate detection pass.
+// The same detection as a hook of the element kernel: the wavefront of the mass part runs it
+// between the element's frame (x₁, x₂, Rₑ, Θ̅₁, Θ̅₂) and its Gauss loop, so the assembly with
+// contact needs no separate detection pass.
 template <typename P> struct DetectHook {
     const ContactArgs<P>& ca;   // the kernel's arguments, by reference (no local copies)
     const uint8_t* is_contact;  // [E]
