@@ -50,15 +50,16 @@ PEAK_HBM_GBS = 8000.0
 
 
 def roofline(kernel, avg_s, beams, peak, traffic=None, peak_fp64=PEAK_FP64_TFLOPS):
-    """The element kernel's roofline record: the dense algorithmic flops per launch (and, beside
-    them, the flops left once structural zeros are folded) over the measured average launch. The
+    """The element kernel's roofline record: the dense algorithmic flops per launch (beams: those of
+    the instances active in the launch, averaged over the timed launches; and, beside them, the
+    flops left once structural zeros are folded) over the measured average launch. The
     kernel runs on the vector ALUs (no matrix cores), part of it in FP64 in the precision form:
     the fraction is given against the FP32 vector peak and, beside it, against the FP64 one."""
     achieved = FLOPS_PER_BEAM * beams / avg_s / 1e12
     nonzero = FLOPS_PER_BEAM_NONZERO * beams / avg_s / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
             "frac_of_fp64_peak": achieved / peak_fp64, "traffic": traffic, "kernel": kernel,
-            "avg_launch_us": avg_s * 1e6, "flops_per_launch": FLOPS_PER_BEAM * beams,
+            "avg_launch_us": avg_s * 1e6, "beams_per_launch": beams, "flops_per_launch": FLOPS_PER_BEAM * beams,
             "nonzero_flops_per_launch": FLOPS_PER_BEAM_NONZERO * beams, "nonzero_achieved": nonzero,
             "nonzero_frac": nonzero / peak}
 
@@ -69,7 +70,7 @@ def gather_bytes(E, nnz, s):
     """gather_matrices forming the tangent: each beam's three 144-entry staging records read once
     (plus its four block-source entries), every tangent entry written once in FP64, and per 6×6
     block its pointer and instance entries."""
-    return E * (3 * 144 * s + 4 * 4) + nnz * 8 + (nnz // 36) * 8
+    return E * (3 * 144 * s + 4 * 4) + nnz * 8 + (nnz / 36) * 8
 
 
 def contact_beam_bytes(s):
@@ -117,9 +118,13 @@ PROFILED = ("beam", "contact", "gather", "gather_matrices", "precond", "krylov",
 
 
 def _kernel_times(eng, contact0):
+    """Per kernel group (total ms, launches) over the timed steps, the contact kernel's work (beams
+    listed, Gauss points in contact) and the work the library counted for the timed launches of
+    the element kernel, the matrix gather and the chain factorisation (beams, tangent entries,
+    chain positions of the instances active in each launch: converged instances are skipped)."""
     out = {k: eng.profile_read(k) for k in PROFILED}
     c1 = eng.contact_stats()
-    return out, (c1[0] - contact0[0], c1[1] - contact0[1])
+    return out, (c1[0] - contact0[0], c1[1] - contact0[1]), eng.work_counters()
 
 
 def timed(eng, params, E, warmup, steps, graph=True, autotune=True):
@@ -138,14 +143,14 @@ def timed(eng, params, E, warmup, steps, graph=True, autotune=True):
     state = eng.run_steps(steps, state)
     wall = time.perf_counter() - t0
     done = state.steps
-    kt, contact = _kernel_times(eng, c0)
+    kt, contact, work = _kernel_times(eng, c0)
     its = state.newton_iterations / max(done, 1)
     return {"steps": done, "attempts": done + state.failed_steps, "wall": wall, "E": E, "nnz": eng.sizes.nnz,
             "value": E * done / wall, "ms_per_step": 1e3 * wall / max(done, 1),
             "newton_iters_per_step": its, "newton_iters_per_step_max": its,
             "failed_steps": state.failed_steps,
             "wall_s_per_simulated_s": wall / (state.t - st.tn) if state.t > st.tn else None,
-            "kernels": kt, "contact": contact,
+            "kernels": kt, "contact": contact, "work": work,
             "exec": {"mode": "graph" if graph else "eager", "graph_captures_replays": list(eng.exec_stats()[1:]),
                      "tuning": tuning[0], "tuning_ms": tuning[1]}}
 
@@ -170,7 +175,7 @@ def timed_batch(eng, warmup, steps, graph=True, autotune=True):
     dn = [r.steps - n for r, n in zip(states, n0)]
     di = [r.newton_iterations - i for r, i in zip(states, i0)]
     sim = [r.t - t for r, t in zip(states, t_0)]
-    kt, contact = _kernel_times(eng, c0)
+    kt, contact, work = _kernel_times(eng, c0)
     return {"steps": min(dn), "attempts": sum(dn) + sum(r.failed_steps - f for r, f in zip(states, f0)),
             "wall": wall, "E": sum(E_s), "nnz": eng.sizes.nnz,
             "value": sum(e * d for e, d in zip(E_s, dn)) / wall, "ms_per_step": 1e3 * wall / max(min(dn), 1),
@@ -178,7 +183,7 @@ def timed_batch(eng, warmup, steps, graph=True, autotune=True):
             "newton_iters_per_step_max": max(i / max(d, 1) for i, d in zip(di, dn)),
             "failed_steps": sum(r.failed_steps - f for r, f in zip(states, f0)),
             "wall_s_per_simulated_s": wall / (sum(sim) / len(sim)) if sum(sim) > 0 else None,
-            "kernels": kt, "contact": contact,
+            "kernels": kt, "contact": contact, "work": work,
             "exec": {"mode": "graph" if graph else "eager", "graph_captures_replays": list(eng.exec_stats()[1:]),
                      "tuning": tuning[0], "tuning_ms": tuning[1]}}
 
@@ -188,13 +193,23 @@ def kernel_avg_s(form, k):
     return ms * 1e-3 / n if n else None
 
 
+def active_per_launch(form, group, which):
+    """Units (beams, tangent entries, chain positions) one timed launch of a kernel group worked
+    on, averaged over the launches: the library counts those of the instances active in each."""
+    n = form["kernels"][group][1]
+    return form["work"][which] / n if n else 0.0
+
+
 def hbm_records(form, s):
     """The HBM roofline records of the matrix gather and the contact kernel of a timed form."""
     out = []
     g = kernel_avg_s(form, "gather_matrices")
     if g:
-        out.append(hbm_record("gather_matrices", gather_bytes(form["E"], form["nnz"], s), g,
-                              launches=form["kernels"]["gather_matrices"][1]))
+        entries = active_per_launch(form, "gather_matrices", 1)
+        beams = entries * form["E"] / form["nnz"]   # beams of the active instances (same beams per entry)
+        out.append(hbm_record("gather_matrices", gather_bytes(beams, entries, s), g,
+                              launches=form["kernels"]["gather_matrices"][1], entries_per_launch=entries,
+                              entries_all_instances=form["nnz"]))
     c = kernel_avg_s(form, "contact")
     n = form["kernels"]["contact"][1]
     if c and n and form["contact"][0] > 0:
@@ -319,14 +334,15 @@ def sweep(args):
     value /= wall
     its = its_sum / max(steps_sum, 1)
     sim = sum(sims) / len(sims)
-    kms = kn = 0.0
+    kms = kn = kbeams = 0.0
     for e in engs:
         ms, n = e.profile_read("beam")
         kms += ms
         kn += n
+        kbeams += e.work_counters()[0]
     chosen = [L.predictor_stats(e).tolist() for e in engs]
     avg_s = (kms / max(kn, 1)) * 1e-3
-    E_launch = sum(int(e.beam_off[-1]) for e in engs) / len(engs)
+    E_launch = kbeams / max(kn, 1)   # beams of the active instances per launch
     out = {
         "metric": METRIC, "value": value, "unit": "element-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
@@ -407,8 +423,9 @@ def small_config(args):
                                       **err},
         "cpu_baseline": {"value": E * args.compare_steps / cpu_wall, "unit": "element-steps/s", "cores": 1,
                          "kind": "port", "sample": f"first {args.compare_steps} steps, C++ FP64 restatement, 1 thread"},
-        "roofline": {**roofline("beam_kernel<float>" if inter is None else "beam_detect_kernel<float>", avg_s, E,
-                                PEAK_FP32_TFLOPS), "launches": f32["kernels"]["beam"][1]},
+        "roofline": {**roofline("beam_kernel<float>" if inter is None else "beam_detect_kernel<float>", avg_s,
+                                active_per_launch(f32, "beam", 0), PEAK_FP32_TFLOPS),
+                     "launches": f32["kernels"]["beam"][1]},
         "hbm_rooflines": hbm_records(f32, 4),
     }
     print(json.dumps(out))
@@ -545,7 +562,7 @@ def main():
         if os.path.exists(tf):
             k = json.load(open(tf))["kernels"].get("ebd::beam_detect_kernel<float>")
             if k:
-                traffic = (k["fetch_bytes"] + k["write_bytes"]) / (16 * 4800) * main_form["E"]
+                traffic = (k["fetch_bytes"] + k["write_bytes"]) / (16 * 4800) * active_per_launch(main_form, "beam", 0)
     avg_s = kernel_avg_s(main_form, "beam")
     peak = PEAK_FP32_TFLOPS if reduced else PEAK_FP64_TFLOPS
     s_bytes = 4 if reduced else 8
@@ -575,7 +592,8 @@ def main():
         "wall_s_per_simulated_s": main_form["wall_s_per_simulated_s"],
         "exec": main_form["exec"],
         "roofline": {**roofline("beam_detect_kernel<float>" if reduced else "beam_detect_kernel<double>", avg_s,
-                                main_form["E"], peak, traffic), "launches": main_form["kernels"]["beam"][1]},
+                                active_per_launch(main_form, "beam", 0), peak, traffic),
+                     "launches": main_form["kernels"]["beam"][1], "beams_all_instances": main_form["E"]},
         "hbm_rooflines": hbm_records(main_form, s_bytes),
         "ms_per_step_by_kernel": breakdown(main_form),
     }

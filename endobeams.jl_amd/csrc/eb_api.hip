@@ -192,6 +192,11 @@ struct eb_ctx {
     int64_t counters[4] = {0, 0, 0, 0};
     // launch timing
     uint32_t prof_mask = 0;
+    // work of the launches timed since eb_profile_enable: [0] beams through the element kernel,
+    // [1] tangent entries through gather_matrices, [2] chain positions factored; per instance
+    // (3 per instance) the same three sizes, summed over the instances active at each launch
+    unsigned long long* work = nullptr;
+    int64_t* inst_work = nullptr;
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
     double prof_ms[EB_KERNEL_COUNT] = {};
@@ -296,6 +301,28 @@ void prof_harvest_graph(eb_ctx* ctx) {
 }
 
 inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// work[which] += Σ over instances active in the launch of inst_work[3 s + which] (one workgroup)
+__global__ void __launch_bounds__(256) work_account(int64_t ninst, const uint8_t* __restrict__ active,
+                                                     const int64_t* __restrict__ sizes, int which,
+                                                     unsigned long long* work) {
+    __shared__ unsigned long long part[256];
+    unsigned long long acc = 0;
+    for (int64_t s = threadIdx.x; s < ninst; s += 256) acc += active[s] ? (unsigned long long)sizes[3 * s + which] : 0ull;
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) work[which] += part[0];
+}
+// counts the work of a timed launch (only while its kernel group is profiled)
+inline void count_work(eb_ctx* ctx, int prof_id, int which) {
+    if (!(ctx->prof_mask & (1u << prof_id)) || !ctx->work) return;
+    hipLaunchKernelGGL(work_account, dim3(1), dim3(256), 0, ctx->stream, (int64_t)ctx->S, ctx->active, ctx->inst_work,
+                       which, ctx->work);
+}
 
 // launch shapes chosen by eb_autotune (0: the built-in default)
 constexpr int32_t TUNE_DEFAULT[EB_TUNE_COUNT] = {48, 1, 256, EB_CHAIN_BLOCK_LU};
@@ -460,6 +487,7 @@ template <typename P> int launch_beams(eb_ctx* ctx) {
     launch_beam_kernel<P>(nblk(ctx->E, BEAM_TILE), ctx->stream, node_state(ctx), beam_consts(ctx), beam_stage<P>(ctx),
                           ctx->gauss, ctx->active, ctx->beam_inst);
     prof_end(ctx, EB_KERNEL_BEAM);
+    count_work(ctx, EB_KERNEL_BEAM, 0);
     ctx->counters[3]++;
     return check_launch(ctx, "beam_kernel");
 }
@@ -501,6 +529,7 @@ template <typename P> int launch_beams_contact(eb_ctx* ctx, double alpha) {
                                  beam_stage<P>(ctx), ctx->gauss, ctx->active, ctx->beam_inst, ca, ctx->is_contact,
                                  ctx->clist, ctx->clist_n);
     prof_end(ctx, EB_KERNEL_BEAM);
+    count_work(ctx, EB_KERNEL_BEAM, 0);
     ctx->counters[3]++;
     EB_TRY(check_launch(ctx, "beam_detect_kernel"));
     return launch_contact_tangent<P>(ctx, ca);
@@ -535,7 +564,10 @@ template <typename P> int launch_gathers(eb_ctx* ctx, bool with_contact, bool ta
                            ctx->blk_ptr, ctx->blk_src, (const P*)ctx->sK, (const P*)ctx->sC, (const P*)ctx->sM,
                            tangent ? nullptr : ctx->K, tangent ? nullptr : ctx->C, tangent ? nullptr : ctx->M, ctx->Ktan,
                            c1, (const StepCoef*)ctx->coef, tangent ? 1 : 0, tangent ? 0 : 1, active_dev(ctx));
-        if (tangent) prof_end(ctx, EB_KERNEL_GATHER_MATRICES);
+        if (tangent) {
+            prof_end(ctx, EB_KERNEL_GATHER_MATRICES);
+            count_work(ctx, EB_KERNEL_GATHER_MATRICES, 1);
+        }
         EB_TRY(check_launch(ctx, "gather_matrices"));
         ctx->kcm_pending = tangent;
         if (tangent) ctx->ktan_tag = eb_ctx::TanTag{ctx->stage_ver, c1, ctx->coef_ver};
@@ -632,6 +664,7 @@ int precond_factor(eb_ctx* ctx) {
                                ctx->Ktan, ctx->pivot_inst, ctx->active);
         ctx->factor_tag = ctx->ktan_tag;
         prof_end(ctx, EB_KERNEL_PRECOND);
+        count_work(ctx, EB_KERNEL_PRECOND, 2);
         return check_launch(ctx, "chain_lu_factor");
     }
     // the reduced chain stays in LDS once it fits in the budget (48 KB by default: with the
@@ -665,6 +698,7 @@ int precond_factor(eb_ctx* ctx) {
     }
     ctx->factor_tag = ctx->ktan_tag;
     prof_end(ctx, EB_KERNEL_PRECOND);
+    count_work(ctx, EB_KERNEL_PRECOND, 2);
     return check_launch(ctx, "chain_factor");
 }
 
@@ -1803,6 +1837,22 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
             CK(dalloc(ctx, ctx->cl_Z, c.cl_zoff.back()));
             CK(dalloc(ctx, ctx->cl_H, 144 * (int64_t)ctx->n_closures));
         }
+        // per instance: beams, tangent entries and chain positions (both covers), the units of the
+        // work counters
+        std::vector<int64_t> iw((size_t)(3 * S), 0);
+        for (int64_t s = 0; s < S; ++s) {
+            iw[3 * s] = boff[s + 1] - boff[s];
+            iw[3 * s + 1] = 36 * (int64_t)(P.bptr[noff[s + 1]] - P.bptr[noff[s]]);
+        }
+        for (const ebh::Chains* cv : {&ctx->chains, &ctx->chains2})
+            for (size_t k = 0; k + 1 < cv->ptr.size(); ++k) {
+                const int64_t n0 = cv->node[cv->ptr[k]];
+                const int64_t s = std::upper_bound(noff.begin(), noff.end(), n0) - noff.begin() - 1;
+                iw[3 * s + 2] += cv->ptr[k + 1] - cv->ptr[k];
+            }
+        CK(upload_vec(ctx, ctx->inst_work, iw));
+        CK(dalloc(ctx, ctx->work, 4));
+        if (hipMemsetAsync(ctx->work, 0, 4 * sizeof(unsigned long long), ctx->stream) != hipSuccess) return bail(EB_ERR_DEVICE);
     }
 #undef CK
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return bail(EB_ERR_DEVICE);
@@ -2568,6 +2618,20 @@ int eb_profile_enable(eb_ctx* ctx, uint32_t mask) {
     prof_harvest(ctx);
     for (int k = 0; k < EB_KERNEL_COUNT; ++k) { ctx->prof_ms[k] = 0; ctx->prof_n[k] = 0; ctx->ev_open[k] = nullptr; }
     ctx->prof_mask = mask;
+    if (ctx->work) {
+        EB_HIP(hipMemsetAsync(ctx->work, 0, 4 * sizeof(unsigned long long), ctx->stream));
+        EB_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    graph_release(ctx);   // a captured iteration holds the profiling launches of its capture
+    return EB_OK;
+}
+
+int eb_get_work_counters(eb_ctx* ctx, int64_t* out3) {
+    if (!ctx || !out3) return EB_ERR_ARG;
+    unsigned long long w[4] = {};
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->work) EB_HIP(hipMemcpy(w, ctx->work, sizeof w, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; ++k) out3[k] = (int64_t)w[k];
     return EB_OK;
 }
 

@@ -135,6 +135,7 @@ PROTOTYPES = {
     "output_points": (_i, [_vp, C.c_int32, _dp, _dp, _dp, _dp, _dp]),
     "output_points_noalloc": (_i, [_vp, C.c_int32, _dp, _dp, _dp, _dp, _dp]),
     "get_contact_stats": (_i, [_vp, _lp]),
+    "get_work_counters": (_i, [_vp, _lp]),
     "get_sdf_coefficients": (_i, [_vp, C.c_int64, _dp, C.c_int64]),
     "set_exec_mode": (_i, [_vp, _i]),
     "get_exec_stats": (_i, [_vp, C.POINTER(C.c_int32), _lp, _lp]),

@@ -115,6 +115,14 @@ class Engine:
         self._check(self.fn["get_contact_stats"](self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "get_contact_stats")
         return int(out[0]), int(out[1])
 
+    def work_counters(self):
+        """(beams through the element kernel, tangent entries through the matrix gather, chain
+        positions factored) over the launches timed since profile_enable, counting only the
+        instances active in each launch."""
+        out = np.zeros(3, np.int64)
+        self._check(self.fn["get_work_counters"](self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "get_work_counters")
+        return int(out[0]), int(out[1]), int(out[2])
+
     def profile_enable(self, kernels=()):
         mask = 0
         for k in kernels:

@@ -274,6 +274,38 @@ function run_steps!(dev, conf, params, state::RunState, max_steps::Integer)
     return state
 end
 
+# eb_run_steps_batch: the same loop per batch instance, each with its own time and Δt; states[s]
+# is instance s's record (RunState's fields in an isbits struct, so a Vector is contiguous)
+struct RunStateRec
+    t::Float64
+    dt::Float64
+    successive::Int32
+    stopped::Int32
+    steps::Int64
+    failed_steps::Int64
+    newton_iterations::Int64
+    last_k::Int32
+    pad::Int32
+end
+RunStateRec(params) = RunStateRec(0.0, params.initial_timestep, 0, 0, 0, 0, 0, 0, 0)
+
+"""run_simulation!'s time loop (src/run_simulation.jl:38-112) for every instance of a batch with
+its own time and Δt, until no running instance has fewer than `max_steps` new converged steps.
+Constant loads by default; for time-dependent ones pass
+`cb = @cfunction(f, Cint, (Ptr{Cvoid}, Int64, Float64, Ptr{Cvoid}))`, called with the instance
+and its time before each of its attempts."""
+function run_steps_batch!(dev, params, states::Vector{RunStateRec}, max_steps::Integer;
+                          cb::Ptr{Cvoid}=C_NULL, user::Ptr{Cvoid}=C_NULL)
+    tp = TimeParams(params.initial_timestep, params.min_timestep, params.max_timestep, params.simulation_end_time,
+                    params.accelerate_after_success_it, 0, params.energy_threshold)
+    np = newton(params)
+    st = ccall((:eb_run_steps_batch, LIB), Cint,
+               (Ptr{Cvoid}, Ref{NewtonParams}, Ref{TimeParams}, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{RunStateRec}),
+               dev.ctx, np, tp, max_steps, cb, user, states)
+    check(st, dev.ctx, "eb_run_steps_batch")
+    return states
+end
+
 """recompute_at_gausspts_beams! (src/IO/visualization_beams.jl:132-181) on the device: positions,
 displacements, velocities (3 × n per beam), strain and stress at n points along every beam."""
 function output_points(dev, nbeams::Integer, n::Integer=10)

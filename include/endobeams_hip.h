@@ -316,6 +316,13 @@ int eb_profile_read(eb_ctx* ctx, int id, double* total_ms, int64_t* launches);
 /* beams listed in contact by the detection and their Gauss points in contact, summed over the
  * contact-tangent launches since eb_create (out2[0], out2[1]): the work of those launches */
 int eb_get_contact_stats(eb_ctx* ctx, int64_t* out2);
+/* the work of the launches timed since eb_profile_enable (out3): [0] beams through the element
+ * kernel, [1] tangent entries through the matrix gather, [2] chain positions factored, each summed
+ * over the instances taking part in each launch (instances frozen in the Newton iteration are
+ * skipped by the kernels, so a launch's work is that of its active instances). Counted only for
+ * the kernel groups eb_profile_enable times (EB_KERNEL_BEAM, EB_KERNEL_GATHER_MATRICES,
+ * EB_KERNEL_PRECOND); eb_profile_enable resets them. */
+int eb_get_work_counters(eb_ctx* ctx, int64_t* out3);
 
 /* --- output ----------------------------------------------------------------------------------- */
 

@@ -801,6 +801,11 @@ int ebo_get_contact_stats(void* h, int64_t* out2) {
     out2[0] = out2[1] = 0;
     return EB_OK;
 }
+// device launch accounting: no counterpart on the CPU
+int ebo_get_work_counters(void*, int64_t* out3) {
+    out3[0] = out3[1] = out3[2] = 0;
+    return EB_OK;
+}
 
 int ebo_create_batch(const eb_model_desc* d, int64_t n_instances, void** out) {
     if (n_instances != 1) return EB_ERR_ARG;
