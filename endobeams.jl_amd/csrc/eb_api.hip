@@ -781,6 +781,24 @@ int krylov_iteration(eb_ctx* ctx) {
     return check_launch(ctx, "krylov_iteration");
 }
 
+// the first step of every solve: x₀ = M⁻¹ b, which ends the solve where the preconditioner holds
+// every coupling (open chains, rings); elsewhere (a net's crossing lines) BiCGStab continues from
+// x₀. The same step for every instance, so that an instance's arithmetic does not depend on what
+// else its batch holds.
+int direct_step(eb_ctx* ctx) {
+    const Segs sg = segs(ctx);
+    const unsigned nc = (unsigned)ctx->nchunks;
+    prof_begin(ctx, EB_KERNEL_KRYLOV);
+    EB_TRY(precond_apply(ctx, ctx->krhat, ctx->kx));
+    hipLaunchKernelGGL(krylov_direct_residual, dim3(nc), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->bptr, ctx->bcol,
+                       ctx->is_fixed, ctx->Ktan, ctx->kx, ctx->krhat, ctx->kr, ctx->partials, ctx->kctl);
+    hipLaunchKernelGGL(krylov_direct_fin, dim3(1), dim3(FIN_BLOCK), 0, ctx->stream, ctx->partials, sg, ctx->kctl,
+                       ctx->kall);
+    prof_end(ctx, EB_KERNEL_KRYLOV);
+    return check_launch(ctx, "krylov_direct");
+}
+int first_iteration(eb_ctx* ctx) { return direct_step(ctx); }
+
 // The linear solve in pieces: start (factorisation, BiCGStab initialisation), iterations, a poll
 // of the all-instances record (one small copy and one synchronisation), the acceptance check and
 // the scatter of the solution into ΔD. Ktan (block-row order) and res must be current.
@@ -863,7 +881,7 @@ int scatter_launch(eb_ctx* ctx, const KrylovAll* gate) {
 // their ΔD[free]
 int solve_free(eb_ctx* ctx, const eb_newton_params* p, int* its_out, bool factor = true) {
     EB_TRY(solve_start(ctx, p, factor));
-    EB_TRY(krylov_iteration(ctx));
+    EB_TRY(first_iteration(ctx));
     EB_TRY(solve_poll(ctx));
     EB_TRY(solve_finish(ctx, p, 1));
     EB_TRY(solve_check(ctx, its_out));
@@ -873,7 +891,7 @@ int solve_free(eb_ctx* ctx, const eb_newton_params* p, int* its_out, bool factor
 // the predictor's solve: instances whose solve fails are flagged in pfail (and leave `active`)
 int solve_free_screened(eb_ctx* ctx, const eb_newton_params* p, bool factor) {
     EB_TRY(solve_start(ctx, p, factor));
-    EB_TRY(krylov_iteration(ctx));
+    EB_TRY(first_iteration(ctx));
     EB_TRY(solve_poll(ctx));
     EB_TRY(solve_finish(ctx, p, 1));
     solve_account(ctx, nullptr);
@@ -1003,7 +1021,7 @@ int newton_body(eb_ctx* ctx, const eb_newton_params* p, double dt, bool start_do
     ctx->start_assembled = false;
     EB_TRY(bc_lift_launch(ctx));
     EB_TRY(solve_start(ctx, p));
-    EB_TRY(krylov_iteration(ctx));
+    EB_TRY(first_iteration(ctx));
     EB_TRY(screen_launch(ctx, p->max_iterations + 1, nullptr, ctx->kall));
     EB_TRY(scatter_launch(ctx, ctx->kall));
     EB_TRY(global_corrector_async(ctx, ctx->kall));

@@ -1681,6 +1681,50 @@ __global__ void __launch_bounds__(FIN_BLOCK) krylov_rho_fin(const double* partia
     __syncthreads();
     krylov_summary(kc, sg.ninst, all);
 }
+// ---- the direct first step ----------------------------------------------------------------------
+// When the chain cover (with the closure corrections) holds every coupling of the tangent, the
+// preconditioner M is the solve's operator A up to rounding and x = M⁻¹ b solves the system. Every
+// solve starts with that step: x = M⁻¹ b (precond_apply), r = b − A x, and the solve ends where ‖r‖
+// meets the tolerance; an instance where it does not (a net, whose crossing lines no chain holds)
+// continues with BiCGStab from that x (r₀ = r, r̂ = b, as after krylov_init).
+//
+// r = b − A x on the chunks of unfinished solves (b = r̂, which krylov_init set to the right-hand
+// side); partials: r̂·r, ‖r‖²
+__global__ void __launch_bounds__(RED_BLOCK) krylov_direct_residual(Segs sg, const int32_t* __restrict__ bptr,
+                                                                    const int32_t* __restrict__ bcol,
+                                                                    const uint8_t* __restrict__ fixed,
+                                                                    const double* __restrict__ A, const double* __restrict__ x,
+                                                                    const double* __restrict__ b, double* __restrict__ r,
+                                                                    double* partials, const KrylovCtl* kc) {
+    if (kc[sg.inst[blockIdx.x]].done) return;
+    double q[2] = {0, 0};
+    for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
+        const double ri = b[i] - masked_row(i, bptr, bcol, fixed, A, x);
+        r[i] = ri;
+        q[0] += b[i] * ri;
+        q[1] += ri * ri;
+    }
+    seg_block_store<2>(q, partials, sg.nchunks);
+}
+__global__ void __launch_bounds__(FIN_BLOCK) krylov_direct_fin(const double* partials, Segs sg, KrylovCtl* kc,
+                                                               KrylovAll* all) {
+    for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
+        KrylovCtl& k = kc[s];
+        if (k.done) continue;
+        double o[2];
+        seg_sum<2>(partials, sg, s, o);
+        k.rr2 = o[1];
+        if (o[1] <= k.tol2 * k.bnorm2) {
+            k.iters = 1;
+            k.done = 1;
+        } else {
+            k.rho = o[0];   // BiCGStab from this x: p = r at its first iteration (iters = 0)
+        }
+    }
+    __syncthreads();
+    krylov_summary(kc, sg.ninst, all);
+}
+
 // ΔD[free] = x[free] for the instances active in the Newton iteration
 // Kernels of the corrector's update that take a `gate` run only once the linear solve is done: the
 // corrector launches them behind a first Krylov iteration without waiting for the host, and
