@@ -197,6 +197,10 @@ struct eb_ctx {
     // (3 per instance) the same three sizes, summed over the instances active at each launch
     unsigned long long* work = nullptr;
     int64_t* inst_work = nullptr;
+    // per instance: 1 where the chain preconditioner holds all couplings (the direct first step);
+    // how many instances are of each kind
+    uint8_t* inst_exact = nullptr;
+    int64_t n_exact = 0, n_inexact = 0;
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
     double prof_ms[EB_KERNEL_COUNT] = {};
@@ -702,20 +706,20 @@ int precond_factor(eb_ctx* ctx) {
     return check_launch(ctx, "chain_factor");
 }
 
-int precond_apply(eb_ctx* ctx, const double* b, double* x) {
+int precond_apply(eb_ctx* ctx, const double* b, double* x, int direct_only = 0) {
     int64_t nch = (int64_t)ctx->chains.ptr.size() - 1;
     if (chain_lu(ctx)) {
         hipLaunchKernelGGL(chain_lu_solve<false>, dim3(lu_grid(nch)), dim3(64), 0, ctx->stream, chain_dev(ctx), b, x,
-                           ctx->kctl);
+                           ctx->kctl, direct_only);
         if (ctx->n_closures > 0)
             hipLaunchKernelGGL(closure_apply, dim3((unsigned)ctx->n_closures), dim3(256), 0, ctx->stream, chain_dev(ctx),
-                               closure_dev(ctx), x, ctx->kctl);
-        if (ctx->dev2.n_chains > 0) {
+                               closure_dev(ctx), x, ctx->kctl, direct_only);
+        if (ctx->dev2.n_chains > 0 && !direct_only) {   // no direct instance has a crossing cover
             const Segs sg = segs(ctx);
             hipLaunchKernelGGL(residual_masked, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->bptr,
-                               ctx->bcol, ctx->is_fixed, ctx->Ktan, x, b, ctx->cross_r, ctx->kctl);
+                               ctx->bcol, ctx->is_fixed, ctx->Ktan, x, b, ctx->cross_r, ctx->kctl, direct_only);
             hipLaunchKernelGGL(chain_lu_solve<true>, dim3(lu_grid(ctx->dev2.n_chains)), dim3(64), 0, ctx->stream, ctx->dev2,
-                               ctx->cross_r, x, ctx->kctl);
+                               ctx->cross_r, x, ctx->kctl, direct_only);
         }
         return check_launch(ctx, "chain_lu_solve");
     }
@@ -723,37 +727,37 @@ int precond_apply(eb_ctx* ctx, const double* b, double* x) {
     const bool many = nch >= SOLVE_MANY_CHAINS;
     const dim3 grid((unsigned)nch), block(many ? SOLVE_THREADS_MANY : SOLVE_THREADS_FEW);
     if (smem && many)
-        hipLaunchKernelGGL((chain_solve<true, SOLVE_THREADS_MANY>), grid, block, smem, ctx->stream, chain_dev(ctx), b, x, ctx->kctl);
+        hipLaunchKernelGGL((chain_solve<true, SOLVE_THREADS_MANY>), grid, block, smem, ctx->stream, chain_dev(ctx), b, x, ctx->kctl, direct_only);
     else if (smem)
-        hipLaunchKernelGGL((chain_solve<true, SOLVE_THREADS_FEW>), grid, block, smem, ctx->stream, chain_dev(ctx), b, x, ctx->kctl);
+        hipLaunchKernelGGL((chain_solve<true, SOLVE_THREADS_FEW>), grid, block, smem, ctx->stream, chain_dev(ctx), b, x, ctx->kctl, direct_only);
     else if (many)
-        hipLaunchKernelGGL((chain_solve<false, SOLVE_THREADS_MANY>), grid, block, 0, ctx->stream, chain_dev(ctx), b, x, ctx->kctl);
+        hipLaunchKernelGGL((chain_solve<false, SOLVE_THREADS_MANY>), grid, block, 0, ctx->stream, chain_dev(ctx), b, x, ctx->kctl, direct_only);
     else
-        hipLaunchKernelGGL((chain_solve<false, SOLVE_THREADS_FEW>), grid, block, 0, ctx->stream, chain_dev(ctx), b, x, ctx->kctl);
+        hipLaunchKernelGGL((chain_solve<false, SOLVE_THREADS_FEW>), grid, block, 0, ctx->stream, chain_dev(ctx), b, x, ctx->kctl, direct_only);
     if (ctx->n_closures > 0)
         hipLaunchKernelGGL(closure_apply, dim3((unsigned)ctx->n_closures), dim3(256), 0, ctx->stream, chain_dev(ctx),
-                           closure_dev(ctx), x, ctx->kctl);
-    if (ctx->dev2.n_chains > 0) {
+                           closure_dev(ctx), x, ctx->kctl, direct_only);
+    if (ctx->dev2.n_chains > 0 && !direct_only) {   // no direct instance has a crossing cover
         // the crossing cover: x += T₂⁻¹ (b − A x)
         const Segs sg = segs(ctx);
         hipLaunchKernelGGL(residual_masked, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->bptr,
-                           ctx->bcol, ctx->is_fixed, ctx->Ktan, x, b, ctx->cross_r, ctx->kctl);
+                           ctx->bcol, ctx->is_fixed, ctx->Ktan, x, b, ctx->cross_r, ctx->kctl, direct_only);
         const int64_t n2 = ctx->dev2.n_chains;
         const size_t smem2 = tuned(ctx, EB_TUNE_CHAIN_SOLVE_LDS) == 1 ? chain_solve_lds_bytes(ctx->chains2.max_len, ctx->solve_lds_max) : 0;
         const bool many2 = n2 >= SOLVE_MANY_CHAINS;
         const dim3 grid2((unsigned)n2), block2(many2 ? SOLVE_THREADS_MANY : SOLVE_THREADS_FEW);
         if (smem2 && many2)
             hipLaunchKernelGGL((chain_solve<true, SOLVE_THREADS_MANY, true>), grid2, block2, smem2, ctx->stream, ctx->dev2,
-                               ctx->cross_r, x, ctx->kctl);
+                               ctx->cross_r, x, ctx->kctl, direct_only);
         else if (smem2)
             hipLaunchKernelGGL((chain_solve<true, SOLVE_THREADS_FEW, true>), grid2, block2, smem2, ctx->stream, ctx->dev2,
-                               ctx->cross_r, x, ctx->kctl);
+                               ctx->cross_r, x, ctx->kctl, direct_only);
         else if (many2)
             hipLaunchKernelGGL((chain_solve<false, SOLVE_THREADS_MANY, true>), grid2, block2, 0, ctx->stream, ctx->dev2,
-                               ctx->cross_r, x, ctx->kctl);
+                               ctx->cross_r, x, ctx->kctl, direct_only);
         else
             hipLaunchKernelGGL((chain_solve<false, SOLVE_THREADS_FEW, true>), grid2, block2, 0, ctx->stream, ctx->dev2,
-                               ctx->cross_r, x, ctx->kctl);
+                               ctx->cross_r, x, ctx->kctl, direct_only);
     }
     return check_launch(ctx, "chain_solve");
 }
@@ -781,15 +785,15 @@ int krylov_iteration(eb_ctx* ctx) {
     return check_launch(ctx, "krylov_iteration");
 }
 
-// the first step of every solve: x₀ = M⁻¹ b, which ends the solve where the preconditioner holds
-// every coupling (open chains, rings); elsewhere (a net's crossing lines) BiCGStab continues from
-// x₀. The same step for every instance, so that an instance's arithmetic does not depend on what
-// else its batch holds.
+// the first step of a solve: x = M⁻¹ b for the instances whose chain preconditioner holds every
+// coupling (open chains, rings), which ends their solve; a BiCGStab iteration for the others (a
+// net's crossing lines). Which instance takes which depends on the instance alone, so that its
+// arithmetic does not depend on what else its batch holds.
 int direct_step(eb_ctx* ctx) {
     const Segs sg = segs(ctx);
     const unsigned nc = (unsigned)ctx->nchunks;
     prof_begin(ctx, EB_KERNEL_KRYLOV);
-    EB_TRY(precond_apply(ctx, ctx->krhat, ctx->kx));
+    EB_TRY(precond_apply(ctx, ctx->krhat, ctx->kx, 1));
     hipLaunchKernelGGL(krylov_direct_residual, dim3(nc), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->bptr, ctx->bcol,
                        ctx->is_fixed, ctx->Ktan, ctx->kx, ctx->krhat, ctx->kr, ctx->partials, ctx->kctl);
     hipLaunchKernelGGL(krylov_direct_fin, dim3(1), dim3(FIN_BLOCK), 0, ctx->stream, ctx->partials, sg, ctx->kctl,
@@ -797,7 +801,10 @@ int direct_step(eb_ctx* ctx) {
     prof_end(ctx, EB_KERNEL_KRYLOV);
     return check_launch(ctx, "krylov_direct");
 }
-int first_iteration(eb_ctx* ctx) { return direct_step(ctx); }
+int first_iteration(eb_ctx* ctx) {
+    if (ctx->n_exact > 0) EB_TRY(direct_step(ctx));
+    return ctx->n_inexact > 0 ? krylov_iteration(ctx) : EB_OK;
+}
 
 // The linear solve in pieces: start (factorisation, BiCGStab initialisation), iterations, a poll
 // of the all-instances record (one small copy and one synchronisation), the acceptance check and
@@ -813,7 +820,7 @@ int solve_start(eb_ctx* ctx, const eb_newton_params* p, bool factor = true) {
     hipLaunchKernelGGL(krylov_init, dim3((unsigned)ctx->nchunks), dim3(RED_BLOCK), 0, ctx->stream, sg, ctx->is_fixed,
                        ctx->res, ctx->kx, ctx->kr, ctx->krhat, ctx->kp, ctx->kv, ctx->partials);
     hipLaunchKernelGGL(krylov_init_fin, dim3(1), dim3(FIN_BLOCK), 0, ctx->stream, ctx->partials, sg, ctx->kctl, tol,
-                       ctx->active, ctx->pivot_inst, ctx->kall);
+                       ctx->active, ctx->pivot_inst, ctx->inst_exact, ctx->kall);
     return check_launch(ctx, "krylov_init");
 }
 
@@ -1804,6 +1811,7 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
         CK(dalloc(ctx, ctx->ch_bw, 6 * N));
         CK(dalloc(ctx, ctx->ch_yw, 6 * N));
         ctx->n_closures = (int32_t)c.cl_chain.size();
+        std::vector<uint8_t> exact((size_t)S, 1);
         if (ebh::chains_left_out(c, P) > 0) {
             // the crossing cover holds the nodes of the instances that have couplings the first
             // cover leaves out (nets), and no node of the others, whose solve is complete without it
@@ -1813,6 +1821,7 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
                 for (int64_t n = noff[s]; n < noff[s + 1]; ++n) inst_left[s] |= left[n];
             for (int64_t s = 0; s < S; ++s)
                 for (int64_t n = noff[s]; n < noff[s + 1]; ++n) cover[n] = inst_left[s];
+            for (int64_t s = 0; s < S; ++s) exact[s] = inst_left[s] ? 0 : 1;
             ebh::build_chains(ctx->chains2, P, d->X0, 0, &c, &cover);
             const ebh::Chains& c2 = ctx->chains2;
             std::vector<int32_t> chi2(c2.ptr.size() - 1);
@@ -1869,6 +1878,8 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
                 iw[3 * s + 2] += cv->ptr[k + 1] - cv->ptr[k];
             }
         CK(upload_vec(ctx, ctx->inst_work, iw));
+        CK(upload_vec(ctx, ctx->inst_exact, exact));
+        for (int64_t s = 0; s < S; ++s) (exact[s] ? ctx->n_exact : ctx->n_inexact) += 1;
         CK(dalloc(ctx, ctx->work, 4));
         if (hipMemsetAsync(ctx->work, 0, 4 * sizeof(unsigned long long), ctx->stream) != hipSuccess) return bail(EB_ERR_DEVICE);
     }

@@ -350,7 +350,11 @@ struct KrylovCtl {
     double rho, rho_old, alpha, omega, bnorm2, rr2, ss2, tol2;
     int done, iters, breakdown;
     int pivot;     // 1: the instance's factorisation met a singular pivot block (its solve failed)
+    int direct;    // 1: the chain preconditioner holds all the instance's couplings (direct first step)
 };
+// whether a preconditioner application of this solve works on the instance: its solve is open
+// and, in the direct step (direct_only), its preconditioner is exact
+EBD bool krylov_open(const KrylovCtl& k, int direct_only) { return !k.done && (!direct_only || k.direct); }
 
 // row i of the solve's operator times x: the identity on fixed dofs, the tangent's block row
 // (node i / 6, row i % 6) elsewhere, summed block by block in column order
@@ -394,8 +398,8 @@ __global__ void __launch_bounds__(RED_BLOCK) residual_masked(Segs sg, const int3
                                                              const uint8_t* __restrict__ fixed,
                                                              const double* __restrict__ A, const double* __restrict__ x,
                                                              const double* __restrict__ b, double* __restrict__ r,
-                                                             const KrylovCtl* kc) {
-    if (kc[sg.inst[blockIdx.x]].done) return;
+                                                             const KrylovCtl* kc, int direct_only) {
+    if (!krylov_open(kc[sg.inst[blockIdx.x]], direct_only)) return;
     for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
         const double acc = masked_row(i, bptr, bcol, fixed, A, x);
         r[i] = b[i] - acc;
@@ -938,9 +942,10 @@ EBD void chain_solve_body(const ChainDev& cd, int64_t s0, int32_t n, double* bw,
 // ACCUM: x += M⁻¹ b (the crossing cover's correction) instead of x = M⁻¹ b
 template <bool LDSV, int SOLVE_THREADS, bool ACCUM = false>
 __global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const double* __restrict__ b,
-                                                             double* __restrict__ x, const KrylovCtl* kc) {
+                                                             double* __restrict__ x, const KrylovCtl* kc,
+                                                             int direct_only) {
     extern __shared__ __align__(16) double solve_smem[];
-    if (kc[cd.inst[blockIdx.x]].done) return;
+    if (!krylov_open(kc[cd.inst[blockIdx.x]], direct_only)) return;
     const int64_t s0 = cd.ptr[blockIdx.x];
     const int32_t n = cd.ptr[blockIdx.x + 1] - (int32_t)s0;
     const int tid = threadIdx.x;
@@ -1316,11 +1321,11 @@ EBD void chain_lu_solve_group(const ChainDev& cd, const LuHalf& h, int32_t smax,
 // x = M⁻¹ b (ACCUM: x += M⁻¹ b) along each chain of an unfinished solve
 template <bool ACCUM>
 __global__ void __launch_bounds__(64) chain_lu_solve(ChainDev cd, const double* __restrict__ b, double* __restrict__ x,
-                                                     const KrylovCtl* kc) {
+                                                     const KrylovCtl* kc, int direct_only) {
     __shared__ __align__(16) double ex[LU_GROUPS * 12];
     const Lane6 ln = lane6(nullptr);
     const int64_t ch = (int64_t)blockIdx.x * LU_CHAINS + (ln.group >> 1);
-    const bool live = ln.r < 6 && ch < cd.n_chains && !kc[cd.inst[ch]].done;
+    const bool live = ln.r < 6 && ch < cd.n_chains && krylov_open(kc[cd.inst[ch]], direct_only);
     const LuHalf h = lu_half(cd, ch, live, ln.group);
     if (wave_max_i32(live ? h.n : 0) == 0) return;
     const int32_t smax = wave_max_i32(h.steps);
@@ -1477,11 +1482,11 @@ __global__ void __launch_bounds__(64) closure_setup(ChainDev cd, ClosureDev cl, 
 
 // y −= Z (H (Uᵀ y)) on each closed chain of an unfinished solve, after the chain solve
 __global__ void __launch_bounds__(256) closure_apply(ChainDev cd, ClosureDev cl, double* __restrict__ y,
-                                                     const KrylovCtl* __restrict__ kc) {
+                                                     const KrylovCtl* __restrict__ kc, int direct_only) {
     __shared__ double sv[12];
     const int c = blockIdx.x;
     const int32_t ch = cl.chain[c];
-    if (kc[cd.inst[ch]].done) return;
+    if (!krylov_open(kc[cd.inst[ch]], direct_only)) return;
     const int64_t s0 = cd.ptr[ch];
     const int32_t n = cd.ptr[ch + 1] - (int32_t)s0;
     if (threadIdx.x < 12) {
@@ -1578,7 +1583,7 @@ __global__ void __launch_bounds__(RED_BLOCK) krylov_init(Segs sg, const uint8_t*
 // instances whose factorisation met a singular pivot block start done and failed
 __global__ void __launch_bounds__(FIN_BLOCK) krylov_init_fin(const double* partials, Segs sg, KrylovCtl* kc, double tol,
                                                              const uint8_t* active, const int32_t* pivot_inst,
-                                                             KrylovAll* all) {
+                                                             const uint8_t* exact, KrylovAll* all) {
     for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
         double o[1];
         seg_sum<1>(partials, sg, s, o);
@@ -1586,6 +1591,7 @@ __global__ void __launch_bounds__(FIN_BLOCK) krylov_init_fin(const double* parti
         k.bnorm2 = o[0]; k.rho = o[0]; k.rho_old = 1; k.alpha = 1; k.omega = 1;
         k.rr2 = o[0]; k.ss2 = 0; k.tol2 = tol * tol; k.iters = 0; k.breakdown = 0;
         k.pivot = (active[s] && pivot_inst[s]) ? 1 : 0;
+        k.direct = exact[s];
         k.done = (o[0] == 0.0 || !active[s] || k.pivot) ? 1 : 0;
     }
     __syncthreads();
@@ -1683,20 +1689,21 @@ __global__ void __launch_bounds__(FIN_BLOCK) krylov_rho_fin(const double* partia
 }
 // ---- the direct first step ----------------------------------------------------------------------
 // When the chain cover (with the closure corrections) holds every coupling of the tangent, the
-// preconditioner M is the solve's operator A up to rounding and x = M⁻¹ b solves the system. Every
-// solve starts with that step: x = M⁻¹ b (precond_apply), r = b − A x, and the solve ends where ‖r‖
-// meets the tolerance; an instance where it does not (a net, whose crossing lines no chain holds)
-// continues with BiCGStab from that x (r₀ = r, r̂ = b, as after krylov_init).
+// preconditioner M is the solve's operator A up to rounding and x = M⁻¹ b solves the system. The
+// solve of such an instance (KrylovCtl::direct, a property of the instance alone: no coupling left
+// to a second chain cover) starts with that step: x = M⁻¹ b (precond_apply), r = b − A x, and ends
+// where ‖r‖ meets the tolerance; if it does not, BiCGStab continues from that x (r₀ = r, r̂ = b, as
+// after krylov_init). The other instances (nets) run BiCGStab from x = 0.
 //
-// r = b − A x on the chunks of unfinished solves (b = r̂, which krylov_init set to the right-hand
-// side); partials: r̂·r, ‖r‖²
+// r = b − A x on the chunks of unfinished direct solves (b = r̂, which krylov_init set to the
+// right-hand side); partials: r̂·r, ‖r‖²
 __global__ void __launch_bounds__(RED_BLOCK) krylov_direct_residual(Segs sg, const int32_t* __restrict__ bptr,
                                                                     const int32_t* __restrict__ bcol,
                                                                     const uint8_t* __restrict__ fixed,
                                                                     const double* __restrict__ A, const double* __restrict__ x,
                                                                     const double* __restrict__ b, double* __restrict__ r,
                                                                     double* partials, const KrylovCtl* kc) {
-    if (kc[sg.inst[blockIdx.x]].done) return;
+    if (!krylov_open(kc[sg.inst[blockIdx.x]], 1)) return;
     double q[2] = {0, 0};
     for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
         const double ri = b[i] - masked_row(i, bptr, bcol, fixed, A, x);
@@ -1710,7 +1717,7 @@ __global__ void __launch_bounds__(FIN_BLOCK) krylov_direct_fin(const double* par
                                                                KrylovAll* all) {
     for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
         KrylovCtl& k = kc[s];
-        if (k.done) continue;
+        if (!krylov_open(k, 1)) continue;
         double o[2];
         seg_sum<2>(partials, sg, s, o);
         k.rr2 = o[1];
