@@ -448,21 +448,8 @@ __global__ void __launch_bounds__(4 * BEAM_TILE) beam_detect_kernel(NodeState ns
     beam_kernel_body<S>(ns, bc, st, gp, active, beam_inst, DetectHook<S>{ca, is_contact, list, count});
 }
 
-// One block column: FP64 throughout for the FP64 form; for the precision form (C = float) the
-// tangent in FP32, its contact force and energy (FP64: they enter the residual) computed by
-// contact_force_kernel before.
-template <typename C, int B>
-EBD void contact_part(int64_t e, unsigned mask, const NodeState& ns, const BeamConsts& bc, const BeamStage<C>& st,
-                      const GaussPts& gp, const ContactArgs<C>& ca) {
-    if constexpr (sizeof(C) == sizeof(double)) {
-        contact_column<double, C, B, CONTACT_FORCE | CONTACT_TANGENT>(e, mask, ns, bc, st, gp, ca);
-    } else {
-        contact_column<C, C, B, CONTACT_TANGENT>(e, mask, ns, bc, st, gp, ca);
-    }
-}
-
 // Precision form: the FP64 contact force (Tᶜ of all four node slots) and energy of every listed
-// beam, one lane per beam, with the arithmetic of the FP64 pass of each block column.
+// beam, one lane per beam, with the arithmetic of the FP64 pass of the tangent kernel.
 template <typename P>
 __global__ void __launch_bounds__(256) contact_force_kernel(NodeState ns, BeamConsts bc, BeamStage<P> st, GaussPts gp,
                                                             ContactArgs<P> ca, const uint64_t* __restrict__ list,
@@ -474,19 +461,345 @@ __global__ void __launch_bounds__(256) contact_force_kernel(NodeState ns, BeamCo
                                                                 gp, ca);
 }
 
-// Pass 2: four wavefronts per 64 listed beams, wavefront b computing block column b. Launched for
-// the whole contact list; workgroups past the number of beams found in contact exit at once.
+// ---- the contact tangent in two phases --------------------------------------------------------
+//
+// Per beam and Gauss point, everything up to the derivatives of the contact force, 𝒦ᶠᶜᵇ = ∂𝓯ᶜ/∂dᵇ and
+// 𝒞ᶠᶜᵇ = ∂𝓯ᶜ/∂ḋᵇ (b = 1..4), is shared by the sixteen blocks of Kᶜ and Cᶜ; block (a, b) combines
+// them with quantities of row a alone (H₁ᵃ, RₑH₁ᵃRₑᵀ, Ŝ(H₁ᵃᵀ𝓕ᶜ), RₑGᵃ) and of the Gauss point
+// (𝓕ᶜ, the P₁Pᵇ blocks). Phase 1: wavefront g of a 64-beam tile evaluates Gauss point g of each
+// beam up to 𝒦ᶠᶜ, 𝒞ᶠᶜ and 𝓕ᶜ and leaves them in LDS (component-major, one column per beam, so a
+// wavefront's accesses are consecutive); phase 2: wavefront a sums block row a over the Gauss
+// points in their order. Every expression is the one of contact_column, evaluated once instead of
+// once per block column.
 constexpr int CONTACT_TILE = 64;
+constexpr int CONTACT_GP_CHUNK = 3;   // Gauss points per phase-1 pass (wavefronts 0..2)
 
+template <typename P, bool FORCE> struct ContactShared {
+    P kfc[CONTACT_GP_CHUNK][36][CONTACT_TILE];   // 𝒦ᶠᶜᵇ, b-major, each 3x3 column-major
+    P cfc[CONTACT_GP_CHUNK][36][CONTACT_TILE];   // 𝒞ᶠᶜᵇ
+    P fc[CONTACT_GP_CHUNK][3][CONTACT_TILE];     // 𝓕ᶜ = Rₑᵀ𝓯ᶜ
+    P gn[CONTACT_GP_CHUNK][CONTACT_TILE];        // gap (the smoothstep chain of ηₙ runs over the points)
+    P tt[2][9][CONTACT_TILE];                    // Tₛ⁻¹(Θ₁)ᵀ, Tₛ⁻¹(Θ₂)ᵀ (C rows 2 and 4)
+};
+template <typename P> struct ContactShared<P, true> {
+    P kfc[CONTACT_GP_CHUNK][36][CONTACT_TILE];
+    P cfc[CONTACT_GP_CHUNK][36][CONTACT_TILE];
+    P fc[CONTACT_GP_CHUNK][3][CONTACT_TILE];
+    P gn[CONTACT_GP_CHUNK][CONTACT_TILE];
+    P tt[2][9][CONTACT_TILE];
+    double tc[CONTACT_GP_CHUNK][12][CONTACT_TILE];   // ωᴳ RₑH₁ᵃRₑᵀᵀ𝓯ᶜ, a-major
+    double en[CONTACT_GP_CHUNK][CONTACT_TILE];       // ωᴳ kₙ Πₑ
+};
+
+// A listed beam's frame and the per-beam quantities both phases use, in the physics type P.
+template <typename P> struct ContactBeam {
+    int64_t e;
+    unsigned mask;
+    int32_t inst, na, nb;
+    double rb, l0;
+    V3<double> x1, x2, Tb1, Tb2;
+    M3<double> Red;
+    M3<P> Re, ReT, G1, G2, G4;
+    V3<P> v1, tb1, tb2;
+    P ln;
+};
+
+template <typename P, typename C>
+EBD ContactBeam<P> contact_beam(int64_t e, unsigned mask, const NodeState& ns, const BeamConsts& bc, const ContactArgs<C>& ca) {
+    ContactBeam<P> cb;
+    const int64_t E = bc.E;
+    cb.e = e;
+    cb.mask = mask;
+    cb.inst = ca.beam_inst[e];
+    cb.na = bc.n1[e];
+    cb.nb = bc.n2[e];
+    cb.rb = bc.props[0 * E + e];
+    cb.x1 = ldv(ns.X0, cb.na) + ldv(ns.u, cb.na);
+    cb.x2 = ldv(ns.X0, cb.nb) + ldv(ns.u, cb.nb);
+    const double lnd = norm(cb.x2 - cb.x1);
+    M3<double> Re0;
+    for (int k = 0; k < 9; ++k) Re0.a[k] = bc.Re0[k * E + e];
+    const M3<double> R1 = ldm(ns.R, cb.na), R2 = ldm(ns.R, cb.nb);
+    const LocalFrame f = local_frame(cb.x1, cb.x2, R1, R2, col(Re0, 1), lnd);
+    const M3<double> ReTd = tr(f.Re);
+    cb.Tb1 = toangle((ReTd * R1) * Re0);
+    cb.Tb2 = toangle((ReTd * R2) * Re0);
+    cb.Red = f.Re;
+    cb.l0 = bc.l0[e];
+    cb.Re = cvt<P>(f.Re);
+    cb.ReT = tr(cb.Re);
+    cb.ln = P(lnd);
+    cb.G1 = rows(P(0), P(0), P(f.eta / lnd), P(0), P(0), P(1 / lnd), P(0), P(-1 / lnd), P(0));
+    cb.G2 = rows(P(f.eta12 / 2), P(-f.eta11 / 2), P(0), P(0), P(0), P(0), P(0), P(0), P(0));
+    cb.G4 = rows(P(f.eta22 / 2), P(-f.eta21 / 2), P(0), P(0), P(0), P(0), P(0), P(0), P(0));
+    cb.v1 = cvt<P>(f.v1);
+    cb.tb1 = cvt<P>(cb.Tb1);
+    cb.tb2 = cvt<P>(cb.Tb2);
+    return cb;
+}
+
+// The shape-function quantities of one Gauss point (N₁…N₄, N₇/lₙ, N₇/lₙ², uᵗ, the P₁P blocks).
+template <typename P> struct GaussShape {
+    P N1, N2, N7l, N7l2;
+    M3<P> Sut, PP1, PP2, PP4;
+};
+template <typename P> EBD GaussShape<P> gauss_shape(const ContactBeam<P>& cb, const ShapeF& sd) {
+    GaussShape<P> gs;
+    gs.N1 = P(sd.N1);
+    gs.N2 = P(sd.N2);
+    const P N3 = P(sd.N3), N4 = P(sd.N4), N7 = P(sd.N7);
+    V3<P> ut = {P(0), N3 * cb.tb1.z + N4 * cb.tb2.z, (-N3) * cb.tb1.y + (-N4) * cb.tb2.y};
+    gs.Sut = skew(ut);
+    gs.N7l = N7 / cb.ln;
+    gs.N7l2 = gs.N7l / cb.ln;
+    gs.PP1 = rows(P(0), P(0), P(0), P(0), gs.N7l, P(0), P(0), P(0), gs.N7l);
+    gs.PP2 = rows(P(0), P(0), P(0), P(0), P(0), N3, P(0), -N3, P(0));
+    gs.PP4 = rows(P(0), P(0), P(0), P(0), P(0), N4, P(0), -N4, P(0));
+    return gs;
+}
+// H₁ᵃ (compute&assemble_interactions_beams.jl:84-87), with P₁P³ = -P₁P¹ and Gᵀ³ = -Gᵀ¹
+template <typename P, int A> EBD M3<P> gauss_H1(const ContactBeam<P>& cb, const GaussShape<P>& gs) {
+    const M3<P> I = eye<P>();
+    if constexpr (A == 0) return (gs.N1 * I + gs.PP1) - gs.Sut * cb.G1;
+    else if constexpr (A == 1) return gs.PP2 - gs.Sut * cb.G2;
+    else if constexpr (A == 2) return (gs.N2 * I + (-gs.PP1)) - gs.Sut * (-cb.G1);
+    else return gs.PP4 - gs.Sut * cb.G4;
+}
+
+// ηₙ after the smoothstep of a contacting point with gap gₙ (unchanged where gₙ ≤ 0)
+template <typename P> EBD P eta_after(P eta_n, P gn, P gb) {
+    if (gn <= P(0)) return eta_n;
+    P ss = (gn - gb) / gb;
+    return ((ss * ss) * (P(3) + P(2) * ss)) * eta_n;
+}
+
+// Phase 1: Gauss point ig of one beam, up to 𝒦ᶠᶜᵇ, 𝒞ᶠᶜᵇ, 𝓕ᶜ (and, with FORCE, the point's terms
+// of Tᶜ and of the contact energy) in LDS slot gi. eta_n: ηₙ after the beam's earlier contacting
+// points; gn, n, Hg, h1, H1F1: the point's gap and kinematics, computed before the barrier that
+// published the gaps.
+template <typename P, typename C, bool FORCE>
+EBD void contact_point(const ContactBeam<P>& cb, const NodeState& ns, const GaussPts& gp, const ContactArgs<C>& ca, int ig,
+                       int gi, int lane, P eta_n, P gn, const V3<P>& n, const M3<P>& Hg, const V3<P>& h1,
+                       const M3<P>& H1F1, ContactShared<P, FORCE>& sh) {
+    const GaussShape<P> gs = gauss_shape(cb, shape_fn(cb.l0, gp.z[ig]));
+    M3<P> H1[4];
+    H1[0] = gauss_H1<P, 0>(cb, gs);
+    H1[1] = gauss_H1<P, 1>(cb, gs);
+    H1[2] = gauss_H1<P, 2>(cb, gs);
+    H1[3] = gauss_H1<P, 3>(cb, gs);
+    V3<P> Ud[4];
+    Ud[0] = tmul(cb.Re, cvt<P>(ldv(ns.ud, cb.na)));
+    Ud[1] = tmul(cb.Re, cvt<P>(ldv(ns.wd, cb.na)));
+    Ud[2] = tmul(cb.Re, cvt<P>(ldv(ns.ud, cb.nb)));
+    Ud[3] = tmul(cb.Re, cvt<P>(ldv(ns.wd, cb.nb)));
+    const M3<P> I = eye<P>();
+    const M3<P>& Re = cb.Re;
+    const M3<P>& ReT = cb.ReT;
+    const M3<P> G3 = -cb.G1;
+    const M3<P> G1R = cb.G1 * ReT, G2R = cb.G2 * ReT, G4R = cb.G4 * ReT;
+    const P w = P(gp.w[ig]);
+    const P rb = P(cb.rb);
+    const P kn = P(ca.kn), mu = P(ca.mu), eps_t = P(ca.eps_t);
+    const V3<P> ud0 = Re * h1;
+    // regularize_gap_penalty_beams
+    P gb = rb / P(4), hh = gb / P(2), pn, dpn, Pi;
+    if (gn <= P(0)) {
+        pn = hh - gn; dpn = P(-1);
+        Pi = ((gn * gn) / P(2) - hh * gn) + (gb * gb) / P(6);
+    } else {
+        P aux = (gb - hh) / (gb * gb);
+        pn = (aux * (gn * gn) - gn) + hh;
+        dpn = (P(2) * aux) * gn - P(1);
+        Pi = (((gb - hh) / (P(3) * (gb * gb))) * ((gn * gn) * gn) - (gn * gn) / P(2) + hh * gn) - (gb * gb) / P(6);
+    }
+    // smoothstep_transition_beams
+    P deta;
+    if (gn <= P(0)) {
+        deta = P(0);
+    } else {
+        P ss = (gn - gb) / gb;
+        deta = (((P(6) / gb) * ss) * (P(1) + ss)) * eta_n;
+        eta_n = ((ss * ss) * (P(3) + P(2) * ss)) * eta_n;
+    }
+    P udn_mag = dot(ud0, n);
+    V3<P> udn = udn_mag * n;
+    V3<P> udt = ud0 - udn;
+    P udt2 = dot(udt, udt);
+    if constexpr (FORCE) sh.en[gi][lane] = double((w * kn) * Pi);
+    V3<P> fn = (kn * pn) * n - eta_n * udn;
+    P mureg = mu / sqrt(udt2 + eps_t);
+    V3<P> ft = (((-kn) * pn) * mureg) * udt;
+    V3<P> fc = fn + ft;
+    V3<P> Fc = tmul(Re, fc);
+    sh.fc[gi][0][lane] = Fc.x;
+    sh.fc[gi][1][lane] = Fc.y;
+    sh.fc[gi][2][lane] = Fc.z;
+    M3<P> RHR[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) RHR[a] = (Re * H1[a]) * ReT;
+    if constexpr (FORCE) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            V3<P> t = w * tmul(RHR[a], fc);
+            sh.tc[gi][3 * a][lane] = double(t.x);
+            sh.tc[gi][3 * a + 1][lane] = double(t.y);
+            sh.tc[gi][3 * a + 2][lane] = double(t.z);
+        }
+    }
+    M3<P> aux = dot(n, ud0) * Hg + outer(n, Hg * ud0);
+    M3<P> RSh1 = Re * skew(h1);
+    const V3<P>& Ud1 = Ud[0];
+    const V3<P>& Wd1 = Ud[1];
+    const V3<P>& Ud2 = Ud[2];
+    const V3<P>& Wd2 = Ud[3];
+    const V3<P>& v1 = cb.v1;
+    M3<P> A1Dr = rows(P(0), P(0), P(0), v1.x * (Ud1.y - Ud2.y), v1.y * (Ud1.y - Ud2.y), v1.z * (Ud1.y - Ud2.y),
+                      v1.x * (Ud1.z - Ud2.z), v1.y * (Ud1.z - Ud2.z), v1.z * (Ud1.z - Ud2.z));
+    M3<P> RSGD = Re * skew(((cb.G1 * Ud1 + cb.G2 * Wd1) + G3 * Ud2) + cb.G4 * Wd2);
+    M3<P> RH1SD = Re * H1F1;
+    M3<P> du0dd[4];
+    du0dd[0] = ((-(RSh1 * G1R)) + ((gs.N7l2 * Re) * A1Dr + (RSGD * gs.PP1) * ReT)) + RH1SD * G1R;
+    du0dd[1] = ((-(RSh1 * G2R)) + (RSGD * gs.PP2) * ReT) + RH1SD * G2R;
+    du0dd[2] = -du0dd[0];
+    du0dd[3] = ((-(RSh1 * G4R)) + (RSGD * gs.PP4) * ReT) + RH1SD * G4R;
+    M3<P> nn = outer(n, n);
+    M3<P> auxn = ((kn * dpn) * nn + (kn * pn) * Hg) - outer(deta * udn, n);
+    M3<P> aux1 = outer((((-mureg) * kn) * dpn) * udt, n);
+    M3<P> aux2 = (((-mureg) * kn) * pn) * (I - outer((P(1) / (udt2 + eps_t)) * udt, udt));
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        M3<P> dundd = aux * RHR[b] + nn * du0dd[b];
+        M3<P> dutdd = du0dd[b] - dundd;
+        M3<P> dutdv = RHR[b] - nn * RHR[b];
+        M3<P> Kfn = auxn * RHR[b] - eta_n * dundd;
+        M3<P> Cfn = ((-eta_n) * nn) * RHR[b];
+        M3<P> Kft = aux1 * RHR[b] + aux2 * dutdd;
+        M3<P> Cft = aux2 * dutdv;
+        const M3<P> Kfc = Kfn + Kft;
+        const M3<P> Cfc = Cfn + Cft;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            sh.kfc[gi][9 * b + k][lane] = Kfc.a[k];
+            sh.cfc[gi][9 * b + k][lane] = Cfc.a[k];
+        }
+    }
+}
+
+// Phase 2, block row A of one beam: Kᶜᴬᵇ += ωᴳ (t₁ + t₂ + t₃ + t₄), Cᶜᴬᵇ += ωᴳ RₑH₁ᴬRₑᵀᵀ 𝒞ᶠᶜᵇ over the
+// chunk's contacting points (compute&assemble_interactions_beams.jl:132-212, :308-372)
+template <typename P, bool FORCE, int A>
+EBD void contact_row(const ContactBeam<P>& cb, const GaussPts& gp, int g0, int lane, const ContactShared<P, FORCE>& sh,
+                     M3<P> (&Kc)[4], M3<P> (&Cc)[4], V3<P>& Tca, double& energy) {
+    const M3<P>& Re = cb.Re;
+    const M3<P>& ReT = cb.ReT;
+    const M3<P> G1R = cb.G1 * ReT, G2R = cb.G2 * ReT, G4R = cb.G4 * ReT;
+    const M3<P> RG1 = Re * tr(cb.G1);
+    const M3<P> RGa = A == 0 ? RG1 : (A == 1 ? Re * tr(cb.G2) : (A == 2 ? -RG1 : Re * tr(cb.G4)));
+    const V3<P>& v1 = cb.v1;
+    const int gend = min(gp.n, g0 + CONTACT_GP_CHUNK);
+#pragma unroll 1
+    for (int ig = g0; ig < gend; ++ig) {
+        if (!(cb.mask & (1u << ig))) continue;
+        const int gi = ig - g0;
+        const P w = P(gp.w[ig]);
+        const GaussShape<P> gs = gauss_shape(cb, shape_fn(cb.l0, gp.z[ig]));
+        const M3<P> H1 = gauss_H1<P, A>(cb, gs);
+        const V3<P> Fc = {sh.fc[gi][0][lane], sh.fc[gi][1][lane], sh.fc[gi][2][lane]};
+        if constexpr (FORCE) {
+            Tca += V3<P>{P(sh.tc[gi][3 * A][lane]), P(sh.tc[gi][3 * A + 1][lane]), P(sh.tc[gi][3 * A + 2][lane])};
+            if (A == 0) energy += sh.en[gi][lane];
+        }
+        const M3<P> tRHR = tr((Re * H1) * ReT);
+        const M3<P> SF = skew(Fc);
+        const M3<P> RS = Re * skew(tmul(H1, Fc));
+        const M3<P> RHS = (Re * tr(H1)) * SF;
+        const M3<P> SFP1 = (SF * gs.PP1) * ReT, SFP2 = (SF * gs.PP2) * ReT, SFP4 = (SF * gs.PP4) * ReT;
+        const M3<P> A1F = rows(P(0), P(0), P(0), Fc.y * v1.x, Fc.y * v1.y, Fc.y * v1.z, Fc.z * v1.x, Fc.z * v1.y, Fc.z * v1.z);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            M3<P> Kfc, Cfc;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                Kfc.a[k] = sh.kfc[gi][9 * b + k][lane];
+                Cfc.a[k] = sh.cfc[gi][9 * b + k][lane];
+            }
+            // t₁ᵃᵇ = -Rₑ S(H₁ᵃᵀ𝓕ᶜ) Gᵇ Rₑᵀ with G³ = -G¹; t₃ᵃᵇ = Rₑ H₁ᵃᵀ S𝓕ᶜ Gᵇ Rₑᵀ
+            const M3<P> GbR = b == 0 ? G1R : (b == 1 ? G2R : (b == 2 ? G1R : G4R));
+            const M3<P> t1 = b == 2 ? RS * GbR : -(RS * GbR);
+            const M3<P> t3 = b == 2 ? -(RHS * GbR) : RHS * GbR;
+            // t₂ (:166-184): rows 1 and 3 carry the A₁ᵀ𝓕ᶜr term; row 3 mirrors row 1
+            const M3<P> SFPb = b == 0 ? SFP1 : (b == 1 ? SFP2 : (b == 2 ? SFP1 : SFP4));
+            M3<P> t2;
+            if constexpr (A == 0 || A == 2) {
+                M3<P> base;
+                if (b == 0 || b == 2) base = (gs.N7l2 * Re) * A1F - RG1 * SFP1;
+                else base = -(RG1 * SFPb);
+                const bool neg = (b == 2) != (A == 2);
+                t2 = neg ? -base : base;
+            } else {
+                const M3<P> base = -(RGa * SFP1);
+                if (b == 0) t2 = base;
+                else if (b == 2) t2 = -base;
+                else t2 = -(RGa * SFPb);
+            }
+            const M3<P> t4 = tRHR * Kfc;
+            Kc[b] += w * (((t1 + t2) + t3) + t4);
+            Cc[b] += (w * tRHR) * Cfc;
+        }
+    }
+}
+
+// the end of block row A: l₀/2, Tₛ⁻¹(Θ)ᵀ on the C rows of the rotational dofs (:411-430), folded into
+// the staging (K -= Kᶜ, C -= (1+α)Cᶜ); with FORCE Tᶜ of slot A and (row 0) the contact energy
+template <typename P, typename C, bool FORCE, int A>
+EBD void contact_row_store(const ContactBeam<P>& cb, const BeamStage<C>& st, const ContactArgs<C>& ca, bool first,
+                           const M3<P> (&Kc)[4], const M3<P> (&Cc)[4], const V3<P>& Tca, double energy, const M3<P>& TTt) {
+    const int64_t e = cb.e;
+    const P l02 = P(cb.l0 / 2);
+    if constexpr (FORCE) {
+        if (A == 0) ca.energy[3 * e + 2] = (first ? 0.0 : ca.energy[3 * e + 2]) + double(l02) * energy;
+        V3<P> t = l02 * Tca;
+        double* tc = ca.Tc + e * 12 + 3 * A;
+        tc[0] = (first ? 0.0 : tc[0]) + double(t.x);
+        tc[1] = (first ? 0.0 : tc[1]) + double(t.y);
+        tc[2] = (first ? 0.0 : tc[2]) + double(t.z);
+    }
+    const P c1a = P(1 + ca.alpha);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        M3<P> K = l02 * Kc[b];
+        M3<P> Cm = l02 * Cc[b];
+        if constexpr (A == 1 || A == 3) Cm = Cm * TTt;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int64_t idx = stage_idx(e, 3 * A + i, 3 * b + j);
+                st.K[idx] += C(-K.a[i + 3 * j]);
+                st.C[idx] += C(-(c1a * Cm.a[i + 3 * j]));
+            }
+    }
+}
+
+// One 64-beam tile of the contact list, per chunk of CONTACT_GP_CHUNK Gauss points (one chunk for the
+// reference's three-point rule): phase 1 (wavefront g: point g0+g; the wavefront left over: the
+// Tₛ⁻¹ blocks of the C rows 2 and 4), phase 2 (wavefront a: block row a), each chunk's sums folded
+// into the staging. FP64 form (P = double): the contact force and energy too; precision form
+// (P = float): the tangent alone (contact_force_kernel computes the FP64 force). With more than one
+// chunk the sums over the points are split at chunk boundaries (l₀/2 and Tₛ⁻¹ applied per chunk).
 template <typename P>
 __global__ void __launch_bounds__(4 * CONTACT_TILE) contact_kernel(NodeState ns, BeamConsts bc, BeamStage<P> st,
                                                                    GaussPts gp, ContactArgs<P> ca,
                                                                    const uint64_t* __restrict__ list,
                                                                    const int32_t* __restrict__ count) {
-    const int part = __builtin_amdgcn_readfirstlane(threadIdx.x / CONTACT_TILE);
-    const int64_t li = (int64_t)blockIdx.x * CONTACT_TILE + threadIdx.x % CONTACT_TILE;
+    constexpr bool FORCE = sizeof(P) == sizeof(double);
+    __shared__ ContactShared<P, FORCE> sh;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / CONTACT_TILE);
+    const int lane = threadIdx.x % CONTACT_TILE;
+    const int64_t li = (int64_t)blockIdx.x * CONTACT_TILE + lane;
     const int32_t nlisted = *count;
-    if (part == 0 && ca.stats) {
+    if ((int64_t)blockIdx.x * CONTACT_TILE >= nlisted) return;   // uniform over the workgroup
+    if (wv == 0 && ca.stats) {
         // the work of this launch: listed beams and Gauss points in contact (one wavefront's sums)
         int nb = li < nlisted ? 1 : 0;
         int ng = li < nlisted ? __popc((unsigned)(list[li] & 0xffu)) : 0;
@@ -500,14 +813,80 @@ __global__ void __launch_bounds__(4 * CONTACT_TILE) contact_kernel(NodeState ns,
             atomicAdd(&ca.stats[1], (unsigned long long)ng);
         }
     }
-    if (li >= nlisted) return;
-    const uint64_t ent = list[li];
-    const int64_t e = (int64_t)(ent >> 8);
-    const unsigned mask = (unsigned)(ent & 0xffu);
-    if (part == 0) contact_part<P, 0>(e, mask, ns, bc, st, gp, ca);
-    else if (part == 1) contact_part<P, 1>(e, mask, ns, bc, st, gp, ca);
-    else if (part == 2) contact_part<P, 2>(e, mask, ns, bc, st, gp, ca);
-    else contact_part<P, 3>(e, mask, ns, bc, st, gp, ca);
+    const bool valid = li < nlisted;
+    const uint64_t ent = valid ? list[li] : 0;
+    const ContactBeam<P> cb = contact_beam<P>((int64_t)(ent >> 8), (unsigned)(ent & 0xffu), ns, bc, ca);
+    const P gb = P(cb.rb) / P(4);
+    P eta_chunk = P(ca.eta_n);   // ηₙ after the contacting points of the earlier chunks
+#pragma unroll 1
+    for (int g0 = 0; g0 < gp.n; g0 += CONTACT_GP_CHUNK) {
+        const int ig = g0 + wv;
+        const bool mine = wv < CONTACT_GP_CHUNK && valid && ig < gp.n && (cb.mask & (1u << ig));
+        // phase 1a: the point's kinematics and gap
+        P gn = P(0);
+        V3<P> n, h1;
+        M3<P> Hg, H1F1;
+        if (mine) {
+            const GaussShape<P> gs = gauss_shape(cb, shape_fn(cb.l0, gp.z[ig]));
+            const ShapeF sd = shape_fn(cb.l0, gp.z[ig]);
+            V3<double> utd = {0.0, sd.N3 * cb.Tb1.z + sd.N4 * cb.Tb2.z, (-sd.N3) * cb.Tb1.y + (-sd.N4) * cb.Tb2.y};
+            V3<double> xG = (sd.N1 * cb.x1 + sd.N2 * cb.x2) + cb.Red * utd;
+            const M3<P> H10 = gauss_H1<P, 0>(cb, gs), H11 = gauss_H1<P, 1>(cb, gs);
+            const M3<P> H12 = gauss_H1<P, 2>(cb, gs), H13 = gauss_H1<P, 3>(cb, gs);
+            const V3<P> Ud1 = tmul(cb.Re, cvt<P>(ldv(ns.ud, cb.na))), Wd1 = tmul(cb.Re, cvt<P>(ldv(ns.wd, cb.na)));
+            const V3<P> Ud2 = tmul(cb.Re, cvt<P>(ldv(ns.ud, cb.nb))), Wd2 = tmul(cb.Re, cvt<P>(ldv(ns.wd, cb.nb)));
+            h1 = ((H10 * Ud1 + H11 * Wd1) + H12 * Ud2) + H13 * Wd2;
+            H1F1 = ((H10 * skew(Ud1) + H11 * skew(Wd1)) + H12 * skew(Ud2)) + H13 * skew(Wd2);
+            contact_gap<P, P>(ca, cb.inst, xG, cb.rb, gn, n, Hg);
+            sh.gn[wv][lane] = gn;
+        } else if (wv == CONTACT_GP_CHUNK && valid && g0 == 0) {
+            const M3<P> T1 = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, cb.na)))));
+            const M3<P> T2 = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, cb.nb)))));
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                sh.tt[0][k][lane] = T1.a[k];
+                sh.tt[1][k][lane] = T2.a[k];
+            }
+        }
+        __syncthreads();
+        // phase 1b: ηₙ after the earlier contacting points, then the point's force derivatives
+        if (mine) {
+            P eta_n = eta_chunk;
+            for (int j = g0; j < ig; ++j)
+                if (cb.mask & (1u << j)) eta_n = eta_after(eta_n, sh.gn[j - g0][lane], gb);
+            contact_point<P, P, FORCE>(cb, ns, gp, ca, ig, wv, lane, eta_n, gn, n, Hg, h1, H1F1, sh);
+        }
+        __syncthreads();
+        // phase 2: block row wv over the chunk's points, folded into the staging
+        if (valid) {
+            M3<P> Kc[4], Cc[4];
+            for (int b = 0; b < 4; ++b) { Kc[b] = zm<P>(); Cc[b] = zm<P>(); }
+            V3<P> Tca = zv<P>();
+            double energy = 0;
+            const bool first = g0 == 0;
+            if (wv == 0) {
+                contact_row<P, FORCE, 0>(cb, gp, g0, lane, sh, Kc, Cc, Tca, energy);
+                contact_row_store<P, P, FORCE, 0>(cb, st, ca, first, Kc, Cc, Tca, energy, eye<P>());
+            } else if (wv == 1) {
+                contact_row<P, FORCE, 1>(cb, gp, g0, lane, sh, Kc, Cc, Tca, energy);
+                M3<P> TTt;
+                for (int k = 0; k < 9; ++k) TTt.a[k] = sh.tt[0][k][lane];
+                contact_row_store<P, P, FORCE, 1>(cb, st, ca, first, Kc, Cc, Tca, energy, TTt);
+            } else if (wv == 2) {
+                contact_row<P, FORCE, 2>(cb, gp, g0, lane, sh, Kc, Cc, Tca, energy);
+                contact_row_store<P, P, FORCE, 2>(cb, st, ca, first, Kc, Cc, Tca, energy, eye<P>());
+            } else {
+                contact_row<P, FORCE, 3>(cb, gp, g0, lane, sh, Kc, Cc, Tca, energy);
+                M3<P> TTt;
+                for (int k = 0; k < 9; ++k) TTt.a[k] = sh.tt[1][k][lane];
+                contact_row_store<P, P, FORCE, 3>(cb, st, ca, first, Kc, Cc, Tca, energy, TTt);
+            }
+            const int gend = min(gp.n, g0 + CONTACT_GP_CHUNK);
+            for (int j = g0; j < gend; ++j)
+                if (cb.mask & (1u << j)) eta_chunk = eta_after(eta_chunk, sh.gn[j - g0][lane], gb);
+        }
+        __syncthreads();
+    }
 }
 
 }  // namespace ebd
