@@ -1382,7 +1382,8 @@ namespace {
 bool desc_ok(const eb_model_desc* d) {
     if (d->abi_version != EB_ABI_VERSION) return false;
     if (d->n_nodes <= 0 || d->n_beams <= 0 || !d->X0 || !d->conn || !d->l0 || !d->Re0 || !d->props) return false;
-    if (d->n_gauss <= 0 || d->n_gauss > 8 || !d->gauss_w || !d->gauss_z) return false;
+    // the reference's Gauss rule has at most three points (ωᴳ_beams is a Vec3)
+    if (d->n_gauss <= 0 || d->n_gauss > 3 || !d->gauss_w || !d->gauss_z) return false;
     if (d->precision != EB_PRECISION_FP64 && d->precision != EB_PRECISION_FP32) return false;
     if (d->n_fixed < 0 || (d->n_fixed > 0 && !d->fixed_dofs)) return false;
     return true;

@@ -472,7 +472,7 @@ __global__ void __launch_bounds__(256) contact_force_kernel(NodeState ns, BeamCo
 // points in their order. Every expression is the one of contact_column, evaluated once instead of
 // once per block column.
 constexpr int CONTACT_TILE = 64;
-constexpr int CONTACT_GP_CHUNK = 3;   // Gauss points per phase-1 pass (wavefronts 0..2)
+constexpr int CONTACT_GP_CHUNK = 3;   // Gauss points per beam at most (phase-1 wavefronts 0..2)
 
 template <typename P, bool FORCE> struct ContactShared {
     P kfc[CONTACT_GP_CHUNK][36][CONTACT_TILE];   // 𝒦ᶠᶜᵇ, b-major, each 3x3 column-major
@@ -749,51 +749,98 @@ EBD void contact_row(const ContactBeam<P>& cb, const GaussPts& gp, int g0, int l
     }
 }
 
-// the end of block row A: l₀/2, Tₛ⁻¹(Θ)ᵀ on the C rows of the rotational dofs (:411-430), folded into
-// the staging (K -= Kᶜ, C -= (1+α)Cᶜ); with FORCE Tᶜ of slot A and (row 0) the contact energy
-template <typename P, typename C, bool FORCE, int A>
-EBD void contact_row_store(const ContactBeam<P>& cb, const BeamStage<C>& st, const ContactArgs<C>& ca, bool first,
-                           const M3<P> (&Kc)[4], const M3<P> (&Cc)[4], const V3<P>& Tca, double energy, const M3<P>& TTt) {
+// the end of block row A (the wavefront's row, runtime but uniform): l₀/2 and, on the C rows of the
+// rotational dofs, Tₛ⁻¹(Θ)ᵀ (:411-430); Kc, Cc become the values the staging receives (K -= Kᶜ,
+// C -= (1+α)Cᶜ, assemble_contact!). With FORCE also Tᶜ of slot A and (row 0) the contact energy.
+template <typename P, bool FORCE>
+EBD void contact_row_finish(const ContactBeam<P>& cb, const ContactArgs<P>& ca, int A, M3<P> (&Kc)[4],
+                            M3<P> (&Cc)[4], const V3<P>& Tca, double energy, const M3<P>& TTt) {
     const int64_t e = cb.e;
     const P l02 = P(cb.l0 / 2);
     if constexpr (FORCE) {
-        if (A == 0) ca.energy[3 * e + 2] = (first ? 0.0 : ca.energy[3 * e + 2]) + double(l02) * energy;
+        if (A == 0) ca.energy[3 * e + 2] = double(l02) * energy;
         V3<P> t = l02 * Tca;
-        double* tc = ca.Tc + e * 12 + 3 * A;
-        tc[0] = (first ? 0.0 : tc[0]) + double(t.x);
-        tc[1] = (first ? 0.0 : tc[1]) + double(t.y);
-        tc[2] = (first ? 0.0 : tc[2]) + double(t.z);
+        ca.Tc[e * 12 + 3 * A] = double(t.x);
+        ca.Tc[e * 12 + 3 * A + 1] = double(t.y);
+        ca.Tc[e * 12 + 3 * A + 2] = double(t.z);
     }
     const P c1a = P(1 + ca.alpha);
+    const bool rot = A == 1 || A == 3;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-        M3<P> K = l02 * Kc[b];
         M3<P> Cm = l02 * Cc[b];
-        if constexpr (A == 1 || A == 3) Cm = Cm * TTt;
+        if (rot) Cm = Cm * TTt;
+        Kc[b] = -(l02 * Kc[b]);
+        Cc[b] = -(c1a * Cm);
+    }
+}
+
+// A tile's 12x12 contributions pass through LDS, one record of 144 entries per beam in the staging's
+// order (stride 148: 16-byte aligned rows), so that the read-modify-write of the staging moves whole
+// 16-byte pieces of one contiguous record per beam instead of 3x3 blocks scattered over 64 beams.
+constexpr int CONTACT_REC = 148;
+template <typename P> EBD void contact_row_to_lds(P* buf, int lane, int A, const M3<P> (&V)[4]) {
+    P* r = buf + lane * CONTACT_REC;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-                const int64_t idx = stage_idx(e, 3 * A + i, 3 * b + j);
-                st.K[idx] += C(-K.a[i + 3 * j]);
-                st.C[idx] += C(-(c1a * Cm.a[i + 3 * j]));
+                const int R = 3 * A + i, Cl = 3 * b + j;
+                r[(2 * (R / 6) + Cl / 6) * 36 + (R % 6) * 6 + Cl % 6] = V[b].a[i + 3 * j];
             }
+}
+// 16-byte pieces: 36 per FP32 record, 72 per FP64 record; thread t takes pieces t, t + 256, ...
+template <typename P> struct Piece { P v[16 / sizeof(P)]; };
+template <typename P> constexpr int pieces_per_rec() { return 144 * (int)sizeof(P) / 16; }
+template <typename P> constexpr int pieces_per_thread() { return CONTACT_TILE * pieces_per_rec<P>() / (4 * CONTACT_TILE); }
+template <typename P>
+EBD void contact_tile_load(const P* __restrict__ src, const int64_t* tile_e, int nvalid, Piece<P> (&v)[pieces_per_thread<P>()]) {
+    constexpr int PR = pieces_per_rec<P>(), NP = 16 / sizeof(P);
+    const int total = nvalid * PR;
+#pragma unroll
+    for (int u = 0; u < pieces_per_thread<P>(); ++u) {
+        const int q = threadIdx.x + u * 4 * CONTACT_TILE;
+        const int j = q / PR, k = q - PR * j;
+        if (q < total) v[u] = *reinterpret_cast<const Piece<P>*>(src + tile_e[j] * 144 + NP * k);
+    }
+}
+template <typename P>
+EBD void contact_tile_store(P* __restrict__ dst, const P* buf, const int64_t* tile_e, int nvalid,
+                            const Piece<P> (&v)[pieces_per_thread<P>()]) {
+    constexpr int PR = pieces_per_rec<P>(), NP = 16 / sizeof(P);
+    const int total = nvalid * PR;
+#pragma unroll
+    for (int u = 0; u < pieces_per_thread<P>(); ++u) {
+        const int q = threadIdx.x + u * 4 * CONTACT_TILE;
+        const int j = q / PR, k = q - PR * j;
+        if (q < total) {
+            Piece<P> w;
+#pragma unroll
+            for (int c = 0; c < NP; ++c) w.v[c] = v[u].v[c] + buf[j * CONTACT_REC + NP * k + c];
+            *reinterpret_cast<Piece<P>*>(dst + tile_e[j] * 144 + NP * k) = w;
+        }
     }
 }
 
-// One 64-beam tile of the contact list, per chunk of CONTACT_GP_CHUNK Gauss points (one chunk for the
-// reference's three-point rule): phase 1 (wavefront g: point g0+g; the wavefront left over: the
-// Tₛ⁻¹ blocks of the C rows 2 and 4), phase 2 (wavefront a: block row a), each chunk's sums folded
-// into the staging. FP64 form (P = double): the contact force and energy too; precision form
-// (P = float): the tangent alone (contact_force_kernel computes the FP64 force). With more than one
-// chunk the sums over the points are split at chunk boundaries (l₀/2 and Tₛ⁻¹ applied per chunk).
+// One 64-beam tile of the contact list (at most CONTACT_GP_CHUNK Gauss points per beam: the
+// reference's rule has three, ωᴳ_beams is a Vec3): phase 1 (wavefront g: point g; the wavefront left
+// over: the Tₛ⁻¹ blocks of the C rows 2 and 4), phase 2 (wavefront a: block row a), the sums folded
+// into the staging through LDS. FP64 form (P = double): the contact force and energy too; precision
+// form (P = float): the tangent alone (contact_force_kernel computes the FP64 force).
 template <typename P>
 __global__ void __launch_bounds__(4 * CONTACT_TILE) contact_kernel(NodeState ns, BeamConsts bc, BeamStage<P> st,
                                                                    GaussPts gp, ContactArgs<P> ca,
                                                                    const uint64_t* __restrict__ list,
                                                                    const int32_t* __restrict__ count) {
     constexpr bool FORCE = sizeof(P) == sizeof(double);
-    __shared__ ContactShared<P, FORCE> sh;
+    __shared__ union {
+        ContactShared<P, FORCE> pts;
+        struct { P K[CONTACT_TILE * CONTACT_REC], C[CONTACT_TILE * CONTACT_REC]; } rec;
+    } shu;
+    __shared__ int64_t tile_e[CONTACT_TILE];
+    ContactShared<P, FORCE>& sh = shu.pts;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / CONTACT_TILE);
     const int lane = threadIdx.x % CONTACT_TILE;
     const int64_t li = (int64_t)blockIdx.x * CONTACT_TILE + lane;
@@ -816,11 +863,11 @@ __global__ void __launch_bounds__(4 * CONTACT_TILE) contact_kernel(NodeState ns,
     const bool valid = li < nlisted;
     const uint64_t ent = valid ? list[li] : 0;
     const ContactBeam<P> cb = contact_beam<P>((int64_t)(ent >> 8), (unsigned)(ent & 0xffu), ns, bc, ca);
+    const int nvalid = (int)min<int64_t>(CONTACT_TILE, nlisted - (int64_t)blockIdx.x * CONTACT_TILE);
+    if (wv == 0) tile_e[lane] = cb.e;
     const P gb = P(cb.rb) / P(4);
-    P eta_chunk = P(ca.eta_n);   // ηₙ after the contacting points of the earlier chunks
-#pragma unroll 1
-    for (int g0 = 0; g0 < gp.n; g0 += CONTACT_GP_CHUNK) {
-        const int ig = g0 + wv;
+    {
+        const int g0 = 0, ig = wv;
         const bool mine = wv < CONTACT_GP_CHUNK && valid && ig < gp.n && (cb.mask & (1u << ig));
         // phase 1a: the point's kinematics and gap
         P gn = P(0);
@@ -839,7 +886,7 @@ __global__ void __launch_bounds__(4 * CONTACT_TILE) contact_kernel(NodeState ns,
             H1F1 = ((H10 * skew(Ud1) + H11 * skew(Wd1)) + H12 * skew(Ud2)) + H13 * skew(Wd2);
             contact_gap<P, P>(ca, cb.inst, xG, cb.rb, gn, n, Hg);
             sh.gn[wv][lane] = gn;
-        } else if (wv == CONTACT_GP_CHUNK && valid && g0 == 0) {
+        } else if (wv == CONTACT_GP_CHUNK && valid) {
             const M3<P> T1 = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, cb.na)))));
             const M3<P> T2 = cvt<P>(tr(Ts_inv(toangle(ldm(ns.dR, cb.nb)))));
 #pragma unroll
@@ -851,40 +898,39 @@ __global__ void __launch_bounds__(4 * CONTACT_TILE) contact_kernel(NodeState ns,
         __syncthreads();
         // phase 1b: ηₙ after the earlier contacting points, then the point's force derivatives
         if (mine) {
-            P eta_n = eta_chunk;
+            P eta_n = P(ca.eta_n);
             for (int j = g0; j < ig; ++j)
                 if (cb.mask & (1u << j)) eta_n = eta_after(eta_n, sh.gn[j - g0][lane], gb);
             contact_point<P, P, FORCE>(cb, ns, gp, ca, ig, wv, lane, eta_n, gn, n, Hg, h1, H1F1, sh);
         }
         __syncthreads();
         // phase 2: block row wv over the chunk's points, folded into the staging
+        M3<P> Kc[4], Cc[4];
         if (valid) {
-            M3<P> Kc[4], Cc[4];
             for (int b = 0; b < 4; ++b) { Kc[b] = zm<P>(); Cc[b] = zm<P>(); }
             V3<P> Tca = zv<P>();
             double energy = 0;
-            const bool first = g0 == 0;
-            if (wv == 0) {
-                contact_row<P, FORCE, 0>(cb, gp, g0, lane, sh, Kc, Cc, Tca, energy);
-                contact_row_store<P, P, FORCE, 0>(cb, st, ca, first, Kc, Cc, Tca, energy, eye<P>());
-            } else if (wv == 1) {
-                contact_row<P, FORCE, 1>(cb, gp, g0, lane, sh, Kc, Cc, Tca, energy);
-                M3<P> TTt;
-                for (int k = 0; k < 9; ++k) TTt.a[k] = sh.tt[0][k][lane];
-                contact_row_store<P, P, FORCE, 1>(cb, st, ca, first, Kc, Cc, Tca, energy, TTt);
-            } else if (wv == 2) {
-                contact_row<P, FORCE, 2>(cb, gp, g0, lane, sh, Kc, Cc, Tca, energy);
-                contact_row_store<P, P, FORCE, 2>(cb, st, ca, first, Kc, Cc, Tca, energy, eye<P>());
-            } else {
-                contact_row<P, FORCE, 3>(cb, gp, g0, lane, sh, Kc, Cc, Tca, energy);
-                M3<P> TTt;
-                for (int k = 0; k < 9; ++k) TTt.a[k] = sh.tt[1][k][lane];
-                contact_row_store<P, P, FORCE, 3>(cb, st, ca, first, Kc, Cc, Tca, energy, TTt);
-            }
-            const int gend = min(gp.n, g0 + CONTACT_GP_CHUNK);
-            for (int j = g0; j < gend; ++j)
-                if (cb.mask & (1u << j)) eta_chunk = eta_after(eta_chunk, sh.gn[j - g0][lane], gb);
+            if (wv == 0) contact_row<P, FORCE, 0>(cb, gp, g0, lane, sh, Kc, Cc, Tca, energy);
+            else if (wv == 1) contact_row<P, FORCE, 1>(cb, gp, g0, lane, sh, Kc, Cc, Tca, energy);
+            else if (wv == 2) contact_row<P, FORCE, 2>(cb, gp, g0, lane, sh, Kc, Cc, Tca, energy);
+            else contact_row<P, FORCE, 3>(cb, gp, g0, lane, sh, Kc, Cc, Tca, energy);
+            M3<P> TTt = eye<P>();
+            if (wv == 1 || wv == 3)
+                for (int k = 0; k < 9; ++k) TTt.a[k] = sh.tt[wv == 1 ? 0 : 1][k][lane];
+            contact_row_finish<P, FORCE>(cb, ca, wv, Kc, Cc, Tca, energy, TTt);
         }
+        // the staging records the tile adds to, all in flight at once
+        Piece<P> oldK[pieces_per_thread<P>()], oldC[pieces_per_thread<P>()];
+        contact_tile_load(st.K, tile_e, nvalid, oldK);
+        contact_tile_load(st.C, tile_e, nvalid, oldC);
+        __syncthreads();   // the points' records are read: LDS takes the tile's K and C contributions
+        if (valid) {
+            contact_row_to_lds(shu.rec.K, lane, wv, Kc);
+            contact_row_to_lds(shu.rec.C, lane, wv, Cc);
+        }
+        __syncthreads();
+        contact_tile_store(st.K, shu.rec.K, tile_e, nvalid, oldK);
+        contact_tile_store(st.C, shu.rec.C, tile_e, nvalid, oldC);
         __syncthreads();
     }
 }

@@ -61,7 +61,7 @@ typedef struct eb_model_desc {
     const double* props;          /* EB_NPROPS*n_beams, beam-major */
     int64_t n_fixed;              /* bcs.fixed_dofs (1-based, strictly increasing) */
     const int64_t* fixed_dofs;
-    int32_t n_gauss;              /* params.nᴳ_beams (<= 8) */
+    int32_t n_gauss;              /* params.nᴳ_beams, 1..3 (ωᴳ_beams is a Vec3, src/def_params.jl:31-33) */
     int32_t pad0;
     const double* gauss_w;        /* params.ωᴳ_beams */
     const double* gauss_z;        /* params.zᴳ_beams */
