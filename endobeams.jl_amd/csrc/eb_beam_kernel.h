@@ -478,10 +478,10 @@ constexpr int BEAM_TILE = 64;
 // Tⁱⁿᵗ compute in FP64, and the mass part too, because it yields the inertia forces Tᵏ: both enter
 // the FP64 residual.
 template <typename S, typename Hook>
-EBD void beam_kernel_body(NodeState ns, BeamConsts bc, BeamStage<S> st, GaussPts gp, const uint8_t* __restrict__ active,
-                          const int32_t* __restrict__ beam_inst, const Hook& hook) {
+EBD void beam_kernel_body(int64_t tile, NodeState ns, BeamConsts bc, BeamStage<S> st, GaussPts gp,
+                          const uint8_t* __restrict__ active, const int32_t* __restrict__ beam_inst, const Hook& hook) {
     const int part = __builtin_amdgcn_readfirstlane(threadIdx.x / BEAM_TILE);
-    const int64_t e = (int64_t)blockIdx.x * BEAM_TILE + threadIdx.x % BEAM_TILE;
+    const int64_t e = tile * BEAM_TILE + threadIdx.x % BEAM_TILE;
     // elements of instances frozen in the Newton iteration are skipped
     const bool valid = e < bc.E && active[beam_inst[e]];
     // FP64 form: the mass part computes M with Tᵏ; precision form: Tᵏ alone in FP64 (it enters the
@@ -526,7 +526,7 @@ template <typename S>
 __global__ void __launch_bounds__(4 * BEAM_TILE) beam_kernel(NodeState ns, BeamConsts bc, BeamStage<S> st, GaussPts gp,
                                                               const uint8_t* __restrict__ active,
                                                               const int32_t* __restrict__ beam_inst) {
-    beam_kernel_body<S>(ns, bc, st, gp, active, beam_inst, NoStaticHook{});
+    beam_kernel_body<S>(blockIdx.x, ns, bc, st, gp, active, beam_inst, NoStaticHook{});
 }
 
 }  // namespace ebd

@@ -445,20 +445,24 @@ __global__ void __launch_bounds__(4 * BEAM_TILE) beam_detect_kernel(NodeState ns
                                                                      const int32_t* __restrict__ beam_inst,
                                                                      ContactArgs<S> ca, const uint8_t* is_contact,
                                                                      uint64_t* list, int32_t* count) {
-    beam_kernel_body<S>(ns, bc, st, gp, active, beam_inst, DetectHook<S>{ca, is_contact, list, count});
+    beam_kernel_body<S>(blockIdx.x, ns, bc, st, gp, active, beam_inst, DetectHook<S>{ca, is_contact, list, count});
 }
 
 // Precision form: the FP64 contact force (Tᶜ of all four node slots) and energy of every listed
 // beam, one lane per beam, with the arithmetic of the FP64 pass of the tangent kernel.
 template <typename P>
-__global__ void __launch_bounds__(256) contact_force_kernel(NodeState ns, BeamConsts bc, BeamStage<P> st, GaussPts gp,
-                                                            ContactArgs<P> ca, const uint64_t* __restrict__ list,
-                                                            const int32_t* __restrict__ count) {
-    const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (li >= *count) return;
+EBD void contact_force_item(int64_t li, int32_t nlisted, const NodeState& ns, const BeamConsts& bc, const BeamStage<P>& st,
+                            const GaussPts& gp, const ContactArgs<P>& ca, const uint64_t* __restrict__ list) {
+    if (li >= nlisted) return;
     const uint64_t ent = list[li];
     contact_column<double, P, CONTACT_ALL_SLOTS, CONTACT_FORCE>((int64_t)(ent >> 8), (unsigned)(ent & 0xffu), ns, bc, st,
                                                                 gp, ca);
+}
+template <typename P>
+__global__ void __launch_bounds__(256) contact_force_kernel(NodeState ns, BeamConsts bc, BeamStage<P> st, GaussPts gp,
+                                                            ContactArgs<P> ca, const uint64_t* __restrict__ list,
+                                                            const int32_t* __restrict__ count) {
+    contact_force_item<P>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, *count, ns, bc, st, gp, ca, list);
 }
 
 // ---- the contact tangent in two phases --------------------------------------------------------
@@ -824,28 +828,36 @@ EBD void contact_tile_store(P* __restrict__ dst, const P* buf, const int64_t* ti
     }
 }
 
+// The tile's LDS: the points' records, then the tile's contributions to the staging (FP32: K and C
+// at once; FP64: K, then C, in one record buffer, so that the FP64 tile stays within 146 KB)
+template <typename P> struct ContactLDS {
+    static constexpr bool FORCE = sizeof(P) == sizeof(double);
+    static constexpr int NREC = FORCE ? 1 : 2;
+    union {
+        ContactShared<P, FORCE> pts;
+        P rec[NREC][CONTACT_TILE * CONTACT_REC];
+    } u;
+    int64_t tile_e[CONTACT_TILE];
+};
+
 // One 64-beam tile of the contact list (at most CONTACT_GP_CHUNK Gauss points per beam: the
 // reference's rule has three, ωᴳ_beams is a Vec3): phase 1 (wavefront g: point g; the wavefront left
 // over: the Tₛ⁻¹ blocks of the C rows 2 and 4), phase 2 (wavefront a: block row a), the sums folded
 // into the staging through LDS. FP64 form (P = double): the contact force and energy too; precision
-// form (P = float): the tangent alone (contact_force_kernel computes the FP64 force).
+// form (P = float): the tangent alone (contact_force_kernel computes the FP64 force). Every thread
+// of a 256-thread workgroup calls it for the same tile.
 template <typename P>
-__global__ void __launch_bounds__(4 * CONTACT_TILE) contact_kernel(NodeState ns, BeamConsts bc, BeamStage<P> st,
-                                                                   GaussPts gp, ContactArgs<P> ca,
-                                                                   const uint64_t* __restrict__ list,
-                                                                   const int32_t* __restrict__ count) {
+EBD void contact_tile(int64_t tile, int32_t nlisted, const NodeState& ns, const BeamConsts& bc, const BeamStage<P>& st,
+                      const GaussPts& gp, const ContactArgs<P>& ca, const uint64_t* __restrict__ list,
+                      ContactLDS<P>& lds) {
     constexpr bool FORCE = sizeof(P) == sizeof(double);
-    __shared__ union {
-        ContactShared<P, FORCE> pts;
-        struct { P K[CONTACT_TILE * CONTACT_REC], C[CONTACT_TILE * CONTACT_REC]; } rec;
-    } shu;
-    __shared__ int64_t tile_e[CONTACT_TILE];
+    auto& shu = lds.u;
+    int64_t* tile_e = lds.tile_e;
     ContactShared<P, FORCE>& sh = shu.pts;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x / CONTACT_TILE);
     const int lane = threadIdx.x % CONTACT_TILE;
-    const int64_t li = (int64_t)blockIdx.x * CONTACT_TILE + lane;
-    const int32_t nlisted = *count;
-    if ((int64_t)blockIdx.x * CONTACT_TILE >= nlisted) return;   // uniform over the workgroup
+    const int64_t li = tile * CONTACT_TILE + lane;
+    if (tile * CONTACT_TILE >= nlisted) return;   // uniform over the workgroup
     if (wv == 0 && ca.stats) {
         // the work of this launch: listed beams and Gauss points in contact (one wavefront's sums)
         int nb = li < nlisted ? 1 : 0;
@@ -863,7 +875,7 @@ __global__ void __launch_bounds__(4 * CONTACT_TILE) contact_kernel(NodeState ns,
     const bool valid = li < nlisted;
     const uint64_t ent = valid ? list[li] : 0;
     const ContactBeam<P> cb = contact_beam<P>((int64_t)(ent >> 8), (unsigned)(ent & 0xffu), ns, bc, ca);
-    const int nvalid = (int)min<int64_t>(CONTACT_TILE, nlisted - (int64_t)blockIdx.x * CONTACT_TILE);
+    const int nvalid = (int)min<int64_t>(CONTACT_TILE, nlisted - tile * CONTACT_TILE);
     if (wv == 0) tile_e[lane] = cb.e;
     const P gb = P(cb.rb) / P(4);
     {
@@ -924,15 +936,30 @@ __global__ void __launch_bounds__(4 * CONTACT_TILE) contact_kernel(NodeState ns,
         contact_tile_load(st.K, tile_e, nvalid, oldK);
         contact_tile_load(st.C, tile_e, nvalid, oldC);
         __syncthreads();   // the points' records are read: LDS takes the tile's K and C contributions
+        constexpr int KR = 0, CR = ContactLDS<P>::NREC - 1;
         if (valid) {
-            contact_row_to_lds(shu.rec.K, lane, wv, Kc);
-            contact_row_to_lds(shu.rec.C, lane, wv, Cc);
+            contact_row_to_lds(shu.rec[KR], lane, wv, Kc);
+            if (CR != KR) contact_row_to_lds(shu.rec[CR], lane, wv, Cc);
         }
         __syncthreads();
-        contact_tile_store(st.K, shu.rec.K, tile_e, nvalid, oldK);
-        contact_tile_store(st.C, shu.rec.C, tile_e, nvalid, oldC);
+        contact_tile_store(st.K, shu.rec[KR], tile_e, nvalid, oldK);
+        if (CR == KR) {
+            __syncthreads();
+            if (valid) contact_row_to_lds(shu.rec[CR], lane, wv, Cc);
+            __syncthreads();
+        }
+        contact_tile_store(st.C, shu.rec[CR], tile_e, nvalid, oldC);
         __syncthreads();
     }
+}
+
+template <typename P>
+__global__ void __launch_bounds__(4 * CONTACT_TILE) contact_kernel(NodeState ns, BeamConsts bc, BeamStage<P> st,
+                                                                   GaussPts gp, ContactArgs<P> ca,
+                                                                   const uint64_t* __restrict__ list,
+                                                                   const int32_t* __restrict__ count) {
+    __shared__ ContactLDS<P> lds;
+    contact_tile<P>(blockIdx.x, *count, ns, bc, st, gp, ca, list, lds);
 }
 
 }  // namespace ebd

@@ -25,9 +25,15 @@ EBD void st6(double* p, const double* v) {
     for (int k = 0; k < 3; ++k) q[k] = make_double2(v[2 * k], v[2 * k + 1]);
 }
 
+// Every kernel below is a device body taking the index of the block it works on (VB: blockIdx.x,
+// gridDim.x, blockIdx.y) and a __global__ wrapper passing the launch's own; a kernel that runs
+// several of these phases in one workgroup calls the bodies with the block indices it iterates.
+struct VB { int64_t x, n, y; };
+#define EB_VB (VB{(int64_t)blockIdx.x, (int64_t)gridDim.x, (int64_t)blockIdx.y})
+
 // ---- deterministic reductions ---------------------------------------------------------------
 
-template <int NQ> EBD void block_sum_store(double (&v)[NQ], double* partials, int grid) {
+template <int NQ> EBD void block_sum_store(double (&v)[NQ], double* partials, int grid, int64_t bid) {
     __shared__ double sh[NQ][RED_BLOCK / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -43,7 +49,7 @@ template <int NQ> EBD void block_sum_store(double (&v)[NQ], double* partials, in
         for (int q = 0; q < NQ; ++q) {
             double s = 0;
             for (int w = 0; w < RED_BLOCK / 64; ++w) s += sh[q][w];
-            partials[q * RED_MAX_GRID + blockIdx.x] = s;
+            partials[q * RED_MAX_GRID + bid] = s;
         }
     }
 }
@@ -91,7 +97,7 @@ struct Segs {
     int32_t nchunks, ninst;
 };
 
-template <int NQ> EBD void seg_block_store(double (&v)[NQ], double* partials, int nchunks) {
+template <int NQ> EBD void seg_block_store(double (&v)[NQ], double* partials, int nchunks, int64_t bid) {
     __shared__ double sh[NQ][RED_BLOCK / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -107,7 +113,7 @@ template <int NQ> EBD void seg_block_store(double (&v)[NQ], double* partials, in
         for (int q = 0; q < NQ; ++q) {
             double t = 0;
             for (int w = 0; w < RED_BLOCK / 64; ++w) t += sh[q][w];
-            partials[(int64_t)q * nchunks + blockIdx.x] = t;
+            partials[(int64_t)q * nchunks + bid] = t;
         }
     }
 }
@@ -166,13 +172,13 @@ EBD void st4(double* p, const double (&v)[4]) {
 // four consecutive entries of a 6×6 block (nnz/4 lanes), each entry summed over the block's
 // contributions in beam order
 template <typename P>
-__global__ void __launch_bounds__(256) gather_matrices(int64_t nnz, const int32_t* __restrict__ blk_ptr,
+EBD void gather_matrices_b(int64_t nnz, const int32_t* __restrict__ blk_ptr,
                                                        const uint32_t* __restrict__ blk_src, const P* __restrict__ sK,
                                                        const P* __restrict__ sC, const P* __restrict__ sM, double* K,
                                                        double* C, double* M, double* Ktan, double c1,
                                                        const StepCoef* __restrict__ coef, int with_tangent, int write_kcm,
-                                                       Active act) {
-    const int64_t q = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+                                                       Active act, const VB& vb) {
+    const int64_t q = 4 * ((int64_t)vb.x * blockDim.x + threadIdx.x);
     if (q >= nnz) return;
     const int64_t b = q / 36;
     if (!act.flag[act.blk_inst[b]]) return;
@@ -208,16 +214,23 @@ __global__ void __launch_bounds__(256) gather_matrices(int64_t nnz, const int32_
         st4(Ktan + q, t);
     }
 }
+template <typename P>
+__global__ void __launch_bounds__(256) gather_matrices(int64_t nnz, const int32_t* __restrict__ blk_ptr,
+                                                       const uint32_t* __restrict__ blk_src, const P* __restrict__ sK,
+                                                       const P* __restrict__ sC, const P* __restrict__ sM, double* K,
+                                                       double* C, double* M, double* Ktan, double c1,
+                                                       const StepCoef* __restrict__ coef, int with_tangent, int write_kcm,
+                                                       Active act) { gather_matrices_b<P>(nnz, blk_ptr, blk_src, sK, sC, sM, K, C, M, Ktan, c1, coef, with_tangent, write_kcm, act, EB_VB); }
 
 // Tⁱⁿᵗ, Tᵏ, Tᶜ per dof, optionally followed by the corrector residual (utils_solver.jl:87)
 template <typename P>
-__global__ void __launch_bounds__(256) gather_vectors(int64_t ndofs, const int32_t* __restrict__ inc_ptr,
+EBD void gather_vectors_b(int64_t ndofs, const int32_t* __restrict__ inc_ptr,
                                                       const uint32_t* __restrict__ inc_src, const double* __restrict__ sTint,
                                                       const double* __restrict__ sTk, const double* __restrict__ sTc, int with_contact,
                                                       double* Tint, double* Tk, double* Tc, int with_residual, double alpha,
                                                       const double* fext, const double* fext_n, const double* Tc_n,
-                                                      const double* Tint_n, double* res, Active act) {
-    int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                      const double* Tint_n, double* res, Active act, const VB& vb) {
+    int64_t d = (int64_t)vb.x * blockDim.x + threadIdx.x;
     if (d >= ndofs) return;
     const int64_t n = d / 6;
     if (!act.flag[act.node_inst[n]]) return;
@@ -238,55 +251,72 @@ __global__ void __launch_bounds__(256) gather_vectors(int64_t ndofs, const int32
     if (with_residual)
         res[d] = ((1 + alpha) * ((fext[d] + tc) - ti) - alpha * ((fext_n[d] + Tc_n[d]) - Tint_n[d])) - tk;
 }
+template <typename P>
+__global__ void __launch_bounds__(256) gather_vectors(int64_t ndofs, const int32_t* __restrict__ inc_ptr,
+                                                      const uint32_t* __restrict__ inc_src, const double* __restrict__ sTint,
+                                                      const double* __restrict__ sTk, const double* __restrict__ sTc, int with_contact,
+                                                      double* Tint, double* Tk, double* Tc, int with_residual, double alpha,
+                                                      const double* fext, const double* fext_n, const double* Tc_n,
+                                                      const double* Tint_n, double* res, Active act) { gather_vectors_b<P>(ndofs, inc_ptr, inc_src, sTint, sTk, sTc, with_contact, Tint, Tk, Tc, with_residual, alpha, fext, fext_n, Tc_n, Tint_n, res, act, EB_VB); }
 
 // energies: Σ over beams of [E][3] staging, added to the running totals (never reset, as in
 // the reference's Energy accumulation, assemble_beams.jl:56-57)
-__global__ void __launch_bounds__(RED_BLOCK) energy_partials(int64_t E, const double* st, double* partials, int mask,
-                                                             Active act) {
+EBD void energy_partials_b(int64_t E, const double* st, double* partials, int mask,
+                                                             Active act, const VB& vb) {
     double v[3] = {0, 0, 0};
-    for (int64_t e = (int64_t)blockIdx.x * RED_BLOCK + threadIdx.x; e < E; e += (int64_t)gridDim.x * RED_BLOCK) {
+    for (int64_t e = (int64_t)vb.x * RED_BLOCK + threadIdx.x; e < E; e += (int64_t)vb.n * RED_BLOCK) {
         if (!act.flag[act.beam_inst[e]]) continue;
         if (mask & 1) { v[0] += st[3 * e]; v[1] += st[3 * e + 1]; }
         if (mask & 2) v[2] += st[3 * e + 2];
     }
-    block_sum_store<3>(v, partials, gridDim.x);
+    block_sum_store<3>(v, partials, vb.n, vb.x);
 }
-__global__ void __launch_bounds__(RED_BLOCK) energy_finalize(const double* partials, int grid, double* energy) {
+__global__ void __launch_bounds__(RED_BLOCK) energy_partials(int64_t E, const double* st, double* partials, int mask,
+                                                             Active act) { energy_partials_b(E, st, partials, mask, act, EB_VB); }
+EBD void energy_finalize_b(const double* partials, int grid, double* energy, const VB& vb) {
     double out[3];
     finalize_sum<3>(partials, grid, out);
     if (threadIdx.x == 0) { energy[0] += out[0]; energy[1] += out[1]; energy[2] += out[2]; }
 }
+__global__ void __launch_bounds__(RED_BLOCK) energy_finalize(const double* partials, int grid, double* energy) { energy_finalize_b(partials, grid, energy, EB_VB); }
 
 // ---- HHT combine and residuals (src/utils_solver.jl:54-89) ----------------------------------
 
-__global__ void __launch_bounds__(256) combine_tangent(int64_t nnz, const double* K, const double* C, const double* M,
+EBD void combine_tangent_b(int64_t nnz, const double* K, const double* C, const double* M,
                                                        double* Ktan, double c1, const StepCoef* __restrict__ coef,
-                                                       const int32_t* __restrict__ blk_inst) {
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                       const int32_t* __restrict__ blk_inst, const VB& vb) {
+    int64_t q = (int64_t)vb.x * blockDim.x + threadIdx.x;
     if (q >= nnz) return;
     const StepCoef& sc = coef[blk_inst[q / 36]];
     Ktan[q] = (c1 * K[q] + sc.gbdt * C[q]) + sc.inv * M[q];
 }
+__global__ void __launch_bounds__(256) combine_tangent(int64_t nnz, const double* K, const double* C, const double* M,
+                                                       double* Ktan, double c1, const StepCoef* __restrict__ coef,
+                                                       const int32_t* __restrict__ blk_inst) { combine_tangent_b(nnz, K, C, M, Ktan, c1, coef, blk_inst, EB_VB); }
 
-__global__ void __launch_bounds__(256) residual_corrector(int64_t ndofs, double alpha, const double* fext,
+EBD void residual_corrector_b(int64_t ndofs, double alpha, const double* fext,
                                                           const double* Tc, const double* Tint, const double* fext_n,
                                                           const double* Tc_n, const double* Tint_n, const double* Tk,
-                                                          double* res) {
-    int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                          double* res, const VB& vb) {
+    int64_t d = (int64_t)vb.x * blockDim.x + threadIdx.x;
     if (d < ndofs)
         res[d] = ((1 + alpha) * ((fext[d] + Tc[d]) - Tint[d]) - alpha * ((fext_n[d] + Tc_n[d]) - Tint_n[d])) - Tk[d];
 }
+__global__ void __launch_bounds__(256) residual_corrector(int64_t ndofs, double alpha, const double* fext,
+                                                          const double* Tc, const double* Tint, const double* fext_n,
+                                                          const double* Tc_n, const double* Tint_n, const double* Tk,
+                                                          double* res) { residual_corrector_b(ndofs, alpha, fext, Tc, Tint, fext_n, Tc_n, Tint_n, Tk, res, EB_VB); }
 
 // predictor residual: r = (1+α)f - Tⁱⁿᵗⁿ - Tᵏⁿ - αfⁿ + Cⁿ·t₁ + (1/(βΔt²))·Mⁿ·t₂, the two products
 // accumulated row-wise in increasing column order (what the CSC column sweep of mul! yields per row)
-__global__ void __launch_bounds__(256) residual_predictor(int64_t ndofs, const int32_t* __restrict__ bptr,
+EBD void residual_predictor_b(int64_t ndofs, const int32_t* __restrict__ bptr,
                                                           const int32_t* __restrict__ bcol, double alpha,
                                                           const StepCoef* __restrict__ coef,
                                                           const int32_t* __restrict__ node_inst, const double* fext,
                                                           const double* Tint_n, const double* Tk_n,
                                                           const double* fext_n, const double* Dd, const double* Ddd,
-                                                          const double* Cn, const double* Mn, double* res) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                          const double* Cn, const double* Mn, double* res, const VB& vb) {
+    int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
     if (i >= ndofs) return;
     const int64_t n = i / 6;
     const int a = (int)(i % 6);
@@ -313,14 +343,21 @@ __global__ void __launch_bounds__(256) residual_predictor(int64_t ndofs, const i
     }
     res[i] = r;
 }
+__global__ void __launch_bounds__(256) residual_predictor(int64_t ndofs, const int32_t* __restrict__ bptr,
+                                                          const int32_t* __restrict__ bcol, double alpha,
+                                                          const StepCoef* __restrict__ coef,
+                                                          const int32_t* __restrict__ node_inst, const double* fext,
+                                                          const double* Tint_n, const double* Tk_n,
+                                                          const double* fext_n, const double* Dd, const double* Ddd,
+                                                          const double* Cn, const double* Mn, double* res) { residual_predictor_b(ndofs, bptr, bcol, alpha, coef, node_inst, fext, Tint_n, Tk_n, fext_n, Dd, Ddd, Cn, Mn, res, EB_VB); }
 
 // apply_boundary_conditions! (utils_bcs_and_loads.jl:5-29): ΔD[d] = imposed - D[d] on fixed dofs
 // and r -= ΔD[d]·Ktan[:, d], subtracted per row in increasing fixed-dof order
-__global__ void __launch_bounds__(256) bc_lift(int64_t ndofs, const int32_t* __restrict__ bptr,
+EBD void bc_lift_b(int64_t ndofs, const int32_t* __restrict__ bptr,
                                                const int32_t* __restrict__ bcol, const int32_t* __restrict__ fixidx,
                                                const double* imposed, const double* D, const double* Ktan, double* dD,
-                                               double* res, Active act) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                               double* res, Active act, const VB& vb) {
+    int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
     if (i >= ndofs) return;
     if (!act.flag[act.node_inst[i / 6]]) return;
     if (fixidx[i] >= 0) dD[i] = imposed[fixidx[i]] - D[i];
@@ -342,6 +379,10 @@ __global__ void __launch_bounds__(256) bc_lift(int64_t ndofs, const int32_t* __r
     }
     if (any) res[i] = r;
 }
+__global__ void __launch_bounds__(256) bc_lift(int64_t ndofs, const int32_t* __restrict__ bptr,
+                                               const int32_t* __restrict__ bcol, const int32_t* __restrict__ fixidx,
+                                               const double* imposed, const double* D, const double* Ktan, double* dD,
+                                               double* res, Active act) { bc_lift_b(ndofs, bptr, bcol, fixidx, imposed, D, Ktan, dD, res, act, EB_VB); }
 
 // ---- masked block-row SpMV: y = A_ff x on free dofs, y = x on fixed dofs ---------------------
 // (x is zero on fixed dofs throughout the Krylov iteration). Optional dot products with y.
@@ -375,36 +416,48 @@ EBD double masked_row(int64_t i, const int32_t* __restrict__ bptr, const int32_t
 }
 
 template <int NQ>
-__global__ void __launch_bounds__(RED_BLOCK) spmv_masked(Segs sg, const int32_t* __restrict__ bptr,
+EBD void spmv_masked_b(Segs sg, const int32_t* __restrict__ bptr,
                                                          const int32_t* __restrict__ bcol, const uint8_t* __restrict__ fixed,
                                                          const double* __restrict__ A, const double* __restrict__ x,
                                                          double* __restrict__ y, const double* __restrict__ dotw,
-                                                         double* partials, const KrylovCtl* kc) {
-    if (kc[sg.inst[blockIdx.x]].done) return;
+                                                         double* partials, const KrylovCtl* kc, const VB& vb) {
+    if (kc[sg.inst[vb.x]].done) return;
     double v[NQ > 0 ? NQ : 1] = {0};
-    for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
+    for (int64_t i = sg.lo[vb.x] + threadIdx.x; i < sg.hi[vb.x]; i += RED_BLOCK) {
         const double acc = masked_row(i, bptr, bcol, fixed, A, x);
         y[i] = acc;
         if (NQ >= 1) v[0] += dotw[i] * acc;
         if (NQ >= 2) v[NQ - 1] += acc * acc;
     }
-    if (NQ > 0) seg_block_store<(NQ > 0 ? NQ : 1)>(v, partials, sg.nchunks);
+    if (NQ > 0) seg_block_store<(NQ > 0 ? NQ : 1)>(v, partials, sg.nchunks, vb.x);
 }
+template <int NQ>
+__global__ void __launch_bounds__(RED_BLOCK) spmv_masked(Segs sg, const int32_t* __restrict__ bptr,
+                                                         const int32_t* __restrict__ bcol, const uint8_t* __restrict__ fixed,
+                                                         const double* __restrict__ A, const double* __restrict__ x,
+                                                         double* __restrict__ y, const double* __restrict__ dotw,
+                                                         double* partials, const KrylovCtl* kc) { spmv_masked_b<NQ>(sg, bptr, bcol, fixed, A, x, y, dotw, partials, kc, EB_VB); }
 
 // r = b − A x with the solve's operator (the identity on fixed dofs), on the chunks of unfinished
 // solves: the right-hand side of the crossing chain cover's correction
+EBD void residual_masked_b(Segs sg, const int32_t* __restrict__ bptr,
+                                                             const int32_t* __restrict__ bcol,
+                                                             const uint8_t* __restrict__ fixed,
+                                                             const double* __restrict__ A, const double* __restrict__ x,
+                                                             const double* __restrict__ b, double* __restrict__ r,
+                                                             const KrylovCtl* kc, int direct_only, const VB& vb) {
+    if (!krylov_open(kc[sg.inst[vb.x]], direct_only)) return;
+    for (int64_t i = sg.lo[vb.x] + threadIdx.x; i < sg.hi[vb.x]; i += RED_BLOCK) {
+        const double acc = masked_row(i, bptr, bcol, fixed, A, x);
+        r[i] = b[i] - acc;
+    }
+}
 __global__ void __launch_bounds__(RED_BLOCK) residual_masked(Segs sg, const int32_t* __restrict__ bptr,
                                                              const int32_t* __restrict__ bcol,
                                                              const uint8_t* __restrict__ fixed,
                                                              const double* __restrict__ A, const double* __restrict__ x,
                                                              const double* __restrict__ b, double* __restrict__ r,
-                                                             const KrylovCtl* kc, int direct_only) {
-    if (!krylov_open(kc[sg.inst[blockIdx.x]], direct_only)) return;
-    for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
-        const double acc = masked_row(i, bptr, bcol, fixed, A, x);
-        r[i] = b[i] - acc;
-    }
-}
+                                                             const KrylovCtl* kc, int direct_only) { residual_masked_b(sg, bptr, bcol, fixed, A, x, b, r, kc, direct_only, EB_VB); }
 
 // ---- chain block-tridiagonal preconditioner ------------------------------------------------
 // The node graph is covered by chains (paths); along a chain the masked tangent restricted to
@@ -793,13 +846,12 @@ EBD void chain_level(const ChainDev& cd, const ChainLDS& sm, const double* __res
 // clears the flags before each solve).
 // Two wavefronts per SIMD (at most 256 registers), so that two chains share a CU: a launch has
 // about 1.5 chains per CU.
-__global__ void __launch_bounds__(CHAIN_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
-chain_factor(ChainDev cd, const double* __restrict__ A, int* status, int L0, const uint8_t* __restrict__ active) {
+EBD void chain_factor_b(ChainDev cd, const double* __restrict__ A, int* status, int L0, const uint8_t* __restrict__ active, const VB& vb) {
     extern __shared__ __align__(16) double chain_smem[];
     __shared__ __align__(16) double gj_buf[CHAIN_GROUPS * GJ_BUF];
-    if (!active[cd.inst[blockIdx.x]]) return;   // instance frozen in the Newton iteration
-    const int64_t s0 = cd.ptr[blockIdx.x];
-    const int32_t n = cd.ptr[blockIdx.x + 1] - (int32_t)s0;
+    if (!active[cd.inst[vb.x]]) return;   // instance frozen in the Newton iteration
+    const int64_t s0 = cd.ptr[vb.x];
+    const int32_t n = cd.ptr[vb.x + 1] - (int32_t)s0;
     const int32_t slots = (n + (1 << L0) - 1) >> L0;
     const ChainLDS sm{chain_smem, chain_smem + 36 * (int64_t)slots, chain_smem + 72 * (int64_t)slots, L0};
     const Lane6 ln = lane6(gj_buf);
@@ -831,8 +883,10 @@ chain_factor(ChainDev cd, const double* __restrict__ A, int* status, int L0, con
         const int orow = gj6<6>(w, ln, act, ok, load);
         if (act && ln.r < 6) st6(cd.Dm + 36 * s0 + 6 * orow, w);
     }
-    if (!ok) status[cd.inst[blockIdx.x]] = 1;
+    if (!ok) status[cd.inst[vb.x]] = 1;
 }
+__global__ void __launch_bounds__(CHAIN_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
+chain_factor(ChainDev cd, const double* __restrict__ A, int* status, int L0, const uint8_t* __restrict__ active) { chain_factor_b(cd, A, status, L0, active, EB_VB); }
 
 // smallest L0 whose LDS share for the longest chain stays within `budget` bytes
 inline int chain_lds_level(int64_t max_len, int64_t budget) {
@@ -941,13 +995,13 @@ EBD void chain_solve_body(const ChainDev& cd, int64_t s0, int32_t n, double* bw,
 
 // ACCUM: x += M⁻¹ b (the crossing cover's correction) instead of x = M⁻¹ b
 template <bool LDSV, int SOLVE_THREADS, bool ACCUM = false>
-__global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const double* __restrict__ b,
+EBD void chain_solve_b(ChainDev cd, const double* __restrict__ b,
                                                              double* __restrict__ x, const KrylovCtl* kc,
-                                                             int direct_only) {
+                                                             int direct_only, const VB& vb) {
     extern __shared__ __align__(16) double solve_smem[];
-    if (!krylov_open(kc[cd.inst[blockIdx.x]], direct_only)) return;
-    const int64_t s0 = cd.ptr[blockIdx.x];
-    const int32_t n = cd.ptr[blockIdx.x + 1] - (int32_t)s0;
+    if (!krylov_open(kc[cd.inst[vb.x]], direct_only)) return;
+    const int64_t s0 = cd.ptr[vb.x];
+    const int32_t n = cd.ptr[vb.x + 1] - (int32_t)s0;
     const int tid = threadIdx.x;
     double* bw = LDSV ? solve_smem : cd.bw + 6 * s0;
     double* yw = LDSV ? solve_smem + 6 * (int64_t)n : cd.yw + 6 * s0;
@@ -959,6 +1013,10 @@ __global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const 
         xi = ACCUM ? xi + yw[t] : yw[t];
     }
 }
+template <bool LDSV, int SOLVE_THREADS, bool ACCUM = false>
+__global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const double* __restrict__ b,
+                                                             double* __restrict__ x, const KrylovCtl* kc,
+                                                             int direct_only) { chain_solve_b<LDSV, SOLVE_THREADS, ACCUM>(cd, b, x, kc, direct_only, EB_VB); }
 
 // ---- block LU along each chain, from both ends ----------------------------------------------
 // The same block-tridiagonal system factored position by position (block Thomas), from both ends
@@ -1052,12 +1110,12 @@ EBD void lu_sub_row_block(double* w, const double (&row)[6], const double* G) {
 }
 
 // status[instance] is set to 1 when a pivot block of one of its chains is singular
-__global__ void __launch_bounds__(64) chain_lu_factor(ChainDev cd, const double* __restrict__ A, int* status,
-                                                      const uint8_t* __restrict__ active) {
+EBD void chain_lu_factor_b(ChainDev cd, const double* __restrict__ A, int* status,
+                                                      const uint8_t* __restrict__ active, const VB& vb) {
     __shared__ __align__(16) double gj_buf[LU_GROUPS * GJ_BUF];
     __shared__ __align__(16) double g_prev[LU_GROUPS * 36];
     const Lane6 ln = lane6(gj_buf);
-    const int64_t ch = (int64_t)blockIdx.x * LU_CHAINS + (ln.group >> 1);
+    const int64_t ch = (int64_t)vb.x * LU_CHAINS + (ln.group >> 1);
     const bool mine = ln.r < 6 && ch < cd.n_chains;
     const bool on = mine && active[cd.inst[ch]];
     const LuHalf h = lu_half(cd, ch, on, ln.group);
@@ -1164,6 +1222,8 @@ __global__ void __launch_bounds__(64) chain_lu_factor(ChainDev cd, const double*
     }
     if (mine && !ok) status[cd.inst[ch]] = 1;
 }
+__global__ void __launch_bounds__(64) chain_lu_factor(ChainDev cd, const double* __restrict__ A, int* status,
+                                                      const uint8_t* __restrict__ active) { chain_lu_factor_b(cd, A, status, active, EB_VB); }
 
 // The solve of the lane's half (see LuHalf; smax: the wavefront's longest half) with the factors
 // of chain_lu_factor. io.node(p) is the index io.b and io.put address position p by (loaded two
@@ -1320,11 +1380,11 @@ EBD void chain_lu_solve_group(const ChainDev& cd, const LuHalf& h, int32_t smax,
 
 // x = M⁻¹ b (ACCUM: x += M⁻¹ b) along each chain of an unfinished solve
 template <bool ACCUM>
-__global__ void __launch_bounds__(64) chain_lu_solve(ChainDev cd, const double* __restrict__ b, double* __restrict__ x,
-                                                     const KrylovCtl* kc, int direct_only) {
+EBD void chain_lu_solve_b(ChainDev cd, const double* __restrict__ b, double* __restrict__ x,
+                                                     const KrylovCtl* kc, int direct_only, const VB& vb) {
     __shared__ __align__(16) double ex[LU_GROUPS * 12];
     const Lane6 ln = lane6(nullptr);
-    const int64_t ch = (int64_t)blockIdx.x * LU_CHAINS + (ln.group >> 1);
+    const int64_t ch = (int64_t)vb.x * LU_CHAINS + (ln.group >> 1);
     const bool live = ln.r < 6 && ch < cd.n_chains && krylov_open(kc[cd.inst[ch]], direct_only);
     const LuHalf h = lu_half(cd, ch, live, ln.group);
     if (wave_max_i32(live ? h.n : 0) == 0) return;
@@ -1346,6 +1406,9 @@ __global__ void __launch_bounds__(64) chain_lu_solve(ChainDev cd, const double* 
     double* e = ex + 12 * ln.group;
     chain_lu_solve_group(cd, h, smax, r, live, cd.yw + 6 * h.s0, e, e + 6, io);
 }
+template <bool ACCUM>
+__global__ void __launch_bounds__(64) chain_lu_solve(ChainDev cd, const double* __restrict__ b, double* __restrict__ x,
+                                                     const KrylovCtl* kc, int direct_only) { chain_lu_solve_b<ACCUM>(cd, b, x, kc, direct_only, EB_VB); }
 
 // ---- closed chains (Sherman-Morrison-Woodbury) ----------------------------------------------
 // A chain whose two ends are coupled (a ring) has the matrix A = T + U M Uᵀ: T the block-
@@ -1368,9 +1431,9 @@ EBD int32_t closure_unit(int j, int32_t n) { return j < 6 ? j : 6 * (n - 1) + (j
 // column j = blockIdx.y of Z for closure blockIdx.x: the chain solve of U's column j (in LDS:
 // 12 doubles per position)
 template <int NT>
-__global__ void __launch_bounds__(NT) closure_columns(ChainDev cd, ClosureDev cl, const uint8_t* __restrict__ active) {
+EBD void closure_columns_b(ChainDev cd, ClosureDev cl, const uint8_t* __restrict__ active, const VB& vb) {
     extern __shared__ __align__(16) double cl_smem[];
-    const int c = blockIdx.x, j = blockIdx.y;
+    const int c = vb.x, j = vb.y;
     const int32_t ch = cl.chain[c];
     if (!active[cd.inst[ch]]) return;   // frozen instance: its factors were not refreshed
     const int64_t s0 = cd.ptr[ch];
@@ -1384,13 +1447,15 @@ __global__ void __launch_bounds__(NT) closure_columns(ChainDev cd, ClosureDev cl
     double* z = cl.Z + cl.zoff[c] + (int64_t)j * 6 * n;
     for (int32_t t = threadIdx.x; t < 6 * n; t += NT) z[t] = yw[t];
 }
+template <int NT>
+__global__ void __launch_bounds__(NT) closure_columns(ChainDev cd, ClosureDev cl, const uint8_t* __restrict__ active) { closure_columns_b<NT>(cd, cl, active, EB_VB); }
 
 // the columns of Z with the block-LU factors: one chain's two six-lane groups per (closure,
 // column), the column solved in Z
-__global__ void __launch_bounds__(64) closure_columns_lu(ChainDev cd, ClosureDev cl, const uint8_t* __restrict__ active) {
+EBD void closure_columns_lu_b(ChainDev cd, ClosureDev cl, const uint8_t* __restrict__ active, const VB& vb) {
     __shared__ __align__(16) double ex[LU_GROUPS * 12];
     const Lane6 ln = lane6(nullptr);
-    const int64_t item = (int64_t)blockIdx.x * LU_CHAINS + (ln.group >> 1);
+    const int64_t item = (int64_t)vb.x * LU_CHAINS + (ln.group >> 1);
     const bool mine = ln.r < 6 && item < 12 * (int64_t)cl.n;
     const int c = mine ? (int)(item / 12) : 0, j = (int)(item % 12);
     const bool live = mine && active[cd.inst[cl.chain[c]]];
@@ -1412,22 +1477,24 @@ __global__ void __launch_bounds__(64) closure_columns_lu(ChainDev cd, ClosureDev
     double* e = ex + 12 * ln.group;
     chain_lu_solve_group(cd, h, smax, r, live, z, e, e + 6, io);
 }
+__global__ void __launch_bounds__(64) closure_columns_lu(ChainDev cd, ClosureDev cl, const uint8_t* __restrict__ active) { closure_columns_lu_b(cd, cl, active, EB_VB); }
 
 // H = (I + M G)⁻¹ M per closure, one wavefront each, in LDS: Gauss-Jordan elimination with
 // partial pivoting on [I + M G | M]; a singular or non-finite system leaves H = 0 (no correction:
 // the plain chain preconditioner)
-__global__ void __launch_bounds__(64) closure_setup(ChainDev cd, ClosureDev cl, const double* __restrict__ A,
-                                                    const uint8_t* __restrict__ active) {
+EBD void closure_setup_b(ChainDev cd, ClosureDev cl, const double* __restrict__ A,
+                                                    const uint8_t* __restrict__ active, const VB& vb) {
     __shared__ double M[12][12];
     __shared__ double W[12][24];
     __shared__ double fcol[12];
     __shared__ int piv_row, ok;
-    const int c = blockIdx.x, t = threadIdx.x;
+    // the work of one wavefront (called by a larger workgroup, the other lanes keep the barriers)
+    const int c = vb.x, t = threadIdx.x, tw = t < 64 ? t : (1 << 20);
     const int32_t ch = cl.chain[c];
     if (!active[cd.inst[ch]]) return;
     const int32_t n = cd.ptr[ch + 1] - cd.ptr[ch];
     const double* Z = cl.Z + cl.zoff[c];
-    for (int q = t; q < 144; q += 64) {
+    for (int q = tw; q < 144; q += 64) {
         const int r = q / 12, k = q % 12;
         double v = 0.0;
         if (r < 6 && k >= 6) {
@@ -1441,7 +1508,7 @@ __global__ void __launch_bounds__(64) closure_setup(ChainDev cd, ClosureDev cl, 
     }
     if (t == 0) ok = 1;
     __syncthreads();
-    for (int q = t; q < 144; q += 64) {
+    for (int q = tw; q < 144; q += 64) {
         const int r = q / 12, jc = q % 12;
         double v = r == jc ? 1.0 : 0.0;
         for (int k = 0; k < 12; ++k) v += M[r][k] * Z[(int64_t)jc * 6 * n + closure_unit(k, n)];
@@ -1467,24 +1534,26 @@ __global__ void __launch_bounds__(64) closure_setup(ChainDev cd, ClosureDev cl, 
         if (t < 24) W[p][t] *= inv;
         if (t < 12) fcol[t] = W[t][p];   // the column's factors, read before any row changes
         __syncthreads();
-        for (int q = t; q < 12 * 24; q += 64) {
+        for (int q = tw; q < 12 * 24; q += 64) {
             const int r = q / 24, k = q % 24;
             if (r != p) W[r][k] -= fcol[r] * W[p][k];
         }
         __syncthreads();
     }
     double* H = cl.H + 144 * (int64_t)c;
-    for (int q = t; q < 144; q += 64) {
+    for (int q = tw; q < 144; q += 64) {
         const double v = W[q / 12][12 + q % 12];
         H[q] = ok && isfinite(v) ? v : 0.0;
     }
 }
+__global__ void __launch_bounds__(64) closure_setup(ChainDev cd, ClosureDev cl, const double* __restrict__ A,
+                                                    const uint8_t* __restrict__ active) { closure_setup_b(cd, cl, A, active, EB_VB); }
 
 // y −= Z (H (Uᵀ y)) on each closed chain of an unfinished solve, after the chain solve
-__global__ void __launch_bounds__(256) closure_apply(ChainDev cd, ClosureDev cl, double* __restrict__ y,
-                                                     const KrylovCtl* __restrict__ kc, int direct_only) {
+EBD void closure_apply_b(ChainDev cd, ClosureDev cl, double* __restrict__ y,
+                                                     const KrylovCtl* __restrict__ kc, int direct_only, const VB& vb) {
     __shared__ double sv[12];
-    const int c = blockIdx.x;
+    const int c = vb.x;
     const int32_t ch = cl.chain[c];
     if (!krylov_open(kc[cd.inst[ch]], direct_only)) return;
     const int64_t s0 = cd.ptr[ch];
@@ -1506,6 +1575,8 @@ __global__ void __launch_bounds__(256) closure_apply(ChainDev cd, ClosureDev cl,
         y[6 * (int64_t)cd.node[s0 + q / 6] + q % 6] -= acc;
     }
 }
+__global__ void __launch_bounds__(256) closure_apply(ChainDev cd, ClosureDev cl, double* __restrict__ y,
+                                                     const KrylovCtl* __restrict__ kc, int direct_only) { closure_apply_b(cd, cl, y, kc, direct_only, EB_VB); }
 
 // ---- BiCGStab (van der Vorst), right-preconditioned, FP64 scalars on the device -------------
 // One independent solve per instance, run in lockstep: the scalars live in KrylovCtl[instance],
@@ -1523,10 +1594,11 @@ struct KrylovAll {
 };
 
 // start of a linear solve: clears the per-instance pivot flags the factorisation may set
-__global__ void __launch_bounds__(256) solve_begin(int ninst, int32_t* pivot_inst, KrylovAll* all) {
+EBD void solve_begin_b(int ninst, int32_t* pivot_inst, KrylovAll* all, const VB& vb) {
     for (int s = threadIdx.x; s < ninst; s += blockDim.x) pivot_inst[s] = 0;
     if (threadIdx.x == 0) all->pivot_status = 0;
 }
+__global__ void __launch_bounds__(256) solve_begin(int ninst, int32_t* pivot_inst, KrylovAll* all) { solve_begin_b(ninst, pivot_inst, all, EB_VB); }
 
 // true when the kernel may run: ungated, or the polled solve is done
 EBD bool gate_open(const KrylovAll* gate) { return !gate || gate->all_done; }
@@ -1546,7 +1618,7 @@ EBD void krylov_summary(const KrylovCtl* kc, int ninst, KrylovAll* all) {
     __shared__ int s_done, s_iters, s_fail, s_nfail, s_piv;
     if (threadIdx.x == 0) { s_done = 1; s_iters = 0; s_fail = 0x7fffffff; s_nfail = 0; s_piv = 0; }
     __syncthreads();
-    for (int s = threadIdx.x; s < ninst; s += FIN_BLOCK) {
+    for (int s = threadIdx.x; s < ninst; s += (int)blockDim.x) {
         const KrylovCtl& k = kc[s];
         double rel;
         if (!k.done) atomicAnd(&s_done, 0);
@@ -1569,22 +1641,24 @@ EBD void krylov_summary(const KrylovCtl* kc, int ninst, KrylovAll* all) {
 }
 
 // x = 0, r = r̂ = b (b = residual with fixed dofs zeroed), ‖b‖²
-__global__ void __launch_bounds__(RED_BLOCK) krylov_init(Segs sg, const uint8_t* fixed, const double* res, double* x,
-                                                         double* r, double* rhat, double* p, double* v, double* partials) {
+EBD void krylov_init_b(Segs sg, const uint8_t* fixed, const double* res, double* x,
+                                                         double* r, double* rhat, double* p, double* v, double* partials, const VB& vb) {
     double q[1] = {0};
-    for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
+    for (int64_t i = sg.lo[vb.x] + threadIdx.x; i < sg.hi[vb.x]; i += RED_BLOCK) {
         double bi = fixed[i] ? 0.0 : res[i];
         x[i] = 0; r[i] = bi; rhat[i] = bi; p[i] = 0; v[i] = 0;
         q[0] += bi * bi;
     }
-    seg_block_store<1>(q, partials, sg.nchunks);
+    seg_block_store<1>(q, partials, sg.nchunks, vb.x);
 }
+__global__ void __launch_bounds__(RED_BLOCK) krylov_init(Segs sg, const uint8_t* fixed, const double* res, double* x,
+                                                         double* r, double* rhat, double* p, double* v, double* partials) { krylov_init_b(sg, fixed, res, x, r, rhat, p, v, partials, EB_VB); }
 // instances that are not active in the Newton iteration start done
 // instances whose factorisation met a singular pivot block start done and failed
-__global__ void __launch_bounds__(FIN_BLOCK) krylov_init_fin(const double* partials, Segs sg, KrylovCtl* kc, double tol,
+EBD void krylov_init_fin_b(const double* partials, Segs sg, KrylovCtl* kc, double tol,
                                                              const uint8_t* active, const int32_t* pivot_inst,
-                                                             const uint8_t* exact, KrylovAll* all) {
-    for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
+                                                             const uint8_t* exact, KrylovAll* all, const VB& vb) {
+    for (int s = threadIdx.x; s < sg.ninst; s += (int)blockDim.x) {
         double o[1];
         seg_sum<1>(partials, sg, s, o);
         KrylovCtl& k = kc[s];
@@ -1597,19 +1671,24 @@ __global__ void __launch_bounds__(FIN_BLOCK) krylov_init_fin(const double* parti
     __syncthreads();
     krylov_summary(kc, sg.ninst, all);
 }
+__global__ void __launch_bounds__(FIN_BLOCK) krylov_init_fin(const double* partials, Segs sg, KrylovCtl* kc, double tol,
+                                                             const uint8_t* active, const int32_t* pivot_inst,
+                                                             const uint8_t* exact, KrylovAll* all) { krylov_init_fin_b(partials, sg, kc, tol, active, pivot_inst, exact, all, EB_VB); }
 // p = r + β(p - ωv), β = (ρ/ρ_old)(α/ω); first iteration p = r
-__global__ void __launch_bounds__(RED_BLOCK) krylov_p(Segs sg, const KrylovCtl* kc, const double* r, double* p,
-                                                      const double* v) {
-    const KrylovCtl& k = kc[sg.inst[blockIdx.x]];
+EBD void krylov_p_b(Segs sg, const KrylovCtl* kc, const double* r, double* p,
+                                                      const double* v, const VB& vb) {
+    const KrylovCtl& k = kc[sg.inst[vb.x]];
     if (k.done) return;
     const bool first = k.iters == 0;
     const double beta = first ? 0.0 : (k.rho / k.rho_old) * (k.alpha / k.omega);
     const double om = k.omega;
-    for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK)
+    for (int64_t i = sg.lo[vb.x] + threadIdx.x; i < sg.hi[vb.x]; i += RED_BLOCK)
         p[i] = first ? r[i] : r[i] + beta * (p[i] - om * v[i]);
 }
-__global__ void __launch_bounds__(FIN_BLOCK) krylov_alpha_fin(const double* partials, Segs sg, KrylovCtl* kc) {
-    for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
+__global__ void __launch_bounds__(RED_BLOCK) krylov_p(Segs sg, const KrylovCtl* kc, const double* r, double* p,
+                                                      const double* v) { krylov_p_b(sg, kc, r, p, v, EB_VB); }
+EBD void krylov_alpha_fin_b(const double* partials, Segs sg, KrylovCtl* kc, const VB& vb) {
+    for (int s = threadIdx.x; s < sg.ninst; s += (int)blockDim.x) {
         KrylovCtl& k = kc[s];
         if (k.done) continue;
         double o[1];
@@ -1618,23 +1697,26 @@ __global__ void __launch_bounds__(FIN_BLOCK) krylov_alpha_fin(const double* part
         k.alpha = k.rho / o[0];
     }
 }
+__global__ void __launch_bounds__(FIN_BLOCK) krylov_alpha_fin(const double* partials, Segs sg, KrylovCtl* kc) { krylov_alpha_fin_b(partials, sg, kc, EB_VB); }
 // s = r - αv, x += αy, ‖s‖²
-__global__ void __launch_bounds__(RED_BLOCK) krylov_s(Segs sg, const KrylovCtl* kc, const double* r, const double* v,
-                                                      const double* y, double* s, double* x, double* partials) {
-    const KrylovCtl& k = kc[sg.inst[blockIdx.x]];
+EBD void krylov_s_b(Segs sg, const KrylovCtl* kc, const double* r, const double* v,
+                                                      const double* y, double* s, double* x, double* partials, const VB& vb) {
+    const KrylovCtl& k = kc[sg.inst[vb.x]];
     if (k.done) return;
     const double al = k.alpha;
     double q[1] = {0};
-    for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
+    for (int64_t i = sg.lo[vb.x] + threadIdx.x; i < sg.hi[vb.x]; i += RED_BLOCK) {
         double si = r[i] - al * v[i];
         s[i] = si;
         x[i] += al * y[i];
         q[0] += si * si;
     }
-    seg_block_store<1>(q, partials, sg.nchunks);
+    seg_block_store<1>(q, partials, sg.nchunks, vb.x);
 }
-__global__ void __launch_bounds__(FIN_BLOCK) krylov_s_fin(const double* partials, Segs sg, KrylovCtl* kc) {
-    for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
+__global__ void __launch_bounds__(RED_BLOCK) krylov_s(Segs sg, const KrylovCtl* kc, const double* r, const double* v,
+                                                      const double* y, double* s, double* x, double* partials) { krylov_s_b(sg, kc, r, v, y, s, x, partials, EB_VB); }
+EBD void krylov_s_fin_b(const double* partials, Segs sg, KrylovCtl* kc, const VB& vb) {
+    for (int s = threadIdx.x; s < sg.ninst; s += (int)blockDim.x) {
         KrylovCtl& k = kc[s];
         if (k.done) continue;
         double o[1];
@@ -1643,8 +1725,9 @@ __global__ void __launch_bounds__(FIN_BLOCK) krylov_s_fin(const double* partials
         if (o[0] <= k.tol2 * k.bnorm2) { k.rr2 = o[0]; k.iters += 1; k.done = 1; }
     }
 }
-__global__ void __launch_bounds__(FIN_BLOCK) krylov_omega_fin(const double* partials, Segs sg, KrylovCtl* kc) {
-    for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
+__global__ void __launch_bounds__(FIN_BLOCK) krylov_s_fin(const double* partials, Segs sg, KrylovCtl* kc) { krylov_s_fin_b(partials, sg, kc, EB_VB); }
+EBD void krylov_omega_fin_b(const double* partials, Segs sg, KrylovCtl* kc, const VB& vb) {
+    for (int s = threadIdx.x; s < sg.ninst; s += (int)blockDim.x) {
         KrylovCtl& k = kc[s];
         if (k.done) continue;
         double o[2];
@@ -1653,25 +1736,29 @@ __global__ void __launch_bounds__(FIN_BLOCK) krylov_omega_fin(const double* part
         k.omega = o[0] / o[1];
     }
 }
+__global__ void __launch_bounds__(FIN_BLOCK) krylov_omega_fin(const double* partials, Segs sg, KrylovCtl* kc) { krylov_omega_fin_b(partials, sg, kc, EB_VB); }
 // x += ωz, r = s - ωt, (r̂·r), ‖r‖²
-__global__ void __launch_bounds__(RED_BLOCK) krylov_x(Segs sg, const KrylovCtl* kc, const double* z, const double* s,
+EBD void krylov_x_b(Segs sg, const KrylovCtl* kc, const double* z, const double* s,
                                                       const double* t, const double* rhat, double* x, double* r,
-                                                      double* partials) {
-    const KrylovCtl& k = kc[sg.inst[blockIdx.x]];
+                                                      double* partials, const VB& vb) {
+    const KrylovCtl& k = kc[sg.inst[vb.x]];
     if (k.done) return;
     const double om = k.omega;
     double q[2] = {0, 0};
-    for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
+    for (int64_t i = sg.lo[vb.x] + threadIdx.x; i < sg.hi[vb.x]; i += RED_BLOCK) {
         x[i] += om * z[i];
         double ri = s[i] - om * t[i];
         r[i] = ri;
         q[0] += rhat[i] * ri;
         q[1] += ri * ri;
     }
-    seg_block_store<2>(q, partials, sg.nchunks);
+    seg_block_store<2>(q, partials, sg.nchunks, vb.x);
 }
-__global__ void __launch_bounds__(FIN_BLOCK) krylov_rho_fin(const double* partials, Segs sg, KrylovCtl* kc, KrylovAll* all) {
-    for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
+__global__ void __launch_bounds__(RED_BLOCK) krylov_x(Segs sg, const KrylovCtl* kc, const double* z, const double* s,
+                                                      const double* t, const double* rhat, double* x, double* r,
+                                                      double* partials) { krylov_x_b(sg, kc, z, s, t, rhat, x, r, partials, EB_VB); }
+EBD void krylov_rho_fin_b(const double* partials, Segs sg, KrylovCtl* kc, KrylovAll* all, const VB& vb) {
+    for (int s = threadIdx.x; s < sg.ninst; s += (int)blockDim.x) {
         KrylovCtl& k = kc[s];
         if (!k.done) {
             double o[2];
@@ -1687,6 +1774,7 @@ __global__ void __launch_bounds__(FIN_BLOCK) krylov_rho_fin(const double* partia
     __syncthreads();
     krylov_summary(kc, sg.ninst, all);
 }
+__global__ void __launch_bounds__(FIN_BLOCK) krylov_rho_fin(const double* partials, Segs sg, KrylovCtl* kc, KrylovAll* all) { krylov_rho_fin_b(partials, sg, kc, all, EB_VB); }
 // ---- the direct first step ----------------------------------------------------------------------
 // When the chain cover (with the closure corrections) holds every coupling of the tangent, the
 // preconditioner M is the solve's operator A up to rounding and x = M⁻¹ b solves the system. The
@@ -1697,25 +1785,31 @@ __global__ void __launch_bounds__(FIN_BLOCK) krylov_rho_fin(const double* partia
 //
 // r = b − A x on the chunks of unfinished direct solves (b = r̂, which krylov_init set to the
 // right-hand side); partials: r̂·r, ‖r‖²
-__global__ void __launch_bounds__(RED_BLOCK) krylov_direct_residual(Segs sg, const int32_t* __restrict__ bptr,
+EBD void krylov_direct_residual_b(Segs sg, const int32_t* __restrict__ bptr,
                                                                     const int32_t* __restrict__ bcol,
                                                                     const uint8_t* __restrict__ fixed,
                                                                     const double* __restrict__ A, const double* __restrict__ x,
                                                                     const double* __restrict__ b, double* __restrict__ r,
-                                                                    double* partials, const KrylovCtl* kc) {
-    if (!krylov_open(kc[sg.inst[blockIdx.x]], 1)) return;
+                                                                    double* partials, const KrylovCtl* kc, const VB& vb) {
+    if (!krylov_open(kc[sg.inst[vb.x]], 1)) return;
     double q[2] = {0, 0};
-    for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
+    for (int64_t i = sg.lo[vb.x] + threadIdx.x; i < sg.hi[vb.x]; i += RED_BLOCK) {
         const double ri = b[i] - masked_row(i, bptr, bcol, fixed, A, x);
         r[i] = ri;
         q[0] += b[i] * ri;
         q[1] += ri * ri;
     }
-    seg_block_store<2>(q, partials, sg.nchunks);
+    seg_block_store<2>(q, partials, sg.nchunks, vb.x);
 }
-__global__ void __launch_bounds__(FIN_BLOCK) krylov_direct_fin(const double* partials, Segs sg, KrylovCtl* kc,
-                                                               KrylovAll* all) {
-    for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
+__global__ void __launch_bounds__(RED_BLOCK) krylov_direct_residual(Segs sg, const int32_t* __restrict__ bptr,
+                                                                    const int32_t* __restrict__ bcol,
+                                                                    const uint8_t* __restrict__ fixed,
+                                                                    const double* __restrict__ A, const double* __restrict__ x,
+                                                                    const double* __restrict__ b, double* __restrict__ r,
+                                                                    double* partials, const KrylovCtl* kc) { krylov_direct_residual_b(sg, bptr, bcol, fixed, A, x, b, r, partials, kc, EB_VB); }
+EBD void krylov_direct_fin_b(const double* partials, Segs sg, KrylovCtl* kc,
+                                                               KrylovAll* all, const VB& vb) {
+    for (int s = threadIdx.x; s < sg.ninst; s += (int)blockDim.x) {
         KrylovCtl& k = kc[s];
         if (!krylov_open(k, 1)) continue;
         double o[2];
@@ -1731,25 +1825,29 @@ __global__ void __launch_bounds__(FIN_BLOCK) krylov_direct_fin(const double* par
     __syncthreads();
     krylov_summary(kc, sg.ninst, all);
 }
+__global__ void __launch_bounds__(FIN_BLOCK) krylov_direct_fin(const double* partials, Segs sg, KrylovCtl* kc,
+                                                               KrylovAll* all) { krylov_direct_fin_b(partials, sg, kc, all, EB_VB); }
 
 // ΔD[free] = x[free] for the instances active in the Newton iteration
 // Kernels of the corrector's update that take a `gate` run only once the linear solve is done: the
 // corrector launches them behind a first Krylov iteration without waiting for the host, and
 // launches them again after further iterations when that one did not finish the solve.
-__global__ void __launch_bounds__(256) scatter_solution(int64_t ndofs, const uint8_t* fixed, const double* x, double* dD,
-                                                        Active act, const KrylovAll* gate) {
+EBD void scatter_solution_b(int64_t ndofs, const uint8_t* fixed, const double* x, double* dD,
+                                                        Active act, const KrylovAll* gate, const VB& vb) {
     if (!gate_open(gate)) return;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
     if (i < ndofs && !fixed[i] && act.flag[act.node_inst[i / 6]]) dD[i] = x[i];
 }
+__global__ void __launch_bounds__(256) scatter_solution(int64_t ndofs, const uint8_t* fixed, const double* x, double* dD,
+                                                        Active act, const KrylovAll* gate) { scatter_solution_b(ndofs, fixed, x, dD, act, gate, EB_VB); }
 
 // Instances whose solve failed (a singular pivot block, a breakdown or a stall short of the
 // tolerance, or the iteration cap reached) leave the Newton iteration: the corrector records
 // k = kfail (max_iterations + 1, a step that did not converge: Δt is halved and the step
 // repeated, run_simulation.jl:55-71), the predictor flags them in pfail. The other instances
 // carry on as if alone.
-__global__ void __launch_bounds__(256) solve_screen(int ninst, const KrylovCtl* kc, uint8_t* active, int32_t* kiter,
-                                                    int32_t kfail, uint8_t* pfail, const KrylovAll* gate) {
+EBD void solve_screen_b(int ninst, const KrylovCtl* kc, uint8_t* active, int32_t* kiter,
+                                                    int32_t kfail, uint8_t* pfail, const KrylovAll* gate, const VB& vb) {
     if (!gate_open(gate)) return;
     for (int s = threadIdx.x; s < ninst; s += blockDim.x) {
         if (!active[s]) continue;
@@ -1761,6 +1859,8 @@ __global__ void __launch_bounds__(256) solve_screen(int ninst, const KrylovCtl* 
         }
     }
 }
+__global__ void __launch_bounds__(256) solve_screen(int ninst, const KrylovCtl* kc, uint8_t* active, int32_t* kiter,
+                                                    int32_t kfail, uint8_t* pfail, const KrylovAll* gate) { solve_screen_b(ninst, kc, active, kiter, kfail, pfail, gate, EB_VB); }
 
 // ---- Newmark / rotation updates (src/utils_solver.jl:6-48, :157-245) -----------------------
 
@@ -1772,9 +1872,9 @@ EBD M3<double> ldm3(const double* a, int64_t n) { M3<double> m; for (int k = 0; 
 EBD void stm(double* a, int64_t n, const M3<double>& m) { for (int k = 0; k < 9; ++k) a[9 * n + k] = m.a[k]; }
 
 // initialize_global_corrector! (predictor=0) / initialize_global_predictor! (predictor=1)
-__global__ void __launch_bounds__(256) init_global(int64_t N, NodeFields nf, double* D, double* Dd, double* Ddd, double* dD,
-                                                   int predictor, Active act) {
-    int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+EBD void init_global_b(int64_t N, NodeFields nf, double* D, double* Dd, double* Ddd, double* dD,
+                                                   int predictor, Active act, const VB& vb) {
+    int64_t n = (int64_t)vb.x * blockDim.x + threadIdx.x;
     if (n >= N || !act.flag[act.node_inst[n]]) return;
     for (int k = 0; k < 3; ++k) {
         dD[6 * n + k] = 0; D[6 * n + k] = nf.u[3 * n + k]; Dd[6 * n + k] = nf.ud[3 * n + k]; Ddd[6 * n + k] = nf.udd[3 * n + k];
@@ -1782,23 +1882,28 @@ __global__ void __launch_bounds__(256) init_global(int64_t N, NodeFields nf, dou
         Dd[6 * n + 3 + k] = nf.wd[3 * n + k]; Ddd[6 * n + 3 + k] = nf.wdd[3 * n + k];
     }
 }
+__global__ void __launch_bounds__(256) init_global(int64_t N, NodeFields nf, double* D, double* Dd, double* Ddd, double* dD,
+                                                   int predictor, Active act) { init_global_b(N, nf, D, Dd, Ddd, dD, predictor, act, EB_VB); }
 
 // compute_global_corrector! per dof (all dofs, fixed included)
-__global__ void __launch_bounds__(256) global_corrector(int64_t ndofs, const StepCoef* __restrict__ coef, double* D,
+EBD void global_corrector_b(int64_t ndofs, const StepCoef* __restrict__ coef, double* D,
                                                         double* Dd, double* Ddd, const double* dD, Active act,
-                                                        const KrylovAll* gate) {
+                                                        const KrylovAll* gate, const VB& vb) {
     if (!gate_open(gate)) return;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
     if (i >= ndofs) return;
     const int32_t s = act.node_inst[i / 6];
     if (!act.flag[s]) return;
     const double gb = coef[s].gbdt, inv = coef[s].inv;
     D[i] += dD[i]; Dd[i] += gb * dD[i]; Ddd[i] += inv * dD[i];
 }
+__global__ void __launch_bounds__(256) global_corrector(int64_t ndofs, const StepCoef* __restrict__ coef, double* D,
+                                                        double* Dd, double* Ddd, const double* dD, Active act,
+                                                        const KrylovAll* gate) { global_corrector_b(ndofs, coef, D, Dd, Ddd, dD, act, gate, EB_VB); }
 // compute_global_predictor!
-__global__ void __launch_bounds__(256) global_predictor(int64_t ndofs, const StepCoef* __restrict__ coef, double* D,
-                                                        double* Dd, double* Ddd, const double* dD, Active act) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+EBD void global_predictor_b(int64_t ndofs, const StepCoef* __restrict__ coef, double* D,
+                                                        double* Dd, double* Ddd, const double* dD, Active act, const VB& vb) {
+    int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
     if (i >= ndofs) return;
     const int32_t s = act.node_inst[i / 6];
     if (!act.flag[s]) return;
@@ -1809,14 +1914,16 @@ __global__ void __launch_bounds__(256) global_predictor(int64_t ndofs, const Ste
     Dd[i] += (gbdt * dD[i] - gob * Ddn) + dt2b * Ddd[i];
     Ddd[i] += inv * ((dD[i] - dt * Ddn) - ((dt * dt) / 2) * Ddd[i]);
 }
+__global__ void __launch_bounds__(256) global_predictor(int64_t ndofs, const StepCoef* __restrict__ coef, double* D,
+                                                        double* Dd, double* Ddd, const double* dD, Active act) { global_predictor_b(ndofs, coef, D, Dd, Ddd, dD, act, EB_VB); }
 
 // compute_local_corrector!: u, u̇, ü from D; ΔR ← exp(ΔD_θ)ΔR; w = log ΔR; ẇ, ẅ; R = ΔR Rⁿ
-__global__ void __launch_bounds__(256) local_corrector(int64_t N, NodeFields nf, const double* D, const double* Dd,
+EBD void local_corrector_b(int64_t N, NodeFields nf, const double* D, const double* Dd,
                                                        const double* Ddd, const double* dD,
                                                        const StepCoef* __restrict__ coef, Active act,
-                                                       const KrylovAll* gate) {
+                                                       const KrylovAll* gate, const VB& vb) {
     if (!gate_open(gate)) return;
-    int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t n = (int64_t)vb.x * blockDim.x + threadIdx.x;
     if (n >= N || !act.flag[act.node_inst[n]]) return;
     const HHTc h = coef[act.node_inst[n]].h;
     stv(nf.u, n, {D[6 * n], D[6 * n + 1], D[6 * n + 2]});
@@ -1832,11 +1939,15 @@ __global__ void __launch_bounds__(256) local_corrector(int64_t N, NodeFields nf,
     stv(nf.wdd, n, dR * c);
     stm(nf.R, n, dR * ldm3(nf.R_n, n));
 }
+__global__ void __launch_bounds__(256) local_corrector(int64_t N, NodeFields nf, const double* D, const double* Dd,
+                                                       const double* Ddd, const double* dD,
+                                                       const StepCoef* __restrict__ coef, Active act,
+                                                       const KrylovAll* gate) { local_corrector_b(N, nf, D, Dd, Ddd, dD, coef, act, gate, EB_VB); }
 
 // compute_local_predictor!: w = θ̃ = D_θ, ΔR = exp(θ̃), R = ΔR Rⁿ, ẇ, ẅ from θ̃
-__global__ void __launch_bounds__(256) local_predictor(int64_t N, NodeFields nf, const double* D, const double* Dd,
-                                                       const double* Ddd, const StepCoef* __restrict__ coef, Active act) {
-    int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+EBD void local_predictor_b(int64_t N, NodeFields nf, const double* D, const double* Dd,
+                                                       const double* Ddd, const StepCoef* __restrict__ coef, Active act, const VB& vb) {
+    int64_t n = (int64_t)vb.x * blockDim.x + threadIdx.x;
     if (n >= N || !act.flag[act.node_inst[n]]) return;
     const HHTc h = coef[act.node_inst[n]].h;
     stv(nf.u, n, {D[6 * n], D[6 * n + 1], D[6 * n + 2]});
@@ -1853,14 +1964,16 @@ __global__ void __launch_bounds__(256) local_predictor(int64_t N, NodeFields nf,
     stv(nf.wd, n, dR * a);
     stv(nf.wdd, n, dR * c);
 }
+__global__ void __launch_bounds__(256) local_predictor(int64_t N, NodeFields nf, const double* D, const double* Dd,
+                                                       const double* Ddd, const StepCoef* __restrict__ coef, Active act) { local_predictor_b(N, nf, D, Dd, Ddd, coef, act, EB_VB); }
 
 // compute_norms_corrector (utils_solver.jl:126-151): Σ over free dofs of ΔD², r², (f+Tᶜ+Tᵏ)² and
 // Σ over fixed dofs of (f+Tᶜ+Tᵏ)², per instance
-__global__ void __launch_bounds__(RED_BLOCK) norms_partials(Segs sg, const uint8_t* fixed, const double* dD,
+EBD void norms_partials_b(Segs sg, const uint8_t* fixed, const double* dD,
                                                             const double* res, const double* fext, const double* Tc,
-                                                            const double* Tk, double* partials) {
+                                                            const double* Tk, double* partials, const VB& vb) {
     double q[4] = {0, 0, 0, 0};
-    for (int64_t i = sg.lo[blockIdx.x] + threadIdx.x; i < sg.hi[blockIdx.x]; i += RED_BLOCK) {
+    for (int64_t i = sg.lo[vb.x] + threadIdx.x; i < sg.hi[vb.x]; i += RED_BLOCK) {
         double f = (fext[i] + Tc[i]) + Tk[i];
         if (fixed[i]) {
             q[3] += f * f;
@@ -1870,8 +1983,11 @@ __global__ void __launch_bounds__(RED_BLOCK) norms_partials(Segs sg, const uint8
             q[2] += f * f;
         }
     }
-    seg_block_store<4>(q, partials, sg.nchunks);
+    seg_block_store<4>(q, partials, sg.nchunks, vb.x);
 }
+__global__ void __launch_bounds__(RED_BLOCK) norms_partials(Segs sg, const uint8_t* fixed, const double* dD,
+                                                            const double* res, const double* fext, const double* Tc,
+                                                            const double* Tk, double* partials) { norms_partials_b(sg, fixed, dD, res, fext, Tc, Tk, partials, EB_VB); }
 
 struct NewtonCtl {            // corrector loop control (src/corrector.jl:17-36)
     double tol_res, tol_disp;
@@ -1885,16 +2001,16 @@ struct NewtonAll { int n_active, pad[3]; };
 // recorded as the reference's loop leaves it (iterations done + 1)
 // In a Newton iteration the iteration number is kept on the device: `kdone` counts the
 // iterations finalised since newton_reset, so that one launch sequence serves every iteration.
-__global__ void __launch_bounds__(FIN_BLOCK) norms_finalize(const double* partials, Segs sg, double* out, NewtonCtl nc,
+EBD void norms_finalize_b(const double* partials, Segs sg, double* out, NewtonCtl nc,
                                                             uint8_t* active, int32_t* kiter, NewtonAll* all,
-                                                            const KrylovAll* gate, int32_t* kdone) {
+                                                            const KrylovAll* gate, int32_t* kdone, const VB& vb) {
     if (!gate_open(gate)) return;
     __shared__ int cnt;
     if (threadIdx.x == 0) cnt = 0;
     if (nc.newton) nc.k = *kdone + 1;
     __syncthreads();
     int mine = 0;
-    for (int s = threadIdx.x; s < sg.ninst; s += FIN_BLOCK) {
+    for (int s = threadIdx.x; s < sg.ninst; s += (int)blockDim.x) {
         if (nc.newton && !active[s]) continue;
         double o[4];
         seg_sum<4>(partials, sg, s, o);
@@ -1919,18 +2035,23 @@ __global__ void __launch_bounds__(FIN_BLOCK) norms_finalize(const double* partia
         if (nc.newton) *kdone = nc.k;
     }
 }
+__global__ void __launch_bounds__(FIN_BLOCK) norms_finalize(const double* partials, Segs sg, double* out, NewtonCtl nc,
+                                                            uint8_t* active, int32_t* kiter, NewtonAll* all,
+                                                            const KrylovAll* gate, int32_t* kdone) { norms_finalize_b(partials, sg, out, nc, active, kiter, all, gate, kdone, EB_VB); }
 
 // Start (and end) of a corrector loop: the instances taking part in the step become active. At
 // the start (`pfail` given) an instance whose predictor solve failed stays frozen with k = kfail.
-__global__ void newton_reset(int ninst, const uint8_t* part, const uint8_t* pfail, uint8_t* active, int32_t* kiter,
-                             int32_t* kdone, int32_t kfail) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *kdone = 0;
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ninst; s += gridDim.x * blockDim.x) {
+EBD void newton_reset_b(int ninst, const uint8_t* part, const uint8_t* pfail, uint8_t* active, int32_t* kiter,
+                             int32_t* kdone, int32_t kfail, const VB& vb) {
+    if (vb.x == 0 && threadIdx.x == 0) *kdone = 0;
+    for (int s = vb.x * blockDim.x + threadIdx.x; s < ninst; s += vb.n * blockDim.x) {
         const bool failed = pfail && part[s] && pfail[s];
         active[s] = (part[s] && !failed) ? 1 : 0;
         kiter[s] = failed ? kfail : 0;
     }
 }
+__global__ void newton_reset(int ninst, const uint8_t* part, const uint8_t* pfail, uint8_t* active, int32_t* kiter,
+                             int32_t* kdone, int32_t kfail) { newton_reset_b(ninst, part, pfail, active, kiter, kdone, kfail, EB_VB); }
 
 // several device copies in one launch (the commit and the rollback of a step): list y = blockIdx.y.
 // With `sel`, element i of list y is copied only when its instance, map[y][i / per[y]], is selected.
@@ -1942,20 +2063,23 @@ struct CopyList {
     int32_t per[16];
     const uint8_t* sel;
 };
-__global__ void __launch_bounds__(256) copy_many(CopyList cl) {
-    const int y = blockIdx.y;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cl.n[y]; i += (int64_t)gridDim.x * blockDim.x)
+EBD void copy_many_b(CopyList cl, const VB& vb) {
+    const int y = vb.y;
+    for (int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x; i < cl.n[y]; i += (int64_t)vb.n * blockDim.x)
         if (!cl.sel || cl.sel[cl.map[y][i / cl.per[y]]]) cl.dst[y][i] = cl.src[y][i];
 }
+__global__ void __launch_bounds__(256) copy_many(CopyList cl) { copy_many_b(cl, EB_VB); }
 
 // ---- layout conversions at the ABI ----------------------------------------------------------
-__global__ void __launch_bounds__(256) gather_idx32(int64_t n, const int32_t* idx, const double* src, double* dst) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+EBD void gather_idx32_b(int64_t n, const int32_t* idx, const double* src, double* dst, const VB& vb) {
+    int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = src[idx[i]];
 }
-__global__ void __launch_bounds__(256) scatter_idx32(int64_t n, const int32_t* idx, const double* src, double* dst) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) gather_idx32(int64_t n, const int32_t* idx, const double* src, double* dst) { gather_idx32_b(n, idx, src, dst, EB_VB); }
+EBD void scatter_idx32_b(int64_t n, const int32_t* idx, const double* src, double* dst, const VB& vb) {
+    int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
     if (i < n) dst[idx[i]] = src[i];
 }
+__global__ void __launch_bounds__(256) scatter_idx32(int64_t n, const int32_t* idx, const double* src, double* dst) { scatter_idx32_b(n, idx, src, dst, EB_VB); }
 
 }  // namespace ebd
