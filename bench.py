@@ -151,7 +151,8 @@ def timed(eng, params, E, warmup, steps, graph=True, autotune=True):
             "failed_steps": state.failed_steps,
             "wall_s_per_simulated_s": wall / (state.t - st.tn) if state.t > st.tn else None,
             "kernels": kt, "contact": contact, "work": work,
-            "exec": {"mode": "graph" if graph else "eager", "graph_captures_replays": list(eng.exec_stats()[1:]),
+            "exec": {"mode": graph if graph == "fused" else ("graph" if graph else "eager"),
+                     "graph_captures_replays": list(eng.exec_stats()[1:]),
                      "tuning": tuning[0], "tuning_ms": tuning[1]}}
 
 
@@ -184,7 +185,8 @@ def timed_batch(eng, warmup, steps, graph=True, autotune=True):
             "failed_steps": sum(r.failed_steps - f for r, f in zip(states, f0)),
             "wall_s_per_simulated_s": wall / (sum(sim) / len(sim)) if sum(sim) > 0 else None,
             "kernels": kt, "contact": contact, "work": work,
-            "exec": {"mode": "graph" if graph else "eager", "graph_captures_replays": list(eng.exec_stats()[1:]),
+            "exec": {"mode": graph if graph == "fused" else ("graph" if graph else "eager"),
+                     "graph_captures_replays": list(eng.exec_stats()[1:]),
                      "tuning": tuning[0], "tuning_ms": tuning[1]}}
 
 
@@ -375,6 +377,8 @@ def small_config(args):
     from endobeams_amd import HipEngine, Stepper, abi, configs
     build = {"cantilever": configs.l_cantilever, "ring": configs.ring_drop, "net": configs.net_on_sphere}[args.workload]
     forms = {}
+    # one structure whose chain preconditioner holds every coupling: also each step as one launch
+    fused = args.workload in ("cantilever", "ring") and args.exec != "graph"
     for name, prec in (("fp64", abi.EB_PRECISION_FP64), ("fp32", abi.EB_PRECISION_FP32)):
         conf, params, inter = build()
         eng = HipEngine(conf, params, inter, precision=prec)
@@ -382,6 +386,14 @@ def small_config(args):
                             autotune=not args.no_autotune)
         eng.close()
         print(f"# {args.workload} {name}: {forms[name]['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+        if fused:
+            conf, params, inter = build()
+            eng = HipEngine(conf, params, inter, precision=prec)
+            forms[name + "_fused"] = timed(eng, params, len(conf.beams), args.warmup, args.steps, graph="fused",
+                                           autotune=False)
+            eng.close()
+            print(f"# {args.workload} {name}, one launch per step: {forms[name + '_fused']['ms_per_step']:.3f} ms/step",
+                  file=sys.stderr, flush=True)
     # the comparison starts the ring and the net at the surface, so that contact forces are compared
     near = {"cantilever": configs.l_cantilever, "ring": lambda: configs.ring_drop(height=0.06),
             "net": lambda: configs.net_on_sphere(gap=0.005)}[args.workload]
@@ -404,20 +416,26 @@ def small_config(args):
     E = len(conf.beams)
     f32, f64 = forms["fp32"], forms["fp64"]
     avg_s = kernel_avg_s(f32, "beam")
+    # the line's value: the faster execution of the precision form (one launch per step where it applies)
+    head = forms["fp32_fused"] if fused and forms["fp32_fused"]["value"] > f32["value"] else f32
     out = {
-        "metric": METRIC, "value": f32["value"], "unit": "element-steps/s", "n_gpus": 1, "steps": f32["steps"],
-        "warmup": args.warmup, "ms_per_step": f32["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": head["value"], "unit": "element-steps/s", "n_gpus": 1, "steps": head["steps"],
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": {"cantilever": "L-shaped cantilever, 20 beams",
                                 "ring": "ring of 60 beams dropped on a plane (64x64x32 FP32 SDF)",
                                 "net": "40x40 net of 3120 beams dropped on a sphere (128^3 FP32 SDF)"}[args.workload],
-                   "form": "FP32 element matrices + FP64 forces/solver, one structure", "exec": args.exec,
+                   "form": "FP32 element matrices + FP64 forces/solver, one structure",
+                   "exec": "fused" if head is not f32 else args.exec,
                    "dt": params.initial_timestep, "tolerance": params.tolerance_residual, "beams": E},
         "newton_iters_per_step": f32["newton_iters_per_step"], "failed_steps": f32["failed_steps"],
         "wall_s_per_simulated_s": f32["wall_s_per_simulated_s"],
         "plain_fp64": {"value": f64["value"], "ms_per_step": f64["ms_per_step"],
                        "newton_iters_per_step": f64["newton_iters_per_step"],
                        "wall_s_per_simulated_s": f64["wall_s_per_simulated_s"]},
+        "by_execution": {k: {"value": v["value"], "ms_per_step": v["ms_per_step"],
+                             "newton_iters_per_step": v["newton_iters_per_step"], "failed_steps": v["failed_steps"]}
+                         for k, v in forms.items()},
         "fp32_error_vs_fp64_oracle": {"steps": args.compare_steps,
                                       "start": "at the surface" if args.workload != "cantilever" else "as configured",
                                       **err},

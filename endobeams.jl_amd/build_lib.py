@@ -20,7 +20,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include")]
 
-SOURCES = ["eb_api.hip", "eb_element.hip", "eb_mlp.hip", "eb_setup.cpp"]
+SOURCES = ["eb_api.hip", "eb_element.hip", "eb_fused.hip", "eb_mlp.hip", "eb_setup.cpp"]
 
 # the element and contact kernels fold their structurally zero terms (why the results stay the same
 # is explained at the top of eb_element.hip); no reassociation, no contraction

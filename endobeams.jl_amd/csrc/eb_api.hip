@@ -17,6 +17,7 @@
 #include "eb_beam_kernel.h"
 #include "eb_contact_kernel.h"
 #include "eb_element.h"
+#include "eb_fused_launch.h"
 #include "eb_learned.h"
 #include "eb_linalg.h"
 #include "eb_mlp.h"
@@ -225,6 +226,8 @@ struct eb_ctx {
     // this device's copy of the file-scope staging array (eb_output_points_noalloc)
     double* scratch = nullptr;
     unsigned long long* contact_stats = nullptr;   // [2] listed beams, Gauss points in contact
+    int64_t* fused_out = nullptr;   // [3] what the one-workgroup kernels report (allocated with EB_EXEC_FUSED)
+    int fused_batch = 4;            // corrector iterations launched before the first wait
 };
 
 namespace {
@@ -1376,6 +1379,136 @@ int field_info(eb_ctx* ctx, int f, double*& ptr, int64_t& len, int& kind) {
 
 }  // namespace
 
+// ---- one time step of a small structure in one launch (eb_fused.h) --------------------------
+
+namespace {
+
+bool fused_eligible(const eb_ctx* ctx, std::string* why) {
+    if (ctx->S != 1) { *why = "a context of one structure"; return false; }
+    if (ctx->E > FUSED_MAX_BEAMS) { *why = "at most " + std::to_string(FUSED_MAX_BEAMS) + " beams"; return false; }
+    if (ctx->n_inexact > 0 || ctx->dev2.n_chains > 0) {
+        *why = "a chain preconditioner that holds every coupling (open chains, rings)";
+        return false;
+    }
+    if (!chain_lu(ctx)) { *why = "the block-LU chain algorithm"; return false; }
+    if (ctx->mlp) { *why = "no learned start installed"; return false; }
+    return true;
+}
+
+template <typename P> FusedArgs<P> fused_args(eb_ctx* ctx, const eb_newton_params* p, bool reuse) {
+    FusedArgs<P> a{};
+    a.st = beam_stage<P>(ctx);
+    a.sTc = (const double*)ctx->sTc;
+    a.E = ctx->E; a.N = ctx->N; a.ndofs = ctx->ndofs; a.nnz = ctx->nnz;
+    a.blk_ptr = ctx->blk_ptr; a.blk_src = ctx->blk_src; a.inc_ptr = ctx->inc_ptr; a.inc_src = ctx->inc_src;
+    a.Kn = ctx->Kn; a.Cn = ctx->Cn; a.Mn = ctx->Mn; a.Ktan = ctx->Ktan;
+    a.coef = (const StepCoef*)ctx->coef;
+    a.Tint = ctx->Tint; a.Tk = ctx->Tk; a.Tc = ctx->Tc; a.fext = ctx->fext; a.fext_n = ctx->fext_n; a.Tc_n = ctx->Tc_n;
+    a.Tint_n = ctx->Tint_n; a.Tk_n = ctx->Tk_n; a.res = ctx->res; a.energy = ctx->energy; a.partials = ctx->partials;
+    a.energy_grid = red_grid(ctx->E);
+    a.bptr = ctx->bptr; a.bcol = ctx->bcol; a.fixidx = ctx->fixidx; a.imposed = ctx->imposed; a.is_fixed = ctx->is_fixed;
+    a.n_fixed = ctx->n_fixed;
+    a.nf = node_fields(ctx);
+    a.D = ctx->D; a.Dd = ctx->Dd; a.Ddd = ctx->Ddd; a.dD = ctx->dD; a.pdD = ctx->pdD;
+    a.cd = chain_dev(ctx);
+    a.cl = closure_dev(ctx);
+    a.n_closures = ctx->n_closures;
+    a.n_chains = (int64_t)ctx->chains.ptr.size() - 1;
+    a.sg = segs(ctx);
+    a.kc = ctx->kctl; a.kall = ctx->kall; a.pivot_inst = ctx->pivot_inst; a.inst_exact = ctx->inst_exact;
+    a.kx = ctx->kx; a.kr = ctx->kr; a.krhat = ctx->krhat; a.kp = ctx->kp; a.kv = ctx->kv; a.ky = ctx->ky; a.ks = ctx->ks;
+    a.kz = ctx->kz; a.kt = ctx->kt;
+    a.krylov_tol = p->krylov_tol > 0 ? p->krylov_tol : 1e-11;
+    a.krylov_maxit = p->krylov_maxit > 0 ? p->krylov_maxit : 200;
+    a.act = active_dev(ctx);
+    a.active = ctx->active; a.part = ctx->part; a.pfail = ctx->pfail; a.kiter = ctx->kiter; a.kdone = ctx->kdone;
+    a.nall = ctx->nall; a.norms = ctx->norms;
+    a.alpha = p->alpha;
+    a.nc = NewtonCtl{p->tol_res, p->tol_disp, 0, p->max_iterations, 1};
+    a.reuse_factor = reuse ? 1 : 0;
+    a.out = ctx->fused_out;
+    return a;
+}
+
+}  // namespace
+
+static bool predictor_reuses_factor(const eb_ctx* ctx, double a);
+
+// solve_step_dynamics! (src/solve_step_dynamics.jl:1-14) with the predictor as one launch and each
+// corrector iteration as the element (and contact) kernels and one launch (eb_fused.h); the host
+// bookkeeping is that of eb_predictor + eb_corrector
+template <typename P> int step_fused_t(eb_ctx* ctx, const eb_newton_params* p, int* kout) {
+    const bool reuse = predictor_reuses_factor(ctx, p->alpha);
+    const bool contact = ctx->surface != EB_SURFACE_NONE;
+    const FusedArgs<P> a = fused_args<P>(ctx, p, reuse);
+    auto launched = [&](hipError_t e, const char* what) {
+        if (e == hipSuccess) return EB_OK;
+        ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+        return EB_ERR_DEVICE;
+    };
+    EB_TRY(launched(launch_fused_predictor<P>(a, ctx->stream), "fused_predictor"));
+    ctx->start_assembled = false;
+    if (!reuse) ctx->ktan_tag = eb_ctx::TanTag{};
+    ctx->factor_tag = ctx->ktan_tag;
+    ctx->counters[2] += 1;
+    // corrector iterations launched in batches without waiting (an iteration after the loop's end returns
+    // at once): the first batch as many as the last step took, then doubling; one wait per batch
+    int64_t launched_its = 0, out[3] = {0, 0, 0};
+    int batch = std::max(1, ctx->fused_batch);
+    while (launched_its < p->max_iterations) {
+        const int m = (int)std::min<int64_t>(batch, p->max_iterations - launched_its);
+        for (int j = 0; j < m; ++j) {
+            EB_TRY(contact ? launch_beams_contact<P>(ctx, p->alpha) : launch_beams<P>(ctx));
+            EB_TRY(launched(launch_fused_iteration<P>(a, contact, ctx->stream), "fused_iteration"));
+        }
+        launched_its += m;
+        EB_HIP(hipMemcpyAsync(ctx->h_nall, ctx->nall, sizeof(NewtonAll), hipMemcpyDeviceToHost, ctx->stream));
+        EB_HIP(hipMemcpyAsync(out, ctx->fused_out, sizeof out, hipMemcpyDeviceToHost, ctx->stream));
+        EB_HIP(hipStreamSynchronize(ctx->stream));
+        if (!ctx->ev_pending.empty()) prof_harvest(ctx);
+        if (ctx->h_nall->n_active == 0) break;
+        batch *= 2;
+    }
+    const int64_t iters = out[1];
+    ctx->fused_batch = (int)std::max<int64_t>(1, iters);
+    // the bookkeeping of the iterations that ran (the element launches past the loop's end wrote nothing)
+    if (iters > 0) {
+        ctx->kcm_pending = true;
+        ctx->np1_alias = false;
+        ctx->ktan_tag = eb_ctx::TanTag{ctx->stage_ver, 1 + p->alpha, ctx->coef_ver};
+        ctx->factor_tag = ctx->ktan_tag;
+    }
+    ctx->start_assembled = false;
+    ctx->counters[3] -= launched_its - iters;
+    ctx->counters[0] += iters;
+    ctx->counters[1] += out[0] + out[2];
+    ctx->counters[2] += iters;
+    const int k = (int)(1 + iters);
+    EB_HIP(hipMemcpyAsync(ctx->h_kiter, ctx->kiter, sizeof(int32_t) * ctx->S, hipMemcpyDeviceToHost, ctx->stream));
+    EB_HIP(hipStreamSynchronize(ctx->stream));
+    // as after corrector_impl: every instance taking part active again, the predictor's failure flags spent
+    EB_TRY(newton_reset_launch(ctx));
+    EB_HIP(hipMemsetAsync(ctx->pfail, 0, ctx->S, ctx->stream));
+    int kmax = 0;
+    for (int64_t s = 0; s < ctx->S; ++s) {
+        ctx->k_inst[s] = !ctx->h_part[s] ? 0 : (ctx->h_kiter[s] > 0 ? ctx->h_kiter[s] : k);
+        kmax = std::max(kmax, ctx->k_inst[s]);
+    }
+    *kout = kmax;
+    return EB_OK;
+}
+
+int step_fused(eb_ctx* ctx, const eb_newton_params* p, double dt, int* kout) {
+    std::string why;
+    if (!fused_eligible(ctx, &why)) {   // e.g. a chain algorithm set after the mode
+        ctx->err = "fused execution: " + why;
+        return EB_ERR_STATE;
+    }
+    EB_TRY(set_participation(ctx, nullptr));
+    EB_TRY(set_coefs(ctx, nullptr, dt, p->beta, p->gamma));
+    return ctx->precision == EB_PRECISION_FP32 ? step_fused_t<float>(ctx, p, kout) : step_fused_t<double>(ctx, p, kout);
+}
+
 // =============================================================================================
 namespace {
 
@@ -2338,6 +2471,7 @@ int eb_corrector(eb_ctx* ctx, const eb_newton_params* p, double dt, int* kout) {
 }
 
 int eb_solve_step_dynamics(eb_ctx* ctx, const eb_newton_params* p, double dt, int* k) {
+    if (ctx && p && k && ctx->exec_mode == EB_EXEC_FUSED) return step_fused(ctx, p, dt, k);
     EB_TRY(eb_predictor(ctx, p, dt));
     return eb_corrector(ctx, p, dt, k);
 }
@@ -2758,7 +2892,15 @@ int eb_get_sdf_coefficients(eb_ctx* ctx, int64_t instance, double* out, int64_t 
 }
 
 int eb_set_exec_mode(eb_ctx* ctx, int mode) {
-    if (!ctx || (mode != EB_EXEC_EAGER && mode != EB_EXEC_GRAPH)) return EB_ERR_ARG;
+    if (!ctx || (mode != EB_EXEC_EAGER && mode != EB_EXEC_GRAPH && mode != EB_EXEC_FUSED)) return EB_ERR_ARG;
+    if (mode == EB_EXEC_FUSED) {
+        std::string why;
+        if (!fused_eligible(ctx, &why)) {
+            ctx->err = "fused execution: " + why;
+            return EB_ERR_ARG;
+        }
+        if (!ctx->fused_out) EB_TRY(dalloc(ctx, ctx->fused_out, 3));
+    }
     EB_HIP(hipStreamSynchronize(ctx->stream));
     ctx->exec_mode = mode;
     if (mode == EB_EXEC_EAGER) graph_release(ctx);

@@ -30,6 +30,8 @@ EBD void st6(double* p, const double* v) {
 // several of these phases in one workgroup calls the bodies with the block indices it iterates.
 struct VB { int64_t x, n, y; };
 #define EB_VB (VB{(int64_t)blockIdx.x, (int64_t)gridDim.x, (int64_t)blockIdx.y})
+// EB_BODIES_ONLY: the bodies without their __global__ wrappers (a translation unit that runs the
+// bodies from a kernel of its own, beside the one that launches the wrappers)
 
 // ---- deterministic reductions ---------------------------------------------------------------
 
@@ -214,6 +216,7 @@ EBD void gather_matrices_b(int64_t nnz, const int32_t* __restrict__ blk_ptr,
         st4(Ktan + q, t);
     }
 }
+#ifndef EB_BODIES_ONLY
 template <typename P>
 __global__ void __launch_bounds__(256) gather_matrices(int64_t nnz, const int32_t* __restrict__ blk_ptr,
                                                        const uint32_t* __restrict__ blk_src, const P* __restrict__ sK,
@@ -221,6 +224,7 @@ __global__ void __launch_bounds__(256) gather_matrices(int64_t nnz, const int32_
                                                        double* C, double* M, double* Ktan, double c1,
                                                        const StepCoef* __restrict__ coef, int with_tangent, int write_kcm,
                                                        Active act) { gather_matrices_b<P>(nnz, blk_ptr, blk_src, sK, sC, sM, K, C, M, Ktan, c1, coef, with_tangent, write_kcm, act, EB_VB); }
+#endif
 
 // Tⁱⁿᵗ, Tᵏ, Tᶜ per dof, optionally followed by the corrector residual (utils_solver.jl:87)
 template <typename P>
@@ -251,6 +255,7 @@ EBD void gather_vectors_b(int64_t ndofs, const int32_t* __restrict__ inc_ptr,
     if (with_residual)
         res[d] = ((1 + alpha) * ((fext[d] + tc) - ti) - alpha * ((fext_n[d] + Tc_n[d]) - Tint_n[d])) - tk;
 }
+#ifndef EB_BODIES_ONLY
 template <typename P>
 __global__ void __launch_bounds__(256) gather_vectors(int64_t ndofs, const int32_t* __restrict__ inc_ptr,
                                                       const uint32_t* __restrict__ inc_src, const double* __restrict__ sTint,
@@ -258,6 +263,7 @@ __global__ void __launch_bounds__(256) gather_vectors(int64_t ndofs, const int32
                                                       double* Tint, double* Tk, double* Tc, int with_residual, double alpha,
                                                       const double* fext, const double* fext_n, const double* Tc_n,
                                                       const double* Tint_n, double* res, Active act) { gather_vectors_b<P>(ndofs, inc_ptr, inc_src, sTint, sTk, sTc, with_contact, Tint, Tk, Tc, with_residual, alpha, fext, fext_n, Tc_n, Tint_n, res, act, EB_VB); }
+#endif
 
 // energies: Σ over beams of [E][3] staging, added to the running totals (never reset, as in
 // the reference's Energy accumulation, assemble_beams.jl:56-57)
@@ -271,14 +277,18 @@ EBD void energy_partials_b(int64_t E, const double* st, double* partials, int ma
     }
     block_sum_store<3>(v, partials, vb.n, vb.x);
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(RED_BLOCK) energy_partials(int64_t E, const double* st, double* partials, int mask,
                                                              Active act) { energy_partials_b(E, st, partials, mask, act, EB_VB); }
+#endif
 EBD void energy_finalize_b(const double* partials, int grid, double* energy, const VB& vb) {
     double out[3];
     finalize_sum<3>(partials, grid, out);
     if (threadIdx.x == 0) { energy[0] += out[0]; energy[1] += out[1]; energy[2] += out[2]; }
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(RED_BLOCK) energy_finalize(const double* partials, int grid, double* energy) { energy_finalize_b(partials, grid, energy, EB_VB); }
+#endif
 
 // ---- HHT combine and residuals (src/utils_solver.jl:54-89) ----------------------------------
 
@@ -290,9 +300,11 @@ EBD void combine_tangent_b(int64_t nnz, const double* K, const double* C, const 
     const StepCoef& sc = coef[blk_inst[q / 36]];
     Ktan[q] = (c1 * K[q] + sc.gbdt * C[q]) + sc.inv * M[q];
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(256) combine_tangent(int64_t nnz, const double* K, const double* C, const double* M,
                                                        double* Ktan, double c1, const StepCoef* __restrict__ coef,
                                                        const int32_t* __restrict__ blk_inst) { combine_tangent_b(nnz, K, C, M, Ktan, c1, coef, blk_inst, EB_VB); }
+#endif
 
 EBD void residual_corrector_b(int64_t ndofs, double alpha, const double* fext,
                                                           const double* Tc, const double* Tint, const double* fext_n,
@@ -302,10 +314,12 @@ EBD void residual_corrector_b(int64_t ndofs, double alpha, const double* fext,
     if (d < ndofs)
         res[d] = ((1 + alpha) * ((fext[d] + Tc[d]) - Tint[d]) - alpha * ((fext_n[d] + Tc_n[d]) - Tint_n[d])) - Tk[d];
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(256) residual_corrector(int64_t ndofs, double alpha, const double* fext,
                                                           const double* Tc, const double* Tint, const double* fext_n,
                                                           const double* Tc_n, const double* Tint_n, const double* Tk,
                                                           double* res) { residual_corrector_b(ndofs, alpha, fext, Tc, Tint, fext_n, Tc_n, Tint_n, Tk, res, EB_VB); }
+#endif
 
 // predictor residual: r = (1+α)f - Tⁱⁿᵗⁿ - Tᵏⁿ - αfⁿ + Cⁿ·t₁ + (1/(βΔt²))·Mⁿ·t₂, the two products
 // accumulated row-wise in increasing column order (what the CSC column sweep of mul! yields per row)
@@ -343,6 +357,7 @@ EBD void residual_predictor_b(int64_t ndofs, const int32_t* __restrict__ bptr,
     }
     res[i] = r;
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(256) residual_predictor(int64_t ndofs, const int32_t* __restrict__ bptr,
                                                           const int32_t* __restrict__ bcol, double alpha,
                                                           const StepCoef* __restrict__ coef,
@@ -350,6 +365,7 @@ __global__ void __launch_bounds__(256) residual_predictor(int64_t ndofs, const i
                                                           const double* Tint_n, const double* Tk_n,
                                                           const double* fext_n, const double* Dd, const double* Ddd,
                                                           const double* Cn, const double* Mn, double* res) { residual_predictor_b(ndofs, bptr, bcol, alpha, coef, node_inst, fext, Tint_n, Tk_n, fext_n, Dd, Ddd, Cn, Mn, res, EB_VB); }
+#endif
 
 // apply_boundary_conditions! (utils_bcs_and_loads.jl:5-29): ΔD[d] = imposed - D[d] on fixed dofs
 // and r -= ΔD[d]·Ktan[:, d], subtracted per row in increasing fixed-dof order
@@ -379,10 +395,12 @@ EBD void bc_lift_b(int64_t ndofs, const int32_t* __restrict__ bptr,
     }
     if (any) res[i] = r;
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(256) bc_lift(int64_t ndofs, const int32_t* __restrict__ bptr,
                                                const int32_t* __restrict__ bcol, const int32_t* __restrict__ fixidx,
                                                const double* imposed, const double* D, const double* Ktan, double* dD,
                                                double* res, Active act) { bc_lift_b(ndofs, bptr, bcol, fixidx, imposed, D, Ktan, dD, res, act, EB_VB); }
+#endif
 
 // ---- masked block-row SpMV: y = A_ff x on free dofs, y = x on fixed dofs ---------------------
 // (x is zero on fixed dofs throughout the Krylov iteration). Optional dot products with y.
@@ -431,12 +449,14 @@ EBD void spmv_masked_b(Segs sg, const int32_t* __restrict__ bptr,
     }
     if (NQ > 0) seg_block_store<(NQ > 0 ? NQ : 1)>(v, partials, sg.nchunks, vb.x);
 }
+#ifndef EB_BODIES_ONLY
 template <int NQ>
 __global__ void __launch_bounds__(RED_BLOCK) spmv_masked(Segs sg, const int32_t* __restrict__ bptr,
                                                          const int32_t* __restrict__ bcol, const uint8_t* __restrict__ fixed,
                                                          const double* __restrict__ A, const double* __restrict__ x,
                                                          double* __restrict__ y, const double* __restrict__ dotw,
                                                          double* partials, const KrylovCtl* kc) { spmv_masked_b<NQ>(sg, bptr, bcol, fixed, A, x, y, dotw, partials, kc, EB_VB); }
+#endif
 
 // r = b − A x with the solve's operator (the identity on fixed dofs), on the chunks of unfinished
 // solves: the right-hand side of the crossing chain cover's correction
@@ -452,12 +472,14 @@ EBD void residual_masked_b(Segs sg, const int32_t* __restrict__ bptr,
         r[i] = b[i] - acc;
     }
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(RED_BLOCK) residual_masked(Segs sg, const int32_t* __restrict__ bptr,
                                                              const int32_t* __restrict__ bcol,
                                                              const uint8_t* __restrict__ fixed,
                                                              const double* __restrict__ A, const double* __restrict__ x,
                                                              const double* __restrict__ b, double* __restrict__ r,
                                                              const KrylovCtl* kc, int direct_only) { residual_masked_b(sg, bptr, bcol, fixed, A, x, b, r, kc, direct_only, EB_VB); }
+#endif
 
 // ---- chain block-tridiagonal preconditioner ------------------------------------------------
 // The node graph is covered by chains (paths); along a chain the masked tangent restricted to
@@ -885,8 +907,10 @@ EBD void chain_factor_b(ChainDev cd, const double* __restrict__ A, int* status, 
     }
     if (!ok) status[cd.inst[vb.x]] = 1;
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(CHAIN_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
 chain_factor(ChainDev cd, const double* __restrict__ A, int* status, int L0, const uint8_t* __restrict__ active) { chain_factor_b(cd, A, status, L0, active, EB_VB); }
+#endif
 
 // smallest L0 whose LDS share for the longest chain stays within `budget` bytes
 inline int chain_lds_level(int64_t max_len, int64_t budget) {
@@ -1013,10 +1037,12 @@ EBD void chain_solve_b(ChainDev cd, const double* __restrict__ b,
         xi = ACCUM ? xi + yw[t] : yw[t];
     }
 }
+#ifndef EB_BODIES_ONLY
 template <bool LDSV, int SOLVE_THREADS, bool ACCUM = false>
 __global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const double* __restrict__ b,
                                                              double* __restrict__ x, const KrylovCtl* kc,
                                                              int direct_only) { chain_solve_b<LDSV, SOLVE_THREADS, ACCUM>(cd, b, x, kc, direct_only, EB_VB); }
+#endif
 
 // ---- block LU along each chain, from both ends ----------------------------------------------
 // The same block-tridiagonal system factored position by position (block Thomas), from both ends
@@ -1222,8 +1248,10 @@ EBD void chain_lu_factor_b(ChainDev cd, const double* __restrict__ A, int* statu
     }
     if (mine && !ok) status[cd.inst[ch]] = 1;
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(64) chain_lu_factor(ChainDev cd, const double* __restrict__ A, int* status,
                                                       const uint8_t* __restrict__ active) { chain_lu_factor_b(cd, A, status, active, EB_VB); }
+#endif
 
 // The solve of the lane's half (see LuHalf; smax: the wavefront's longest half) with the factors
 // of chain_lu_factor. io.node(p) is the index io.b and io.put address position p by (loaded two
@@ -1406,9 +1434,11 @@ EBD void chain_lu_solve_b(ChainDev cd, const double* __restrict__ b, double* __r
     double* e = ex + 12 * ln.group;
     chain_lu_solve_group(cd, h, smax, r, live, cd.yw + 6 * h.s0, e, e + 6, io);
 }
+#ifndef EB_BODIES_ONLY
 template <bool ACCUM>
 __global__ void __launch_bounds__(64) chain_lu_solve(ChainDev cd, const double* __restrict__ b, double* __restrict__ x,
                                                      const KrylovCtl* kc, int direct_only) { chain_lu_solve_b<ACCUM>(cd, b, x, kc, direct_only, EB_VB); }
+#endif
 
 // ---- closed chains (Sherman-Morrison-Woodbury) ----------------------------------------------
 // A chain whose two ends are coupled (a ring) has the matrix A = T + U M Uᵀ: T the block-
@@ -1447,8 +1477,10 @@ EBD void closure_columns_b(ChainDev cd, ClosureDev cl, const uint8_t* __restrict
     double* z = cl.Z + cl.zoff[c] + (int64_t)j * 6 * n;
     for (int32_t t = threadIdx.x; t < 6 * n; t += NT) z[t] = yw[t];
 }
+#ifndef EB_BODIES_ONLY
 template <int NT>
 __global__ void __launch_bounds__(NT) closure_columns(ChainDev cd, ClosureDev cl, const uint8_t* __restrict__ active) { closure_columns_b<NT>(cd, cl, active, EB_VB); }
+#endif
 
 // the columns of Z with the block-LU factors: one chain's two six-lane groups per (closure,
 // column), the column solved in Z
@@ -1477,7 +1509,9 @@ EBD void closure_columns_lu_b(ChainDev cd, ClosureDev cl, const uint8_t* __restr
     double* e = ex + 12 * ln.group;
     chain_lu_solve_group(cd, h, smax, r, live, z, e, e + 6, io);
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(64) closure_columns_lu(ChainDev cd, ClosureDev cl, const uint8_t* __restrict__ active) { closure_columns_lu_b(cd, cl, active, EB_VB); }
+#endif
 
 // H = (I + M G)⁻¹ M per closure, one wavefront each, in LDS: Gauss-Jordan elimination with
 // partial pivoting on [I + M G | M]; a singular or non-finite system leaves H = 0 (no correction:
@@ -1546,8 +1580,10 @@ EBD void closure_setup_b(ChainDev cd, ClosureDev cl, const double* __restrict__ 
         H[q] = ok && isfinite(v) ? v : 0.0;
     }
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(64) closure_setup(ChainDev cd, ClosureDev cl, const double* __restrict__ A,
                                                     const uint8_t* __restrict__ active) { closure_setup_b(cd, cl, A, active, EB_VB); }
+#endif
 
 // y −= Z (H (Uᵀ y)) on each closed chain of an unfinished solve, after the chain solve
 EBD void closure_apply_b(ChainDev cd, ClosureDev cl, double* __restrict__ y,
@@ -1575,8 +1611,10 @@ EBD void closure_apply_b(ChainDev cd, ClosureDev cl, double* __restrict__ y,
         y[6 * (int64_t)cd.node[s0 + q / 6] + q % 6] -= acc;
     }
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(256) closure_apply(ChainDev cd, ClosureDev cl, double* __restrict__ y,
                                                      const KrylovCtl* __restrict__ kc, int direct_only) { closure_apply_b(cd, cl, y, kc, direct_only, EB_VB); }
+#endif
 
 // ---- BiCGStab (van der Vorst), right-preconditioned, FP64 scalars on the device -------------
 // One independent solve per instance, run in lockstep: the scalars live in KrylovCtl[instance],
@@ -1598,7 +1636,9 @@ EBD void solve_begin_b(int ninst, int32_t* pivot_inst, KrylovAll* all, const VB&
     for (int s = threadIdx.x; s < ninst; s += blockDim.x) pivot_inst[s] = 0;
     if (threadIdx.x == 0) all->pivot_status = 0;
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(256) solve_begin(int ninst, int32_t* pivot_inst, KrylovAll* all) { solve_begin_b(ninst, pivot_inst, all, EB_VB); }
+#endif
 
 // true when the kernel may run: ungated, or the polled solve is done
 EBD bool gate_open(const KrylovAll* gate) { return !gate || gate->all_done; }
@@ -1651,8 +1691,10 @@ EBD void krylov_init_b(Segs sg, const uint8_t* fixed, const double* res, double*
     }
     seg_block_store<1>(q, partials, sg.nchunks, vb.x);
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(RED_BLOCK) krylov_init(Segs sg, const uint8_t* fixed, const double* res, double* x,
                                                          double* r, double* rhat, double* p, double* v, double* partials) { krylov_init_b(sg, fixed, res, x, r, rhat, p, v, partials, EB_VB); }
+#endif
 // instances that are not active in the Newton iteration start done
 // instances whose factorisation met a singular pivot block start done and failed
 EBD void krylov_init_fin_b(const double* partials, Segs sg, KrylovCtl* kc, double tol,
@@ -1671,9 +1713,11 @@ EBD void krylov_init_fin_b(const double* partials, Segs sg, KrylovCtl* kc, doubl
     __syncthreads();
     krylov_summary(kc, sg.ninst, all);
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(FIN_BLOCK) krylov_init_fin(const double* partials, Segs sg, KrylovCtl* kc, double tol,
                                                              const uint8_t* active, const int32_t* pivot_inst,
                                                              const uint8_t* exact, KrylovAll* all) { krylov_init_fin_b(partials, sg, kc, tol, active, pivot_inst, exact, all, EB_VB); }
+#endif
 // p = r + β(p - ωv), β = (ρ/ρ_old)(α/ω); first iteration p = r
 EBD void krylov_p_b(Segs sg, const KrylovCtl* kc, const double* r, double* p,
                                                       const double* v, const VB& vb) {
@@ -1685,8 +1729,10 @@ EBD void krylov_p_b(Segs sg, const KrylovCtl* kc, const double* r, double* p,
     for (int64_t i = sg.lo[vb.x] + threadIdx.x; i < sg.hi[vb.x]; i += RED_BLOCK)
         p[i] = first ? r[i] : r[i] + beta * (p[i] - om * v[i]);
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(RED_BLOCK) krylov_p(Segs sg, const KrylovCtl* kc, const double* r, double* p,
                                                       const double* v) { krylov_p_b(sg, kc, r, p, v, EB_VB); }
+#endif
 EBD void krylov_alpha_fin_b(const double* partials, Segs sg, KrylovCtl* kc, const VB& vb) {
     for (int s = threadIdx.x; s < sg.ninst; s += (int)blockDim.x) {
         KrylovCtl& k = kc[s];
@@ -1697,7 +1743,9 @@ EBD void krylov_alpha_fin_b(const double* partials, Segs sg, KrylovCtl* kc, cons
         k.alpha = k.rho / o[0];
     }
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(FIN_BLOCK) krylov_alpha_fin(const double* partials, Segs sg, KrylovCtl* kc) { krylov_alpha_fin_b(partials, sg, kc, EB_VB); }
+#endif
 // s = r - αv, x += αy, ‖s‖²
 EBD void krylov_s_b(Segs sg, const KrylovCtl* kc, const double* r, const double* v,
                                                       const double* y, double* s, double* x, double* partials, const VB& vb) {
@@ -1713,8 +1761,10 @@ EBD void krylov_s_b(Segs sg, const KrylovCtl* kc, const double* r, const double*
     }
     seg_block_store<1>(q, partials, sg.nchunks, vb.x);
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(RED_BLOCK) krylov_s(Segs sg, const KrylovCtl* kc, const double* r, const double* v,
                                                       const double* y, double* s, double* x, double* partials) { krylov_s_b(sg, kc, r, v, y, s, x, partials, EB_VB); }
+#endif
 EBD void krylov_s_fin_b(const double* partials, Segs sg, KrylovCtl* kc, const VB& vb) {
     for (int s = threadIdx.x; s < sg.ninst; s += (int)blockDim.x) {
         KrylovCtl& k = kc[s];
@@ -1725,7 +1775,9 @@ EBD void krylov_s_fin_b(const double* partials, Segs sg, KrylovCtl* kc, const VB
         if (o[0] <= k.tol2 * k.bnorm2) { k.rr2 = o[0]; k.iters += 1; k.done = 1; }
     }
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(FIN_BLOCK) krylov_s_fin(const double* partials, Segs sg, KrylovCtl* kc) { krylov_s_fin_b(partials, sg, kc, EB_VB); }
+#endif
 EBD void krylov_omega_fin_b(const double* partials, Segs sg, KrylovCtl* kc, const VB& vb) {
     for (int s = threadIdx.x; s < sg.ninst; s += (int)blockDim.x) {
         KrylovCtl& k = kc[s];
@@ -1736,7 +1788,9 @@ EBD void krylov_omega_fin_b(const double* partials, Segs sg, KrylovCtl* kc, cons
         k.omega = o[0] / o[1];
     }
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(FIN_BLOCK) krylov_omega_fin(const double* partials, Segs sg, KrylovCtl* kc) { krylov_omega_fin_b(partials, sg, kc, EB_VB); }
+#endif
 // x += ωz, r = s - ωt, (r̂·r), ‖r‖²
 EBD void krylov_x_b(Segs sg, const KrylovCtl* kc, const double* z, const double* s,
                                                       const double* t, const double* rhat, double* x, double* r,
@@ -1754,9 +1808,11 @@ EBD void krylov_x_b(Segs sg, const KrylovCtl* kc, const double* z, const double*
     }
     seg_block_store<2>(q, partials, sg.nchunks, vb.x);
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(RED_BLOCK) krylov_x(Segs sg, const KrylovCtl* kc, const double* z, const double* s,
                                                       const double* t, const double* rhat, double* x, double* r,
                                                       double* partials) { krylov_x_b(sg, kc, z, s, t, rhat, x, r, partials, EB_VB); }
+#endif
 EBD void krylov_rho_fin_b(const double* partials, Segs sg, KrylovCtl* kc, KrylovAll* all, const VB& vb) {
     for (int s = threadIdx.x; s < sg.ninst; s += (int)blockDim.x) {
         KrylovCtl& k = kc[s];
@@ -1774,7 +1830,9 @@ EBD void krylov_rho_fin_b(const double* partials, Segs sg, KrylovCtl* kc, Krylov
     __syncthreads();
     krylov_summary(kc, sg.ninst, all);
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(FIN_BLOCK) krylov_rho_fin(const double* partials, Segs sg, KrylovCtl* kc, KrylovAll* all) { krylov_rho_fin_b(partials, sg, kc, all, EB_VB); }
+#endif
 // ---- the direct first step ----------------------------------------------------------------------
 // When the chain cover (with the closure corrections) holds every coupling of the tangent, the
 // preconditioner M is the solve's operator A up to rounding and x = M⁻¹ b solves the system. The
@@ -1801,12 +1859,14 @@ EBD void krylov_direct_residual_b(Segs sg, const int32_t* __restrict__ bptr,
     }
     seg_block_store<2>(q, partials, sg.nchunks, vb.x);
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(RED_BLOCK) krylov_direct_residual(Segs sg, const int32_t* __restrict__ bptr,
                                                                     const int32_t* __restrict__ bcol,
                                                                     const uint8_t* __restrict__ fixed,
                                                                     const double* __restrict__ A, const double* __restrict__ x,
                                                                     const double* __restrict__ b, double* __restrict__ r,
                                                                     double* partials, const KrylovCtl* kc) { krylov_direct_residual_b(sg, bptr, bcol, fixed, A, x, b, r, partials, kc, EB_VB); }
+#endif
 EBD void krylov_direct_fin_b(const double* partials, Segs sg, KrylovCtl* kc,
                                                                KrylovAll* all, const VB& vb) {
     for (int s = threadIdx.x; s < sg.ninst; s += (int)blockDim.x) {
@@ -1825,8 +1885,10 @@ EBD void krylov_direct_fin_b(const double* partials, Segs sg, KrylovCtl* kc,
     __syncthreads();
     krylov_summary(kc, sg.ninst, all);
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(FIN_BLOCK) krylov_direct_fin(const double* partials, Segs sg, KrylovCtl* kc,
                                                                KrylovAll* all) { krylov_direct_fin_b(partials, sg, kc, all, EB_VB); }
+#endif
 
 // ΔD[free] = x[free] for the instances active in the Newton iteration
 // Kernels of the corrector's update that take a `gate` run only once the linear solve is done: the
@@ -1838,8 +1900,10 @@ EBD void scatter_solution_b(int64_t ndofs, const uint8_t* fixed, const double* x
     int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
     if (i < ndofs && !fixed[i] && act.flag[act.node_inst[i / 6]]) dD[i] = x[i];
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(256) scatter_solution(int64_t ndofs, const uint8_t* fixed, const double* x, double* dD,
                                                         Active act, const KrylovAll* gate) { scatter_solution_b(ndofs, fixed, x, dD, act, gate, EB_VB); }
+#endif
 
 // Instances whose solve failed (a singular pivot block, a breakdown or a stall short of the
 // tolerance, or the iteration cap reached) leave the Newton iteration: the corrector records
@@ -1859,8 +1923,10 @@ EBD void solve_screen_b(int ninst, const KrylovCtl* kc, uint8_t* active, int32_t
         }
     }
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(256) solve_screen(int ninst, const KrylovCtl* kc, uint8_t* active, int32_t* kiter,
                                                     int32_t kfail, uint8_t* pfail, const KrylovAll* gate) { solve_screen_b(ninst, kc, active, kiter, kfail, pfail, gate, EB_VB); }
+#endif
 
 // ---- Newmark / rotation updates (src/utils_solver.jl:6-48, :157-245) -----------------------
 
@@ -1882,8 +1948,10 @@ EBD void init_global_b(int64_t N, NodeFields nf, double* D, double* Dd, double* 
         Dd[6 * n + 3 + k] = nf.wd[3 * n + k]; Ddd[6 * n + 3 + k] = nf.wdd[3 * n + k];
     }
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(256) init_global(int64_t N, NodeFields nf, double* D, double* Dd, double* Ddd, double* dD,
                                                    int predictor, Active act) { init_global_b(N, nf, D, Dd, Ddd, dD, predictor, act, EB_VB); }
+#endif
 
 // compute_global_corrector! per dof (all dofs, fixed included)
 EBD void global_corrector_b(int64_t ndofs, const StepCoef* __restrict__ coef, double* D,
@@ -1897,9 +1965,11 @@ EBD void global_corrector_b(int64_t ndofs, const StepCoef* __restrict__ coef, do
     const double gb = coef[s].gbdt, inv = coef[s].inv;
     D[i] += dD[i]; Dd[i] += gb * dD[i]; Ddd[i] += inv * dD[i];
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(256) global_corrector(int64_t ndofs, const StepCoef* __restrict__ coef, double* D,
                                                         double* Dd, double* Ddd, const double* dD, Active act,
                                                         const KrylovAll* gate) { global_corrector_b(ndofs, coef, D, Dd, Ddd, dD, act, gate, EB_VB); }
+#endif
 // compute_global_predictor!
 EBD void global_predictor_b(int64_t ndofs, const StepCoef* __restrict__ coef, double* D,
                                                         double* Dd, double* Ddd, const double* dD, Active act, const VB& vb) {
@@ -1914,8 +1984,10 @@ EBD void global_predictor_b(int64_t ndofs, const StepCoef* __restrict__ coef, do
     Dd[i] += (gbdt * dD[i] - gob * Ddn) + dt2b * Ddd[i];
     Ddd[i] += inv * ((dD[i] - dt * Ddn) - ((dt * dt) / 2) * Ddd[i]);
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(256) global_predictor(int64_t ndofs, const StepCoef* __restrict__ coef, double* D,
                                                         double* Dd, double* Ddd, const double* dD, Active act) { global_predictor_b(ndofs, coef, D, Dd, Ddd, dD, act, EB_VB); }
+#endif
 
 // compute_local_corrector!: u, u̇, ü from D; ΔR ← exp(ΔD_θ)ΔR; w = log ΔR; ẇ, ẅ; R = ΔR Rⁿ
 EBD void local_corrector_b(int64_t N, NodeFields nf, const double* D, const double* Dd,
@@ -1939,10 +2011,12 @@ EBD void local_corrector_b(int64_t N, NodeFields nf, const double* D, const doub
     stv(nf.wdd, n, dR * c);
     stm(nf.R, n, dR * ldm3(nf.R_n, n));
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(256) local_corrector(int64_t N, NodeFields nf, const double* D, const double* Dd,
                                                        const double* Ddd, const double* dD,
                                                        const StepCoef* __restrict__ coef, Active act,
                                                        const KrylovAll* gate) { local_corrector_b(N, nf, D, Dd, Ddd, dD, coef, act, gate, EB_VB); }
+#endif
 
 // compute_local_predictor!: w = θ̃ = D_θ, ΔR = exp(θ̃), R = ΔR Rⁿ, ẇ, ẅ from θ̃
 EBD void local_predictor_b(int64_t N, NodeFields nf, const double* D, const double* Dd,
@@ -1964,8 +2038,10 @@ EBD void local_predictor_b(int64_t N, NodeFields nf, const double* D, const doub
     stv(nf.wd, n, dR * a);
     stv(nf.wdd, n, dR * c);
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(256) local_predictor(int64_t N, NodeFields nf, const double* D, const double* Dd,
                                                        const double* Ddd, const StepCoef* __restrict__ coef, Active act) { local_predictor_b(N, nf, D, Dd, Ddd, coef, act, EB_VB); }
+#endif
 
 // compute_norms_corrector (utils_solver.jl:126-151): Σ over free dofs of ΔD², r², (f+Tᶜ+Tᵏ)² and
 // Σ over fixed dofs of (f+Tᶜ+Tᵏ)², per instance
@@ -1985,9 +2061,11 @@ EBD void norms_partials_b(Segs sg, const uint8_t* fixed, const double* dD,
     }
     seg_block_store<4>(q, partials, sg.nchunks, vb.x);
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(RED_BLOCK) norms_partials(Segs sg, const uint8_t* fixed, const double* dD,
                                                             const double* res, const double* fext, const double* Tc,
                                                             const double* Tk, double* partials) { norms_partials_b(sg, fixed, dD, res, fext, Tc, Tk, partials, EB_VB); }
+#endif
 
 struct NewtonCtl {            // corrector loop control (src/corrector.jl:17-36)
     double tol_res, tol_disp;
@@ -2035,9 +2113,11 @@ EBD void norms_finalize_b(const double* partials, Segs sg, double* out, NewtonCt
         if (nc.newton) *kdone = nc.k;
     }
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(FIN_BLOCK) norms_finalize(const double* partials, Segs sg, double* out, NewtonCtl nc,
                                                             uint8_t* active, int32_t* kiter, NewtonAll* all,
                                                             const KrylovAll* gate, int32_t* kdone) { norms_finalize_b(partials, sg, out, nc, active, kiter, all, gate, kdone, EB_VB); }
+#endif
 
 // Start (and end) of a corrector loop: the instances taking part in the step become active. At
 // the start (`pfail` given) an instance whose predictor solve failed stays frozen with k = kfail.
@@ -2050,8 +2130,10 @@ EBD void newton_reset_b(int ninst, const uint8_t* part, const uint8_t* pfail, ui
         kiter[s] = failed ? kfail : 0;
     }
 }
+#ifndef EB_BODIES_ONLY
 __global__ void newton_reset(int ninst, const uint8_t* part, const uint8_t* pfail, uint8_t* active, int32_t* kiter,
                              int32_t* kdone, int32_t kfail) { newton_reset_b(ninst, part, pfail, active, kiter, kdone, kfail, EB_VB); }
+#endif
 
 // several device copies in one launch (the commit and the rollback of a step): list y = blockIdx.y.
 // With `sel`, element i of list y is copied only when its instance, map[y][i / per[y]], is selected.
@@ -2068,18 +2150,24 @@ EBD void copy_many_b(CopyList cl, const VB& vb) {
     for (int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x; i < cl.n[y]; i += (int64_t)vb.n * blockDim.x)
         if (!cl.sel || cl.sel[cl.map[y][i / cl.per[y]]]) cl.dst[y][i] = cl.src[y][i];
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(256) copy_many(CopyList cl) { copy_many_b(cl, EB_VB); }
+#endif
 
 // ---- layout conversions at the ABI ----------------------------------------------------------
 EBD void gather_idx32_b(int64_t n, const int32_t* idx, const double* src, double* dst, const VB& vb) {
     int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = src[idx[i]];
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(256) gather_idx32(int64_t n, const int32_t* idx, const double* src, double* dst) { gather_idx32_b(n, idx, src, dst, EB_VB); }
+#endif
 EBD void scatter_idx32_b(int64_t n, const int32_t* idx, const double* src, double* dst, const VB& vb) {
     int64_t i = (int64_t)vb.x * blockDim.x + threadIdx.x;
     if (i < n) dst[idx[i]] = src[i];
 }
+#ifndef EB_BODIES_ONLY
 __global__ void __launch_bounds__(256) scatter_idx32(int64_t n, const int32_t* idx, const double* src, double* dst) { scatter_idx32_b(n, idx, src, dst, EB_VB); }
+#endif
 
 }  // namespace ebd

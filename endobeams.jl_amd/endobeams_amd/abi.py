@@ -144,7 +144,7 @@ PROTOTYPES = {
     "get_tuning": (_i, [_vp, C.POINTER(C.c_int32)]),
 }
 
-EB_EXEC_EAGER, EB_EXEC_GRAPH = 0, 1
+EB_EXEC_EAGER, EB_EXEC_GRAPH, EB_EXEC_FUSED = 0, 1, 2
 TUNE_NAMES = ("chain_lds_kb", "chain_solve_lds", "gather_block", "chain_algo")
 EB_TUNE_COUNT, EB_TUNE_CANDIDATES = 4, 4
 EB_CHAIN_CYCLIC_REDUCTION, EB_CHAIN_BLOCK_LU = 1, 2

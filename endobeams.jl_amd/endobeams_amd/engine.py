@@ -157,9 +157,14 @@ class Engine:
         return out.reshape(n[2] + 2, n[1] + 2, n[0] + 2)
 
     # -- execution mode and launch shapes (no reference counterpart) --------------------------------
-    def set_exec_mode(self, graph: bool):
-        """Newton iterations as replays of a captured HIP graph (True) or launched one by one."""
-        mode = abi.EB_EXEC_GRAPH if graph else abi.EB_EXEC_EAGER
+    def set_exec_mode(self, graph):
+        """Newton iterations as replays of a captured HIP graph (True) or launched one by one
+        (False); "fused": each time step (predictor and every Newton iteration) as one launch of
+        one workgroup, for one small structure whose chain preconditioner holds every coupling."""
+        if graph == "fused":
+            mode = abi.EB_EXEC_FUSED
+        else:
+            mode = abi.EB_EXEC_GRAPH if graph else abi.EB_EXEC_EAGER
         self._check(self.fn["set_exec_mode"](self.h, mode), "set_exec_mode")
 
     def exec_stats(self):

@@ -358,9 +358,17 @@ int eb_get_sdf_coefficients(eb_ctx* ctx, int64_t instance, double* out, int64_t 
 /* How eb_corrector issues a Newton iteration. EAGER: the host launches every kernel.
  * GRAPH: the iteration (assembly, lift, factorisation, one BiCGStab iteration, the update and
  * norms gated on the solve's done flag, the two status copies) is captured once as a HIP graph
- * per set of step parameters (Δt, α, β, γ, tolerances) and replayed; results are identical. */
+ * per set of step parameters (Δt, α, β, γ, tolerances) and replayed; results are identical.
+ * FUSED: eb_solve_step_dynamics (and the time loops built on it) runs the whole step, the predictor
+ * and every Newton iteration of the corrector, as one launch of one workgroup that goes through
+ * the same phases as the EAGER path, so the results are the EAGER path's; the host waits once per
+ * step. For a context of one structure of at most 4096 beams whose chain preconditioner holds every
+ * coupling (open chains, rings), with the block-LU chain algorithm and no learned start;
+ * eb_set_exec_mode refuses it for any other context (EB_ERR_ARG, reason in eb_last_error).
+ * eb_predictor and eb_corrector called on their own keep the EAGER launches. */
 #define EB_EXEC_EAGER 0
 #define EB_EXEC_GRAPH 1
+#define EB_EXEC_FUSED 2
 int eb_set_exec_mode(eb_ctx* ctx, int mode);
 /* mode, graph captures and graph replays since eb_create (any pointer may be NULL) */
 int eb_get_exec_stats(const eb_ctx* ctx, int32_t* mode, int64_t* captures, int64_t* replays);
