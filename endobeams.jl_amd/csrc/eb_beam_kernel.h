@@ -40,6 +40,44 @@ EBD M3<double> ldm(const double* a, int64_t i) { M3<double> m; for (int k = 0; k
 // 2(R/6) + C/6 of the beam's record, each block row-major
 EBD int64_t stage_idx(int64_t e, int R, int C) { return e * 144 + (2 * (R / 6) + C / 6) * 36 + (R % 6) * 6 + C % 6; }
 
+// The element kernel writes a tile's K and M records (64 beams × 144 entries) to LDS first, one row of
+// STAGE_REC entries per beam (an odd stride: the lanes' scattered writes hit distinct banks), and
+// copies them out as 16-byte pieces: the tile's records are contiguous in the staging, so the copy
+// is one coalesced stream instead of 144 stores per lane to 64 different cache lines.
+constexpr int STAGE_REC = 145;
+template <typename S> struct StPiece { S v[16 / sizeof(S)]; };
+template <typename S, typename P> EBD void st_rec(S* rec, int bi, int bj, const M3<P>& B) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int R = 3 * bi + i, C = 3 * bj + j;
+            rec[(2 * (R / 6) + C / 6) * 36 + (R % 6) * 6 + C % 6] = S(B.a[i + 3 * j]);
+        }
+}
+// the wavefront's LDS writes in issue order before its lanes read each other's (the LDS executes a
+// wavefront's instructions in order; this keeps the compiler from moving them)
+EBD void wave_lds_fence() {
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+// a wavefront copies the tile's records from LDS to the staging at dst (the tile's first record);
+// vmask: bit j set when beam j of the tile was computed
+template <typename S> EBD void tile_records_out(S* __restrict__ dst, const S* rec, uint64_t vmask, int lane) {
+    constexpr int NP = 16 / sizeof(S), PR = 144 / NP;
+    wave_lds_fence();
+#pragma unroll 4
+    for (int q = lane; q < 64 * PR; q += 64) {
+        const int j = q / PR, k = q - PR * j;
+        if ((vmask >> j) & 1) {
+            StPiece<S> w;
+#pragma unroll
+            for (int c = 0; c < NP; ++c) w.v[c] = rec[j * STAGE_REC + NP * k + c];
+            *reinterpret_cast<StPiece<S>*>(dst + (int64_t)j * 144 + NP * k) = w;
+        }
+    }
+}
+
 template <typename S, typename P> EBD void st_block(S* base, int64_t E, int64_t e, int bi, int bj, const M3<P>& B) {
 #pragma unroll
     for (int j = 0; j < 3; ++j)
@@ -74,16 +112,20 @@ struct StaticOut {
     double eta, eta11, eta12, eta21, eta22, ln;
 };
 
-// no work beside the element (the element kernel without contact)
+// no work beside the element (the element kernel without contact). A hook tests the Gauss points
+// `gsel` of a beam with its frame (test) and, once every part of the tile is done, acts on the
+// combined result (finish).
 struct NoStaticHook {
-    EBD void operator()(int64_t, const NodeState&, const BeamConsts&, const GaussPts&, const V3<double>&,
-                        const V3<double>&, const M3<double>&, const V3<double>&, const V3<double>&) const {}
+    static constexpr bool SPLIT = false;
+    EBD unsigned test(int64_t, const BeamConsts&, const GaussPts&, unsigned, const V3<double>&, const V3<double>&,
+                      const M3<double>&, const V3<double>&, const V3<double>&) const { return 0; }
+    EBD void finish(int64_t, unsigned) const {}
 };
 
 // The static part's tangent Kⁱⁿᵗ = BᵀK̄B + Kₘ + K̃ (compute_beams.jl:76-159) from the frame, Tₛ⁻¹,
 // T̄ⁱⁿᵗ, K̄ₕ and B, all given in the physics type P; stored in the storage type of the staging.
 template <typename P, typename S>
-EBD void beam_static_tangent(int64_t e, int64_t E, const BeamStage<S>& st, const M3<P>& Re, const V3<P>& v1, P eta, P ln,
+EBD void beam_static_tangent(S* krec, const M3<P>& Re, const V3<P>& v1, P eta, P ln,
                              P Tu, P Ku, const M3<P>& Ti1, const M3<P>& Ti2, const V3<P>& TT1, const V3<P>& TT2,
                              const M3<P>& Kh1, const M3<P>& Kh2, const M3<P>& G1, const M3<P>& G2, const M3<P>& G4,
                              const V3<P>& B1, const M3<P>& B11, const M3<P>& B12, const M3<P>& B14, const M3<P>& B21,
@@ -125,26 +167,28 @@ EBD void beam_static_tangent(int64_t e, int64_t E, const BeamStage<S>& st, const
     M3<P> K44 = (tb14 * XB14 + tb24 * YB14) + Kt44;
     // block layout of Kⁱⁿᵗ (compute_beams.jl:148-159)
     M3<P> tK12 = tr(K12), tK14 = tr(K14);
-    st_block<S>(st.K, E, e, 0, 0, cvt<S>(K11));
-    st_block<S>(st.K, E, e, 1, 0, cvt<S>(tK12));
-    st_block<S>(st.K, E, e, 2, 0, cvt<S>(-tr(K11)));      // K¹³ᵀ with K¹³ = -K¹¹
-    st_block<S>(st.K, E, e, 3, 0, cvt<S>(tK14));
-    st_block<S>(st.K, E, e, 0, 1, cvt<S>(K12));
-    st_block<S>(st.K, E, e, 1, 1, cvt<S>(K22));
-    st_block<S>(st.K, E, e, 2, 1, cvt<S>(-K12));          // K²³ᵀ = (-K¹²ᵀ)ᵀ
-    st_block<S>(st.K, E, e, 3, 1, cvt<S>(tr(K24)));
-    st_block<S>(st.K, E, e, 0, 2, cvt<S>(-K11));
-    st_block<S>(st.K, E, e, 1, 2, cvt<S>(-tK12));
-    st_block<S>(st.K, E, e, 2, 2, cvt<S>(K11));
-    st_block<S>(st.K, E, e, 3, 2, cvt<S>(-tK14));         // K³⁴ᵀ = (-K¹⁴)ᵀ
-    st_block<S>(st.K, E, e, 0, 3, cvt<S>(K14));
-    st_block<S>(st.K, E, e, 1, 3, cvt<S>(K24));
-    st_block<S>(st.K, E, e, 2, 3, cvt<S>(-K14));
-    st_block<S>(st.K, E, e, 3, 3, cvt<S>(K44));
+    st_rec<S>(krec, 0, 0, cvt<S>(K11));
+    st_rec<S>(krec, 1, 0, cvt<S>(tK12));
+    st_rec<S>(krec, 2, 0, cvt<S>(-tr(K11)));      // K¹³ᵀ with K¹³ = -K¹¹
+    st_rec<S>(krec, 3, 0, cvt<S>(tK14));
+    st_rec<S>(krec, 0, 1, cvt<S>(K12));
+    st_rec<S>(krec, 1, 1, cvt<S>(K22));
+    st_rec<S>(krec, 2, 1, cvt<S>(-K12));          // K²³ᵀ = (-K¹²ᵀ)ᵀ
+    st_rec<S>(krec, 3, 1, cvt<S>(tr(K24)));
+    st_rec<S>(krec, 0, 2, cvt<S>(-K11));
+    st_rec<S>(krec, 1, 2, cvt<S>(-tK12));
+    st_rec<S>(krec, 2, 2, cvt<S>(K11));
+    st_rec<S>(krec, 3, 2, cvt<S>(-tK14));         // K³⁴ᵀ = (-K¹⁴)ᵀ
+    st_rec<S>(krec, 0, 3, cvt<S>(K14));
+    st_rec<S>(krec, 1, 3, cvt<S>(K24));
+    st_rec<S>(krec, 2, 3, cvt<S>(-K14));
+    st_rec<S>(krec, 3, 3, cvt<S>(K44));
 }
 
-template <typename P>
-EBD StaticOut beam_static(int64_t e, const NodeState& ns, const BeamConsts& bc, const BeamStage<P>& st) {
+// krec: the beam's K record (LDS); with_frame(x₁, x₂, Rₑ, Θ̅₁, Θ̅₂) runs as soon as the frame is known
+template <typename P, typename F>
+EBD StaticOut beam_static(int64_t e, const NodeState& ns, const BeamConsts& bc, const BeamStage<P>& st, P* krec,
+                          const F& with_frame) {
     const int64_t E = bc.E;
     const int32_t a = bc.n1[e], b = bc.n2[e];
     V3<double> x1 = ldv(ns.X0, a) + ldv(ns.u, a), x2 = ldv(ns.X0, b) + ldv(ns.u, b);
@@ -157,6 +201,7 @@ EBD StaticOut beam_static(int64_t e, const NodeState& ns, const BeamConsts& bc, 
     M3<double> ReT = tr(f.Re);
     M3<double> Rb1 = (ReT * R1) * Re0, Rb2 = (ReT * R2) * Re0;
     V3<double> Tb1 = toangle(Rb1), Tb2 = toangle(Rb2);
+    with_frame(x1, x2, f.Re, Tb1, Tb2);
     M3<double> Ti1 = Ts_inv(Tb1), Ti2 = Ts_inv(Tb2);
     double pr[14];
     for (int k = 0; k < 14; ++k) pr[k] = bc.props[k * E + e];
@@ -196,7 +241,7 @@ EBD StaticOut beam_static(int64_t e, const NodeState& ns, const BeamConsts& bc, 
 
     // Kₘ, K̃, BᵀKB downstream of K̄ₕ: in the physics type P (FP64 in the parity form, the same
     // expressions in FP32 in the precision form)
-    beam_static_tangent<P>(e, E, st, cvt<P>(f.Re), cvt<P>(f.v1), P(f.eta), P(ln), P(Tu), P(Ku), cvt<P>(Ti1), cvt<P>(Ti2),
+    beam_static_tangent<P>(krec, cvt<P>(f.Re), cvt<P>(f.v1), P(f.eta), P(ln), P(Tu), P(Ku), cvt<P>(Ti1), cvt<P>(Ti2),
                            cvt<P>(TT1), cvt<P>(TT2), cvt<P>(Kh1), cvt<P>(Kh2), cvt<P>(G1), cvt<P>(G2), cvt<P>(G4),
                            cvt<P>(B1), cvt<P>(B11), cvt<P>(B12), cvt<P>(B14), cvt<P>(B21), cvt<P>(B22), cvt<P>(B24),
                            P(KT0), P(KT1), P(KT2), P(KTT0), P(KTT1), P(KTT2));
@@ -248,7 +293,7 @@ template <typename P> struct MassBlocks { M3<P> M11, M12, M13, M14, M22, M23, M2
 // Dynamic part (inertia, gyroscopic and damping forces, M and Cᵏ), compute_beams.jl:161-517.
 template <typename P, typename S, int WHAT>
 EBD void beam_dynamic(int64_t e, const NodeState& ns, const BeamConsts& bc, const BeamStage<S>& st,
-                      const GaussPts& gp, const StaticOut& so, MassBlocks<P>& mb) {
+                      const GaussPts& gp, const StaticOut& so, MassBlocks<P>& mb, S* mrec) {
     constexpr bool TK = (WHAT & DYN_TK) != 0, WM = (WHAT & DYN_M) != 0;
     constexpr bool CA = (WHAT & DYN_C13) != 0, CB = (WHAT & DYN_C24) != 0;
     const int64_t E = bc.E;
@@ -418,15 +463,15 @@ EBD void beam_dynamic(int64_t e, const NodeState& ns, const BeamConsts& bc, cons
         M22 = (l02Re * M22) * ReT; M23 = (l02Re * M23) * ReT; M24 = (l02Re * M24) * ReT;
         M33 = (l02Re * M33) * ReT; M34 = (l02Re * M34) * ReT; M44 = (l02Re * M44) * ReT;
         mb = MassBlocks<P>{M11, M12, M13, M14, M22, M23, M24, M33, M34, M44};
-        st_block(st.M, E, e, 0, 0, M11); st_block(st.M, E, e, 0, 1, M12);
-        st_block(st.M, E, e, 0, 2, M13); st_block(st.M, E, e, 0, 3, M14);
-        st_block(st.M, E, e, 1, 0, tr(M12) * TT1t);
+        st_rec<S>(mrec, 0, 0, M11); st_rec<S>(mrec, 0, 1, M12);
+        st_rec<S>(mrec, 0, 2, M13); st_rec<S>(mrec, 0, 3, M14);
+        st_rec<S>(mrec, 1, 0, tr(M12) * TT1t);
         M3<P> M23t = M23 * TT1t, M24t = M24 * TT1t;
-        st_block(st.M, E, e, 1, 1, M22 * TT1t); st_block(st.M, E, e, 1, 2, M23t); st_block(st.M, E, e, 1, 3, M24t);
-        st_block(st.M, E, e, 2, 0, tr(M13)); st_block(st.M, E, e, 2, 1, tr(M23t));
-        st_block(st.M, E, e, 2, 2, M33); st_block(st.M, E, e, 2, 3, M34);
-        st_block(st.M, E, e, 3, 0, tr(M14) * TT2t); st_block(st.M, E, e, 3, 1, tr(M24t) * TT2t);
-        st_block(st.M, E, e, 3, 2, tr(M34) * TT2t); st_block(st.M, E, e, 3, 3, M44 * TT2t);
+        st_rec<S>(mrec, 1, 1, M22 * TT1t); st_rec<S>(mrec, 1, 2, M23t); st_rec<S>(mrec, 1, 3, M24t);
+        st_rec<S>(mrec, 2, 0, tr(M13)); st_rec<S>(mrec, 2, 1, tr(M23t));
+        st_rec<S>(mrec, 2, 2, M33); st_rec<S>(mrec, 2, 3, M34);
+        st_rec<S>(mrec, 3, 0, tr(M14) * TT2t); st_rec<S>(mrec, 3, 1, tr(M24t) * TT2t);
+        st_rec<S>(mrec, 3, 2, tr(M34) * TT2t); st_rec<S>(mrec, 3, 3, M44 * TT2t);
     }
     if constexpr (CA) {
         C11 = (l02Re * C11) * ReT; C21 = (l02Re * C21) * ReT; C31 = (l02Re * C31) * ReT; C41 = (l02Re * C41) * ReT;
@@ -480,37 +525,55 @@ constexpr int BEAM_TILE = 64;
 template <typename S, typename Hook>
 EBD void beam_kernel_body(int64_t tile, NodeState ns, BeamConsts bc, BeamStage<S> st, GaussPts gp,
                           const uint8_t* __restrict__ active, const int32_t* __restrict__ beam_inst, const Hook& hook) {
+    __shared__ S k_lds[BEAM_TILE * STAGE_REC];
+    __shared__ S m_lds[BEAM_TILE * STAGE_REC];
+    __shared__ uint8_t hook_bits[BEAM_TILE];
     const int part = __builtin_amdgcn_readfirstlane(threadIdx.x / BEAM_TILE);
-    const int64_t e = tile * BEAM_TILE + threadIdx.x % BEAM_TILE;
+    const int lane = threadIdx.x % BEAM_TILE;
+    const int64_t e = tile * BEAM_TILE + lane;
     // elements of instances frozen in the Newton iteration are skipped
     const bool valid = e < bc.E && active[beam_inst[e]];
+    const uint64_t vmask = __ballot(valid);   // the same 64 elements in every wavefront
     // FP64 form: the mass part computes M with Tᵏ; precision form: Tᵏ alone in FP64 (it enters the
     // residual), M in FP32 with the Coriolis columns 1 and 3
     constexpr bool FP64 = sizeof(S) == sizeof(double);
     constexpr int MASS_WHAT = FP64 ? (DYN_TK | DYN_M) : DYN_TK;
     constexpr int C13_WHAT = FP64 ? DYN_C13 : (DYN_C13 | DYN_M);
     const int m_part = FP64 ? PART_MASS : PART_C13;
+    // the work beside the element (the contact test of its Gauss points) is shared by the two parts
+    // with the least work of their own: the static part takes the middle point, the mass part the others
+    const unsigned all_pts = (1u << gp.n) - 1;
+    const unsigned static_pts = gp.n >= 2 ? 2u : 0u;
     MassBlocks<double> mbd;
     MassBlocks<S> mbs;
+    unsigned own = 0;
     if (valid) {
         if (part == PART_STATIC) {
-            beam_static<S>(e, ns, bc, st);
+            beam_static<S>(e, ns, bc, st, k_lds + lane * STAGE_REC,
+                           [&](const V3<double>& x1, const V3<double>& x2, const M3<double>& Re, const V3<double>& Tb1,
+                               const V3<double>& Tb2) {
+                               if constexpr (Hook::SPLIT)
+                                   hook_bits[lane] = (uint8_t)hook.test(e, bc, gp, static_pts, x1, x2, Re, Tb1, Tb2);
+                           });
         } else {
             const StaticOut so = beam_frame(e, ns, bc);
             if (part == PART_MASS) {
-                // work beside the element, with its frame: hook(e, ns, bc, gp, x₁, x₂, Rₑ, Θ̅₁, Θ̅₂),
-                // in the wavefront with the least work of its own (measured: the mass part takes
-                // about half the cycles of a Coriolis part)
-                const int32_t a = bc.n1[e], b = bc.n2[e];
-                hook(e, ns, bc, gp, ldv(ns.X0, a) + ldv(ns.u, a), ldv(ns.X0, b) + ldv(ns.u, b), so.Re, so.Tb1, so.Tb2);
-                beam_dynamic<double, S, MASS_WHAT>(e, ns, bc, st, gp, so, mbd);
+                if constexpr (Hook::SPLIT) {
+                    const int32_t a = bc.n1[e], b = bc.n2[e];
+                    own = hook.test(e, bc, gp, all_pts & ~static_pts, ldv(ns.X0, a) + ldv(ns.u, a),
+                                    ldv(ns.X0, b) + ldv(ns.u, b), so.Re, so.Tb1, so.Tb2);
+                }
+                beam_dynamic<double, S, MASS_WHAT>(e, ns, bc, st, gp, so, mbd, m_lds + lane * STAGE_REC);
             } else if (part == PART_C13) {
-                beam_dynamic<S, S, C13_WHAT>(e, ns, bc, st, gp, so, mbs);
+                beam_dynamic<S, S, C13_WHAT>(e, ns, bc, st, gp, so, mbs, m_lds + lane * STAGE_REC);
             } else {
-                beam_dynamic<S, S, DYN_C24>(e, ns, bc, st, gp, so, mbs);
+                beam_dynamic<S, S, DYN_C24>(e, ns, bc, st, gp, so, mbs, nullptr);
             }
         }
     }
+    // each wavefront copies out the records it staged
+    if (part == PART_STATIC) tile_records_out(st.K + tile * BEAM_TILE * 144, k_lds, vmask, lane);
+    if (part == m_part) tile_records_out(st.M + tile * BEAM_TILE * 144, m_lds, vmask, lane);
     // every wavefront sees the same 64 elements, so this test is uniform over the workgroup
     const bool damped = __ballot(valid && bc.props[13 * bc.E + e] > 0.0) != 0;
     if (damped) {
@@ -519,6 +582,10 @@ EBD void beam_kernel_body(int64_t tile, NodeState ns, BeamConsts bc, BeamStage<S
             if constexpr (FP64) beam_damping<double, S>(e, ns, bc, st, mbd);
             else beam_damping<S, S>(e, ns, bc, st, mbs);
         }
+    }
+    if constexpr (Hook::SPLIT) {
+        __syncthreads();   // the static part's test is in LDS
+        if (part == PART_MASS && valid) hook.finish(e, own | hook_bits[lane]);
     }
 }
 

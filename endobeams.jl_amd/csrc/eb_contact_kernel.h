@@ -407,28 +407,35 @@ __global__ void __launch_bounds__(256) contact_detect(NodeState ns, BeamConsts b
     list[slot] = (uint64_t(e) << 8) | mask;
 }
 
-// The same detection as a hook of the element kernel: the wavefront of the mass part runs it
-// between the element's frame (x₁, x₂, Rₑ, Θ̅₁, Θ̅₂) and its Gauss loop, so the assembly with
-// contact needs no separate detection pass.
+// The same detection as a hook of the element kernel: each of the two parts that run it tests its
+// Gauss points with the element's frame (x₁, x₂, Rₑ, Θ̅₁, Θ̅₂), and once the tile is done the mass
+// part appends the beams with points in contact to the work list, so the assembly with contact
+// needs no separate detection pass.
 template <typename P> struct DetectHook {
+    static constexpr bool SPLIT = true;
     const ContactArgs<P>& ca;   // the kernel's arguments, by reference (no local copies)
     const uint8_t* is_contact;  // [E]
     uint64_t* list;
     int32_t* count;
-    EBD void operator()(int64_t e, const NodeState& ns, const BeamConsts& bc, const GaussPts& gp, const V3<double>& x1,
-                        const V3<double>& x2, const M3<double>& Re, const V3<double>& Tb1, const V3<double>& Tb2) const {
-        if (!is_contact[e]) return;
+    EBD unsigned test(int64_t e, const BeamConsts& bc, const GaussPts& gp, unsigned gsel, const V3<double>& x1,
+                      const V3<double>& x2, const M3<double>& Re, const V3<double>& Tb1, const V3<double>& Tb2) const {
+        if (!is_contact[e] || !gsel) return 0;
         const int64_t E = bc.E;
         const int32_t inst = ca.beam_inst[e];
         const double rb = bc.props[0 * E + e];
         const double l0 = bc.l0[e];
         unsigned mask = 0;
         for (int ig = 0; ig < gp.n; ++ig) {
+            if (!((gsel >> ig) & 1)) continue;
             ShapeF s = shape_fn(l0, gp.z[ig]);
             V3<double> ut = {0.0, s.N3 * Tb1.z + s.N4 * Tb2.z, (-s.N3) * Tb1.y + (-s.N4) * Tb2.y};
             V3<double> xG = (s.N1 * x1 + s.N2 * x2) + Re * ut;
             if (in_contact<P>(ca, inst, xG, rb)) mask |= 1u << ig;
         }
+        return mask;
+    }
+    EBD void finish(int64_t e, unsigned mask) const {
+        if (!is_contact[e]) return;
         if (!mask) {
             for (int k = 0; k < 12; ++k) ca.Tc[e * 12 + k] = 0.0;
             ca.energy[3 * e + 2] = 0.0;
