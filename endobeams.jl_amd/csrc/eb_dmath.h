@@ -44,25 +44,37 @@ template <typename T> EBD M3<T> operator*(const M3<T>& a, T s) { M3<T> r; for (i
 template <typename T> EBD M3<T> operator/(const M3<T>& a, T s) { M3<T> r; for (int k = 0; k < 9; ++k) r.a[k] = a.a[k] / s; return r; }
 template <typename T> EBD M3<T>& operator+=(M3<T>& a, const M3<T>& b) { a = a + b; return a; }
 
+// a·b + c·d + e·f, the inner sum of every 3x3 product. FP64: two roundings per product term in the
+// reference's left-to-right order (contraction is off in this library). FP32, the precision form's
+// matrices: the compiler may fuse each multiply with the following add, one rounding per term
+// and a third fewer instructions; FP32 results need not reproduce the reference's bits.
+template <typename T> EBD T sum3(T a, T b, T c, T d, T e, T f) {
+    if constexpr (sizeof(T) == sizeof(float)) {
+#pragma clang fp contract(fast)
+        return a * b + c * d + e * f;
+    } else {
+        return a * b + c * d + e * f;
+    }
+}
 template <typename T> EBD M3<T> operator*(const M3<T>& a, const M3<T>& b) {
     M3<T> r;
 #pragma unroll
     for (int j = 0; j < 3; ++j)
 #pragma unroll
         for (int i = 0; i < 3; ++i)
-            r.a[i + 3 * j] = a.a[i] * b.a[3 * j] + a.a[i + 3] * b.a[3 * j + 1] + a.a[i + 6] * b.a[3 * j + 2];
+            r.a[i + 3 * j] = sum3(a.a[i], b.a[3 * j], a.a[i + 3], b.a[3 * j + 1], a.a[i + 6], b.a[3 * j + 2]);
     return r;
 }
 template <typename T> EBD V3<T> operator*(const M3<T>& a, V3<T> x) {
-    return {a.a[0] * x.x + a.a[3] * x.y + a.a[6] * x.z,
-            a.a[1] * x.x + a.a[4] * x.y + a.a[7] * x.z,
-            a.a[2] * x.x + a.a[5] * x.y + a.a[8] * x.z};
+    return {sum3(a.a[0], x.x, a.a[3], x.y, a.a[6], x.z),
+            sum3(a.a[1], x.x, a.a[4], x.y, a.a[7], x.z),
+            sum3(a.a[2], x.x, a.a[5], x.y, a.a[8], x.z)};
 }
 // Aᵀ x without forming Aᵀ
 template <typename T> EBD V3<T> tmul(const M3<T>& a, V3<T> x) {
-    return {a.a[0] * x.x + a.a[1] * x.y + a.a[2] * x.z,
-            a.a[3] * x.x + a.a[4] * x.y + a.a[5] * x.z,
-            a.a[6] * x.x + a.a[7] * x.y + a.a[8] * x.z};
+    return {sum3(a.a[0], x.x, a.a[1], x.y, a.a[2], x.z),
+            sum3(a.a[3], x.x, a.a[4], x.y, a.a[5], x.z),
+            sum3(a.a[6], x.x, a.a[7], x.y, a.a[8], x.z)};
 }
 template <typename T> EBD M3<T> tr(const M3<T>& a) {
     M3<T> r;
@@ -71,7 +83,7 @@ template <typename T> EBD M3<T> tr(const M3<T>& a) {
     r.a[6] = a.a[2]; r.a[7] = a.a[5]; r.a[8] = a.a[8];
     return r;
 }
-template <typename T> EBD T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename T> EBD T dot(V3<T> a, V3<T> b) { return sum3(a.x, b.x, a.y, b.y, a.z, b.z); }
 template <typename T> EBD T norm(V3<T> a) { return sqrt(dot(a, a)); }
 template <typename T> EBD V3<T> cross(V3<T> a, V3<T> b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
