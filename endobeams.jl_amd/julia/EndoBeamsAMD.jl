@@ -208,10 +208,13 @@ update_not_converged!(dev::Device) = check(ccall((:eb_update_not_converged, LIB)
 # execution mode and launch shapes (no counterpart in the reference)
 const EB_EXEC_EAGER = 0
 const EB_EXEC_GRAPH = 1
+const EB_EXEC_FUSED = 2
 """Newton iterations as replays of a captured HIP graph (`graph = true`) or launched one by one."""
-set_exec_mode!(dev, graph::Bool) =
-    check(ccall((:eb_set_exec_mode, LIB), Cint, (Ptr{Cvoid}, Cint), dev.ctx, graph ? EB_EXEC_GRAPH : EB_EXEC_EAGER),
-          dev.ctx, "eb_set_exec_mode")
+set_exec_mode!(dev, graph::Bool) = set_exec_mode!(dev, graph ? EB_EXEC_GRAPH : EB_EXEC_EAGER)
+"""`EB_EXEC_EAGER`, `EB_EXEC_GRAPH` or `EB_EXEC_FUSED` (one small structure: each Newton iteration
+as the element kernel and one single-workgroup launch; refused for contexts it does not cover)."""
+set_exec_mode!(dev, mode::Integer) =
+    check(ccall((:eb_set_exec_mode, LIB), Cint, (Ptr{Cvoid}, Cint), dev.ctx, Cint(mode)), dev.ctx, "eb_set_exec_mode")
 """Times the candidate launch shapes on the live context (after a time step) and keeps the
 fastest; returns (chain LDS budget in KB, chain-solve LDS mode, gather workgroup size)."""
 function autotune!(dev)

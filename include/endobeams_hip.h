@@ -359,10 +359,12 @@ int eb_get_sdf_coefficients(eb_ctx* ctx, int64_t instance, double* out, int64_t 
  * GRAPH: the iteration (assembly, lift, factorisation, one BiCGStab iteration, the update and
  * norms gated on the solve's done flag, the two status copies) is captured once as a HIP graph
  * per set of step parameters (Δt, α, β, γ, tolerances) and replayed; results are identical.
- * FUSED: eb_solve_step_dynamics (and the time loops built on it) runs the whole step, the predictor
- * and every Newton iteration of the corrector, as one launch of one workgroup that goes through
- * the same phases as the EAGER path, so the results are the EAGER path's; the host waits once per
- * step. For a context of one structure of at most 4096 beams whose chain preconditioner holds every
+ * FUSED: eb_solve_step_dynamics (and the time loops built on it) runs the predictor as one launch
+ * and each Newton iteration of the corrector as two: the element kernel, then one workgroup going
+ * through the gathers, lift, solve, updates and norms on the device code of the EAGER path, so the
+ * results are the EAGER path's, bit for bit. The iterations of a step are launched in batches (the
+ * previous step's count, then doubling), the host waiting once per batch; an iteration launched
+ * after convergence returns at once. For a context of one structure of at most 4096 beams whose chain preconditioner holds every
  * coupling (open chains, rings), with the block-LU chain algorithm and no learned start;
  * eb_set_exec_mode refuses it for any other context (EB_ERR_ARG, reason in eb_last_error).
  * eb_predictor and eb_corrector called on their own keep the EAGER launches. */

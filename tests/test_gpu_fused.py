@@ -1,9 +1,10 @@
-"""One time step of a small structure as one launch of one workgroup (eb_set_exec_mode(EB_EXEC_FUSED)):
-the predictor and every Newton iteration of the corrector (src/predictor.jl:1-27,
-src/corrector.jl:15-53) run as phases of a single kernel on the same device bodies the eagerly
-launched path uses, so every trajectory, Newton iteration count, energy and matrix is the eager
-path's, bit for bit (which the parity suite checks against the oracle); one case is also checked
-against the oracle directly. Contexts the fused step does not cover are refused."""
+"""The one-workgroup Newton iteration of a small structure (eb_set_exec_mode(EB_EXEC_FUSED)): the
+predictor (src/predictor.jl:1-27) as one launch and each iteration of the corrector
+(src/corrector.jl:15-53) as the element kernel plus one single-workgroup launch running the same
+device bodies the eagerly launched path uses, iterations launched in batches without host round
+trips. Every trajectory, Newton iteration count, energy and matrix is the eager path's, bit for bit
+(which the parity suite checks against the oracle); one case is also checked against the oracle
+directly. Contexts the mode does not cover are refused."""
 import numpy as np
 import pytest
 
