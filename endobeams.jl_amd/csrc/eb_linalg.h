@@ -1062,14 +1062,19 @@ __global__ void __launch_bounds__(SOLVE_THREADS) chain_solve(ChainDev cd, const 
 // workgroup.
 //
 // Loads run in two stages ahead of the step that uses them: the indices of the step LU_AHEAD
-// further on (block numbers, fixed-dof masks, node numbers) two stages ahead, the data at those
-// indices one stage ahead. No load waits on a load issued in the same step, so a step waits only
-// for data requested LU_AHEAD steps earlier. Steps past the end of a half read position 0 (any
-// valid position): they are inactive and store nothing. The wavefront runs its longest half's
-// steps, rounded up to a multiple of LU_AHEAD; the step body has no branch but masked stores.
+// (the factorisation: LU_FACTOR_AHEAD) further on (block numbers, fixed-dof masks, node numbers)
+// two stages ahead, the data at those indices one stage ahead. No load waits on a load issued in
+// the same step, so a step waits only for data requested that many steps earlier. Steps past the
+// end of a half read position 0 (any valid position): they are inactive and store nothing. The
+// wavefront runs its longest half's steps, rounded up to a multiple of that distance; the step body
+// has no branch but masked stores.
 constexpr int LU_GROUPS = 10;
 constexpr int LU_CHAINS = LU_GROUPS / 2;
 constexpr int LU_AHEAD = 4;
+// the factorisation's distance: with two steps of blocks in flight it needs at most 256 VGPRs per
+// lane, so a SIMD holds two of its wavefronts (with four steps: 358 VGPRs, one wavefront), which
+// on a batch of many chains outweighs the longer waits
+constexpr int LU_FACTOR_AHEAD = 2;
 
 EBD int32_t wave_max_i32(int32_t v) {
 #pragma unroll
@@ -1155,28 +1160,28 @@ EBD void chain_lu_factor_b(ChainDev cd, const double* __restrict__ A, int* statu
     }
     wave_lds_order();
     bool ok = true;
-    LuIx far[LU_AHEAD], near[LU_AHEAD];
-    double raw[LU_AHEAD][18];
+    LuIx far[LU_FACTOR_AHEAD], near[LU_FACTOR_AHEAD];
+    double raw[LU_FACTOR_AHEAD][18];
     // the first steps' loads in the order of the loop's own (data of step k, then the indices of
-    // step k + LU_AHEAD), so that the loop starts with the waits of its steady state
+    // step k + LU_FACTOR_AHEAD), so that the loop starts with the waits of its steady state
 #pragma unroll
-    for (int k = 0; k < LU_AHEAD; ++k) lu_ix(cd, h.s0 + h.at(k), r, near[k]);
+    for (int k = 0; k < LU_FACTOR_AHEAD; ++k) lu_ix(cd, h.s0 + h.at(k), r, near[k]);
 #pragma unroll
-    for (int k = 0; k < LU_AHEAD; ++k) {
+    for (int k = 0; k < LU_FACTOR_AHEAD; ++k) {
         lu_raw(A, near[k], r, raw[k]);
-        lu_ix(cd, h.s0 + h.at(LU_AHEAD + k), r, far[k]);
+        lu_ix(cd, h.s0 + h.at(LU_FACTOR_AHEAD + k), r, far[k]);
     }
-    for (int32_t q0 = 0; q0 < smax; q0 += LU_AHEAD) {
+    for (int32_t q0 = 0; q0 < smax; q0 += LU_FACTOR_AHEAD) {
 #pragma unroll
-        for (int k = 0; k < LU_AHEAD; ++k) {
+        for (int k = 0; k < LU_FACTOR_AHEAD; ++k) {
             const int32_t q = q0 + k;
             double v[18];
 #pragma unroll
             for (int c = 0; c < 18; ++c) v[c] = raw[k][c];
             const LuIx ix = near[k];
             near[k] = far[k];
-            lu_raw(A, near[k], r, raw[k]);                        // step q + LU_AHEAD
-            lu_ix(cd, h.s0 + h.at(q + 2 * LU_AHEAD), r, far[k]);     // step q + 2 LU_AHEAD
+            lu_raw(A, near[k], r, raw[k]);                        // step q + LU_FACTOR_AHEAD
+            lu_ix(cd, h.s0 + h.at(q + 2 * LU_FACTOR_AHEAD), r, far[k]);     // step q + 2 LU_FACTOR_AHEAD
             lu_mask(ix, r, v);
             const bool act = q < h.steps;
             // the block towards the finished side (lo) and towards the middle (up)
