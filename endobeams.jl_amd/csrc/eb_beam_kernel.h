@@ -534,6 +534,7 @@ EBD void beam_kernel_body(int64_t tile, NodeState ns, BeamConsts bc, BeamStage<S
     // elements of instances frozen in the Newton iteration are skipped
     const bool valid = e < bc.E && active[beam_inst[e]];
     const uint64_t vmask = __ballot(valid);   // the same 64 elements in every wavefront
+    if (vmask == 0) return;                   // the whole workgroup: a tile of frozen instances
     // FP64 form: the mass part computes M with Tᵏ; precision form: Tᵏ alone in FP64 (it enters the
     // residual), M in FP32 with the Coriolis columns 1 and 3
     constexpr bool FP64 = sizeof(S) == sizeof(double);
