@@ -458,7 +458,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=0,
                     help="stents advanced together by the batched forms (0: picked by the batch-shape autotuner)")
-    ap.add_argument("--batch-candidates", default="64,128,256,512", help="sizes the batch-shape autotuner times")
+    ap.add_argument("--batch-candidates", default="128,256,512,1024", help="sizes the batch-shape autotuner times")
     ap.add_argument("--sdf-n", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
