@@ -17,8 +17,8 @@ Four forms are timed in the same run:
                 reference predictor stays the per-instance fallback.
 The batch size is the one the batch-shape autotuner picks (`--batch-candidates`, contexts of each
 size timed over their first steps). `value` is the fast form's whole-job throughput: the learned
-start is timed beside it and not quoted, because on these stents its start never beats the
-reference predictor's (it costs its evaluations and saves no iteration). The other forms are
+start is timed beside it and not quoted, because on these stents its start saves no Newton
+iteration over the reference predictor's (it costs its evaluations). The other forms are
 reported with the ratios (the FP64 batch separates batching from the FP32 element matrices). Also reported: the roofline of the element kernel (algorithmic flops per launch ÷ average
 launch time from HIP events in this run), HBM roofline records of the matrix gather and the
 contact kernel (algorithmic bytes per launch ÷ average launch time), the CPU restatement's
