@@ -521,14 +521,21 @@ constexpr int BEAM_TILE = 64;
 // S: storage type of the element matrices (double: the FP64 form; float: the precision form, whose
 // Coriolis parts and the static part's tangent also compute in FP32). The static part's frame and
 // Tⁱⁿᵗ compute in FP64, and the mass part too, because it yields the inertia forces Tᵏ: both enter
-// the FP64 residual.
-template <typename S, typename Hook>
+// the FP64 residual. A workgroup runs the NP parts P0 .. P0 + NP − 1 of its tile, one wavefront each
+// (all four, or, in the precision form, the static and mass parts in one launch and the two
+// Coriolis parts in the next).
+template <typename S, int P0, int NP, typename Hook>
 EBD void beam_kernel_body(int64_t tile, NodeState ns, BeamConsts bc, BeamStage<S> st, GaussPts gp,
                           const uint8_t* __restrict__ active, const int32_t* __restrict__ beam_inst, const Hook& hook) {
+    constexpr bool HAS_S = P0 <= PART_STATIC && PART_STATIC < P0 + NP;
+    constexpr bool HAS_M = P0 <= PART_MASS && PART_MASS < P0 + NP;
+    constexpr bool HAS_13 = P0 <= PART_C13 && PART_C13 < P0 + NP;
+    constexpr bool HAS_24 = P0 <= PART_C24 && PART_C24 < P0 + NP;
+    static_assert(!Hook::SPLIT || (HAS_S && HAS_M), "the contact test is shared by the static and mass parts");
     __shared__ S k_lds[BEAM_TILE * STAGE_REC];
     __shared__ S m_lds[BEAM_TILE * STAGE_REC];
     __shared__ uint8_t hook_bits[BEAM_TILE];
-    const int part = __builtin_amdgcn_readfirstlane(threadIdx.x / BEAM_TILE);
+    const int part = P0 + __builtin_amdgcn_readfirstlane(threadIdx.x / BEAM_TILE);
     const int lane = threadIdx.x % BEAM_TILE;
     const int64_t e = tile * BEAM_TILE + lane;
     // elements of instances frozen in the Newton iteration are skipped
@@ -541,6 +548,7 @@ EBD void beam_kernel_body(int64_t tile, NodeState ns, BeamConsts bc, BeamStage<S
     constexpr int MASS_WHAT = FP64 ? (DYN_TK | DYN_M) : DYN_TK;
     constexpr int C13_WHAT = FP64 ? DYN_C13 : (DYN_C13 | DYN_M);
     const int m_part = FP64 ? PART_MASS : PART_C13;
+    constexpr bool HAS_MP = FP64 ? HAS_M : HAS_13;   // the workgroup holds the part computing M
     // the work beside the element (the contact test of its Gauss points) is shared by the two parts
     // with the least work of their own: the static part takes the middle point, the mass part the others
     const unsigned all_pts = (1u << gp.n) - 1;
@@ -549,37 +557,39 @@ EBD void beam_kernel_body(int64_t tile, NodeState ns, BeamConsts bc, BeamStage<S
     MassBlocks<S> mbs;
     unsigned own = 0;
     if (valid) {
-        if (part == PART_STATIC) {
+        if (HAS_S && part == PART_STATIC) {
             beam_static<S>(e, ns, bc, st, k_lds + lane * STAGE_REC,
                            [&](const V3<double>& x1, const V3<double>& x2, const M3<double>& Re, const V3<double>& Tb1,
                                const V3<double>& Tb2) {
                                if constexpr (Hook::SPLIT)
                                    hook_bits[lane] = (uint8_t)hook.test(e, bc, gp, static_pts, x1, x2, Re, Tb1, Tb2);
                            });
-        } else {
+        } else if constexpr (HAS_M || HAS_13 || HAS_24) {
             const StaticOut so = beam_frame(e, ns, bc);
-            if (part == PART_MASS) {
+            if (HAS_M && part == PART_MASS) {
                 if constexpr (Hook::SPLIT) {
                     const int32_t a = bc.n1[e], b = bc.n2[e];
                     own = hook.test(e, bc, gp, all_pts & ~static_pts, ldv(ns.X0, a) + ldv(ns.u, a),
                                     ldv(ns.X0, b) + ldv(ns.u, b), so.Re, so.Tb1, so.Tb2);
                 }
                 beam_dynamic<double, S, MASS_WHAT>(e, ns, bc, st, gp, so, mbd, m_lds + lane * STAGE_REC);
-            } else if (part == PART_C13) {
+            } else if (HAS_13 && part == PART_C13) {
                 beam_dynamic<S, S, C13_WHAT>(e, ns, bc, st, gp, so, mbs, m_lds + lane * STAGE_REC);
-            } else {
+            } else if constexpr (HAS_24) {
                 beam_dynamic<S, S, DYN_C24>(e, ns, bc, st, gp, so, mbs, nullptr);
             }
         }
     }
     // each wavefront copies out the records it staged
-    if (part == PART_STATIC) tile_records_out(st.K + tile * BEAM_TILE * 144, k_lds, vmask, lane);
-    if (part == m_part) tile_records_out(st.M + tile * BEAM_TILE * 144, m_lds, vmask, lane);
+    if (HAS_S && part == PART_STATIC) tile_records_out(st.K + tile * BEAM_TILE * 144, k_lds, vmask, lane);
+    if (HAS_MP && part == m_part) tile_records_out(st.M + tile * BEAM_TILE * 144, m_lds, vmask, lane);
     // every wavefront sees the same 64 elements, so this test is uniform over the workgroup
     const bool damped = __ballot(valid && bc.props[13 * bc.E + e] > 0.0) != 0;
     if (damped) {
         __syncthreads();
-        if (part == m_part && valid && bc.props[13 * bc.E + e] > 0.0) {
+        // (Cᵏ is complete here only when the workgroup holds both Coriolis parts: the four-part
+        // kernel, or the second launch of the precision form, where M is computed too)
+        if (HAS_MP && HAS_13 && HAS_24 && part == m_part && valid && bc.props[13 * bc.E + e] > 0.0) {
             if constexpr (FP64) beam_damping<double, S>(e, ns, bc, st, mbd);
             else beam_damping<S, S>(e, ns, bc, st, mbs);
         }
@@ -594,7 +604,16 @@ template <typename S>
 __global__ void __launch_bounds__(4 * BEAM_TILE) beam_kernel(NodeState ns, BeamConsts bc, BeamStage<S> st, GaussPts gp,
                                                               const uint8_t* __restrict__ active,
                                                               const int32_t* __restrict__ beam_inst) {
-    beam_kernel_body<S>(blockIdx.x, ns, bc, st, gp, active, beam_inst, NoStaticHook{});
+    beam_kernel_body<S, 0, 4>(blockIdx.x, ns, bc, st, gp, active, beam_inst, NoStaticHook{});
+}
+// Two of the four parts per workgroup (P0 = PART_STATIC: static and mass; P0 = PART_C13: the Coriolis
+// parts): the precision form's element kernel as two launches. Each launch's code is about half
+// the four-part kernel's, and a CU holds two of its workgroups, one wavefront per SIMD as before.
+template <typename S, int P0>
+__global__ void __launch_bounds__(2 * BEAM_TILE) beam_kernel_half(NodeState ns, BeamConsts bc, BeamStage<S> st,
+                                                                   GaussPts gp, const uint8_t* __restrict__ active,
+                                                                   const int32_t* __restrict__ beam_inst) {
+    beam_kernel_body<S, P0, 2>(blockIdx.x, ns, bc, st, gp, active, beam_inst, NoStaticHook{});
 }
 
 }  // namespace ebd

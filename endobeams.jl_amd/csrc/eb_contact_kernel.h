@@ -452,7 +452,17 @@ __global__ void __launch_bounds__(4 * BEAM_TILE) beam_detect_kernel(NodeState ns
                                                                      const int32_t* __restrict__ beam_inst,
                                                                      ContactArgs<S> ca, const uint8_t* is_contact,
                                                                      uint64_t* list, int32_t* count) {
-    beam_kernel_body<S>(blockIdx.x, ns, bc, st, gp, active, beam_inst, DetectHook<S>{ca, is_contact, list, count});
+    beam_kernel_body<S, 0, 4>(blockIdx.x, ns, bc, st, gp, active, beam_inst, DetectHook<S>{ca, is_contact, list, count});
+}
+// the static and mass parts with the contact detection (the precision form's first launch; the
+// Coriolis parts follow in beam_kernel_half<S, PART_C13>)
+template <typename S>
+__global__ void __launch_bounds__(2 * BEAM_TILE) beam_detect_kernel_half(NodeState ns, BeamConsts bc, BeamStage<S> st,
+                                                                          GaussPts gp, const uint8_t* __restrict__ active,
+                                                                          const int32_t* __restrict__ beam_inst,
+                                                                          ContactArgs<S> ca, const uint8_t* is_contact,
+                                                                          uint64_t* list, int32_t* count) {
+    beam_kernel_body<S, 0, 2>(blockIdx.x, ns, bc, st, gp, active, beam_inst, DetectHook<S>{ca, is_contact, list, count});
 }
 
 // Precision form: the FP64 contact force (Tᶜ of all four node slots) and energy of every listed
