@@ -608,9 +608,11 @@ __global__ void __launch_bounds__(4 * BEAM_TILE) beam_kernel(NodeState ns, BeamC
 }
 // Two of the four parts per workgroup (P0 = PART_STATIC: static and mass; P0 = PART_C13: the Coriolis
 // parts): the precision form's element kernel as two launches. Each launch's code is about half
-// the four-part kernel's, and a CU holds two of its workgroups, one wavefront per SIMD as before.
+// the four-part kernel's. The static and mass parts need about 320 VGPRs (one wavefront per SIMD:
+// a CU holds two of their workgroups); the Coriolis parts are held to 256 (10 spilled), two
+// wavefronts per SIMD, four workgroups per CU.
 template <typename S, int P0>
-__global__ void __launch_bounds__(2 * BEAM_TILE) beam_kernel_half(NodeState ns, BeamConsts bc, BeamStage<S> st,
+__global__ void __launch_bounds__(2 * BEAM_TILE, P0 == PART_C13 ? 2 : 1) beam_kernel_half(NodeState ns, BeamConsts bc, BeamStage<S> st,
                                                                    GaussPts gp, const uint8_t* __restrict__ active,
                                                                    const int32_t* __restrict__ beam_inst) {
     beam_kernel_body<S, P0, 2>(blockIdx.x, ns, bc, st, gp, active, beam_inst, NoStaticHook{});
