@@ -14,17 +14,24 @@
 
 namespace ebd {
 
+// The precision form's element kernel as two launches of two parts each (throughput: the tiles
+// fill the device several times over) from this many tiles on; below, one launch of all four parts
+// (latency: the parts of a tile run side by side, which is what a small structure's few tiles need)
+constexpr unsigned SPLIT_MIN_TILES = 1024;
+
 template <typename P>
 void launch_beam_kernel(unsigned grid, hipStream_t stream, NodeState ns, BeamConsts bc, BeamStage<P> st,
                               GaussPts gp, const uint8_t* active, const int32_t* beam_inst) {
     if constexpr (sizeof(P) == sizeof(float)) {
-        hipLaunchKernelGGL((beam_kernel_half<P, PART_STATIC>), dim3(grid), dim3(2 * BEAM_TILE), 0, stream, ns, bc, st, gp,
-                           active, beam_inst);
-        hipLaunchKernelGGL((beam_kernel_half<P, PART_C13>), dim3(grid), dim3(2 * BEAM_TILE), 0, stream, ns, bc, st, gp,
-                           active, beam_inst);
-    } else {
-        hipLaunchKernelGGL(beam_kernel<P>, dim3(grid), dim3(4 * BEAM_TILE), 0, stream, ns, bc, st, gp, active, beam_inst);
+        if (grid >= SPLIT_MIN_TILES) {
+            hipLaunchKernelGGL((beam_kernel_half<P, PART_STATIC>), dim3(grid), dim3(2 * BEAM_TILE), 0, stream, ns, bc, st,
+                               gp, active, beam_inst);
+            hipLaunchKernelGGL((beam_kernel_half<P, PART_C13>), dim3(grid), dim3(2 * BEAM_TILE), 0, stream, ns, bc, st,
+                               gp, active, beam_inst);
+            return;
+        }
     }
+    hipLaunchKernelGGL(beam_kernel<P>, dim3(grid), dim3(4 * BEAM_TILE), 0, stream, ns, bc, st, gp, active, beam_inst);
 }
 
 template <typename P>
@@ -32,14 +39,16 @@ void launch_beam_detect_kernel(unsigned grid, hipStream_t stream, NodeState ns, 
                                      GaussPts gp, const uint8_t* active, const int32_t* beam_inst, ContactArgs<P> ca,
                                      const uint8_t* is_contact, uint64_t* list, int32_t* count) {
     if constexpr (sizeof(P) == sizeof(float)) {
-        hipLaunchKernelGGL(beam_detect_kernel_half<P>, dim3(grid), dim3(2 * BEAM_TILE), 0, stream, ns, bc, st, gp, active,
-                           beam_inst, ca, is_contact, list, count);
-        hipLaunchKernelGGL((beam_kernel_half<P, PART_C13>), dim3(grid), dim3(2 * BEAM_TILE), 0, stream, ns, bc, st, gp,
-                           active, beam_inst);
-    } else {
-        hipLaunchKernelGGL(beam_detect_kernel<P>, dim3(grid), dim3(4 * BEAM_TILE), 0, stream, ns, bc, st, gp, active,
-                           beam_inst, ca, is_contact, list, count);
+        if (grid >= SPLIT_MIN_TILES) {
+            hipLaunchKernelGGL(beam_detect_kernel_half<P>, dim3(grid), dim3(2 * BEAM_TILE), 0, stream, ns, bc, st, gp,
+                               active, beam_inst, ca, is_contact, list, count);
+            hipLaunchKernelGGL((beam_kernel_half<P, PART_C13>), dim3(grid), dim3(2 * BEAM_TILE), 0, stream, ns, bc, st,
+                               gp, active, beam_inst);
+            return;
+        }
     }
+    hipLaunchKernelGGL(beam_detect_kernel<P>, dim3(grid), dim3(4 * BEAM_TILE), 0, stream, ns, bc, st, gp, active,
+                       beam_inst, ca, is_contact, list, count);
 }
 
 template <typename P>
