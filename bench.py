@@ -68,6 +68,16 @@ def roofline(kernel, avg_s, beams, peak, traffic=None, peak_fp64=PEAK_FP64_TFLOP
 
 # Algorithmic HBM bytes of the HBM-bound kernels (s: bytes of the staging type, 4 in the precision
 # form, 8 in FP64).
+def element_kernels(reduced, contact, beams):
+    """The element kernel's launches for one assembly of `beams` beams: the precision form runs two
+    launches of two parts each from 1 024 tiles of 64 beams on, one launch of all four below."""
+    if reduced and -(-beams // 64) >= 1024:
+        return ("beam_detect_kernel_half<float>" if contact else "beam_kernel_half<float, 0>") + \
+            " + beam_kernel_half<float, 2>"
+    t = "float" if reduced else "double"
+    return f"beam_detect_kernel<{t}>" if contact else f"beam_kernel<{t}>"
+
+
 def gather_bytes(E, nnz, s):
     """gather_matrices forming the tangent: each beam's three 144-entry staging records read once
     (plus its four block-source entries), every tangent entry written once in FP64, and per 6×6
@@ -362,7 +372,8 @@ def sweep(args):
         "batch_autotune": {"instances_per_context": size, "element_steps_per_s": rates,
                            "launch_shapes": shapes.get(size)},
         "learned_choices_per_context": chosen,
-        "roofline": {**roofline("beam_detect_kernel<float>", avg_s, E_launch, PEAK_FP32_TFLOPS), "launches": int(kn)},
+        "roofline": {**roofline(element_kernels(True, True, sum(len(c.beams) for (c, _, _) in groups[0])), avg_s,
+                                E_launch, PEAK_FP32_TFLOPS), "launches": int(kn)},
     }
     for e in engs:
         e.close()
@@ -443,7 +454,7 @@ def small_config(args):
                                       **err},
         "cpu_baseline": {"value": E * args.compare_steps / cpu_wall, "unit": "element-steps/s", "cores": 1,
                          "kind": "port", "sample": f"first {args.compare_steps} steps, C++ FP64 restatement, 1 thread"},
-        "roofline": {**roofline("beam_kernel<float>" if inter is None else "beam_detect_kernel<float>", avg_s,
+        "roofline": {**roofline(element_kernels(True, inter is not None, E), avg_s,
                                 active_per_launch(f32, "beam", 0), PEAK_FP32_TFLOPS),
                      "launches": f32["kernels"]["beam"][1]},
         "hbm_rooflines": hbm_records(f32, 4),
@@ -579,13 +590,16 @@ def main():
     reduced = main_form is fast or main_form is lrn
     traffic = None
     if reduced and args.sdf_n == 256:
-        # HBM bytes per launch of this kernel, from the committed FETCH_SIZE / WRITE_SIZE profiler
-        # runs on 16 of these stents (scaled by a device copy of known size), per beam × beams
+        # HBM bytes of one element assembly, from the committed FETCH_SIZE / WRITE_SIZE profiler
+        # runs on 16 of these stents (scaled by a device copy of known size; 1 200 tiles: the two
+        # launches of the precision form), per beam × beams
         tf = os.path.join(REPO, "profiles", "braided_stent_batch16_fp32_hbm_traffic.json")
         if os.path.exists(tf):
-            k = json.load(open(tf))["kernels"].get("ebd::beam_detect_kernel<float>")
-            if k:
-                traffic = (k["fetch_bytes"] + k["write_bytes"]) / (16 * 4800) * active_per_launch(main_form, "beam", 0)
+            ks = json.load(open(tf))["kernels"]
+            names = ["ebd::" + n.strip() for n in element_kernels(True, True, 16 * 4800).split("+")]
+            if all(n in ks for n in names):
+                traffic = sum(ks[n]["fetch_bytes"] + ks[n]["write_bytes"] for n in names) / (16 * 4800) * \
+                    active_per_launch(main_form, "beam", 0)
     avg_s = kernel_avg_s(main_form, "beam")
     peak = PEAK_FP32_TFLOPS if reduced else PEAK_FP64_TFLOPS
     s_bytes = 4 if reduced else 8
@@ -616,7 +630,7 @@ def main():
         "failed_steps": main_form["failed_steps"],
         "wall_s_per_simulated_s": main_form["wall_s_per_simulated_s"],
         "exec": main_form["exec"],
-        "roofline": {**roofline("beam_detect_kernel<float>" if reduced else "beam_detect_kernel<double>", avg_s,
+        "roofline": {**roofline(element_kernels(reduced, True, main_form["E"]), avg_s,
                                 active_per_launch(main_form, "beam", 0), peak, traffic),
                      "launches": main_form["kernels"]["beam"][1], "beams_all_instances": main_form["E"]},
         "hbm_rooflines": hbm_records(main_form, s_bytes),
