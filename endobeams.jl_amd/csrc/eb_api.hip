@@ -85,7 +85,6 @@ struct eb_ctx {
     float *lin_scale = nullptr, *lout_scale = nullptr, *ones = nullptr, *lX = nullptr, *lT = nullptr;
     double* hist = nullptr;         // rates of the last three converged steps of each instance (3 slots of 6N)
     int32_t* newest = nullptr;      // [S] slot each instance wrote last
-    int32_t* h_newest = nullptr;    // pinned copy
     std::vector<int32_t> hcount;    // [S] converged steps recorded (at most 3)
     uint8_t* eligible = nullptr;    // [S] taking part in the step with three steps recorded
     uint8_t* h_eligible = nullptr;  // pinned copy
@@ -1452,9 +1451,12 @@ template <typename P> int step_fused_t(eb_ctx* ctx, const eb_newton_params* p, i
     ctx->factor_tag = ctx->ktan_tag;
     ctx->counters[2] += 1;
     // corrector iterations launched in batches without waiting (an iteration after the loop's end returns
-    // at once): the first batch as many as the last step took, then doubling; one wait per batch
+    // at once, a few µs for its two launches, where a wait and the next batch's launch cost tens of µs):
+    // the first batch two more than the last step took, then batches of a quarter of that (at least
+    // two); one wait per batch, which also fetches each instance's iteration count
     int64_t launched_its = 0, out[3] = {0, 0, 0};
-    int batch = std::max(1, ctx->fused_batch);
+    int batch = std::max(1, ctx->fused_batch + 2);
+    const int more = std::max(2, batch / 4);
     while (launched_its < p->max_iterations) {
         const int m = (int)std::min<int64_t>(batch, p->max_iterations - launched_its);
         for (int j = 0; j < m; ++j) {
@@ -1464,10 +1466,11 @@ template <typename P> int step_fused_t(eb_ctx* ctx, const eb_newton_params* p, i
         launched_its += m;
         EB_HIP(hipMemcpyAsync(ctx->h_nall, ctx->nall, sizeof(NewtonAll), hipMemcpyDeviceToHost, ctx->stream));
         EB_HIP(hipMemcpyAsync(out, ctx->fused_out, sizeof out, hipMemcpyDeviceToHost, ctx->stream));
+        EB_HIP(hipMemcpyAsync(ctx->h_kiter, ctx->kiter, sizeof(int32_t) * ctx->S, hipMemcpyDeviceToHost, ctx->stream));
         EB_HIP(hipStreamSynchronize(ctx->stream));
         if (!ctx->ev_pending.empty()) prof_harvest(ctx);
         if (ctx->h_nall->n_active == 0) break;
-        batch *= 2;
+        batch = more;
     }
     const int64_t iters = out[1];
     ctx->fused_batch = (int)std::max<int64_t>(1, iters);
@@ -1484,8 +1487,11 @@ template <typename P> int step_fused_t(eb_ctx* ctx, const eb_newton_params* p, i
     ctx->counters[1] += out[0] + out[2];
     ctx->counters[2] += iters;
     const int k = (int)(1 + iters);
-    EB_HIP(hipMemcpyAsync(ctx->h_kiter, ctx->kiter, sizeof(int32_t) * ctx->S, hipMemcpyDeviceToHost, ctx->stream));
-    EB_HIP(hipStreamSynchronize(ctx->stream));
+    // h_kiter holds the counts of the last wait (nothing has run since)
+    if (launched_its == 0) {
+        EB_HIP(hipMemcpyAsync(ctx->h_kiter, ctx->kiter, sizeof(int32_t) * ctx->S, hipMemcpyDeviceToHost, ctx->stream));
+        EB_HIP(hipStreamSynchronize(ctx->stream));
+    }
     // as after corrector_impl: every instance taking part active again, the predictor's failure flags spent
     EB_TRY(newton_reset_launch(ctx));
     EB_HIP(hipMemsetAsync(ctx->pfail, 0, ctx->S, ctx->stream));
@@ -1876,15 +1882,13 @@ int create_impl(const eb_model_desc* ds, int64_t S, eb_ctx** out) {
         hipHostMalloc((void**)&ctx->h_coef, sizeof(StepCoef) * S, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_part, S, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_sel, S, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&ctx->h_eligible, S, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&ctx->h_newest, sizeof(int32_t) * S, hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc((void**)&ctx->h_eligible, S, hipHostMallocDefault) != hipSuccess)
         return bail(EB_ERR_DEVICE);
     for (int k = 0; k < 3; ++k) ctx->h_pred[k] = 0;
     for (int64_t s = 0; s < S; ++s) {
         ctx->h_part[s] = 1;
         ctx->h_sel[s] = 0;
         ctx->h_eligible[s] = 0;
-        ctx->h_newest[s] = 0;
     }
     // contact
     ctx->surface = d->surface;
@@ -2052,7 +2056,7 @@ int eb_destroy(eb_ctx* ctx) {
     if (ctx->h_pred) (void)hipHostFree(ctx->h_pred);
     if (ctx->h_kiter) (void)hipHostFree(ctx->h_kiter);
     for (float* q : ctx->lact) (void)hipFree(q);
-    for (void* q : {(void*)ctx->h_coef, (void*)ctx->h_part, (void*)ctx->h_sel, (void*)ctx->h_eligible, (void*)ctx->h_newest})
+    for (void* q : {(void*)ctx->h_coef, (void*)ctx->h_part, (void*)ctx->h_sel, (void*)ctx->h_eligible})
         if (q) (void)hipHostFree(q);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -2476,6 +2480,8 @@ int eb_solve_step_dynamics(eb_ctx* ctx, const eb_newton_params* p, double dt, in
     return eb_corrector(ctx, p, dt, k);
 }
 
+static int update_converged(eb_ctx* ctx, bool wait);
+
 int eb_run_steps(eb_ctx* ctx, const eb_newton_params* p, const eb_time_params* tp, int64_t max_steps,
                  eb_step_callback callback, void* user, eb_run_state* state) {
     if (!ctx || !p || !tp || !state || max_steps < 0) return EB_ERR_ARG;
@@ -2510,14 +2516,17 @@ int eb_run_steps(eb_ctx* ctx, const eb_newton_params* p, const eb_time_params* t
             state->dt = std::min(state->dt * 1.5, tp->max_timestep);
             state->successive = 0;
         }
-        EB_TRY(eb_update_converged(ctx));
+        // a callback may read the state through its own copies: the commit is then waited for
+        EB_TRY(update_converged(ctx, callback != nullptr));
         state->t = tnp1;
         if (tp->stop_on_energy_threshold) {
             double en[3];
-            EB_HIP(hipMemcpy(en, ctx->energy, sizeof en, hipMemcpyDeviceToHost));
+            EB_HIP(hipMemcpyAsync(en, ctx->energy, sizeof en, hipMemcpyDeviceToHost, ctx->stream));
+            EB_HIP(hipStreamSynchronize(ctx->stream));
             if (en[1] < tp->energy_threshold) state->stopped = 1;
         }
     }
+    EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
 }
 
@@ -2525,21 +2534,18 @@ int eb_run_steps(eb_ctx* ctx, const eb_newton_params* p, const eb_time_params* t
 // `sel_host` (all when null); `sel_dev` is its device copy
 static int record_history(eb_ctx* ctx, const uint8_t* sel_host, const uint8_t* sel_dev) {
     if (!ctx->coef_set) return EB_OK;
-    EB_HIP(hipStreamSynchronize(ctx->stream));
-    for (int64_t s = 0; s < ctx->S; ++s) {
-        if (sel_host && !sel_host[s]) continue;
-        ctx->h_newest[s] = (ctx->h_newest[s] + 1) % 3;
-        ctx->hcount[s] = std::min(ctx->hcount[s] + 1, 3);
-    }
-    EB_HIP(hipMemcpyAsync(ctx->newest, ctx->h_newest, sizeof(int32_t) * ctx->S, hipMemcpyHostToDevice, ctx->stream));
+    for (int64_t s = 0; s < ctx->S; ++s)
+        if (!sel_host || sel_host[s]) ctx->hcount[s] = std::min(ctx->hcount[s] + 1, 3);
+    // the slot index advances on the device: no host copy, so no wait for the previous one
+    hipLaunchKernelGGL(learned_advance, dim3(nblk(ctx->S, 256)), dim3(256), 0, ctx->stream, ctx->S, sel_dev, ctx->newest);
     hipLaunchKernelGGL(learned_record, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, ctx->node[EB_F_U],
                        ctx->node[EB_F_U_N], ctx->node[EB_F_W], (const StepCoef*)ctx->coef, ctx->node_inst, sel_dev,
                        ctx->newest, ctx->hist);
     return check_launch(ctx, "learned_record");
 }
 
-int eb_update_converged(eb_ctx* ctx) {
-    if (!ctx) return EB_ERR_ARG;
+// wait false: the commit stays queued on the stream (the time loop's next step follows it in order)
+static int update_converged(eb_ctx* ctx, bool wait) {
     // rate of the step that converged, newest first in the history of the learned warm start
     EB_TRY(record_history(ctx, nullptr, nullptr));
     EB_TRY(copy_state(ctx, true));
@@ -2552,8 +2558,13 @@ int eb_update_converged(eb_ctx* ctx) {
         ctx->np1_alias = true;
         ctx->kcm_n_ver = ctx->kcm_ver;
     }
-    EB_HIP(hipStreamSynchronize(ctx->stream));
+    if (wait) EB_HIP(hipStreamSynchronize(ctx->stream));
     return EB_OK;
+}
+
+int eb_update_converged(eb_ctx* ctx) {
+    if (!ctx) return EB_ERR_ARG;
+    return update_converged(ctx, true);
 }
 
 int eb_update_not_converged(eb_ctx* ctx) {

@@ -29,6 +29,14 @@ __global__ void __launch_bounds__(256) learned_record(int64_t N, const double* u
     }
 }
 
+// newest[s] = (newest[s] + 1) % 3 for the instances selected in `sel` (all when null): the slot the
+// step just converged is recorded in
+__global__ void __launch_bounds__(256) learned_advance(int64_t S, const uint8_t* sel, int32_t* newest) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S || (sel && !sel[s])) return;
+    newest[s] = (newest[s] + 1) % 3;
+}
+
 // inputs per node: the reference predictor's increment p, the three recorded rates × Δt (newest
 // first), then Tᶜⁿ (translations), × scale
 __global__ void __launch_bounds__(256) learned_features(int64_t N, const double* pdD, const double* hist,
