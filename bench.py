@@ -70,8 +70,8 @@ def roofline(kernel, avg_s, beams, peak, traffic=None, peak_fp64=PEAK_FP64_TFLOP
 # form, 8 in FP64).
 def element_kernels(reduced, contact, beams):
     """The element kernel's launches for one assembly of `beams` beams: the precision form runs two
-    launches of two parts each from 1 024 tiles of 64 beams on, one launch of all four below."""
-    if reduced and -(-beams // 64) >= 1024:
+    launches of two parts each from 2 048 tiles of 64 beams on, one launch of all four below."""
+    if reduced and -(-beams // 64) >= 2048:
         return ("beam_detect_kernel_half<float>" if contact else "beam_kernel_half<float, 0>") + \
             " + beam_kernel_half<float, 2>"
     t = "float" if reduced else "double"
@@ -605,14 +605,14 @@ def main():
     traffic = None
     if reduced and args.sdf_n == 256:
         # HBM bytes of one element assembly, from the committed FETCH_SIZE / WRITE_SIZE profiler
-        # runs on 16 of these stents (scaled by a device copy of known size; 1 200 tiles: the two
+        # runs on 32 of these stents (scaled by a device copy of known size; 2 400 tiles: the two
         # launches of the precision form), per beam × beams
-        tf = os.path.join(REPO, "profiles", "braided_stent_batch16_fp32_hbm_traffic.json")
+        tf = os.path.join(REPO, "profiles", "braided_stent_batch32_fp32_hbm_traffic.json")
         if os.path.exists(tf):
             ks = json.load(open(tf))["kernels"]
-            names = ["ebd::" + n.strip() for n in element_kernels(True, True, 16 * 4800).split("+")]
+            names = ["ebd::" + n.strip() for n in element_kernels(True, True, 32 * 4800).split("+")]
             if all(n in ks for n in names):
-                traffic = sum(ks[n]["fetch_bytes"] + ks[n]["write_bytes"] for n in names) / (16 * 4800) * \
+                traffic = sum(ks[n]["fetch_bytes"] + ks[n]["write_bytes"] for n in names) / (32 * 4800) * \
                     active_per_launch(main_form, "beam", 0)
     avg_s = kernel_avg_s(main_form, "beam")
     peak = PEAK_FP32_TFLOPS if reduced else PEAK_FP64_TFLOPS

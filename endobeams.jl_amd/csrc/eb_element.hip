@@ -16,8 +16,10 @@ namespace ebd {
 
 // The precision form's element kernel as two launches of two parts each (throughput: the tiles
 // fill the device several times over) from this many tiles on; below, one launch of all four parts
-// (latency: the parts of a tile run side by side, which is what a small structure's few tiles need)
-constexpr unsigned SPLIT_MIN_TILES = 1024;
+// (latency: the parts of a tile run side by side, which is what a small structure's few tiles need).
+// Measured per element assembly on batches of the 4 800-beam stent (four-part / two launches):
+// 1 200 tiles 101 / 114 µs, 2 400 tiles 170 / 169, 4 800 tiles 297 / 285, 9 600 tiles 540 / 485.
+constexpr unsigned SPLIT_MIN_TILES = 2048;
 
 template <typename P>
 void launch_beam_kernel(unsigned grid, hipStream_t stream, NodeState ns, BeamConsts bc, BeamStage<P> st,

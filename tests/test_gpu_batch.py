@@ -65,7 +65,7 @@ def test_batch_instances_follow_their_standalone_runs(precision):
 
 
 def test_large_fp32_batch_two_launch_element_kernel_follows_standalone_runs():
-    """A precision-form batch of more than 1 024 element tiles (65 536 beams) runs its element
+    """A precision-form batch of at least 2 048 element tiles (131 072 beams) runs its element
     kernel as two launches of two parts each (static and mass, then the Coriolis parts); a
     structure alone runs the four-part kernel. Instances of the batch follow their standalone runs
     attempt by attempt with the same Newton counts, as in the test above."""
@@ -75,13 +75,13 @@ def test_large_fp32_batch_two_launch_element_kernel_follows_standalone_runs():
         return configs.braided_stent(sdf_n=64, seed=s, dt=1.25e-4)
 
     prec = abi.EB_PRECISION_FP32
-    insts = [build(s) for s in range(14)]
-    assert sum(len(c.beams) for (c, _, _) in insts) > 64 * 1024
+    insts = [build(s) for s in range(28)]
+    assert sum(len(c.beams) for (c, _, _) in insts) >= 64 * 2048
     batch = HipBatchEngine(insts, precision=prec)
     sched = batch_schedule(batch, 3)
     print("attempts (dt, accepted):", [(dt, ok) for (_, dt, ok, _) in sched], flush=True)
     assert sum(ok for (_, _, ok, _) in sched) >= 2
-    for s in (0, 13):
+    for s in (0, 27):
         conf, params, inter = build(s)
         alone = HipEngine(conf, params, inter, precision=prec)
         ka = replay(alone, sched)
